@@ -1,0 +1,405 @@
+"""Python host mirror of the hot-path C ABI (include/astro_hot.h).
+
+The reference drives its world from C++ (src/sim/world.cuh:183-243); this
+module gives the same calls to Python callers, tests and the benchmark, over
+ctypes. It mirrors the reference's names and error behaviour: every entry point
+returns an astro status code, and a non-zero code raises AstroError with the
+world's error detail.
+
+`load()` binds the HIP library built in-tree by build.py. It raises when the
+library is missing or no GPU is visible: there is no CPU fallback.
+
+`Binding` is generic over the symbol prefix so that a caller holding another
+library with the same ABI (the CPU restatement used by the tests) can drive it
+through the same wrapper.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "lib" / "libastro_hot.so"
+
+ABI_VERSION = 1
+STATUS = ["Ok", "OutOfMemory", "DeviceError", "InvalidArgument", "NotFound", "ParseError",
+          "ContractVersionMismatch", "CapacityExceeded", "FileIoError", "Unsupported"]
+OK, OUT_OF_MEMORY, DEVICE_ERROR, INVALID_ARGUMENT, NOT_FOUND, PARSE_ERROR, \
+    CONTRACT_VERSION_MISMATCH, CAPACITY_EXCEEDED, FILE_IO_ERROR, UNSUPPORTED = range(10)
+
+FP64_CONTRACT, FP32_HOT = 0, 1
+BOUNDARY_REFLECTING, BOUNDARY_PERIODIC, BOUNDARY_ABSORBING = 0, 1, 2
+GRAVITY_Y, GRAVITY_Z = 0, 1
+STORE_VOID, STORE_FLASH, STORE_RETAIN = 0, 1, 2
+TEMPERATURE, CO2, N2, IRRADIANCE = 0, 1, 2, 3
+PLACE_UNIFORM, PLACE_GAUSSIAN, PLACE_GRID, PLACE_DISC = 0, 1, 2, 3
+DIST_CONSTANT, DIST_UNIFORM, DIST_NORMAL, DIST_LOG_UNIFORM = 0, 1, 2, 3
+
+# contract flags (contracts/cell_store_v1.h:20-28)
+FLAG_OCCUPIED, FLAG_ALIVE, FLAG_AWAKE, FLAG_STUCK = 1, 2, 4, 8
+DEATH_NONE, DEATH_STARVED, DEATH_PREDATED, DEATH_OVERHEATED, DEATH_CULLED = range(5)
+
+# Cell SoA in contract-v1 order (contracts/cell_store_v1.h:42-74).
+CELL_FIELDS = [
+    ("id", np.uint64), ("flags", np.uint32), ("death_cause", np.uint8),
+    ("x", np.float64), ("y", np.float64), ("z", np.float64),
+    ("vx", np.float64), ("vy", np.float64), ("vz", np.float64),
+    ("energy", np.float64), ("temp_cell", np.float32), ("emit_power", np.float32),
+    ("dir_x", np.float32), ("dir_y", np.float32), ("dir_z", np.float32),
+    ("biomass", np.float64), ("co2_held", np.float64), ("age_s", np.float32),
+    ("rng_state", np.uint64), ("taxis_memory", np.float32), ("run_timer", np.float32),
+    ("t_local", np.float32), ("irradiance", np.float32), ("co2_local", np.float32),
+    ("n2_local", np.float32),
+]
+
+
+class AstroError(RuntimeError):
+    def __init__(self, status: int, detail: str = ""):
+        name = STATUS[status] if 0 <= status < len(STATUS) else f"status {status}"
+        super().__init__(f"{name}: {detail}" if detail else name)
+        self.status = status
+
+
+class GridDims(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32)]
+
+
+class WorldDesc(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_int32), ("capacity", C.c_int32), ("seed", C.c_uint64),
+        ("chamber_w", C.c_double), ("chamber_h", C.c_double), ("chamber_d", C.c_double),
+        ("precision", C.c_int32), ("_pad0", C.c_int32), ("hash_cell_size", C.c_double),
+        ("grid", GridDims * 4), ("_pad1", C.c_int32),
+        ("boundary_x", C.c_uint8), ("boundary_y", C.c_uint8), ("gravity_axis", C.c_uint8),
+        ("thermal_noise", C.c_uint8), ("contact_enabled", C.c_uint8), ("adhesion_enabled", C.c_uint8),
+        ("thermal_enabled", C.c_uint8), ("emission_enabled", C.c_uint8), ("taxis_enabled", C.c_uint8),
+        ("occlusion_exact", C.c_uint8), ("store_disposition", C.c_uint8), ("compaction_enabled", C.c_uint8),
+        ("lifecycle_enabled", C.c_uint8), ("_pad2", C.c_uint8 * 7),
+        ("ambient_temp", C.c_double), ("co2_init", C.c_double), ("division_offset", C.c_double),
+        ("physics_rate", C.c_double), ("biology_rate", C.c_double),
+        ("petrova_max_power", C.c_double), ("co2_mass_per_division", C.c_double),
+        ("light_dir_x", C.c_float), ("light_dir_y", C.c_float), ("light_irradiance", C.c_float),
+        ("ambient_irradiance", C.c_float), ("light_enabled", C.c_uint8), ("_pad3", C.c_uint8 * 7),
+    ]
+
+
+class SpawnParams(C.Structure):
+    _fields_ = [
+        ("count", C.c_int32), ("placement", C.c_int32),
+        ("place_x", C.c_double), ("place_y", C.c_double), ("place_radius", C.c_double),
+        ("charge_dist", C.c_int32), ("awake", C.c_int32),
+        ("charge_a", C.c_double), ("charge_b", C.c_double),
+    ]
+
+
+class CellSoA(C.Structure):
+    _fields_ = [(name, C.c_void_p) for name, _ in CELL_FIELDS]
+
+
+class Counters(C.Structure):
+    _fields_ = [
+        ("count", C.c_int32), ("capacity", C.c_int32), ("next_id", C.c_uint64), ("tick", C.c_uint64),
+        ("sim_time_s", C.c_double), ("divisions_total", C.c_int64), ("deaths_total", C.c_int64),
+        ("dead_slots_prev", C.c_int32), ("_pad", C.c_int32),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("tick", C.c_uint64), ("sim_time_s", C.c_double),
+        ("n_live", C.c_int32), ("n_dead", C.c_int32), ("n_dormant", C.c_int32), ("n_awake", C.c_int32),
+        ("n_taumoeba", C.c_int32), ("_pad0", C.c_int32),
+        ("mean_charge", C.c_double), ("total_energy_j", C.c_double), ("mean_temp_cell_k", C.c_double),
+        ("mean_temp_medium_k", C.c_double), ("max_temp_medium_k", C.c_double), ("co2_total_kg", C.c_double),
+        ("n2_total_kg", C.c_double), ("mean_tau_tolerance", C.c_double),
+        ("divisions_this_window", C.c_int32), ("deaths_this_window", C.c_int32),
+        ("boil_events", C.c_int32), ("ignitions_this_window", C.c_int32),
+        ("physics_rate", C.c_double), ("biology_rate", C.c_double),
+        ("non_canon_run", C.c_uint8), ("_pad1", C.c_uint8 * 7),
+    ]
+
+
+assert C.sizeof(WorldDesc) == 208, C.sizeof(WorldDesc)
+assert C.sizeof(SpawnParams) == 56
+assert C.sizeof(Counters) == 56
+assert C.sizeof(Stats) == 144
+
+# Every entry point of include/astro_hot.h: name -> (restype, argtypes).
+_P = C.c_void_p
+_I = C.c_int32
+_D = C.c_double
+SIGNATURES = {
+    "world_desc_default": (None, [C.POINTER(WorldDesc)]),
+    "world_create": (_I, [C.POINTER(WorldDesc), C.POINTER(_P)]),
+    "world_destroy": (None, [_P]),
+    "status_str": (C.c_char_p, [_I]),
+    "last_error": (C.c_char_p, [_P]),
+    "world_desc_get": (_I, [_P, C.POINTER(WorldDesc)]),
+    "cells_spawn": (_I, [_P, C.POINTER(SpawnParams), C.c_uint64]),
+    "cells_upload": (_I, [_P, C.POINTER(CellSoA), _I]),
+    "cells_download": (_I, [_P, C.POINTER(CellSoA), _I, C.POINTER(_I)]),
+    "world_get_counters": (_I, [_P, C.POINTER(Counters)]),
+    "world_set_counters": (_I, [_P, C.POINTER(Counters)]),
+    "field_dims": (_I, [_P, _I, C.POINTER(GridDims), C.POINTER(_D), C.POINTER(_D), C.POINTER(_D)]),
+    "field_upload": (_I, [_P, _I, _P, C.c_size_t]),
+    "field_download": (_I, [_P, _I, _P, C.c_size_t]),
+    "field_deposit_download": (_I, [_P, _I, _P, C.c_size_t]),
+    "world_step": (_I, [_P, _I]),
+    "world_sync": (_I, [_P]),
+    "world_stats": (_I, [_P, C.POINTER(Stats)]),
+    "world_state_hash": (_I, [_P, C.POINTER(C.c_uint64)]),
+    "world_set_clock": (_I, [_P, _D, _D]),
+    "stage_hash_build": (_I, [_P]),
+    "hash_dims": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_D)]),
+    "hash_download": (_I, [_P, _P, _I, _P, _P, _I]),
+    "stage_thermal": (_I, [_P, _D]),
+    "stage_emission": (_I, [_P, _D]),
+    "stage_taxis": (_I, [_P, _D]),
+    "stage_forces": (_I, [_P]),
+    "forces_download": (_I, [_P, _P, _P, _P, _I]),
+    "stage_integrate": (_I, [_P, _D]),
+    "stage_motion": (_I, [_P, _D]),
+    "stage_slow_fields": (_I, [_P, _D]),
+    "stage_diffuse": (_I, [_P, _I, _D]),
+    "stage_lifecycle": (_I, [_P, _D]),
+    "stage_compact": (_I, [_P]),
+}
+PRODUCT_ONLY = {
+    "profile_enable": (_I, [_P, _I]),
+    "profile_read": (_I, [_P, _P, _I, _P, _P, _I, C.POINTER(_I)]),
+    "profile_reset": (_I, [_P]),
+    "world_stream": (_P, [_P]),
+}
+
+
+class Binding:
+    """The ABI of one shared library whose symbols carry `prefix`."""
+
+    def __init__(self, path: Path | str, prefix: str = "astro_", extra: dict | None = None):
+        path = Path(path)
+        if not path.exists():
+            raise FileNotFoundError(f"{path} is not built")
+        self.path = path
+        self.lib = C.CDLL(str(path))
+        self.prefix = prefix
+        sigs = dict(SIGNATURES)
+        if extra:
+            sigs.update(extra)
+        for name, (res, args) in sigs.items():
+            fn = getattr(self.lib, prefix + name)
+            fn.restype = res
+            fn.argtypes = args
+            setattr(self, name, fn)
+
+    def desc(self, **kw) -> WorldDesc:
+        """A WorldDesc with the reference defaults, overridden by `kw`.
+
+        `grid` takes {field_id: (nx, ny, nz)}."""
+        d = WorldDesc()
+        self.world_desc_default(C.byref(d))
+        grids = kw.pop("grid", {})
+        for k, v in kw.items():
+            if not hasattr(d, k):
+                raise KeyError(k)
+            setattr(d, k, v)
+        for f, dims in grids.items():
+            d.grid[f].nx, d.grid[f].ny, d.grid[f].nz = dims
+        return d
+
+
+_PRODUCT: Binding | None = None
+
+
+def load() -> Binding:
+    """Binds libastro_hot.so. Raises if it is not built or no GPU is usable."""
+    global _PRODUCT
+    if _PRODUCT is None:
+        if not LIB_PATH.exists():
+            raise FileNotFoundError(
+                f"{LIB_PATH} is missing: build it with `python astrophage-microscope_amd/build.py`")
+        _PRODUCT = Binding(LIB_PATH, "astro_", PRODUCT_ONLY)
+    return _PRODUCT
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class World:
+    """One simulated chamber (src/sim/world.cuh:89-180) behind the C ABI."""
+
+    def __init__(self, binding: Binding, desc: WorldDesc | None = None, **kw):
+        self.b = binding
+        if desc is None:
+            desc = binding.desc(**kw)
+        h = C.c_void_p()
+        self._check(binding.world_create(C.byref(desc), C.byref(h)), what="world_create")
+        self.h = h
+        d = WorldDesc()
+        self._check(binding.world_desc_get(self.h, C.byref(d)))
+        self.desc = d
+
+    # -- plumbing ----------------------------------------------------------
+    def _check(self, st: int, what: str = ""):
+        if st != OK:
+            detail = ""
+            if getattr(self, "h", None):
+                detail = (self.b.last_error(self.h) or b"").decode()
+            raise AstroError(st, detail or what)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.b.world_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- population --------------------------------------------------------
+    def spawn(self, count: int, seed: int, placement: int = PLACE_UNIFORM, place_x: float = 0.0,
+              place_y: float = 0.0, place_radius: float = 0.0, charge_dist: int = DIST_CONSTANT,
+              charge_a: float = 0.0, charge_b: float = 0.0, awake: bool = False):
+        p = SpawnParams(count=count, placement=placement, place_x=place_x, place_y=place_y,
+                        place_radius=place_radius, charge_dist=charge_dist, awake=int(awake),
+                        charge_a=charge_a, charge_b=charge_b)
+        self._check(self.b.cells_spawn(self.h, C.byref(p), C.c_uint64(seed)), "spawn")
+
+    def upload(self, cells: dict[str, np.ndarray]):
+        n = len(cells["id"])
+        arrays = {}
+        soa = CellSoA()
+        for name, dt in CELL_FIELDS:
+            a = cells.get(name)
+            if a is None:
+                a = np.zeros(n, dt)
+            a = np.ascontiguousarray(a, dtype=dt)
+            if len(a) != n:
+                raise ValueError(f"{name} has {len(a)} entries, expected {n}")
+            arrays[name] = a
+            setattr(soa, name, _ptr(a))
+        self._check(self.b.cells_upload(self.h, C.byref(soa), n), "upload")
+
+    def download(self) -> dict[str, np.ndarray]:
+        n = self.counters().count
+        out = {name: np.zeros(n, dt) for name, dt in CELL_FIELDS}
+        soa = CellSoA()
+        for name, _ in CELL_FIELDS:
+            setattr(soa, name, _ptr(out[name]))
+        got = C.c_int32()
+        self._check(self.b.cells_download(self.h, C.byref(soa), n, C.byref(got)), "download")
+        return out
+
+    def counters(self) -> Counters:
+        c = Counters()
+        self._check(self.b.world_get_counters(self.h, C.byref(c)))
+        return c
+
+    def set_counters(self, c: Counters):
+        self._check(self.b.world_set_counters(self.h, C.byref(c)))
+
+    # -- fields -----------------------------------------------------------
+    def field_dims(self, f: int):
+        g = GridDims()
+        dx, dy, dz = C.c_double(), C.c_double(), C.c_double()
+        self._check(self.b.field_dims(self.h, f, C.byref(g), C.byref(dx), C.byref(dy), C.byref(dz)))
+        return (g.nx, g.ny, g.nz), (dx.value, dy.value, dz.value)
+
+    def field(self, f: int) -> np.ndarray:
+        (nx, ny, nz), _ = self.field_dims(f)
+        a = np.zeros(nx * ny * nz, np.float32)
+        self._check(self.b.field_download(self.h, f, _ptr(a), a.size), "field_download")
+        return a.reshape(nz, ny, nx)
+
+    def set_field(self, f: int, v: np.ndarray):
+        a = np.ascontiguousarray(v, np.float32).ravel()
+        self._check(self.b.field_upload(self.h, f, _ptr(a), a.size), "field_upload")
+
+    def deposit(self, f: int) -> np.ndarray:
+        (nx, ny, nz), _ = self.field_dims(f)
+        a = np.zeros(nx * ny * nz, np.int64)
+        self._check(self.b.field_deposit_download(self.h, f, _ptr(a), a.size))
+        return a.reshape(nz, ny, nx)
+
+    # -- stepping ----------------------------------------------------------
+    def step(self, n: int = 1):
+        self._check(self.b.world_step(self.h, n), "step")
+
+    def sync(self):
+        self._check(self.b.world_sync(self.h), "sync")
+
+    def set_clock(self, physics: float, biology: float):
+        self._check(self.b.world_set_clock(self.h, physics, biology))
+
+    @property
+    def dt(self) -> float:
+        return 1.0e-3 * self.desc.physics_rate
+
+    def stats(self) -> dict:
+        s = Stats()
+        self._check(self.b.world_stats(self.h, C.byref(s)), "stats")
+        return {name: getattr(s, name) for name, _ in Stats._fields_ if not name.startswith("_")}
+
+    def state_hash(self) -> int:
+        v = C.c_uint64()
+        self._check(self.b.world_state_hash(self.h, C.byref(v)), "state_hash")
+        return v.value
+
+    # -- stages --------------------------------------------------------------
+    def hash_dims(self):
+        nx, ny, nz, cell = C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
+        self._check(self.b.hash_dims(self.h, C.byref(nx), C.byref(ny), C.byref(nz), C.byref(cell)))
+        return nx.value, ny.value, nz.value, cell.value
+
+    def hash_build(self):
+        self._check(self.b.stage_hash_build(self.h), "hash_build")
+
+    def hash_download(self):
+        n = self.counters().count
+        nx, ny, nz, _ = self.hash_dims()
+        nb = nx * ny * nz
+        vals = np.zeros(n, np.uint32)
+        bs = np.zeros(nb, np.int32)
+        be = np.zeros(nb, np.int32)
+        self._check(self.b.hash_download(self.h, _ptr(vals), n, _ptr(bs), _ptr(be), nb))
+        return vals, bs, be
+
+    def stage(self, name: str, *args):
+        fn = getattr(self.b, "stage_" + name)
+        self._check(fn(self.h, *args), name)
+
+    def forces(self):
+        n = self.counters().count
+        fx, fy, fz = (np.zeros(n) for _ in range(3))
+        self._check(self.b.forces_download(self.h, _ptr(fx), _ptr(fy), _ptr(fz), n))
+        return fx, fy, fz
+
+    # -- measurement (product library only) --------------------------------
+    def profile(self, on: bool):
+        self._check(self.b.profile_enable(self.h, int(on)))
+
+    def profile_reset(self):
+        self._check(self.b.profile_reset(self.h))
+
+    def profile_read(self, max_kernels: int = 64) -> dict[str, tuple[int, float]]:
+        stride = 64
+        names = C.create_string_buffer(stride * max_kernels)
+        launches = np.zeros(max_kernels, np.int64)
+        ms = np.zeros(max_kernels, np.float64)
+        n = C.c_int32()
+        self._check(self.b.profile_read(self.h, names, stride, _ptr(launches), _ptr(ms), max_kernels, C.byref(n)))
+        out = {}
+        for k in range(n.value):
+            name = names.raw[k * stride:(k + 1) * stride].split(b"\0", 1)[0].decode()
+            out[name] = (int(launches[k]), float(ms[k]))
+        return out

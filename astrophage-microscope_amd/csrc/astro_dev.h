@@ -1,0 +1,338 @@
+// astro_dev.h -- device-side data layout and physics of the MI355X hot path.
+//
+// Layout in HBM (one world):
+//   * cell SoA in slot order, one 256 B-aligned carve per array, in the
+//     reference's contract order (contracts/cell_store_v1.h:42-74). Position
+//     and velocity are `P` (double for the contract layout, float for the hot
+//     layout); every other array keeps its contract type.
+//   * a second SoA of the same shape, the out-of-place compaction target.
+//   * the spatial hash as bucket-sorted neighbour records (position + slot +
+//     alive bit), a dense bucket prefix `bstart[nb + 2]`, and the slot ->
+//     sorted-position map `spos`, rebuilt every tick.
+//   * per field: value, ping-pong scratch, int64 fixed-point accumulator.
+//   * a small Counters block: the population count, next id and the per-tick
+//     tallies live on the device, so a tick needs no host synchronisation.
+//
+// All arithmetic on positions, velocities and energies is double precision,
+// whatever `P` is; `P` only sets the storage width. Compiled with
+// -ffp-contract=off so expressions round as written.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "astro_canon.h"
+#include "astro_hot.h"
+
+namespace astro_dev {
+
+// ---------------------------------------------------------------------------
+// Views passed by value to kernels
+// ---------------------------------------------------------------------------
+template <class P>
+struct Cells {
+    uint64_t* id;
+    uint32_t* flags;
+    uint8_t*  death_cause;
+    P *x, *y, *z, *vx, *vy, *vz;
+    double*   energy;
+    float*    temp_cell;
+    float*    emit_power;
+    float *dir_x, *dir_y, *dir_z;
+    double*   biomass;
+    double*   co2_held;
+    float*    age_s;
+    uint64_t* rng_state;
+    float*    taxis_memory;
+    float*    run_timer;
+    float *t_local, *irradiance, *co2_local, *n2_local;
+};
+
+// Device-resident bookkeeping (src/sim/cell_store.cuh:36-59,
+// src/sim/world.cuh:111-121), so no tick has to read a count back.
+struct Counters {
+    int32_t  count;            // slots [0, count) may be occupied
+    int32_t  capacity;
+    uint64_t next_id;
+    int64_t  divisions_total;
+    int64_t  deaths_total;
+    int32_t  dead_slots_prev;
+    int32_t  births;           // this tick (unclamped)
+    int32_t  deads;            // occupied-but-dead slots this tick
+    int32_t  live;             // compaction survivors
+    int32_t  compacting;       // 1 while this tick compacts
+    uint32_t scan_epoch;       // look-back scan generation, never reset
+    uint32_t error_flags;      // sticky device-side errors (see ERR_*)
+    int32_t  _pad;
+};
+
+enum : uint32_t { ERR_SCAN_SPIN = 1u, ERR_COEFF = 2u };
+
+// One scalar field (contracts/fields_v1.h:164-174), extended to nz planes.
+struct Grid {
+    float* value;
+    float* scratch;
+    unsigned long long* deposit;
+    int32_t nx, ny, nz;
+    float   fdx, fdy, fdz;
+    double  scale;
+};
+
+// Spatial-hash geometry (src/sim/hash.cuh:25-34).
+struct HashGeo {
+    int32_t nx, ny, nz, nb;
+    float   inv, ox, oy, oz;
+};
+
+// A bucket-sorted neighbour record: position plus slot (bits 0-29),
+// "unoccupied" (bit 30) and "alive" (bit 31).
+template <class P> struct Rec;
+template <> struct alignas(32) Rec<double> { double x, y, z; uint32_t meta, _pad; };
+template <> struct alignas(16) Rec<float>  { float x, y, z; uint32_t meta; };
+
+constexpr uint32_t REC_SLOT_MASK = 0x3FFFFFFFu;
+constexpr uint32_t REC_UNOCC = 1u << 30;
+constexpr uint32_t REC_ALIVE = 1u << 31;
+
+struct alignas(32) Force { double x, y, z, _pad; };
+
+// ---------------------------------------------------------------------------
+// PCG32 per-cell streams (src/core/rng.cuh:23-117)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+struct Pcg {
+    uint64_t state, inc;
+    __device__ __forceinline__ uint32_t next() {
+        const uint64_t old = state;
+        state = old * 6364136223846793005ull + inc;
+        const uint32_t xs = static_cast<uint32_t>(((old >> 18) ^ old) >> 27);
+        const uint32_t rot = static_cast<uint32_t>(old >> 59);
+        return (xs >> rot) | (xs << ((0u - rot) & 31u));
+    }
+    __device__ __forceinline__ double uniform01d() {
+        const uint64_t hi = static_cast<uint64_t>(next()) << 21;
+        const uint64_t lo = static_cast<uint64_t>(next()) >> 11;
+        return static_cast<double>((hi ^ lo) & ((1ull << 53) - 1)) * (1.0 / 9007199254740992.0);
+    }
+    __device__ __forceinline__ double uniform(double lo, double hi) { return lo + (hi - lo) * uniform01d(); }
+    __device__ __forceinline__ void gaussian_pair(double& z0, double& z1) {
+        double u1 = uniform01d();
+        if (u1 < 1.0e-300) u1 = 1.0e-300;
+        const double u2 = uniform01d();
+        const double mag = sqrt(-2.0 * log(u1));
+        const double ang = 6.283185307179586476925286766559 * u2;
+        double s, c;
+        sincos(ang, &s, &c);
+        z0 = mag * c;
+        z1 = mag * s;
+    }
+};
+
+__device__ __forceinline__ uint64_t stream_inc(uint64_t id) {
+    return (splitmix64(id ^ 0xA5A5A5A5DEADBEEFull) << 1u) | 1u;
+}
+__device__ __forceinline__ Pcg cell_rng(uint64_t st, uint64_t id) { return Pcg{st, stream_inc(id)}; }
+__device__ __forceinline__ uint64_t cell_rng_init(uint64_t seed, uint64_t id) {
+    Pcg r{0ull, (splitmix64(id + 1) << 1u) | 1u};
+    r.next();
+    r.state += seed ^ splitmix64(id);
+    r.next();
+    return r.state;
+}
+__device__ __forceinline__ uint64_t pcg_split(uint64_t parent, uint64_t did) {
+    return splitmix64(parent ^ splitmix64(did * 0x2545F4914F6CDD1Dull));
+}
+
+// ---------------------------------------------------------------------------
+// Fixed point (src/core/fixed_atomic.cuh:27-44)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int64_t to_fixed(double v, double scale) { return static_cast<int64_t>(llround(v * scale)); }
+__device__ __forceinline__ double from_fixed(unsigned long long raw, double scale) {
+    return static_cast<double>(static_cast<int64_t>(raw)) / scale;
+}
+__device__ __forceinline__ void deposit_add(unsigned long long* acc, double v, double scale) {
+    atomicAdd(acc, static_cast<unsigned long long>(to_fixed(v, scale)));
+}
+
+// ---------------------------------------------------------------------------
+// Physics (src/sim/integrator.cuh, thermal.cuh, emission.cuh, taxis.cuh,
+// lifecycle.cuh), same expressions as the reference so they round alike.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double water_viscosity(double t) { return ASTRO_VF_A * exp(ASTRO_VF_B / (t - ASTRO_VF_C)); }
+__device__ __forceinline__ double drag_coefficient(double t) { return 6.0 * ASTRO_PI * water_viscosity(t) * ASTRO_CELL_RADIUS; }
+__device__ __forceinline__ double cell_mass(double biomass, double energy) {
+    return biomass + energy / (ASTRO_C_LIGHT * ASTRO_C_LIGHT);
+}
+__device__ __forceinline__ double buoyant_weight(double mass) {
+    return -(mass - ASTRO_WATER_DENSITY * ASTRO_CELL_VOLUME) * ASTRO_G_ACCEL;
+}
+
+__device__ __forceinline__ double ou_position_shape(double x) {
+    if (x < 1.0e-2) {
+        const double x3 = x * x * x;
+        return x3 * (2.0 / 3.0 + x * (-0.5 + x * (7.0 / 30.0 + x * (-1.0 / 12.0))));
+    }
+    const double e = exp(-x);
+    return 2.0 * x - 3.0 + 4.0 * e - e * e;
+}
+
+struct Ou { double one_minus_e1, inv_beta, sigma_rr, sigma_vv, sigma_rv; };
+
+__device__ __forceinline__ Ou ou_coefficients(double mass, double gamma, double t, double dt) {
+    const double beta = gamma / mass;
+    const double x = beta * dt;
+    Ou c;
+    c.one_minus_e1 = -expm1(-x);
+    c.inv_beta = 1.0 / beta;
+    const double kT_over_m = ASTRO_K_BOLTZ * t / mass;
+    const double var_vv = kT_over_m * (-expm1(-2.0 * x));
+    const double var_rr = kT_over_m * c.inv_beta * c.inv_beta * ou_position_shape(x);
+    c.sigma_vv = sqrt(var_vv > 0.0 ? var_vv : 0.0);
+    c.sigma_rr = sqrt(var_rr > 0.0 ? var_rr : 0.0);
+    c.sigma_rv = kT_over_m * c.inv_beta * c.one_minus_e1 * c.one_minus_e1;
+    return c;
+}
+
+__device__ __forceinline__ void ou_advance_axis(double& pos, double& vel, double force, double gamma,
+                                                const Ou& c, double dt, bool noise, double z1, double z2) {
+    const double v_term = force / gamma;
+    const double dpos = v_term * dt + (vel - v_term) * c.one_minus_e1 * c.inv_beta;
+    const double dvel = (v_term - vel) * c.one_minus_e1;
+    double npos = pos + dpos;
+    double nvel = vel + dvel;
+    if (noise && c.sigma_rr > 0.0) {
+        const double a = c.sigma_rv / c.sigma_rr;
+        const double b2 = c.sigma_vv * c.sigma_vv - a * a;
+        const double b = sqrt(b2 > 0.0 ? b2 : 0.0);
+        npos += c.sigma_rr * z1;
+        nvel += a * z1 + b * z2;
+    } else if (noise) {
+        nvel += c.sigma_vv * z2;
+    }
+    pos = npos;
+    vel = nvel;
+}
+
+__device__ __forceinline__ bool apply_boundary_axis(double& p, double& v, double half, double radius, int mode) {
+    const double lo = -half + radius;
+    const double hi = half - radius;
+    if (p >= lo && p <= hi) return true;
+    if (mode == ASTRO_BOUNDARY_PERIODIC) {
+        const double span = 2.0 * half;
+        while (p < -half) p += span;
+        while (p > half) p -= span;
+        return true;
+    }
+    if (mode == ASTRO_BOUNDARY_ABSORBING) return false;
+    p = (p < lo) ? lo : hi;
+    v = 0.0;
+    return true;
+}
+
+__device__ __forceinline__ double lumped_exchange_energy(double tc, double tg, double g, double cap, double dt) {
+    if (cap <= 0.0 || g <= 0.0) return 0.0;
+    const double relax = -expm1(-g * dt / cap);
+    return cap * (tc - tg) * relax;
+}
+
+__device__ __forceinline__ double disc_overlap_fraction(double p, double a) {
+    if (p <= 0.0) return 1.0;
+    if (p >= 2.0 * a) return 0.0;
+    const double x = p / (2.0 * a);
+    return (2.0 / ASTRO_PI) * (acos(x) - x * sqrt(1.0 - x * x));
+}
+
+__device__ __forceinline__ double column_transmittance(double blocked, double face) {
+    if (face <= 0.0) return 1.0;
+    const double t = 1.0 - blocked / face;
+    return t > 0.0 ? t : 0.0;
+}
+
+__device__ __forceinline__ double co2_uptake_rate(double c) {
+    if (c <= 0.0) return 0.0;
+    return ASTRO_LIFE_CO2_UPTAKE_MAX * c / (c + ASTRO_LIFE_CO2_HALF_SATURATION);
+}
+
+// ---------------------------------------------------------------------------
+// Field access (src/fields/grid.cuh:109-141), trilinear/voxel when nz > 1.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int clampi(int v, int n) { return v < 0 ? 0 : (v >= n ? n - 1 : v); }
+
+__device__ __forceinline__ float grid_sample(const Grid& g, float x, float y, float z) {
+    const float half_x = 0.5f * g.nx * g.fdx;
+    const float half_y = 0.5f * g.ny * g.fdy;
+    const float gx = (x + half_x) / g.fdx - 0.5f;
+    const float gy = (y + half_y) / g.fdy - 0.5f;
+    const int x0 = static_cast<int>(floorf(gx));
+    const int y0 = static_cast<int>(floorf(gy));
+    const float tx = gx - static_cast<float>(x0);
+    const float ty = gy - static_cast<float>(y0);
+    const int i0 = clampi(x0, g.nx), i1 = clampi(x0 + 1, g.nx);
+    const int j0 = clampi(y0, g.ny), j1 = clampi(y0 + 1, g.ny);
+    const size_t pl = static_cast<size_t>(g.nx) * g.ny;
+    auto plane = [&](int k) -> float {
+        const float* v = g.value + static_cast<size_t>(k) * pl;
+        return (1.0f - tx) * (1.0f - ty) * v[j0 * g.nx + i0] + tx * (1.0f - ty) * v[j0 * g.nx + i1] +
+               (1.0f - tx) * ty * v[j1 * g.nx + i0] + tx * ty * v[j1 * g.nx + i1];
+    };
+    if (g.nz == 1) return plane(0);
+    const float half_z = 0.5f * g.nz * g.fdz;
+    const float gz = (z + half_z) / g.fdz - 0.5f;
+    const int z0 = static_cast<int>(floorf(gz));
+    const float tz = gz - static_cast<float>(z0);
+    const float b0 = plane(clampi(z0, g.nz));
+    const float b1 = plane(clampi(z0 + 1, g.nz));
+    return (1.0f - tz) * b0 + tz * b1;
+}
+
+__device__ __forceinline__ uint32_t grid_nearest(const Grid& g, float x, float y, float z) {
+    const float half_x = 0.5f * g.nx * g.fdx;
+    const float half_y = 0.5f * g.ny * g.fdy;
+    const int i = clampi(static_cast<int>((x + half_x) / g.fdx), g.nx);
+    const int j = clampi(static_cast<int>((y + half_y) / g.fdy), g.ny);
+    int k = 0;
+    if (g.nz > 1) {
+        const float half_z = 0.5f * g.nz * g.fdz;
+        k = clampi(static_cast<int>((z + half_z) / g.fdz), g.nz);
+    }
+    return static_cast<uint32_t>((static_cast<size_t>(k) * g.ny + j) * g.nx + i);
+}
+
+// ---------------------------------------------------------------------------
+// Spatial hash (src/sim/hash.cuh:59-78)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void hash_coords(const HashGeo& h, double x, double y, double z,
+                                            int& gx, int& gy, int& gz) {
+    auto cl = [](int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); };
+    gx = cl(static_cast<int32_t>((x - static_cast<double>(h.ox)) * static_cast<double>(h.inv)), h.nx - 1);
+    gy = cl(static_cast<int32_t>((y - static_cast<double>(h.oy)) * static_cast<double>(h.inv)), h.ny - 1);
+    gz = cl(static_cast<int32_t>((z - static_cast<double>(h.oz)) * static_cast<double>(h.inv)), h.nz - 1);
+}
+__device__ __forceinline__ uint32_t hash_key(const HashGeo& h, double x, double y, double z) {
+    int gx, gy, gz;
+    hash_coords(h, x, y, z, gx, gy, gz);
+    return static_cast<uint32_t>((gz * h.ny + gy) * h.nx + gx);
+}
+
+// ---------------------------------------------------------------------------
+// Launch helpers
+// ---------------------------------------------------------------------------
+constexpr int BLOCK = 256;
+constexpr int NUM_XCD = 8;
+
+// Remaps a linear block id so each XCD (blocks are dealt to the 8 XCDs round
+// robin) gets one contiguous run of tiles, which keeps neighbouring tiles in
+// the same L2. Launch `blocks_for_xcd(n)` blocks; ids >= n come back >= n.
+__host__ __device__ __forceinline__ int blocks_for_xcd(int n) { return ((n + NUM_XCD - 1) / NUM_XCD) * NUM_XCD; }
+__device__ __forceinline__ int xcd_tile(int b, int nlaunched) {
+    const int per = nlaunched / NUM_XCD;
+    return (b % NUM_XCD) * per + b / NUM_XCD;
+}
+
+}  // namespace astro_dev

@@ -1,0 +1,213 @@
+// kernels_fields.hip -- FTCS diffusion, the light sweep and occlusion stamp,
+// and the field half of the telemetry reduction.
+//
+// FTCS (src/fields/diffuse.cu:19-44): one explicit ping-pong substep. The
+// square depth-averaged grid keeps the reference's 5-point expression so it
+// rounds identically; a grid with nz > 1 uses the 7-point form
+//   u' = u + cx((l+r)-2u) + cy((d+u)-2u) + cz((b+f)-2u).
+// Both are pure HBM streams (4 B read + 4 B written per point and substep):
+// blocks are 64 x 4 threads so each wave reads one contiguous 256 B row, the
+// 3D kernel marches along z keeping the column in registers, and tiles are
+// remapped so each XCD streams one contiguous slab (neighbour rows hit L2).
+#include "kernels.h"
+
+namespace astro_dev {
+
+namespace {
+
+__device__ __forceinline__ float ghost_of(float c, int bc, float amb, float robin) {
+    if (bc == ASTRO_BC_DIRICHLET) return amb;
+    if (bc == ASTRO_BC_ROBIN) return c - robin * (c - amb);
+    return c;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_ftcs2d(const float* __restrict__ src, float* __restrict__ dst, int n,
+                                                  float coeff, int bc, float amb, float robin) {
+    const int tx = (n + 63) / 64, ty = (n + 3) / 4;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    if (t >= tx * ty) return;
+    const int i = (t % tx) * 64 + (threadIdx.x & 63);
+    const int j = (t / tx) * 4 + (threadIdx.x >> 6);
+    if (i >= n || j >= n) return;
+    const size_t id = static_cast<size_t>(j) * n + i;
+    const float c = src[id];
+    const float l = (i > 0) ? src[id - 1] : ghost_of(c, bc, amb, robin);
+    const float r = (i < n - 1) ? src[id + 1] : ghost_of(c, bc, amb, robin);
+    const float d = (j > 0) ? src[id - n] : ghost_of(c, bc, amb, robin);
+    const float u = (j < n - 1) ? src[id + n] : ghost_of(c, bc, amb, robin);
+    dst[id] = c + coeff * (l + r + d + u - 4.0f * c);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_ftcs3d(const float* __restrict__ src, float* __restrict__ dst, int nx,
+                                                  int ny, int nz, int zc, float cx, float cy, float cz, int bc,
+                                                  float amb, float rx, float ry, float rz) {
+    const int tx = (nx + 63) / 64, ty = (ny + 3) / 4, tz = (nz + zc - 1) / zc;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    if (t >= tx * ty * tz) return;
+    const int i = (t % tx) * 64 + (threadIdx.x & 63);
+    const int j = ((t / tx) % ty) * 4 + (threadIdx.x >> 6);
+    const int k0 = (t / (tx * ty)) * zc;
+    if (i >= nx || j >= ny) return;
+    const int k1 = k0 + zc < nz ? k0 + zc : nz;
+    const size_t pl = static_cast<size_t>(nx) * ny;
+    const size_t col = static_cast<size_t>(j) * nx + i;
+    float below = (k0 > 0) ? src[(k0 - 1) * pl + col] : 0.0f;
+    float cur = src[k0 * pl + col];
+    for (int k = k0; k < k1; ++k) {
+        const size_t id = k * pl + col;
+        const float above = (k < nz - 1) ? src[id + pl] : 0.0f;
+        const float c = cur;
+        const float l = (i > 0) ? src[id - 1] : ghost_of(c, bc, amb, rx);
+        const float r = (i < nx - 1) ? src[id + 1] : ghost_of(c, bc, amb, rx);
+        const float d = (j > 0) ? src[id - nx] : ghost_of(c, bc, amb, ry);
+        const float u = (j < ny - 1) ? src[id + nx] : ghost_of(c, bc, amb, ry);
+        float acc = cx * ((l + r) - 2.0f * c) + cy * ((d + u) - 2.0f * c);
+        if (nz > 1) {
+            const float b = (k > 0) ? below : ghost_of(c, bc, amb, rz);
+            const float f = (k < nz - 1) ? above : ghost_of(c, bc, amb, rz);
+            acc = acc + cz * ((b + f) - 2.0f * c);
+        }
+        dst[id] = c + acc;
+        below = cur;
+        cur = above;
+    }
+}
+
+// occlusion_stamp_kernel (src/sim/emission.cu:17-27).
+template <class P>
+__global__ __launch_bounds__(BLOCK) void k_stamp(Cells<P> c, const Counters* ctr, Grid irr) {
+    const int n = ctr->count;
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t f = c.flags[i];
+    if (!(f & ASTRO_FLAG_OCCUPIED)) return;
+    const double area = (f & ASTRO_FLAG_ALIVE) ? ASTRO_CELL_CROSS_SECTION
+                                               : ASTRO_CELL_CROSS_SECTION * (1.0 - ASTRO_CELL_ALBEDO_DEAD);
+    const uint32_t idx = grid_nearest(irr, static_cast<float>(c.x[i]), static_cast<float>(c.y[i]),
+                                      static_cast<float>(c.z[i]));
+    deposit_add(&irr.deposit[idx], area, irr.scale);
+}
+
+// sweep_kernel (src/sim/emission.cu:37-53): one sequential Beer-Lambert march
+// per line, the product kept in the reference's order so transmittances (and
+// an exact 0) match bit for bit. The occlusion accumulator is consumed here:
+// nonzero entries are cleared as they are read, which replaces the reference's
+// dense clear_kernel at the start of the next tick.
+__global__ __launch_bounds__(BLOCK) void k_sweep(unsigned long long* blocked, float* out, int nx, int ny, int nz,
+                                                 int axis, int sign, double scale, double face, float source,
+                                                 float ambient) {
+    const int other = axis == 0 ? ny : nx;
+    const int len = axis == 0 ? nx : ny;
+    const int line = blockIdx.x * BLOCK + threadIdx.x;
+    if (line >= other * nz) return;
+    const int k = line / other, l = line % other;
+    double t = 1.0;
+    for (int s = 0; s < len; ++s) {
+        const int step = sign > 0 ? s : (len - 1 - s);
+        const size_t idx = axis == 0 ? (static_cast<size_t>(k) * ny + l) * nx + step
+                                     : (static_cast<size_t>(k) * ny + step) * nx + l;
+        out[idx] = static_cast<float>(static_cast<double>(source) * t) + ambient;
+        const unsigned long long raw = blocked[idx];
+        if (raw != 0ull) blocked[idx] = 0ull;
+        t *= column_transmittance(from_fixed(raw, scale), face);
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_fill(float* v, size_t n, float value) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * BLOCK + threadIdx.x;
+    if (i < n) v[i] = value;
+}
+
+__device__ __forceinline__ unsigned long long wsum(unsigned long long v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+__device__ __forceinline__ unsigned long long wmax(unsigned long long v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const unsigned long long o = __shfl_xor(v, d, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// stats_field_kernel (src/sim/stats.cu:45-60): each value rounded to fixed
+// point on its own, then summed as integers.
+__global__ __launch_bounds__(BLOCK) void k_stats_fields(Grid T, Grid C, Grid N, int c_on, int n_on, size_t nt,
+                                                        size_t nc, size_t nn, unsigned long long* acc) {
+    const size_t most = nt > nc ? (nt > nn ? nt : nn) : (nc > nn ? nc : nn);
+    unsigned long long st = 0, mx = 0, sc = 0, sn = 0;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * BLOCK + threadIdx.x; i < most;
+         i += static_cast<size_t>(gridDim.x) * BLOCK) {
+        if (i < nt) {
+            const double t = static_cast<double>(T.value[i]);
+            const unsigned long long ft = static_cast<unsigned long long>(to_fixed(t, ASTRO_DEPOSIT_SCALE_STATS));
+            st += ft;
+            mx = ft > mx ? ft : mx;
+        }
+        if (c_on && i < nc) sc += static_cast<unsigned long long>(to_fixed(static_cast<double>(C.value[i]), ASTRO_STATS_SCALE_FIELD));
+        if (n_on && i < nn) sn += static_cast<unsigned long long>(to_fixed(static_cast<double>(N.value[i]), ASTRO_STATS_SCALE_FIELD));
+    }
+    st = wsum(st); sc = wsum(sc); sn = wsum(sn); mx = wmax(mx);
+    if ((threadIdx.x & 63) == 0) {
+        if (st) atomicAdd(&acc[SW_TEMP_MEDIUM], st);
+        if (sc) atomicAdd(&acc[SW_CO2], sc);
+        if (sn) atomicAdd(&acc[SW_N2], sn);
+        atomicMax(&acc[SW_MAX_TEMP], mx);
+    }
+}
+
+}  // namespace
+
+void launch_ftcs(const Launch& L, const Grid& g, const Ftcs& f) {
+    if (f.square2d) {
+        const int tiles = ((g.nx + 63) / 64) * ((g.ny + 3) / 4);
+        if (L.prof) L.prof->begin("ftcs2d");
+        k_ftcs2d<<<blocks_for_xcd(tiles), BLOCK, 0, L.stream>>>(g.value, g.scratch, g.nx, f.coeff, f.bc, f.amb, f.rx);
+        if (L.prof) L.prof->end();
+        return;
+    }
+    const int txy = ((g.nx + 63) / 64) * ((g.ny + 3) / 4);
+    int zc = g.nz;
+    while (zc > 4 && txy * ((g.nz + zc - 1) / zc) < 2048) zc = (zc + 1) / 2;
+    const int tiles = txy * ((g.nz + zc - 1) / zc);
+    if (L.prof) L.prof->begin("ftcs3d");
+    k_ftcs3d<<<blocks_for_xcd(tiles), BLOCK, 0, L.stream>>>(g.value, g.scratch, g.nx, g.ny, g.nz, zc, f.cx, f.cy,
+                                                             f.cz, f.bc, f.amb, f.rx, f.ry, f.rz);
+    if (L.prof) L.prof->end();
+}
+
+template <class P>
+void launch_stamp(const Launch& L, const WorldDev<P>& w) {
+    if (L.prof) L.prof->begin("occlusion_stamp");
+    k_stamp<P><<<(w.capacity + BLOCK - 1) / BLOCK, BLOCK, 0, L.stream>>>(w.c, w.ctr, w.g[ASTRO_FIELD_IRRADIANCE]);
+    if (L.prof) L.prof->end();
+}
+
+void launch_sweep(const Launch& L, const Grid& irr, const TickParams& t) {
+    const int lines = (t.sweep_axis == 0 ? irr.ny : irr.nx) * irr.nz;
+    if (L.prof) L.prof->begin("light_sweep");
+    k_sweep<<<(lines + BLOCK - 1) / BLOCK, BLOCK, 0, L.stream>>>(irr.deposit, irr.value, irr.nx, irr.ny, irr.nz,
+                                                                 t.sweep_axis, t.sweep_sign, irr.scale, t.face_area,
+                                                                 t.source, t.ambient_irr);
+    if (L.prof) L.prof->end();
+}
+
+void launch_fill(const Launch& L, float* v, size_t n, float value) {
+    if (n == 0) return;
+    k_fill<<<static_cast<unsigned>((n + BLOCK - 1) / BLOCK), BLOCK, 0, L.stream>>>(v, n, value);
+}
+
+void launch_stats_fields(const Launch& L, const Grid& T, const Grid& C, const Grid& N, bool c_on, bool n_on,
+                         unsigned long long* stats) {
+    const size_t nt = static_cast<size_t>(T.nx) * T.ny * T.nz;
+    const size_t nc = c_on ? static_cast<size_t>(C.nx) * C.ny * C.nz : 0;
+    const size_t nn = n_on ? static_cast<size_t>(N.nx) * N.ny * N.nz : 0;
+    k_stats_fields<<<1024, BLOCK, 0, L.stream>>>(T, C, N, c_on ? 1 : 0, n_on ? 1 : 0, nt, nc, nn, stats);
+}
+
+template void launch_stamp<double>(const Launch&, const WorldDev<double>&);
+template void launch_stamp<float>(const Launch&, const WorldDev<float>&);
+
+}  // namespace astro_dev

@@ -1,0 +1,1199 @@
+// world.hip -- the C ABI of include/astro_hot.h: world lifetime, host<->device
+// copies, the tick sequence of src/sim/step.cu:109-184, graph capture and
+// per-kernel timing.
+//
+// A tick never synchronizes: the population count, next id and per-tick
+// tallies live in the device Counters block, every grid is sized for the
+// capacity, and each kernel reads the live count itself. astro_world_step
+// therefore captures one tick (two, when a field's ping-pong parity would not
+// return after one) into a hipGraph and replays it.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "astro_hot.h"
+#include "kernels.h"
+
+using namespace astro_dev;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Field bookkeeping (src/fields/diffuse.cu:91-119), extended to nz planes.
+// ---------------------------------------------------------------------------
+struct FieldHost {
+    bool on = false, square2d = false;
+    int nx = 0, ny = 0, nz = 0;
+    double dx = 0, dy = 0, dz = 0, D = 0, ambient = 0, scale = 1;
+    int bc = ASTRO_BC_NEUMANN;
+    double rob_x = 0, rob_y = 0, rob_z = 0;
+    float* value = nullptr;
+    float* scratch = nullptr;
+    unsigned long long* deposit = nullptr;
+    size_t count() const { return on ? static_cast<size_t>(nx) * ny * nz : 0; }
+    Grid grid() const {
+        Grid g{};
+        g.value = value;
+        g.scratch = scratch;
+        g.deposit = deposit;
+        g.nx = nx; g.ny = ny; g.nz = nz;
+        g.fdx = static_cast<float>(dx);
+        g.fdy = static_cast<float>(dy);
+        g.fdz = static_cast<float>(dz);
+        g.scale = scale;
+        return g;
+    }
+};
+
+void field_init(FieldHost& g, astro_grid_dims d, double w, double h, double depth, double D, double ambient,
+                double scale, int bc) {
+    g.on = d.nx > 0;
+    if (!g.on) return;
+    g.nx = d.nx; g.ny = d.ny; g.nz = d.nz;
+    g.square2d = (g.nz == 1 && g.nx == g.ny);
+    g.dx = w / static_cast<double>(g.nx);
+    g.dy = g.square2d ? g.dx : h / static_cast<double>(g.ny);
+    g.dz = depth / static_cast<double>(g.nz);
+    g.D = D; g.ambient = ambient; g.scale = scale; g.bc = bc;
+    g.rob_x = g.dx * ASTRO_AIR_CONVECTION_H / ASTRO_WATER_CONDUCTIVITY;
+    g.rob_y = g.dy * ASTRO_AIR_CONVECTION_H / ASTRO_WATER_CONDUCTIVITY;
+    g.rob_z = g.dz * ASTRO_AIR_CONVECTION_H / ASTRO_WATER_CONDUCTIVITY;
+}
+
+// substeps_for_dt (src/fields/grid.cuh:51-53); 3D uses the 7-point bound.
+int substeps_for_dt(const FieldHost& g, double dt) {
+    if (g.square2d) return static_cast<int>(dt / (g.dx * g.dx / (4.0 * g.D))) + 1;
+    double inv = 1.0 / (g.dx * g.dx) + 1.0 / (g.dy * g.dy);
+    if (g.nz > 1) inv += 1.0 / (g.dz * g.dz);
+    return static_cast<int>(dt / (1.0 / (2.0 * g.D * inv))) + 1;
+}
+
+// Coefficients of one substep; false when unstable (diffuse.cu:146, :172).
+bool ftcs_coeffs(const FieldHost& g, double dt_sub, Ftcs& f) {
+    f = Ftcs{};
+    f.square2d = g.square2d;
+    f.bc = g.bc;
+    f.amb = static_cast<float>(g.ambient);
+    f.rx = static_cast<float>(g.rob_x);
+    f.ry = static_cast<float>(g.rob_y);
+    f.rz = static_cast<float>(g.rob_z);
+    if (g.square2d) {
+        f.coeff = static_cast<float>(g.D * dt_sub / (g.dx * g.dx));
+        return f.coeff <= 0.25f;
+    }
+    f.cx = static_cast<float>(g.D * dt_sub / (g.dx * g.dx));
+    f.cy = static_cast<float>(g.D * dt_sub / (g.dy * g.dy));
+    f.cz = g.nz > 1 ? static_cast<float>(g.D * dt_sub / (g.dz * g.dz)) : 0.0f;
+    return static_cast<double>(f.cx) + f.cy + f.cz <= 0.5;
+}
+
+// HIP-event timing of every launch on the world's stream.
+struct EventProfiler final : Profiler {
+    hipStream_t stream = nullptr;
+    struct Pending { std::string name; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    std::map<std::string, std::pair<int64_t, double>> totals;
+    std::vector<std::string> order;
+    Pending cur;
+
+    hipEvent_t take() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        hipEventCreate(&e);
+        return e;
+    }
+    void begin(const char* name) override {
+        cur.name = name;
+        cur.a = take();
+        cur.b = take();
+        hipEventRecord(cur.a, stream);
+    }
+    void end() override {
+        hipEventRecord(cur.b, stream);
+        pending.push_back(cur);
+    }
+    void drain() {
+        hipStreamSynchronize(stream);
+        for (Pending& p : pending) {
+            float ms = 0.0f;
+            hipEventElapsedTime(&ms, p.a, p.b);
+            auto it = totals.find(p.name);
+            if (it == totals.end()) {
+                order.push_back(p.name);
+                it = totals.emplace(p.name, std::make_pair<int64_t, double>(0, 0.0)).first;
+            }
+            it->second.first += 1;
+            it->second.second += ms;
+            pool.push_back(p.a);
+            pool.push_back(p.b);
+        }
+        pending.clear();
+    }
+    ~EventProfiler() override {
+        for (Pending& p : pending) { pool.push_back(p.a); pool.push_back(p.b); }
+        for (hipEvent_t e : pool) hipEventDestroy(e);
+    }
+};
+
+// One device allocation carved into 256 B-aligned arrays
+// (the reference's carve-once blob, src/sim/cell_store.cu:139-248).
+struct Carver {
+    char* base = nullptr;
+    size_t off = 0;
+    template <class T>
+    T* take(size_t n) {
+        const size_t bytes = ((sizeof(T) * n + 255) / 256) * 256;
+        T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+        off += bytes ? bytes : 256;
+        return p;
+    }
+};
+
+template <class P>
+void carve_cells(Carver& cv, Cells<P>& c, size_t cap) {
+    c.id = cv.take<uint64_t>(cap); c.flags = cv.take<uint32_t>(cap); c.death_cause = cv.take<uint8_t>(cap);
+    c.x = cv.take<P>(cap); c.y = cv.take<P>(cap); c.z = cv.take<P>(cap);
+    c.vx = cv.take<P>(cap); c.vy = cv.take<P>(cap); c.vz = cv.take<P>(cap);
+    c.energy = cv.take<double>(cap); c.temp_cell = cv.take<float>(cap); c.emit_power = cv.take<float>(cap);
+    c.dir_x = cv.take<float>(cap); c.dir_y = cv.take<float>(cap); c.dir_z = cv.take<float>(cap);
+    c.biomass = cv.take<double>(cap); c.co2_held = cv.take<double>(cap); c.age_s = cv.take<float>(cap);
+    c.rng_state = cv.take<uint64_t>(cap); c.taxis_memory = cv.take<float>(cap); c.run_timer = cv.take<float>(cap);
+    c.t_local = cv.take<float>(cap); c.irradiance = cv.take<float>(cap); c.co2_local = cv.take<float>(cap);
+    c.n2_local = cv.take<float>(cap);
+}
+
+}  // namespace
+
+struct astro_world {
+    astro_world_desc desc{};
+    hipStream_t stream = nullptr;
+    int device = 0;
+    int32_t capacity = 0;
+    bool fp32 = false;
+    void* pool = nullptr;
+    size_t pool_bytes = 0;
+
+    Cells<double> c64[2]{};
+    Cells<float> c32[2]{};
+    Counters* ctr = nullptr;
+    FieldHost f[ASTRO_FIELD_COUNT];
+
+    HashGeo hg{};
+    double hcell = 0.0;
+    uint32_t *keys = nullptr, *rin = nullptr, *bcount = nullptr, *vals_sorted = nullptr;
+    int32_t *bstart = nullptr, *spos = nullptr, *dflag = nullptr, *prefix = nullptr, *scan_total = nullptr;
+    void *trec = nullptr, *rec = nullptr;
+    Force* fs = nullptr;
+    double* dirs = nullptr;
+    unsigned long long* co2_demand = nullptr;
+    unsigned long long *scan_status = nullptr, *tile_ctr = nullptr, *stats = nullptr;
+
+    double physics_rate = 1.0, biology_rate = 1.0;
+    uint64_t tick = 0;
+    double sim_time = 0.0;
+    int64_t div_reported = 0, deaths_reported = 0;
+    bool hash_valid = false;
+
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool graph_ok = false;
+    int graph_ticks = 1;
+
+    std::unique_ptr<EventProfiler> prof;
+    bool prof_on = false;
+    std::string err;
+};
+
+namespace {
+
+int32_t fail(astro_world* w, int32_t st, const std::string& what) {
+    if (w) w->err = what;
+    return st;
+}
+
+int32_t hip_check(astro_world* w, hipError_t e, const char* what) {
+    if (e == hipSuccess) return ASTRO_OK;
+    return fail(w, e == hipErrorOutOfMemory ? ASTRO_OUT_OF_MEMORY : ASTRO_DEVICE_ERROR,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class P>
+WorldDev<P> view(astro_world* w);
+
+template <>
+WorldDev<double> view<double>(astro_world* w) {
+    WorldDev<double> d{};
+    d.c = w->c64[0];
+    d.alt = w->c64[1];
+    d.rec = static_cast<Rec<double>*>(w->rec);
+    d.trec = static_cast<Rec<double>*>(w->trec);
+    return d;
+}
+template <>
+WorldDev<float> view<float>(astro_world* w) {
+    WorldDev<float> d{};
+    d.c = w->c32[0];
+    d.alt = w->c32[1];
+    d.rec = static_cast<Rec<float>*>(w->rec);
+    d.trec = static_cast<Rec<float>*>(w->trec);
+    return d;
+}
+
+template <class P>
+WorldDev<P> make_view(astro_world* w) {
+    WorldDev<P> d = view<P>(w);
+    d.ctr = w->ctr;
+    d.capacity = w->capacity;
+    for (int i = 0; i < ASTRO_FIELD_COUNT; ++i) {
+        d.g[i] = w->f[i].grid();
+        d.on[i] = w->f[i].on;
+    }
+    d.hg = w->hg;
+    d.keys = w->keys; d.rin = w->rin; d.bcount = w->bcount; d.bstart = w->bstart;
+    d.vals_sorted = w->vals_sorted; d.spos = w->spos; d.fs = w->fs; d.dirs = w->dirs;
+    d.dflag = w->dflag; d.prefix = w->prefix; d.co2_demand = w->co2_demand; d.scan_total = w->scan_total;
+    d.sc = ScanCtl{w->scan_status, w->tile_ctr, w->ctr};
+    d.stats = w->stats;
+    return d;
+}
+
+Launch launch_of(astro_world* w) { return Launch{w->stream, w->prof_on ? w->prof.get() : nullptr}; }
+
+TickParams tick_params(const astro_world* w, double dt) {
+    const astro_world_desc& d = w->desc;
+    TickParams t{};
+    t.dt = dt;
+    t.dt_bio = dt * w->biology_rate;
+    t.chamber_w = d.chamber_w; t.chamber_h = d.chamber_h; t.chamber_d = d.chamber_d;
+    t.boundary_x = d.boundary_x; t.boundary_y = d.boundary_y; t.gravity_axis = d.gravity_axis;
+    t.thermal_noise = d.thermal_noise; t.contact = d.contact_enabled; t.adhesion = d.adhesion_enabled;
+    t.occlusion_exact = d.occlusion_exact;
+    t.store_disposition = d.store_disposition; t.compaction = d.compaction_enabled;
+    t.k_contact = ASTRO_CONTACT_STIFFNESS / w->physics_rate;
+    t.petrova_max_power = d.petrova_max_power;
+    t.quota = d.co2_mass_per_division;
+    t.division_offset = d.division_offset;
+    t.source = d.light_enabled ? d.light_irradiance : 0.0f;
+    t.ambient_irr = d.ambient_irradiance;
+    // normalize(Vec3{dir_x, dir_y, 0}) (src/sim/emission.cu:158)
+    const double lx = d.light_dir_x, ly = d.light_dir_y, lz = 0.0;
+    const double l = std::sqrt(lx * lx + ly * ly + lz * lz);
+    if (l > 0.0) {
+        const double inv = 1.0 / l;
+        t.ldx = lx * inv; t.ldy = ly * inv; t.ldz = lz * inv;
+    } else {
+        t.ldx = 0.0; t.ldy = 0.0; t.ldz = 1.0;
+    }
+    t.sweep_axis = (std::fabs(t.ldx) >= std::fabs(t.ldy)) ? 0 : 1;
+    t.sweep_sign = t.sweep_axis == 0 ? (t.ldx >= 0.0 ? 1 : -1) : (t.ldy >= 0.0 ? 1 : -1);
+    const FieldHost& I = w->f[ASTRO_FIELD_IRRADIANCE];
+    t.face_area = (t.sweep_axis == 0 ? I.dy : I.dx) * I.dz;
+    const FieldHost& T = w->f[ASTRO_FIELD_TEMPERATURE];
+    t.T_cap = ASTRO_WATER_DENSITY * ASTRO_WATER_SPECIFIC_HEAT * T.dx * T.dy * T.dz;
+    const FieldHost& C = w->f[ASTRO_FIELD_CO2];
+    t.co2_gv = C.dx * C.dy * C.dz;
+    return t;
+}
+
+bool lit(const astro_world* w) {
+    const float source = w->desc.light_enabled ? w->desc.light_irradiance : 0.0f;
+    return !(source <= 0.0f && w->desc.ambient_irradiance <= 0.0f);
+}
+
+// grid_diffuse / grid_diffuse_substep: enqueue substeps, swapping on the host.
+int32_t enqueue_diffuse(astro_world* w, const Launch& L, int field, double dt) {
+    FieldHost& g = w->f[field];
+    if (!g.on) return ASTRO_OK;
+    const int steps = substeps_for_dt(g, dt);
+    if (steps <= 0) return ASTRO_OK;
+    const double dt_sub = dt / steps;
+    Ftcs fc;
+    if (!ftcs_coeffs(g, dt_sub, fc)) return fail(w, ASTRO_INVALID_ARGUMENT, "diffusion coefficient unstable");
+    for (int s = 0; s < steps; ++s) {
+        launch_ftcs(L, g.grid(), fc);
+        std::swap(g.value, g.scratch);
+    }
+    return ASTRO_OK;
+}
+
+// thermal_step (src/sim/thermal.cu:75-103).
+template <class P>
+int32_t enqueue_thermal(astro_world* w, const Launch& L, double dt) {
+    FieldHost& T = w->f[ASTRO_FIELD_TEMPERATURE];
+    const int substeps = substeps_for_dt(T, dt);
+    const double dt_sub = dt / substeps;
+    const double cap = ASTRO_WATER_DENSITY * ASTRO_WATER_SPECIFIC_HEAT * T.dx * T.dy * T.dz;
+    Ftcs fc;
+    if (!ftcs_coeffs(T, dt_sub, fc)) return fail(w, ASTRO_INVALID_ARGUMENT, "diffusion coefficient unstable");
+    for (int s = 0; s < substeps; ++s) {
+        const WorldDev<P> d = make_view<P>(w);
+        launch_thermal_substep<P>(L, d, dt_sub, cap, w->hash_valid);
+        launch_apply_awake<P>(L, d, ASTRO_FIELD_TEMPERATURE);
+        launch_ftcs(L, T.grid(), fc);
+        std::swap(T.value, T.scratch);
+    }
+    return ASTRO_OK;
+}
+
+// emission_step, directional path (src/sim/emission.cu:145-176), minus the
+// absorption which rides in the feed/taxis kernel.
+template <class P>
+void enqueue_light(astro_world* w, const Launch& L, const TickParams& t) {
+    const WorldDev<P> d = make_view<P>(w);
+    if (t.source > 0.0f) launch_stamp<P>(L, d);
+    launch_sweep(L, w->f[ASTRO_FIELD_IRRADIANCE].grid(), t);
+    launch_feed_walk<P>(L, d, t);
+}
+
+// One tick of world_step (src/sim/step.cu:109-184). Host clock untouched.
+template <class P>
+int32_t enqueue_tick(astro_world* w, const Launch& L) {
+    const double dt = ASTRO_DT_PHYSICS * w->physics_rate;
+    const TickParams t = tick_params(w, dt);
+    const astro_world_desc& ds = w->desc;
+    launch_hash_build<P>(L, make_view<P>(w));
+    w->hash_valid = true;
+    int32_t st;
+    if (ds.thermal_enabled && (st = enqueue_thermal<P>(w, L, dt)) != ASTRO_OK) return st;
+    const bool feed = ds.emission_enabled && lit(w);
+    if (feed) enqueue_light<P>(w, L, t);
+    if (feed || ds.taxis_enabled) launch_feed_taxis<P>(L, make_view<P>(w), t, feed, ds.taxis_enabled != 0);
+    {
+        const WorldDev<P> d = make_view<P>(w);
+        launch_sample_base<P>(L, d, t);
+        launch_forces_walk<P>(L, d, t);
+        launch_integrate<P>(L, d, t);
+    }
+    // Stages 7-8 (step.cu:168-171). Both accumulators are empty here: CO2
+    // uptake is folded inside lifecycle and nothing in this path deposits N2,
+    // so the reference's dense flush is a no-op and is not issued.
+    if ((st = enqueue_diffuse(w, L, ASTRO_FIELD_CO2, dt)) != ASTRO_OK) return st;
+    if ((st = enqueue_diffuse(w, L, ASTRO_FIELD_N2, dt)) != ASTRO_OK) return st;
+    if (ds.lifecycle_enabled) launch_lifecycle<P>(L, make_view<P>(w), t);
+    w->hash_valid = false;
+    return ASTRO_OK;
+}
+
+int32_t tick_once(astro_world* w, const Launch& L) {
+    return w->fp32 ? enqueue_tick<float>(w, L) : enqueue_tick<double>(w, L);
+}
+
+void advance_clock(astro_world* w, int32_t ticks) {
+    const double dt = ASTRO_DT_PHYSICS * w->physics_rate;
+    for (int32_t i = 0; i < ticks; ++i) {
+        ++w->tick;
+        w->sim_time += dt;
+    }
+}
+
+// Ticks per captured graph: two when some field's ping-pong ends swapped.
+int ticks_per_graph(const astro_world* w) {
+    const double dt = ASTRO_DT_PHYSICS * w->physics_rate;
+    int swaps[3] = {0, 0, 0};
+    if (w->desc.thermal_enabled) swaps[0] = substeps_for_dt(w->f[ASTRO_FIELD_TEMPERATURE], dt);
+    if (w->f[ASTRO_FIELD_CO2].on) swaps[1] = substeps_for_dt(w->f[ASTRO_FIELD_CO2], dt);
+    if (w->f[ASTRO_FIELD_N2].on) swaps[2] = substeps_for_dt(w->f[ASTRO_FIELD_N2], dt);
+    for (int s : swaps)
+        if (s & 1) return 2;
+    return 1;
+}
+
+void drop_graph(astro_world* w) {
+    if (w->gexec) hipGraphExecDestroy(w->gexec);
+    if (w->graph) hipGraphDestroy(w->graph);
+    w->gexec = nullptr;
+    w->graph = nullptr;
+    w->graph_ok = false;
+}
+
+int32_t check_device_errors(astro_world* w) {
+    Counters c{};
+    int32_t st = hip_check(w, hipMemcpy(&c, w->ctr, sizeof(c), hipMemcpyDeviceToHost), "read counters");
+    if (st != ASTRO_OK) return st;
+    if (c.error_flags & ERR_SCAN_SPIN) return fail(w, ASTRO_DEVICE_ERROR, "prefix scan look-back did not complete");
+    return ASTRO_OK;
+}
+
+int32_t read_counters(astro_world* w, Counters& c) {
+    int32_t st = hip_check(w, hipStreamSynchronize(w->stream), "synchronize");
+    if (st != ASTRO_OK) return st;
+    st = hip_check(w, hipMemcpy(&c, w->ctr, sizeof(c), hipMemcpyDeviceToHost), "read counters");
+    if (st != ASTRO_OK) return st;
+    if (c.error_flags & ERR_SCAN_SPIN) return fail(w, ASTRO_DEVICE_ERROR, "prefix scan look-back did not complete");
+    return ASTRO_OK;
+}
+
+int32_t write_counters(astro_world* w, const Counters& c) {
+    int32_t st = hip_check(w, hipMemcpyAsync(w->ctr, &c, sizeof(c), hipMemcpyHostToDevice, w->stream), "write counters");
+    if (st != ASTRO_OK) return st;
+    return hip_check(w, hipStreamSynchronize(w->stream), "synchronize");
+}
+
+// Host <-> device copies of one array, converting the position width.
+template <class D, class H>
+int32_t put(astro_world* w, D* dst, const H* src, int32_t n) {
+    if (!src || n <= 0) return ASTRO_OK;
+    if constexpr (std::is_same<D, H>::value) {
+        return hip_check(w, hipMemcpy(dst, src, sizeof(D) * n, hipMemcpyHostToDevice), "upload");
+    } else {
+        std::vector<D> tmp(n);
+        for (int32_t i = 0; i < n; ++i) tmp[i] = static_cast<D>(src[i]);
+        return hip_check(w, hipMemcpy(dst, tmp.data(), sizeof(D) * n, hipMemcpyHostToDevice), "upload");
+    }
+}
+template <class D, class H>
+int32_t get(astro_world* w, H* dst, const D* src, int32_t n) {
+    if (!dst || n <= 0) return ASTRO_OK;
+    if constexpr (std::is_same<D, H>::value) {
+        return hip_check(w, hipMemcpy(dst, src, sizeof(D) * n, hipMemcpyDeviceToHost), "download");
+    } else {
+        std::vector<D> tmp(n);
+        int32_t st = hip_check(w, hipMemcpy(tmp.data(), src, sizeof(D) * n, hipMemcpyDeviceToHost), "download");
+        if (st != ASTRO_OK) return st;
+        for (int32_t i = 0; i < n; ++i) dst[i] = static_cast<H>(tmp[i]);
+        return ASTRO_OK;
+    }
+}
+
+template <class P>
+int32_t upload_cells(astro_world* w, const Cells<P>& c, const astro_cell_soa* s, int32_t n) {
+    int32_t st = ASTRO_OK;
+#define PUT(a) \
+    if ((st = put(w, c.a, s->a, n)) != ASTRO_OK) return st;
+    PUT(id) PUT(flags) PUT(death_cause) PUT(x) PUT(y) PUT(z) PUT(vx) PUT(vy) PUT(vz) PUT(energy) PUT(temp_cell)
+    PUT(emit_power) PUT(dir_x) PUT(dir_y) PUT(dir_z) PUT(biomass) PUT(co2_held) PUT(age_s) PUT(rng_state)
+    PUT(taxis_memory) PUT(run_timer) PUT(t_local) PUT(irradiance) PUT(co2_local) PUT(n2_local)
+#undef PUT
+    return st;
+}
+
+template <class P>
+int32_t download_cells(astro_world* w, const Cells<P>& c, astro_cell_soa* s, int32_t n) {
+    int32_t st = ASTRO_OK;
+#define GET(a) \
+    if ((st = get(w, s->a, c.a, n)) != ASTRO_OK) return st;
+    GET(id) GET(flags) GET(death_cause) GET(x) GET(y) GET(z) GET(vx) GET(vy) GET(vz) GET(energy) GET(temp_cell)
+    GET(emit_power) GET(dir_x) GET(dir_y) GET(dir_z) GET(biomass) GET(co2_held) GET(age_s) GET(rng_state)
+    GET(taxis_memory) GET(run_timer) GET(t_local) GET(irradiance) GET(co2_local) GET(n2_local)
+#undef GET
+    return st;
+}
+
+uint64_t fnv1a64(const void* data, size_t n, uint64_t h) {
+    const unsigned char* p = static_cast<const unsigned char*>(data);
+    for (size_t i = 0; i < n; ++i) {
+        h ^= p[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+int32_t ensure_hash(astro_world* w) {
+    if (w->hash_valid) return ASTRO_OK;
+    const Launch L = launch_of(w);
+    if (w->fp32) launch_hash_build<float>(L, make_view<float>(w));
+    else launch_hash_build<double>(L, make_view<double>(w));
+    w->hash_valid = true;
+    return hip_check(w, hipGetLastError(), "hash_build");
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+void astro_world_desc_default(astro_world_desc* d) {
+    std::memset(d, 0, sizeof(*d));
+    d->abi_version = ASTRO_ABI_VERSION;
+    d->seed = 20260802ull;
+    d->chamber_w = ASTRO_CHAMBER_W;
+    d->chamber_h = ASTRO_CHAMBER_H;
+    d->chamber_d = ASTRO_CHAMBER_D;
+    d->precision = ASTRO_FP64_CONTRACT;
+    d->boundary_x = d->boundary_y = ASTRO_BOUNDARY_REFLECTING;
+    d->gravity_axis = ASTRO_GRAVITY_Y;
+    d->thermal_noise = d->contact_enabled = d->adhesion_enabled = 1;
+    d->thermal_enabled = d->emission_enabled = d->taxis_enabled = d->occlusion_exact = 1;
+    d->store_disposition = ASTRO_STORE_VOID;
+    d->lifecycle_enabled = 1;
+    d->ambient_temp = ASTRO_AMBIENT_TEMP_DEFAULT;
+    d->physics_rate = 1.0;
+    d->biology_rate = 1.0;
+    d->petrova_max_power = ASTRO_PETROVA_MAX_POWER;
+    d->co2_mass_per_division = ASTRO_CO2_MASS_PER_DIVISION;
+    d->light_dir_x = 1.0f;
+}
+
+const char* astro_status_str(int32_t s) {
+    static const char* names[] = {"Ok", "OutOfMemory", "DeviceError", "InvalidArgument", "NotFound",
+                                  "ParseError", "ContractVersionMismatch", "CapacityExceeded",
+                                  "FileIoError", "Unsupported"};
+    return (s >= 0 && s <= 9) ? names[s] : "Unknown";
+}
+
+const char* astro_last_error(const astro_world* w) { return w ? w->err.c_str() : ""; }
+
+int32_t astro_world_desc_get(const astro_world* w, astro_world_desc* out) {
+    if (!w || !out) return ASTRO_INVALID_ARGUMENT;
+    *out = w->desc;
+    return ASTRO_OK;
+}
+
+void astro_world_destroy(astro_world* w) {
+    if (!w) return;
+    if (w->stream) hipStreamSynchronize(w->stream);
+    drop_graph(w);
+    w->prof.reset();
+    if (w->pool) hipFree(w->pool);
+    if (w->stream) hipStreamDestroy(w->stream);
+    delete w;
+}
+
+int32_t astro_world_create(const astro_world_desc* din, astro_world** out) {
+    if (!din || !out) return ASTRO_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (din->abi_version != ASTRO_ABI_VERSION) return ASTRO_CONTRACT_VERSION_MISMATCH;
+    astro_world_desc d = *din;
+    const int32_t cap = d.capacity > 0 ? d.capacity : ASTRO_DEFAULT_CELLS;
+    if (cap <= 0 || cap > ASTRO_MAX_CELLS) return ASTRO_INVALID_ARGUMENT;
+    if (!(d.chamber_w > 0 && d.chamber_h > 0 && d.chamber_d > 0)) return ASTRO_INVALID_ARGUMENT;
+    if (d.precision != ASTRO_FP64_CONTRACT && d.precision != ASTRO_FP32_HOT) return ASTRO_INVALID_ARGUMENT;
+    d.capacity = cap;
+    if (d.division_offset <= 0.0) d.division_offset = ASTRO_CELL_RADIUS;
+    const int defaults[4] = {ASTRO_FIELD_N_TEMP, ASTRO_FIELD_N_CO2, ASTRO_FIELD_N_N2, ASTRO_FIELD_N_IRRAD};
+    for (int f = 0; f < 4; ++f) {
+        astro_grid_dims& g = d.grid[f];
+        if (g.nx == 0 && g.ny == 0 && g.nz == 0) { g.nx = g.ny = defaults[f]; g.nz = 1; }
+        const bool optional = (f == ASTRO_FIELD_N2);
+        if (!(optional && g.nx == 0) && (g.nx <= 1 || g.ny <= 1 || g.nz < 1)) return ASTRO_INVALID_ARGUMENT;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return ASTRO_DEVICE_ERROR;
+
+    auto* w = new astro_world();
+    w->desc = d;
+    w->capacity = cap;
+    w->fp32 = d.precision == ASTRO_FP32_HOT;
+    w->physics_rate = d.physics_rate > 0 ? d.physics_rate : 1.0;
+    w->biology_rate = d.biology_rate > 0 ? d.biology_rate : 1.0;
+    hipGetDevice(&w->device);
+    if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete w;
+        return ASTRO_DEVICE_ERROR;
+    }
+    const double W = d.chamber_w, H = d.chamber_h, Dd = d.chamber_d;
+    field_init(w->f[ASTRO_FIELD_TEMPERATURE], d.grid[0], W, H, Dd, ASTRO_WATER_THERMAL_DIFFUSIVITY, d.ambient_temp,
+               ASTRO_DEPOSIT_SCALE_TEMPERATURE, ASTRO_BC_ROBIN);
+    field_init(w->f[ASTRO_FIELD_CO2], d.grid[1], W, H, Dd, ASTRO_CO2_DIFFUSIVITY_WATER, d.co2_init,
+               ASTRO_DEPOSIT_SCALE_CO2, ASTRO_BC_NEUMANN);
+    field_init(w->f[ASTRO_FIELD_N2], d.grid[2], W, H, Dd, ASTRO_N2_DIFFUSIVITY, 0.0, ASTRO_DEPOSIT_SCALE_N2,
+               ASTRO_BC_NEUMANN);
+    field_init(w->f[ASTRO_FIELD_IRRADIANCE], d.grid[3], W, H, Dd, ASTRO_IRRADIANCE_NOMINAL_DIFFUSIVITY, 0.0,
+               ASTRO_DEPOSIT_SCALE_IRRADIANCE, ASTRO_BC_NEUMANN);
+    // hash_create (src/sim/hash.cu:46-80), hash_view (:90-104)
+    w->hcell = d.hash_cell_size > 0.0 ? d.hash_cell_size : ASTRO_HASH_CELL_FACTOR * ASTRO_CELL_DIAMETER;
+    w->hg.nx = std::max(1, static_cast<int32_t>(std::ceil(W / w->hcell)));
+    w->hg.ny = std::max(1, static_cast<int32_t>(std::ceil(H / w->hcell)));
+    w->hg.nz = std::max(1, static_cast<int32_t>(std::ceil(Dd / w->hcell)));
+    const int64_t nb64 = static_cast<int64_t>(w->hg.nx) * w->hg.ny * w->hg.nz;
+    if (nb64 >= (1ll << 30)) {
+        delete w;
+        return ASTRO_INVALID_ARGUMENT;
+    }
+    w->hg.nb = static_cast<int32_t>(nb64);
+    w->hg.inv = static_cast<float>(1.0 / w->hcell);
+    w->hg.ox = static_cast<float>(-0.5 * w->hg.nx * w->hcell);
+    w->hg.oy = static_cast<float>(-0.5 * w->hg.ny * w->hcell);
+    w->hg.oz = static_cast<float>(-0.5 * w->hg.nz * w->hcell);
+
+    // Everything in one allocation, carved twice (measure, then assign).
+    const size_t n = static_cast<size_t>(cap);
+    const size_t nb = static_cast<size_t>(w->hg.nb);
+    const size_t rec_bytes = w->fp32 ? sizeof(Rec<float>) : sizeof(Rec<double>);
+    const size_t max_tiles = static_cast<size_t>(std::max(scan_tiles_for(cap), scan_tiles_for(static_cast<int>(nb + 1)))) + 1;
+    auto layout = [&](Carver& cv) {
+        for (int s = 0; s < 2; ++s) {
+            if (w->fp32) carve_cells(cv, w->c32[s], n);
+            else carve_cells(cv, w->c64[s], n);
+        }
+        w->ctr = cv.take<Counters>(1);
+        for (int fi = 0; fi < ASTRO_FIELD_COUNT; ++fi) {
+            FieldHost& g = w->f[fi];
+            const size_t cnt = g.count();
+            g.value = cnt ? cv.take<float>(cnt) : nullptr;
+            g.scratch = cnt ? cv.take<float>(cnt) : nullptr;
+            g.deposit = cnt ? cv.take<unsigned long long>(cnt) : nullptr;
+        }
+        w->keys = cv.take<uint32_t>(n);
+        w->rin = cv.take<uint32_t>(n);
+        w->bcount = cv.take<uint32_t>(nb + 1);
+        w->bstart = cv.take<int32_t>(nb + 2);
+        w->trec = cv.take<char>(n * rec_bytes);
+        w->rec = cv.take<char>(n * rec_bytes);
+        w->vals_sorted = cv.take<uint32_t>(n);
+        w->spos = cv.take<int32_t>(n);
+        w->fs = cv.take<Force>(n);
+        w->dirs = cv.take<double>(n);
+        w->dflag = cv.take<int32_t>(n);
+        w->prefix = cv.take<int32_t>(n);
+        w->scan_total = cv.take<int32_t>(1);
+        w->co2_demand = cv.take<unsigned long long>(std::max<size_t>(1, w->f[ASTRO_FIELD_CO2].count()));
+        w->scan_status = cv.take<unsigned long long>(max_tiles);
+        w->tile_ctr = cv.take<unsigned long long>(1);
+        w->stats = cv.take<unsigned long long>(SW_COUNT);
+    };
+    Carver measure;
+    layout(measure);
+    w->pool_bytes = measure.off;
+    hipError_t e = hipMalloc(&w->pool, w->pool_bytes);
+    if (e != hipSuccess) {
+        hipStreamDestroy(w->stream);
+        delete w;
+        return e == hipErrorOutOfMemory ? ASTRO_OUT_OF_MEMORY : ASTRO_DEVICE_ERROR;
+    }
+    Carver assign;
+    assign.base = static_cast<char*>(w->pool);
+    layout(assign);
+    if (hipMemsetAsync(w->pool, 0, w->pool_bytes, w->stream) != hipSuccess) {
+        astro_world_destroy(w);
+        return ASTRO_DEVICE_ERROR;
+    }
+    Counters c0{};
+    c0.count = 0;
+    c0.capacity = cap;
+    c0.next_id = 1;
+    c0.scan_epoch = 1;
+    hipMemcpyAsync(w->ctr, &c0, sizeof(c0), hipMemcpyHostToDevice, w->stream);
+    const Launch L{w->stream, nullptr};
+    for (int fi = 0; fi < ASTRO_FIELD_COUNT; ++fi) {
+        FieldHost& g = w->f[fi];
+        launch_fill(L, g.value, g.count(), static_cast<float>(g.ambient));
+        launch_fill(L, g.scratch, g.count(), static_cast<float>(g.ambient));
+    }
+    w->prof = std::make_unique<EventProfiler>();
+    w->prof->stream = w->stream;
+    if (hipStreamSynchronize(w->stream) != hipSuccess || hipGetLastError() != hipSuccess) {
+        astro_world_destroy(w);
+        return ASTRO_DEVICE_ERROR;
+    }
+    *out = w;
+    return ASTRO_OK;
+}
+
+int32_t astro_cells_spawn(astro_world* w, const astro_spawn_params* p, uint64_t seed) {
+    if (!w || !p) return ASTRO_INVALID_ARGUMENT;
+    if (p->count <= 0) return ASTRO_OK;
+    if (p->charge_dist == ASTRO_DIST_LOG_UNIFORM && !(p->charge_a > 0.0 && p->charge_b > 0.0))
+        return fail(w, ASTRO_INVALID_ARGUMENT, "log-uniform charge needs positive bounds");
+    Counters c{};
+    int32_t st = read_counters(w, c);
+    if (st != ASTRO_OK) return st;
+    if (c.count + p->count > w->capacity) return fail(w, ASTRO_CAPACITY_EXCEEDED, "spawn exceeds cell store capacity");
+    SpawnArgs a{};
+    a.count = p->count; a.placement = p->placement; a.charge_dist = p->charge_dist; a.awake = p->awake;
+    a.place_x = p->place_x; a.place_y = p->place_y; a.place_radius = p->place_radius;
+    a.charge_a = p->charge_a; a.charge_b = p->charge_b;
+    a.chamber_w = w->desc.chamber_w; a.chamber_h = w->desc.chamber_h; a.chamber_d = w->desc.chamber_d;
+    const Launch L = launch_of(w);
+    if (w->fp32) launch_spawn<float>(L, make_view<float>(w), a, c.count, c.next_id, seed);
+    else launch_spawn<double>(L, make_view<double>(w), a, c.count, c.next_id, seed);
+    if ((st = hip_check(w, hipGetLastError(), "spawn_kernel")) != ASTRO_OK) return st;
+    c.count += p->count;
+    c.next_id += static_cast<uint64_t>(p->count);
+    w->hash_valid = false;
+    return write_counters(w, c);
+}
+
+int32_t astro_cells_upload(astro_world* w, const astro_cell_soa* s, int32_t n) {
+    if (!w || !s || n < 0) return ASTRO_INVALID_ARGUMENT;
+    if (n > w->capacity) return fail(w, ASTRO_CAPACITY_EXCEEDED, "upload exceeds capacity");
+    Counters c{};
+    int32_t st = read_counters(w, c);
+    if (st != ASTRO_OK) return st;
+    st = w->fp32 ? upload_cells(w, w->c32[0], s, n) : upload_cells(w, w->c64[0], s, n);
+    if (st != ASTRO_OK) return st;
+    c.count = n;
+    w->hash_valid = false;
+    return write_counters(w, c);
+}
+
+int32_t astro_cells_download(astro_world* w, astro_cell_soa* s, int32_t n, int32_t* n_out) {
+    if (!w || !s) return ASTRO_INVALID_ARGUMENT;
+    Counters c{};
+    int32_t st = read_counters(w, c);
+    if (st != ASTRO_OK) return st;
+    const int32_t m = std::min(n, c.count);
+    st = w->fp32 ? download_cells(w, w->c32[0], s, m) : download_cells(w, w->c64[0], s, m);
+    if (st != ASTRO_OK) return st;
+    if (n_out) *n_out = m;
+    return ASTRO_OK;
+}
+
+int32_t astro_world_get_counters(astro_world* w, astro_counters* o) {
+    if (!w || !o) return ASTRO_INVALID_ARGUMENT;
+    Counters c{};
+    int32_t st = read_counters(w, c);
+    if (st != ASTRO_OK) return st;
+    std::memset(o, 0, sizeof(*o));
+    o->count = c.count;
+    o->capacity = w->capacity;
+    o->next_id = c.next_id;
+    o->tick = w->tick;
+    o->sim_time_s = w->sim_time;
+    o->divisions_total = c.divisions_total;
+    o->deaths_total = c.deaths_total;
+    o->dead_slots_prev = c.dead_slots_prev;
+    return ASTRO_OK;
+}
+
+int32_t astro_world_set_counters(astro_world* w, const astro_counters* in) {
+    if (!w || !in) return ASTRO_INVALID_ARGUMENT;
+    if (in->count < 0 || in->count > w->capacity) return ASTRO_CAPACITY_EXCEEDED;
+    Counters c{};
+    int32_t st = read_counters(w, c);
+    if (st != ASTRO_OK) return st;
+    c.count = in->count;
+    c.next_id = in->next_id;
+    c.divisions_total = in->divisions_total;
+    c.deaths_total = in->deaths_total;
+    c.dead_slots_prev = in->dead_slots_prev;
+    w->tick = in->tick;
+    w->sim_time = in->sim_time_s;
+    w->hash_valid = false;
+    return write_counters(w, c);
+}
+
+int32_t astro_field_dims(const astro_world* w, int32_t f, astro_grid_dims* o, double* dx, double* dy, double* dz) {
+    if (!w || f < 0 || f >= ASTRO_FIELD_COUNT) return ASTRO_INVALID_ARGUMENT;
+    const FieldHost& g = w->f[f];
+    if (o) { o->nx = g.nx; o->ny = g.ny; o->nz = g.nz; }
+    if (dx) *dx = g.dx;
+    if (dy) *dy = g.dy;
+    if (dz) *dz = g.dz;
+    return ASTRO_OK;
+}
+
+int32_t astro_field_upload(astro_world* w, int32_t f, const float* v, size_t count) {
+    if (!w || f < 0 || f >= ASTRO_FIELD_COUNT || !v) return ASTRO_INVALID_ARGUMENT;
+    FieldHost& g = w->f[f];
+    if (!g.on || count != g.count()) return fail(w, ASTRO_INVALID_ARGUMENT, "field size mismatch");
+    int32_t st = hip_check(w, hipStreamSynchronize(w->stream), "synchronize");
+    if (st != ASTRO_OK) return st;
+    st = hip_check(w, hipMemcpy(g.value, v, sizeof(float) * count, hipMemcpyHostToDevice), "field upload");
+    if (st != ASTRO_OK) return st;
+    return hip_check(w, hipMemcpy(g.scratch, v, sizeof(float) * count, hipMemcpyHostToDevice), "field upload");
+}
+
+int32_t astro_field_download(astro_world* w, int32_t f, float* v, size_t count) {
+    if (!w || f < 0 || f >= ASTRO_FIELD_COUNT || !v) return ASTRO_INVALID_ARGUMENT;
+    FieldHost& g = w->f[f];
+    if (!g.on || count != g.count()) return fail(w, ASTRO_INVALID_ARGUMENT, "field size mismatch");
+    int32_t st = hip_check(w, hipStreamSynchronize(w->stream), "synchronize");
+    if (st != ASTRO_OK) return st;
+    return hip_check(w, hipMemcpy(v, g.value, sizeof(float) * count, hipMemcpyDeviceToHost), "field download");
+}
+
+int32_t astro_field_deposit_download(astro_world* w, int32_t f, int64_t* v, size_t count) {
+    if (!w || f < 0 || f >= ASTRO_FIELD_COUNT || !v) return ASTRO_INVALID_ARGUMENT;
+    FieldHost& g = w->f[f];
+    if (!g.on || count != g.count()) return fail(w, ASTRO_INVALID_ARGUMENT, "field size mismatch");
+    int32_t st = hip_check(w, hipStreamSynchronize(w->stream), "synchronize");
+    if (st != ASTRO_OK) return st;
+    return hip_check(w, hipMemcpy(v, g.deposit, sizeof(int64_t) * count, hipMemcpyDeviceToHost), "deposit download");
+}
+
+int32_t astro_world_step(astro_world* w, int32_t n_ticks) {
+    if (!w || n_ticks < 0) return ASTRO_INVALID_ARGUMENT;
+    if (n_ticks == 0) return ASTRO_OK;
+    int32_t st;
+    int32_t done = 0;
+    if (!w->prof_on) {
+        const int k = ticks_per_graph(w);
+        if (n_ticks >= k) {
+            if (!w->graph_ok) {
+                drop_graph(w);
+                st = hip_check(w, hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+                if (st != ASTRO_OK) return st;
+                const bool hv = w->hash_valid;
+                int32_t cst = ASTRO_OK;
+                for (int i = 0; i < k && cst == ASTRO_OK; ++i) cst = tick_once(w, Launch{w->stream, nullptr});
+                hipGraph_t g = nullptr;
+                const hipError_t ee = hipStreamEndCapture(w->stream, &g);
+                w->hash_valid = hv;
+                if (cst != ASTRO_OK) {
+                    if (g) hipGraphDestroy(g);
+                    return cst;
+                }
+                if ((st = hip_check(w, ee, "end capture")) != ASTRO_OK) return st;
+                w->graph = g;
+                if ((st = hip_check(w, hipGraphInstantiate(&w->gexec, g, nullptr, nullptr, 0), "instantiate")) != ASTRO_OK) {
+                    drop_graph(w);
+                    return st;
+                }
+                w->graph_ok = true;
+                w->graph_ticks = k;
+            }
+            while (n_ticks - done >= w->graph_ticks) {
+                if ((st = hip_check(w, hipGraphLaunch(w->gexec, w->stream), "graph launch")) != ASTRO_OK) return st;
+                advance_clock(w, w->graph_ticks);
+                done += w->graph_ticks;
+            }
+            w->hash_valid = false;
+        }
+    }
+    const Launch L = launch_of(w);
+    while (done < n_ticks) {
+        if ((st = tick_once(w, L)) != ASTRO_OK) return st;
+        if ((st = hip_check(w, hipGetLastError(), "tick launch")) != ASTRO_OK) return st;
+        advance_clock(w, 1);
+        ++done;
+        // An eager tick leaves a field's ping-pong swapped when its substep
+        // count is odd; the captured graph baked the other parity.
+        if (w->graph_ok && ticks_per_graph(w) == 2) drop_graph(w);
+    }
+    return ASTRO_OK;
+}
+
+int32_t astro_world_sync(astro_world* w) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    int32_t st = hip_check(w, hipStreamSynchronize(w->stream), "synchronize");
+    if (st != ASTRO_OK) return st;
+    return check_device_errors(w);
+}
+
+int32_t astro_world_set_clock(astro_world* w, double physics, double biology) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    w->physics_rate = physics < ASTRO_CLOCK_PHYSICS_MIN ? ASTRO_CLOCK_PHYSICS_MIN
+                    : (physics > ASTRO_CLOCK_PHYSICS_MAX ? ASTRO_CLOCK_PHYSICS_MAX : physics);
+    w->biology_rate = biology < ASTRO_CLOCK_BIOLOGY_MIN ? ASTRO_CLOCK_BIOLOGY_MIN
+                    : (biology > ASTRO_CLOCK_BIOLOGY_MAX ? ASTRO_CLOCK_BIOLOGY_MAX : biology);
+    w->desc.physics_rate = w->physics_rate;
+    w->desc.biology_rate = w->biology_rate;
+    drop_graph(w);
+    return ASTRO_OK;
+}
+
+// stats_step + world_stats (src/sim/stats.cu:79-153).
+int32_t astro_world_stats(astro_world* w, astro_stats* s) {
+    if (!w || !s) return ASTRO_INVALID_ARGUMENT;
+    int32_t st = hip_check(w, hipMemsetAsync(w->stats, 0, sizeof(unsigned long long) * SW_COUNT, w->stream), "stats clear");
+    if (st != ASTRO_OK) return st;
+    const Launch L{w->stream, nullptr};
+    if (w->fp32) launch_stats_cells<float>(L, make_view<float>(w));
+    else launch_stats_cells<double>(L, make_view<double>(w));
+    launch_stats_fields(L, w->f[0].grid(), w->f[1].grid(), w->f[2].grid(), w->f[1].on, w->f[2].on, w->stats);
+    if ((st = hip_check(w, hipGetLastError(), "stats kernels")) != ASTRO_OK) return st;
+    unsigned long long a[SW_COUNT];
+    Counters c{};
+    if ((st = read_counters(w, c)) != ASTRO_OK) return st;
+    if ((st = hip_check(w, hipMemcpy(a, w->stats, sizeof(a), hipMemcpyDeviceToHost), "stats read")) != ASTRO_OK) return st;
+    auto from_fixed = [](unsigned long long raw, double scale) { return static_cast<double>(static_cast<int64_t>(raw)) / scale; };
+    std::memset(s, 0, sizeof(*s));
+    s->tick = w->tick;
+    s->sim_time_s = w->sim_time;
+    s->physics_rate = w->physics_rate;
+    s->biology_rate = w->biology_rate;
+    const uint32_t na = static_cast<uint32_t>(a[SW_N_ALIVE]), nd = static_cast<uint32_t>(a[SW_N_DEAD]);
+    s->n_live = static_cast<int32_t>(na);
+    s->n_dead = static_cast<int32_t>(nd);
+    s->n_awake = static_cast<int32_t>(a[SW_N_AWAKE]);
+    s->n_dormant = static_cast<int32_t>(a[SW_N_DORMANT]);
+    s->total_energy_j = from_fixed(a[SW_ENERGY], ASTRO_DEPOSIT_SCALE_STATS);
+    s->mean_charge = (na + nd) > 0u ? s->total_energy_j / ((na + nd) * ASTRO_CELL_ENERGY_MAX) : 0.0;
+    s->mean_temp_cell_k = na > 0u ? from_fixed(a[SW_TEMP_CELL], ASTRO_DEPOSIT_SCALE_STATS) / na : 0.0;
+    const FieldHost& T = w->f[ASTRO_FIELD_TEMPERATURE];
+    const double cells = static_cast<double>(T.count());
+    s->mean_temp_medium_k = cells > 0.0 ? from_fixed(a[SW_TEMP_MEDIUM], ASTRO_DEPOSIT_SCALE_STATS) / cells : 0.0;
+    s->max_temp_medium_k = from_fixed(a[SW_MAX_TEMP], ASTRO_DEPOSIT_SCALE_STATS);
+    const FieldHost& C = w->f[ASTRO_FIELD_CO2];
+    const FieldHost& N = w->f[ASTRO_FIELD_N2];
+    s->co2_total_kg = from_fixed(a[SW_CO2], ASTRO_STATS_SCALE_FIELD) * (C.dx * C.dy * C.dz);
+    s->n2_total_kg = from_fixed(a[SW_N2], ASTRO_STATS_SCALE_FIELD) * (N.on ? N.dx * N.dy * N.dz : 0.0);
+    s->divisions_this_window = static_cast<int32_t>(c.divisions_total - w->div_reported);
+    s->deaths_this_window = static_cast<int32_t>(c.deaths_total - w->deaths_reported);
+    w->div_reported = c.divisions_total;
+    w->deaths_reported = c.deaths_total;
+    s->boil_events = s->max_temp_medium_k >= 373.14999999999998 ? 1 : 0;
+    s->non_canon_run = (w->desc.petrova_max_power != ASTRO_PETROVA_MAX_POWER ||
+                        w->desc.co2_mass_per_division != ASTRO_CO2_MASS_PER_DIVISION) ? 1 : 0;
+    return ASTRO_OK;
+}
+
+// snapshot_state_hash (src/sim/snapshot.cu:100-129).
+int32_t astro_world_state_hash(astro_world* w, uint64_t* out) {
+    if (!w || !out) return ASTRO_INVALID_ARGUMENT;
+    Counters c{};
+    int32_t st = read_counters(w, c);
+    if (st != ASTRO_OK) return st;
+    const int32_t n = c.count;
+    uint64_t h = 1469598103934665603ull;
+    struct Ov { char key[48]; double value; };
+    auto ov = [&](const char* key, double v, double canon) {
+        if (v == canon) return;
+        Ov o;
+        std::memset(&o, 0, sizeof(o));
+        std::strncpy(o.key, key, sizeof(o.key) - 1);
+        o.value = v;
+        h = fnv1a64(&o, sizeof(o), h);
+    };
+    ov("PETROVA_MAX_POWER", w->desc.petrova_max_power, ASTRO_PETROVA_MAX_POWER);
+    ov("CO2_MASS_PER_DIVISION", w->desc.co2_mass_per_division, ASTRO_CO2_MASS_PER_DIVISION);
+    if (n > 0) {
+        std::vector<uint64_t> id(n), rng(n);
+        std::vector<uint32_t> flags(n);
+        std::vector<uint8_t> dc(n);
+        std::vector<double> x(n), y(n), z(n), vx(n), vy(n), vz(n), energy(n), biomass(n), co2(n);
+        std::vector<float> tc(n), ep(n), dxv(n), dyv(n), dzv(n), age(n), tm(n), rt(n), tl(n), ir(n), cl(n), nl(n);
+        astro_cell_soa s{id.data(), flags.data(), dc.data(), x.data(), y.data(), z.data(), vx.data(), vy.data(),
+                         vz.data(), energy.data(), tc.data(), ep.data(), dxv.data(), dyv.data(), dzv.data(),
+                         biomass.data(), co2.data(), age.data(), rng.data(), tm.data(), rt.data(), tl.data(),
+                         ir.data(), cl.data(), nl.data()};
+        st = w->fp32 ? download_cells(w, w->c32[0], &s, n) : download_cells(w, w->c64[0], &s, n);
+        if (st != ASTRO_OK) return st;
+        auto span = [&](const auto& v) { h = fnv1a64(v.data(), sizeof(v[0]) * v.size(), h); };
+        span(id); span(flags); span(dc); span(x); span(y); span(z); span(vx); span(vy); span(vz);
+        span(energy); span(tc); span(ep); span(dxv); span(dyv); span(dzv); span(biomass); span(co2); span(age);
+        span(rng); span(tm); span(rt); span(tl); span(ir); span(cl); span(nl);
+    }
+    for (int fi : {ASTRO_FIELD_TEMPERATURE, ASTRO_FIELD_CO2, ASTRO_FIELD_N2}) {
+        const FieldHost& g = w->f[fi];
+        if (!g.on) continue;
+        std::vector<float> v(g.count());
+        if ((st = hip_check(w, hipMemcpy(v.data(), g.value, sizeof(float) * v.size(), hipMemcpyDeviceToHost), "hash field")) != ASTRO_OK)
+            return st;
+        h = fnv1a64(v.data(), sizeof(float) * v.size(), h);
+    }
+    *out = h;
+    return ASTRO_OK;
+}
+
+// ---- stage entry points ----------------------------------------------------
+int32_t astro_stage_hash_build(astro_world* w) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    w->hash_valid = false;
+    return ensure_hash(w);
+}
+
+int32_t astro_hash_dims(const astro_world* w, int32_t* nx, int32_t* ny, int32_t* nz, double* cell) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    if (nx) *nx = w->hg.nx;
+    if (ny) *ny = w->hg.ny;
+    if (nz) *nz = w->hg.nz;
+    if (cell) *cell = w->hcell;
+    return ASTRO_OK;
+}
+
+int32_t astro_hash_download(astro_world* w, uint32_t* vals, int32_t n, int32_t* bs, int32_t* be, int32_t nb) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    Counters c{};
+    int32_t st = read_counters(w, c);
+    if (st != ASTRO_OK) return st;
+    const int32_t m = std::min(n, c.count);
+    if (vals && m > 0 &&
+        (st = hip_check(w, hipMemcpy(vals, w->vals_sorted, sizeof(uint32_t) * m, hipMemcpyDeviceToHost), "hash vals")) != ASTRO_OK)
+        return st;
+    if (bs || be) {
+        std::vector<int32_t> dense(static_cast<size_t>(w->hg.nb) + 2);
+        if ((st = hip_check(w, hipMemcpy(dense.data(), w->bstart, sizeof(int32_t) * dense.size(), hipMemcpyDeviceToHost),
+                            "hash ranges")) != ASTRO_OK)
+            return st;
+        const int32_t k = std::min(nb, w->hg.nb);
+        for (int32_t b = 0; b < k; ++b) {
+            const bool any = dense[b + 1] > dense[b];
+            if (bs) bs[b] = any ? dense[b] : 0;
+            if (be) be[b] = any ? dense[b + 1] : 0;
+        }
+    }
+    return ASTRO_OK;
+}
+
+int32_t astro_stage_thermal(astro_world* w, double dt) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    const Launch L = launch_of(w);
+    const int32_t st = w->fp32 ? enqueue_thermal<float>(w, L, dt) : enqueue_thermal<double>(w, L, dt);
+    drop_graph(w);
+    if (st != ASTRO_OK) return st;
+    return hip_check(w, hipGetLastError(), "thermal");
+}
+
+int32_t astro_stage_emission(astro_world* w, double dt) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    if (!lit(w)) return ASTRO_OK;
+    int32_t st = ensure_hash(w);
+    if (st != ASTRO_OK) return st;
+    const Launch L = launch_of(w);
+    const TickParams t = tick_params(w, dt);
+    if (w->fp32) {
+        enqueue_light<float>(w, L, t);
+        launch_feed_taxis<float>(L, make_view<float>(w), t, true, false);
+    } else {
+        enqueue_light<double>(w, L, t);
+        launch_feed_taxis<double>(L, make_view<double>(w), t, true, false);
+    }
+    return hip_check(w, hipGetLastError(), "emission");
+}
+
+int32_t astro_stage_taxis(astro_world* w, double dt) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    const Launch L = launch_of(w);
+    const TickParams t = tick_params(w, dt);
+    if (w->fp32) launch_feed_taxis<float>(L, make_view<float>(w), t, false, true);
+    else launch_feed_taxis<double>(L, make_view<double>(w), t, false, true);
+    return hip_check(w, hipGetLastError(), "taxis");
+}
+
+int32_t astro_stage_forces(astro_world* w) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    int32_t st = ensure_hash(w);
+    if (st != ASTRO_OK) return st;
+    const Launch L = launch_of(w);
+    const TickParams t = tick_params(w, ASTRO_DT_PHYSICS * w->physics_rate);
+    if (w->fp32) {
+        launch_sample_base<float>(L, make_view<float>(w), t);
+        launch_forces_walk<float>(L, make_view<float>(w), t);
+    } else {
+        launch_sample_base<double>(L, make_view<double>(w), t);
+        launch_forces_walk<double>(L, make_view<double>(w), t);
+    }
+    return hip_check(w, hipGetLastError(), "forces");
+}
+
+int32_t astro_forces_download(astro_world* w, double* fx, double* fy, double* fz, int32_t n) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    Counters c{};
+    int32_t st = read_counters(w, c);
+    if (st != ASTRO_OK) return st;
+    const int32_t m = std::min(n, c.count);
+    if (m <= 0) return ASTRO_OK;
+    std::vector<Force> f(m);
+    std::vector<int32_t> sp(m);
+    if ((st = hip_check(w, hipMemcpy(f.data(), w->fs, sizeof(Force) * m, hipMemcpyDeviceToHost), "forces")) != ASTRO_OK) return st;
+    if ((st = hip_check(w, hipMemcpy(sp.data(), w->spos, sizeof(int32_t) * m, hipMemcpyDeviceToHost), "spos")) != ASTRO_OK) return st;
+    for (int32_t i = 0; i < m; ++i) {
+        const Force& F = f[sp[i]];
+        if (fx) fx[i] = F.x;
+        if (fy) fy[i] = F.y;
+        if (fz) fz[i] = F.z;
+    }
+    return ASTRO_OK;
+}
+
+int32_t astro_stage_integrate(astro_world* w, double dt) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    const Launch L = launch_of(w);
+    const TickParams t = tick_params(w, dt);
+    if (w->fp32) launch_integrate<float>(L, make_view<float>(w), t);
+    else launch_integrate<double>(L, make_view<double>(w), t);
+    w->hash_valid = false;
+    return hip_check(w, hipGetLastError(), "integrate");
+}
+
+int32_t astro_stage_motion(astro_world* w, double dt) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    int32_t st = ensure_hash(w);
+    if (st != ASTRO_OK) return st;
+    const Launch L = launch_of(w);
+    const TickParams t = tick_params(w, dt);
+    if (w->fp32) {
+        const WorldDev<float> d = make_view<float>(w);
+        launch_sample_base<float>(L, d, t);
+        launch_forces_walk<float>(L, d, t);
+        launch_integrate<float>(L, d, t);
+    } else {
+        const WorldDev<double> d = make_view<double>(w);
+        launch_sample_base<double>(L, d, t);
+        launch_forces_walk<double>(L, d, t);
+        launch_integrate<double>(L, d, t);
+    }
+    w->hash_valid = false;
+    return hip_check(w, hipGetLastError(), "motion");
+}
+
+int32_t astro_stage_slow_fields(astro_world* w, double dt) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    const Launch L = launch_of(w);
+    int32_t st = enqueue_diffuse(w, L, ASTRO_FIELD_CO2, dt);
+    if (st == ASTRO_OK) st = enqueue_diffuse(w, L, ASTRO_FIELD_N2, dt);
+    drop_graph(w);
+    if (st != ASTRO_OK) return st;
+    return hip_check(w, hipGetLastError(), "slow fields");
+}
+
+int32_t astro_stage_diffuse(astro_world* w, int32_t f, double dt) {
+    if (!w || f < 0 || f >= ASTRO_FIELD_COUNT) return ASTRO_INVALID_ARGUMENT;
+    const int32_t st = enqueue_diffuse(w, launch_of(w), f, dt);
+    drop_graph(w);
+    if (st != ASTRO_OK) return st;
+    return hip_check(w, hipGetLastError(), "diffuse");
+}
+
+int32_t astro_stage_lifecycle(astro_world* w, double dt) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    const Launch L = launch_of(w);
+    const TickParams t = tick_params(w, dt);
+    if (w->fp32) launch_lifecycle<float>(L, make_view<float>(w), t);
+    else launch_lifecycle<double>(L, make_view<double>(w), t);
+    w->hash_valid = false;
+    return hip_check(w, hipGetLastError(), "lifecycle");
+}
+
+int32_t astro_stage_compact(astro_world* w) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    const Launch L = launch_of(w);
+    if (w->fp32) launch_compact<float>(L, make_view<float>(w), true);
+    else launch_compact<double>(L, make_view<double>(w), true);
+    w->hash_valid = false;
+    return hip_check(w, hipGetLastError(), "compact");
+}
+
+// ---- measurement -------------------------------------------------------------
+int32_t astro_profile_enable(astro_world* w, int32_t on) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    w->prof_on = on != 0;
+    return ASTRO_OK;
+}
+
+int32_t astro_profile_read(astro_world* w, char* names, int32_t stride, int64_t* launches, double* total_ms,
+                           int32_t max, int32_t* n_out) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    w->prof->drain();
+    int32_t k = 0;
+    for (const std::string& name : w->prof->order) {
+        if (k >= max) break;
+        const auto& v = w->prof->totals[name];
+        if (names && stride > 0) {
+            std::strncpy(names + static_cast<size_t>(k) * stride, name.c_str(), stride - 1);
+            names[static_cast<size_t>(k) * stride + stride - 1] = '\0';
+        }
+        if (launches) launches[k] = v.first;
+        if (total_ms) total_ms[k] = v.second;
+        ++k;
+    }
+    if (n_out) *n_out = k;
+    return ASTRO_OK;
+}
+
+int32_t astro_profile_reset(astro_world* w) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    w->prof->drain();
+    w->prof->totals.clear();
+    w->prof->order.clear();
+    return ASTRO_OK;
+}
+
+void* astro_world_stream(astro_world* w) { return w ? static_cast<void*>(w->stream) : nullptr; }
+
+}  // extern "C"

@@ -1,0 +1,259 @@
+/* astro_hot.h -- C ABI of the MI355X per-step hot path.
+ *
+ * This is the drop-in boundary for the reference simulator's world API
+ * (src/sim/world.cuh:183-243), cell store (src/sim/cell_store.cuh:61-105),
+ * spatial hash (src/sim/hash.cuh:50-53) and field grids
+ * (src/fields/grid.cuh:87-98). Every entry point is plain C: opaque handles,
+ * plain pointers and sizes, int32 status codes. Host buffers passed in are
+ * caller-owned and are copied synchronously; the world owns all device memory,
+ * allocated once at create (no allocation inside a step).
+ *
+ * Threading: calls on one world are serialized by the caller (one host thread
+ * per world, as in the reference, docs/ARCHITECTURE.md:73-78). Distinct worlds
+ * may be driven concurrently. Each world runs on its own HIP stream. Step calls
+ * return once work is enqueued; download, stats and hash calls synchronize.
+ *
+ * Precision: ASTRO_FP64_CONTRACT keeps the reference's contract-v1 SoA
+ * (f64 kinematics, contracts/cell_store_v1.h:42-78); ASTRO_FP32_HOT stores
+ * position and velocity as f32 in device memory. Host-side views are always the
+ * f64 contract layout.
+ */
+#ifndef ASTRO_HOT_H
+#define ASTRO_HOT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ASTRO_ABI_VERSION 1
+
+/* Status codes, in the order of astro::Status (src/core/result.h:12-23). */
+enum {
+    ASTRO_OK = 0,
+    ASTRO_OUT_OF_MEMORY = 1,
+    ASTRO_DEVICE_ERROR = 2,            /* reference: CudaError */
+    ASTRO_INVALID_ARGUMENT = 3,
+    ASTRO_NOT_FOUND = 4,
+    ASTRO_PARSE_ERROR = 5,
+    ASTRO_CONTRACT_VERSION_MISMATCH = 6,
+    ASTRO_CAPACITY_EXCEEDED = 7,
+    ASTRO_FILE_IO_ERROR = 8,
+    ASTRO_UNSUPPORTED = 9
+};
+
+enum { ASTRO_FP64_CONTRACT = 0, ASTRO_FP32_HOT = 1 };
+
+/* Boundary (src/sim/integrator.cuh:227), gravity axis (:175),
+ * store disposition (contracts/scenario_v3.h:35). */
+enum { ASTRO_BOUNDARY_REFLECTING = 0, ASTRO_BOUNDARY_PERIODIC = 1, ASTRO_BOUNDARY_ABSORBING = 2 };
+enum { ASTRO_GRAVITY_Y = 0, ASTRO_GRAVITY_Z = 1 };
+enum { ASTRO_STORE_VOID = 0, ASTRO_STORE_FLASH = 1, ASTRO_STORE_RETAIN = 2 };
+
+/* Field ids and boundary conditions (contracts/fields_v1.h:98-110). */
+enum { ASTRO_FIELD_TEMPERATURE = 0, ASTRO_FIELD_CO2 = 1, ASTRO_FIELD_N2 = 2,
+       ASTRO_FIELD_IRRADIANCE = 3, ASTRO_FIELD_COUNT = 4 };
+enum { ASTRO_BC_DIRICHLET = 0, ASTRO_BC_NEUMANN = 1, ASTRO_BC_ROBIN = 2 };
+
+/* Spawn placement and charge distributions (src/sim/cell_store.cuh:15-16);
+ * ASTRO_DIST_LOG_UNIFORM is an addition: charge = exp(U[ln a, ln b]). */
+enum { ASTRO_PLACE_UNIFORM = 0, ASTRO_PLACE_GAUSSIAN = 1, ASTRO_PLACE_GRID = 2, ASTRO_PLACE_DISC = 3 };
+enum { ASTRO_DIST_CONSTANT = 0, ASTRO_DIST_UNIFORM = 1, ASTRO_DIST_NORMAL = 2,
+       ASTRO_DIST_LOG_UNIFORM = 3 };
+
+/* One grid's resolution. nz == 1 is the reference's depth-averaged 2D grid
+ * (src/fields/grid.cuh:72-85); nz > 1 resolves the chamber depth with a
+ * 7-point stencil. nx == 0 disables the field (N2 only). */
+typedef struct astro_grid_dims {
+    int32_t nx, ny, nz;
+} astro_grid_dims;
+
+/* Mirrors WorldDesc + MotionConfig + the World knobs the step reads
+ * (src/sim/world.cuh:48-180). astro_world_desc_default() fills the
+ * reference defaults. */
+typedef struct astro_world_desc {
+    int32_t  abi_version;        /* ASTRO_ABI_VERSION */
+    int32_t  capacity;           /* 0 = ASTRO_DEFAULT_CELLS */
+    uint64_t seed;
+    double   chamber_w, chamber_h, chamber_d;
+    int32_t  precision;          /* ASTRO_FP64_CONTRACT | ASTRO_FP32_HOT */
+    int32_t  _pad0;
+    double   hash_cell_size;     /* 0 = HASH_CELL_FACTOR * CELL_DIAMETER (22 um) */
+    astro_grid_dims grid[ASTRO_FIELD_COUNT];  /* per field; zeros = reference n x n x 1 */
+    int32_t  _pad1;
+    /* MotionConfig */
+    uint8_t  boundary_x, boundary_y, gravity_axis, thermal_noise;
+    uint8_t  contact_enabled, adhesion_enabled, thermal_enabled, emission_enabled;
+    uint8_t  taxis_enabled, occlusion_exact, store_disposition, compaction_enabled;
+    uint8_t  lifecycle_enabled, _pad2[7];
+    double   ambient_temp;
+    double   co2_init;           /* initial CO2 concentration [kg/m^3] */
+    double   division_offset;    /* daughter offset along the axis; 0 = CELL_RADIUS */
+    /* clock (src/sim/step.cu:133, :190-215) */
+    double   physics_rate, biology_rate;
+    /* curated overrides (src/sim/world.cuh:168-170) */
+    double   petrova_max_power;
+    double   co2_mass_per_division;
+    /* directional light (contracts/fields_v1.h:189-194) */
+    float    light_dir_x, light_dir_y, light_irradiance, ambient_irradiance;
+    uint8_t  light_enabled, _pad3[7];
+} astro_world_desc;
+
+/* SpawnParams (src/sim/cell_store.cuh:18-27). */
+typedef struct astro_spawn_params {
+    int32_t count;
+    int32_t placement;
+    double  place_x, place_y, place_radius;
+    int32_t charge_dist;
+    int32_t awake;
+    double  charge_a, charge_b;
+} astro_spawn_params;
+
+/* Host-side view of the cell SoA in contract-v1 declaration order
+ * (contracts/cell_store_v1.h:42-74). Each pointer addresses n elements. */
+typedef struct astro_cell_soa {
+    uint64_t* id;
+    uint32_t* flags;
+    uint8_t*  death_cause;
+    double *x, *y, *z;
+    double *vx, *vy, *vz;
+    double*   energy;
+    float*    temp_cell;
+    float*    emit_power;
+    float *dir_x, *dir_y, *dir_z;
+    double*   biomass;
+    double*   co2_held;
+    float*    age_s;
+    uint64_t* rng_state;
+    float*    taxis_memory;
+    float*    run_timer;
+    float *t_local, *irradiance, *co2_local, *n2_local;
+} astro_cell_soa;
+
+/* Host bookkeeping the reference keeps in CellStore/World
+ * (src/sim/cell_store.cuh:36-59, src/sim/world.cuh:111-129). */
+typedef struct astro_counters {
+    int32_t  count;              /* slots [0, count) may be occupied */
+    int32_t  capacity;
+    uint64_t next_id;
+    uint64_t tick;
+    double   sim_time_s;
+    int64_t  divisions_total;
+    int64_t  deaths_total;
+    int32_t  dead_slots_prev;
+    int32_t  _pad;
+} astro_counters;
+
+/* contract::Stats (contracts/telemetry_v1.h:17-39). */
+typedef struct astro_stats {
+    uint64_t tick;
+    double   sim_time_s;
+    int32_t  n_live, n_dead, n_dormant, n_awake, n_taumoeba;
+    int32_t  _pad0;
+    double   mean_charge;
+    double   total_energy_j;
+    double   mean_temp_cell_k;
+    double   mean_temp_medium_k;
+    double   max_temp_medium_k;
+    double   co2_total_kg;
+    double   n2_total_kg;
+    double   mean_tau_tolerance;
+    int32_t  divisions_this_window;
+    int32_t  deaths_this_window;
+    int32_t  boil_events;
+    int32_t  ignitions_this_window;
+    double   physics_rate, biology_rate;
+    uint8_t  non_canon_run, _pad1[7];
+} astro_stats;
+
+typedef struct astro_world astro_world;
+
+/* ---- lifetime (src/sim/world.cuh:222-223) ---------------------------------- */
+void        astro_world_desc_default(astro_world_desc* d);
+int32_t     astro_world_create(const astro_world_desc* d, astro_world** out);
+void        astro_world_destroy(astro_world* w);
+const char* astro_status_str(int32_t status);
+const char* astro_last_error(const astro_world* w);
+int32_t     astro_world_desc_get(const astro_world* w, astro_world_desc* out);
+
+/* ---- population (src/sim/cell_store.cuh:61-105) ---------------------------- */
+/* cell_store_spawn: appends p->count cells seeded from (seed, id). */
+int32_t astro_cells_spawn(astro_world* w, const astro_spawn_params* p, uint64_t seed);
+/* Replaces slots [0, n) with the given cells and sets count = n. */
+int32_t astro_cells_upload(astro_world* w, const astro_cell_soa* src, int32_t n);
+/* Copies slots [0, min(count, n)) out; returns the number copied in *n_out. */
+int32_t astro_cells_download(astro_world* w, astro_cell_soa* dst, int32_t n, int32_t* n_out);
+int32_t astro_world_get_counters(astro_world* w, astro_counters* out);
+int32_t astro_world_set_counters(astro_world* w, const astro_counters* in);
+
+/* ---- fields (src/fields/grid.cuh:87-98) ------------------------------------ */
+int32_t astro_field_dims(const astro_world* w, int32_t field, astro_grid_dims* out,
+                         double* dx, double* dy, double* dz);
+int32_t astro_field_upload(astro_world* w, int32_t field, const float* v, size_t count);
+int32_t astro_field_download(astro_world* w, int32_t field, float* v, size_t count);
+/* Fixed-point deposit accumulator of a field (contracts/fields_v1.h:166). */
+int32_t astro_field_deposit_download(astro_world* w, int32_t field, int64_t* v, size_t count);
+
+/* ---- the step (src/sim/step.cu:109-184) ------------------------------------ */
+int32_t astro_world_step(astro_world* w, int32_t n_ticks);
+int32_t astro_world_sync(astro_world* w);
+/* world_stats (src/sim/stats.cu:102-153): runs the fixed-point reduction. */
+int32_t astro_world_stats(astro_world* w, astro_stats* out);
+/* snapshot_state_hash (src/sim/snapshot.cu:123-129): FNV-1a 64 over the
+ * contract-ordered SoA [0, count) then the T, CO2, N2 values. */
+int32_t astro_world_state_hash(astro_world* w, uint64_t* out);
+/* world_set_clock with explicit rates (src/sim/step.cu:190-215). */
+int32_t astro_world_set_clock(astro_world* w, double physics_rate, double biology_rate);
+
+/* ---- stage entry points for kernel-level parity ---------------------------- */
+/* hash_build (src/sim/hash.cu:106-133) on the current population. */
+int32_t astro_stage_hash_build(astro_world* w);
+int32_t astro_hash_dims(const astro_world* w, int32_t* nx, int32_t* ny, int32_t* nz,
+                        double* cell_size);
+/* vals_sorted[0,count), bucket_start/bucket_end[0,bucket_count) with the
+ * reference's semantics (empty bucket: start == end == 0). */
+int32_t astro_hash_download(astro_world* w, uint32_t* vals_sorted, int32_t n,
+                            int32_t* bucket_start, int32_t* bucket_end, int32_t nb);
+/* thermal_step (src/sim/thermal.cu:75-103). */
+int32_t astro_stage_thermal(astro_world* w, double dt);
+/* emission_step (src/sim/emission.cu:119-177), directional path. */
+int32_t astro_stage_emission(astro_world* w, double dt);
+/* taxis_step (src/sim/taxis.cu:82-87). */
+int32_t astro_stage_taxis(astro_world* w, double dt);
+/* field_sample + forces_kernel (src/sim/integrator.cu:20-76) into the force
+ * buffer; astro_forces_download reads it back in slot order. */
+int32_t astro_stage_forces(astro_world* w);
+int32_t astro_forces_download(astro_world* w, double* fx, double* fy, double* fz, int32_t n);
+/* integrate_kernel (src/sim/integrator.cu:79-150) from the force buffer. */
+int32_t astro_stage_integrate(astro_world* w, double dt);
+/* motion_step = sample + forces + integrate (src/sim/integrator.cu:154-177). */
+int32_t astro_stage_motion(astro_world* w, double dt);
+/* stages 7-8 for CO2 and N2 (src/sim/step.cu:168-171). */
+int32_t astro_stage_slow_fields(astro_world* w, double dt);
+/* one grid_diffuse (src/fields/diffuse.cu:159-186) of one field. */
+int32_t astro_stage_diffuse(astro_world* w, int32_t field, double dt);
+/* lifecycle_step (src/sim/lifecycle.cu:208-294) incl. the compaction trigger. */
+int32_t astro_stage_lifecycle(astro_world* w, double dt);
+/* cell_store_compact (src/sim/cell_store.cu:414-456). */
+int32_t astro_stage_compact(astro_world* w);
+
+/* ---- measurement ------------------------------------------------------------ */
+/* Per-kernel HIP-event timing on the world's stream. While enabled, every
+ * kernel launch of the step is bracketed by events and the step runs eagerly
+ * (no graph). */
+int32_t astro_profile_enable(astro_world* w, int32_t on);
+/* Fills up to max entries: kernel name, launches, total milliseconds. */
+int32_t astro_profile_read(astro_world* w, char* names, int32_t name_stride,
+                           int64_t* launches, double* total_ms, int32_t max,
+                           int32_t* n_out);
+int32_t astro_profile_reset(astro_world* w);
+/* Opaque hipStream_t of the world, for callers that order their own work. */
+void*   astro_world_stream(astro_world* w);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ASTRO_HOT_H */

