@@ -102,7 +102,7 @@ class Counters(C.Structure):
     _fields_ = [
         ("count", C.c_int32), ("capacity", C.c_int32), ("next_id", C.c_uint64), ("tick", C.c_uint64),
         ("sim_time_s", C.c_double), ("divisions_total", C.c_int64), ("deaths_total", C.c_int64),
-        ("dead_slots_prev", C.c_int32), ("_pad", C.c_int32),
+        ("dead_slots_prev", C.c_int32), ("_pad", C.c_int32), ("cell_steps", C.c_int64),
     ]
 
 
@@ -123,7 +123,7 @@ class Stats(C.Structure):
 
 assert C.sizeof(WorldDesc) == 208, C.sizeof(WorldDesc)
 assert C.sizeof(SpawnParams) == 56
-assert C.sizeof(Counters) == 56
+assert C.sizeof(Counters) == 64
 assert C.sizeof(Stats) == 144
 
 # Every entry point of include/astro_hot.h: name -> (restype, argtypes).

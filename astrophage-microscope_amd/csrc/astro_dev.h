@@ -64,6 +64,8 @@ struct Counters {
     uint32_t scan_epoch;       // look-back scan generation, never reset
     uint32_t error_flags;      // sticky device-side errors (see ERR_*)
     int32_t  _pad;
+    int64_t  cell_steps;       // sum of the count each tick began with
+    unsigned long long scan_base;  // tiles handed out by all earlier scans
 };
 
 enum : uint32_t { ERR_SCAN_SPIN = 1u, ERR_COEFF = 2u };

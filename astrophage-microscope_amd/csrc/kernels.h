@@ -87,7 +87,7 @@ struct Ftcs {
 };
 
 // ---- hash (kernels_hash.hip) ----
-template <class P> void launch_hash_build(const Launch& L, const WorldDev<P>& w);
+template <class P> void launch_hash_build(const Launch& L, const WorldDev<P>& w, bool count_tick);
 template <class P> void launch_forces_walk(const Launch& L, const WorldDev<P>& w, const TickParams& t);
 template <class P> void launch_feed_walk(const Launch& L, const WorldDev<P>& w, const TickParams& t);
 
@@ -111,7 +111,7 @@ template <class P> void launch_compact(const Launch& L, const WorldDev<P>& w, bo
 template <class P> void launch_stats_cells(const Launch& L, const WorldDev<P>& w);
 
 // ---- fields (kernels_fields.hip) ----
-void launch_ftcs(const Launch& L, const Grid& g, const Ftcs& f);
+void launch_ftcs(const Launch& L, const Grid& g, const Ftcs& f, const char* name);
 template <class P> void launch_stamp(const Launch& L, const WorldDev<P>& w);
 void launch_sweep(const Launch& L, const Grid& irr, const TickParams& t);
 void launch_fill(const Launch& L, float* v, size_t n, float value);

@@ -486,7 +486,9 @@ __global__ __launch_bounds__(BLOCK) void k_divide_scan(Counters* ctr, const int3
                                                        int32_t* total, ScanCtl sc) {
     const int n = ctr->births > 0 ? ctr->count : 0;
     if (n == 0) return;
-    scan_exclusive(n, [&](int i) { return dflag[i]; }, [&](int i, int32_t ex) { prefix[i] = ex; }, sc, total);
+    scan_exclusive(
+        n, [&](int i) { return load4_i32(dflag, i, n); }, [&](int i, int4 ex) { store4_i32(prefix, i, n, ex); }, sc,
+        total);
 }
 
 // divide_kernel (lifecycle.cu:140-204). Daughters go to slot count + prefix,
@@ -583,13 +585,19 @@ __global__ __launch_bounds__(BLOCK) void k_compact_scan(Counters* ctr, const uin
                                                         ScanCtl sc) {
     const int n = ctr->compacting ? ctr->count : 0;
     if (n == 0) return;
+    auto keep = [](uint32_t f) { return ((f & ASTRO_FLAG_OCCUPIED) && (f & ASTRO_FLAG_ALIVE)) ? 1 : 0; };
     scan_exclusive(
         n,
         [&](int i) {
-            const uint32_t f = flags[i];
-            return ((f & ASTRO_FLAG_OCCUPIED) && (f & ASTRO_FLAG_ALIVE)) ? 1 : 0;
+            const int4 f = load4_i32(reinterpret_cast<const int32_t*>(flags), i, n);
+            int4 v;
+            v.x = i < n ? keep(static_cast<uint32_t>(f.x)) : 0;
+            v.y = i + 1 < n ? keep(static_cast<uint32_t>(f.y)) : 0;
+            v.z = i + 2 < n ? keep(static_cast<uint32_t>(f.z)) : 0;
+            v.w = i + 3 < n ? keep(static_cast<uint32_t>(f.w)) : 0;
+            return v;
         },
-        [&](int i, int32_t ex) { prefix[i] = ex; }, sc, &ctr->live);
+        [&](int i, int4 ex) { store4_i32(prefix, i, n, ex); }, sc, &ctr->live);
 }
 
 template <class P>

@@ -160,10 +160,10 @@ __global__ __launch_bounds__(BLOCK) void k_stats_fields(Grid T, Grid C, Grid N, 
 
 }  // namespace
 
-void launch_ftcs(const Launch& L, const Grid& g, const Ftcs& f) {
+void launch_ftcs(const Launch& L, const Grid& g, const Ftcs& f, const char* name) {
     if (f.square2d) {
         const int tiles = ((g.nx + 63) / 64) * ((g.ny + 3) / 4);
-        if (L.prof) L.prof->begin("ftcs2d");
+        if (L.prof) L.prof->begin(name);
         k_ftcs2d<<<blocks_for_xcd(tiles), BLOCK, 0, L.stream>>>(g.value, g.scratch, g.nx, f.coeff, f.bc, f.amb, f.rx);
         if (L.prof) L.prof->end();
         return;
@@ -172,7 +172,7 @@ void launch_ftcs(const Launch& L, const Grid& g, const Ftcs& f) {
     int zc = g.nz;
     while (zc > 4 && txy * ((g.nz + zc - 1) / zc) < 2048) zc = (zc + 1) / 2;
     const int tiles = txy * ((g.nz + zc - 1) / zc);
-    if (L.prof) L.prof->begin("ftcs3d");
+    if (L.prof) L.prof->begin(name);
     k_ftcs3d<<<blocks_for_xcd(tiles), BLOCK, 0, L.stream>>>(g.value, g.scratch, g.nx, g.ny, g.nz, zc, f.cx, f.cy,
                                                              f.cz, f.bc, f.amb, f.rx, f.ry, f.rz);
     if (L.prof) L.prof->end();

@@ -22,10 +22,12 @@ namespace astro_dev {
 namespace {
 
 template <class P>
-__global__ __launch_bounds__(BLOCK) void k_hash_key(Cells<P> c, const Counters* ctr, HashGeo h,
-                                                    uint32_t* keys, uint32_t* rin, uint32_t* bcount) {
+__global__ __launch_bounds__(BLOCK) void k_hash_key(Cells<P> c, Counters* ctr, HashGeo h,
+                                                    uint32_t* keys, uint32_t* rin, uint32_t* bcount,
+                                                    int count_tick) {
     const int n = ctr->count;
     const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (count_tick && i == 0) ctr->cell_steps += n;
     if (i >= n) return;
     const uint32_t f = c.flags[i];
     const uint32_t key = (f & ASTRO_FLAG_OCCUPIED)
@@ -36,14 +38,15 @@ __global__ __launch_bounds__(BLOCK) void k_hash_key(Cells<P> c, const Counters* 
 }
 
 __global__ __launch_bounds__(BLOCK) void k_bucket_scan(uint32_t* bcount, int32_t* bstart, int nbp1, ScanCtl sc) {
+    int32_t* counts = reinterpret_cast<int32_t*>(bcount);
     scan_exclusive(
         nbp1,
         [&](int i) {
-            const int32_t v = static_cast<int32_t>(bcount[i]);
-            if (v) bcount[i] = 0u;
+            const int4 v = load4_i32(counts, i, nbp1);
+            if (v.x | v.y | v.z | v.w) store4_i32(counts, i, nbp1, int4{0, 0, 0, 0});
             return v;
         },
-        [&](int i, int32_t ex) { bstart[i] = ex; }, sc, bstart + nbp1);
+        [&](int i, int4 ex) { store4_i32(bstart, i, nbp1, ex); }, sc, bstart + nbp1);
 }
 
 template <class P>
@@ -193,10 +196,10 @@ inline int grid_for(int n) { return (n + BLOCK - 1) / BLOCK; }
 }  // namespace
 
 template <class P>
-void launch_hash_build(const Launch& L, const WorldDev<P>& w) {
+void launch_hash_build(const Launch& L, const WorldDev<P>& w, bool count_tick) {
     const int g = grid_for(w.capacity);
     if (L.prof) L.prof->begin("hash_key");
-    k_hash_key<P><<<g, BLOCK, 0, L.stream>>>(w.c, w.ctr, w.hg, w.keys, w.rin, w.bcount);
+    k_hash_key<P><<<g, BLOCK, 0, L.stream>>>(w.c, w.ctr, w.hg, w.keys, w.rin, w.bcount, count_tick ? 1 : 0);
     if (L.prof) L.prof->end();
     const int nbp1 = w.hg.nb + 1;
     if (L.prof) L.prof->begin("hash_bucket_scan");
@@ -228,8 +231,8 @@ void launch_feed_walk(const Launch& L, const WorldDev<P>& w, const TickParams& t
     if (L.prof) L.prof->end();
 }
 
-template void launch_hash_build<double>(const Launch&, const WorldDev<double>&);
-template void launch_hash_build<float>(const Launch&, const WorldDev<float>&);
+template void launch_hash_build<double>(const Launch&, const WorldDev<double>&, bool);
+template void launch_hash_build<float>(const Launch&, const WorldDev<float>&, bool);
 template void launch_forces_walk<double>(const Launch&, const WorldDev<double>&, const TickParams&);
 template void launch_forces_walk<float>(const Launch&, const WorldDev<float>&, const TickParams&);
 template void launch_feed_walk<double>(const Launch&, const WorldDev<double>&, const TickParams&);

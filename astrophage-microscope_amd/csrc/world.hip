@@ -320,8 +320,9 @@ int32_t enqueue_diffuse(astro_world* w, const Launch& L, int field, double dt) {
     const double dt_sub = dt / steps;
     Ftcs fc;
     if (!ftcs_coeffs(g, dt_sub, fc)) return fail(w, ASTRO_INVALID_ARGUMENT, "diffusion coefficient unstable");
+    static const char* names[ASTRO_FIELD_COUNT] = {"ftcs_T", "ftcs_CO2", "ftcs_N2", "ftcs_irradiance"};
     for (int s = 0; s < steps; ++s) {
-        launch_ftcs(L, g.grid(), fc);
+        launch_ftcs(L, g.grid(), fc, names[field]);
         std::swap(g.value, g.scratch);
     }
     return ASTRO_OK;
@@ -340,7 +341,7 @@ int32_t enqueue_thermal(astro_world* w, const Launch& L, double dt) {
         const WorldDev<P> d = make_view<P>(w);
         launch_thermal_substep<P>(L, d, dt_sub, cap, w->hash_valid);
         launch_apply_awake<P>(L, d, ASTRO_FIELD_TEMPERATURE);
-        launch_ftcs(L, T.grid(), fc);
+        launch_ftcs(L, T.grid(), fc, "ftcs_T");
         std::swap(T.value, T.scratch);
     }
     return ASTRO_OK;
@@ -362,7 +363,7 @@ int32_t enqueue_tick(astro_world* w, const Launch& L) {
     const double dt = ASTRO_DT_PHYSICS * w->physics_rate;
     const TickParams t = tick_params(w, dt);
     const astro_world_desc& ds = w->desc;
-    launch_hash_build<P>(L, make_view<P>(w));
+    launch_hash_build<P>(L, make_view<P>(w), true);
     w->hash_valid = true;
     int32_t st;
     if (ds.thermal_enabled && (st = enqueue_thermal<P>(w, L, dt)) != ASTRO_OK) return st;
@@ -502,8 +503,8 @@ uint64_t fnv1a64(const void* data, size_t n, uint64_t h) {
 int32_t ensure_hash(astro_world* w) {
     if (w->hash_valid) return ASTRO_OK;
     const Launch L = launch_of(w);
-    if (w->fp32) launch_hash_build<float>(L, make_view<float>(w));
-    else launch_hash_build<double>(L, make_view<double>(w));
+    if (w->fp32) launch_hash_build<float>(L, make_view<float>(w), false);
+    else launch_hash_build<double>(L, make_view<double>(w), false);
     w->hash_valid = true;
     return hip_check(w, hipGetLastError(), "hash_build");
 }
@@ -577,8 +578,8 @@ int32_t astro_world_create(const astro_world_desc* din, astro_world** out) {
     for (int f = 0; f < 4; ++f) {
         astro_grid_dims& g = d.grid[f];
         if (g.nx == 0 && g.ny == 0 && g.nz == 0) { g.nx = g.ny = defaults[f]; g.nz = 1; }
-        const bool optional = (f == ASTRO_FIELD_N2);
-        if (!(optional && g.nx == 0) && (g.nx <= 1 || g.ny <= 1 || g.nz < 1)) return ASTRO_INVALID_ARGUMENT;
+        if (f == ASTRO_FIELD_N2 && g.nx < 0) continue;   // disabled
+        if (g.nx <= 1 || g.ny <= 1 || g.nz < 1) return ASTRO_INVALID_ARGUMENT;
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return ASTRO_DEVICE_ERROR;
@@ -756,6 +757,7 @@ int32_t astro_world_get_counters(astro_world* w, astro_counters* o) {
     o->divisions_total = c.divisions_total;
     o->deaths_total = c.deaths_total;
     o->dead_slots_prev = c.dead_slots_prev;
+    o->cell_steps = c.cell_steps;
     return ASTRO_OK;
 }
 
@@ -770,6 +772,7 @@ int32_t astro_world_set_counters(astro_world* w, const astro_counters* in) {
     c.divisions_total = in->divisions_total;
     c.deaths_total = in->deaths_total;
     c.dead_slots_prev = in->dead_slots_prev;
+    c.cell_steps = in->cell_steps;
     w->tick = in->tick;
     w->sim_time = in->sim_time_s;
     w->hash_valid = false;

@@ -1,0 +1,60 @@
+"""Algorithmic bytes per kernel launch, for the HBM-roofline figures.
+
+"Algorithmic" means compulsory traffic: every per-cell or per-voxel array a
+kernel must touch is counted once per read and once per write, at its stored
+width. Gathers from a field grid count one element; neighbour records and
+bucket starts reached by the contact and shadow walks count zero (they are
+re-reads of data another thread of the same pass fetched). Per-cell sizes
+depend on the position width `p` (4 for the FP32 layout, 8 for FP64) and the
+neighbour-record size `r` (16 / 32 B).
+
+`counts` carries the population figures of the interval measured: n (cells),
+awake, alive, births, dead (per tick), compacting (fraction of ticks that
+compacted), plus grid sizes: pts[field] and buckets.
+"""
+from __future__ import annotations
+
+# bytes per cell of the whole SoA in each layout (contracts/cell_store_v1.h
+# order; positions and velocities at the layout's width)
+SOA_BYTES = {4: 117, 8: 141}
+
+
+def kernel_bytes(name: str, c: dict) -> float:
+    n, p = c["n"], c["p"]
+    r = 16 if p == 4 else 32
+    awake, alive = c.get("awake", 0), c.get("alive", n)
+    pts = c["pts"]
+    k_fields = c.get("sampled_fields", 2)
+    S = SOA_BYTES[p]
+    table = {
+        "hash_key": n * (16 + 3 * p),
+        "hash_bucket_scan": 8 * (c["buckets"] + 1) + 4 * min(n, c["buckets"]),
+        "hash_scatter": n * (16 + 3 * p + r),
+        "hash_rank": n * (2 * r + 16),
+        "thermal": n * (12 + 3 * p) + awake * 24,
+        "deposit_apply": 4 * n + awake * (3 * p + 16),
+        "ftcs_T": 8 * pts.get(0, 0),
+        "ftcs_CO2": 8 * pts.get(1, 0),
+        "ftcs_N2": 8 * pts.get(2, 0),
+        "taxis": 4 * n + 4 * max(alive - awake, 0) + 28 * awake,
+        "feed_taxis": 4 * n + 24 * alive + 28 * awake,
+        "sample_base": n * (72 + 3 * p + 4 * k_fields),
+        "forces_walk": n * (r + 64),
+        "integrate": n * (96 + 12 * p),
+        "co2_demand": alive * (20 + 3 * p) + 4 * n,
+        "co2_uptake": 4 * n + alive * 36,
+        "life_mark": n * 20 + alive * 28,
+        "divide_scan": 8 * n if c.get("births", 0) > 0 else 0,
+        "divide": 12 * n + c.get("births", 0) * 2 * S,
+        "life_finalize": 0,
+        "compact_scan": 8 * n * c.get("compacting", 0),
+        "compact_gather": (4 * n + 2 * S * (n - c.get("dead", 0))) * c.get("compacting", 0),
+        "compact_copyback": 2 * S * (n - c.get("dead", 0)) * c.get("compacting", 0),
+        "compact_finalize": 0,
+        "occlusion_stamp": n * (4 + 3 * p + 8),
+        "light_sweep": 12 * pts.get(3, 0),
+        "feed_walk": alive * (r + 8 + 4),
+        "spawn": n * S,
+    }
+    return float(table.get(name, 0.0))
+

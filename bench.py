@@ -1,0 +1,195 @@
+"""Throughput of the per-step hot path on one MI355X.
+
+Workload (the 1M-cell "plain method, scale" configuration): 1,000,000 cells
+uniform in a 4 x 4 x 0.2 mm chamber; 512 x 512 x 32 FP32 temperature and CO2
+grids; spatial hash with 10 um buckets; dt = 1e-4 s (physics_rate 0.1, two
+temperature substeps per tick); division on with daughters offset +/-0.5 um;
+compaction on. To make division and compaction do work every tick:
+  * 10 % of the cells are awake with a near-empty store, so the thermostat
+    starves a trickle of them each tick (about 100 deaths per tick);
+  * CO2 starts at 1 kg/m^3, the biology clock runs at 1e5, and each cell
+    starts with a uniform fraction of the division quota, so a trickle divide;
+  * the other 90 % are dormant with log-uniform charge in [1e-3, 0.2].
+The light is off. Cell state is stored FP32 (positions, velocities; FP64
+arithmetic), energies and CO2 stores FP64, fields FP32.
+
+`value` = cells x ticks / second over the timed ticks (the cell count of each
+tick summed on the device), with all state resident in HBM; the ticks are
+replayed from a captured hipGraph. Afterwards a shorter pass re-runs the same
+ticks eagerly with HIP events around every kernel on the world's stream to
+give per-kernel times and achieved bandwidth against algorithmic bytes.
+The CPU baseline is the CPU restatement of the same step (FP64, one thread)
+on the same workload for a few ticks.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+PKG = ROOT / "astrophage-microscope_amd"
+sys.path.insert(0, str(PKG))
+
+import numpy as np  # noqa: E402
+
+import astro_hot as ah  # noqa: E402
+import roofline  # noqa: E402
+
+METRIC = "cell-steps/s at 1M cells; field steps/s surrogate vs FTCS at 1e-3 rel err"
+HBM_PEAK_GBS = 8000.0
+QUOTA = 2.1e-14
+
+
+def workload(binding: ah.Binding, cells: int, precision: int):
+    return binding.desc(
+        capacity=int(cells * 1.25), seed=1, chamber_w=4e-3, chamber_h=4e-3, chamber_d=2e-4,
+        precision=precision, hash_cell_size=1e-5,
+        grid={ah.TEMPERATURE: (512, 512, 32), ah.CO2: (512, 512, 32), ah.N2: (-1, 0, 0),
+              ah.IRRADIANCE: (64, 64, 1)},
+        physics_rate=0.1, biology_rate=1e5, co2_init=1.0, division_offset=0.5e-6, compaction_enabled=1,
+    )
+
+
+def populate(w: ah.World, cells: int):
+    awake = cells // 10
+    w.spawn(cells - awake, 1, charge_dist=ah.DIST_LOG_UNIFORM, charge_a=1e-3, charge_b=0.2)
+    w.spawn(awake, 2, charge_dist=ah.DIST_UNIFORM, charge_a=0.0, charge_b=1e-10, awake=True)
+    c = w.download()
+    c["co2_held"] = np.random.default_rng(1).uniform(0.0, 1.0, cells) * QUOTA
+    w.upload(c)
+    ctr = w.counters()
+    ctr.next_id = cells + 1
+    w.set_counters(ctr)
+
+
+def population(w: ah.World) -> dict:
+    s = w.stats()
+    return {"n": s["n_live"] + s["n_dead"], "alive": s["n_live"], "awake": s["n_awake"],
+            "dead": s["n_dead"], "divisions": s["divisions_this_window"], "deaths": s["deaths_this_window"]}
+
+
+def cpu_baseline(cells: int, steps: int) -> dict:
+    oracle_lib = ROOT / "oracle" / "liboracle.so"
+    if not oracle_lib.exists():
+        subprocess.run(["make", "-C", str(ROOT / "oracle"), "oracle"], check=True, stdout=subprocess.DEVNULL)
+    ob = ah.Binding(oracle_lib, "oracle_")
+    with ah.World(ob, workload(ob, cells, ah.FP64_CONTRACT)) as w:
+        populate(w, cells)
+        w.step(1)
+        c0 = w.counters().cell_steps
+        t0 = time.perf_counter()
+        w.step(steps)
+        dt = time.perf_counter() - t0
+        done = w.counters().cell_steps - c0
+    return {"value": done / dt, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} ticks of the same {cells}-cell workload after 1 warm-up tick, "
+                      f"FP64 CPU restatement, single thread, {dt:.1f} s",
+            "seconds": dt}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cells", type=int, default=1_000_000)
+    ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(one chamber per GPU; see the sharding notes in DESIGN.md).", file=sys.stderr)
+        return 2
+
+    gpu = ah.load()
+    precision = ah.FP32_HOT if a.precision == "fp32" else ah.FP64_CONTRACT
+    w = ah.World(gpu, workload(gpu, a.cells, precision))
+    populate(w, a.cells)
+    w.step(a.warmup)
+    w.sync()
+
+    c0 = w.counters()
+    t0 = time.perf_counter()
+    w.step(a.steps)
+    w.sync()
+    elapsed = time.perf_counter() - t0
+    c1 = w.counters()
+    cell_steps = c1.cell_steps - c0.cell_steps
+    value = cell_steps / elapsed
+
+    # Per-kernel pass: the same ticks, eager, HIP events on the world's stream.
+    before = population(w)
+    w.profile(True)
+    w.profile_reset()
+    w.step(a.profile_steps)
+    kern = w.profile_read()
+    w.profile(False)
+    after = population(w)
+    ticks = a.profile_steps
+    pbytes = 4 if precision == ah.FP32_HOT else 8
+    counts = {
+        "n": 0.5 * (before["n"] + after["n"]), "alive": 0.5 * (before["alive"] + after["alive"]),
+        "awake": 0.5 * (before["awake"] + after["awake"]), "dead": after["deaths"] / max(ticks, 1),
+        "births": after["divisions"] / max(ticks, 1), "compacting": 1.0 if after["deaths"] > 0 else 0.0,
+        "p": pbytes, "buckets": 400 * 400 * 20, "pts": {0: 512 * 512 * 32, 1: 512 * 512 * 32},
+        "sampled_fields": 2,
+    }
+    stages = {}
+    total_ms = 0.0
+    for name, (launches, ms) in kern.items():
+        total_ms += ms
+        per = ms / max(launches, 1)
+        algo = roofline.kernel_bytes(name, counts)
+        stages[name] = {"launches": launches, "avg_us": round(per * 1e3, 2),
+                        "algo_bytes_per_launch": algo,
+                        "achieved_GBs": round(algo / (per * 1e-3) / 1e9, 1) if per > 0 else None}
+    dom = max(kern.items(), key=lambda kv: kv[1][1])[0]
+    d = stages[dom]
+    traffic = None
+    tfile = ROOT / "profiles" / "traffic_c3.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get(dom)
+    roof = {"kernel": dom, "bound": "hbm", "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(d["achieved_GBs"] / HBM_PEAK_GBS, 4) if d["achieved_GBs"] else None,
+            "traffic": traffic, "avg_launch_us": d["avg_us"], "algo_bytes_per_launch": d["algo_bytes_per_launch"]}
+    algo_step = sum(roofline.kernel_bytes(k, counts) * v[0] for k, v in kern.items()) / max(ticks, 1)
+    step_roof = {"algo_bytes_per_tick": algo_step,
+                 "achieved_GBs_eager": round(algo_step / (total_ms / ticks * 1e-3) / 1e9, 1),
+                 "achieved_GBs_graph": round(algo_step / (elapsed / a.steps) / 1e9, 1)}
+    step_roof["frac_graph"] = round(step_roof["achieved_GBs_graph"] / HBM_PEAK_GBS, 4)
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(a.cells, a.cpu_steps)
+        cpu.pop("seconds", None)
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": elapsed / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32" if precision == ah.FP32_HOT else "f64", "data": "synthetic",
+        "config": {"workload": "plain method, scale: 1M cells, 4x4x0.2 mm, 512x512x32 FP32 T+CO2, 10 um hash, "
+                               "division + compaction on, dt 1e-4 s",
+                   "cells": a.cells, "cells_end": c1.count, "storage": a.precision,
+                   "arithmetic": "fp64 kinematics and energies, fp32 fields", "graph_replay": True,
+                   "divisions": c1.divisions_total - c0.divisions_total,
+                   "deaths": c1.deaths_total - c0.deaths_total},
+        "roofline": roof,
+        "step_roofline": step_roof,
+        "cpu_baseline": cpu,
+        "stages": stages,
+    }
+    w.close()
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -65,7 +65,8 @@ enum { ASTRO_DIST_CONSTANT = 0, ASTRO_DIST_UNIFORM = 1, ASTRO_DIST_NORMAL = 2,
 
 /* One grid's resolution. nz == 1 is the reference's depth-averaged 2D grid
  * (src/fields/grid.cuh:72-85); nz > 1 resolves the chamber depth with a
- * 7-point stencil. nx == 0 disables the field (N2 only). */
+ * 7-point stencil. All zero selects the reference's n x n x 1 for that field;
+ * nx < 0 disables the field (N2 only). */
 typedef struct astro_grid_dims {
     int32_t nx, ny, nz;
 } astro_grid_dims;
@@ -144,6 +145,7 @@ typedef struct astro_counters {
     int64_t  deaths_total;
     int32_t  dead_slots_prev;
     int32_t  _pad;
+    int64_t  cell_steps;         /* sum over ticks of the count each tick began with */
 } astro_counters;
 
 /* contract::Stats (contracts/telemetry_v1.h:17-39). */

@@ -503,6 +503,7 @@ struct oracle_world {
     int64_t  divisions_total = 0, deaths_total = 0;
     int64_t  divisions_reported = 0, deaths_reported = 0;
     int32_t  dead_slots_prev = 0;
+    int64_t  cell_steps = 0;
 
     Grid f[ASTRO_FIELD_COUNT];
 
@@ -1028,6 +1029,7 @@ int slow_fields(oracle_world* w, double dt) {                    // step.cu:168-
 // world_step (src/sim/step.cu:109-184).
 int world_step_once(oracle_world* w) {
     const double dt = ASTRO_DT_PHYSICS * w->physics_rate;
+    w->cell_steps += w->count;
     hash_build(w);
     const astro_world_desc& d = w->desc;
     int st;
@@ -1114,8 +1116,8 @@ int32_t oracle_world_create(const astro_world_desc* din, oracle_world** out) {
     for (int f = 0; f < 4; ++f) {
         astro_grid_dims& g = d.grid[f];
         if (g.nx == 0 && g.ny == 0 && g.nz == 0) { g.nx = g.ny = defaults[f]; g.nz = 1; }
-        if (f != ASTRO_FIELD_N2 && (g.nx <= 1 || g.ny <= 1 || g.nz < 1)) return ASTRO_INVALID_ARGUMENT;
-        if (f == ASTRO_FIELD_N2 && g.nx != 0 && (g.nx <= 1 || g.ny <= 1 || g.nz < 1)) return ASTRO_INVALID_ARGUMENT;
+        if (f == ASTRO_FIELD_N2 && g.nx < 0) continue;   // disabled
+        if (g.nx <= 1 || g.ny <= 1 || g.nz < 1) return ASTRO_INVALID_ARGUMENT;
     }
     auto* w = new oracle_world();
     w->desc = d;
@@ -1290,6 +1292,7 @@ int32_t oracle_world_get_counters(oracle_world* w, astro_counters* o) {
     o->count = w->count; o->capacity = w->capacity; o->next_id = w->next_id; o->tick = w->tick;
     o->sim_time_s = w->sim_time; o->divisions_total = w->divisions_total;
     o->deaths_total = w->deaths_total; o->dead_slots_prev = w->dead_slots_prev;
+    o->cell_steps = w->cell_steps;
     return ASTRO_OK;
 }
 
@@ -1299,6 +1302,7 @@ int32_t oracle_world_set_counters(oracle_world* w, const astro_counters* c) {
     w->count = c->count; w->next_id = c->next_id; w->tick = c->tick; w->sim_time = c->sim_time_s;
     w->divisions_total = c->divisions_total; w->deaths_total = c->deaths_total;
     w->dead_slots_prev = c->dead_slots_prev;
+    w->cell_steps = c->cell_steps;
     w->hash_valid = false;
     return ASTRO_OK;
 }

@@ -61,10 +61,10 @@ def gpu() -> ah.Binding:
     return b
 
 
-def oracle_cells(oracle_binding, n: int, seed: int, spawn: dict | None = None, **desc_kw) -> dict:
+def oracle_cells(oracle_binding, n: int, spawn_seed: int, spawn: dict | None = None, **desc_kw) -> dict:
     """Cells spawned by the oracle (reference spawn order), as host arrays."""
     with ah.World(oracle_binding, **desc_kw) as w:
-        w.spawn(n, seed, **(spawn or {}))
+        w.spawn(n, spawn_seed, **(spawn or {}))
         return w.download()
 
 

@@ -16,7 +16,7 @@ from conftest import ah, compare_cells, oracle_cells
 pytestmark = pytest.mark.gpu
 
 C1 = dict(chamber_w=550e-6, chamber_h=550e-6, chamber_d=100e-6, physics_rate=0.1,
-          grid={0: (64, 64, 64), 1: (64, 64, 64), 2: (0, 0, 0), 3: (64, 64, 64)})
+          grid={0: (64, 64, 64), 1: (64, 64, 64), 2: (-1, 0, 0), 3: (64, 64, 64)})
 REF2D = dict(grid={0: (128, 128, 1), 1: (64, 64, 1), 2: (32, 32, 1), 3: (128, 128, 1)})
 
 
@@ -64,7 +64,7 @@ def test_hash_parity(oracle, gpu, case):
         kw, n, spawn = dict(capacity=4000, seed=7), 4000, dict(placement=ah.PLACE_GAUSSIAN, place_radius=1.5e-4)
     else:
         kw = dict(capacity=1_000_000, chamber_w=4e-3, chamber_h=4e-3, chamber_d=2e-4, hash_cell_size=1e-5,
-                  grid={0: (64, 64, 1), 1: (32, 32, 1), 2: (0, 0, 0), 3: (64, 64, 1)})
+                  grid={0: (64, 64, 1), 1: (32, 32, 1), 2: (-1, 0, 0), 3: (64, 64, 1)})
         n, spawn = 1_000_000, {}
     cells = oracle_cells(oracle, n, 1, spawn=spawn, **kw)
     # a few unoccupied slots exercise the sentinel bucket
@@ -92,7 +92,7 @@ def test_hash_parity(oracle, gpu, case):
 @pytest.mark.parametrize("dims,bc_field", [((128, 128, 1), ah.TEMPERATURE), ((96, 80, 24), ah.TEMPERATURE),
                                            ((64, 64, 16), ah.CO2)])
 def test_ftcs_parity(oracle, gpu, dims, bc_field):
-    kw = dict(capacity=16, grid={0: dims, 1: dims if bc_field == ah.CO2 else (32, 32, 1), 2: (0, 0, 0),
+    kw = dict(capacity=16, grid={0: dims, 1: dims if bc_field == ah.CO2 else (32, 32, 1), 2: (-1, 0, 0),
                                  3: (32, 32, 1)})
     o, g = pair(oracle, gpu, **kw)
     rng = np.random.default_rng(5)
@@ -192,9 +192,13 @@ def test_lifecycle_division_and_compaction(oracle, gpu):
         assert co.count < 3000 + 1500
         a, b = o.download(), g.download()
         compare_cells(a, b)
-        tol = 0.0 if precision == ah.FP64_CONTRACT else 1e-12
-        for name in ("x", "y", "z", "energy", "co2_held"):
+        # FP32 storage rounds each daughter coordinate to a float: half an
+        # ulp at 2 mm is 1.2e-10 m.
+        tol = 0.0 if precision == ah.FP64_CONTRACT else 1.25e-10
+        for name in ("x", "y", "z"):
             rel_close(a[name], b[name], 1e-12, atol=tol, name=name)
+        for name in ("energy", "co2_held"):
+            rel_close(a[name], b[name], 1e-12, name=name)
         o.close()
         g.close()
 
