@@ -28,9 +28,22 @@ namespace astro_dev {
 
 // ---------------------------------------------------------------------------
 // Views passed by value to kernels
+//
+// Physical vs logical order. The reference's slot order (its SoA index) is the
+// spawn order, which is spatially random, so a kernel walking cells in slot
+// order gathers field voxels and neighbours at random. Here the SoA is kept
+// *physically* in spatial-hash order and every cell carries its reference
+// slot in `lslot`. Everything the reference can observe -- downloads, the
+// hash's vals_sorted, division slots and ids, compaction -- is computed in
+// logical-slot order; only the storage order differs. Physical entries whose
+// cell was compacted away are tombstones (lslot == NO_SLOT, flags == 0) until
+// the next re-sort packs them out.
 // ---------------------------------------------------------------------------
+constexpr uint32_t NO_SLOT = 0xFFFFFFFFu;
+
 template <class P>
 struct Cells {
+    uint32_t* lslot;     // reference slot of this physical entry, or NO_SLOT
     uint64_t* id;
     uint32_t* flags;
     uint8_t*  death_cause;
@@ -51,8 +64,12 @@ struct Cells {
 // Device-resident bookkeeping (src/sim/cell_store.cuh:36-59,
 // src/sim/world.cuh:111-121), so no tick has to read a count back.
 struct Counters {
-    int32_t  count;            // slots [0, count) may be occupied
-    int32_t  capacity;
+    int32_t  count;            // reference slots [0, count) may be occupied
+    int32_t  capacity;         // reference capacity; physical extent is 2x
+    int32_t  nphys;            // physical entries [0, nphys) in use (incl. tombstones)
+    int32_t  resort;           // 1 while this tick re-sorts the SoA
+    uint32_t tick_ctr;         // ticks run, for the re-sort cadence
+    int32_t  _pad1;
     uint64_t next_id;
     int64_t  divisions_total;
     int64_t  deaths_total;
@@ -322,11 +339,34 @@ __device__ __forceinline__ uint32_t hash_key(const HashGeo& h, double x, double 
     return static_cast<uint32_t>((gz * h.ny + gy) * h.nx + gx);
 }
 
+// Copies one cell's whole state (every SoA array, incl. lslot).
+template <class P>
+__device__ __forceinline__ void copy_cell(const Cells<P>& s, int i, const Cells<P>& d, int k) {
+    d.lslot[k] = s.lslot[i];
+    d.id[k] = s.id[i]; d.flags[k] = s.flags[i]; d.death_cause[k] = s.death_cause[i];
+    d.x[k] = s.x[i]; d.y[k] = s.y[i]; d.z[k] = s.z[i];
+    d.vx[k] = s.vx[i]; d.vy[k] = s.vy[i]; d.vz[k] = s.vz[i];
+    d.energy[k] = s.energy[i]; d.temp_cell[k] = s.temp_cell[i]; d.emit_power[k] = s.emit_power[i];
+    d.dir_x[k] = s.dir_x[i]; d.dir_y[k] = s.dir_y[i]; d.dir_z[k] = s.dir_z[i];
+    d.biomass[k] = s.biomass[i]; d.co2_held[k] = s.co2_held[i]; d.age_s[k] = s.age_s[i];
+    d.rng_state[k] = s.rng_state[i]; d.taxis_memory[k] = s.taxis_memory[i]; d.run_timer[k] = s.run_timer[i];
+    d.t_local[k] = s.t_local[i]; d.irradiance[k] = s.irradiance[i]; d.co2_local[k] = s.co2_local[i];
+    d.n2_local[k] = s.n2_local[i];
+}
+
 // ---------------------------------------------------------------------------
 // Launch helpers
 // ---------------------------------------------------------------------------
 constexpr int BLOCK = 256;
 constexpr int NUM_XCD = 8;
+// Per-cell kernels loop grid-stride over a grid of at most MAX_BLOCKS (8
+// resident blocks per CU), so a grid sized for the capacity costs nothing
+// when the population is smaller and conditional passes exit cheaply.
+constexpr int MAX_BLOCKS = 2048;
+__host__ __device__ __forceinline__ int stride_grid(int n) {
+    const int b = (n + BLOCK - 1) / BLOCK;
+    return b < 1 ? 1 : (b < MAX_BLOCKS ? b : MAX_BLOCKS);
+}
 
 // Remaps a linear block id so each XCD (blocks are dealt to the 8 XCDs round
 // robin) gets one contiguous run of tiles, which keeps neighbouring tiles in

@@ -26,31 +26,40 @@ struct Launch {
 };
 
 // Everything the device side of one world needs, by value.
+// Sizes: `cap` is the reference capacity (logical slots, sorted positions);
+// `pcap` = 2 x cap is the physical extent, room for a tick in which every
+// cell divides on top of the tombstones left since the last re-sort.
 template <class P>
 struct WorldDev {
-    Cells<P> c;          // the live SoA (slot order)
-    Cells<P> alt;        // out-of-place compaction target
+    Cells<P> c;          // the live SoA, physical order (pcap entries)
+    Cells<P> alt;        // re-sort target (pcap entries)
     Counters* ctr;
     int32_t capacity;
+    int32_t pcap;
+    int32_t resort_period;  // re-sort at least every this many ticks
 
     Grid g[ASTRO_FIELD_COUNT];
     bool on[ASTRO_FIELD_COUNT];
 
     HashGeo hg;
-    uint32_t* keys;         // [cap] bucket of each slot
-    uint32_t* rin;          // [cap] arrival rank inside the bucket; reused as
-                            //       the CO2 deposit index during lifecycle
+    uint32_t* keys;         // [pcap] bucket of each physical entry
+    uint32_t* rin;          // [pcap] arrival rank inside the bucket; reused as
+                            //        the CO2 voxel index during lifecycle
     uint32_t* bcount;       // [nb + 1] bucket populations, zero between builds
     int32_t*  bstart;       // [nb + 2] dense exclusive prefix of bcount
     Rec<P>*   trec;         // [cap] records in arrival order
+    uint32_t* tphys;        // [cap] physical index of each arrival-order record
     Rec<P>*   rec;          // [cap] records in stable bucket order
-    uint32_t* vals_sorted;  // [cap] slot of each sorted record
-    int32_t*  spos;         // [cap] sorted position of each slot
+    uint32_t* vals_sorted;  // [cap] reference slot of each sorted record
+    uint32_t* sphys;        // [cap] physical index of each sorted record
+    int32_t*  spos;         // [pcap] sorted position of each physical entry
     Force*    fs;           // [cap] force in sorted order
     double*   dirs;         // [cap] directional irradiance in sorted order
 
-    int32_t*  dflag;        // [cap] divides / keep predicate
+    int32_t*  dflag;        // [cap] divides flag by reference slot
     int32_t*  prefix;       // [cap] exclusive prefix of dflag
+    int32_t*  drop;         // [cap] compaction drop flag by reference slot
+    int32_t*  dprefix;      // [cap] exclusive prefix of drop
     unsigned long long* co2_demand;   // [CO2 grid] fixed-point demand
     int32_t*  scan_total;   // scratch total of the last scan
     ScanCtl   sc;
@@ -90,6 +99,8 @@ struct Ftcs {
 template <class P> void launch_hash_build(const Launch& L, const WorldDev<P>& w, bool count_tick);
 template <class P> void launch_forces_walk(const Launch& L, const WorldDev<P>& w, const TickParams& t);
 template <class P> void launch_feed_walk(const Launch& L, const WorldDev<P>& w, const TickParams& t);
+// Re-sorts the SoA into bucket order when due (device-side decision).
+template <class P> void launch_resort(const Launch& L, const WorldDev<P>& w);
 
 // ---- cells (kernels_cells.hip) ----
 struct SpawnArgs {
@@ -98,7 +109,7 @@ struct SpawnArgs {
     double chamber_w, chamber_h, chamber_d;
 };
 template <class P> void launch_spawn(const Launch& L, const WorldDev<P>& w, const SpawnArgs& a,
-                                     int32_t base, uint64_t first_id, uint64_t seed);
+                                     int32_t base, int32_t lbase, uint64_t first_id, uint64_t seed);
 template <class P> void launch_thermal_substep(const Launch& L, const WorldDev<P>& w, double dt_sub,
                                                double cap, bool update_rec);
 template <class P> void launch_apply_awake(const Launch& L, const WorldDev<P>& w, int field);
@@ -108,6 +119,8 @@ template <class P> void launch_sample_base(const Launch& L, const WorldDev<P>& w
 template <class P> void launch_integrate(const Launch& L, const WorldDev<P>& w, const TickParams& t);
 template <class P> void launch_lifecycle(const Launch& L, const WorldDev<P>& w, const TickParams& t);
 template <class P> void launch_compact(const Launch& L, const WorldDev<P>& w, bool forced);
+// Clears the per-slot lifecycle flags (the tick does this in its key pass).
+template <class P> void launch_life_clear(const Launch& L, const WorldDev<P>& w);
 template <class P> void launch_stats_cells(const Launch& L, const WorldDev<P>& w);
 
 // ---- fields (kernels_fields.hip) ----

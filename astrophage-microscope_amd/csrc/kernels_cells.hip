@@ -21,11 +21,12 @@ __device__ __forceinline__ void wave_count_add(int32_t* dst, bool pred) {
 // spawn_kernel (src/sim/cell_store.cu:42-130); LOG_UNIFORM charge added.
 // ---------------------------------------------------------------------------
 template <class P>
-__global__ __launch_bounds__(BLOCK) void k_spawn(Cells<P> c, SpawnArgs p, int32_t base, uint64_t first_id,
-                                                 uint64_t seed) {
+__global__ __launch_bounds__(BLOCK) void k_spawn(Cells<P> c, SpawnArgs p, int32_t base, int32_t lbase,
+                                                 uint64_t first_id, uint64_t seed) {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= p.count) return;
     const int32_t slot = base + i;
+    c.lslot[slot] = static_cast<uint32_t>(lbase + i);
     const uint64_t id = first_id + static_cast<uint64_t>(i);
     Pcg r = cell_rng(cell_rng_init(seed, id), id);
     double x = 0.0, y = 0.0;
@@ -113,9 +114,8 @@ __global__ __launch_bounds__(BLOCK) void k_spawn(Cells<P> c, SpawnArgs p, int32_
 template <class P>
 __global__ __launch_bounds__(BLOCK) void k_thermal(Cells<P> c, const Counters* ctr, Grid T, double dt_sub,
                                                    double cap, Rec<P>* rec, const int32_t* spos, int update_rec) {
-    const int n = ctr->count;
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
+    const int n = ctr->nphys;
+    const auto body = [&](const int i) {
     const uint32_t fl = c.flags[i];
     if (!(fl & ASTRO_FLAG_OCCUPIED)) return;
     const uint32_t idx = grid_nearest(T, static_cast<float>(c.x[i]), static_cast<float>(c.y[i]),
@@ -150,6 +150,8 @@ __global__ __launch_bounds__(BLOCK) void k_thermal(Cells<P> c, const Counters* c
         c.emit_power[i] = 0.0f;
         if (update_rec) rec[spos[i]].meta &= ~REC_ALIVE;
     }
+    };
+    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) body(i);
 }
 
 // Folds a field's fixed-point accumulator into its values at the cells that
@@ -160,15 +162,16 @@ __global__ __launch_bounds__(BLOCK) void k_thermal(Cells<P> c, const Counters* c
 // thermostat depositor (thermal.cu:130), dead or alive afterwards.
 template <class P>
 __global__ __launch_bounds__(BLOCK) void k_apply_awake(Cells<P> c, const Counters* ctr, Grid g) {
-    const int n = ctr->count;
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
+    const int n = ctr->nphys;
+    const auto body = [&](const int i) {
     const uint32_t fl = c.flags[i];
     if (!(fl & ASTRO_FLAG_OCCUPIED) || !(fl & ASTRO_FLAG_AWAKE)) return;
     const uint32_t idx = grid_nearest(g, static_cast<float>(c.x[i]), static_cast<float>(c.y[i]),
                                       static_cast<float>(c.z[i]));
     const unsigned long long raw = atomicExch(&g.deposit[idx], 0ull);
     if (raw != 0ull) g.value[idx] += static_cast<float>(from_fixed(raw, g.scale));
+    };
+    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) body(i);
 }
 
 // ---------------------------------------------------------------------------
@@ -195,9 +198,8 @@ __device__ __forceinline__ void any_perpendicular(double hx, double hy, double h
 template <class P, bool FEED, bool TAXIS>
 __global__ __launch_bounds__(BLOCK) void k_feed_taxis(Cells<P> c, const Counters* ctr, const int32_t* spos,
                                                       const double* dirs, float ambient, double dt, double max_power) {
-    const int n = ctr->count;
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
+    const int n = ctr->nphys;
+    const auto body = [&](const int i) {
     const uint32_t f = c.flags[i];
     if (!(f & ASTRO_FLAG_OCCUPIED) || !(f & ASTRO_FLAG_ALIVE)) return;
     double energy = c.energy[i];
@@ -296,6 +298,8 @@ __global__ __launch_bounds__(BLOCK) void k_feed_taxis(Cells<P> c, const Counters
         c.emit_power[i] = static_cast<float>(p);
         c.energy[i] = energy - p * dt;
     }
+    };
+    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) body(i);
 }
 
 // ---------------------------------------------------------------------------
@@ -306,9 +310,9 @@ __global__ __launch_bounds__(BLOCK) void k_feed_taxis(Cells<P> c, const Counters
 template <class P>
 __global__ __launch_bounds__(BLOCK) void k_sample_base(Cells<P> c, const Counters* ctr, Grid T, Grid C, Grid N,
                                                        int c_on, int n_on, int gaxis, const int32_t* spos, Force* fs) {
-    const int n = ctr->count;
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
+    const int n = ctr->nphys;
+    const auto body = [&](const int i) {
+    if (c.lslot[i] == NO_SLOT) return;
     const uint32_t fl = c.flags[i];
     Force F{0.0, 0.0, 0.0, 0.0};
     if (fl & ASTRO_FLAG_OCCUPIED) {
@@ -332,6 +336,8 @@ __global__ __launch_bounds__(BLOCK) void k_sample_base(Cells<P> c, const Counter
         }
     }
     fs[spos[i]] = F;
+    };
+    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) body(i);
 }
 
 // ---------------------------------------------------------------------------
@@ -340,9 +346,8 @@ __global__ __launch_bounds__(BLOCK) void k_sample_base(Cells<P> c, const Counter
 template <class P>
 __global__ __launch_bounds__(BLOCK) void k_integrate(Cells<P> c, const Counters* ctr, const int32_t* spos,
                                                      const Force* fs, TickParams t) {
-    const int n = ctr->count;
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
+    const int n = ctr->nphys;
+    const auto body = [&](const int i) {
     const uint32_t fl = c.flags[i];
     if (!(fl & ASTRO_FLAG_OCCUPIED)) return;
     const double dt = t.dt;
@@ -398,6 +403,8 @@ __global__ __launch_bounds__(BLOCK) void k_integrate(Cells<P> c, const Counters*
     c.vy[i] = static_cast<P>(vy);
     c.vz[i] = static_cast<P>(vz);
     c.age_s[i] += static_cast<float>(dt);
+    };
+    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) body(i);
 }
 
 // ---------------------------------------------------------------------------
@@ -411,9 +418,8 @@ constexpr uint32_t NO_VOXEL = 0xFFFFFFFFu;
 template <class P>
 __global__ __launch_bounds__(BLOCK) void k_demand(Cells<P> c, const Counters* ctr, Grid C, double dt_bio, double gv,
                                                   unsigned long long* demand, uint32_t* vox) {
-    const int n = ctr->count;
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
+    const int n = ctr->nphys;
+    const auto body = [&](const int i) {
     vox[i] = NO_VOXEL;
     const uint32_t f = c.flags[i];
     if (!(f & ASTRO_FLAG_OCCUPIED) || !(f & ASTRO_FLAG_ALIVE)) return;
@@ -423,15 +429,16 @@ __global__ __launch_bounds__(BLOCK) void k_demand(Cells<P> c, const Counters* ct
     if (want <= 0.0) return;
     deposit_add(&demand[idx], want / gv, C.scale);
     vox[i] = idx;
+    };
+    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) body(i);
 }
 
 // uptake_kernel (lifecycle.cu:62-86).
 template <class P>
 __global__ __launch_bounds__(BLOCK) void k_uptake(Cells<P> c, const Counters* ctr, Grid C, double dt_bio, double gv,
                                                   const unsigned long long* demand, const uint32_t* vox) {
-    const int n = ctr->count;
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
+    const int n = ctr->nphys;
+    const auto body = [&](const int i) {
     const uint32_t idx = vox[i];
     if (idx == NO_VOXEL) return;
     const double available = static_cast<double>(C.value[idx]);
@@ -443,6 +450,8 @@ __global__ __launch_bounds__(BLOCK) void k_uptake(Cells<P> c, const Counters* ct
     if (taken <= 0.0) return;
     c.co2_held[i] += taken;
     deposit_add(&C.deposit[idx], -taken / gv, C.scale);
+    };
+    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) body(i);
 }
 
 // The CO2 flush (lifecycle.cu:235), death_kernel (:100-112) and mark_kernel
@@ -451,9 +460,10 @@ __global__ __launch_bounds__(BLOCK) void k_uptake(Cells<P> c, const Counters* ct
 template <class P>
 __global__ __launch_bounds__(BLOCK) void k_life_mark(Cells<P> c, Counters* ctr, Grid C, int c_on,
                                                      unsigned long long* demand, const uint32_t* vox,
-                                                     int disposition, double quota, int32_t* dflag) {
-    const int n = ctr->count;
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
+                                                     int disposition, double quota, int32_t* dflag, int32_t* drop) {
+    const int n = ctr->nphys;
+    for (int base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
+    const int i = base + threadIdx.x;
     bool divides = false, dead = false;
     if (i < n) {
         if (c_on) {
@@ -476,10 +486,15 @@ __global__ __launch_bounds__(BLOCK) void k_life_mark(Cells<P> c, Counters* ctr, 
         const bool occ = (f & ASTRO_FLAG_OCCUPIED) != 0u, alive = (f & ASTRO_FLAG_ALIVE) != 0u;
         divides = occ && alive && c.co2_held[i] >= quota;
         dead = occ && !alive;
-        dflag[i] = divides ? 1 : 0;
+        const uint32_t s = c.lslot[i];
+        if (s != NO_SLOT) {
+            if (divides) dflag[s] = 1;          // cleared at the start of the tick
+            if (!(occ && alive)) drop[s] = 1;   // compaction keeps occupied && alive
+        }
     }
     wave_count_add(&ctr->births, divides);
     wave_count_add(&ctr->deads, dead);
+    }
 }
 
 __global__ __launch_bounds__(BLOCK) void k_divide_scan(Counters* ctr, const int32_t* dflag, int32_t* prefix,
@@ -491,21 +506,25 @@ __global__ __launch_bounds__(BLOCK) void k_divide_scan(Counters* ctr, const int3
         total);
 }
 
-// divide_kernel (lifecycle.cu:140-204). Daughters go to slot count + prefix,
-// ids next_id + prefix; both read before the finalize moves them.
+// divide_kernel (lifecycle.cu:140-204). The daughter's reference slot is
+// count + prefix (the prefix taken over reference-slot order) and its id
+// next_id + prefix; physically it is appended after the last entry.
 template <class P>
 __global__ __launch_bounds__(BLOCK) void k_divide(Cells<P> c, const Counters* ctr, const int32_t* prefix,
                                                   TickParams t) {
     if (ctr->births <= 0) return;
+    const int np = ctr->nphys;
     const int n = ctr->count;
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
+    const auto body = [&](const int i) {
     const uint32_t f = c.flags[i];
     if (!(f & ASTRO_FLAG_OCCUPIED) || !(f & ASTRO_FLAG_ALIVE)) return;
     if (!(c.co2_held[i] >= t.quota)) return;
-    const int32_t slot = n + prefix[i];
-    if (slot >= ctr->capacity) return;
-    const uint64_t did = ctr->next_id + static_cast<uint64_t>(prefix[i]);
+    const int32_t pre = prefix[c.lslot[i]];
+    const int32_t lslot = n + pre;
+    if (lslot >= ctr->capacity) return;
+    const int32_t slot = np + pre;
+    c.lslot[slot] = static_cast<uint32_t>(lslot);
+    const uint64_t did = ctr->next_id + static_cast<uint64_t>(pre);
     const double half0 = 0.5 * (c.energy[i] - ASTRO_LIFE_DIVISION_ENERGY_COST);
     const double half = half0 > 0.0 ? half0 : 0.0;
     c.energy[i] = half;
@@ -552,6 +571,8 @@ __global__ __launch_bounds__(BLOCK) void k_divide(Cells<P> c, const Counters* ct
     c.irradiance[slot] = 0.0f;
     c.co2_local[slot] = c.co2_local[i];
     c.n2_local[slot] = c.n2_local[i];
+    };
+    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < np; i += gridDim.x * BLOCK) body(i);
 }
 
 // The host bookkeeping of lifecycle_step (lifecycle.cu:251-294), on device.
@@ -567,6 +588,7 @@ __global__ void k_life_finalize(Counters* ctr, int compaction) {
             ctr->divisions_total += births;
             ctr->next_id += static_cast<uint64_t>(births);
             ctr->count = n + births;
+            ctr->nphys += births;
         }
     }
     const bool compact = compaction && deads > 0;
@@ -577,72 +599,63 @@ __global__ void k_life_finalize(Counters* ctr, int compaction) {
 }
 
 // ---------------------------------------------------------------------------
-// cell_store_compact (src/sim/cell_store.cu:414-456): keep = occupied && alive,
-// exclusive scan, one fused gather of every array into the alternate SoA, one
-// fused copy back. All kernels return at once unless `compacting` is set.
+// cell_store_compact (src/sim/cell_store.cu:414-456): survivors (occupied &&
+// alive) keep their order and are renumbered 0..live-1 by an exclusive scan
+// of the drop flags over reference-slot order; dropped entries become
+// tombstones. Nothing is copied: storage order is the spatial sort order, and
+// the next re-sort packs the tombstones out. All kernels return at once
+// unless `compacting` is set.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BLOCK) void k_compact_scan(Counters* ctr, const uint32_t* flags, int32_t* prefix,
-                                                        ScanCtl sc) {
+__global__ __launch_bounds__(BLOCK) void k_compact_scan(Counters* ctr, const int32_t* drop, int32_t* dprefix,
+                                                        int32_t* total, ScanCtl sc) {
     const int n = ctr->compacting ? ctr->count : 0;
     if (n == 0) return;
-    auto keep = [](uint32_t f) { return ((f & ASTRO_FLAG_OCCUPIED) && (f & ASTRO_FLAG_ALIVE)) ? 1 : 0; };
     scan_exclusive(
-        n,
-        [&](int i) {
-            const int4 f = load4_i32(reinterpret_cast<const int32_t*>(flags), i, n);
-            int4 v;
-            v.x = i < n ? keep(static_cast<uint32_t>(f.x)) : 0;
-            v.y = i + 1 < n ? keep(static_cast<uint32_t>(f.y)) : 0;
-            v.z = i + 2 < n ? keep(static_cast<uint32_t>(f.z)) : 0;
-            v.w = i + 3 < n ? keep(static_cast<uint32_t>(f.w)) : 0;
-            return v;
-        },
-        [&](int i, int4 ex) { store4_i32(prefix, i, n, ex); }, sc, &ctr->live);
+        n, [&](int i) { return load4_i32(drop, i, n); }, [&](int i, int4 ex) { store4_i32(dprefix, i, n, ex); }, sc,
+        total);
 }
 
 template <class P>
-__device__ __forceinline__ void copy_cell(const Cells<P>& s, int i, const Cells<P>& d, int k) {
-    d.id[k] = s.id[i]; d.flags[k] = s.flags[i]; d.death_cause[k] = s.death_cause[i];
-    d.x[k] = s.x[i]; d.y[k] = s.y[i]; d.z[k] = s.z[i];
-    d.vx[k] = s.vx[i]; d.vy[k] = s.vy[i]; d.vz[k] = s.vz[i];
-    d.energy[k] = s.energy[i]; d.temp_cell[k] = s.temp_cell[i]; d.emit_power[k] = s.emit_power[i];
-    d.dir_x[k] = s.dir_x[i]; d.dir_y[k] = s.dir_y[i]; d.dir_z[k] = s.dir_z[i];
-    d.biomass[k] = s.biomass[i]; d.co2_held[k] = s.co2_held[i]; d.age_s[k] = s.age_s[i];
-    d.rng_state[k] = s.rng_state[i]; d.taxis_memory[k] = s.taxis_memory[i]; d.run_timer[k] = s.run_timer[i];
-    d.t_local[k] = s.t_local[i]; d.irradiance[k] = s.irradiance[i]; d.co2_local[k] = s.co2_local[i];
-    d.n2_local[k] = s.n2_local[i];
-}
-
-template <class P>
-__global__ __launch_bounds__(BLOCK) void k_compact_gather(Cells<P> a, Cells<P> b, const Counters* ctr,
-                                                          const int32_t* prefix) {
+__global__ __launch_bounds__(BLOCK) void k_compact_renumber(Cells<P> c, const Counters* ctr, const int32_t* drop,
+                                                            const int32_t* dprefix) {
     if (!ctr->compacting) return;
-    const int n = ctr->count;
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t f = a.flags[i];
-    if (!((f & ASTRO_FLAG_OCCUPIED) && (f & ASTRO_FLAG_ALIVE))) return;
-    copy_cell(a, i, b, prefix[i]);
+    const int np = ctr->nphys;
+    const auto body = [&](const int i) {
+    const uint32_t s = c.lslot[i];
+    if (s == NO_SLOT) return;
+    if (drop[s]) {
+        c.lslot[i] = NO_SLOT;
+        c.flags[i] = 0u;
+    } else {
+        c.lslot[i] = s - static_cast<uint32_t>(dprefix[s]);
+    }
+    };
+    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < np; i += gridDim.x * BLOCK) body(i);
 }
 
-template <class P>
-__global__ __launch_bounds__(BLOCK) void k_compact_copyback(Cells<P> a, Cells<P> b, const Counters* ctr) {
-    if (!ctr->compacting) return;
-    const int k = blockIdx.x * BLOCK + threadIdx.x;
-    if (k >= ctr->live) return;
-    copy_cell(b, k, a, k);
-}
-
-__global__ void k_compact_finalize(Counters* ctr) {
+__global__ void k_compact_finalize(Counters* ctr, const int32_t* total) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     if (ctr->compacting) {
-        ctr->count = ctr->live;
+        ctr->count -= *total;
+        ctr->live = ctr->count;
         ctr->compacting = 0;
     }
 }
 
-__global__ void k_compact_force(Counters* ctr) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) ctr->compacting = 1;
+// A direct compaction request (cell_store_compact outside lifecycle): mark
+// the drop set afresh -- every live entry writes its own slot's flag -- and
+// run the same three passes.
+template <class P>
+__global__ __launch_bounds__(BLOCK) void k_compact_mark(Cells<P> c, Counters* ctr, int32_t* drop) {
+    const int np = ctr->nphys;
+    const int gid = blockIdx.x * BLOCK + threadIdx.x;
+    if (gid == 0) ctr->compacting = 1;
+    for (int i = gid; i < np; i += gridDim.x * BLOCK) {
+        const uint32_t s = c.lslot[i];
+        if (s == NO_SLOT) continue;
+        const uint32_t f = c.flags[i];
+        drop[s] = ((f & ASTRO_FLAG_OCCUPIED) && (f & ASTRO_FLAG_ALIVE)) ? 0 : 1;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -657,8 +670,9 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 
 template <class P>
 __global__ __launch_bounds__(BLOCK) void k_stats_cells(Cells<P> c, const Counters* ctr, unsigned long long* acc) {
-    const int n = ctr->count;
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    const int n = ctr->nphys;
+    for (int base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
+    const int i = base + threadIdx.x;
     unsigned long long e = 0, tc = 0, na = 0, nd = 0, nw = 0, nm = 0;
     if (i < n) {
         const uint32_t f = c.flags[i];
@@ -679,6 +693,7 @@ __global__ __launch_bounds__(BLOCK) void k_stats_cells(Cells<P> c, const Counter
         if (nw) atomicAdd(&acc[SW_N_AWAKE], nw);
         if (nm) atomicAdd(&acc[SW_N_DORMANT], nm);
     }
+    }
 }
 
 }  // namespace
@@ -692,17 +707,17 @@ __global__ __launch_bounds__(BLOCK) void k_stats_cells(Cells<P> c, const Counter
     if (L.prof) L.prof->end()
 
 template <class P>
-void launch_spawn(const Launch& L, const WorldDev<P>& w, const SpawnArgs& a, int32_t base, uint64_t first_id,
-                  uint64_t seed) {
+void launch_spawn(const Launch& L, const WorldDev<P>& w, const SpawnArgs& a, int32_t base, int32_t lbase,
+                  uint64_t first_id, uint64_t seed) {
     PROF_BEGIN("spawn");
-    k_spawn<P><<<grid_for(a.count), BLOCK, 0, L.stream>>>(w.c, a, base, first_id, seed);
+    k_spawn<P><<<grid_for(a.count), BLOCK, 0, L.stream>>>(w.c, a, base, lbase, first_id, seed);
     PROF_END();
 }
 
 template <class P>
 void launch_thermal_substep(const Launch& L, const WorldDev<P>& w, double dt_sub, double cap, bool update_rec) {
     PROF_BEGIN("thermal");
-    k_thermal<P><<<grid_for(w.capacity), BLOCK, 0, L.stream>>>(w.c, w.ctr, w.g[ASTRO_FIELD_TEMPERATURE], dt_sub, cap,
+    k_thermal<P><<<stride_grid(w.pcap), BLOCK, 0, L.stream>>>(w.c, w.ctr, w.g[ASTRO_FIELD_TEMPERATURE], dt_sub, cap,
                                                                w.rec, w.spos, update_rec ? 1 : 0);
     PROF_END();
 }
@@ -710,13 +725,13 @@ void launch_thermal_substep(const Launch& L, const WorldDev<P>& w, double dt_sub
 template <class P>
 void launch_apply_awake(const Launch& L, const WorldDev<P>& w, int field) {
     PROF_BEGIN("deposit_apply");
-    k_apply_awake<P><<<grid_for(w.capacity), BLOCK, 0, L.stream>>>(w.c, w.ctr, w.g[field]);
+    k_apply_awake<P><<<stride_grid(w.pcap), BLOCK, 0, L.stream>>>(w.c, w.ctr, w.g[field]);
     PROF_END();
 }
 
 template <class P>
 void launch_feed_taxis(const Launch& L, const WorldDev<P>& w, const TickParams& t, bool feed, bool taxis) {
-    const int g = grid_for(w.capacity);
+    const int g = stride_grid(w.pcap);
     PROF_BEGIN(feed ? (taxis ? "feed_taxis" : "feed") : "taxis");
     if (feed && taxis)
         k_feed_taxis<P, true, true><<<g, BLOCK, 0, L.stream>>>(w.c, w.ctr, w.spos, w.dirs, t.ambient_irr, t.dt,
@@ -733,7 +748,7 @@ void launch_feed_taxis(const Launch& L, const WorldDev<P>& w, const TickParams& 
 template <class P>
 void launch_sample_base(const Launch& L, const WorldDev<P>& w, const TickParams& t) {
     PROF_BEGIN("sample_base");
-    k_sample_base<P><<<grid_for(w.capacity), BLOCK, 0, L.stream>>>(
+    k_sample_base<P><<<stride_grid(w.pcap), BLOCK, 0, L.stream>>>(
         w.c, w.ctr, w.g[ASTRO_FIELD_TEMPERATURE], w.g[ASTRO_FIELD_CO2], w.g[ASTRO_FIELD_N2],
         w.on[ASTRO_FIELD_CO2] ? 1 : 0, w.on[ASTRO_FIELD_N2] ? 1 : 0, t.gravity_axis, w.spos, w.fs);
     PROF_END();
@@ -742,13 +757,13 @@ void launch_sample_base(const Launch& L, const WorldDev<P>& w, const TickParams&
 template <class P>
 void launch_integrate(const Launch& L, const WorldDev<P>& w, const TickParams& t) {
     PROF_BEGIN("integrate");
-    k_integrate<P><<<grid_for(w.capacity), BLOCK, 0, L.stream>>>(w.c, w.ctr, w.spos, w.fs, t);
+    k_integrate<P><<<stride_grid(w.pcap), BLOCK, 0, L.stream>>>(w.c, w.ctr, w.spos, w.fs, t);
     PROF_END();
 }
 
 template <class P>
 void launch_lifecycle(const Launch& L, const WorldDev<P>& w, const TickParams& t) {
-    const int g = grid_for(w.capacity);
+    const int g = stride_grid(w.pcap);
     const bool c_on = w.on[ASTRO_FIELD_CO2];
     if (c_on) {
         PROF_BEGIN("co2_demand");
@@ -762,7 +777,7 @@ void launch_lifecycle(const Launch& L, const WorldDev<P>& w, const TickParams& t
     }
     PROF_BEGIN("life_mark");
     k_life_mark<P><<<g, BLOCK, 0, L.stream>>>(w.c, w.ctr, w.g[ASTRO_FIELD_CO2], c_on ? 1 : 0, w.co2_demand, w.rin,
-                                               t.store_disposition, t.quota, w.dflag);
+                                               t.store_disposition, t.quota, w.dflag, w.drop);
     PROF_END();
     PROF_BEGIN("divide_scan");
     k_divide_scan<<<scan_tiles_for(w.capacity), BLOCK, 0, L.stream>>>(w.ctr, w.dflag, w.prefix, w.scan_total, w.sc);
@@ -777,31 +792,38 @@ void launch_lifecycle(const Launch& L, const WorldDev<P>& w, const TickParams& t
 }
 
 template <class P>
+void launch_life_clear(const Launch& L, const WorldDev<P>& w) {
+    hipMemsetAsync(w.dflag, 0, sizeof(int32_t) * w.capacity, L.stream);
+    hipMemsetAsync(w.drop, 0, sizeof(int32_t) * w.capacity, L.stream);
+}
+
+template <class P>
 void launch_compact(const Launch& L, const WorldDev<P>& w, bool forced) {
-    const int g = grid_for(w.capacity);
-    if (forced) k_compact_force<<<1, 64, 0, L.stream>>>(w.ctr);
+    const int g = stride_grid(w.pcap);
+    if (forced) {
+        PROF_BEGIN("compact_mark");
+        k_compact_mark<P><<<g, BLOCK, 0, L.stream>>>(w.c, w.ctr, w.drop);
+        PROF_END();
+    }
     PROF_BEGIN("compact_scan");
-    k_compact_scan<<<scan_tiles_for(w.capacity), BLOCK, 0, L.stream>>>(w.ctr, w.c.flags, w.prefix, w.sc);
+    k_compact_scan<<<scan_tiles_for(w.capacity), BLOCK, 0, L.stream>>>(w.ctr, w.drop, w.dprefix, w.scan_total, w.sc);
     PROF_END();
-    PROF_BEGIN("compact_gather");
-    k_compact_gather<P><<<g, BLOCK, 0, L.stream>>>(w.c, w.alt, w.ctr, w.prefix);
-    PROF_END();
-    PROF_BEGIN("compact_copyback");
-    k_compact_copyback<P><<<g, BLOCK, 0, L.stream>>>(w.c, w.alt, w.ctr);
+    PROF_BEGIN("compact_renumber");
+    k_compact_renumber<P><<<g, BLOCK, 0, L.stream>>>(w.c, w.ctr, w.drop, w.dprefix);
     PROF_END();
     PROF_BEGIN("compact_finalize");
-    k_compact_finalize<<<1, 64, 0, L.stream>>>(w.ctr);
+    k_compact_finalize<<<1, 64, 0, L.stream>>>(w.ctr, w.scan_total);
     PROF_END();
 }
 
 template <class P>
 void launch_stats_cells(const Launch& L, const WorldDev<P>& w) {
-    k_stats_cells<P><<<grid_for(w.capacity), BLOCK, 0, L.stream>>>(w.c, w.ctr, w.stats);
+    k_stats_cells<P><<<stride_grid(w.pcap), BLOCK, 0, L.stream>>>(w.c, w.ctr, w.stats);
 }
 
 #define INSTANTIATE(P)                                                                                          \
-    template void launch_spawn<P>(const Launch&, const WorldDev<P>&, const SpawnArgs&, int32_t, uint64_t,      \
-                                  uint64_t);                                                                    \
+    template void launch_spawn<P>(const Launch&, const WorldDev<P>&, const SpawnArgs&, int32_t, int32_t,       \
+                                  uint64_t, uint64_t);                                                          \
     template void launch_thermal_substep<P>(const Launch&, const WorldDev<P>&, double, double, bool);          \
     template void launch_apply_awake<P>(const Launch&, const WorldDev<P>&, int);                                \
     template void launch_feed_taxis<P>(const Launch&, const WorldDev<P>&, const TickParams&, bool, bool);       \
@@ -809,6 +831,7 @@ void launch_stats_cells(const Launch& L, const WorldDev<P>& w) {
     template void launch_integrate<P>(const Launch&, const WorldDev<P>&, const TickParams&);                    \
     template void launch_lifecycle<P>(const Launch&, const WorldDev<P>&, const TickParams&);                    \
     template void launch_compact<P>(const Launch&, const WorldDev<P>&, bool);                                    \
+    template void launch_life_clear<P>(const Launch&, const WorldDev<P>&);                                      \
     template void launch_stats_cells<P>(const Launch&, const WorldDev<P>&);
 
 INSTANTIATE(double)

@@ -76,9 +76,8 @@ __global__ __launch_bounds__(BLOCK) void k_ftcs3d(const float* __restrict__ src,
 // occlusion_stamp_kernel (src/sim/emission.cu:17-27).
 template <class P>
 __global__ __launch_bounds__(BLOCK) void k_stamp(Cells<P> c, const Counters* ctr, Grid irr) {
-    const int n = ctr->count;
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
+    const int n = ctr->nphys;
+    const auto body = [&](const int i) {
     const uint32_t f = c.flags[i];
     if (!(f & ASTRO_FLAG_OCCUPIED)) return;
     const double area = (f & ASTRO_FLAG_ALIVE) ? ASTRO_CELL_CROSS_SECTION
@@ -86,6 +85,8 @@ __global__ __launch_bounds__(BLOCK) void k_stamp(Cells<P> c, const Counters* ctr
     const uint32_t idx = grid_nearest(irr, static_cast<float>(c.x[i]), static_cast<float>(c.y[i]),
                                       static_cast<float>(c.z[i]));
     deposit_add(&irr.deposit[idx], area, irr.scale);
+    };
+    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) body(i);
 }
 
 // sweep_kernel (src/sim/emission.cu:37-53): one sequential Beer-Lambert march
@@ -181,7 +182,7 @@ void launch_ftcs(const Launch& L, const Grid& g, const Ftcs& f, const char* name
 template <class P>
 void launch_stamp(const Launch& L, const WorldDev<P>& w) {
     if (L.prof) L.prof->begin("occlusion_stamp");
-    k_stamp<P><<<(w.capacity + BLOCK - 1) / BLOCK, BLOCK, 0, L.stream>>>(w.c, w.ctr, w.g[ASTRO_FIELD_IRRADIANCE]);
+    k_stamp<P><<<stride_grid(w.pcap), BLOCK, 0, L.stream>>>(w.c, w.ctr, w.g[ASTRO_FIELD_IRRADIANCE]);
     if (L.prof) L.prof->end();
 }
 

@@ -13,7 +13,7 @@
 //   * Tile status words carry a 32-bit generation ("epoch"). The last tile
 //     advances the epoch and the base, so nothing is cleared between scans:
 //     no memset, and the scan can sit inside a captured graph.
-//   * Look-back is done by one wave over a 64-tile window per probe.
+//   * Look-back is done by one wave over a 512-tile window per probe.
 //   * The spin on predecessors is bounded; on expiry it raises ERR_SCAN_SPIN
 //     in Counters::error_flags (checked by the host) instead of hanging.
 //   * Sizes come from device memory, so the grid is sized for the maximum and
@@ -142,25 +142,44 @@ __device__ void scan_exclusive(int n, Load4 load4, Store4 store4, const ScanCtl&
             if (lane == 0)
                 __hip_atomic_store(&sc.status[tile], scan_word(epoch, SCAN_AGG, static_cast<uint32_t>(block_total)),
                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // Each probe covers 512 predecessors: lane l reads tiles
+            // j - 8l - m, m = 0..7, nearest first, all loads in flight at once.
             int j = tile - 1;   // nearest predecessor not yet folded in
             uint32_t spins = 0;
             for (;;) {
-                const int idx = j - lane;
-                uint32_t kind = SCAN_INC, val = 0;   // idx < 0 reads as an inclusive 0
-                if (idx >= 0) {
-                    const unsigned long long w =
-                        __hip_atomic_load(&sc.status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    kind = static_cast<uint32_t>(w >> 32) == epoch ? static_cast<uint32_t>(w >> 30) & 3u : 0u;
-                    val = static_cast<uint32_t>(w & 0x3FFFFFFFull);
+                uint32_t kind[8], val[8];
+#pragma unroll
+                for (int m = 0; m < 8; ++m) {
+                    const int idx = j - 8 * lane - m;
+                    kind[m] = SCAN_INC;   // before tile 0 reads as an inclusive 0
+                    val[m] = 0u;
+                    if (idx >= 0) {
+                        const unsigned long long w =
+                            __hip_atomic_load(&sc.status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        kind[m] = static_cast<uint32_t>(w >> 32) == epoch ? static_cast<uint32_t>(w >> 30) & 3u : 0u;
+                        val[m] = static_cast<uint32_t>(w & 0x3FFFFFFFull);
+                    }
                 }
-                const unsigned long long inc_mask = __ballot(kind == SCAN_INC);
-                const unsigned long long bad_mask = __ballot(kind == 0u);
+                int32_t lsum = 0;
+                bool has_inc = false, ready = true;
+#pragma unroll
+                for (int m = 0; m < 8; ++m) {
+                    if (has_inc || !ready) continue;
+                    if (kind[m] == 0u) {
+                        ready = false;
+                        continue;
+                    }
+                    lsum += static_cast<int32_t>(val[m]);
+                    if (kind[m] == SCAN_INC) has_inc = true;
+                }
+                const unsigned long long inc_mask = __ballot(has_inc);
+                const unsigned long long not_ready = __ballot(!ready);
                 const int stop = inc_mask ? __ffsll(static_cast<long long>(inc_mask)) - 1 : 64;
                 const unsigned long long upto = stop >= 63 ? ~0ull : ((1ull << (stop + 1)) - 1ull);
-                if ((bad_mask & upto) == 0ull) {
-                    excl += wave_sum(lane <= stop ? static_cast<int32_t>(val) : 0);
+                if ((not_ready & upto) == 0ull) {
+                    excl += wave_sum(lane <= stop ? lsum : 0);
                     if (stop < 64) break;
-                    j -= 64;
+                    j -= 512;
                     continue;
                 }
                 if (++spins > SCAN_SPIN_LIMIT) {

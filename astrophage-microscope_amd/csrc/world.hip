@@ -162,6 +162,7 @@ struct Carver {
 
 template <class P>
 void carve_cells(Carver& cv, Cells<P>& c, size_t cap) {
+    c.lslot = cv.take<uint32_t>(cap);
     c.id = cv.take<uint64_t>(cap); c.flags = cv.take<uint32_t>(cap); c.death_cause = cv.take<uint8_t>(cap);
     c.x = cv.take<P>(cap); c.y = cv.take<P>(cap); c.z = cv.take<P>(cap);
     c.vx = cv.take<P>(cap); c.vy = cv.take<P>(cap); c.vz = cv.take<P>(cap);
@@ -191,8 +192,12 @@ struct astro_world {
 
     HashGeo hg{};
     double hcell = 0.0;
+    int32_t pcap = 0;
+    int32_t resort_period = 16;
     uint32_t *keys = nullptr, *rin = nullptr, *bcount = nullptr, *vals_sorted = nullptr;
+    uint32_t *tphys = nullptr, *sphys = nullptr;
     int32_t *bstart = nullptr, *spos = nullptr, *dflag = nullptr, *prefix = nullptr, *scan_total = nullptr;
+    int32_t *drop = nullptr, *dprefix = nullptr;
     void *trec = nullptr, *rec = nullptr;
     Force* fs = nullptr;
     double* dirs = nullptr;
@@ -255,6 +260,8 @@ WorldDev<P> make_view(astro_world* w) {
     WorldDev<P> d = view<P>(w);
     d.ctr = w->ctr;
     d.capacity = w->capacity;
+    d.pcap = w->pcap;
+    d.resort_period = w->resort_period;
     for (int i = 0; i < ASTRO_FIELD_COUNT; ++i) {
         d.g[i] = w->f[i].grid();
         d.on[i] = w->f[i].on;
@@ -262,7 +269,9 @@ WorldDev<P> make_view(astro_world* w) {
     d.hg = w->hg;
     d.keys = w->keys; d.rin = w->rin; d.bcount = w->bcount; d.bstart = w->bstart;
     d.vals_sorted = w->vals_sorted; d.spos = w->spos; d.fs = w->fs; d.dirs = w->dirs;
+    d.tphys = w->tphys; d.sphys = w->sphys;
     d.dflag = w->dflag; d.prefix = w->prefix; d.co2_demand = w->co2_demand; d.scan_total = w->scan_total;
+    d.drop = w->drop; d.dprefix = w->dprefix;
     d.sc = ScanCtl{w->scan_status, w->tile_ctr, w->ctr};
     d.stats = w->stats;
     return d;
@@ -364,6 +373,7 @@ int32_t enqueue_tick(astro_world* w, const Launch& L) {
     const TickParams t = tick_params(w, dt);
     const astro_world_desc& ds = w->desc;
     launch_hash_build<P>(L, make_view<P>(w), true);
+    launch_resort<P>(L, make_view<P>(w));
     w->hash_valid = true;
     int32_t st;
     if (ds.thermal_enabled && (st = enqueue_thermal<P>(w, L, dt)) != ASTRO_OK) return st;
@@ -470,6 +480,10 @@ int32_t get(astro_world* w, H* dst, const D* src, int32_t n) {
 template <class P>
 int32_t upload_cells(astro_world* w, const Cells<P>& c, const astro_cell_soa* s, int32_t n) {
     int32_t st = ASTRO_OK;
+    // Uploaded cells are stored in reference order: physical == logical.
+    std::vector<uint32_t> iota(n);
+    for (int32_t i = 0; i < n; ++i) iota[i] = static_cast<uint32_t>(i);
+    if ((st = put(w, c.lslot, iota.data(), n)) != ASTRO_OK) return st;
 #define PUT(a) \
     if ((st = put(w, c.a, s->a, n)) != ASTRO_OK) return st;
     PUT(id) PUT(flags) PUT(death_cause) PUT(x) PUT(y) PUT(z) PUT(vx) PUT(vy) PUT(vz) PUT(energy) PUT(temp_cell)
@@ -479,11 +493,34 @@ int32_t upload_cells(astro_world* w, const Cells<P>& c, const astro_cell_soa* s,
     return st;
 }
 
+// Device array over physical entries [0, np) -> host array in reference-slot
+// order, entries [0, m).
+template <class D, class H>
+int32_t get_logical(astro_world* w, H* dst, const D* src, int32_t np, const std::vector<uint32_t>& lslot, int32_t m) {
+    if (!dst || np <= 0) return ASTRO_OK;
+    std::vector<D> tmp(np);
+    int32_t st = hip_check(w, hipMemcpy(tmp.data(), src, sizeof(D) * np, hipMemcpyDeviceToHost), "download");
+    if (st != ASTRO_OK) return st;
+    for (int32_t p = 0; p < np; ++p) {
+        const uint32_t s = lslot[p];
+        if (s != NO_SLOT && s < static_cast<uint32_t>(m)) dst[s] = static_cast<H>(tmp[p]);
+    }
+    return ASTRO_OK;
+}
+
 template <class P>
 int32_t download_cells(astro_world* w, const Cells<P>& c, astro_cell_soa* s, int32_t n) {
     int32_t st = ASTRO_OK;
+    Counters k{};
+    if ((st = hip_check(w, hipMemcpy(&k, w->ctr, sizeof(k), hipMemcpyDeviceToHost), "counters")) != ASTRO_OK)
+        return st;
+    const int32_t np = k.nphys;
+    std::vector<uint32_t> lslot(np > 0 ? np : 0);
+    if (np > 0 &&
+        (st = hip_check(w, hipMemcpy(lslot.data(), c.lslot, sizeof(uint32_t) * np, hipMemcpyDeviceToHost), "lslot")) != ASTRO_OK)
+        return st;
 #define GET(a) \
-    if ((st = get(w, s->a, c.a, n)) != ASTRO_OK) return st;
+    if ((st = get_logical(w, s->a, c.a, np, lslot, n)) != ASTRO_OK) return st;
     GET(id) GET(flags) GET(death_cause) GET(x) GET(y) GET(z) GET(vx) GET(vy) GET(vz) GET(energy) GET(temp_cell)
     GET(emit_power) GET(dir_x) GET(dir_y) GET(dir_z) GET(biomass) GET(co2_held) GET(age_s) GET(rng_state)
     GET(taxis_memory) GET(run_timer) GET(t_local) GET(irradiance) GET(co2_local) GET(n2_local)
@@ -498,6 +535,29 @@ uint64_t fnv1a64(const void* data, size_t n, uint64_t h) {
         h *= 1099511628211ull;
     }
     return h;
+}
+
+// Rewrites the SoA in reference-slot order (physical == logical, no
+// tombstones), by a host round trip. Used by the rare host-side edits that
+// address slots directly; updates `c` (the host copy of the counters).
+int32_t normalize(astro_world* w, Counters& c) {
+    const int32_t n = c.count;
+    std::vector<uint64_t> id(n), rng(n);
+    std::vector<uint32_t> flags(n);
+    std::vector<uint8_t> dc(n);
+    std::vector<double> x(n), y(n), z(n), vx(n), vy(n), vz(n), energy(n), biomass(n), co2(n);
+    std::vector<float> tc(n), ep(n), dxv(n), dyv(n), dzv(n), age(n), tm(n), rt(n), tl(n), ir(n), cl(n), nl(n);
+    astro_cell_soa s{id.data(), flags.data(), dc.data(), x.data(), y.data(), z.data(), vx.data(), vy.data(),
+                     vz.data(), energy.data(), tc.data(), ep.data(), dxv.data(), dyv.data(), dzv.data(),
+                     biomass.data(), co2.data(), age.data(), rng.data(), tm.data(), rt.data(), tl.data(),
+                     ir.data(), cl.data(), nl.data()};
+    int32_t st = w->fp32 ? download_cells(w, w->c32[0], &s, n) : download_cells(w, w->c64[0], &s, n);
+    if (st != ASTRO_OK) return st;
+    st = w->fp32 ? upload_cells(w, w->c32[0], &s, n) : upload_cells(w, w->c64[0], &s, n);
+    if (st != ASTRO_OK) return st;
+    c.nphys = n;
+    w->hash_valid = false;
+    return write_counters(w, c);
 }
 
 int32_t ensure_hash(astro_world* w) {
@@ -587,6 +647,7 @@ int32_t astro_world_create(const astro_world_desc* din, astro_world** out) {
     auto* w = new astro_world();
     w->desc = d;
     w->capacity = cap;
+    w->pcap = 2 * cap;
     w->fp32 = d.precision == ASTRO_FP32_HOT;
     w->physics_rate = d.physics_rate > 0 ? d.physics_rate : 1.0;
     w->biology_rate = d.biology_rate > 0 ? d.biology_rate : 1.0;
@@ -625,10 +686,14 @@ int32_t astro_world_create(const astro_world_desc* din, astro_world** out) {
     const size_t nb = static_cast<size_t>(w->hg.nb);
     const size_t rec_bytes = w->fp32 ? sizeof(Rec<float>) : sizeof(Rec<double>);
     const size_t max_tiles = static_cast<size_t>(std::max(scan_tiles_for(cap), scan_tiles_for(static_cast<int>(nb + 1)))) + 1;
+    const size_t pn = 2 * n;
     auto layout = [&](Carver& cv) {
-        for (int s = 0; s < 2; ++s) {
-            if (w->fp32) carve_cells(cv, w->c32[s], n);
-            else carve_cells(cv, w->c64[s], n);
+        if (w->fp32) {
+            carve_cells(cv, w->c32[0], pn);
+            carve_cells(cv, w->c32[1], n);
+        } else {
+            carve_cells(cv, w->c64[0], pn);
+            carve_cells(cv, w->c64[1], n);
         }
         w->ctr = cv.take<Counters>(1);
         for (int fi = 0; fi < ASTRO_FIELD_COUNT; ++fi) {
@@ -638,18 +703,22 @@ int32_t astro_world_create(const astro_world_desc* din, astro_world** out) {
             g.scratch = cnt ? cv.take<float>(cnt) : nullptr;
             g.deposit = cnt ? cv.take<unsigned long long>(cnt) : nullptr;
         }
-        w->keys = cv.take<uint32_t>(n);
-        w->rin = cv.take<uint32_t>(n);
+        w->keys = cv.take<uint32_t>(pn);
+        w->rin = cv.take<uint32_t>(pn);
         w->bcount = cv.take<uint32_t>(nb + 1);
         w->bstart = cv.take<int32_t>(nb + 2);
         w->trec = cv.take<char>(n * rec_bytes);
+        w->tphys = cv.take<uint32_t>(n);
         w->rec = cv.take<char>(n * rec_bytes);
         w->vals_sorted = cv.take<uint32_t>(n);
-        w->spos = cv.take<int32_t>(n);
+        w->sphys = cv.take<uint32_t>(n);
+        w->spos = cv.take<int32_t>(pn);
         w->fs = cv.take<Force>(n);
         w->dirs = cv.take<double>(n);
         w->dflag = cv.take<int32_t>(n);
         w->prefix = cv.take<int32_t>(n);
+        w->drop = cv.take<int32_t>(n);
+        w->dprefix = cv.take<int32_t>(n);
         w->scan_total = cv.take<int32_t>(1);
         w->co2_demand = cv.take<unsigned long long>(std::max<size_t>(1, w->f[ASTRO_FIELD_CO2].count()));
         w->scan_status = cv.take<unsigned long long>(max_tiles);
@@ -703,16 +772,20 @@ int32_t astro_cells_spawn(astro_world* w, const astro_spawn_params* p, uint64_t 
     int32_t st = read_counters(w, c);
     if (st != ASTRO_OK) return st;
     if (c.count + p->count > w->capacity) return fail(w, ASTRO_CAPACITY_EXCEEDED, "spawn exceeds cell store capacity");
+    if (c.nphys + p->count > w->pcap) {   // pack tombstones first
+        if ((st = normalize(w, c)) != ASTRO_OK) return st;
+    }
     SpawnArgs a{};
     a.count = p->count; a.placement = p->placement; a.charge_dist = p->charge_dist; a.awake = p->awake;
     a.place_x = p->place_x; a.place_y = p->place_y; a.place_radius = p->place_radius;
     a.charge_a = p->charge_a; a.charge_b = p->charge_b;
     a.chamber_w = w->desc.chamber_w; a.chamber_h = w->desc.chamber_h; a.chamber_d = w->desc.chamber_d;
     const Launch L = launch_of(w);
-    if (w->fp32) launch_spawn<float>(L, make_view<float>(w), a, c.count, c.next_id, seed);
-    else launch_spawn<double>(L, make_view<double>(w), a, c.count, c.next_id, seed);
+    if (w->fp32) launch_spawn<float>(L, make_view<float>(w), a, c.nphys, c.count, c.next_id, seed);
+    else launch_spawn<double>(L, make_view<double>(w), a, c.nphys, c.count, c.next_id, seed);
     if ((st = hip_check(w, hipGetLastError(), "spawn_kernel")) != ASTRO_OK) return st;
     c.count += p->count;
+    c.nphys += p->count;
     c.next_id += static_cast<uint64_t>(p->count);
     w->hash_valid = false;
     return write_counters(w, c);
@@ -727,6 +800,7 @@ int32_t astro_cells_upload(astro_world* w, const astro_cell_soa* s, int32_t n) {
     st = w->fp32 ? upload_cells(w, w->c32[0], s, n) : upload_cells(w, w->c64[0], s, n);
     if (st != ASTRO_OK) return st;
     c.count = n;
+    c.nphys = n;
     w->hash_valid = false;
     return write_counters(w, c);
 }
@@ -767,7 +841,9 @@ int32_t astro_world_set_counters(astro_world* w, const astro_counters* in) {
     Counters c{};
     int32_t st = read_counters(w, c);
     if (st != ASTRO_OK) return st;
+    if ((st = normalize(w, c)) != ASTRO_OK) return st;
     c.count = in->count;
+    c.nphys = in->count;
     c.next_id = in->next_id;
     c.divisions_total = in->divisions_total;
     c.deaths_total = in->deaths_total;
@@ -1081,12 +1157,18 @@ int32_t astro_forces_download(astro_world* w, double* fx, double* fy, double* fz
     if (st != ASTRO_OK) return st;
     const int32_t m = std::min(n, c.count);
     if (m <= 0) return ASTRO_OK;
-    std::vector<Force> f(m);
-    std::vector<int32_t> sp(m);
-    if ((st = hip_check(w, hipMemcpy(f.data(), w->fs, sizeof(Force) * m, hipMemcpyDeviceToHost), "forces")) != ASTRO_OK) return st;
-    if ((st = hip_check(w, hipMemcpy(sp.data(), w->spos, sizeof(int32_t) * m, hipMemcpyDeviceToHost), "spos")) != ASTRO_OK) return st;
-    for (int32_t i = 0; i < m; ++i) {
-        const Force& F = f[sp[i]];
+    const int32_t np = c.nphys;
+    std::vector<Force> f(c.count);
+    std::vector<int32_t> sp(np);
+    std::vector<uint32_t> ls(np);
+    const uint32_t* lslot_dev = w->fp32 ? w->c32[0].lslot : w->c64[0].lslot;
+    if ((st = hip_check(w, hipMemcpy(f.data(), w->fs, sizeof(Force) * c.count, hipMemcpyDeviceToHost), "forces")) != ASTRO_OK) return st;
+    if ((st = hip_check(w, hipMemcpy(sp.data(), w->spos, sizeof(int32_t) * np, hipMemcpyDeviceToHost), "spos")) != ASTRO_OK) return st;
+    if ((st = hip_check(w, hipMemcpy(ls.data(), lslot_dev, sizeof(uint32_t) * np, hipMemcpyDeviceToHost), "lslot")) != ASTRO_OK) return st;
+    for (int32_t p = 0; p < np; ++p) {
+        const uint32_t i = ls[p];
+        if (i == NO_SLOT || i >= static_cast<uint32_t>(m)) continue;
+        const Force& F = f[sp[p]];
         if (fx) fx[i] = F.x;
         if (fy) fy[i] = F.y;
         if (fz) fz[i] = F.z;
@@ -1147,8 +1229,13 @@ int32_t astro_stage_lifecycle(astro_world* w, double dt) {
     if (!w) return ASTRO_INVALID_ARGUMENT;
     const Launch L = launch_of(w);
     const TickParams t = tick_params(w, dt);
-    if (w->fp32) launch_lifecycle<float>(L, make_view<float>(w), t);
-    else launch_lifecycle<double>(L, make_view<double>(w), t);
+    if (w->fp32) {
+        launch_life_clear<float>(L, make_view<float>(w));
+        launch_lifecycle<float>(L, make_view<float>(w), t);
+    } else {
+        launch_life_clear<double>(L, make_view<double>(w));
+        launch_lifecycle<double>(L, make_view<double>(w), t);
+    }
     w->hash_valid = false;
     return hip_check(w, hipGetLastError(), "lifecycle");
 }
