@@ -69,7 +69,7 @@ struct Counters {
     int32_t  nphys;            // physical entries [0, nphys) in use (incl. tombstones)
     int32_t  resort;           // 1 while this tick re-sorts the SoA
     uint32_t tick_ctr;         // ticks run, for the re-sort cadence
-    int32_t  _pad1;
+    uint32_t blocks_done;      // finished-block ticket of the division pass
     uint64_t next_id;
     int64_t  divisions_total;
     int64_t  deaths_total;
@@ -77,15 +77,14 @@ struct Counters {
     int32_t  births;           // this tick (unclamped)
     int32_t  deads;            // occupied-but-dead slots this tick
     int32_t  live;             // compaction survivors
-    int32_t  compacting;       // 1 while this tick compacts
-    uint32_t scan_epoch;       // look-back scan generation, never reset
+    int32_t  div_total;        // total of this tick's divide-flag scan
+    int32_t  drop_total;       // total of this tick's drop-flag scan
     uint32_t error_flags;      // sticky device-side errors (see ERR_*)
     int32_t  _pad;
     int64_t  cell_steps;       // sum of the count each tick began with
-    unsigned long long scan_base;  // tiles handed out by all earlier scans
 };
 
-enum : uint32_t { ERR_SCAN_SPIN = 1u, ERR_COEFF = 2u };
+enum : uint32_t { ERR_SCAN = 1u, ERR_COEFF = 2u };
 
 // One scalar field (contracts/fields_v1.h:164-174), extended to nz planes.
 struct Grid {
@@ -292,13 +291,25 @@ __device__ __forceinline__ float grid_sample(const Grid& g, float x, float y, fl
     const int y0 = static_cast<int>(floorf(gy));
     const float tx = gx - static_cast<float>(x0);
     const float ty = gy - static_cast<float>(y0);
-    const int i0 = clampi(x0, g.nx), i1 = clampi(x0 + 1, g.nx);
     const int j0 = clampi(y0, g.ny), j1 = clampi(y0 + 1, g.ny);
     const size_t pl = static_cast<size_t>(g.nx) * g.ny;
+    // Values at (i0, j) and (i1, j) of one row as one 8-byte load: the pair
+    // (b, b + 1) with b = clamp(x0, 0, nx - 2) holds both, also at the clamped
+    // edges (x0 < 0: both are element b; x0 > nx - 2: both are b + 1).
+    const int xb = x0 < 0 ? 0 : (x0 > g.nx - 2 ? g.nx - 2 : x0);
+    const bool lo_edge = x0 < 0, hi_edge = x0 > g.nx - 2;
+    auto pair = [&](const float* row, float& a, float& b) {
+        float2 q;
+        __builtin_memcpy(&q, row + xb, sizeof(q));
+        a = hi_edge ? q.y : q.x;
+        b = lo_edge ? q.x : q.y;
+    };
     auto plane = [&](int k) -> float {
         const float* v = g.value + static_cast<size_t>(k) * pl;
-        return (1.0f - tx) * (1.0f - ty) * v[j0 * g.nx + i0] + tx * (1.0f - ty) * v[j0 * g.nx + i1] +
-               (1.0f - tx) * ty * v[j1 * g.nx + i0] + tx * ty * v[j1 * g.nx + i1];
+        float v00, v10, v01, v11;
+        pair(v + static_cast<size_t>(j0) * g.nx, v00, v10);
+        pair(v + static_cast<size_t>(j1) * g.nx, v01, v11);
+        return (1.0f - tx) * (1.0f - ty) * v00 + tx * (1.0f - ty) * v10 + (1.0f - tx) * ty * v01 + tx * ty * v11;
     };
     if (g.nz == 1) return plane(0);
     const float half_z = 0.5f * g.nz * g.fdz;
@@ -375,6 +386,29 @@ __host__ __device__ __forceinline__ int blocks_for_xcd(int n) { return ((n + NUM
 __device__ __forceinline__ int xcd_tile(int b, int nlaunched) {
     const int per = nlaunched / NUM_XCD;
     return (b % NUM_XCD) * per + b / NUM_XCD;
+}
+
+// Grid size of a per-cell pass over up to n entries: at most MAX_BLOCKS, a
+// multiple of NUM_XCD.
+__host__ __device__ __forceinline__ int cell_grid(int n) { return blocks_for_xcd(stride_grid(n)); }
+
+// Runs body(i, valid) for i over [0, n) in BLOCK-wide tiles. The tiles are
+// split into NUM_XCD contiguous chunks and the blocks of XCD x walk chunk x,
+// so the cells one L2 serves are spatial neighbours (cells are stored in
+// spatial-hash order) and share field voxels and neighbour records. Whole
+// blocks iterate together (`valid` is false past n), so the body may use
+// wave-wide operations. gridDim.x must be a multiple of NUM_XCD.
+template <class Body>
+__device__ __forceinline__ void xcd_tiles(int n, Body&& body) {
+    const int ntiles = (n + BLOCK - 1) / BLOCK;
+    const int x = static_cast<int>(blockIdx.x) % NUM_XCD;
+    const int nb = static_cast<int>(gridDim.x) / NUM_XCD;
+    const int per = (ntiles + NUM_XCD - 1) / NUM_XCD;
+    const int t1 = min(ntiles, (x + 1) * per);
+    for (int t = x * per + static_cast<int>(blockIdx.x) / NUM_XCD; t < t1; t += nb) {
+        const int i = t * BLOCK + static_cast<int>(threadIdx.x);
+        body(i, i < n);
+    }
 }
 
 }  // namespace astro_dev

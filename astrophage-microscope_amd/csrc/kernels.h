@@ -44,7 +44,8 @@ struct WorldDev {
     HashGeo hg;
     uint32_t* keys;         // [pcap] bucket of each physical entry
     uint32_t* rin;          // [pcap] arrival rank inside the bucket; reused as
-                            //        the CO2 voxel index during lifecycle
+                            //        the CO2 voxel of each cell during lifecycle
+    float*    co2_avail;    // [pcap] CO2 concentration each cell's demand saw
     uint32_t* bcount;       // [nb + 1] bucket populations, zero between builds
     int32_t*  bstart;       // [nb + 2] dense exclusive prefix of bcount
     Rec<P>*   trec;         // [cap] records in arrival order
@@ -56,13 +57,15 @@ struct WorldDev {
     Force*    fs;           // [cap] force in sorted order
     double*   dirs;         // [cap] directional irradiance in sorted order
 
+    int32_t*  bsum;         // [tiles(nb + 1)] per-scan-tile totals of bcount
     int32_t*  dflag;        // [cap] divides flag by reference slot
     int32_t*  prefix;       // [cap] exclusive prefix of dflag
+    int32_t*  dsum;         // [tiles(cap)] per-scan-tile totals of dflag
     int32_t*  drop;         // [cap] compaction drop flag by reference slot
     int32_t*  dprefix;      // [cap] exclusive prefix of drop
+    int32_t*  xsum;         // [tiles(cap)] per-scan-tile totals of drop
+    uint32_t* dlist;        // [cap] dropped reference slots, ascending
     unsigned long long* co2_demand;   // [CO2 grid] fixed-point demand
-    int32_t*  scan_total;   // scratch total of the last scan
-    ScanCtl   sc;
     unsigned long long* stats;        // StatsAccum words (see launch_stats)
 };
 
@@ -97,7 +100,8 @@ struct Ftcs {
 
 // ---- hash (kernels_hash.hip) ----
 template <class P> void launch_hash_build(const Launch& L, const WorldDev<P>& w, bool count_tick);
-template <class P> void launch_forces_walk(const Launch& L, const WorldDev<P>& w, const TickParams& t);
+// The contact half of forces_kernel over the sorted records, adding to fs.
+template <class P> void launch_contact_walk(const Launch& L, const WorldDev<P>& w, const TickParams& t);
 template <class P> void launch_feed_walk(const Launch& L, const WorldDev<P>& w, const TickParams& t);
 // Re-sorts the SoA into bucket order when due (device-side decision).
 template <class P> void launch_resort(const Launch& L, const WorldDev<P>& w);
@@ -115,11 +119,20 @@ template <class P> void launch_thermal_substep(const Launch& L, const WorldDev<P
 template <class P> void launch_apply_awake(const Launch& L, const WorldDev<P>& w, int field);
 template <class P> void launch_feed_taxis(const Launch& L, const WorldDev<P>& w, const TickParams& t,
                                           bool feed, bool taxis);
-template <class P> void launch_sample_base(const Launch& L, const WorldDev<P>& w, const TickParams& t);
-template <class P> void launch_integrate(const Launch& L, const WorldDev<P>& w, const TickParams& t);
-template <class P> void launch_lifecycle(const Launch& L, const WorldDev<P>& w, const TickParams& t);
-template <class P> void launch_compact(const Launch& L, const WorldDev<P>& w, bool forced);
-// Clears the per-slot lifecycle flags (the tick does this in its key pass).
+// integrate_kernel; with `demand` it also books each moved cell's CO2 demand
+// (the lifecycle's first kernel; the CO2 field must have diffused already).
+// Feeding/taxis (optional), field sample + body force, then the contact walk:
+// leaves each cell's total force in fs (sorted order).
+template <class P> void launch_cell_forces(const Launch& L, const WorldDev<P>& w, const TickParams& t, bool feed,
+                                           bool taxis);
+template <class P> void launch_integrate(const Launch& L, const WorldDev<P>& w, const TickParams& t, bool demand);
+// lifecycle_step; `demand_booked` when the integrate pass booked the demand.
+template <class P> void launch_lifecycle(const Launch& L, const WorldDev<P>& w, const TickParams& t,
+                                         bool demand_booked);
+// cell_store_compact outside the lifecycle: every dead or empty slot goes.
+template <class P> void launch_compact(const Launch& L, const WorldDev<P>& w);
+// Clears the per-slot lifecycle flags and their tile totals (the tick does
+// this in its key pass).
 template <class P> void launch_life_clear(const Launch& L, const WorldDev<P>& w);
 template <class P> void launch_stats_cells(const Launch& L, const WorldDev<P>& w);
 

@@ -2,6 +2,9 @@
 // field sample + body force, the OU integrator, lifecycle and compaction, and
 // the per-cell telemetry reduction. One thread per slot; every per-cell array
 // is read and written coalesced in slot order.
+#include <algorithm>
+
+#include "cell_ops.h"
 #include "kernels.h"
 
 namespace astro_dev {
@@ -151,7 +154,7 @@ __global__ __launch_bounds__(BLOCK) void k_thermal(Cells<P> c, const Counters* c
         if (update_rec) rec[spos[i]].meta &= ~REC_ALIVE;
     }
     };
-    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) body(i);
+    xcd_tiles(n, [&](const int i, const bool valid) { if (valid) body(i); });
 }
 
 // Folds a field's fixed-point accumulator into its values at the cells that
@@ -171,185 +174,92 @@ __global__ __launch_bounds__(BLOCK) void k_apply_awake(Cells<P> c, const Counter
     const unsigned long long raw = atomicExch(&g.deposit[idx], 0ull);
     if (raw != 0ull) g.value[idx] += static_cast<float>(from_fixed(raw, g.scale));
     };
-    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) body(i);
+    xcd_tiles(n, [&](const int i, const bool valid) { if (valid) body(i); });
 }
 
 // ---------------------------------------------------------------------------
-// feed_kernel's absorption half (src/sim/emission.cu:107-114) fused with
-// taxis_kernel (src/sim/taxis.cu:13-79): both touch only the cell's own
-// irradiance, energy and emission state, in that order.
+// Stand-alone feeding / taxis stage (emission absorption, taxis_step) over
+// physical entries; the tick runs the same body inside the fused force pass.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void any_perpendicular(double hx, double hy, double hz, double& ux, double& uy, double& uz) {
-    const double ax = fabs(hx), ay = fabs(hy), az = fabs(hz);
-    double bx = 0.0, by = 0.0, bz = 0.0;
-    if (ax <= ay && ax <= az) bx = 1.0;
-    else if (ay <= az) by = 1.0;
-    else bz = 1.0;
-    double cx = hy * bz - hz * by, cy = hz * bx - hx * bz, cz = hx * by - hy * bx;
-    const double l = sqrt(cx * cx + cy * cy + cz * cz);
-    if (l > 0.0) {
-        const double inv = 1.0 / l;
-        ux = cx * inv; uy = cy * inv; uz = cz * inv;
-    } else {
-        ux = 0.0; uy = 0.0; uz = 1.0;
-    }
-}
-
 template <class P, bool FEED, bool TAXIS>
 __global__ __launch_bounds__(BLOCK) void k_feed_taxis(Cells<P> c, const Counters* ctr, const int32_t* spos,
                                                       const double* dirs, float ambient, double dt, double max_power) {
     const int n = ctr->nphys;
-    const auto body = [&](const int i) {
-    const uint32_t f = c.flags[i];
-    if (!(f & ASTRO_FLAG_OCCUPIED) || !(f & ASTRO_FLAG_ALIVE)) return;
-    double energy = c.energy[i];
-    float irr_f = 0.0f;
-    if constexpr (FEED) {
-        const double directional = dirs[spos[i]];
-        irr_f = static_cast<float>(directional);
-        c.irradiance[i] = irr_f;
-        const double gained = (directional * ASTRO_CELL_CROSS_SECTION +
-                               static_cast<double>(ambient) * ASTRO_CELL_SURFACE_AREA) * dt;
-        double e = energy + gained;
-        if (e > ASTRO_CELL_ENERGY_MAX) e = ASTRO_CELL_ENERGY_MAX;
-        energy = e;
-        c.energy[i] = e;
-    }
-    if constexpr (TAXIS) {
-        if (!(f & ASTRO_FLAG_AWAKE)) {
-            c.emit_power[i] = 0.0f;
-            return;
-        }
-        if constexpr (!FEED) irr_f = c.irradiance[i];
-        const double irr = static_cast<double>(irr_f);
-        const double co2 = static_cast<double>(c.co2_local[i]);
-        const double charge = energy / ASTRO_CELL_ENERGY_MAX;
-        // taxis_select_state (src/sim/taxis.cuh:55-62)
-        const bool dark = irr < ASTRO_TAXIS_DARK_THRESHOLD;
-        const bool co2_here = co2 > 0.0;
-        int state = 0;
-        if (dark && !co2_here) state = 0;
-        else if (!dark && charge < ASTRO_TAXIS_SEEK_FEED_BELOW) state = 1;
-        else if (charge > ASTRO_TAXIS_SEEK_BREED_ABOVE && co2_here) state = 2;
-        if (state == 0) {
-            c.emit_power[i] = 0.0f;
-            c.run_timer[i] = 0.0f;
-            c.taxis_memory[i] = 0.0f;
-            return;
-        }
-        const double signal = state == 1 ? irr : co2;
-        const float mem = c.taxis_memory[i];
-        const double delta = signal - static_cast<double>(mem);
-        const double alpha = 1.0 - exp(-dt / ASTRO_TAXIS_MEMORY_TIME);
-        c.taxis_memory[i] = static_cast<float>(static_cast<double>(mem) + alpha * (signal - static_cast<double>(mem)));
-        double hx = -static_cast<double>(c.dir_x[i]);
-        double hy = -static_cast<double>(c.dir_y[i]);
-        double hz = -static_cast<double>(c.dir_z[i]);
-        double run = static_cast<double>(c.run_timer[i]) + dt;
-        if (delta <= 0.0 || run >= ASTRO_TAXIS_RUN_MAX) {
-            // taxis_tumble (src/sim/taxis.cuh:108-118)
-            Pcg r = cell_rng(c.rng_state[i], c.id[i]);
-            double theta = -ASTRO_TAXIS_TUMBLE_ANGLE_MEAN * log(1.0 - r.uniform01d());
-            if (theta > ASTRO_PI) theta = ASTRO_PI;
-            const double azimuth = ASTRO_TWO_PI * r.uniform01d();
-            c.rng_state[i] = r.state;
-            double l = sqrt(hx * hx + hy * hy + hz * hz);
-            if (l > 0.0) {
-                const double inv = 1.0 / l;
-                hx *= inv; hy *= inv; hz *= inv;
-            } else {
-                hx = 0.0; hy = 0.0; hz = 1.0;
-            }
-            double ux, uy, uz;
-            any_perpendicular(hx, hy, hz, ux, uy, uz);
-            const double wx = hy * uz - hz * uy, wy = hz * ux - hx * uz, wz = hx * uy - hy * ux;
-            const double ca = cos(azimuth), sa = sin(azimuth);
-            const double px = ux * ca + wx * sa, py = uy * ca + wy * sa, pz = uz * ca + wz * sa;
-            const double ct = cos(theta), st = sin(theta);
-            double nx = hx * ct + px * st, ny = hy * ct + py * st, nz = hz * ct + pz * st;
-            l = sqrt(nx * nx + ny * ny + nz * nz);
-            if (l > 0.0) {
-                const double inv = 1.0 / l;
-                nx *= inv; ny *= inv; nz *= inv;
-            } else {
-                nx = 0.0; ny = 0.0; nz = 1.0;
-            }
-            hx = nx; hy = ny; hz = nz;
-            run = 0.0;
-        }
-        c.run_timer[i] = static_cast<float>(run);
-        // taxis_emit_dir = -normalize(heading)
-        double l = sqrt(hx * hx + hy * hy + hz * hz);
-        double ex, ey, ez;
-        if (l > 0.0) {
-            const double inv = 1.0 / l;
-            ex = hx * inv; ey = hy * inv; ez = hz * inv;
-        } else {
-            ex = 0.0; ey = 0.0; ez = 1.0;
-        }
-        c.dir_x[i] = static_cast<float>(-ex);
-        c.dir_y[i] = static_cast<float>(-ey);
-        c.dir_z[i] = static_cast<float>(-ez);
-        double p = 0.0;
-        if (energy > 0.0) {
-            const double avail = dt > 0.0 ? energy / dt : 0.0;
-            p = max_power < avail ? max_power : avail;
-        }
-        c.emit_power[i] = static_cast<float>(p);
-        c.energy[i] = energy - p * dt;
-    }
-    };
-    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) body(i);
+    xcd_tiles(n, [&](const int i, const bool valid) {
+        if (!valid) return;
+        const uint32_t f = c.flags[i];
+        if (!(f & ASTRO_FLAG_OCCUPIED) || !(f & ASTRO_FLAG_ALIVE)) return;
+        const double directional = FEED ? dirs[spos[i]] : 0.0;
+        feed_taxis_cell<P, FEED, TAXIS>(c, i, f, directional, ambient, dt, max_power);
+    });
 }
 
 // ---------------------------------------------------------------------------
-// field_sample_kernel (src/sim/integrator.cu:20-30) + the body-force half of
-// forces_kernel (:40-58): buoyancy and photon recoil, written to the cell's
-// sorted position for the contact walk to continue.
+// field_sample_kernel (src/sim/integrator.cu:20-30) and the body-force half of
+// forces_kernel (:40-58), per physical entry; the force goes to the cell's
+// sorted position for the contact walk to add to.
 // ---------------------------------------------------------------------------
 template <class P>
 __global__ __launch_bounds__(BLOCK) void k_sample_base(Cells<P> c, const Counters* ctr, Grid T, Grid C, Grid N,
                                                        int c_on, int n_on, int gaxis, const int32_t* spos, Force* fs) {
     const int n = ctr->nphys;
-    const auto body = [&](const int i) {
-    if (c.lslot[i] == NO_SLOT) return;
-    const uint32_t fl = c.flags[i];
-    Force F{0.0, 0.0, 0.0, 0.0};
-    if (fl & ASTRO_FLAG_OCCUPIED) {
-        const float x = static_cast<float>(c.x[i]);
-        const float y = static_cast<float>(c.y[i]);
-        const float z = static_cast<float>(c.z[i]);
-        c.t_local[i] = grid_sample(T, x, y, z);
-        c.co2_local[i] = c_on ? grid_sample(C, x, y, z) : 0.0f;
-        c.n2_local[i] = n_on ? grid_sample(N, x, y, z) : 0.0f;
-        const double mass = cell_mass(c.biomass[i], c.energy[i]);
-        const bool alive = (fl & ASTRO_FLAG_ALIVE) != 0u;
-        const double emit = alive ? static_cast<double>(c.emit_power[i]) : 0.0;
-        const double w = buoyant_weight(mass);
-        if (gaxis == ASTRO_GRAVITY_Y) F.y += w;
-        else F.z += w;
-        const double thrust = emit / ASTRO_C_LIGHT;
-        if (thrust != 0.0) {
-            F.x -= static_cast<double>(c.dir_x[i]) * thrust;
-            F.y -= static_cast<double>(c.dir_y[i]) * thrust;
-            F.z -= static_cast<double>(c.dir_z[i]) * thrust;
-        }
-    }
-    fs[spos[i]] = F;
-    };
-    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) body(i);
+    xcd_tiles(n, [&](const int i, const bool valid) {
+        if (!valid || c.lslot[i] == NO_SLOT) return;
+        const Force F = sample_force_cell<P>(c, i, c.flags[i], static_cast<float>(c.x[i]), static_cast<float>(c.y[i]),
+                                             static_cast<float>(c.z[i]), T, C, N, c_on, n_on, gaxis);
+        fs[spos[i]] = F;
+    });
 }
 
 // ---------------------------------------------------------------------------
 // integrate_kernel (src/sim/integrator.cu:79-150).
 // ---------------------------------------------------------------------------
-template <class P>
-__global__ __launch_bounds__(BLOCK) void k_integrate(Cells<P> c, const Counters* ctr, const int32_t* spos,
-                                                     const Force* fs, TickParams t) {
+// Books one live cell's CO2 demand against its nearest voxel: demand_kernel
+// (src/sim/lifecycle.cu:41-52) for one cell. Records the voxel (NO_VOXEL if
+// none) and the CO2 it saw there, for the uptake.
+constexpr uint32_t NO_VOXEL = 0xFFFFFFFFu;
+// Set on the one cell per shared voxel whose uptake deposit found the
+// accumulator empty: that cell alone flushes the voxel. Voxel indices stay
+// below 2^31 - 1.
+constexpr uint32_t VOX_OWNER = 0x80000000u;
+
+struct DemandArgs {
+    Grid C;
+    double dt_bio, gv;
+    unsigned long long* demand;
+    uint32_t* vox;
+    float* avail;
+};
+
+__device__ __forceinline__ void book_demand(const DemandArgs& a, int i, bool live, float x, float y, float z) {
+    uint32_t v = NO_VOXEL;
+    if (live) {
+        const uint32_t idx = grid_nearest(a.C, x, y, z);
+        const float cv = a.C.value[idx];
+        const double want = co2_uptake_rate(static_cast<double>(cv)) * a.dt_bio;
+        if (want > 0.0) {
+            const int64_t fxd = to_fixed(want / a.gv, a.C.scale);
+            if (fxd != 0) atomicAdd(&a.demand[idx], static_cast<unsigned long long>(fxd));
+            v = idx;
+            a.avail[i] = cv;
+        }
+    }
+    a.vox[i] = v;
+}
+
+// integrate_kernel (src/sim/integrator.cu:79-150). With DEMAND the same pass
+// then books the moved cell's CO2 demand (lifecycle's first kernel, which
+// reads nothing the steps in between change once the CO2 field has diffused).
+template <class P, bool DEMAND>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_integrate(
+    Cells<P> c, const Counters* ctr, const int32_t* spos, const Force* fs, TickParams t, DemandArgs dm) {
     const int n = ctr->nphys;
     const auto body = [&](const int i) {
     const uint32_t fl = c.flags[i];
-    if (!(fl & ASTRO_FLAG_OCCUPIED)) return;
+    if (!(fl & ASTRO_FLAG_OCCUPIED)) {
+        if constexpr (DEMAND) dm.vox[i] = NO_VOXEL;
+        return;
+    }
     const double dt = t.dt;
     const double t_local = static_cast<double>(c.t_local[i]);
     const double mass = cell_mass(c.biomass[i], c.energy[i]);
@@ -363,14 +273,16 @@ __global__ __launch_bounds__(BLOCK) void k_integrate(Cells<P> c, const Counters*
     const uint64_t id = c.id[i];
     Pcg r = cell_rng(c.rng_state[i], id);
     const Ou oc = ou_coefficients(mass, gamma, t_drag, dt);
-    double z1, z2, z3, z4, z5, z6;
-    r.gaussian_pair(z1, z2);
-    r.gaussian_pair(z3, z4);
-    r.gaussian_pair(z5, z6);
+    // The three pairs are drawn in the reference's order (x, y, z); each is
+    // used as soon as it is drawn, which keeps fewer values live.
     const bool noise = t.thermal_noise != 0;
+    double z1, z2;
+    r.gaussian_pair(z1, z2);
     ou_advance_axis(px, vx, F.x, gamma, oc, dt, noise, z1, z2);
-    ou_advance_axis(py, vy, F.y, gamma, oc, dt, noise, z3, z4);
-    ou_advance_axis(pz, vz, F.z, gamma, oc, dt, noise, z5, z6);
+    r.gaussian_pair(z1, z2);
+    ou_advance_axis(py, vy, F.y, gamma, oc, dt, noise, z1, z2);
+    r.gaussian_pair(z1, z2);
+    ou_advance_axis(pz, vz, F.z, gamma, oc, dt, noise, z1, z2);
     const double a = ASTRO_CELL_RADIUS;
     bool kept = apply_boundary_axis(px, vx, 0.5 * t.chamber_w, a, t.boundary_x);
     kept = apply_boundary_axis(py, vy, 0.5 * t.chamber_h, a, t.boundary_y) && kept;
@@ -396,271 +308,339 @@ __global__ __launch_bounds__(BLOCK) void k_integrate(Cells<P> c, const Counters*
         c.emit_power[i] = 0.0f;
     }
     c.flags[i] = nf;
-    c.x[i] = static_cast<P>(px);
-    c.y[i] = static_cast<P>(py);
-    c.z[i] = static_cast<P>(pz);
+    const P sx = static_cast<P>(px), sy = static_cast<P>(py), sz = static_cast<P>(pz);
+    c.x[i] = sx;
+    c.y[i] = sy;
+    c.z[i] = sz;
     c.vx[i] = static_cast<P>(vx);
     c.vy[i] = static_cast<P>(vy);
     c.vz[i] = static_cast<P>(vz);
     c.age_s[i] += static_cast<float>(dt);
+    if constexpr (DEMAND)
+        book_demand(dm, i, (nf & ASTRO_FLAG_ALIVE) != 0u, static_cast<float>(sx), static_cast<float>(sy),
+                    static_cast<float>(sz));
     };
-    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) body(i);
+    xcd_tiles(n, [&](const int i, const bool valid) { if (valid) body(i); });
 }
 
 // ---------------------------------------------------------------------------
 // lifecycle_step (src/sim/lifecycle.cu:208-294), host syncs replaced by
 // device counters.
 // ---------------------------------------------------------------------------
-constexpr uint32_t NO_VOXEL = 0xFFFFFFFFu;
-
-// demand_kernel (lifecycle.cu:41-52). Also records the voxel each cell booked
-// against (NO_VOXEL if none), for the uptake and the sparse flush.
+// demand_kernel (lifecycle.cu:41-52) on its own, for the stage entry point.
 template <class P>
-__global__ __launch_bounds__(BLOCK) void k_demand(Cells<P> c, const Counters* ctr, Grid C, double dt_bio, double gv,
-                                                  unsigned long long* demand, uint32_t* vox) {
+__global__ __launch_bounds__(BLOCK) void k_demand(Cells<P> c, const Counters* ctr, DemandArgs dm) {
     const int n = ctr->nphys;
-    const auto body = [&](const int i) {
-    vox[i] = NO_VOXEL;
-    const uint32_t f = c.flags[i];
-    if (!(f & ASTRO_FLAG_OCCUPIED) || !(f & ASTRO_FLAG_ALIVE)) return;
-    const uint32_t idx = grid_nearest(C, static_cast<float>(c.x[i]), static_cast<float>(c.y[i]),
-                                      static_cast<float>(c.z[i]));
-    const double want = co2_uptake_rate(static_cast<double>(C.value[idx])) * dt_bio;
-    if (want <= 0.0) return;
-    deposit_add(&demand[idx], want / gv, C.scale);
-    vox[i] = idx;
-    };
-    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) body(i);
+    xcd_tiles(n, [&](const int i, const bool valid) {
+        if (!valid) return;
+        const uint32_t f = c.flags[i];
+        book_demand(dm, i, (f & ASTRO_FLAG_OCCUPIED) && (f & ASTRO_FLAG_ALIVE), static_cast<float>(c.x[i]),
+                    static_cast<float>(c.y[i]), static_cast<float>(c.z[i]));
+    });
 }
 
-// uptake_kernel (lifecycle.cu:62-86).
+// uptake_kernel (lifecycle.cu:62-86), death_kernel (:100-112) and
+// mark_kernel (:119-135) in one pass; each touches only the cell's own state
+// except the uptake's voxel.
+//   * Uptake. Nothing in this pass reads the CO2 field itself (the demand
+//     pass stored the concentration each cell saw), so the flush that follows
+//     every uptake in the reference may be done per voxel as soon as that
+//     voxel's total is known. A cell whose own demand is its voxel's whole
+//     demand is the voxel's only depositor (demands are nonnegative and a
+//     zero-demand cell's take rounds to a zero deposit), so it applies
+//     `value += float(from_fixed(deposit))` itself, with no accumulator. In a
+//     shared voxel the deposits accumulate atomically; they are all negative,
+//     so exactly one depositor sees the accumulator still at zero, and it is
+//     marked the voxel's owner for the flush in the apply pass.
+//   * Divide and drop flags are set by reference slot, each with its scan-tile
+//     total.
 template <class P>
-__global__ __launch_bounds__(BLOCK) void k_uptake(Cells<P> c, const Counters* ctr, Grid C, double dt_bio, double gv,
-                                                  const unsigned long long* demand, const uint32_t* vox) {
+__global__ __launch_bounds__(BLOCK) void k_uptake_mark(Cells<P> c, Counters* ctr, DemandArgs dm, int c_on,
+                                                       int disposition, double quota, int32_t* dflag, int32_t* dsum,
+                                                       int32_t* drop, int32_t* xsum) {
     const int n = ctr->nphys;
-    const auto body = [&](const int i) {
-    const uint32_t idx = vox[i];
-    if (idx == NO_VOXEL) return;
-    const double available = static_cast<double>(C.value[idx]);
-    const double want = co2_uptake_rate(available) * dt_bio;
-    if (want <= 0.0) return;
-    const double asked = from_fixed(demand[idx], C.scale);
-    double taken = want;
-    if (asked > available && asked > 0.0) taken = want * (available / asked);
-    if (taken <= 0.0) return;
-    c.co2_held[i] += taken;
-    deposit_add(&C.deposit[idx], -taken / gv, C.scale);
-    };
-    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) body(i);
-}
-
-// The CO2 flush (lifecycle.cu:235), death_kernel (:100-112) and mark_kernel
-// (:119-135) in one pass: each touches only the cell's own state or, for the
-// flush, its voxel through the exchange protocol of k_apply_awake.
-template <class P>
-__global__ __launch_bounds__(BLOCK) void k_life_mark(Cells<P> c, Counters* ctr, Grid C, int c_on,
-                                                     unsigned long long* demand, const uint32_t* vox,
-                                                     int disposition, double quota, int32_t* dflag, int32_t* drop) {
-    const int n = ctr->nphys;
-    for (int base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
-    const int i = base + threadIdx.x;
-    bool divides = false, dead = false;
-    if (i < n) {
-        if (c_on) {
-            const uint32_t idx = vox[i];
-            if (idx != NO_VOXEL) {
-                const unsigned long long raw = atomicExch(&C.deposit[idx], 0ull);
-                if (raw != 0ull) C.value[idx] += static_cast<float>(from_fixed(raw, C.scale));
-                demand[idx] = 0ull;
+    xcd_tiles(n, [&](const int i, const bool valid) {
+        bool divides = false, dead = false;
+        if (valid) {
+            if (c_on) {
+                const uint32_t idx = dm.vox[i];
+                if (idx != NO_VOXEL) {
+                    const double available = static_cast<double>(dm.avail[i]);
+                    const double want = co2_uptake_rate(available) * dm.dt_bio;
+                    const unsigned long long raw = dm.demand[idx];
+                    const double asked = from_fixed(raw, dm.C.scale);
+                    double taken = want;
+                    if (asked > available && asked > 0.0) taken = want * (available / asked);
+                    if (taken > 0.0) {
+                        c.co2_held[i] += taken;
+                        const int64_t fxd = to_fixed(-taken / dm.gv, dm.C.scale);
+                        if (fxd != 0) {
+                            if (static_cast<int64_t>(raw) == to_fixed(want / dm.gv, dm.C.scale)) {
+                                dm.C.value[idx] += static_cast<float>(
+                                    from_fixed(static_cast<unsigned long long>(fxd), dm.C.scale));
+                            } else {
+                                const unsigned long long old =
+                                    atomicAdd(&dm.C.deposit[idx], static_cast<unsigned long long>(fxd));
+                                if (old == 0ull) dm.vox[i] = idx | VOX_OWNER;
+                            }
+                        }
+                    }
+                }
+            }
+            uint32_t f = c.flags[i];
+            if ((f & ASTRO_FLAG_OCCUPIED) && (f & ASTRO_FLAG_ALIVE) &&
+                static_cast<double>(c.t_local[i]) > ASTRO_CELL_LETHAL_TEMP) {
+                f &= ~ASTRO_FLAG_ALIVE;
+                c.flags[i] = f;
+                c.death_cause[i] = ASTRO_DEATH_OVERHEATED;
+                c.emit_power[i] = 0.0f;
+                c.energy[i] = disposition == ASTRO_STORE_RETAIN ? c.energy[i] : 0.0;
+            }
+            const bool occ = (f & ASTRO_FLAG_OCCUPIED) != 0u, alive = (f & ASTRO_FLAG_ALIVE) != 0u;
+            divides = occ && alive && c.co2_held[i] >= quota;
+            dead = occ && !alive;
+            const uint32_t s = c.lslot[i];
+            if (s != NO_SLOT) {
+                // Both arrays and their tile totals start the tick cleared.
+                if (divides) {
+                    dflag[s] = 1;
+                    atomicAdd(&dsum[s >> SCAN_TILE_SHIFT], 1);
+                }
+                if (!(occ && alive)) {   // compaction keeps occupied && alive
+                    drop[s] = 1;
+                    atomicAdd(&xsum[s >> SCAN_TILE_SHIFT], 1);
+                }
             }
         }
-        uint32_t f = c.flags[i];
-        if ((f & ASTRO_FLAG_OCCUPIED) && (f & ASTRO_FLAG_ALIVE) &&
-            static_cast<double>(c.t_local[i]) > ASTRO_CELL_LETHAL_TEMP) {
-            f &= ~ASTRO_FLAG_ALIVE;
-            c.flags[i] = f;
-            c.death_cause[i] = ASTRO_DEATH_OVERHEATED;
-            c.emit_power[i] = 0.0f;
-            c.energy[i] = disposition == ASTRO_STORE_RETAIN ? c.energy[i] : 0.0;
-        }
-        const bool occ = (f & ASTRO_FLAG_OCCUPIED) != 0u, alive = (f & ASTRO_FLAG_ALIVE) != 0u;
-        divides = occ && alive && c.co2_held[i] >= quota;
-        dead = occ && !alive;
-        const uint32_t s = c.lslot[i];
-        if (s != NO_SLOT) {
-            if (divides) dflag[s] = 1;          // cleared at the start of the tick
-            if (!(occ && alive)) drop[s] = 1;   // compaction keeps occupied && alive
-        }
-    }
-    wave_count_add(&ctr->births, divides);
-    wave_count_add(&ctr->deads, dead);
+        wave_count_add(&ctr->births, divides);
+        wave_count_add(&ctr->deads, dead);
+    });
+}
+
+// Whether this tick compacts: lifecycle.cu:289 (compaction on and at least
+// one occupied dead slot), or a direct cell_store_compact request.
+__device__ __forceinline__ bool compacting_now(const Counters* ctr, int compaction, int forced) {
+    return forced != 0 || (compaction != 0 && ctr->deads > 0);
+}
+
+// Both lifecycle scans in one launch, over reference slots [0, count): blocks
+// [0, T) scan the divide flags (when anything divides), blocks [T, 2T) the
+// drop flags (when compacting), T = tiles(capacity). The drop scan also lists
+// the dropped slots in ascending order for the renumbering pass.
+__global__ __launch_bounds__(BLOCK) void k_life_scan(Counters* ctr, int tiles, int compaction, int forced,
+                                                     const int32_t* dflag, const int32_t* dsum, int32_t* prefix,
+                                                     const int32_t* drop, const int32_t* xsum, int32_t* dprefix,
+                                                     uint32_t* dlist) {
+    const int n = ctr->count;
+    if (static_cast<int>(blockIdx.x) < tiles) {
+        if (ctr->births <= 0) return;
+        scan_tile(
+            n, static_cast<int>(blockIdx.x), [&](int i) { return load4_i32(dflag, i, n); },
+            [&](int i, int4 ex, int4) { store4_i32(prefix, i, n, ex); }, dsum, &ctr->div_total, &ctr->error_flags);
+    } else {
+        if (!compacting_now(ctr, compaction, forced)) return;
+        scan_tile(
+            n, static_cast<int>(blockIdx.x) - tiles, [&](int i) { return load4_i32(drop, i, n); },
+            [&](int i, int4 ex, int4 v) {
+                store4_i32(dprefix, i, n, ex);
+                if (v.x) dlist[ex.x] = static_cast<uint32_t>(i);
+                if (v.y) dlist[ex.y] = static_cast<uint32_t>(i + 1);
+                if (v.z) dlist[ex.z] = static_cast<uint32_t>(i + 2);
+                if (v.w) dlist[ex.w] = static_cast<uint32_t>(i + 3);
+            },
+            xsum, &ctr->drop_total, &ctr->error_flags);
     }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_divide_scan(Counters* ctr, const int32_t* dflag, int32_t* prefix,
-                                                       int32_t* total, ScanCtl sc) {
-    const int n = ctr->births > 0 ? ctr->count : 0;
-    if (n == 0) return;
-    scan_exclusive(
-        n, [&](int i) { return load4_i32(dflag, i, n); }, [&](int i, int4 ex) { store4_i32(prefix, i, n, ex); }, sc,
-        total);
-}
+// Dropped slots a block keeps in LDS for the renumbering binary search; with
+// more drops than this the pass reads the dense prefix instead.
+constexpr int DROP_LDS_MAX = 4096;
 
-// divide_kernel (lifecycle.cu:140-204). The daughter's reference slot is
-// count + prefix (the prefix taken over reference-slot order) and its id
-// next_id + prefix; physically it is appended after the last entry.
+// divide_kernel (lifecycle.cu:140-204) and the renumbering half of
+// cell_store_compact (cell_store.cu:414-456) in one pass over physical
+// entries.
+//   * A daughter's reference slot is count + prefix (the prefix over
+//     reference-slot order) and its id next_id + prefix; physically it is
+//     appended after the last entry. Compaction then moves it down by the
+//     number of dropped slots, all of which lie below count.
+//   * A survivor's slot moves down by the number of dropped slots below it; a
+//     dropped entry becomes a tombstone. Nothing is copied: storage order is
+//     the spatial order, and the next re-sort packs tombstones out.
 template <class P>
-__global__ __launch_bounds__(BLOCK) void k_divide(Cells<P> c, const Counters* ctr, const int32_t* prefix,
-                                                  TickParams t) {
-    if (ctr->births <= 0) return;
+__global__ __launch_bounds__(BLOCK) void k_life_apply(Cells<P> c, Counters* ctr, const int32_t* prefix,
+                                                      const int32_t* drop, const int32_t* dprefix,
+                                                      const uint32_t* dlist, TickParams t, int forced, int c_on,
+                                                      DemandArgs dm) {
+    __shared__ uint32_t s_drop[DROP_LDS_MAX];
     const int np = ctr->nphys;
     const int n = ctr->count;
-    const auto body = [&](const int i) {
-    const uint32_t f = c.flags[i];
-    if (!(f & ASTRO_FLAG_OCCUPIED) || !(f & ASTRO_FLAG_ALIVE)) return;
-    if (!(c.co2_held[i] >= t.quota)) return;
-    const int32_t pre = prefix[c.lslot[i]];
-    const int32_t lslot = n + pre;
-    if (lslot >= ctr->capacity) return;
-    const int32_t slot = np + pre;
-    c.lslot[slot] = static_cast<uint32_t>(lslot);
-    const uint64_t did = ctr->next_id + static_cast<uint64_t>(pre);
-    const double half0 = 0.5 * (c.energy[i] - ASTRO_LIFE_DIVISION_ENERGY_COST);
-    const double half = half0 > 0.0 ? half0 : 0.0;
-    c.energy[i] = half;
-    c.energy[slot] = half;
-    c.co2_held[i] = 0.0;
-    c.co2_held[slot] = 0.0;
-    c.biomass[slot] = ASTRO_CELL_MASS_DRY;
-    // division_axis (src/sim/lifecycle.cuh:87-99)
-    const uint64_t h = splitmix64(splitmix64(did));
-    const double u = static_cast<double>(h >> 33) * (1.0 / 2147483648.0) - 1.0;
-    const double phi = static_cast<double>(h & 0xFFFFFFFFull) * (ASTRO_TWO_PI / 4294967296.0);
-    const double s = sqrt(u * u < 1.0 ? 1.0 - u * u : 0.0);
-    const double ax = s * cos(phi), ay = s * sin(phi), az = u;
-    const double a = t.division_offset;
-    const double px = static_cast<double>(c.x[i]), py = static_cast<double>(c.y[i]), pz = static_cast<double>(c.z[i]);
-    double dx = px + a * ax, dy = py + a * ay, dz = pz + a * az;
-    double qx = px - a * ax, qy = py - a * ay, qz = pz - a * az;
-    const double hx = 0.5 * t.chamber_w - ASTRO_CELL_RADIUS;
-    const double hy = 0.5 * t.chamber_h - ASTRO_CELL_RADIUS;
-    const double hz = 0.5 * t.chamber_d - ASTRO_CELL_RADIUS;
-    auto cl = [](double v, double hh) { return v < -hh ? -hh : (v > hh ? hh : v); };
-    // In FP32 storage the reference's read-back of the stored daughter
-    // coordinate before clamping is a rounding of the same value.
-    dx = cl(static_cast<double>(static_cast<P>(dx)), hx);
-    dy = cl(static_cast<double>(static_cast<P>(dy)), hy);
-    dz = cl(static_cast<double>(static_cast<P>(dz)), hz);
-    qx = cl(static_cast<double>(static_cast<P>(qx)), hx);
-    qy = cl(static_cast<double>(static_cast<P>(qy)), hy);
-    qz = cl(static_cast<double>(static_cast<P>(qz)), hz);
-    c.x[slot] = static_cast<P>(dx); c.y[slot] = static_cast<P>(dy); c.z[slot] = static_cast<P>(dz);
-    c.x[i] = static_cast<P>(qx); c.y[i] = static_cast<P>(qy); c.z[i] = static_cast<P>(qz);
-    c.vx[slot] = c.vx[i]; c.vy[slot] = c.vy[i]; c.vz[slot] = c.vz[i];
-    c.id[slot] = did;
-    c.flags[slot] = f;
-    c.death_cause[slot] = ASTRO_DEATH_NONE;
-    c.temp_cell[slot] = c.temp_cell[i];
-    c.emit_power[slot] = 0.0f;
-    c.dir_x[slot] = c.dir_x[i]; c.dir_y[slot] = c.dir_y[i]; c.dir_z[slot] = c.dir_z[i];
-    c.age_s[slot] = 0.0f;
-    c.rng_state[slot] = pcg_split(c.rng_state[i], did);
-    c.taxis_memory[slot] = 0.0f;
-    c.run_timer[slot] = 0.0f;
-    c.t_local[slot] = c.t_local[i];
-    c.irradiance[slot] = 0.0f;
-    c.co2_local[slot] = c.co2_local[i];
-    c.n2_local[slot] = c.n2_local[i];
-    };
-    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < np; i += gridDim.x * BLOCK) body(i);
+    const int births = forced ? 0 : ctr->births;
+    const bool compacting = compacting_now(ctr, t.compaction, forced);
+    const int ndrop = compacting ? ctr->drop_total : 0;
+    const bool in_lds = ndrop <= DROP_LDS_MAX;
+    if (compacting && in_lds) {
+        for (int k = threadIdx.x; k < ndrop; k += BLOCK) s_drop[k] = dlist[k];
+        __syncthreads();
+    }
+    if (births > 0 || compacting || c_on) {
+        xcd_tiles(np, [&](const int i, const bool valid) {
+            if (!valid) return;
+            if (c_on) {
+                // The CO2 flush of shared voxels (lifecycle.cu:235) by their
+                // owners, and the demand cleared for the next tick.
+                const uint32_t v = dm.vox[i];
+                if (v != NO_VOXEL) {
+                    const uint32_t idx = v & ~VOX_OWNER;
+                    if (v & VOX_OWNER) {
+                        const unsigned long long raw = dm.C.deposit[idx];
+                        if (raw != 0ull) {
+                            dm.C.value[idx] += static_cast<float>(from_fixed(raw, dm.C.scale));
+                            dm.C.deposit[idx] = 0ull;
+                        }
+                    }
+                    dm.demand[idx] = 0ull;
+                }
+            }
+            const uint32_t s = c.lslot[i];
+            if (s == NO_SLOT) return;
+            uint32_t below = 0u;
+            if (compacting) {
+                bool dropped;
+                if (in_lds) {
+                    int lo = 0, hi = ndrop;   // first entry >= s
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (s_drop[mid] < s) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    dropped = lo < ndrop && s_drop[lo] == s;
+                    below = static_cast<uint32_t>(lo);
+                } else {
+                    dropped = drop[s] != 0;
+                    below = static_cast<uint32_t>(dprefix[s]);
+                }
+                if (dropped) {
+                    c.lslot[i] = NO_SLOT;
+                    c.flags[i] = 0u;
+                    return;
+                }
+                if (below) c.lslot[i] = s - below;
+            }
+            if (births <= 0) return;
+            const uint32_t f = c.flags[i];
+            if (!(f & ASTRO_FLAG_OCCUPIED) || !(f & ASTRO_FLAG_ALIVE)) return;
+            if (!(c.co2_held[i] >= t.quota)) return;
+            const int32_t pre = prefix[s];
+            const int32_t lslot = n + pre;
+            if (lslot >= ctr->capacity) return;
+            const int32_t slot = np + pre;
+            c.lslot[slot] = static_cast<uint32_t>(lslot - ndrop);
+            const uint64_t did = ctr->next_id + static_cast<uint64_t>(pre);
+            const double half0 = 0.5 * (c.energy[i] - ASTRO_LIFE_DIVISION_ENERGY_COST);
+            const double half = half0 > 0.0 ? half0 : 0.0;
+            c.energy[i] = half;
+            c.energy[slot] = half;
+            c.co2_held[i] = 0.0;
+            c.co2_held[slot] = 0.0;
+            c.biomass[slot] = ASTRO_CELL_MASS_DRY;
+            // division_axis (src/sim/lifecycle.cuh:87-99)
+            const uint64_t h = splitmix64(splitmix64(did));
+            const double u = static_cast<double>(h >> 33) * (1.0 / 2147483648.0) - 1.0;
+            const double phi = static_cast<double>(h & 0xFFFFFFFFull) * (ASTRO_TWO_PI / 4294967296.0);
+            const double sq = sqrt(u * u < 1.0 ? 1.0 - u * u : 0.0);
+            const double ax = sq * cos(phi), ay = sq * sin(phi), az = u;
+            const double a = t.division_offset;
+            const double px = static_cast<double>(c.x[i]), py = static_cast<double>(c.y[i]),
+                         pz = static_cast<double>(c.z[i]);
+            double dx = px + a * ax, dy = py + a * ay, dz = pz + a * az;
+            double qx = px - a * ax, qy = py - a * ay, qz = pz - a * az;
+            const double hx = 0.5 * t.chamber_w - ASTRO_CELL_RADIUS;
+            const double hy = 0.5 * t.chamber_h - ASTRO_CELL_RADIUS;
+            const double hz = 0.5 * t.chamber_d - ASTRO_CELL_RADIUS;
+            auto cl = [](double v, double hh) { return v < -hh ? -hh : (v > hh ? hh : v); };
+            // In FP32 storage the reference's read-back of the stored daughter
+            // coordinate before clamping is a rounding of the same value.
+            dx = cl(static_cast<double>(static_cast<P>(dx)), hx);
+            dy = cl(static_cast<double>(static_cast<P>(dy)), hy);
+            dz = cl(static_cast<double>(static_cast<P>(dz)), hz);
+            qx = cl(static_cast<double>(static_cast<P>(qx)), hx);
+            qy = cl(static_cast<double>(static_cast<P>(qy)), hy);
+            qz = cl(static_cast<double>(static_cast<P>(qz)), hz);
+            c.x[slot] = static_cast<P>(dx); c.y[slot] = static_cast<P>(dy); c.z[slot] = static_cast<P>(dz);
+            c.x[i] = static_cast<P>(qx); c.y[i] = static_cast<P>(qy); c.z[i] = static_cast<P>(qz);
+            c.vx[slot] = c.vx[i]; c.vy[slot] = c.vy[i]; c.vz[slot] = c.vz[i];
+            c.id[slot] = did;
+            c.flags[slot] = f;
+            c.death_cause[slot] = ASTRO_DEATH_NONE;
+            c.temp_cell[slot] = c.temp_cell[i];
+            c.emit_power[slot] = 0.0f;
+            c.dir_x[slot] = c.dir_x[i]; c.dir_y[slot] = c.dir_y[i]; c.dir_z[slot] = c.dir_z[i];
+            c.age_s[slot] = 0.0f;
+            c.rng_state[slot] = pcg_split(c.rng_state[i], did);
+            c.taxis_memory[slot] = 0.0f;
+            c.run_timer[slot] = 0.0f;
+            c.t_local[slot] = c.t_local[i];
+            c.irradiance[slot] = 0.0f;
+            c.co2_local[slot] = c.co2_local[i];
+            c.n2_local[slot] = c.n2_local[i];
+        });
+    }
 }
 
-// The host bookkeeping of lifecycle_step (lifecycle.cu:251-294), on device.
-__global__ void k_life_finalize(Counters* ctr, int compaction) {
+// The host bookkeeping of lifecycle_step and cell_store_compact
+// (lifecycle.cu:251-294, cell_store.cu:449-455), on device, after the apply
+// pass has read the counters.
+__global__ void k_life_finalize(Counters* ctr, int compaction, int forced) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const int32_t n = ctr->count, deads = ctr->deads;
-    int32_t births = ctr->births;
+    const int32_t n = ctr->count;
+    const bool compacting = compacting_now(ctr, compaction, forced);
+    const int32_t ndrop = compacting ? ctr->drop_total : 0;
+    if (forced) {
+        ctr->count = n - ndrop;
+        ctr->live = ctr->count;
+        return;
+    }
+    const int32_t deads = ctr->deads;
     const int32_t new_deaths = deads - ctr->dead_slots_prev;
     if (new_deaths > 0) ctr->deaths_total += new_deaths;
-    if (births > 0) {
-        if (n + births > ctr->capacity) births = ctr->capacity - n;
-        if (births > 0) {
-            ctr->divisions_total += births;
-            ctr->next_id += static_cast<uint64_t>(births);
-            ctr->count = n + births;
-            ctr->nphys += births;
+    int32_t b = ctr->births;
+    if (b > 0) {
+        if (n + b > ctr->capacity) b = ctr->capacity - n;
+        if (b > 0) {
+            ctr->divisions_total += b;
+            ctr->next_id += static_cast<uint64_t>(b);
+            ctr->nphys += b;
         }
     }
-    const bool compact = compaction && deads > 0;
-    ctr->compacting = compact ? 1 : 0;
-    ctr->dead_slots_prev = compact ? 0 : deads;
+    if (b < 0) b = 0;
+    ctr->count = n + b - ndrop;
+    if (compacting) ctr->live = ctr->count;
+    ctr->dead_slots_prev = compacting ? 0 : deads;
     ctr->births = 0;
     ctr->deads = 0;
 }
 
-// ---------------------------------------------------------------------------
-// cell_store_compact (src/sim/cell_store.cu:414-456): survivors (occupied &&
-// alive) keep their order and are renumbered 0..live-1 by an exclusive scan
-// of the drop flags over reference-slot order; dropped entries become
-// tombstones. Nothing is copied: storage order is the spatial sort order, and
-// the next re-sort packs the tombstones out. All kernels return at once
-// unless `compacting` is set.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BLOCK) void k_compact_scan(Counters* ctr, const int32_t* drop, int32_t* dprefix,
-                                                        int32_t* total, ScanCtl sc) {
-    const int n = ctr->compacting ? ctr->count : 0;
-    if (n == 0) return;
-    scan_exclusive(
-        n, [&](int i) { return load4_i32(drop, i, n); }, [&](int i, int4 ex) { store4_i32(dprefix, i, n, ex); }, sc,
-        total);
-}
-
+// A direct compaction request (cell_store_compact outside lifecycle): every
+// live entry writes its own slot's drop flag and tile total afresh.
 template <class P>
-__global__ __launch_bounds__(BLOCK) void k_compact_renumber(Cells<P> c, const Counters* ctr, const int32_t* drop,
-                                                            const int32_t* dprefix) {
-    if (!ctr->compacting) return;
+__global__ __launch_bounds__(BLOCK) void k_compact_mark(Cells<P> c, const Counters* ctr, int32_t* drop, int32_t* xsum) {
     const int np = ctr->nphys;
-    const auto body = [&](const int i) {
-    const uint32_t s = c.lslot[i];
-    if (s == NO_SLOT) return;
-    if (drop[s]) {
-        c.lslot[i] = NO_SLOT;
-        c.flags[i] = 0u;
-    } else {
-        c.lslot[i] = s - static_cast<uint32_t>(dprefix[s]);
-    }
-    };
-    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < np; i += gridDim.x * BLOCK) body(i);
-}
-
-__global__ void k_compact_finalize(Counters* ctr, const int32_t* total) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (ctr->compacting) {
-        ctr->count -= *total;
-        ctr->live = ctr->count;
-        ctr->compacting = 0;
-    }
-}
-
-// A direct compaction request (cell_store_compact outside lifecycle): mark
-// the drop set afresh -- every live entry writes its own slot's flag -- and
-// run the same three passes.
-template <class P>
-__global__ __launch_bounds__(BLOCK) void k_compact_mark(Cells<P> c, Counters* ctr, int32_t* drop) {
-    const int np = ctr->nphys;
-    const int gid = blockIdx.x * BLOCK + threadIdx.x;
-    if (gid == 0) ctr->compacting = 1;
-    for (int i = gid; i < np; i += gridDim.x * BLOCK) {
+    xcd_tiles(np, [&](const int i, const bool valid) {
+        if (!valid) return;
         const uint32_t s = c.lslot[i];
-        if (s == NO_SLOT) continue;
+        if (s == NO_SLOT) return;
         const uint32_t f = c.flags[i];
-        drop[s] = ((f & ASTRO_FLAG_OCCUPIED) && (f & ASTRO_FLAG_ALIVE)) ? 0 : 1;
-    }
+        const bool keep = (f & ASTRO_FLAG_OCCUPIED) && (f & ASTRO_FLAG_ALIVE);
+        drop[s] = keep ? 0 : 1;
+        if (!keep) atomicAdd(&xsum[s >> SCAN_TILE_SHIFT], 1);
+    });
 }
 
 // ---------------------------------------------------------------------------
-// stats_cells_kernel (src/sim/stats.cu:27-43): per-element fixed point, wave
-// reduction, one atomic per wave and word.
+// stats_cells_kernel (src/sim/stats.cu:27-43): per-element fixed point, summed
+// in registers over the thread's cells, then one block reduction and one
+// atomic per block and word (the sums are exact integers, so the order of
+// the additions does not matter).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
 #pragma unroll
@@ -668,31 +648,42 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
     return v;
 }
 
+constexpr int STATS_WORDS = 6;
+
 template <class P>
 __global__ __launch_bounds__(BLOCK) void k_stats_cells(Cells<P> c, const Counters* ctr, unsigned long long* acc) {
+    __shared__ unsigned long long s_part[BLOCK / 64][STATS_WORDS];
     const int n = ctr->nphys;
-    for (int base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
-    const int i = base + threadIdx.x;
-    unsigned long long e = 0, tc = 0, na = 0, nd = 0, nw = 0, nm = 0;
-    if (i < n) {
+    unsigned long long v[STATS_WORDS] = {0, 0, 0, 0, 0, 0};
+    xcd_tiles(n, [&](const int i, const bool valid) {
+        if (!valid) return;
         const uint32_t f = c.flags[i];
-        if (f & ASTRO_FLAG_OCCUPIED) {
-            const bool alive = (f & ASTRO_FLAG_ALIVE) != 0u, awake = (f & ASTRO_FLAG_AWAKE) != 0u;
-            na = alive; nd = !alive; nw = awake; nm = !awake;
-            e = static_cast<unsigned long long>(to_fixed(c.energy[i], ASTRO_DEPOSIT_SCALE_STATS));
-            if (alive) tc = static_cast<unsigned long long>(to_fixed(static_cast<double>(c.temp_cell[i]), ASTRO_DEPOSIT_SCALE_STATS));
+        if (!(f & ASTRO_FLAG_OCCUPIED)) return;
+        const bool alive = (f & ASTRO_FLAG_ALIVE) != 0u, awake = (f & ASTRO_FLAG_AWAKE) != 0u;
+        v[0] += static_cast<unsigned long long>(to_fixed(c.energy[i], ASTRO_DEPOSIT_SCALE_STATS));
+        if (alive)
+            v[1] += static_cast<unsigned long long>(
+                to_fixed(static_cast<double>(c.temp_cell[i]), ASTRO_DEPOSIT_SCALE_STATS));
+        v[2] += alive; v[3] += !alive; v[4] += awake; v[5] += !awake;
+    });
+#pragma unroll
+    for (int k = 0; k < STATS_WORDS; ++k) v[k] = wave_sum_u64(v[k]);
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < STATS_WORDS; ++k) s_part[threadIdx.x >> 6][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x < STATS_WORDS) {
+        unsigned long long t = 0;
+        for (int w = 0; w < BLOCK / 64; ++w) t += s_part[w][threadIdx.x];
+        int word = SW_ENERGY;
+        switch (threadIdx.x) {
+            case 1: word = SW_TEMP_CELL; break;
+            case 2: word = SW_N_ALIVE; break;
+            case 3: word = SW_N_DEAD; break;
+            case 4: word = SW_N_AWAKE; break;
+            case 5: word = SW_N_DORMANT; break;
+            default: break;
         }
-    }
-    e = wave_sum_u64(e); tc = wave_sum_u64(tc); na = wave_sum_u64(na);
-    nd = wave_sum_u64(nd); nw = wave_sum_u64(nw); nm = wave_sum_u64(nm);
-    if ((threadIdx.x & 63) == 0) {
-        if (e) atomicAdd(&acc[SW_ENERGY], e);
-        if (tc) atomicAdd(&acc[SW_TEMP_CELL], tc);
-        if (na) atomicAdd(&acc[SW_N_ALIVE], na);
-        if (nd) atomicAdd(&acc[SW_N_DEAD], nd);
-        if (nw) atomicAdd(&acc[SW_N_AWAKE], nw);
-        if (nm) atomicAdd(&acc[SW_N_DORMANT], nm);
-    }
+        if (t) atomicAdd(&acc[word], t);
     }
 }
 
@@ -717,21 +708,21 @@ void launch_spawn(const Launch& L, const WorldDev<P>& w, const SpawnArgs& a, int
 template <class P>
 void launch_thermal_substep(const Launch& L, const WorldDev<P>& w, double dt_sub, double cap, bool update_rec) {
     PROF_BEGIN("thermal");
-    k_thermal<P><<<stride_grid(w.pcap), BLOCK, 0, L.stream>>>(w.c, w.ctr, w.g[ASTRO_FIELD_TEMPERATURE], dt_sub, cap,
-                                                               w.rec, w.spos, update_rec ? 1 : 0);
+    k_thermal<P><<<cell_grid(w.pcap), BLOCK, 0, L.stream>>>(w.c, w.ctr, w.g[ASTRO_FIELD_TEMPERATURE], dt_sub, cap,
+                                                             w.rec, w.spos, update_rec ? 1 : 0);
     PROF_END();
 }
 
 template <class P>
 void launch_apply_awake(const Launch& L, const WorldDev<P>& w, int field) {
     PROF_BEGIN("deposit_apply");
-    k_apply_awake<P><<<stride_grid(w.pcap), BLOCK, 0, L.stream>>>(w.c, w.ctr, w.g[field]);
+    k_apply_awake<P><<<cell_grid(w.pcap), BLOCK, 0, L.stream>>>(w.c, w.ctr, w.g[field]);
     PROF_END();
 }
 
 template <class P>
 void launch_feed_taxis(const Launch& L, const WorldDev<P>& w, const TickParams& t, bool feed, bool taxis) {
-    const int g = stride_grid(w.pcap);
+    const int g = cell_grid(w.pcap);
     PROF_BEGIN(feed ? (taxis ? "feed_taxis" : "feed") : "taxis");
     if (feed && taxis)
         k_feed_taxis<P, true, true><<<g, BLOCK, 0, L.stream>>>(w.c, w.ctr, w.spos, w.dirs, t.ambient_irr, t.dt,
@@ -746,79 +737,93 @@ void launch_feed_taxis(const Launch& L, const WorldDev<P>& w, const TickParams& 
 }
 
 template <class P>
-void launch_sample_base(const Launch& L, const WorldDev<P>& w, const TickParams& t) {
+void launch_cell_forces(const Launch& L, const WorldDev<P>& w, const TickParams& t, bool feed, bool taxis) {
+    if (feed || taxis) launch_feed_taxis<P>(L, w, t, feed, taxis);
     PROF_BEGIN("sample_base");
-    k_sample_base<P><<<stride_grid(w.pcap), BLOCK, 0, L.stream>>>(
+    k_sample_base<P><<<cell_grid(w.pcap), BLOCK, 0, L.stream>>>(
         w.c, w.ctr, w.g[ASTRO_FIELD_TEMPERATURE], w.g[ASTRO_FIELD_CO2], w.g[ASTRO_FIELD_N2],
         w.on[ASTRO_FIELD_CO2] ? 1 : 0, w.on[ASTRO_FIELD_N2] ? 1 : 0, t.gravity_axis, w.spos, w.fs);
     PROF_END();
+    launch_contact_walk<P>(L, w, t);
 }
 
 template <class P>
-void launch_integrate(const Launch& L, const WorldDev<P>& w, const TickParams& t) {
-    PROF_BEGIN("integrate");
-    k_integrate<P><<<stride_grid(w.pcap), BLOCK, 0, L.stream>>>(w.c, w.ctr, w.spos, w.fs, t);
-    PROF_END();
+DemandArgs demand_args(const WorldDev<P>& w, const TickParams& t) {
+    return DemandArgs{w.g[ASTRO_FIELD_CO2], t.dt_bio, t.co2_gv, w.co2_demand, w.rin, w.co2_avail};
 }
 
 template <class P>
-void launch_lifecycle(const Launch& L, const WorldDev<P>& w, const TickParams& t) {
-    const int g = stride_grid(w.pcap);
-    const bool c_on = w.on[ASTRO_FIELD_CO2];
-    if (c_on) {
-        PROF_BEGIN("co2_demand");
-        k_demand<P><<<g, BLOCK, 0, L.stream>>>(w.c, w.ctr, w.g[ASTRO_FIELD_CO2], t.dt_bio, t.co2_gv, w.co2_demand,
-                                                w.rin);
-        PROF_END();
-        PROF_BEGIN("co2_uptake");
-        k_uptake<P><<<g, BLOCK, 0, L.stream>>>(w.c, w.ctr, w.g[ASTRO_FIELD_CO2], t.dt_bio, t.co2_gv, w.co2_demand,
-                                                w.rin);
-        PROF_END();
+void launch_integrate(const Launch& L, const WorldDev<P>& w, const TickParams& t, bool demand) {
+    const DemandArgs dm = demand_args(w, t);
+    if (demand) {
+        PROF_BEGIN("integrate_demand");
+        k_integrate<P, true><<<cell_grid(w.pcap), BLOCK, 0, L.stream>>>(w.c, w.ctr, w.spos, w.fs, t, dm);
+    } else {
+        PROF_BEGIN("integrate");
+        k_integrate<P, false><<<cell_grid(w.pcap), BLOCK, 0, L.stream>>>(w.c, w.ctr, w.spos, w.fs, t, dm);
     }
-    PROF_BEGIN("life_mark");
-    k_life_mark<P><<<g, BLOCK, 0, L.stream>>>(w.c, w.ctr, w.g[ASTRO_FIELD_CO2], c_on ? 1 : 0, w.co2_demand, w.rin,
-                                               t.store_disposition, t.quota, w.dflag, w.drop);
     PROF_END();
-    PROF_BEGIN("divide_scan");
-    k_divide_scan<<<scan_tiles_for(w.capacity), BLOCK, 0, L.stream>>>(w.ctr, w.dflag, w.prefix, w.scan_total, w.sc);
+}
+
+// The division / compaction tail shared by the lifecycle and a direct
+// compaction: both scans, then the apply pass with its bookkeeping.
+template <class P>
+void launch_life_tail(const Launch& L, const WorldDev<P>& w, const TickParams& t, bool forced) {
+    const int tiles = scan_tiles_for(w.capacity);
+    const int c_on = (!forced && w.on[ASTRO_FIELD_CO2]) ? 1 : 0;
+    PROF_BEGIN("life_scan");
+    k_life_scan<<<2 * tiles, BLOCK, 0, L.stream>>>(w.ctr, tiles, t.compaction, forced ? 1 : 0, w.dflag, w.dsum,
+                                                  w.prefix, w.drop, w.xsum, w.dprefix, w.dlist);
     PROF_END();
-    PROF_BEGIN("divide");
-    k_divide<P><<<g, BLOCK, 0, L.stream>>>(w.c, w.ctr, w.prefix, t);
+    PROF_BEGIN("life_apply");
+    k_life_apply<P><<<cell_grid(w.pcap), BLOCK, 0, L.stream>>>(w.c, w.ctr, w.prefix, w.drop, w.dprefix, w.dlist, t,
+                                                                forced ? 1 : 0, c_on, demand_args(w, t));
     PROF_END();
     PROF_BEGIN("life_finalize");
-    k_life_finalize<<<1, 64, 0, L.stream>>>(w.ctr, t.compaction);
+    k_life_finalize<<<1, 64, 0, L.stream>>>(w.ctr, t.compaction, forced ? 1 : 0);
     PROF_END();
-    if (t.compaction) launch_compact<P>(L, w, false);
+}
+
+template <class P>
+void launch_lifecycle(const Launch& L, const WorldDev<P>& w, const TickParams& t, bool demand_booked) {
+    const int g = cell_grid(w.pcap);
+    const bool c_on = w.on[ASTRO_FIELD_CO2];
+    const DemandArgs dm = demand_args(w, t);
+    if (c_on && !demand_booked) {
+        PROF_BEGIN("co2_demand");
+        k_demand<P><<<g, BLOCK, 0, L.stream>>>(w.c, w.ctr, dm);
+        PROF_END();
+    }
+    PROF_BEGIN("uptake_mark");
+    k_uptake_mark<P><<<g, BLOCK, 0, L.stream>>>(w.c, w.ctr, dm, c_on ? 1 : 0, t.store_disposition, t.quota, w.dflag,
+                                                 w.dsum, w.drop, w.xsum);
+    PROF_END();
+    launch_life_tail<P>(L, w, t, false);
 }
 
 template <class P>
 void launch_life_clear(const Launch& L, const WorldDev<P>& w) {
+    const size_t tiles = static_cast<size_t>(scan_tiles_for(w.capacity));
     hipMemsetAsync(w.dflag, 0, sizeof(int32_t) * w.capacity, L.stream);
     hipMemsetAsync(w.drop, 0, sizeof(int32_t) * w.capacity, L.stream);
+    hipMemsetAsync(w.dsum, 0, sizeof(int32_t) * tiles, L.stream);
+    hipMemsetAsync(w.xsum, 0, sizeof(int32_t) * tiles, L.stream);
 }
 
 template <class P>
-void launch_compact(const Launch& L, const WorldDev<P>& w, bool forced) {
-    const int g = stride_grid(w.pcap);
-    if (forced) {
-        PROF_BEGIN("compact_mark");
-        k_compact_mark<P><<<g, BLOCK, 0, L.stream>>>(w.c, w.ctr, w.drop);
-        PROF_END();
-    }
-    PROF_BEGIN("compact_scan");
-    k_compact_scan<<<scan_tiles_for(w.capacity), BLOCK, 0, L.stream>>>(w.ctr, w.drop, w.dprefix, w.scan_total, w.sc);
+void launch_compact(const Launch& L, const WorldDev<P>& w) {
+    launch_life_clear<P>(L, w);
+    PROF_BEGIN("compact_mark");
+    k_compact_mark<P><<<cell_grid(w.pcap), BLOCK, 0, L.stream>>>(w.c, w.ctr, w.drop, w.xsum);
     PROF_END();
-    PROF_BEGIN("compact_renumber");
-    k_compact_renumber<P><<<g, BLOCK, 0, L.stream>>>(w.c, w.ctr, w.drop, w.dprefix);
-    PROF_END();
-    PROF_BEGIN("compact_finalize");
-    k_compact_finalize<<<1, 64, 0, L.stream>>>(w.ctr, w.scan_total);
-    PROF_END();
+    TickParams t{};
+    launch_life_tail<P>(L, w, t, true);
 }
 
 template <class P>
 void launch_stats_cells(const Launch& L, const WorldDev<P>& w) {
-    k_stats_cells<P><<<stride_grid(w.pcap), BLOCK, 0, L.stream>>>(w.c, w.ctr, w.stats);
+    const int g = std::min(cell_grid(w.pcap), 512);
+    k_stats_cells<P><<<g, BLOCK, 0, L.stream>>>(w.c, w.ctr, w.stats);
 }
 
 #define INSTANTIATE(P)                                                                                          \
@@ -827,10 +832,10 @@ void launch_stats_cells(const Launch& L, const WorldDev<P>& w) {
     template void launch_thermal_substep<P>(const Launch&, const WorldDev<P>&, double, double, bool);          \
     template void launch_apply_awake<P>(const Launch&, const WorldDev<P>&, int);                                \
     template void launch_feed_taxis<P>(const Launch&, const WorldDev<P>&, const TickParams&, bool, bool);       \
-    template void launch_sample_base<P>(const Launch&, const WorldDev<P>&, const TickParams&);                  \
-    template void launch_integrate<P>(const Launch&, const WorldDev<P>&, const TickParams&);                    \
-    template void launch_lifecycle<P>(const Launch&, const WorldDev<P>&, const TickParams&);                    \
-    template void launch_compact<P>(const Launch&, const WorldDev<P>&, bool);                                    \
+    template void launch_integrate<P>(const Launch&, const WorldDev<P>&, const TickParams&, bool);              \
+    template void launch_cell_forces<P>(const Launch&, const WorldDev<P>&, const TickParams&, bool, bool);      \
+    template void launch_lifecycle<P>(const Launch&, const WorldDev<P>&, const TickParams&, bool);              \
+    template void launch_compact<P>(const Launch&, const WorldDev<P>&);                                          \
     template void launch_life_clear<P>(const Launch&, const WorldDev<P>&);                                      \
     template void launch_stats_cells<P>(const Launch&, const WorldDev<P>&);
 

@@ -5,8 +5,9 @@
 // key *is* the sort digit, so the build is a counting sort over the bucket
 // index (a one-digit LSD radix sort):
 //   1. key:     bucket of each slot; arrival rank = atomicAdd on its count.
-//   2. scan:    dense exclusive prefix of the counts (look-back scan); the
-//               counts are zeroed in the same pass for the next build.
+//   2. scan:    dense exclusive prefix of the counts (one block per tile,
+//               from per-tile totals the key pass accumulated); the counts
+//               are zeroed in the same pass for the next build.
 //   3. scatter: records to bucket base + arrival rank (order inside a bucket
 //               is still arrival order, i.e. scheduling dependent).
 //   4. rank:    each record re-ranks itself by slot inside its bucket, which
@@ -28,8 +29,9 @@ namespace {
 
 template <class P>
 __global__ __launch_bounds__(BLOCK) void k_hash_key(Cells<P> c, Counters* ctr, HashGeo h, uint32_t* keys,
-                                                    uint32_t* rin, uint32_t* bcount, int32_t* dflag, int32_t* drop,
-                                                    int cap, int count_tick, int resort_period) {
+                                                    uint32_t* rin, uint32_t* bcount, int32_t* bsum, int32_t* dflag,
+                                                    int32_t* drop, int32_t* dsum, int32_t* xsum, int cap,
+                                                    int count_tick, int resort_period) {
     const int np = ctr->nphys;
     const int gid = blockIdx.x * BLOCK + threadIdx.x, stride = gridDim.x * BLOCK;
     if (count_tick) {
@@ -40,33 +42,43 @@ __global__ __launch_bounds__(BLOCK) void k_hash_key(Cells<P> c, Counters* ctr, H
             ctr->resort = (t % static_cast<uint32_t>(resort_period) == 0u || np > cap) ? 1 : 0;
             ctr->tick_ctr = t + 1u;
         }
-        // The lifecycle's per-slot flags start each tick cleared.
+        // The lifecycle's per-slot flags and their scan-tile totals start
+        // each tick cleared.
         for (int q = gid; q < cap; q += stride) {
             dflag[q] = 0;
             drop[q] = 0;
         }
+        for (int q = gid; q < scan_tiles_for(cap); q += stride) {
+            dsum[q] = 0;
+            xsum[q] = 0;
+        }
     }
-    for (int p = gid; p < np; p += stride) {
-        if (c.lslot[p] == NO_SLOT) continue;
-        const uint32_t f = c.flags[p];
-        const uint32_t key = (f & ASTRO_FLAG_OCCUPIED)
-            ? hash_key(h, static_cast<double>(c.x[p]), static_cast<double>(c.y[p]), static_cast<double>(c.z[p]))
-            : static_cast<uint32_t>(h.nb);
-        keys[p] = key;
-        rin[p] = atomicAdd(&bcount[key], 1u);
-    }
+    xcd_tiles(np, [&](const int p, const bool valid) {
+        const bool live = valid && c.lslot[p] != NO_SLOT;
+        uint32_t key = 0u;
+        if (live) {
+            const uint32_t f = c.flags[p];
+            key = (f & ASTRO_FLAG_OCCUPIED)
+                ? hash_key(h, static_cast<double>(c.x[p]), static_cast<double>(c.y[p]), static_cast<double>(c.z[p]))
+                : static_cast<uint32_t>(h.nb);
+            keys[p] = key;
+            rin[p] = atomicAdd(&bcount[key], 1u);
+        }
+        wave_count_by_key(bsum, key >> SCAN_TILE_SHIFT, live);
+    });
 }
 
-__global__ __launch_bounds__(BLOCK) void k_bucket_scan(uint32_t* bcount, int32_t* bstart, int nbp1, ScanCtl sc) {
+__global__ __launch_bounds__(BLOCK) void k_bucket_scan(Counters* ctr, uint32_t* bcount, const int32_t* bsum,
+                                                       int32_t* bstart, int nbp1) {
     int32_t* counts = reinterpret_cast<int32_t*>(bcount);
-    scan_exclusive(
-        nbp1,
+    scan_tile(
+        nbp1, static_cast<int>(blockIdx.x),
         [&](int i) {
             const int4 v = load4_i32(counts, i, nbp1);
             if (v.x | v.y | v.z | v.w) store4_i32(counts, i, nbp1, int4{0, 0, 0, 0});
             return v;
         },
-        [&](int i, int4 ex) { store4_i32(bstart, i, nbp1, ex); }, sc, bstart + nbp1);
+        [&](int i, int4 ex, int4) { store4_i32(bstart, i, nbp1, ex); }, bsum, bstart + nbp1, &ctr->error_flags);
 }
 
 template <class P>
@@ -88,15 +100,19 @@ __global__ __launch_bounds__(BLOCK) void k_hash_scatter(Cells<P> c, const Counte
     trec[dst] = r;
     tphys[dst] = static_cast<uint32_t>(p);
     };
-    for (int p = blockIdx.x * BLOCK + threadIdx.x; p < np; p += gridDim.x * BLOCK) body(p);
+    xcd_tiles(np, [&](const int p, const bool valid) { if (valid) body(p); });
 }
 
 // Stable order: ascending reference slot inside each bucket.
 template <class P>
 __global__ __launch_bounds__(BLOCK) void k_hash_rank(const Counters* ctr, HashGeo h, const Rec<P>* trec,
                                                      const uint32_t* tphys, const int32_t* bstart, Rec<P>* rec,
-                                                     uint32_t* vals_sorted, uint32_t* sphys, int32_t* spos) {
+                                                     uint32_t* vals_sorted, uint32_t* sphys, int32_t* spos,
+                                                     int32_t* bsum, int bsum_tiles) {
     const int n = ctr->count;
+    // The bucket scan has consumed the tile totals; clear them for the next
+    // build's key pass.
+    for (int q = blockIdx.x * BLOCK + threadIdx.x; q < bsum_tiles; q += gridDim.x * BLOCK) bsum[q] = 0;
     const auto body = [&](const int k) {
     const Rec<P> r = trec[k];
     const uint32_t slot = r.meta & REC_SLOT_MASK;
@@ -130,10 +146,13 @@ __global__ __launch_bounds__(BLOCK) void k_resort_gather(Cells<P> a, Cells<P> b,
 }
 
 template <class P>
-__global__ __launch_bounds__(BLOCK) void k_resort_copyback(Cells<P> a, Cells<P> b, const Counters* ctr,
-                                                           uint32_t* sphys, int32_t* spos) {
+__global__ __launch_bounds__(BLOCK) void k_resort_copyback(Cells<P> a, Cells<P> b, Counters* ctr, uint32_t* sphys,
+                                                           int32_t* spos) {
     if (!ctr->resort) return;
     const int n = ctr->count;
+    // No block of this pass reads nphys, so one thread can already publish
+    // the packed extent for the kernels that follow.
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr->nphys = n;
     const auto body = [&](const int k) {
     copy_cell(b, k, a, k);
     sphys[k] = static_cast<uint32_t>(k);
@@ -142,88 +161,181 @@ __global__ __launch_bounds__(BLOCK) void k_resort_copyback(Cells<P> a, Cells<P> 
     for (int k = blockIdx.x * BLOCK + threadIdx.x; k < n; k += gridDim.x * BLOCK) body(k);
 }
 
-__global__ void k_resort_finalize(Counters* ctr) {
-    if (threadIdx.x == 0 && blockIdx.x == 0 && ctr->resort) {
-        ctr->nphys = ctr->count;
-        ctr->resort = 0;
-    }
-}
+// The record ranges of the 27 buckets around bucket (gx, gy, gz), as nine
+// rows in the reference's walk order (z, then y, then x, then sorted position,
+// src/sim/hash.cuh:88-111). The three x-adjacent buckets of one (y, z) row are
+// contiguous in the sorted array, so each row is a single range
+// [bstart[x_lo], bstart[x_hi + 1]); all eighteen loads are issued together.
+struct RowRanges {
+    int32_t s[9], e[9];
+};
 
-// Walks the 27 buckets around (px,py,pz) in the reference order (z, then y,
-// then x, then sorted position, src/sim/hash.cuh:88-111). The three x-adjacent
-// buckets of one (y,z) row are contiguous in the sorted array, so each row is
-// a single range [bstart[x_lo], bstart[x_hi + 1]).
-template <class P, class Fn>
-__device__ __forceinline__ void walk_rows(const HashGeo& h, const int32_t* bstart, const Rec<P>* rec,
-                                          double px, double py, double pz, Fn&& fn) {
-    int gx, gy, gz;
-    hash_coords(h, px, py, pz, gx, gy, gz);
+template <class Start>
+__device__ __forceinline__ RowRanges row_ranges(const HashGeo& h, Start&& start, int gx, int gy, int gz) {
+    RowRanges rr;
     const int xlo = gx > 0 ? gx - 1 : 0;
     const int xhi = gx < h.nx - 1 ? gx + 1 : h.nx - 1;
-    // All nine row ranges, then each row's first record, are requested before
-    // any is used: neighbourhoods hold about one record per row, so this turns
-    // a chain of ~20 dependent loads into three rounds.
-    int32_t s[9], e[9];
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
         const int cz = gz + r / 3 - 1, cy = gy + r % 3 - 1;
         const bool ok = cz >= 0 && cz < h.nz && cy >= 0 && cy < h.ny;
         const int row = ok ? (cz * h.ny + cy) * h.nx : 0;
-        s[r] = ok ? bstart[row + xlo] : 0;
-        e[r] = ok ? bstart[row + xhi + 1] : 0;
+        rr.s[r] = ok ? start(r / 3, row + xlo) : 0;
+        rr.e[r] = ok ? start(r / 3, row + xhi + 1) : 0;
     }
-    Rec<P> first[9];
-#pragma unroll
-    for (int r = 0; r < 9; ++r)
-        if (s[r] < e[r]) first[r] = rec[s[r]];
-#pragma unroll
-    for (int r = 0; r < 9; ++r) {
-        if (s[r] >= e[r]) continue;
-        fn(first[r]);
-        for (int32_t q = s[r] + 1; q < e[r]; ++q) fn(rec[q]);
-    }
+    return rr;
 }
 
-// forces_kernel's contact half (src/sim/integrator.cu:60-67), one thread per
-// sorted record, starting from the buoyancy + recoil force already in fs.
+// visit(d, q) for every sorted record index q of the walk, in order; d is the
+// row's z-offset range (0, 1, 2 for dz = -1, 0, +1).
+template <class Visit>
+__device__ __forceinline__ void walk_rows(const RowRanges& rr, Visit&& visit) {
+#pragma unroll
+    for (int r = 0; r < 9; ++r)
+        for (int32_t q = rr.s[r]; q < rr.e[r]; ++q) visit(r / 3, q);
+}
+
+// Position difference, as the float the contact screen uses: exact to one
+// float rounding of the double difference the reference computes.
+__device__ __forceinline__ float screen_diff(float a, float b) { return a - b; }
+__device__ __forceinline__ float screen_diff(double a, double b) { return static_cast<float>(a - b); }
+
+// Possible contacts a thread queues before the exact pass (the queue is
+// drained early, in order, if it fills).
+constexpr int CONTACT_QUEUE = 4;
+
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = min(v, __shfl_xor(v, d, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = max(v, __shfl_xor(v, d, 64));
+    return v;
+}
+// Orders one wave's LDS writes before its later LDS reads by other lanes.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The contact half of forces_kernel (src/sim/integrator.cu:60-67), one thread
+// per sorted record, adding to the body force already in fs[k]. Neighbour
+// positions come from the hash's records, fixed since the hash build.
+//   * Staging, per wave and without block barriers: the 64 consecutive
+//     records of a wave have keys in [kmin, kmax], and keys are linear in
+//     (z, y, x), so row r of every lane's walk lies in one contiguous record
+//     range (about 64 records). For each z offset the wave copies its three
+//     rows' ranges into its own LDS slice with coalesced loads, then every
+//     lane walks there. A range that does not fit is walked in global memory;
+//     the visit order is the same either way.
+//   * Each neighbour is first screened in float: the float r^2 of a pair is
+//     within a few float roundings (relative 2^-21) of the double r^2, so a
+//     2^-10 margin keeps every pair the reference counts as touching. Only the
+//     survivors -- about one per cell -- take the exact double-precision
+//     contact_force, in walk order, so F sums the same terms in the same order
+//     as the reference. Pairs the reference finds apart add a zero vector
+//     there; F's components are sums of nonzero terms onto a nonzero or +0.0
+//     start and so never -0.0, which makes those adds exact no-ops.
 template <class P>
-__global__ __launch_bounds__(BLOCK) void k_forces_walk(const Counters* ctr, HashGeo h, const Rec<P>* rec,
-                                                       const int32_t* bstart, Force* fs, double k_contact) {
+__global__ __launch_bounds__(BLOCK) void k_contact_walk(const Counters* ctr, HashGeo h, const Rec<P>* rec,
+                                                        const int32_t* bstart, Force* fs, double k_contact) {
+    constexpr int SEG = (6 * 1024) / static_cast<int>(sizeof(Rec<P>));   // records per wave and z offset
+    __shared__ Rec<P> s_seg[BLOCK / 64][SEG];
+    __shared__ int32_t s_queue[CONTACT_QUEUE * BLOCK];
+    Rec<P>* seg = s_seg[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
     const int n = ctr->count;
-    const int lb = xcd_tile(blockIdx.x, gridDim.x);
-    const auto body = [&](const int k) {
-    const Rec<P> r = rec[k];
-    if (r.meta & REC_UNOCC) return;
-    const uint32_t self = r.meta & REC_SLOT_MASK;
-    const double px = static_cast<double>(r.x), py = static_cast<double>(r.y), pz = static_cast<double>(r.z);
-    Force F = fs[k];
     const double two_a = ASTRO_CELL_DIAMETER;
-    walk_rows<P>(h, bstart, rec, px, py, pz, [&](const Rec<P>& o) {
-        if ((o.meta & REC_SLOT_MASK) == self) return;
-        // contact_force (src/sim/integrator.cuh:210-219)
-        const double dx = px - static_cast<double>(o.x);
-        const double dy = py - static_cast<double>(o.y);
-        const double dz = pz - static_cast<double>(o.z);
-        const double r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 >= two_a * two_a || r2 <= 0.0) {
-            F.x += 0.0;
-            F.y += 0.0;
-            F.z += 0.0;
-            return;
+    const float screen = static_cast<float>(two_a * two_a) * (1.0f + 1.0f / 1024.0f);
+    xcd_tiles(n, [&](const int k, const bool valid) {
+        Rec<P> r{};
+        bool occ = false;
+        int gx = 0, gy = 0, gz = 0;
+        if (valid) {
+            r = rec[k];
+            occ = !(r.meta & REC_UNOCC);
+            if (occ)
+                hash_coords(h, static_cast<double>(r.x), static_cast<double>(r.y), static_cast<double>(r.z), gx, gy,
+                            gz);
         }
-        const double rr = sqrt(r2);
-        const double overlap = two_a - rr;
-        const double s = k_contact * overlap / rr;
-        F.x += dx * s;
-        F.y += dy * s;
-        F.z += dz * s;
+        RowRanges rr{};
+        if (occ) rr = row_ranges(h, [&](int, int b) { return bstart[b]; }, gx, gy, gz);
+        const uint32_t self = r.meta & REC_SLOT_MASK;
+        int32_t* queue = s_queue + threadIdx.x;   // entry j at queue[j * BLOCK]
+        int queued = 0;
+        Force F{};
+        bool loaded = false;
+        const auto drain = [&]() {
+            if (!loaded) {
+                F = fs[k];
+                loaded = true;
+            }
+            const double px = static_cast<double>(r.x), py = static_cast<double>(r.y), pz = static_cast<double>(r.z);
+            for (int j = 0; j < queued; ++j) {
+                const Rec<P> o = rec[queue[j * BLOCK]];
+                // contact_force (src/sim/integrator.cuh:210-219)
+                const double dx = px - static_cast<double>(o.x);
+                const double dy = py - static_cast<double>(o.y);
+                const double dz = pz - static_cast<double>(o.z);
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 >= two_a * two_a || r2 <= 0.0) continue;
+                const double rr2 = sqrt(r2);
+                const double overlap = two_a - rr2;
+                const double s = k_contact * overlap / rr2;
+                F.x += dx * s;
+                F.y += dy * s;
+                F.z += dz * s;
+            }
+            queued = 0;
+        };
+        const auto screen_pair = [&](const Rec<P>& o, int32_t q) {
+            if ((o.meta & REC_SLOT_MASK) == self) return;
+            const float dx = screen_diff(r.x, o.x), dy = screen_diff(r.y, o.y), dz = screen_diff(r.z, o.z);
+            if (dx * dx + dy * dy + dz * dz >= screen) return;
+            if (queued == CONTACT_QUEUE) drain();
+            queue[queued * BLOCK] = q;
+            ++queued;
+        };
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            int lo[3], off[3], total = 0;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int row = 3 * d + j;
+                const bool any = rr.s[row] < rr.e[row];
+                const int a = wave_min_i(any ? rr.s[row] : 0x7FFFFFFF);
+                const int b = wave_max_i(any ? rr.e[row] : -1);
+                lo[j] = a;
+                off[j] = total;
+                total += b > a ? b - a : 0;
+            }
+            if (total <= SEG) {
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const int len = (j < 2 ? off[j + 1] : total) - off[j];
+                    for (int t = lane; t < len; t += 64) seg[off[j] + t] = rec[lo[j] + t];
+                }
+                wave_sync();
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const int row = 3 * d + j;
+                    for (int32_t q = rr.s[row]; q < rr.e[row]; ++q) screen_pair(seg[off[j] + (q - lo[j])], q);
+                }
+                wave_sync();   // the next range reuses the slice
+            } else {
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const int row = 3 * d + j;
+                    for (int32_t q = rr.s[row]; q < rr.e[row]; ++q) screen_pair(rec[q], q);
+                }
+            }
+        }
+        if (queued > 0) drain();
+        if (loaded) fs[k] = F;
     });
-    fs[k] = F;
-    };
-    for (int t = lb; t * BLOCK < n; t += gridDim.x) {
-        const int k = t * BLOCK + threadIdx.x;
-        if (k < n) body(k);
-    }
 }
 
 // feed_kernel's irradiance half (src/sim/emission.cu:81-105): far-field sample
@@ -233,7 +345,6 @@ __global__ __launch_bounds__(BLOCK) void k_feed_walk(const Counters* ctr, HashGe
                                                      const int32_t* bstart, Grid irr, double ldx, double ldy,
                                                      double ldz, float ambient, int occl, double* dirs) {
     const int n = ctr->count;
-    const int lb = xcd_tile(blockIdx.x, gridDim.x);
     const auto body = [&](const int k) {
     const Rec<P> r = rec[k];
     if ((r.meta & REC_UNOCC) || !(r.meta & REC_ALIVE)) return;
@@ -245,7 +356,10 @@ __global__ __launch_bounds__(BLOCK) void k_feed_walk(const Counters* ctr, HashGe
     if (directional < 0.0) directional = 0.0;
     if (occl) {
         double open = 1.0;
-        walk_rows<P>(h, bstart, rec, px, py, pz, [&](const Rec<P>& o) {
+        int gx, gy, gz;
+        hash_coords(h, px, py, pz, gx, gy, gz);
+        walk_rows(row_ranges(h, [&](int, int b) { return bstart[b]; }, gx, gy, gz), [&](int, int32_t q) {
+            const Rec<P> o = rec[q];
             if ((o.meta & REC_SLOT_MASK) == self || !(o.meta & REC_ALIVE)) return;
             // shadow_fraction (src/sim/emission.cuh:30-37)
             const double dx = px - static_cast<double>(o.x);
@@ -264,10 +378,7 @@ __global__ __launch_bounds__(BLOCK) void k_feed_walk(const Counters* ctr, HashGe
     }
     dirs[k] = directional;
     };
-    for (int t = lb; t * BLOCK < n; t += gridDim.x) {
-        const int k = t * BLOCK + threadIdx.x;
-        if (k < n) body(k);
-    }
+    xcd_tiles(n, [&](const int k, const bool valid) { if (valid) body(k); });
 }
 
 inline int grid_for(int n) { return (n + BLOCK - 1) / BLOCK; }
@@ -276,23 +387,22 @@ inline int grid_for(int n) { return (n + BLOCK - 1) / BLOCK; }
 
 template <class P>
 void launch_hash_build(const Launch& L, const WorldDev<P>& w, bool count_tick) {
-    const int gp = stride_grid(w.pcap);
+    const int gp = cell_grid(w.pcap);
     const int gc = stride_grid(w.capacity);
-    if (L.prof) L.prof->begin("hash_key");
-    k_hash_key<P><<<gp, BLOCK, 0, L.stream>>>(
-        w.c, w.ctr, w.hg, w.keys, w.rin, w.bcount, w.dflag, w.drop, w.capacity, count_tick ? 1 : 0,
-        w.resort_period);
-    if (L.prof) L.prof->end();
     const int nbp1 = w.hg.nb + 1;
+    if (L.prof) L.prof->begin("hash_key");
+    k_hash_key<P><<<gp, BLOCK, 0, L.stream>>>(w.c, w.ctr, w.hg, w.keys, w.rin, w.bcount, w.bsum, w.dflag, w.drop,
+                                              w.dsum, w.xsum, w.capacity, count_tick ? 1 : 0, w.resort_period);
+    if (L.prof) L.prof->end();
     if (L.prof) L.prof->begin("hash_bucket_scan");
-    k_bucket_scan<<<scan_tiles_for(nbp1), BLOCK, 0, L.stream>>>(w.bcount, w.bstart, nbp1, w.sc);
+    k_bucket_scan<<<scan_tiles_for(nbp1), BLOCK, 0, L.stream>>>(w.ctr, w.bcount, w.bsum, w.bstart, nbp1);
     if (L.prof) L.prof->end();
     if (L.prof) L.prof->begin("hash_scatter");
     k_hash_scatter<P><<<gp, BLOCK, 0, L.stream>>>(w.c, w.ctr, w.keys, w.rin, w.bstart, w.trec, w.tphys);
     if (L.prof) L.prof->end();
     if (L.prof) L.prof->begin("hash_rank");
     k_hash_rank<P><<<gc, BLOCK, 0, L.stream>>>(w.ctr, w.hg, w.trec, w.tphys, w.bstart, w.rec, w.vals_sorted,
-                                                w.sphys, w.spos);
+                                                w.sphys, w.spos, w.bsum, scan_tiles_for(nbp1));
     if (L.prof) L.prof->end();
 }
 
@@ -305,23 +415,20 @@ void launch_resort(const Launch& L, const WorldDev<P>& w) {
     if (L.prof) L.prof->begin("resort_copyback");
     k_resort_copyback<P><<<gc, BLOCK, 0, L.stream>>>(w.c, w.alt, w.ctr, w.sphys, w.spos);
     if (L.prof) L.prof->end();
-    if (L.prof) L.prof->begin("resort_finalize");
-    k_resort_finalize<<<1, 64, 0, L.stream>>>(w.ctr);
-    if (L.prof) L.prof->end();
 }
 
 template <class P>
-void launch_forces_walk(const Launch& L, const WorldDev<P>& w, const TickParams& t) {
+void launch_contact_walk(const Launch& L, const WorldDev<P>& w, const TickParams& t) {
     if (!t.contact) return;
-    const int g = blocks_for_xcd(stride_grid(w.capacity));
-    if (L.prof) L.prof->begin("forces_walk");
-    k_forces_walk<P><<<g, BLOCK, 0, L.stream>>>(w.ctr, w.hg, w.rec, w.bstart, w.fs, t.k_contact);
+    if (L.prof) L.prof->begin("contact_walk");
+    k_contact_walk<P><<<cell_grid(w.capacity), BLOCK, 0, L.stream>>>(w.ctr, w.hg, w.rec, w.bstart, w.fs,
+                                                                     t.k_contact);
     if (L.prof) L.prof->end();
 }
 
 template <class P>
 void launch_feed_walk(const Launch& L, const WorldDev<P>& w, const TickParams& t) {
-    const int g = blocks_for_xcd(stride_grid(w.capacity));
+    const int g = cell_grid(w.capacity);
     if (L.prof) L.prof->begin("feed_walk");
     k_feed_walk<P><<<g, BLOCK, 0, L.stream>>>(w.ctr, w.hg, w.rec, w.bstart, w.g[ASTRO_FIELD_IRRADIANCE], t.ldx,
                                               t.ldy, t.ldz, t.ambient_irr, t.occlusion_exact, w.dirs);
@@ -330,8 +437,8 @@ void launch_feed_walk(const Launch& L, const WorldDev<P>& w, const TickParams& t
 
 template void launch_hash_build<double>(const Launch&, const WorldDev<double>&, bool);
 template void launch_hash_build<float>(const Launch&, const WorldDev<float>&, bool);
-template void launch_forces_walk<double>(const Launch&, const WorldDev<double>&, const TickParams&);
-template void launch_forces_walk<float>(const Launch&, const WorldDev<float>&, const TickParams&);
+template void launch_contact_walk<double>(const Launch&, const WorldDev<double>&, const TickParams&);
+template void launch_contact_walk<float>(const Launch&, const WorldDev<float>&, const TickParams&);
 template void launch_feed_walk<double>(const Launch&, const WorldDev<double>&, const TickParams&);
 template void launch_feed_walk<float>(const Launch&, const WorldDev<float>&, const TickParams&);
 template void launch_resort<double>(const Launch&, const WorldDev<double>&);

@@ -196,13 +196,15 @@ struct astro_world {
     int32_t resort_period = 16;
     uint32_t *keys = nullptr, *rin = nullptr, *bcount = nullptr, *vals_sorted = nullptr;
     uint32_t *tphys = nullptr, *sphys = nullptr;
-    int32_t *bstart = nullptr, *spos = nullptr, *dflag = nullptr, *prefix = nullptr, *scan_total = nullptr;
-    int32_t *drop = nullptr, *dprefix = nullptr;
+    int32_t *bstart = nullptr, *spos = nullptr, *dflag = nullptr, *prefix = nullptr;
+    int32_t *drop = nullptr, *dprefix = nullptr, *bsum = nullptr, *dsum = nullptr, *xsum = nullptr;
+    uint32_t* dlist = nullptr;
+    float* co2_avail = nullptr;
     void *trec = nullptr, *rec = nullptr;
     Force* fs = nullptr;
     double* dirs = nullptr;
     unsigned long long* co2_demand = nullptr;
-    unsigned long long *scan_status = nullptr, *tile_ctr = nullptr, *stats = nullptr;
+    unsigned long long* stats = nullptr;
 
     double physics_rate = 1.0, biology_rate = 1.0;
     uint64_t tick = 0;
@@ -270,9 +272,10 @@ WorldDev<P> make_view(astro_world* w) {
     d.keys = w->keys; d.rin = w->rin; d.bcount = w->bcount; d.bstart = w->bstart;
     d.vals_sorted = w->vals_sorted; d.spos = w->spos; d.fs = w->fs; d.dirs = w->dirs;
     d.tphys = w->tphys; d.sphys = w->sphys;
-    d.dflag = w->dflag; d.prefix = w->prefix; d.co2_demand = w->co2_demand; d.scan_total = w->scan_total;
-    d.drop = w->drop; d.dprefix = w->dprefix;
-    d.sc = ScanCtl{w->scan_status, w->tile_ctr, w->ctr};
+    d.dflag = w->dflag; d.prefix = w->prefix; d.co2_demand = w->co2_demand;
+    d.drop = w->drop; d.dprefix = w->dprefix; d.bsum = w->bsum; d.dsum = w->dsum; d.xsum = w->xsum;
+    d.dlist = w->dlist;
+    d.co2_avail = w->co2_avail;
     d.stats = w->stats;
     return d;
 }
@@ -379,19 +382,18 @@ int32_t enqueue_tick(astro_world* w, const Launch& L) {
     if (ds.thermal_enabled && (st = enqueue_thermal<P>(w, L, dt)) != ASTRO_OK) return st;
     const bool feed = ds.emission_enabled && lit(w);
     if (feed) enqueue_light<P>(w, L, t);
-    if (feed || ds.taxis_enabled) launch_feed_taxis<P>(L, make_view<P>(w), t, feed, ds.taxis_enabled != 0);
-    {
-        const WorldDev<P> d = make_view<P>(w);
-        launch_sample_base<P>(L, d, t);
-        launch_forces_walk<P>(L, d, t);
-        launch_integrate<P>(L, d, t);
-    }
+    launch_cell_forces<P>(L, make_view<P>(w), t, feed, ds.taxis_enabled != 0);
     // Stages 7-8 (step.cu:168-171). Both accumulators are empty here: CO2
-    // uptake is folded inside lifecycle and nothing in this path deposits N2,
-    // so the reference's dense flush is a no-op and is not issued.
+    // uptake is flushed inside lifecycle and nothing in this path deposits N2,
+    // so the reference's dense flush is a no-op and is not issued. The CO2
+    // substeps run before the integrator instead of after it: nothing between
+    // reads the CO2 field (the sample above has already taken its values), and
+    // the integrate pass can then book the lifecycle's CO2 demand itself.
     if ((st = enqueue_diffuse(w, L, ASTRO_FIELD_CO2, dt)) != ASTRO_OK) return st;
+    const bool book = ds.lifecycle_enabled && w->f[ASTRO_FIELD_CO2].on;
+    launch_integrate<P>(L, make_view<P>(w), t, book);
     if ((st = enqueue_diffuse(w, L, ASTRO_FIELD_N2, dt)) != ASTRO_OK) return st;
-    if (ds.lifecycle_enabled) launch_lifecycle<P>(L, make_view<P>(w), t);
+    if (ds.lifecycle_enabled) launch_lifecycle<P>(L, make_view<P>(w), t, book);
     w->hash_valid = false;
     return ASTRO_OK;
 }
@@ -432,7 +434,7 @@ int32_t check_device_errors(astro_world* w) {
     Counters c{};
     int32_t st = hip_check(w, hipMemcpy(&c, w->ctr, sizeof(c), hipMemcpyDeviceToHost), "read counters");
     if (st != ASTRO_OK) return st;
-    if (c.error_flags & ERR_SCAN_SPIN) return fail(w, ASTRO_DEVICE_ERROR, "prefix scan look-back did not complete");
+    if (c.error_flags & ERR_SCAN) return fail(w, ASTRO_DEVICE_ERROR, "prefix scan tile total mismatch");
     return ASTRO_OK;
 }
 
@@ -441,7 +443,7 @@ int32_t read_counters(astro_world* w, Counters& c) {
     if (st != ASTRO_OK) return st;
     st = hip_check(w, hipMemcpy(&c, w->ctr, sizeof(c), hipMemcpyDeviceToHost), "read counters");
     if (st != ASTRO_OK) return st;
-    if (c.error_flags & ERR_SCAN_SPIN) return fail(w, ASTRO_DEVICE_ERROR, "prefix scan look-back did not complete");
+    if (c.error_flags & ERR_SCAN) return fail(w, ASTRO_DEVICE_ERROR, "prefix scan tile total mismatch");
     return ASTRO_OK;
 }
 
@@ -640,6 +642,8 @@ int32_t astro_world_create(const astro_world_desc* din, astro_world** out) {
         if (g.nx == 0 && g.ny == 0 && g.nz == 0) { g.nx = g.ny = defaults[f]; g.nz = 1; }
         if (f == ASTRO_FIELD_N2 && g.nx < 0) continue;   // disabled
         if (g.nx <= 1 || g.ny <= 1 || g.nz < 1) return ASTRO_INVALID_ARGUMENT;
+        // voxel indices travel as uint32 with the top bit as a flag
+        if (static_cast<int64_t>(g.nx) * g.ny * g.nz >= 0x7FFFFFFFll) return ASTRO_INVALID_ARGUMENT;
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return ASTRO_DEVICE_ERROR;
@@ -685,7 +689,6 @@ int32_t astro_world_create(const astro_world_desc* din, astro_world** out) {
     const size_t n = static_cast<size_t>(cap);
     const size_t nb = static_cast<size_t>(w->hg.nb);
     const size_t rec_bytes = w->fp32 ? sizeof(Rec<float>) : sizeof(Rec<double>);
-    const size_t max_tiles = static_cast<size_t>(std::max(scan_tiles_for(cap), scan_tiles_for(static_cast<int>(nb + 1)))) + 1;
     const size_t pn = 2 * n;
     auto layout = [&](Carver& cv) {
         if (w->fp32) {
@@ -705,6 +708,7 @@ int32_t astro_world_create(const astro_world_desc* din, astro_world** out) {
         }
         w->keys = cv.take<uint32_t>(pn);
         w->rin = cv.take<uint32_t>(pn);
+        w->co2_avail = cv.take<float>(pn);
         w->bcount = cv.take<uint32_t>(nb + 1);
         w->bstart = cv.take<int32_t>(nb + 2);
         w->trec = cv.take<char>(n * rec_bytes);
@@ -719,10 +723,11 @@ int32_t astro_world_create(const astro_world_desc* din, astro_world** out) {
         w->prefix = cv.take<int32_t>(n);
         w->drop = cv.take<int32_t>(n);
         w->dprefix = cv.take<int32_t>(n);
-        w->scan_total = cv.take<int32_t>(1);
+        w->dlist = cv.take<uint32_t>(n);
+        w->bsum = cv.take<int32_t>(static_cast<size_t>(scan_tiles_for(static_cast<int>(nb + 1))));
+        w->dsum = cv.take<int32_t>(static_cast<size_t>(scan_tiles_for(cap)));
+        w->xsum = cv.take<int32_t>(static_cast<size_t>(scan_tiles_for(cap)));
         w->co2_demand = cv.take<unsigned long long>(std::max<size_t>(1, w->f[ASTRO_FIELD_CO2].count()));
-        w->scan_status = cv.take<unsigned long long>(max_tiles);
-        w->tile_ctr = cv.take<unsigned long long>(1);
         w->stats = cv.take<unsigned long long>(SW_COUNT);
     };
     Carver measure;
@@ -745,7 +750,6 @@ int32_t astro_world_create(const astro_world_desc* din, astro_world** out) {
     c0.count = 0;
     c0.capacity = cap;
     c0.next_id = 1;
-    c0.scan_epoch = 1;
     hipMemcpyAsync(w->ctr, &c0, sizeof(c0), hipMemcpyHostToDevice, w->stream);
     const Launch L{w->stream, nullptr};
     for (int fi = 0; fi < ASTRO_FIELD_COUNT; ++fi) {
@@ -1140,13 +1144,8 @@ int32_t astro_stage_forces(astro_world* w) {
     if (st != ASTRO_OK) return st;
     const Launch L = launch_of(w);
     const TickParams t = tick_params(w, ASTRO_DT_PHYSICS * w->physics_rate);
-    if (w->fp32) {
-        launch_sample_base<float>(L, make_view<float>(w), t);
-        launch_forces_walk<float>(L, make_view<float>(w), t);
-    } else {
-        launch_sample_base<double>(L, make_view<double>(w), t);
-        launch_forces_walk<double>(L, make_view<double>(w), t);
-    }
+    if (w->fp32) launch_cell_forces<float>(L, make_view<float>(w), t, false, false);
+    else launch_cell_forces<double>(L, make_view<double>(w), t, false, false);
     return hip_check(w, hipGetLastError(), "forces");
 }
 
@@ -1180,8 +1179,8 @@ int32_t astro_stage_integrate(astro_world* w, double dt) {
     if (!w) return ASTRO_INVALID_ARGUMENT;
     const Launch L = launch_of(w);
     const TickParams t = tick_params(w, dt);
-    if (w->fp32) launch_integrate<float>(L, make_view<float>(w), t);
-    else launch_integrate<double>(L, make_view<double>(w), t);
+    if (w->fp32) launch_integrate<float>(L, make_view<float>(w), t, false);
+    else launch_integrate<double>(L, make_view<double>(w), t, false);
     w->hash_valid = false;
     return hip_check(w, hipGetLastError(), "integrate");
 }
@@ -1194,14 +1193,12 @@ int32_t astro_stage_motion(astro_world* w, double dt) {
     const TickParams t = tick_params(w, dt);
     if (w->fp32) {
         const WorldDev<float> d = make_view<float>(w);
-        launch_sample_base<float>(L, d, t);
-        launch_forces_walk<float>(L, d, t);
-        launch_integrate<float>(L, d, t);
+        launch_cell_forces<float>(L, d, t, false, false);
+        launch_integrate<float>(L, d, t, false);
     } else {
         const WorldDev<double> d = make_view<double>(w);
-        launch_sample_base<double>(L, d, t);
-        launch_forces_walk<double>(L, d, t);
-        launch_integrate<double>(L, d, t);
+        launch_cell_forces<double>(L, d, t, false, false);
+        launch_integrate<double>(L, d, t, false);
     }
     w->hash_valid = false;
     return hip_check(w, hipGetLastError(), "motion");
@@ -1231,10 +1228,10 @@ int32_t astro_stage_lifecycle(astro_world* w, double dt) {
     const TickParams t = tick_params(w, dt);
     if (w->fp32) {
         launch_life_clear<float>(L, make_view<float>(w));
-        launch_lifecycle<float>(L, make_view<float>(w), t);
+        launch_lifecycle<float>(L, make_view<float>(w), t, false);
     } else {
         launch_life_clear<double>(L, make_view<double>(w));
-        launch_lifecycle<double>(L, make_view<double>(w), t);
+        launch_lifecycle<double>(L, make_view<double>(w), t, false);
     }
     w->hash_valid = false;
     return hip_check(w, hipGetLastError(), "lifecycle");
@@ -1243,8 +1240,8 @@ int32_t astro_stage_lifecycle(astro_world* w, double dt) {
 int32_t astro_stage_compact(astro_world* w) {
     if (!w) return ASTRO_INVALID_ARGUMENT;
     const Launch L = launch_of(w);
-    if (w->fp32) launch_compact<float>(L, make_view<float>(w), true);
-    else launch_compact<double>(L, make_view<double>(w), true);
+    if (w->fp32) launch_compact<float>(L, make_view<float>(w));
+    else launch_compact<double>(L, make_view<double>(w));
     w->hash_valid = false;
     return hip_check(w, hipGetLastError(), "compact");
 }

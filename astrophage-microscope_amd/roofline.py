@@ -36,21 +36,25 @@ def kernel_bytes(name: str, c: dict) -> float:
         "ftcs_T": 8 * pts.get(0, 0),
         "ftcs_CO2": 8 * pts.get(1, 0),
         "ftcs_N2": 8 * pts.get(2, 0),
+        # fused pass: rec + sphys + flags + biomass + energy + emit_power + fs,
+        # local-field writes and gathers; recoil direction for awake cells;
+        # the taxis state of awake cells; the feed irradiance when lit
+        "forces": n * (16 + 4 + 4 + 8 + 8 + 4 + 32 + 12 + 4 * k_fields) + awake * 12,
+        "taxis_forces": n * (16 + 4 + 4 + 8 + 8 + 4 + 32 + 12 + 4 * k_fields) + awake * (12 + 36),
+        "feed_forces": n * (16 + 4 + 4 + 8 + 8 + 4 + 32 + 12 + 4 * k_fields) + awake * 12 + alive * 24,
+        "feed_taxis_forces": n * (16 + 4 + 4 + 8 + 8 + 4 + 32 + 12 + 4 * k_fields) + awake * (12 + 36) + alive * 24,
         "taxis": 4 * n + 4 * max(alive - awake, 0) + 28 * awake,
         "feed_taxis": 4 * n + 24 * alive + 28 * awake,
-        "sample_base": n * (72 + 3 * p + 4 * k_fields),
-        "forces_walk": n * (r + 64),
         "integrate": n * (96 + 12 * p),
         "co2_demand": alive * (20 + 3 * p) + 4 * n,
         "co2_uptake": 4 * n + alive * 36,
         "life_mark": n * 20 + alive * 28,
-        "divide_scan": 8 * n if c.get("births", 0) > 0 else 0,
-        "divide": 12 * n + c.get("births", 0) * 2 * S,
-        "life_finalize": 0,
-        "compact_scan": 8 * n * c.get("compacting", 0),
-        "compact_gather": (4 * n + 2 * S * (n - c.get("dead", 0))) * c.get("compacting", 0),
-        "compact_copyback": 2 * S * (n - c.get("dead", 0)) * c.get("compacting", 0),
-        "compact_finalize": 0,
+        # both scans: flags read + prefix written, when each runs
+        "life_scan": 8 * n * ((1 if c.get("births", 0) > 0 else 0) + c.get("compacting", 0)),
+        # renumbering reads every slot and rewrites survivors' slots below a
+        # drop; daughters are written whole
+        "life_apply": 8 * n * c.get("compacting", 0) + c.get("births", 0) * 2 * S,
+        "compact_mark": 8 * n,
         "occlusion_stamp": n * (4 + 3 * p + 8),
         "light_sweep": 12 * pts.get(3, 0),
         "feed_walk": alive * (r + 8 + 4),

@@ -203,6 +203,50 @@ def test_lifecycle_division_and_compaction(oracle, gpu):
         g.close()
 
 
+@pytest.mark.parametrize("precision", [ah.FP64_CONTRACT, ah.FP32_HOT])
+def test_tick_parity_co2_rationing(oracle, gpu, precision):
+    """Whole ticks with a crowded, CO2-starved culture: most voxels hold
+    several cells, so demand is rationed (lifecycle.cu:62-86), uptakes share
+    accumulators and the flush sums them; division and compaction run every
+    tick. Counters, slots, ids and RNG state are bit-exact and the CO2 field
+    matches the oracle's bit for bit."""
+    kw = dict(REF2D, capacity=6000, compaction_enabled=1, co2_init=2e-4, biology_rate=1e6, precision=precision)
+    cells = oracle_cells(oracle, 3000, 21, spawn=dict(placement=ah.PLACE_GAUSSIAN, place_radius=60e-6,
+                                                      charge_dist=ah.DIST_UNIFORM, charge_b=0.9), **kw)
+    rng = np.random.default_rng(5)
+    quota = 2.1e-14
+    cells["co2_held"] = rng.uniform(0.0, 1.0, 3000) * quota
+    # a tenth of the culture is a few ticks' uptake short of dividing
+    near = rng.random(3000) < 0.1
+    cells["co2_held"][near] = quota - rng.uniform(0.0, 4.0, near.sum()) * 6e-20
+    dead = rng.random(3000) < 0.05
+    cells["flags"][dead] &= ~np.uint32(ah.FLAG_ALIVE)
+    if precision == ah.FP32_HOT:
+        for k in ("x", "y", "z", "vx", "vy", "vz"):
+            cells[k] = cells[k].astype(np.float32).astype(np.float64)
+    o, g = pair(oracle, gpu, cells, **kw)
+    with o, g:
+        (nx, ny, nz), _ = o.field_dims(ah.CO2)
+        occupied = np.unique(np.floor((cells["x"] + 0.5 * o.desc.chamber_w) / (o.desc.chamber_w / nx)) * ny +
+                             np.floor((cells["y"] + 0.5 * o.desc.chamber_h) / (o.desc.chamber_h / ny)))
+        assert occupied.size < 0.5 * 3000   # most voxels are shared
+        for _ in range(4):
+            o.step(1)
+            g.step(1)
+            co, cg = o.counters(), g.counters()
+            assert (co.count, co.next_id, co.divisions_total, co.deaths_total) == \
+                   (cg.count, cg.next_id, cg.divisions_total, cg.deaths_total)
+            assert np.array_equal(o.field(ah.CO2), g.field(ah.CO2))
+        assert co.divisions_total > 0 and co.deaths_total > 0
+        a, b = o.download(), g.download()
+        compare_cells(a, b)
+        tol = 0.0 if precision == ah.FP64_CONTRACT else 1.25e-10
+        for name in ("x", "y", "z"):
+            rel_close(a[name], b[name], 1e-12, atol=tol, name=name)
+        rel_close(a["co2_held"], b["co2_held"], 1e-12, name="co2_held")
+        rel_close(a["energy"], b["energy"], 1e-12, name="energy")
+
+
 def test_replay_and_graph_determinism(gpu, oracle):
     """Same inputs -> same state hash, eager or graph-replayed
     (tools/headless.cpp:348-381)."""
