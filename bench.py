@@ -100,7 +100,7 @@ def main() -> int:
     ap.add_argument("--cells", type=int, default=1_000_000)
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--profile-steps", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     a = ap.parse_args()
 
