@@ -200,7 +200,9 @@ __device__ __forceinline__ double ou_position_shape(double x) {
     return 2.0 * x - 3.0 + 4.0 * e - e * e;
 }
 
-struct Ou { double one_minus_e1, inv_beta, sigma_rr, sigma_vv, sigma_rv; };
+// The joint OU step's coefficients; noise_a / noise_b are the Cholesky
+// factors ou_advance_axis derives from the sigmas, the same for every axis.
+struct Ou { double one_minus_e1, inv_beta, sigma_rr, sigma_vv, sigma_rv, noise_a, noise_b; };
 
 __device__ __forceinline__ Ou ou_coefficients(double mass, double gamma, double t, double dt) {
     const double beta = gamma / mass;
@@ -214,6 +216,13 @@ __device__ __forceinline__ Ou ou_coefficients(double mass, double gamma, double 
     c.sigma_vv = sqrt(var_vv > 0.0 ? var_vv : 0.0);
     c.sigma_rr = sqrt(var_rr > 0.0 ? var_rr : 0.0);
     c.sigma_rv = kT_over_m * c.inv_beta * c.one_minus_e1 * c.one_minus_e1;
+    c.noise_a = 0.0;
+    c.noise_b = 0.0;
+    if (c.sigma_rr > 0.0) {
+        c.noise_a = c.sigma_rv / c.sigma_rr;
+        const double b2 = c.sigma_vv * c.sigma_vv - c.noise_a * c.noise_a;
+        c.noise_b = sqrt(b2 > 0.0 ? b2 : 0.0);
+    }
     return c;
 }
 
@@ -225,11 +234,8 @@ __device__ __forceinline__ void ou_advance_axis(double& pos, double& vel, double
     double npos = pos + dpos;
     double nvel = vel + dvel;
     if (noise && c.sigma_rr > 0.0) {
-        const double a = c.sigma_rv / c.sigma_rr;
-        const double b2 = c.sigma_vv * c.sigma_vv - a * a;
-        const double b = sqrt(b2 > 0.0 ? b2 : 0.0);
         npos += c.sigma_rr * z1;
-        nvel += a * z1 + b * z2;
+        nvel += c.noise_a * z1 + c.noise_b * z2;
     } else if (noise) {
         nvel += c.sigma_vv * z2;
     }

@@ -299,15 +299,29 @@ __global__ __launch_bounds__(BLOCK) void k_contact_walk(const Counters* ctr, Has
             queue[queued * BLOCK] = q;
             ++queued;
         };
+        // The wave's key span bounds every row it walks: row (dz, dy) of any
+        // lane lies in buckets [kmin + o - 1, kmax + o + 1], o = dz*nx*ny +
+        // dy*nx. Lane m < 18 fetches the bucket start of bound m (the start of
+        // row m/2's range, or the end), and the wave reads them back by lane.
+        const int key = (gz * h.ny + gy) * h.nx + gx;
+        const int kmin = wave_min_i(occ ? key : 0x7FFFFFFF);
+        const int kmax = wave_max_i(occ ? key : -1);
+        int bound = 0;
+        if (kmax >= 0 && lane < 18) {
+            const int row = lane >> 1;
+            const int64_t o = static_cast<int64_t>(row / 3 - 1) * h.nx * h.ny + static_cast<int64_t>(row % 3 - 1) * h.nx;
+            int64_t b = (lane & 1) ? static_cast<int64_t>(kmax) + o + 2 : static_cast<int64_t>(kmin) + o - 1;
+            b = b < 0 ? 0 : (b > h.nb ? h.nb : b);
+            bound = bstart[b];
+        }
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             int lo[3], off[3], total = 0;
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
                 const int row = 3 * d + j;
-                const bool any = rr.s[row] < rr.e[row];
-                const int a = wave_min_i(any ? rr.s[row] : 0x7FFFFFFF);
-                const int b = wave_max_i(any ? rr.e[row] : -1);
+                const int a = __builtin_amdgcn_readlane(bound, 2 * row);
+                const int b = __builtin_amdgcn_readlane(bound, 2 * row + 1);
                 lo[j] = a;
                 off[j] = total;
                 total += b > a ? b - a : 0;
