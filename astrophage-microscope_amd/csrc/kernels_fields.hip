@@ -74,66 +74,79 @@ __global__ __launch_bounds__(BLOCK) void k_ftcs3d(const float* __restrict__ src,
 }
 
 // The same 7-point substep, vectorised: a lane owns 4 consecutive x points
-// (one 16-byte load and store), a wave one 256-point row segment, a block 4
-// rows; blocks march zc planes along z with the next plane's load issued a
-// plane ahead. The in-row neighbours come from the adjacent lanes' registers,
-// so a wave moves 1 KB per plane per memory round trip instead of 256 B.
-// Needs nx % 4 == 0; per point the arithmetic is exactly k_ftcs3d's.
+// (16-byte loads and stores), a wave one 256-point row segment, a block 4
+// rows, and each block ZC planes. All of a block's inputs -- its ZC + 2
+// planes of its rows, the rows above and below, and the row-end neighbours --
+// are requested before any is used, so a wave keeps ~14 independent loads in
+// flight; in-row neighbours come from the adjacent lanes' registers. Needs
+// nx % 4 == 0; per point the arithmetic is exactly k_ftcs3d's.
 __device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast<const float4*>(p); }
 
+constexpr int FTCS_ZC = 4;
+
 __global__ __launch_bounds__(BLOCK) void k_ftcs3d_v4(const float* __restrict__ src, float* __restrict__ dst, int nx,
-                                                     int ny, int nz, int zc, float cx, float cy, float cz, int bc,
-                                                     float amb, float rx, float ry, float rz) {
-    const int tx = (nx + 255) / 256, ty = (ny + 3) / 4, tz = (nz + zc - 1) / zc;
+                                                     int ny, int nz, float cx, float cy, float cz, int bc, float amb,
+                                                     float rx, float ry, float rz) {
+    constexpr int ZC = FTCS_ZC;
+    const int tx = (nx + 255) / 256, ty = (ny + 3) / 4, tz = (nz + ZC - 1) / ZC;
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     if (t >= tx * ty * tz) return;   // whole block: shuffles below stay uniform
     const int lane = threadIdx.x & 63;
     const int i0 = (t % tx) * 256 + lane * 4;
     const int j = ((t / tx) % ty) * 4 + (threadIdx.x >> 6);
-    const int k0 = (t / (tx * ty)) * zc;
-    const int k1 = k0 + zc < nz ? k0 + zc : nz;
+    const int k0 = (t / (tx * ty)) * ZC;
     const bool on = i0 < nx && j < ny;
     const size_t pl = static_cast<size_t>(nx) * ny;
     const size_t col = on ? static_cast<size_t>(j) * nx + i0 : 0;
     const float4 zero4{0.0f, 0.0f, 0.0f, 0.0f};
-    float4 below = (on && k0 > 0) ? ld4(src + (k0 - 1) * pl + col) : zero4;
-    float4 cur = on ? ld4(src + k0 * pl + col) : zero4;
-    float4 above = (on && k0 + 1 < nz) ? ld4(src + (k0 + 1) * pl + col) : zero4;
-    for (int k = k0; k < k1; ++k) {
+    float4 planes[ZC + 2], dn4[ZC], up4[ZC];
+    float lf[ZC], rg[ZC];
+#pragma unroll
+    for (int m = 0; m < ZC + 2; ++m) {
+        const int k = k0 - 1 + m;
+        planes[m] = (on && k >= 0 && k < nz) ? ld4(src + k * pl + col) : zero4;
+    }
+#pragma unroll
+    for (int m = 0; m < ZC; ++m) {
+        const int k = k0 + m;
+        const bool ok = on && k < nz;
         const size_t id = k * pl + col;
-        const float4 next = (on && k + 2 < nz && k + 1 < k1) ? ld4(src + id + 2 * pl) : zero4;
-        const float4 d = (on && j > 0) ? ld4(src + id - nx) : zero4;
-        const float4 u = (on && j < ny - 1) ? ld4(src + id + nx) : zero4;
+        dn4[m] = (ok && j > 0) ? ld4(src + id - nx) : zero4;
+        up4[m] = (ok && j < ny - 1) ? ld4(src + id + nx) : zero4;
+        lf[m] = (ok && lane == 0 && i0 > 0) ? src[id - 1] : 0.0f;
+        rg[m] = (ok && (lane == 63 || i0 + 4 >= nx) && i0 + 4 < nx) ? src[id + 4] : 0.0f;
+    }
+#pragma unroll
+    for (int m = 0; m < ZC; ++m) {
+        const int k = k0 + m;
+        const float4 cur = planes[m + 1], below = planes[m], above = planes[m + 2];
         // x neighbours of the lane's first and last point
         float lft = __shfl_up(cur.w, 1, 64);
         float rgt = __shfl_down(cur.x, 1, 64);
-        if (on) {
-            if (lane == 0) lft = i0 > 0 ? src[id - 1] : 0.0f;
-            if (lane == 63 || i0 + 4 >= nx) rgt = i0 + 4 < nx ? src[id + 4] : 0.0f;
-            const float c[4] = {cur.x, cur.y, cur.z, cur.w};
-            const float dd[4] = {d.x, d.y, d.z, d.w}, uu[4] = {u.x, u.y, u.z, u.w};
-            const float bb[4] = {below.x, below.y, below.z, below.w}, ff[4] = {above.x, above.y, above.z, above.w};
-            float o[4];
+        if (lane == 0) lft = lf[m];
+        if (lane == 63 || i0 + 4 >= nx) rgt = rg[m];
+        if (!on || k >= nz) continue;
+        const float c[4] = {cur.x, cur.y, cur.z, cur.w};
+        const float dd[4] = {dn4[m].x, dn4[m].y, dn4[m].z, dn4[m].w};
+        const float uu[4] = {up4[m].x, up4[m].y, up4[m].z, up4[m].w};
+        const float bb[4] = {below.x, below.y, below.z, below.w}, ff[4] = {above.x, above.y, above.z, above.w};
+        float o[4];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const float cc = c[e];
-                const float l = e > 0 ? c[e - 1] : (i0 > 0 ? lft : ghost_of(cc, bc, amb, rx));
-                const float r = e < 3 ? c[e + 1] : (i0 + 4 < nx ? rgt : ghost_of(cc, bc, amb, rx));
-                const float dn = j > 0 ? dd[e] : ghost_of(cc, bc, amb, ry);
-                const float up = j < ny - 1 ? uu[e] : ghost_of(cc, bc, amb, ry);
-                float acc = cx * ((l + r) - 2.0f * cc) + cy * ((dn + up) - 2.0f * cc);
-                if (nz > 1) {
-                    const float b = k > 0 ? bb[e] : ghost_of(cc, bc, amb, rz);
-                    const float f = k < nz - 1 ? ff[e] : ghost_of(cc, bc, amb, rz);
-                    acc = acc + cz * ((b + f) - 2.0f * cc);
-                }
-                o[e] = cc + acc;
+        for (int e = 0; e < 4; ++e) {
+            const float cc = c[e];
+            const float l = e > 0 ? c[e - 1] : (i0 > 0 ? lft : ghost_of(cc, bc, amb, rx));
+            const float r = e < 3 ? c[e + 1] : (i0 + 4 < nx ? rgt : ghost_of(cc, bc, amb, rx));
+            const float dn = j > 0 ? dd[e] : ghost_of(cc, bc, amb, ry);
+            const float up = j < ny - 1 ? uu[e] : ghost_of(cc, bc, amb, ry);
+            float acc = cx * ((l + r) - 2.0f * cc) + cy * ((dn + up) - 2.0f * cc);
+            if (nz > 1) {
+                const float b = k > 0 ? bb[e] : ghost_of(cc, bc, amb, rz);
+                const float f = k < nz - 1 ? ff[e] : ghost_of(cc, bc, amb, rz);
+                acc = acc + cz * ((b + f) - 2.0f * cc);
             }
-            *reinterpret_cast<float4*>(dst + id) = float4{o[0], o[1], o[2], o[3]};
+            o[e] = cc + acc;
         }
-        below = cur;
-        cur = above;
-        above = next;
+        *reinterpret_cast<float4*>(dst + k * pl + col) = float4{o[0], o[1], o[2], o[3]};
     }
 }
 
@@ -234,13 +247,10 @@ void launch_ftcs(const Launch& L, const Grid& g, const Ftcs& f, const char* name
         return;
     }
     if (g.nx % 4 == 0) {
-        const int txy = ((g.nx + 255) / 256) * ((g.ny + 3) / 4);
-        int zc = g.nz;
-        while (zc > 2 && txy * ((g.nz + zc - 1) / zc) < 2048) zc = (zc + 1) / 2;
-        const int tiles = txy * ((g.nz + zc - 1) / zc);
+        const int tiles = ((g.nx + 255) / 256) * ((g.ny + 3) / 4) * ((g.nz + FTCS_ZC - 1) / FTCS_ZC);
         if (L.prof) L.prof->begin(name);
-        k_ftcs3d_v4<<<blocks_for_xcd(tiles), BLOCK, 0, L.stream>>>(g.value, g.scratch, g.nx, g.ny, g.nz, zc, f.cx,
-                                                                    f.cy, f.cz, f.bc, f.amb, f.rx, f.ry, f.rz);
+        k_ftcs3d_v4<<<blocks_for_xcd(tiles), BLOCK, 0, L.stream>>>(g.value, g.scratch, g.nx, g.ny, g.nz, f.cx, f.cy,
+                                                                    f.cz, f.bc, f.amb, f.rx, f.ry, f.rz);
         if (L.prof) L.prof->end();
         return;
     }

@@ -26,34 +26,47 @@ def kernel_bytes(name: str, c: dict) -> float:
     pts = c["pts"]
     k_fields = c.get("sampled_fields", 2)
     S = SOA_BYTES[p]
+    births, dead = c.get("births", 0), c.get("dead", 0)
+    compacting = c.get("compacting", 0)
+    resort_frac = c.get("resort_frac", 0.0)
+    # per-cell bytes of the integrator: flags, t_local, biomass, energy, the
+    # force (via spos), position and velocity, id, RNG state and age in;
+    # RNG state, flags, position, velocity and age out
+    integ = n * (96 + 12 * p)
     table = {
         "hash_key": n * (16 + 3 * p),
         "hash_bucket_scan": 8 * (c["buckets"] + 1) + 4 * min(n, c["buckets"]),
         "hash_scatter": n * (16 + 3 * p + r),
         "hash_rank": n * (2 * r + 16),
+        # a re-sort moves the whole SoA twice (gather, copy back), every
+        # `resort_frac` of the ticks
+        "resort_gather": resort_frac * n * (2 * S + 4),
+        "resort_copyback": resort_frac * n * 2 * S,
         "thermal": n * (12 + 3 * p) + awake * 24,
         "deposit_apply": 4 * n + awake * (3 * p + 16),
         "ftcs_T": 8 * pts.get(0, 0),
         "ftcs_CO2": 8 * pts.get(1, 0),
         "ftcs_N2": 8 * pts.get(2, 0),
-        # fused pass: rec + sphys + flags + biomass + energy + emit_power + fs,
-        # local-field writes and gathers; recoil direction for awake cells;
-        # the taxis state of awake cells; the feed irradiance when lit
-        "forces": n * (16 + 4 + 4 + 8 + 8 + 4 + 32 + 12 + 4 * k_fields) + awake * 12,
-        "taxis_forces": n * (16 + 4 + 4 + 8 + 8 + 4 + 32 + 12 + 4 * k_fields) + awake * (12 + 36),
-        "feed_forces": n * (16 + 4 + 4 + 8 + 8 + 4 + 32 + 12 + 4 * k_fields) + awake * 12 + alive * 24,
-        "feed_taxis_forces": n * (16 + 4 + 4 + 8 + 8 + 4 + 32 + 12 + 4 * k_fields) + awake * (12 + 36) + alive * 24,
         "taxis": 4 * n + 4 * max(alive - awake, 0) + 28 * awake,
         "feed_taxis": 4 * n + 24 * alive + 28 * awake,
-        "integrate": n * (96 + 12 * p),
+        "feed": 4 * n + 24 * alive,
+        # lslot, flags, position, biomass, energy, emit power, spos in; the
+        # local field samples (one value per sampled field) and the force out
+        "sample_base": n * (4 + 4 + 3 * p + 8 + 8 + 4 + 4 + 4 * k_fields + 12 + 32) + awake * 12,
+        # own record, the force read and written; neighbour records and
+        # bucket starts are re-reads of data other lanes fetched
+        "contact_walk": n * (r + 64),
+        "integrate": integ,
+        # + the voxel and concentration each live cell books
+        "integrate_demand": integ + 4 * n + alive * 16,
         "co2_demand": alive * (20 + 3 * p) + 4 * n,
-        "co2_uptake": 4 * n + alive * 36,
-        "life_mark": n * 20 + alive * 28,
-        # both scans: flags read + prefix written, when each runs
-        "life_scan": 8 * n * ((1 if c.get("births", 0) > 0 else 0) + c.get("compacting", 0)),
-        # renumbering reads every slot and rewrites survivors' slots below a
-        # drop; daughters are written whole
-        "life_apply": 8 * n * c.get("compacting", 0) + c.get("births", 0) * 2 * S,
+        # voxel, flags, t_local, lslot per cell; concentration, demand and
+        # the store per live cell
+        "uptake_mark": 16 * n + alive * 36,
+        "life_scan": 8 * n * ((1 if births > 0 else 0) + compacting),
+        # the slot of every entry (flush and renumbering) and each daughter
+        "life_apply": 8 * n + births * 2 * S,
+        "life_finalize": 0,
         "compact_mark": 8 * n,
         "occlusion_stamp": n * (4 + 3 * p + 8),
         "light_sweep": 12 * pts.get(3, 0),

@@ -140,7 +140,7 @@ def main() -> int:
         "awake": 0.5 * (before["awake"] + after["awake"]), "dead": after["deaths"] / max(ticks, 1),
         "births": after["divisions"] / max(ticks, 1), "compacting": 1.0 if after["deaths"] > 0 else 0.0,
         "p": pbytes, "buckets": 400 * 400 * 20, "pts": {0: 512 * 512 * 32, 1: 512 * 512 * 32},
-        "sampled_fields": 2,
+        "sampled_fields": 2, "resort_frac": 1.0 / 16.0,
     }
     stages = {}
     total_ms = 0.0
