@@ -171,6 +171,17 @@ PRODUCT_ONLY = {
     "profile_read": (_I, [_P, _P, _I, _P, _P, _I, C.POINTER(_I)]),
     "profile_reset": (_I, [_P]),
     "world_stream": (_P, [_P]),
+    "ensemble_create": (_I, [C.POINTER(_P), _I, C.POINTER(_P)]),
+    "ensemble_destroy": (None, [_P]),
+    "ensemble_last_error": (C.c_char_p, [_P]),
+    "ensemble_size": (_I, [_P]),
+    "ensemble_chamber": (_P, [_P, _I]),
+    "ensemble_step": (_I, [_P, _I]),
+    "ensemble_sync": (_I, [_P]),
+    "ensemble_profile_enable": (_I, [_P, _I]),
+    "ensemble_profile_read": (_I, [_P, _P, _I, _P, _P, _I, C.POINTER(_I)]),
+    "ensemble_profile_reset": (_I, [_P]),
+    "ensemble_stream": (_P, [_P]),
 }
 
 
@@ -403,3 +414,79 @@ class World:
             name = names.raw[k * stride:(k + 1) * stride].split(b"\0", 1)[0].decode()
             out[name] = (int(launches[k]), float(ms[k]))
         return out
+
+
+def _read_profile(fn, h, max_kernels: int) -> dict[str, tuple[int, float]]:
+    stride = 64
+    names = C.create_string_buffer(stride * max_kernels)
+    launches = np.zeros(max_kernels, np.int64)
+    ms = np.zeros(max_kernels, np.float64)
+    n = C.c_int32()
+    st = fn(h, names, stride, _ptr(launches), _ptr(ms), max_kernels, C.byref(n))
+    if st != OK:
+        raise AstroError(st, "profile_read")
+    out = {}
+    for k in range(n.value):
+        name = names.raw[k * stride:(k + 1) * stride].split(b"\0", 1)[0].decode()
+        out[name] = (int(launches[k]), float(ms[k]))
+    return out
+
+
+class Ensemble:
+    """Independent chambers stepped together: one batched launch per stage
+    over every chamber, one graph launch per step.
+
+    The reference has no ensemble; each chamber's results are those of
+    World.step on that world alone. The worlds stay owned by the caller and
+    must outlive the ensemble."""
+
+    def __init__(self, binding: Binding, worlds: list[World]):
+        self.b = binding
+        self.worlds = list(worlds)
+        arr = (C.c_void_p * len(self.worlds))(*[w.h.value for w in self.worlds])
+        h = C.c_void_p()
+        st = binding.ensemble_create(arr, len(self.worlds), C.byref(h))
+        if st != OK:
+            raise AstroError(st, "ensemble_create: chambers must agree on precision, clock, stages, fields "
+                                 "and substep counts")
+        self.h = h
+
+    def _check(self, st: int, what: str = ""):
+        if st != OK:
+            detail = (self.b.ensemble_last_error(self.h) or b"").decode() if getattr(self, "h", None) else ""
+            raise AstroError(st, detail or what)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.b.ensemble_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self) -> int:
+        return int(self.b.ensemble_size(self.h))
+
+    def step(self, n: int = 1):
+        self._check(self.b.ensemble_step(self.h, n), "ensemble_step")
+
+    def sync(self):
+        self._check(self.b.ensemble_sync(self.h), "ensemble_sync")
+
+    def profile(self, on: bool):
+        self._check(self.b.ensemble_profile_enable(self.h, int(on)))
+
+    def profile_reset(self):
+        self._check(self.b.ensemble_profile_reset(self.h))
+
+    def profile_read(self, max_kernels: int = 64) -> dict[str, tuple[int, float]]:
+        return _read_profile(self.b.ensemble_profile_read, self.h, max_kernels)

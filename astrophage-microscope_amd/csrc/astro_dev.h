@@ -398,20 +398,26 @@ __device__ __forceinline__ int xcd_tile(int b, int nlaunched) {
 // multiple of NUM_XCD.
 __host__ __device__ __forceinline__ int cell_grid(int n) { return blocks_for_xcd(stride_grid(n)); }
 
+// A kernel body's own block index and block count: the launch's for a
+// single world, the chamber's share of it for a batched launch.
+struct VGrid {
+    int block, grid;
+};
+
 // Runs body(i, valid) for i over [0, n) in BLOCK-wide tiles. The tiles are
 // split into NUM_XCD contiguous chunks and the blocks of XCD x walk chunk x,
 // so the cells one L2 serves are spatial neighbours (cells are stored in
 // spatial-hash order) and share field voxels and neighbour records. Whole
 // blocks iterate together (`valid` is false past n), so the body may use
-// wave-wide operations. gridDim.x must be a multiple of NUM_XCD.
+// wave-wide operations. vg.grid must be a multiple of NUM_XCD.
 template <class Body>
-__device__ __forceinline__ void xcd_tiles(int n, Body&& body) {
+__device__ __forceinline__ void xcd_tiles(VGrid vg, int n, Body&& body) {
     const int ntiles = (n + BLOCK - 1) / BLOCK;
-    const int x = static_cast<int>(blockIdx.x) % NUM_XCD;
-    const int nb = static_cast<int>(gridDim.x) / NUM_XCD;
+    const int x = vg.block % NUM_XCD;
+    const int nb = vg.grid / NUM_XCD;
     const int per = (ntiles + NUM_XCD - 1) / NUM_XCD;
     const int t1 = min(ntiles, (x + 1) * per);
-    for (int t = x * per + static_cast<int>(blockIdx.x) / NUM_XCD; t < t1; t += nb) {
+    for (int t = x * per + vg.block / NUM_XCD; t < t1; t += nb) {
         const int i = t * BLOCK + static_cast<int>(threadIdx.x);
         body(i, i < n);
     }

@@ -6,9 +6,12 @@
 // a hipGraph and replayed.
 #pragma once
 
+#include <span>
 #include <string>
+#include <vector>
 
 #include "astro_dev.h"
+#include "dispatch.h"
 #include "scan.h"
 
 namespace astro_dev {
@@ -22,7 +25,47 @@ struct Profiler {
 
 struct Launch {
     hipStream_t stream;
-    Profiler*   prof;   // null unless profiling
+    Profiler*   prof;    // null unless profiling
+    Recorder*   rec;     // null: launch now for one world; else record a batched launch
+};
+
+// Launches body K for each chamber: args[i] with blocks[i] blocks (a multiple
+// of NUM_XCD for bodies that tile with xcd_tiles). Without a recorder there is
+// one chamber and the launch happens now; with one, a single batched launch
+// over all chambers is recorded.
+template <class K, bool W3 = false>
+void launch_k(const Launch& L, const char* name, const std::vector<typename K::Args>& args,
+              const std::vector<int>& blocks) {
+    if (!L.rec) {
+        if (L.prof) L.prof->begin(name);
+        if constexpr (W3) k_one_w3<K><<<blocks[0], BLOCK, 0, L.stream>>>(args[0]);
+        else k_one<K><<<blocks[0], BLOCK, 0, L.stream>>>(args[0]);
+        if (L.prof) L.prof->end();
+        return;
+    }
+    std::vector<int> first(args.size() + 1, 0);
+    for (size_t i = 0; i < args.size(); ++i) first[i + 1] = first[i] + blocks[i];
+    const int total = first.back(), nch = static_cast<int>(args.size());
+    if (total == 0) return;
+    const size_t a_off = L.rec->put(args.data(), sizeof(typename K::Args) * args.size());
+    const size_t f_off = L.rec->put(first.data(), sizeof(int) * first.size());
+    L.rec->launches.push_back({name, [=](hipStream_t s, const char* arena) {
+        const auto* a = reinterpret_cast<const typename K::Args*>(arena + a_off);
+        const int* f = reinterpret_cast<const int*>(arena + f_off);
+        if constexpr (W3) k_many_w3<K><<<total, BLOCK, 0, s>>>(a, f, nch);
+        else k_many<K><<<total, BLOCK, 0, s>>>(a, f, nch);
+    }});
+}
+
+// Per-chamber argument lists and block counts of one launch.
+template <class K>
+struct Batch {
+    std::vector<typename K::Args> args;
+    std::vector<int> blocks;
+    void add(const typename K::Args& a, int nb) {
+        args.push_back(a);
+        blocks.push_back(nb);
+    }
 };
 
 // Everything the device side of one world needs, by value.
@@ -89,6 +132,13 @@ struct TickParams {
     double co2_gv;
 };
 
+// The device views and tick scalars of the chambers one launch covers.
+template <class P> using Views = std::span<const WorldDev<P>>;
+using Ticks = std::span<const TickParams>;
+
+// A span of one: the argument of a launcher for a single world.
+template <class T> std::span<const T> one(const T& x) { return std::span<const T>(&x, 1); }
+
 // FTCS coefficients of one field for one substep.
 struct Ftcs {
     bool  square2d;
@@ -98,13 +148,18 @@ struct Ftcs {
     float amb, rx, ry, rz;
 };
 
+// Launchers of the tick take the views (and tick scalars) of every chamber
+// the launch covers: one for a world, all of an ensemble's chambers under a
+// recorder. Per-chamber host decisions (which kernels run, substep counts)
+// must be the same across the chambers of one launch.
+
 // ---- hash (kernels_hash.hip) ----
-template <class P> void launch_hash_build(const Launch& L, const WorldDev<P>& w, bool count_tick);
+template <class P> void launch_hash_build(const Launch& L, Views<P> ws, bool count_tick);
 // The contact half of forces_kernel over the sorted records, adding to fs.
-template <class P> void launch_contact_walk(const Launch& L, const WorldDev<P>& w, const TickParams& t);
-template <class P> void launch_feed_walk(const Launch& L, const WorldDev<P>& w, const TickParams& t);
+template <class P> void launch_contact_walk(const Launch& L, Views<P> ws, Ticks ts);
+template <class P> void launch_feed_walk(const Launch& L, Views<P> ws, Ticks ts);
 // Re-sorts the SoA into bucket order when due (device-side decision).
-template <class P> void launch_resort(const Launch& L, const WorldDev<P>& w);
+template <class P> void launch_resort(const Launch& L, Views<P> ws);
 
 // ---- cells (kernels_cells.hip) ----
 struct SpawnArgs {
@@ -114,21 +169,18 @@ struct SpawnArgs {
 };
 template <class P> void launch_spawn(const Launch& L, const WorldDev<P>& w, const SpawnArgs& a,
                                      int32_t base, int32_t lbase, uint64_t first_id, uint64_t seed);
-template <class P> void launch_thermal_substep(const Launch& L, const WorldDev<P>& w, double dt_sub,
-                                               double cap, bool update_rec);
-template <class P> void launch_apply_awake(const Launch& L, const WorldDev<P>& w, int field);
-template <class P> void launch_feed_taxis(const Launch& L, const WorldDev<P>& w, const TickParams& t,
-                                          bool feed, bool taxis);
-// integrate_kernel; with `demand` it also books each moved cell's CO2 demand
-// (the lifecycle's first kernel; the CO2 field must have diffused already).
+template <class P> void launch_thermal_substep(const Launch& L, Views<P> ws, Ticks ts, double dt_sub,
+                                               bool update_rec);
+template <class P> void launch_apply_awake(const Launch& L, Views<P> ws, int field);
+template <class P> void launch_feed_taxis(const Launch& L, Views<P> ws, Ticks ts, bool feed, bool taxis);
 // Feeding/taxis (optional), field sample + body force, then the contact walk:
 // leaves each cell's total force in fs (sorted order).
-template <class P> void launch_cell_forces(const Launch& L, const WorldDev<P>& w, const TickParams& t, bool feed,
-                                           bool taxis);
-template <class P> void launch_integrate(const Launch& L, const WorldDev<P>& w, const TickParams& t, bool demand);
+template <class P> void launch_cell_forces(const Launch& L, Views<P> ws, Ticks ts, bool feed, bool taxis);
+// integrate_kernel; with `demand` it also books each moved cell's CO2 demand
+// (the lifecycle's first kernel; the CO2 field must have diffused already).
+template <class P> void launch_integrate(const Launch& L, Views<P> ws, Ticks ts, bool demand);
 // lifecycle_step; `demand_booked` when the integrate pass booked the demand.
-template <class P> void launch_lifecycle(const Launch& L, const WorldDev<P>& w, const TickParams& t,
-                                         bool demand_booked);
+template <class P> void launch_lifecycle(const Launch& L, Views<P> ws, Ticks ts, bool demand_booked);
 // cell_store_compact outside the lifecycle: every dead or empty slot goes.
 template <class P> void launch_compact(const Launch& L, const WorldDev<P>& w);
 // Clears the per-slot lifecycle flags and their tile totals (the tick does
@@ -137,9 +189,9 @@ template <class P> void launch_life_clear(const Launch& L, const WorldDev<P>& w)
 template <class P> void launch_stats_cells(const Launch& L, const WorldDev<P>& w);
 
 // ---- fields (kernels_fields.hip) ----
-void launch_ftcs(const Launch& L, const Grid& g, const Ftcs& f, const char* name);
-template <class P> void launch_stamp(const Launch& L, const WorldDev<P>& w);
-void launch_sweep(const Launch& L, const Grid& irr, const TickParams& t);
+void launch_ftcs(const Launch& L, std::span<const Grid> gs, std::span<const Ftcs> fs, const char* name);
+template <class P> void launch_stamp(const Launch& L, Views<P> ws);
+void launch_sweep(const Launch& L, std::span<const Grid> irr, Ticks ts);
 void launch_fill(const Launch& L, float* v, size_t n, float value);
 void launch_stats_fields(const Launch& L, const Grid& T, const Grid& C, const Grid& N, bool c_on,
                          bool n_on, unsigned long long* stats);

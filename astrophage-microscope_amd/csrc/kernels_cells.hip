@@ -115,8 +115,20 @@ __global__ __launch_bounds__(BLOCK) void k_spawn(Cells<P> c, SpawnArgs p, int32_
 // sees the same flags the reference's feed_kernel reads.
 // ---------------------------------------------------------------------------
 template <class P>
-__global__ __launch_bounds__(BLOCK) void k_thermal(Cells<P> c, const Counters* ctr, Grid T, double dt_sub,
-                                                   double cap, Rec<P>* rec, const int32_t* spos, int update_rec) {
+struct KThermal {
+    struct Args {
+        Cells<P> c;
+        const Counters* ctr;
+        Grid T;
+        double dt_sub;
+        double cap;
+        Rec<P>* rec;
+        const int32_t* spos;
+        int update_rec;
+    };
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [c, ctr, T, dt_sub, cap, rec, spos, update_rec] = a;
+
     const int n = ctr->nphys;
     const auto body = [&](const int i) {
     const uint32_t fl = c.flags[i];
@@ -154,8 +166,9 @@ __global__ __launch_bounds__(BLOCK) void k_thermal(Cells<P> c, const Counters* c
         if (update_rec) rec[spos[i]].meta &= ~REC_ALIVE;
     }
     };
-    xcd_tiles(n, [&](const int i, const bool valid) { if (valid) body(i); });
-}
+    xcd_tiles(vg, n, [&](const int i, const bool valid) { if (valid) body(i); });
+    }
+};
 
 // Folds a field's fixed-point accumulator into its values at the cells that
 // may have deposited (flush_kernel, src/fields/diffuse.cu:51-60, made sparse).
@@ -164,7 +177,15 @@ __global__ __launch_bounds__(BLOCK) void k_thermal(Cells<P> c, const Counters* c
 // `value += float(from_fixed(total))` as a dense flush. AWAKE marks every
 // thermostat depositor (thermal.cu:130), dead or alive afterwards.
 template <class P>
-__global__ __launch_bounds__(BLOCK) void k_apply_awake(Cells<P> c, const Counters* ctr, Grid g) {
+struct KApplyAwake {
+    struct Args {
+        Cells<P> c;
+        const Counters* ctr;
+        Grid g;
+    };
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [c, ctr, g] = a;
+
     const int n = ctr->nphys;
     const auto body = [&](const int i) {
     const uint32_t fl = c.flags[i];
@@ -174,25 +195,38 @@ __global__ __launch_bounds__(BLOCK) void k_apply_awake(Cells<P> c, const Counter
     const unsigned long long raw = atomicExch(&g.deposit[idx], 0ull);
     if (raw != 0ull) g.value[idx] += static_cast<float>(from_fixed(raw, g.scale));
     };
-    xcd_tiles(n, [&](const int i, const bool valid) { if (valid) body(i); });
-}
+    xcd_tiles(vg, n, [&](const int i, const bool valid) { if (valid) body(i); });
+    }
+};
 
 // ---------------------------------------------------------------------------
 // Stand-alone feeding / taxis stage (emission absorption, taxis_step) over
 // physical entries; the tick runs the same body inside the fused force pass.
 // ---------------------------------------------------------------------------
 template <class P, bool FEED, bool TAXIS>
-__global__ __launch_bounds__(BLOCK) void k_feed_taxis(Cells<P> c, const Counters* ctr, const int32_t* spos,
-                                                      const double* dirs, float ambient, double dt, double max_power) {
+struct KFeedTaxis {
+    struct Args {
+        Cells<P> c;
+        const Counters* ctr;
+        const int32_t* spos;
+        const double* dirs;
+        float ambient;
+        double dt;
+        double max_power;
+    };
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [c, ctr, spos, dirs, ambient, dt, max_power] = a;
+
     const int n = ctr->nphys;
-    xcd_tiles(n, [&](const int i, const bool valid) {
+    xcd_tiles(vg, n, [&](const int i, const bool valid) {
         if (!valid) return;
         const uint32_t f = c.flags[i];
         if (!(f & ASTRO_FLAG_OCCUPIED) || !(f & ASTRO_FLAG_ALIVE)) return;
         const double directional = FEED ? dirs[spos[i]] : 0.0;
         feed_taxis_cell<P, FEED, TAXIS>(c, i, f, directional, ambient, dt, max_power);
     });
-}
+    }
+};
 
 // ---------------------------------------------------------------------------
 // field_sample_kernel (src/sim/integrator.cu:20-30) and the body-force half of
@@ -200,16 +234,31 @@ __global__ __launch_bounds__(BLOCK) void k_feed_taxis(Cells<P> c, const Counters
 // sorted position for the contact walk to add to.
 // ---------------------------------------------------------------------------
 template <class P>
-__global__ __launch_bounds__(BLOCK) void k_sample_base(Cells<P> c, const Counters* ctr, Grid T, Grid C, Grid N,
-                                                       int c_on, int n_on, int gaxis, const int32_t* spos, Force* fs) {
+struct KSampleBase {
+    struct Args {
+        Cells<P> c;
+        const Counters* ctr;
+        Grid T;
+        Grid C;
+        Grid N;
+        int c_on;
+        int n_on;
+        int gaxis;
+        const int32_t* spos;
+        Force* fs;
+    };
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [c, ctr, T, C, N, c_on, n_on, gaxis, spos, fs] = a;
+
     const int n = ctr->nphys;
-    xcd_tiles(n, [&](const int i, const bool valid) {
+    xcd_tiles(vg, n, [&](const int i, const bool valid) {
         if (!valid || c.lslot[i] == NO_SLOT) return;
         const Force F = sample_force_cell<P>(c, i, c.flags[i], static_cast<float>(c.x[i]), static_cast<float>(c.y[i]),
                                              static_cast<float>(c.z[i]), T, C, N, c_on, n_on, gaxis);
         fs[spos[i]] = F;
     });
-}
+    }
+};
 
 // ---------------------------------------------------------------------------
 // integrate_kernel (src/sim/integrator.cu:79-150).
@@ -251,8 +300,18 @@ __device__ __forceinline__ void book_demand(const DemandArgs& a, int i, bool liv
 // then books the moved cell's CO2 demand (lifecycle's first kernel, which
 // reads nothing the steps in between change once the CO2 field has diffused).
 template <class P, bool DEMAND>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_integrate(
-    Cells<P> c, const Counters* ctr, const int32_t* spos, const Force* fs, TickParams t, DemandArgs dm) {
+struct KIntegrate {
+    struct Args {
+        Cells<P> c;
+        const Counters* ctr;
+        const int32_t* spos;
+        const Force* fs;
+        TickParams t;
+        DemandArgs dm;
+    };
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [c, ctr, spos, fs, t, dm] = a;
+
     const int n = ctr->nphys;
     const auto body = [&](const int i) {
     const uint32_t fl = c.flags[i];
@@ -320,8 +379,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 8))) v
         book_demand(dm, i, (nf & ASTRO_FLAG_ALIVE) != 0u, static_cast<float>(sx), static_cast<float>(sy),
                     static_cast<float>(sz));
     };
-    xcd_tiles(n, [&](const int i, const bool valid) { if (valid) body(i); });
-}
+    xcd_tiles(vg, n, [&](const int i, const bool valid) { if (valid) body(i); });
+    }
+};
 
 // ---------------------------------------------------------------------------
 // lifecycle_step (src/sim/lifecycle.cu:208-294), host syncs replaced by
@@ -329,15 +389,24 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 8))) v
 // ---------------------------------------------------------------------------
 // demand_kernel (lifecycle.cu:41-52) on its own, for the stage entry point.
 template <class P>
-__global__ __launch_bounds__(BLOCK) void k_demand(Cells<P> c, const Counters* ctr, DemandArgs dm) {
+struct KDemand {
+    struct Args {
+        Cells<P> c;
+        const Counters* ctr;
+        DemandArgs dm;
+    };
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [c, ctr, dm] = a;
+
     const int n = ctr->nphys;
-    xcd_tiles(n, [&](const int i, const bool valid) {
+    xcd_tiles(vg, n, [&](const int i, const bool valid) {
         if (!valid) return;
         const uint32_t f = c.flags[i];
         book_demand(dm, i, (f & ASTRO_FLAG_OCCUPIED) && (f & ASTRO_FLAG_ALIVE), static_cast<float>(c.x[i]),
                     static_cast<float>(c.y[i]), static_cast<float>(c.z[i]));
     });
-}
+    }
+};
 
 // uptake_kernel (lifecycle.cu:62-86), death_kernel (:100-112) and
 // mark_kernel (:119-135) in one pass; each touches only the cell's own state
@@ -355,11 +424,24 @@ __global__ __launch_bounds__(BLOCK) void k_demand(Cells<P> c, const Counters* ct
 //   * Divide and drop flags are set by reference slot, each with its scan-tile
 //     total.
 template <class P>
-__global__ __launch_bounds__(BLOCK) void k_uptake_mark(Cells<P> c, Counters* ctr, DemandArgs dm, int c_on,
-                                                       int disposition, double quota, int32_t* dflag, int32_t* dsum,
-                                                       int32_t* drop, int32_t* xsum) {
+struct KUptakeMark {
+    struct Args {
+        Cells<P> c;
+        Counters* ctr;
+        DemandArgs dm;
+        int c_on;
+        int disposition;
+        double quota;
+        int32_t* dflag;
+        int32_t* dsum;
+        int32_t* drop;
+        int32_t* xsum;
+    };
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [c, ctr, dm, c_on, disposition, quota, dflag, dsum, drop, xsum] = a;
+
     const int n = ctr->nphys;
-    xcd_tiles(n, [&](const int i, const bool valid) {
+    xcd_tiles(vg, n, [&](const int i, const bool valid) {
         bool divides = false, dead = false;
         if (valid) {
             if (c_on) {
@@ -415,7 +497,8 @@ __global__ __launch_bounds__(BLOCK) void k_uptake_mark(Cells<P> c, Counters* ctr
         wave_count_add(&ctr->births, divides);
         wave_count_add(&ctr->deads, dead);
     });
-}
+    }
+};
 
 // Whether this tick compacts: lifecycle.cu:289 (compaction on and at least
 // one occupied dead slot), or a direct cell_store_compact request.
@@ -427,20 +510,33 @@ __device__ __forceinline__ bool compacting_now(const Counters* ctr, int compacti
 // [0, T) scan the divide flags (when anything divides), blocks [T, 2T) the
 // drop flags (when compacting), T = tiles(capacity). The drop scan also lists
 // the dropped slots in ascending order for the renumbering pass.
-__global__ __launch_bounds__(BLOCK) void k_life_scan(Counters* ctr, int tiles, int compaction, int forced,
-                                                     const int32_t* dflag, const int32_t* dsum, int32_t* prefix,
-                                                     const int32_t* drop, const int32_t* xsum, int32_t* dprefix,
-                                                     uint32_t* dlist) {
+struct KLifeScan {
+    struct Args {
+        Counters* ctr;
+        int tiles;
+        int compaction;
+        int forced;
+        const int32_t* dflag;
+        const int32_t* dsum;
+        int32_t* prefix;
+        const int32_t* drop;
+        const int32_t* xsum;
+        int32_t* dprefix;
+        uint32_t* dlist;
+    };
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [ctr, tiles, compaction, forced, dflag, dsum, prefix, drop, xsum, dprefix, dlist] = a;
+
     const int n = ctr->count;
-    if (static_cast<int>(blockIdx.x) < tiles) {
+    if (vg.block < tiles) {
         if (ctr->births <= 0) return;
         scan_tile(
-            n, static_cast<int>(blockIdx.x), [&](int i) { return load4_i32(dflag, i, n); },
+            n, vg.block, [&](int i) { return load4_i32(dflag, i, n); },
             [&](int i, int4 ex, int4) { store4_i32(prefix, i, n, ex); }, dsum, &ctr->div_total, &ctr->error_flags);
     } else {
         if (!compacting_now(ctr, compaction, forced)) return;
         scan_tile(
-            n, static_cast<int>(blockIdx.x) - tiles, [&](int i) { return load4_i32(drop, i, n); },
+            n, vg.block - tiles, [&](int i) { return load4_i32(drop, i, n); },
             [&](int i, int4 ex, int4 v) {
                 store4_i32(dprefix, i, n, ex);
                 if (v.x) dlist[ex.x] = static_cast<uint32_t>(i);
@@ -450,7 +546,8 @@ __global__ __launch_bounds__(BLOCK) void k_life_scan(Counters* ctr, int tiles, i
             },
             xsum, &ctr->drop_total, &ctr->error_flags);
     }
-}
+    }
+};
 
 // Dropped slots a block keeps in LDS for the renumbering binary search; with
 // more drops than this the pass reads the dense prefix instead.
@@ -467,10 +564,22 @@ constexpr int DROP_LDS_MAX = 4096;
 //     dropped entry becomes a tombstone. Nothing is copied: storage order is
 //     the spatial order, and the next re-sort packs tombstones out.
 template <class P>
-__global__ __launch_bounds__(BLOCK) void k_life_apply(Cells<P> c, Counters* ctr, const int32_t* prefix,
-                                                      const int32_t* drop, const int32_t* dprefix,
-                                                      const uint32_t* dlist, TickParams t, int forced, int c_on,
-                                                      DemandArgs dm) {
+struct KLifeApply {
+    struct Args {
+        Cells<P> c;
+        Counters* ctr;
+        const int32_t* prefix;
+        const int32_t* drop;
+        const int32_t* dprefix;
+        const uint32_t* dlist;
+        TickParams t;
+        int forced;
+        int c_on;
+        DemandArgs dm;
+    };
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [c, ctr, prefix, drop, dprefix, dlist, t, forced, c_on, dm] = a;
+
     __shared__ uint32_t s_drop[DROP_LDS_MAX];
     const int np = ctr->nphys;
     const int n = ctr->count;
@@ -483,7 +592,7 @@ __global__ __launch_bounds__(BLOCK) void k_life_apply(Cells<P> c, Counters* ctr,
         __syncthreads();
     }
     if (births > 0 || compacting || c_on) {
-        xcd_tiles(np, [&](const int i, const bool valid) {
+        xcd_tiles(vg, np, [&](const int i, const bool valid) {
             if (!valid) return;
             if (c_on) {
                 // The CO2 flush of shared voxels (lifecycle.cu:235) by their
@@ -585,13 +694,22 @@ __global__ __launch_bounds__(BLOCK) void k_life_apply(Cells<P> c, Counters* ctr,
             c.n2_local[slot] = c.n2_local[i];
         });
     }
-}
+    }
+};
 
 // The host bookkeeping of lifecycle_step and cell_store_compact
 // (lifecycle.cu:251-294, cell_store.cu:449-455), on device, after the apply
 // pass has read the counters.
-__global__ void k_life_finalize(Counters* ctr, int compaction, int forced) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+struct KLifeFinalize {
+    struct Args {
+        Counters* ctr;
+        int compaction;
+        int forced;
+    };
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [ctr, compaction, forced] = a;
+
+    if (threadIdx.x != 0 || vg.block != 0) return;
     const int32_t n = ctr->count;
     const bool compacting = compacting_now(ctr, compaction, forced);
     const int32_t ndrop = compacting ? ctr->drop_total : 0;
@@ -618,14 +736,24 @@ __global__ void k_life_finalize(Counters* ctr, int compaction, int forced) {
     ctr->dead_slots_prev = compacting ? 0 : deads;
     ctr->births = 0;
     ctr->deads = 0;
-}
+    }
+};
 
 // A direct compaction request (cell_store_compact outside lifecycle): every
 // live entry writes its own slot's drop flag and tile total afresh.
 template <class P>
-__global__ __launch_bounds__(BLOCK) void k_compact_mark(Cells<P> c, const Counters* ctr, int32_t* drop, int32_t* xsum) {
+struct KCompactMark {
+    struct Args {
+        Cells<P> c;
+        const Counters* ctr;
+        int32_t* drop;
+        int32_t* xsum;
+    };
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [c, ctr, drop, xsum] = a;
+
     const int np = ctr->nphys;
-    xcd_tiles(np, [&](const int i, const bool valid) {
+    xcd_tiles(vg, np, [&](const int i, const bool valid) {
         if (!valid) return;
         const uint32_t s = c.lslot[i];
         if (s == NO_SLOT) return;
@@ -634,7 +762,8 @@ __global__ __launch_bounds__(BLOCK) void k_compact_mark(Cells<P> c, const Counte
         drop[s] = keep ? 0 : 1;
         if (!keep) atomicAdd(&xsum[s >> SCAN_TILE_SHIFT], 1);
     });
-}
+    }
+};
 
 // ---------------------------------------------------------------------------
 // stats_cells_kernel (src/sim/stats.cu:27-43): per-element fixed point, summed
@@ -655,7 +784,7 @@ __global__ __launch_bounds__(BLOCK) void k_stats_cells(Cells<P> c, const Counter
     __shared__ unsigned long long s_part[BLOCK / 64][STATS_WORDS];
     const int n = ctr->nphys;
     unsigned long long v[STATS_WORDS] = {0, 0, 0, 0, 0, 0};
-    xcd_tiles(n, [&](const int i, const bool valid) {
+    xcd_tiles(VGrid{static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x)}, n, [&](const int i, const bool valid) {
         if (!valid) return;
         const uint32_t f = c.flags[i];
         if (!(f & ASTRO_FLAG_OCCUPIED)) return;
@@ -706,45 +835,57 @@ void launch_spawn(const Launch& L, const WorldDev<P>& w, const SpawnArgs& a, int
 }
 
 template <class P>
-void launch_thermal_substep(const Launch& L, const WorldDev<P>& w, double dt_sub, double cap, bool update_rec) {
-    PROF_BEGIN("thermal");
-    k_thermal<P><<<cell_grid(w.pcap), BLOCK, 0, L.stream>>>(w.c, w.ctr, w.g[ASTRO_FIELD_TEMPERATURE], dt_sub, cap,
-                                                             w.rec, w.spos, update_rec ? 1 : 0);
-    PROF_END();
+void launch_thermal_substep(const Launch& L, Views<P> ws, Ticks ts, double dt_sub, bool update_rec) {
+    using K = KThermal<P>;
+    Batch<K> b;
+    for (size_t i = 0; i < ws.size(); ++i) {
+        const WorldDev<P>& w = ws[i];
+        b.add({w.c, w.ctr, w.g[ASTRO_FIELD_TEMPERATURE], dt_sub, ts[i].T_cap, w.rec, w.spos, update_rec ? 1 : 0},
+              cell_grid(w.pcap));
+    }
+    launch_k<K>(L, "thermal", b.args, b.blocks);
 }
 
 template <class P>
-void launch_apply_awake(const Launch& L, const WorldDev<P>& w, int field) {
-    PROF_BEGIN("deposit_apply");
-    k_apply_awake<P><<<cell_grid(w.pcap), BLOCK, 0, L.stream>>>(w.c, w.ctr, w.g[field]);
-    PROF_END();
+void launch_apply_awake(const Launch& L, Views<P> ws, int field) {
+    using K = KApplyAwake<P>;
+    Batch<K> b;
+    for (const WorldDev<P>& w : ws) b.add({w.c, w.ctr, w.g[field]}, cell_grid(w.pcap));
+    launch_k<K>(L, "deposit_apply", b.args, b.blocks);
+}
+
+template <class P, bool FEED, bool TAXIS>
+void launch_feed_taxis_t(const Launch& L, Views<P> ws, Ticks ts, const char* name) {
+    using K = KFeedTaxis<P, FEED, TAXIS>;
+    Batch<K> b;
+    for (size_t i = 0; i < ws.size(); ++i) {
+        const WorldDev<P>& w = ws[i];
+        b.add({w.c, w.ctr, w.spos, w.dirs, ts[i].ambient_irr, ts[i].dt, ts[i].petrova_max_power},
+              cell_grid(w.pcap));
+    }
+    launch_k<K>(L, name, b.args, b.blocks);
 }
 
 template <class P>
-void launch_feed_taxis(const Launch& L, const WorldDev<P>& w, const TickParams& t, bool feed, bool taxis) {
-    const int g = cell_grid(w.pcap);
-    PROF_BEGIN(feed ? (taxis ? "feed_taxis" : "feed") : "taxis");
-    if (feed && taxis)
-        k_feed_taxis<P, true, true><<<g, BLOCK, 0, L.stream>>>(w.c, w.ctr, w.spos, w.dirs, t.ambient_irr, t.dt,
-                                                               t.petrova_max_power);
-    else if (feed)
-        k_feed_taxis<P, true, false><<<g, BLOCK, 0, L.stream>>>(w.c, w.ctr, w.spos, w.dirs, t.ambient_irr, t.dt,
-                                                                t.petrova_max_power);
-    else if (taxis)
-        k_feed_taxis<P, false, true><<<g, BLOCK, 0, L.stream>>>(w.c, w.ctr, w.spos, w.dirs, t.ambient_irr, t.dt,
-                                                                t.petrova_max_power);
-    PROF_END();
+void launch_feed_taxis(const Launch& L, Views<P> ws, Ticks ts, bool feed, bool taxis) {
+    if (feed && taxis) launch_feed_taxis_t<P, true, true>(L, ws, ts, "feed_taxis");
+    else if (feed) launch_feed_taxis_t<P, true, false>(L, ws, ts, "feed");
+    else if (taxis) launch_feed_taxis_t<P, false, true>(L, ws, ts, "taxis");
 }
 
 template <class P>
-void launch_cell_forces(const Launch& L, const WorldDev<P>& w, const TickParams& t, bool feed, bool taxis) {
-    if (feed || taxis) launch_feed_taxis<P>(L, w, t, feed, taxis);
-    PROF_BEGIN("sample_base");
-    k_sample_base<P><<<cell_grid(w.pcap), BLOCK, 0, L.stream>>>(
-        w.c, w.ctr, w.g[ASTRO_FIELD_TEMPERATURE], w.g[ASTRO_FIELD_CO2], w.g[ASTRO_FIELD_N2],
-        w.on[ASTRO_FIELD_CO2] ? 1 : 0, w.on[ASTRO_FIELD_N2] ? 1 : 0, t.gravity_axis, w.spos, w.fs);
-    PROF_END();
-    launch_contact_walk<P>(L, w, t);
+void launch_cell_forces(const Launch& L, Views<P> ws, Ticks ts, bool feed, bool taxis) {
+    if (feed || taxis) launch_feed_taxis<P>(L, ws, ts, feed, taxis);
+    using K = KSampleBase<P>;
+    Batch<K> b;
+    for (size_t i = 0; i < ws.size(); ++i) {
+        const WorldDev<P>& w = ws[i];
+        b.add({w.c, w.ctr, w.g[ASTRO_FIELD_TEMPERATURE], w.g[ASTRO_FIELD_CO2], w.g[ASTRO_FIELD_N2],
+               w.on[ASTRO_FIELD_CO2] ? 1 : 0, w.on[ASTRO_FIELD_N2] ? 1 : 0, ts[i].gravity_axis, w.spos, w.fs},
+              cell_grid(w.pcap));
+    }
+    launch_k<K>(L, "sample_base", b.args, b.blocks);
+    launch_contact_walk<P>(L, ws, ts);
 }
 
 template <class P>
@@ -752,53 +893,64 @@ DemandArgs demand_args(const WorldDev<P>& w, const TickParams& t) {
     return DemandArgs{w.g[ASTRO_FIELD_CO2], t.dt_bio, t.co2_gv, w.co2_demand, w.rin, w.co2_avail};
 }
 
+template <class P, bool DEMAND>
+void launch_integrate_t(const Launch& L, Views<P> ws, Ticks ts, const char* name) {
+    using K = KIntegrate<P, DEMAND>;
+    Batch<K> b;
+    for (size_t i = 0; i < ws.size(); ++i)
+        b.add({ws[i].c, ws[i].ctr, ws[i].spos, ws[i].fs, ts[i], demand_args(ws[i], ts[i])}, cell_grid(ws[i].pcap));
+    launch_k<K, true>(L, name, b.args, b.blocks);
+}
+
 template <class P>
-void launch_integrate(const Launch& L, const WorldDev<P>& w, const TickParams& t, bool demand) {
-    const DemandArgs dm = demand_args(w, t);
-    if (demand) {
-        PROF_BEGIN("integrate_demand");
-        k_integrate<P, true><<<cell_grid(w.pcap), BLOCK, 0, L.stream>>>(w.c, w.ctr, w.spos, w.fs, t, dm);
-    } else {
-        PROF_BEGIN("integrate");
-        k_integrate<P, false><<<cell_grid(w.pcap), BLOCK, 0, L.stream>>>(w.c, w.ctr, w.spos, w.fs, t, dm);
-    }
-    PROF_END();
+void launch_integrate(const Launch& L, Views<P> ws, Ticks ts, bool demand) {
+    if (demand) launch_integrate_t<P, true>(L, ws, ts, "integrate_demand");
+    else launch_integrate_t<P, false>(L, ws, ts, "integrate");
 }
 
 // The division / compaction tail shared by the lifecycle and a direct
-// compaction: both scans, then the apply pass with its bookkeeping.
+// compaction: both scans, then the apply pass, then the bookkeeping.
 template <class P>
-void launch_life_tail(const Launch& L, const WorldDev<P>& w, const TickParams& t, bool forced) {
-    const int tiles = scan_tiles_for(w.capacity);
-    const int c_on = (!forced && w.on[ASTRO_FIELD_CO2]) ? 1 : 0;
-    PROF_BEGIN("life_scan");
-    k_life_scan<<<2 * tiles, BLOCK, 0, L.stream>>>(w.ctr, tiles, t.compaction, forced ? 1 : 0, w.dflag, w.dsum,
-                                                  w.prefix, w.drop, w.xsum, w.dprefix, w.dlist);
-    PROF_END();
-    PROF_BEGIN("life_apply");
-    k_life_apply<P><<<cell_grid(w.pcap), BLOCK, 0, L.stream>>>(w.c, w.ctr, w.prefix, w.drop, w.dprefix, w.dlist, t,
-                                                                forced ? 1 : 0, c_on, demand_args(w, t));
-    PROF_END();
-    PROF_BEGIN("life_finalize");
-    k_life_finalize<<<1, 64, 0, L.stream>>>(w.ctr, t.compaction, forced ? 1 : 0);
-    PROF_END();
+void launch_life_tail(const Launch& L, Views<P> ws, Ticks ts, bool forced) {
+    Batch<KLifeScan> scan;
+    Batch<KLifeApply<P>> apply;
+    Batch<KLifeFinalize> fin;
+    for (size_t i = 0; i < ws.size(); ++i) {
+        const WorldDev<P>& w = ws[i];
+        const TickParams& t = ts[i];
+        const int tiles = scan_tiles_for(w.capacity);
+        const int c_on = (!forced && w.on[ASTRO_FIELD_CO2]) ? 1 : 0;
+        scan.add({w.ctr, tiles, t.compaction, forced ? 1 : 0, w.dflag, w.dsum, w.prefix, w.drop, w.xsum, w.dprefix,
+                  w.dlist},
+                 2 * tiles);
+        apply.add({w.c, w.ctr, w.prefix, w.drop, w.dprefix, w.dlist, t, forced ? 1 : 0, c_on, demand_args(w, t)},
+                  cell_grid(w.pcap));
+        fin.add({w.ctr, t.compaction, forced ? 1 : 0}, 1);
+    }
+    launch_k<KLifeScan>(L, "life_scan", scan.args, scan.blocks);
+    launch_k<KLifeApply<P>>(L, "life_apply", apply.args, apply.blocks);
+    launch_k<KLifeFinalize>(L, "life_finalize", fin.args, fin.blocks);
 }
 
 template <class P>
-void launch_lifecycle(const Launch& L, const WorldDev<P>& w, const TickParams& t, bool demand_booked) {
-    const int g = cell_grid(w.pcap);
-    const bool c_on = w.on[ASTRO_FIELD_CO2];
-    const DemandArgs dm = demand_args(w, t);
+void launch_lifecycle(const Launch& L, Views<P> ws, Ticks ts, bool demand_booked) {
+    const bool c_on = ws[0].on[ASTRO_FIELD_CO2];
     if (c_on && !demand_booked) {
-        PROF_BEGIN("co2_demand");
-        k_demand<P><<<g, BLOCK, 0, L.stream>>>(w.c, w.ctr, dm);
-        PROF_END();
+        using K = KDemand<P>;
+        Batch<K> b;
+        for (size_t i = 0; i < ws.size(); ++i) b.add({ws[i].c, ws[i].ctr, demand_args(ws[i], ts[i])}, cell_grid(ws[i].pcap));
+        launch_k<K>(L, "co2_demand", b.args, b.blocks);
     }
-    PROF_BEGIN("uptake_mark");
-    k_uptake_mark<P><<<g, BLOCK, 0, L.stream>>>(w.c, w.ctr, dm, c_on ? 1 : 0, t.store_disposition, t.quota, w.dflag,
-                                                 w.dsum, w.drop, w.xsum);
-    PROF_END();
-    launch_life_tail<P>(L, w, t, false);
+    using K = KUptakeMark<P>;
+    Batch<K> b;
+    for (size_t i = 0; i < ws.size(); ++i) {
+        const WorldDev<P>& w = ws[i];
+        b.add({w.c, w.ctr, demand_args(w, ts[i]), c_on ? 1 : 0, ts[i].store_disposition, ts[i].quota, w.dflag, w.dsum,
+               w.drop, w.xsum},
+              cell_grid(w.pcap));
+    }
+    launch_k<K>(L, "uptake_mark", b.args, b.blocks);
+    launch_life_tail<P>(L, ws, ts, false);
 }
 
 template <class P>
@@ -813,11 +965,10 @@ void launch_life_clear(const Launch& L, const WorldDev<P>& w) {
 template <class P>
 void launch_compact(const Launch& L, const WorldDev<P>& w) {
     launch_life_clear<P>(L, w);
-    PROF_BEGIN("compact_mark");
-    k_compact_mark<P><<<cell_grid(w.pcap), BLOCK, 0, L.stream>>>(w.c, w.ctr, w.drop, w.xsum);
-    PROF_END();
+    using K = KCompactMark<P>;
+    launch_k<K>(L, "compact_mark", {{w.c, w.ctr, w.drop, w.xsum}}, {cell_grid(w.pcap)});
     TickParams t{};
-    launch_life_tail<P>(L, w, t, true);
+    launch_life_tail<P>(L, Views<P>{&w, 1}, Ticks{&t, 1}, true);
 }
 
 template <class P>
@@ -829,12 +980,12 @@ void launch_stats_cells(const Launch& L, const WorldDev<P>& w) {
 #define INSTANTIATE(P)                                                                                          \
     template void launch_spawn<P>(const Launch&, const WorldDev<P>&, const SpawnArgs&, int32_t, int32_t,       \
                                   uint64_t, uint64_t);                                                          \
-    template void launch_thermal_substep<P>(const Launch&, const WorldDev<P>&, double, double, bool);          \
-    template void launch_apply_awake<P>(const Launch&, const WorldDev<P>&, int);                                \
-    template void launch_feed_taxis<P>(const Launch&, const WorldDev<P>&, const TickParams&, bool, bool);       \
-    template void launch_integrate<P>(const Launch&, const WorldDev<P>&, const TickParams&, bool);              \
-    template void launch_cell_forces<P>(const Launch&, const WorldDev<P>&, const TickParams&, bool, bool);      \
-    template void launch_lifecycle<P>(const Launch&, const WorldDev<P>&, const TickParams&, bool);              \
+    template void launch_thermal_substep<P>(const Launch&, Views<P>, Ticks, double, bool);                      \
+    template void launch_apply_awake<P>(const Launch&, Views<P>, int);                                          \
+    template void launch_feed_taxis<P>(const Launch&, Views<P>, Ticks, bool, bool);                             \
+    template void launch_integrate<P>(const Launch&, Views<P>, Ticks, bool);                                    \
+    template void launch_cell_forces<P>(const Launch&, Views<P>, Ticks, bool, bool);                            \
+    template void launch_lifecycle<P>(const Launch&, Views<P>, Ticks, bool);                                    \
     template void launch_compact<P>(const Launch&, const WorldDev<P>&);                                          \
     template void launch_life_clear<P>(const Launch&, const WorldDev<P>&);                                      \
     template void launch_stats_cells<P>(const Launch&, const WorldDev<P>&);

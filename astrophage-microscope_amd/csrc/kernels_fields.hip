@@ -21,10 +21,21 @@ __device__ __forceinline__ float ghost_of(float c, int bc, float amb, float robi
     return c;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_ftcs2d(const float* __restrict__ src, float* __restrict__ dst, int n,
-                                                  float coeff, int bc, float amb, float robin) {
+struct KFtcs2d {
+    struct Args {
+        const float* __restrict__ src;
+        float* __restrict__ dst;
+        int n;
+        float coeff;
+        int bc;
+        float amb;
+        float robin;
+    };
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [src, dst, n, coeff, bc, amb, robin] = a;
+
     const int tx = (n + 63) / 64, ty = (n + 3) / 4;
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int t = xcd_tile(vg.block, vg.grid);
     if (t >= tx * ty) return;
     const int i = (t % tx) * 64 + (threadIdx.x & 63);
     const int j = (t / tx) * 4 + (threadIdx.x >> 6);
@@ -36,13 +47,31 @@ __global__ __launch_bounds__(BLOCK) void k_ftcs2d(const float* __restrict__ src,
     const float d = (j > 0) ? src[id - n] : ghost_of(c, bc, amb, robin);
     const float u = (j < n - 1) ? src[id + n] : ghost_of(c, bc, amb, robin);
     dst[id] = c + coeff * (l + r + d + u - 4.0f * c);
-}
+    }
+};
 
-__global__ __launch_bounds__(BLOCK) void k_ftcs3d(const float* __restrict__ src, float* __restrict__ dst, int nx,
-                                                  int ny, int nz, int zc, float cx, float cy, float cz, int bc,
-                                                  float amb, float rx, float ry, float rz) {
+struct KFtcs3d {
+    struct Args {
+        const float* __restrict__ src;
+        float* __restrict__ dst;
+        int nx;
+        int ny;
+        int nz;
+        int zc;
+        float cx;
+        float cy;
+        float cz;
+        int bc;
+        float amb;
+        float rx;
+        float ry;
+        float rz;
+    };
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [src, dst, nx, ny, nz, zc, cx, cy, cz, bc, amb, rx, ry, rz] = a;
+
     const int tx = (nx + 63) / 64, ty = (ny + 3) / 4, tz = (nz + zc - 1) / zc;
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int t = xcd_tile(vg.block, vg.grid);
     if (t >= tx * ty * tz) return;
     const int i = (t % tx) * 64 + (threadIdx.x & 63);
     const int j = ((t / tx) % ty) * 4 + (threadIdx.x >> 6);
@@ -71,7 +100,8 @@ __global__ __launch_bounds__(BLOCK) void k_ftcs3d(const float* __restrict__ src,
         below = cur;
         cur = above;
     }
-}
+    }
+};
 
 // The same 7-point substep, vectorised: a lane owns 4 consecutive x points
 // (16-byte loads and stores), a wave one 256-point row segment, a block 4
@@ -84,12 +114,28 @@ __device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast
 
 constexpr int FTCS_ZC = 4;
 
-__global__ __launch_bounds__(BLOCK) void k_ftcs3d_v4(const float* __restrict__ src, float* __restrict__ dst, int nx,
-                                                     int ny, int nz, float cx, float cy, float cz, int bc, float amb,
-                                                     float rx, float ry, float rz) {
+struct KFtcs3dV4 {
+    struct Args {
+        const float* __restrict__ src;
+        float* __restrict__ dst;
+        int nx;
+        int ny;
+        int nz;
+        float cx;
+        float cy;
+        float cz;
+        int bc;
+        float amb;
+        float rx;
+        float ry;
+        float rz;
+    };
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [src, dst, nx, ny, nz, cx, cy, cz, bc, amb, rx, ry, rz] = a;
+
     constexpr int ZC = FTCS_ZC;
     const int tx = (nx + 255) / 256, ty = (ny + 3) / 4, tz = (nz + ZC - 1) / ZC;
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int t = xcd_tile(vg.block, vg.grid);
     if (t >= tx * ty * tz) return;   // whole block: shuffles below stay uniform
     const int lane = threadIdx.x & 63;
     const int i0 = (t % tx) * 256 + lane * 4;
@@ -148,11 +194,20 @@ __global__ __launch_bounds__(BLOCK) void k_ftcs3d_v4(const float* __restrict__ s
         }
         *reinterpret_cast<float4*>(dst + k * pl + col) = float4{o[0], o[1], o[2], o[3]};
     }
-}
+    }
+};
 
 // occlusion_stamp_kernel (src/sim/emission.cu:17-27).
 template <class P>
-__global__ __launch_bounds__(BLOCK) void k_stamp(Cells<P> c, const Counters* ctr, Grid irr) {
+struct KStamp {
+    struct Args {
+        Cells<P> c;
+        const Counters* ctr;
+        Grid irr;
+    };
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [c, ctr, irr] = a;
+
     const int n = ctr->nphys;
     const auto body = [&](const int i) {
     const uint32_t f = c.flags[i];
@@ -163,20 +218,35 @@ __global__ __launch_bounds__(BLOCK) void k_stamp(Cells<P> c, const Counters* ctr
                                       static_cast<float>(c.z[i]));
     deposit_add(&irr.deposit[idx], area, irr.scale);
     };
-    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) body(i);
-}
+    for (int i = vg.block * BLOCK + threadIdx.x; i < n; i += vg.grid * BLOCK) body(i);
+    }
+};
 
 // sweep_kernel (src/sim/emission.cu:37-53): one sequential Beer-Lambert march
 // per line, the product kept in the reference's order so transmittances (and
 // an exact 0) match bit for bit. The occlusion accumulator is consumed here:
 // nonzero entries are cleared as they are read, which replaces the reference's
 // dense clear_kernel at the start of the next tick.
-__global__ __launch_bounds__(BLOCK) void k_sweep(unsigned long long* blocked, float* out, int nx, int ny, int nz,
-                                                 int axis, int sign, double scale, double face, float source,
-                                                 float ambient) {
+struct KSweep {
+    struct Args {
+        unsigned long long* blocked;
+        float* out;
+        int nx;
+        int ny;
+        int nz;
+        int axis;
+        int sign;
+        double scale;
+        double face;
+        float source;
+        float ambient;
+    };
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [blocked, out, nx, ny, nz, axis, sign, scale, face, source, ambient] = a;
+
     const int other = axis == 0 ? ny : nx;
     const int len = axis == 0 ? nx : ny;
-    const int line = blockIdx.x * BLOCK + threadIdx.x;
+    const int line = vg.block * BLOCK + threadIdx.x;
     if (line >= other * nz) return;
     const int k = line / other, l = line % other;
     double t = 1.0;
@@ -189,7 +259,8 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(unsigned long long* blocked, fl
         if (raw != 0ull) blocked[idx] = 0ull;
         t *= column_transmittance(from_fixed(raw, scale), face);
     }
-}
+    }
+};
 
 __global__ __launch_bounds__(BLOCK) void k_fill(float* v, size_t n, float value) {
     const size_t i = static_cast<size_t>(blockIdx.x) * BLOCK + threadIdx.x;
@@ -238,46 +309,55 @@ __global__ __launch_bounds__(BLOCK) void k_stats_fields(Grid T, Grid C, Grid N, 
 
 }  // namespace
 
-void launch_ftcs(const Launch& L, const Grid& g, const Ftcs& f, const char* name) {
-    if (f.square2d) {
-        const int tiles = ((g.nx + 63) / 64) * ((g.ny + 3) / 4);
-        if (L.prof) L.prof->begin(name);
-        k_ftcs2d<<<blocks_for_xcd(tiles), BLOCK, 0, L.stream>>>(g.value, g.scratch, g.nx, f.coeff, f.bc, f.amb, f.rx);
-        if (L.prof) L.prof->end();
-        return;
+// The three FTCS variants: square 2D, general 3D with nx % 4 == 0 (float4
+// lanes), and the scalar general 3D kernel. Chambers of one launch are grouped
+// by variant, one batched launch per variant present.
+void launch_ftcs(const Launch& L, std::span<const Grid> gs, std::span<const Ftcs> fs, const char* name) {
+    Batch<KFtcs2d> sq;
+    Batch<KFtcs3dV4> v4;
+    Batch<KFtcs3d> sc;
+    for (size_t i = 0; i < gs.size(); ++i) {
+        const Grid& g = gs[i];
+        const Ftcs& f = fs[i];
+        if (f.square2d) {
+            const int tiles = ((g.nx + 63) / 64) * ((g.ny + 3) / 4);
+            sq.add({g.value, g.scratch, g.nx, f.coeff, f.bc, f.amb, f.rx}, blocks_for_xcd(tiles));
+        } else if (g.nx % 4 == 0) {
+            const int tiles = ((g.nx + 255) / 256) * ((g.ny + 3) / 4) * ((g.nz + FTCS_ZC - 1) / FTCS_ZC);
+            v4.add({g.value, g.scratch, g.nx, g.ny, g.nz, f.cx, f.cy, f.cz, f.bc, f.amb, f.rx, f.ry, f.rz},
+                   blocks_for_xcd(tiles));
+        } else {
+            const int txy = ((g.nx + 63) / 64) * ((g.ny + 3) / 4);
+            int zc = g.nz;
+            while (zc > 4 && txy * ((g.nz + zc - 1) / zc) < 2048) zc = (zc + 1) / 2;
+            const int tiles = txy * ((g.nz + zc - 1) / zc);
+            sc.add({g.value, g.scratch, g.nx, g.ny, g.nz, zc, f.cx, f.cy, f.cz, f.bc, f.amb, f.rx, f.ry, f.rz},
+                   blocks_for_xcd(tiles));
+        }
     }
-    if (g.nx % 4 == 0) {
-        const int tiles = ((g.nx + 255) / 256) * ((g.ny + 3) / 4) * ((g.nz + FTCS_ZC - 1) / FTCS_ZC);
-        if (L.prof) L.prof->begin(name);
-        k_ftcs3d_v4<<<blocks_for_xcd(tiles), BLOCK, 0, L.stream>>>(g.value, g.scratch, g.nx, g.ny, g.nz, f.cx, f.cy,
-                                                                    f.cz, f.bc, f.amb, f.rx, f.ry, f.rz);
-        if (L.prof) L.prof->end();
-        return;
-    }
-    const int txy = ((g.nx + 63) / 64) * ((g.ny + 3) / 4);
-    int zc = g.nz;
-    while (zc > 4 && txy * ((g.nz + zc - 1) / zc) < 2048) zc = (zc + 1) / 2;
-    const int tiles = txy * ((g.nz + zc - 1) / zc);
-    if (L.prof) L.prof->begin(name);
-    k_ftcs3d<<<blocks_for_xcd(tiles), BLOCK, 0, L.stream>>>(g.value, g.scratch, g.nx, g.ny, g.nz, zc, f.cx, f.cy,
-                                                             f.cz, f.bc, f.amb, f.rx, f.ry, f.rz);
-    if (L.prof) L.prof->end();
+    if (!sq.args.empty()) launch_k<KFtcs2d>(L, name, sq.args, sq.blocks);
+    if (!v4.args.empty()) launch_k<KFtcs3dV4>(L, name, v4.args, v4.blocks);
+    if (!sc.args.empty()) launch_k<KFtcs3d>(L, name, sc.args, sc.blocks);
 }
 
 template <class P>
-void launch_stamp(const Launch& L, const WorldDev<P>& w) {
-    if (L.prof) L.prof->begin("occlusion_stamp");
-    k_stamp<P><<<stride_grid(w.pcap), BLOCK, 0, L.stream>>>(w.c, w.ctr, w.g[ASTRO_FIELD_IRRADIANCE]);
-    if (L.prof) L.prof->end();
+void launch_stamp(const Launch& L, Views<P> ws) {
+    Batch<KStamp<P>> b;
+    for (const WorldDev<P>& w : ws) b.add({w.c, w.ctr, w.g[ASTRO_FIELD_IRRADIANCE]}, stride_grid(w.pcap));
+    launch_k<KStamp<P>>(L, "occlusion_stamp", b.args, b.blocks);
 }
 
-void launch_sweep(const Launch& L, const Grid& irr, const TickParams& t) {
-    const int lines = (t.sweep_axis == 0 ? irr.ny : irr.nx) * irr.nz;
-    if (L.prof) L.prof->begin("light_sweep");
-    k_sweep<<<(lines + BLOCK - 1) / BLOCK, BLOCK, 0, L.stream>>>(irr.deposit, irr.value, irr.nx, irr.ny, irr.nz,
-                                                                 t.sweep_axis, t.sweep_sign, irr.scale, t.face_area,
-                                                                 t.source, t.ambient_irr);
-    if (L.prof) L.prof->end();
+void launch_sweep(const Launch& L, std::span<const Grid> irr, Ticks ts) {
+    Batch<KSweep> b;
+    for (size_t i = 0; i < irr.size(); ++i) {
+        const Grid& g = irr[i];
+        const TickParams& t = ts[i];
+        const int lines = (t.sweep_axis == 0 ? g.ny : g.nx) * g.nz;
+        b.add({g.deposit, g.value, g.nx, g.ny, g.nz, t.sweep_axis, t.sweep_sign, g.scale, t.face_area, t.source,
+               t.ambient_irr},
+              (lines + BLOCK - 1) / BLOCK);
+    }
+    launch_k<KSweep>(L, "light_sweep", b.args, b.blocks);
 }
 
 void launch_fill(const Launch& L, float* v, size_t n, float value) {
@@ -293,7 +373,7 @@ void launch_stats_fields(const Launch& L, const Grid& T, const Grid& C, const Gr
     k_stats_fields<<<1024, BLOCK, 0, L.stream>>>(T, C, N, c_on ? 1 : 0, n_on ? 1 : 0, nt, nc, nn, stats);
 }
 
-template void launch_stamp<double>(const Launch&, const WorldDev<double>&);
-template void launch_stamp<float>(const Launch&, const WorldDev<float>&);
+template void launch_stamp<double>(const Launch&, Views<double>);
+template void launch_stamp<float>(const Launch&, Views<float>);
 
 }  // namespace astro_dev

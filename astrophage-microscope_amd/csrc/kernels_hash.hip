@@ -28,12 +28,28 @@ namespace astro_dev {
 namespace {
 
 template <class P>
-__global__ __launch_bounds__(BLOCK) void k_hash_key(Cells<P> c, Counters* ctr, HashGeo h, uint32_t* keys,
-                                                    uint32_t* rin, uint32_t* bcount, int32_t* bsum, int32_t* dflag,
-                                                    int32_t* drop, int32_t* dsum, int32_t* xsum, int cap,
-                                                    int count_tick, int resort_period) {
+struct KHashKey {
+    struct Args {
+        Cells<P> c;
+        Counters* ctr;
+        HashGeo h;
+        uint32_t* keys;
+        uint32_t* rin;
+        uint32_t* bcount;
+        int32_t* bsum;
+        int32_t* dflag;
+        int32_t* drop;
+        int32_t* dsum;
+        int32_t* xsum;
+        int cap;
+        int count_tick;
+        int resort_period;
+    };
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [c, ctr, h, keys, rin, bcount, bsum, dflag, drop, dsum, xsum, cap, count_tick, resort_period] = a;
+
     const int np = ctr->nphys;
-    const int gid = blockIdx.x * BLOCK + threadIdx.x, stride = gridDim.x * BLOCK;
+    const int gid = vg.block * BLOCK + threadIdx.x, stride = vg.grid * BLOCK;
     if (count_tick) {
         if (gid == 0) {
             const int n = ctr->count;
@@ -53,7 +69,7 @@ __global__ __launch_bounds__(BLOCK) void k_hash_key(Cells<P> c, Counters* ctr, H
             xsum[q] = 0;
         }
     }
-    xcd_tiles(np, [&](const int p, const bool valid) {
+    xcd_tiles(vg, np, [&](const int p, const bool valid) {
         const bool live = valid && c.lslot[p] != NO_SLOT;
         uint32_t key = 0u;
         if (live) {
@@ -66,25 +82,46 @@ __global__ __launch_bounds__(BLOCK) void k_hash_key(Cells<P> c, Counters* ctr, H
         }
         wave_count_by_key(bsum, key >> SCAN_TILE_SHIFT, live);
     });
-}
+    }
+};
 
-__global__ __launch_bounds__(BLOCK) void k_bucket_scan(Counters* ctr, uint32_t* bcount, const int32_t* bsum,
-                                                       int32_t* bstart, int nbp1) {
+struct KBucketScan {
+    struct Args {
+        Counters* ctr;
+        uint32_t* bcount;
+        const int32_t* bsum;
+        int32_t* bstart;
+        int nbp1;
+    };
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [ctr, bcount, bsum, bstart, nbp1] = a;
+
     int32_t* counts = reinterpret_cast<int32_t*>(bcount);
     scan_tile(
-        nbp1, static_cast<int>(blockIdx.x),
+        nbp1, vg.block,
         [&](int i) {
             const int4 v = load4_i32(counts, i, nbp1);
             if (v.x | v.y | v.z | v.w) store4_i32(counts, i, nbp1, int4{0, 0, 0, 0});
             return v;
         },
         [&](int i, int4 ex, int4) { store4_i32(bstart, i, nbp1, ex); }, bsum, bstart + nbp1, &ctr->error_flags);
-}
+    }
+};
 
 template <class P>
-__global__ __launch_bounds__(BLOCK) void k_hash_scatter(Cells<P> c, const Counters* ctr, const uint32_t* keys,
-                                                        const uint32_t* rin, const int32_t* bstart, Rec<P>* trec,
-                                                        uint32_t* tphys) {
+struct KHashScatter {
+    struct Args {
+        Cells<P> c;
+        const Counters* ctr;
+        const uint32_t* keys;
+        const uint32_t* rin;
+        const int32_t* bstart;
+        Rec<P>* trec;
+        uint32_t* tphys;
+    };
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [c, ctr, keys, rin, bstart, trec, tphys] = a;
+
     const int np = ctr->nphys;
     const auto body = [&](const int p) {
     const uint32_t s = c.lslot[p];
@@ -100,19 +137,33 @@ __global__ __launch_bounds__(BLOCK) void k_hash_scatter(Cells<P> c, const Counte
     trec[dst] = r;
     tphys[dst] = static_cast<uint32_t>(p);
     };
-    xcd_tiles(np, [&](const int p, const bool valid) { if (valid) body(p); });
-}
+    xcd_tiles(vg, np, [&](const int p, const bool valid) { if (valid) body(p); });
+    }
+};
 
 // Stable order: ascending reference slot inside each bucket.
 template <class P>
-__global__ __launch_bounds__(BLOCK) void k_hash_rank(const Counters* ctr, HashGeo h, const Rec<P>* trec,
-                                                     const uint32_t* tphys, const int32_t* bstart, Rec<P>* rec,
-                                                     uint32_t* vals_sorted, uint32_t* sphys, int32_t* spos,
-                                                     int32_t* bsum, int bsum_tiles) {
+struct KHashRank {
+    struct Args {
+        const Counters* ctr;
+        HashGeo h;
+        const Rec<P>* trec;
+        const uint32_t* tphys;
+        const int32_t* bstart;
+        Rec<P>* rec;
+        uint32_t* vals_sorted;
+        uint32_t* sphys;
+        int32_t* spos;
+        int32_t* bsum;
+        int bsum_tiles;
+    };
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [ctr, h, trec, tphys, bstart, rec, vals_sorted, sphys, spos, bsum, bsum_tiles] = a;
+
     const int n = ctr->count;
     // The bucket scan has consumed the tile totals; clear them for the next
     // build's key pass.
-    for (int q = blockIdx.x * BLOCK + threadIdx.x; q < bsum_tiles; q += gridDim.x * BLOCK) bsum[q] = 0;
+    for (int q = vg.block * BLOCK + threadIdx.x; q < bsum_tiles; q += vg.grid * BLOCK) bsum[q] = 0;
     const auto body = [&](const int k) {
     const Rec<P> r = trec[k];
     const uint32_t slot = r.meta & REC_SLOT_MASK;
@@ -129,37 +180,57 @@ __global__ __launch_bounds__(BLOCK) void k_hash_rank(const Counters* ctr, HashGe
     sphys[dst] = p;
     spos[p] = dst;
     };
-    for (int k = blockIdx.x * BLOCK + threadIdx.x; k < n; k += gridDim.x * BLOCK) body(k);
-}
+    for (int k = vg.block * BLOCK + threadIdx.x; k < n; k += vg.grid * BLOCK) body(k);
+    }
+};
 
 // Re-sort: move every live entry to its sorted position (drops tombstones),
 // then copy back. Runs only on ticks the key kernel marked.
 template <class P>
-__global__ __launch_bounds__(BLOCK) void k_resort_gather(Cells<P> a, Cells<P> b, const Counters* ctr,
-                                                         const uint32_t* sphys) {
+struct KResortGather {
+    struct Args {
+        Cells<P> live;
+        Cells<P> alt;
+        const Counters* ctr;
+        const uint32_t* sphys;
+    };
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [live, alt, ctr, sphys] = a;
+
     if (!ctr->resort) return;
     const int n = ctr->count;
     const auto body = [&](const int k) {
-    copy_cell(a, static_cast<int>(sphys[k]), b, k);
+    copy_cell(live, static_cast<int>(sphys[k]), alt, k);
     };
-    for (int k = blockIdx.x * BLOCK + threadIdx.x; k < n; k += gridDim.x * BLOCK) body(k);
-}
+    for (int k = vg.block * BLOCK + threadIdx.x; k < n; k += vg.grid * BLOCK) body(k);
+    }
+};
 
 template <class P>
-__global__ __launch_bounds__(BLOCK) void k_resort_copyback(Cells<P> a, Cells<P> b, Counters* ctr, uint32_t* sphys,
-                                                           int32_t* spos) {
+struct KResortCopyback {
+    struct Args {
+        Cells<P> live;
+        Cells<P> alt;
+        Counters* ctr;
+        uint32_t* sphys;
+        int32_t* spos;
+    };
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [live, alt, ctr, sphys, spos] = a;
+
     if (!ctr->resort) return;
     const int n = ctr->count;
     // No block of this pass reads nphys, so one thread can already publish
     // the packed extent for the kernels that follow.
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctr->nphys = n;
+    if (vg.block == 0 && threadIdx.x == 0) ctr->nphys = n;
     const auto body = [&](const int k) {
-    copy_cell(b, k, a, k);
+    copy_cell(alt, k, live, k);
     sphys[k] = static_cast<uint32_t>(k);
     spos[k] = k;
     };
-    for (int k = blockIdx.x * BLOCK + threadIdx.x; k < n; k += gridDim.x * BLOCK) body(k);
-}
+    for (int k = vg.block * BLOCK + threadIdx.x; k < n; k += vg.grid * BLOCK) body(k);
+    }
+};
 
 // The record ranges of the 27 buckets around bucket (gx, gy, gz), as nine
 // rows in the reference's walk order (z, then y, then x, then sorted position,
@@ -240,8 +311,18 @@ __device__ __forceinline__ void wave_sync() {
 //     there; F's components are sums of nonzero terms onto a nonzero or +0.0
 //     start and so never -0.0, which makes those adds exact no-ops.
 template <class P>
-__global__ __launch_bounds__(BLOCK) void k_contact_walk(const Counters* ctr, HashGeo h, const Rec<P>* rec,
-                                                        const int32_t* bstart, Force* fs, double k_contact) {
+struct KContactWalk {
+    struct Args {
+        const Counters* ctr;
+        HashGeo h;
+        const Rec<P>* rec;
+        const int32_t* bstart;
+        Force* fs;
+        double k_contact;
+    };
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [ctr, h, rec, bstart, fs, k_contact] = a;
+
     constexpr int SEG = (6 * 1024) / static_cast<int>(sizeof(Rec<P>));   // records per wave and z offset
     __shared__ Rec<P> s_seg[BLOCK / 64][SEG];
     __shared__ int32_t s_queue[CONTACT_QUEUE * BLOCK];
@@ -250,7 +331,7 @@ __global__ __launch_bounds__(BLOCK) void k_contact_walk(const Counters* ctr, Has
     const int n = ctr->count;
     const double two_a = ASTRO_CELL_DIAMETER;
     const float screen = static_cast<float>(two_a * two_a) * (1.0f + 1.0f / 1024.0f);
-    xcd_tiles(n, [&](const int k, const bool valid) {
+    xcd_tiles(vg, n, [&](const int k, const bool valid) {
         Rec<P> r{};
         bool occ = false;
         int gx = 0, gy = 0, gz = 0;
@@ -350,14 +431,29 @@ __global__ __launch_bounds__(BLOCK) void k_contact_walk(const Counters* ctr, Has
         if (queued > 0) drain();
         if (loaded) fs[k] = F;
     });
-}
+    }
+};
 
 // feed_kernel's irradiance half (src/sim/emission.cu:81-105): far-field sample
 // times the exact near-field shadow product over upstream live neighbours.
 template <class P>
-__global__ __launch_bounds__(BLOCK) void k_feed_walk(const Counters* ctr, HashGeo h, const Rec<P>* rec,
-                                                     const int32_t* bstart, Grid irr, double ldx, double ldy,
-                                                     double ldz, float ambient, int occl, double* dirs) {
+struct KFeedWalk {
+    struct Args {
+        const Counters* ctr;
+        HashGeo h;
+        const Rec<P>* rec;
+        const int32_t* bstart;
+        Grid irr;
+        double ldx;
+        double ldy;
+        double ldz;
+        float ambient;
+        int occl;
+        double* dirs;
+    };
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [ctr, h, rec, bstart, irr, ldx, ldy, ldz, ambient, occl, dirs] = a;
+
     const int n = ctr->count;
     const auto body = [&](const int k) {
     const Rec<P> r = rec[k];
@@ -392,70 +488,77 @@ __global__ __launch_bounds__(BLOCK) void k_feed_walk(const Counters* ctr, HashGe
     }
     dirs[k] = directional;
     };
-    xcd_tiles(n, [&](const int k, const bool valid) { if (valid) body(k); });
-}
-
-inline int grid_for(int n) { return (n + BLOCK - 1) / BLOCK; }
+    xcd_tiles(vg, n, [&](const int k, const bool valid) { if (valid) body(k); });
+    }
+};
 
 }  // namespace
 
 template <class P>
-void launch_hash_build(const Launch& L, const WorldDev<P>& w, bool count_tick) {
-    const int gp = cell_grid(w.pcap);
-    const int gc = stride_grid(w.capacity);
-    const int nbp1 = w.hg.nb + 1;
-    if (L.prof) L.prof->begin("hash_key");
-    k_hash_key<P><<<gp, BLOCK, 0, L.stream>>>(w.c, w.ctr, w.hg, w.keys, w.rin, w.bcount, w.bsum, w.dflag, w.drop,
-                                              w.dsum, w.xsum, w.capacity, count_tick ? 1 : 0, w.resort_period);
-    if (L.prof) L.prof->end();
-    if (L.prof) L.prof->begin("hash_bucket_scan");
-    k_bucket_scan<<<scan_tiles_for(nbp1), BLOCK, 0, L.stream>>>(w.ctr, w.bcount, w.bsum, w.bstart, nbp1);
-    if (L.prof) L.prof->end();
-    if (L.prof) L.prof->begin("hash_scatter");
-    k_hash_scatter<P><<<gp, BLOCK, 0, L.stream>>>(w.c, w.ctr, w.keys, w.rin, w.bstart, w.trec, w.tphys);
-    if (L.prof) L.prof->end();
-    if (L.prof) L.prof->begin("hash_rank");
-    k_hash_rank<P><<<gc, BLOCK, 0, L.stream>>>(w.ctr, w.hg, w.trec, w.tphys, w.bstart, w.rec, w.vals_sorted,
-                                                w.sphys, w.spos, w.bsum, scan_tiles_for(nbp1));
-    if (L.prof) L.prof->end();
+void launch_hash_build(const Launch& L, Views<P> ws, bool count_tick) {
+    Batch<KHashKey<P>> key;
+    Batch<KBucketScan> scan;
+    Batch<KHashScatter<P>> scatter;
+    Batch<KHashRank<P>> rank;
+    for (const WorldDev<P>& w : ws) {
+        const int gp = cell_grid(w.pcap);
+        const int nbp1 = w.hg.nb + 1;
+        key.add({w.c, w.ctr, w.hg, w.keys, w.rin, w.bcount, w.bsum, w.dflag, w.drop, w.dsum, w.xsum, w.capacity,
+                 count_tick ? 1 : 0, w.resort_period},
+                gp);
+        scan.add({w.ctr, w.bcount, w.bsum, w.bstart, nbp1}, scan_tiles_for(nbp1));
+        scatter.add({w.c, w.ctr, w.keys, w.rin, w.bstart, w.trec, w.tphys}, gp);
+        rank.add({w.ctr, w.hg, w.trec, w.tphys, w.bstart, w.rec, w.vals_sorted, w.sphys, w.spos, w.bsum,
+                  scan_tiles_for(nbp1)},
+                 stride_grid(w.capacity));
+    }
+    launch_k<KHashKey<P>>(L, "hash_key", key.args, key.blocks);
+    launch_k<KBucketScan>(L, "hash_bucket_scan", scan.args, scan.blocks);
+    launch_k<KHashScatter<P>>(L, "hash_scatter", scatter.args, scatter.blocks);
+    launch_k<KHashRank<P>>(L, "hash_rank", rank.args, rank.blocks);
 }
 
 template <class P>
-void launch_resort(const Launch& L, const WorldDev<P>& w) {
-    const int gc = stride_grid(w.capacity);
-    if (L.prof) L.prof->begin("resort_gather");
-    k_resort_gather<P><<<gc, BLOCK, 0, L.stream>>>(w.c, w.alt, w.ctr, w.sphys);
-    if (L.prof) L.prof->end();
-    if (L.prof) L.prof->begin("resort_copyback");
-    k_resort_copyback<P><<<gc, BLOCK, 0, L.stream>>>(w.c, w.alt, w.ctr, w.sphys, w.spos);
-    if (L.prof) L.prof->end();
+void launch_resort(const Launch& L, Views<P> ws) {
+    Batch<KResortGather<P>> gather;
+    Batch<KResortCopyback<P>> copyback;
+    for (const WorldDev<P>& w : ws) {
+        gather.add({w.c, w.alt, w.ctr, w.sphys}, stride_grid(w.capacity));
+        copyback.add({w.c, w.alt, w.ctr, w.sphys, w.spos}, stride_grid(w.capacity));
+    }
+    launch_k<KResortGather<P>>(L, "resort_gather", gather.args, gather.blocks);
+    launch_k<KResortCopyback<P>>(L, "resort_copyback", copyback.args, copyback.blocks);
 }
 
 template <class P>
-void launch_contact_walk(const Launch& L, const WorldDev<P>& w, const TickParams& t) {
-    if (!t.contact) return;
-    if (L.prof) L.prof->begin("contact_walk");
-    k_contact_walk<P><<<cell_grid(w.capacity), BLOCK, 0, L.stream>>>(w.ctr, w.hg, w.rec, w.bstart, w.fs,
-                                                                     t.k_contact);
-    if (L.prof) L.prof->end();
+void launch_contact_walk(const Launch& L, Views<P> ws, Ticks ts) {
+    if (!ts[0].contact) return;
+    Batch<KContactWalk<P>> b;
+    for (size_t i = 0; i < ws.size(); ++i)
+        b.add({ws[i].ctr, ws[i].hg, ws[i].rec, ws[i].bstart, ws[i].fs, ts[i].k_contact}, cell_grid(ws[i].capacity));
+    launch_k<KContactWalk<P>>(L, "contact_walk", b.args, b.blocks);
 }
 
 template <class P>
-void launch_feed_walk(const Launch& L, const WorldDev<P>& w, const TickParams& t) {
-    const int g = cell_grid(w.capacity);
-    if (L.prof) L.prof->begin("feed_walk");
-    k_feed_walk<P><<<g, BLOCK, 0, L.stream>>>(w.ctr, w.hg, w.rec, w.bstart, w.g[ASTRO_FIELD_IRRADIANCE], t.ldx,
-                                              t.ldy, t.ldz, t.ambient_irr, t.occlusion_exact, w.dirs);
-    if (L.prof) L.prof->end();
+void launch_feed_walk(const Launch& L, Views<P> ws, Ticks ts) {
+    Batch<KFeedWalk<P>> b;
+    for (size_t i = 0; i < ws.size(); ++i) {
+        const WorldDev<P>& w = ws[i];
+        const TickParams& t = ts[i];
+        b.add({w.ctr, w.hg, w.rec, w.bstart, w.g[ASTRO_FIELD_IRRADIANCE], t.ldx, t.ldy, t.ldz, t.ambient_irr,
+               t.occlusion_exact, w.dirs},
+              cell_grid(w.capacity));
+    }
+    launch_k<KFeedWalk<P>>(L, "feed_walk", b.args, b.blocks);
 }
 
-template void launch_hash_build<double>(const Launch&, const WorldDev<double>&, bool);
-template void launch_hash_build<float>(const Launch&, const WorldDev<float>&, bool);
-template void launch_contact_walk<double>(const Launch&, const WorldDev<double>&, const TickParams&);
-template void launch_contact_walk<float>(const Launch&, const WorldDev<float>&, const TickParams&);
-template void launch_feed_walk<double>(const Launch&, const WorldDev<double>&, const TickParams&);
-template void launch_feed_walk<float>(const Launch&, const WorldDev<float>&, const TickParams&);
-template void launch_resort<double>(const Launch&, const WorldDev<double>&);
-template void launch_resort<float>(const Launch&, const WorldDev<float>&);
+#define INSTANTIATE(P)                                                         \
+    template void launch_hash_build<P>(const Launch&, Views<P>, bool);         \
+    template void launch_contact_walk<P>(const Launch&, Views<P>, Ticks);      \
+    template void launch_feed_walk<P>(const Launch&, Views<P>, Ticks);         \
+    template void launch_resort<P>(const Launch&, Views<P>);
+INSTANTIATE(double)
+INSTANTIATE(float)
+#undef INSTANTIATE
 
 }  // namespace astro_dev

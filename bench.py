@@ -20,6 +20,14 @@ ticks eagerly with HIP events around every kernel on the world's stream to
 give per-kernel times and achieved bandwidth against algorithmic bytes.
 The CPU baseline is the CPU restatement of the same step (FP64, one thread)
 on the same workload for a few ticks.
+
+The `ensemble` object measures the batched-ensemble workload: 256 independent
+chambers (1 x 1 x 0.2 mm) with cell counts log-uniform in [500, 50,000] drawn
+from seed 7, each with its own 64^3 FP32 temperature and irradiance grids and a
+random in-plane light direction, every cell awake. All chambers advance
+together, each stage one batched launch over every chamber and each step one
+graph launch; the same chambers stepped one after another with their own
+graphs give the sequential figure beside it.
 """
 from __future__ import annotations
 
@@ -92,6 +100,100 @@ def cpu_baseline(cells: int, steps: int) -> dict:
             "seconds": dt}
 
 
+ENSEMBLE_GRID = {ah.TEMPERATURE: (64, 64, 64), ah.CO2: (16, 16, 4), ah.N2: (-1, 0, 0), ah.IRRADIANCE: (64, 64, 64)}
+
+
+def ensemble_chambers(binding: ah.Binding, chambers: int, seed: int = 7) -> list[ah.World]:
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(np.log(500.0), np.log(50_000.0), chambers)).astype(np.int64)
+    angles = rng.uniform(0.0, 2.0 * np.pi, chambers)
+    worlds = []
+    for i, (n, ang) in enumerate(zip(counts, angles)):
+        d = binding.desc(capacity=int(n * 1.25) + 64, seed=seed * 1000 + i, chamber_w=1e-3, chamber_h=1e-3,
+                         chamber_d=2e-4, precision=ah.FP32_HOT, grid=ENSEMBLE_GRID, physics_rate=0.1,
+                         light_enabled=1, light_irradiance=1e4, light_dir_x=float(np.cos(ang)),
+                         light_dir_y=float(np.sin(ang)))
+        w = ah.World(binding, d)
+        w.spawn(int(n), seed * 1000 + i, charge_dist=ah.DIST_LOG_UNIFORM, charge_a=1e-3, charge_b=0.2, awake=True)
+        worlds.append(w)
+    return worlds
+
+
+def total_cell_steps(worlds: list[ah.World]) -> int:
+    return sum(w.counters().cell_steps for w in worlds)
+
+
+def bench_ensemble(binding: ah.Binding, chambers: int, steps: int, warmup: int, profile_steps: int,
+                   seq_steps: int) -> dict:
+    worlds = ensemble_chambers(binding, chambers)
+    cells = [w.counters().count for w in worlds]
+    e = ah.Ensemble(binding, worlds)
+    e.step(warmup)
+    e.sync()
+    c0 = total_cell_steps(worlds)
+    t0 = time.perf_counter()
+    e.step(steps)
+    e.sync()
+    elapsed = time.perf_counter() - t0
+    done = total_cell_steps(worlds) - c0
+
+    # The same chambers one after another, each replaying its own graph.
+    for w in worlds:
+        w.step(2)
+        w.sync()
+    s0 = total_cell_steps(worlds)
+    t1 = time.perf_counter()
+    for w in worlds:
+        w.step(seq_steps)
+        w.sync()
+    seq_elapsed = time.perf_counter() - t1
+    seq_done = total_cell_steps(worlds) - s0
+
+    # Per-stage pass: eager batched launches with HIP events on the ensemble's stream.
+    e.profile(True)
+    e.profile_reset()
+    e.step(profile_steps)
+    kern = e.profile_read()
+    e.profile(False)
+    per_chamber = []
+    for w in worlds:
+        st = w.stats()
+        per_chamber.append({"n": st["n_live"] + st["n_dead"], "alive": st["n_live"], "awake": st["n_awake"],
+                            "dead": 0, "births": 0, "compacting": 0.0, "p": 4, "buckets": int(np.prod(w.hash_dims()[:3])),
+                            "pts": {0: 64 ** 3, 1: 16 * 16 * 4, 3: 64 ** 3}, "sampled_fields": 2,
+                            "resort_frac": 1.0 / 16.0})
+    stages, total_ms = {}, 0.0
+    for name, (launches, ms) in kern.items():
+        total_ms += ms
+        per = ms / max(launches, 1)
+        algo = sum(roofline.kernel_bytes(name, c) for c in per_chamber)
+        stages[name] = {"launches": launches, "avg_us": round(per * 1e3, 2), "algo_bytes_per_launch": algo,
+                        "achieved_GBs": round(algo / (per * 1e-3) / 1e9, 1) if per > 0 else None}
+    algo_step = sum(stages[k]["algo_bytes_per_launch"] * v[0] for k, v in kern.items()) / max(profile_steps, 1)
+    dom = max(kern.items(), key=lambda kv: kv[1][1])[0]
+    value = done / elapsed
+    seq_value = seq_done / seq_elapsed
+    out = {
+        "workload": "batched ensemble: 256 chambers 1x1x0.2 mm, cells log-uniform [500, 50000] (seed 7), "
+                    "64^3 FP32 T + irradiance each, random in-plane light, all awake, dt 1e-4 s",
+        "chambers": len(worlds), "cells": int(sum(cells)), "cells_min": int(min(cells)), "cells_max": int(max(cells)),
+        "metric": "cell-steps/s (all chambers)", "value": value, "steps": steps, "warmup": warmup,
+        "ms_per_step": elapsed / steps * 1e3, "chamber_steps_per_s": len(worlds) * steps / elapsed,
+        "sequential": {"value": seq_value, "ms_per_step_all_chambers": seq_elapsed / seq_steps * 1e3,
+                       "steps": seq_steps, "how": "each chamber replays its own graph, one chamber after another"},
+        "speedup_vs_sequential": value / seq_value,
+        "step_roofline": {"algo_bytes_per_step": algo_step,
+                          "achieved_GBs_graph": round(algo_step / (elapsed / steps) / 1e9, 1),
+                          "frac_graph": round(algo_step / (elapsed / steps) / 1e9 / HBM_PEAK_GBS, 4),
+                          "dominant_stage": dom},
+        "stages": stages,
+    }
+    e.close()
+    for w in worlds:
+        w.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +204,10 @@ def main() -> int:
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-only", action="store_true")
+    ap.add_argument("--ensemble-chambers", type=int, default=256)
+    ap.add_argument("--ensemble-steps", type=int, default=100)
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -110,6 +216,9 @@ def main() -> int:
         return 2
 
     gpu = ah.load()
+    if a.ensemble_only:
+        print(json.dumps(bench_ensemble(gpu, a.ensemble_chambers, a.ensemble_steps, 10, 10, 10)))
+        return 0
     precision = ah.FP32_HOT if a.precision == "fp32" else ah.FP64_CONTRACT
     w = ah.World(gpu, workload(gpu, a.cells, precision))
     populate(w, a.cells)
@@ -166,6 +275,11 @@ def main() -> int:
                  "achieved_GBs_graph": round(algo_step / (elapsed / a.steps) / 1e9, 1)}
     step_roof["frac_graph"] = round(step_roof["achieved_GBs_graph"] / HBM_PEAK_GBS, 4)
 
+    w.close()
+    ens = None
+    if not a.no_ensemble:
+        ens = bench_ensemble(gpu, a.ensemble_chambers, a.ensemble_steps, 10, 10, 10)
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(a.cells, a.cpu_steps)
@@ -185,8 +299,8 @@ def main() -> int:
         "step_roofline": step_roof,
         "cpu_baseline": cpu,
         "stages": stages,
+        "ensemble": ens,
     }
-    w.close()
     print(json.dumps(out))
     return 0
 

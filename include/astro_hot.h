@@ -254,6 +254,43 @@ int32_t astro_profile_reset(astro_world* w);
 /* Opaque hipStream_t of the world, for callers that order their own work. */
 void*   astro_world_stream(astro_world* w);
 
+/* ---- ensembles: independent chambers stepped together -------------------------
+ * No counterpart in the reference, which steps one world per call
+ * (src/sim/step.cu:109-184). An ensemble borrows n existing worlds (chambers
+ * of any population and capacity) and advances all of them one tick at a
+ * time: each stage is one batched kernel launch over every chamber, and a
+ * step is one hipGraph launch. Every chamber's results are those of
+ * astro_world_step on that world alone.
+ *
+ * The chambers must agree on what the host decides per tick: precision,
+ * physics clock, enabled stages (thermal, light, taxis, lifecycle, contact),
+ * which fields exist and each field's substep count. They may differ in cell
+ * count, capacity, chamber size, light direction and irradiance, seeds and
+ * field contents. Disagreement is reported by create and by step.
+ *
+ * The ensemble runs on a stream of its own, ordered after each chamber's
+ * pending work at every step; each chamber's stream in turn waits for the
+ * step, so the per-world calls (downloads, stats, state hash) see its result.
+ * Worlds must outlive the ensemble and must not be stepped by other callers
+ * while it uses them. */
+typedef struct astro_ensemble astro_ensemble;
+int32_t       astro_ensemble_create(astro_world* const* chambers, int32_t n, astro_ensemble** out);
+void          astro_ensemble_destroy(astro_ensemble* e);
+const char*   astro_ensemble_last_error(const astro_ensemble* e);
+int32_t       astro_ensemble_size(const astro_ensemble* e);
+astro_world*  astro_ensemble_chamber(astro_ensemble* e, int32_t i);
+int32_t       astro_ensemble_step(astro_ensemble* e, int32_t n_ticks);
+/* Waits for the ensemble's work and checks every chamber's device errors. */
+int32_t       astro_ensemble_sync(astro_ensemble* e);
+/* Per-stage HIP-event timing of the batched launches; while enabled, steps
+ * run eagerly (no graph). Same layout as astro_profile_read. */
+int32_t       astro_ensemble_profile_enable(astro_ensemble* e, int32_t on);
+int32_t       astro_ensemble_profile_read(astro_ensemble* e, char* names, int32_t name_stride,
+                                          int64_t* launches, double* total_ms, int32_t max,
+                                          int32_t* n_out);
+int32_t       astro_ensemble_profile_reset(astro_ensemble* e);
+void*         astro_ensemble_stream(astro_ensemble* e);
+
 #ifdef __cplusplus
 }
 #endif

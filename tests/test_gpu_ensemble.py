@@ -1,0 +1,158 @@
+"""Batched ensembles: chambers of uneven populations stepped together, each
+stage one launch over every chamber and each step one graph launch.
+
+The reference steps one world per call (src/sim/step.cu:109-184) and has no
+ensemble, so the oracle here is the world path itself: every chamber of an
+ensemble must end bit-identical (state hash over every SoA array and field,
+plus the counters) to the same world stepped alone, which the world parity
+tests pin against the CPU restatement.
+"""
+import numpy as np
+import pytest
+
+from conftest import ah, compare_cells
+
+pytestmark = pytest.mark.gpu
+
+# Small chambers with 64^3 temperature and irradiance grids, lit from a
+# per-chamber in-plane direction, every cell awake (feed, taxis, light).
+LIT = dict(chamber_w=550e-6, chamber_h=550e-6, chamber_d=100e-6, physics_rate=0.1, light_enabled=1,
+           light_irradiance=1e4, grid={0: (64, 64, 64), 1: (16, 16, 16), 2: (-1, 0, 0), 3: (64, 64, 64)})
+# 2D reference-shaped fields with CO2 rationing, division and compaction.
+LIFE = dict(grid={0: (128, 128, 1), 1: (64, 64, 1), 2: (32, 32, 1), 3: (128, 128, 1)}, compaction_enabled=1,
+            co2_init=2e-4, biology_rate=1e6)
+
+
+def make_world(gpu, n, seed, kw, spawn):
+    w = ah.World(gpu, capacity=2 * n + 64, **kw)
+    w.spawn(n, seed, **spawn)
+    return w
+
+
+def assert_same(a: ah.World, b: ah.World, tag: str):
+    ca, cb = a.counters(), b.counters()
+    assert (ca.count, ca.next_id, ca.divisions_total, ca.deaths_total) == \
+           (cb.count, cb.next_id, cb.divisions_total, cb.deaths_total), tag
+    if a.state_hash() != b.state_hash():
+        # name the first array that differs
+        da, db = a.download(), b.download()
+        compare_cells(da, db)
+        for k in da:
+            assert np.array_equal(da[k], db[k]), f"{tag}: {k}"
+        for f in (ah.TEMPERATURE, ah.CO2, ah.N2):
+            if a.field_dims(f)[0][0] > 0:
+                assert np.array_equal(a.field(f), b.field(f)), f"{tag}: field {f}"
+        raise AssertionError(f"{tag}: state hash differs")
+
+
+def lit_chambers(gpu, counts, precision, seed0):
+    rng = np.random.default_rng(seed0)
+    out = []
+    for i, n in enumerate(counts):
+        ang = rng.uniform(0.0, 2.0 * np.pi)
+        kw = dict(LIT, precision=precision, light_dir_x=float(np.cos(ang)), light_dir_y=float(np.sin(ang)),
+                  seed=seed0 + i)
+        spawn = dict(charge_dist=ah.DIST_UNIFORM, charge_a=0.0, charge_b=0.2, awake=True)
+        out.append((n, seed0 + 100 + i, kw, spawn))
+    return out
+
+
+@pytest.mark.parametrize("precision", [ah.FP32_HOT, ah.FP64_CONTRACT])
+def test_ensemble_matches_standalone_lit(gpu, precision):
+    """Six lit chambers, 40 to 5,000 cells, each with its own light
+    direction: graph-replayed and left-over eager ticks both match."""
+    specs = lit_chambers(gpu, [40, 700, 2500, 150, 1800, 5000], precision, 11)
+    ens_worlds = [make_world(gpu, *s) for s in specs]
+    solo = [make_world(gpu, *s) for s in specs]
+    with ah.Ensemble(gpu, ens_worlds) as e:
+        assert len(e) == len(specs)
+        for ticks in (5, 1, 2):
+            e.step(ticks)
+            for w in solo:
+                w.step(ticks)
+            e.sync()
+            for i, (a, b) in enumerate(zip(ens_worlds, solo)):
+                assert_same(a, b, f"chamber {i} after {ticks}")
+    for w in ens_worlds + solo:
+        w.close()
+
+
+@pytest.mark.parametrize("precision", [ah.FP64_CONTRACT, ah.FP32_HOT])
+def test_ensemble_matches_standalone_lifecycle(gpu, precision):
+    """Crowded CO2-starved chambers that divide, die and compact every tick,
+    with different populations and seeds."""
+    specs = []
+    for i, n in enumerate([300, 2200, 900, 60]):
+        kw = dict(LIFE, precision=precision, seed=30 + i)
+        spawn = dict(placement=ah.PLACE_GAUSSIAN, place_radius=60e-6, charge_dist=ah.DIST_UNIFORM, charge_b=0.9)
+        specs.append((n, 40 + i, kw, spawn))
+    ens_worlds, solo = [], []
+    rng = np.random.default_rng(3)
+    quota = 2.1e-14
+    for s in specs:
+        a, b = make_world(gpu, *s), make_world(gpu, *s)
+        cells = a.download()
+        n = len(cells["id"])
+        cells["co2_held"] = rng.uniform(0.0, 1.0, n) * quota
+        near = rng.random(n) < 0.1
+        cells["co2_held"][near] = quota - rng.uniform(0.0, 4.0, near.sum()) * 6e-20
+        dead = rng.random(n) < 0.05
+        cells["flags"][dead] &= ~np.uint32(ah.FLAG_ALIVE)
+        a.upload(cells)
+        b.upload(cells)
+        ens_worlds.append(a)
+        solo.append(b)
+    with ah.Ensemble(gpu, ens_worlds) as e:
+        for _ in range(4):
+            e.step(1)
+            for w in solo:
+                w.step(1)
+        e.step(6)
+        for w in solo:
+            w.step(6)
+        e.sync()
+        for i, (a, b) in enumerate(zip(ens_worlds, solo)):
+            assert_same(a, b, f"chamber {i}")
+        assert sum(w.counters().divisions_total for w in solo) > 0
+        assert sum(w.counters().deaths_total for w in solo) > 0
+    for w in ens_worlds + solo:
+        w.close()
+
+
+def test_ensemble_profiled_steps(gpu):
+    """Profiled (eager, per-stage timed) ensemble steps give the same state
+    and time every batched stage."""
+    specs = lit_chambers(gpu, [500, 3000, 90], ah.FP32_HOT, 70)
+    ens_worlds = [make_world(gpu, *s) for s in specs]
+    solo = [make_world(gpu, *s) for s in specs]
+    with ah.Ensemble(gpu, ens_worlds) as e:
+        e.profile(True)
+        e.step(3)
+        for w in solo:
+            w.step(3)
+        e.sync()
+        prof = e.profile_read()
+        for stage in ("hash_key", "thermal", "feed_walk", "contact_walk", "integrate_demand", "ftcs_T", "life_apply"):
+            assert stage in prof and prof[stage][0] > 0, stage
+        for i, (a, b) in enumerate(zip(ens_worlds, solo)):
+            assert_same(a, b, f"chamber {i}")
+    for w in ens_worlds + solo:
+        w.close()
+
+
+def test_ensemble_rejects_mismatched_chambers(gpu):
+    a = ah.World(gpu, capacity=100, precision=ah.FP32_HOT, **LIT)
+    b = ah.World(gpu, capacity=100, precision=ah.FP64_CONTRACT, **LIT)
+    c = ah.World(gpu, capacity=100, precision=ah.FP32_HOT, **LIT)
+    with pytest.raises(ah.AstroError) as err:
+        ah.Ensemble(gpu, [a, b])
+    assert err.value.status == ah.INVALID_ARGUMENT
+    with pytest.raises(ah.AstroError):
+        ah.Ensemble(gpu, [a, a])
+    with ah.Ensemble(gpu, [a, c]) as e:
+        c.set_clock(0.2, 1.0)   # a different physics clock: caught at the next step
+        with pytest.raises(ah.AstroError) as err:
+            e.step(1)
+        assert err.value.status == ah.INVALID_ARGUMENT
+    for w in (a, b, c):
+        w.close()
