@@ -16,6 +16,7 @@ through the same wrapper.
 from __future__ import annotations
 
 import ctypes as C
+import json
 from pathlib import Path
 
 import numpy as np
@@ -178,6 +179,8 @@ PRODUCT_ONLY = {
     "ensemble_chamber": (_P, [_P, _I]),
     "ensemble_step": (_I, [_P, _I]),
     "ensemble_sync": (_I, [_P]),
+    "ensemble_autotune": (_I, [_P, _I]),
+    "ensemble_tuning": (_I, [_P, C.c_char_p, _I]),
     "ensemble_profile_enable": (_I, [_P, _I]),
     "ensemble_profile_read": (_I, [_P, _P, _I, _P, _P, _I, C.POINTER(_I)]),
     "ensemble_profile_reset": (_I, [_P]),
@@ -481,6 +484,17 @@ class Ensemble:
 
     def sync(self):
         self._check(self.b.ensemble_sync(self.h), "ensemble_sync")
+
+    def autotune(self, ticks_per_trial: int = 2) -> dict:
+        """Times candidate launch shapes per stage on real ticks and keeps the
+        fastest; the chambers advance by eleven trials of `ticks_per_trial`."""
+        self._check(self.b.ensemble_autotune(self.h, ticks_per_trial), "ensemble_autotune")
+        return self.tuning()
+
+    def tuning(self) -> dict:
+        buf = C.create_string_buffer(8192)
+        self._check(self.b.ensemble_tuning(self.h, buf, len(buf)), "ensemble_tuning")
+        return json.loads(buf.value.decode())
 
     def profile(self, on: bool):
         self._check(self.b.ensemble_profile_enable(self.h, int(on)))

@@ -400,6 +400,23 @@ __host__ __device__ __forceinline__ int cell_grid(int n) { return blocks_for_xcd
 
 // A kernel body's own block index and block count: the launch's for a
 // single world, the chamber's share of it for a batched launch.
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = min(v, __shfl_xor(v, d, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = max(v, __shfl_xor(v, d, 64));
+    return v;
+}
+// Orders one wave's LDS writes before its later LDS reads by other lanes.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 struct VGrid {
     int block, grid;
 };
@@ -409,10 +426,18 @@ struct VGrid {
 // so the cells one L2 serves are spatial neighbours (cells are stored in
 // spatial-hash order) and share field voxels and neighbour records. Whole
 // blocks iterate together (`valid` is false past n), so the body may use
-// wave-wide operations. vg.grid must be a multiple of NUM_XCD.
+// wave-wide operations. A grid that is not a multiple of NUM_XCD (a small
+// chamber of a batched launch) walks the tiles in plain grid-stride order.
 template <class Body>
 __device__ __forceinline__ void xcd_tiles(VGrid vg, int n, Body&& body) {
     const int ntiles = (n + BLOCK - 1) / BLOCK;
+    if (vg.grid % NUM_XCD != 0) {
+        for (int t = vg.block; t < ntiles; t += vg.grid) {
+            const int i = t * BLOCK + static_cast<int>(threadIdx.x);
+            body(i, i < n);
+        }
+        return;
+    }
     const int x = vg.block % NUM_XCD;
     const int nb = vg.grid / NUM_XCD;
     const int per = (ntiles + NUM_XCD - 1) / NUM_XCD;

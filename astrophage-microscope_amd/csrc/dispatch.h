@@ -5,12 +5,12 @@
 // world's device view and scalars, by value) and a device body
 // K::run(args, vgrid). `vgrid` is the body's own block index and block count:
 //   * k_one<K>:  one world; vgrid = (blockIdx.x, gridDim.x);
-//   * k_many<K>: chamber c owns blocks [first[c], first[c + 1]) of the launch
-//     and reads its arguments from args[c] in device memory; vgrid is the
-//     block's index inside that range and the range's size.
-// Each chamber's block count is a multiple of NUM_XCD, so a block's XCD
-// (blockIdx.x % NUM_XCD) equals its virtual block's, and the XCD-contiguous
-// tile split of xcd_tiles holds per chamber.
+//   * k_many<K>: chamber c owns a contiguous range of the launch's blocks,
+//     found through a per-block map, and reads its arguments from args[c] in
+//     device memory; vgrid is the block's index inside that range and the
+//     range's size.
+// Blocks of one chamber that share vgrid.block % NUM_XCD share an XCD, so the
+// XCD-contiguous tile split of xcd_tiles holds per chamber.
 //
 // On the host, a batched launch goes through a Recorder: the per-chamber
 // argument arrays and block prefixes are appended to one staging buffer, and
@@ -20,6 +20,8 @@
 #pragma once
 
 #include <functional>
+#include <map>
+#include <string>
 #include <span>
 #include <vector>
 
@@ -27,17 +29,8 @@
 
 namespace astro_dev {
 
-__device__ __forceinline__ int chamber_of(const int* first, int nch) {
-    const int b = static_cast<int>(blockIdx.x);
-    int lo = 0, hi = nch;   // largest c with first[c] <= b
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (first[mid] <= b) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
+// Block b of a batched launch belongs to chamber map[b].x, whose blocks start
+// at map[b].y and number map[b].z.
 template <class K>
 __global__ __launch_bounds__(BLOCK) void k_one(typename K::Args a) {
     K::run(a, VGrid{static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x)});
@@ -46,10 +39,10 @@ __global__ __launch_bounds__(BLOCK) void k_one(typename K::Args a) {
 // The chamber's arguments are copied out of device memory first: read through
 // the pointer, every field would be re-read after each store the body makes.
 template <class K>
-__global__ __launch_bounds__(BLOCK) void k_many(const typename K::Args* a, const int* first, int nch) {
-    const int c = chamber_of(first, nch);
-    const typename K::Args args = a[c];
-    K::run(args, VGrid{static_cast<int>(blockIdx.x) - first[c], first[c + 1] - first[c]});
+__global__ __launch_bounds__(BLOCK) void k_many(const typename K::Args* a, const int4* map) {
+    const int4 m = map[blockIdx.x];
+    const typename K::Args args = a[m.x];
+    K::run(args, VGrid{static_cast<int>(blockIdx.x) - m.y, m.z});
 }
 
 // The same wrappers for bodies that need three waves per SIMD of registers.
@@ -60,10 +53,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 8))) v
 
 template <class K>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_many_w3(
-    const typename K::Args* a, const int* first, int nch) {
-    const int c = chamber_of(first, nch);
-    const typename K::Args args = a[c];
-    K::run(args, VGrid{static_cast<int>(blockIdx.x) - first[c], first[c + 1] - first[c]});
+    const typename K::Args* a, const int4* map) {
+    const int4 m = map[blockIdx.x];
+    const typename K::Args args = a[m.x];
+    K::run(args, VGrid{static_cast<int>(blockIdx.x) - m.y, m.z});
 }
 
 // Host: batched launches of one ensemble step, recorded for later issue.
@@ -72,8 +65,21 @@ struct Recorder {
         const char* name;                                // stage name, for timing
         std::function<void(hipStream_t, const char*)> fn;  // issues the launch given the device arena
     };
-    std::vector<char> staging;   // arguments and block prefixes, 256-byte aligned entries
+    std::vector<char> staging;   // arguments and block maps, 256-byte aligned entries
     std::vector<Entry> launches;
+    std::map<std::vector<int>, size_t> maps;   // block counts -> offset of their block map
+    // Total blocks a batched launch of a grid-stride body is spread over
+    // (0: one block per tile of each chamber's capacity, as a world alone),
+    // by default and per stage name.
+    int stride_blocks = 0;
+    std::map<std::string, int> stage_blocks;
+    // Lanes per x row of the float4 FTCS per stage name (0: widest that nx fills).
+    std::map<std::string, int> ftcs_lanes;
+
+    int blocks_for(const char* stage) const {
+        const auto it = stage_blocks.find(stage);
+        return it == stage_blocks.end() ? stride_blocks : it->second;
+    }
 
     size_t put(const void* p, size_t n) {
         const size_t off = (staging.size() + 255) & ~static_cast<size_t>(255);
@@ -81,10 +87,40 @@ struct Recorder {
         std::copy(static_cast<const char*>(p), static_cast<const char*>(p) + n, staging.data() + off);
         return off;
     }
+    // The block map of a launch with these per-chamber block counts.
+    size_t block_map(const std::vector<int>& blocks) {
+        const auto it = maps.find(blocks);
+        if (it != maps.end()) return it->second;
+        std::vector<int4> m;
+        int first = 0;
+        for (size_t c = 0; c < blocks.size(); ++c) {
+            for (int b = 0; b < blocks[c]; ++b) m.push_back(int4{static_cast<int>(c), first, blocks[c], 0});
+            first += blocks[c];
+        }
+        const size_t off = put(m.data(), sizeof(int4) * m.size());
+        maps.emplace(blocks, off);
+        return off;
+    }
     void clear() {
         staging.clear();
         launches.clear();
+        maps.clear();
     }
 };
+
+// Spreads a grid-stride launch over about `target` blocks in total, each
+// chamber's share proportional to its block count: at least one block, and a
+// multiple of NUM_XCD once it reaches NUM_XCD.
+inline void spread_blocks(std::vector<int>& blocks, int target) {
+    long long total = 0;
+    for (int b : blocks) total += b;
+    if (target <= 0 || total <= target) return;
+    for (int& b : blocks) {
+        int s = static_cast<int>((static_cast<long long>(b) * target + total - 1) / total);
+        if (s < 1) s = 1;
+        if (s >= NUM_XCD) s = ((s + NUM_XCD - 1) / NUM_XCD) * NUM_XCD;
+        b = s < b ? s : b;
+    }
+}
 
 }  // namespace astro_dev

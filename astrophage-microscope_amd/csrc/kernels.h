@@ -33,9 +33,12 @@ struct Launch {
 // of NUM_XCD for bodies that tile with xcd_tiles). Without a recorder there is
 // one chamber and the launch happens now; with one, a single batched launch
 // over all chambers is recorded.
+template <class K>
+concept GridStride = requires { K::stride; } && static_cast<bool>(K::stride);
+
 template <class K, bool W3 = false>
 void launch_k(const Launch& L, const char* name, const std::vector<typename K::Args>& args,
-              const std::vector<int>& blocks) {
+              std::vector<int> blocks) {
     if (!L.rec) {
         if (L.prof) L.prof->begin(name);
         if constexpr (W3) k_one_w3<K><<<blocks[0], BLOCK, 0, L.stream>>>(args[0]);
@@ -43,17 +46,17 @@ void launch_k(const Launch& L, const char* name, const std::vector<typename K::A
         if (L.prof) L.prof->end();
         return;
     }
-    std::vector<int> first(args.size() + 1, 0);
-    for (size_t i = 0; i < args.size(); ++i) first[i + 1] = first[i] + blocks[i];
-    const int total = first.back(), nch = static_cast<int>(args.size());
+    if constexpr (GridStride<K>) spread_blocks(blocks, L.rec->blocks_for(name));
+    int total = 0;
+    for (int b : blocks) total += b;
     if (total == 0) return;
     const size_t a_off = L.rec->put(args.data(), sizeof(typename K::Args) * args.size());
-    const size_t f_off = L.rec->put(first.data(), sizeof(int) * first.size());
+    const size_t m_off = L.rec->block_map(blocks);
     L.rec->launches.push_back({name, [=](hipStream_t s, const char* arena) {
         const auto* a = reinterpret_cast<const typename K::Args*>(arena + a_off);
-        const int* f = reinterpret_cast<const int*>(arena + f_off);
-        if constexpr (W3) k_many_w3<K><<<total, BLOCK, 0, s>>>(a, f, nch);
-        else k_many<K><<<total, BLOCK, 0, s>>>(a, f, nch);
+        const int4* m = reinterpret_cast<const int4*>(arena + m_off);
+        if constexpr (W3) k_many_w3<K><<<total, BLOCK, 0, s>>>(a, m);
+        else k_many<K><<<total, BLOCK, 0, s>>>(a, m);
     }});
 }
 

@@ -116,6 +116,7 @@ __global__ __launch_bounds__(BLOCK) void k_spawn(Cells<P> c, SpawnArgs p, int32_
 // ---------------------------------------------------------------------------
 template <class P>
 struct KThermal {
+    static constexpr bool stride = true;   // any block count covers the work
     struct Args {
         Cells<P> c;
         const Counters* ctr;
@@ -178,6 +179,7 @@ struct KThermal {
 // thermostat depositor (thermal.cu:130), dead or alive afterwards.
 template <class P>
 struct KApplyAwake {
+    static constexpr bool stride = true;   // any block count covers the work
     struct Args {
         Cells<P> c;
         const Counters* ctr;
@@ -205,6 +207,7 @@ struct KApplyAwake {
 // ---------------------------------------------------------------------------
 template <class P, bool FEED, bool TAXIS>
 struct KFeedTaxis {
+    static constexpr bool stride = true;   // any block count covers the work
     struct Args {
         Cells<P> c;
         const Counters* ctr;
@@ -235,6 +238,7 @@ struct KFeedTaxis {
 // ---------------------------------------------------------------------------
 template <class P>
 struct KSampleBase {
+    static constexpr bool stride = true;   // any block count covers the work
     struct Args {
         Cells<P> c;
         const Counters* ctr;
@@ -301,6 +305,7 @@ __device__ __forceinline__ void book_demand(const DemandArgs& a, int i, bool liv
 // reads nothing the steps in between change once the CO2 field has diffused).
 template <class P, bool DEMAND>
 struct KIntegrate {
+    static constexpr bool stride = true;   // any block count covers the work
     struct Args {
         Cells<P> c;
         const Counters* ctr;
@@ -390,6 +395,7 @@ struct KIntegrate {
 // demand_kernel (lifecycle.cu:41-52) on its own, for the stage entry point.
 template <class P>
 struct KDemand {
+    static constexpr bool stride = true;   // any block count covers the work
     struct Args {
         Cells<P> c;
         const Counters* ctr;
@@ -425,6 +431,7 @@ struct KDemand {
 //     total.
 template <class P>
 struct KUptakeMark {
+    static constexpr bool stride = true;   // any block count covers the work
     struct Args {
         Cells<P> c;
         Counters* ctr;
@@ -565,6 +572,7 @@ constexpr int DROP_LDS_MAX = 4096;
 //     the spatial order, and the next re-sort packs tombstones out.
 template <class P>
 struct KLifeApply {
+    static constexpr bool stride = true;   // any block count covers the work
     struct Args {
         Cells<P> c;
         Counters* ctr;
@@ -743,6 +751,7 @@ struct KLifeFinalize {
 // live entry writes its own slot's drop flag and tile total afresh.
 template <class P>
 struct KCompactMark {
+    static constexpr bool stride = true;   // any block count covers the work
     struct Args {
         Cells<P> c;
         const Counters* ctr;
@@ -833,6 +842,9 @@ void launch_spawn(const Launch& L, const WorldDev<P>& w, const SpawnArgs& a, int
     k_spawn<P><<<grid_for(a.count), BLOCK, 0, L.stream>>>(w.c, a, base, lbase, first_id, seed);
     PROF_END();
 }
+
+static_assert(GridStride<KThermal<double>> && GridStride<KLifeApply<float>>);
+static_assert(!GridStride<KLifeScan> && !GridStride<KLifeFinalize>);
 
 template <class P>
 void launch_thermal_substep(const Launch& L, Views<P> ws, Ticks ts, double dt_sub, bool update_rec) {

@@ -114,32 +114,39 @@ __device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast
 
 constexpr int FTCS_ZC = 4;
 
+// W lanes per x row (4 points each): a wave covers 64 / W rows, a block
+// 4 x 64 / W rows, so narrow grids keep every lane busy.
+struct Ftcs3Args {
+    const float* __restrict__ src;
+    float* __restrict__ dst;
+    int nx;
+    int ny;
+    int nz;
+    float cx;
+    float cy;
+    float cz;
+    int bc;
+    float amb;
+    float rx;
+    float ry;
+    float rz;
+};
+
+template <int W>
 struct KFtcs3dV4 {
-    struct Args {
-        const float* __restrict__ src;
-        float* __restrict__ dst;
-        int nx;
-        int ny;
-        int nz;
-        float cx;
-        float cy;
-        float cz;
-        int bc;
-        float amb;
-        float rx;
-        float ry;
-        float rz;
-    };
+    using Args = Ftcs3Args;
     static __device__ void run(const Args& a, VGrid vg) {
         const auto& [src, dst, nx, ny, nz, cx, cy, cz, bc, amb, rx, ry, rz] = a;
 
     constexpr int ZC = FTCS_ZC;
-    const int tx = (nx + 255) / 256, ty = (ny + 3) / 4, tz = (nz + ZC - 1) / ZC;
+    constexpr int RB = 4 * (64 / W);   // rows per block
+    const int tx = (nx + 4 * W - 1) / (4 * W), ty = (ny + RB - 1) / RB, tz = (nz + ZC - 1) / ZC;
     const int t = xcd_tile(vg.block, vg.grid);
     if (t >= tx * ty * tz) return;   // whole block: shuffles below stay uniform
     const int lane = threadIdx.x & 63;
-    const int i0 = (t % tx) * 256 + lane * 4;
-    const int j = ((t / tx) % ty) * 4 + (threadIdx.x >> 6);
+    const int xl = lane % W;           // lane within its row
+    const int i0 = (t % tx) * (4 * W) + xl * 4;
+    const int j = ((t / tx) % ty) * RB + static_cast<int>(threadIdx.x) / W;
     const int k0 = (t / (tx * ty)) * ZC;
     const bool on = i0 < nx && j < ny;
     const size_t pl = static_cast<size_t>(nx) * ny;
@@ -159,8 +166,8 @@ struct KFtcs3dV4 {
         const size_t id = k * pl + col;
         dn4[m] = (ok && j > 0) ? ld4(src + id - nx) : zero4;
         up4[m] = (ok && j < ny - 1) ? ld4(src + id + nx) : zero4;
-        lf[m] = (ok && lane == 0 && i0 > 0) ? src[id - 1] : 0.0f;
-        rg[m] = (ok && (lane == 63 || i0 + 4 >= nx) && i0 + 4 < nx) ? src[id + 4] : 0.0f;
+        lf[m] = (ok && xl == 0 && i0 > 0) ? src[id - 1] : 0.0f;
+        rg[m] = (ok && (xl == W - 1 || i0 + 4 >= nx) && i0 + 4 < nx) ? src[id + 4] : 0.0f;
     }
 #pragma unroll
     for (int m = 0; m < ZC; ++m) {
@@ -169,8 +176,8 @@ struct KFtcs3dV4 {
         // x neighbours of the lane's first and last point
         float lft = __shfl_up(cur.w, 1, 64);
         float rgt = __shfl_down(cur.x, 1, 64);
-        if (lane == 0) lft = lf[m];
-        if (lane == 63 || i0 + 4 >= nx) rgt = rg[m];
+        if (xl == 0) lft = lf[m];
+        if (xl == W - 1 || i0 + 4 >= nx) rgt = rg[m];
         if (!on || k >= nz) continue;
         const float c[4] = {cur.x, cur.y, cur.z, cur.w};
         const float dd[4] = {dn4[m].x, dn4[m].y, dn4[m].z, dn4[m].w};
@@ -200,6 +207,7 @@ struct KFtcs3dV4 {
 // occlusion_stamp_kernel (src/sim/emission.cu:17-27).
 template <class P>
 struct KStamp {
+    static constexpr bool stride = true;   // any block count covers the work
     struct Args {
         Cells<P> c;
         const Counters* ctr;
@@ -227,20 +235,22 @@ struct KStamp {
 // an exact 0) match bit for bit. The occlusion accumulator is consumed here:
 // nonzero entries are cleared as they are read, which replaces the reference's
 // dense clear_kernel at the start of the next tick.
+struct SweepArgs {
+    unsigned long long* blocked;
+    float* out;
+    int nx;
+    int ny;
+    int nz;
+    int axis;
+    int sign;
+    double scale;
+    double face;
+    float source;
+    float ambient;
+};
+
 struct KSweep {
-    struct Args {
-        unsigned long long* blocked;
-        float* out;
-        int nx;
-        int ny;
-        int nz;
-        int axis;
-        int sign;
-        double scale;
-        double face;
-        float source;
-        float ambient;
-    };
+    using Args = SweepArgs;
     static __device__ void run(const Args& a, VGrid vg) {
         const auto& [blocked, out, nx, ny, nz, axis, sign, scale, face, source, ambient] = a;
 
@@ -259,6 +269,57 @@ struct KSweep {
         if (raw != 0ull) blocked[idx] = 0ull;
         t *= column_transmittance(from_fixed(raw, scale), face);
     }
+    }
+};
+
+// The same march for lines along x (axis 0), whose points are contiguous in
+// memory: one wave per 64 lines, in chunks of SWEEP_CHUNK steps. The wave
+// loads a chunk of all 64 lines coalesced (two lines per load) and turns each
+// deposit into its column transmittance, each lane then marches its own line
+// through the chunk in LDS, in the reference's order, leaving the irradiance
+// in place of the factor, and the wave stores the chunk back coalesced.
+constexpr int SWEEP_CHUNK = 32;
+
+struct KSweepRows {
+    using Args = SweepArgs;
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [blocked, out, nx, ny, nz, axis, sign, scale, face, source, ambient] = a;
+        __shared__ double tile[BLOCK / 64][64][SWEEP_CHUNK + 1];
+        const int wave = static_cast<int>(threadIdx.x) >> 6, lane = static_cast<int>(threadIdx.x) & 63;
+        const int lines = ny * nz;
+        const int base = (vg.block * (BLOCK / 64) + wave) * 64;
+        if (base >= lines) return;   // whole wave
+        double(*f)[SWEEP_CHUNK + 1] = tile[wave];
+        const int half = lane >> 5, pl = lane & 31;
+        const double src = static_cast<double>(source);
+        double t = 1.0;
+        for (int c0 = 0; c0 < nx; c0 += SWEEP_CHUNK) {
+            const int pos = c0 + pl;
+            const size_t step = static_cast<size_t>(sign > 0 ? pos : nx - 1 - pos);
+            for (int rr = 0; rr < 64; rr += 2) {
+                const int r = rr + half, line = base + r;
+                if (line < lines && pos < nx) {
+                    const size_t idx = static_cast<size_t>(line) * nx + step;
+                    const unsigned long long raw = blocked[idx];
+                    if (raw != 0ull) blocked[idx] = 0ull;
+                    f[r][pl] = column_transmittance(from_fixed(raw, scale), face);
+                }
+            }
+            wave_sync();
+            const int steps = min(SWEEP_CHUNK, nx - c0);
+            for (int p = 0; p < steps; ++p) {
+                const double fac = f[lane][p];
+                *reinterpret_cast<float*>(&f[lane][p]) = static_cast<float>(src * t) + ambient;
+                t *= fac;
+            }
+            wave_sync();
+            for (int rr = 0; rr < 64; rr += 2) {
+                const int r = rr + half, line = base + r;
+                if (line < lines && pos < nx)
+                    out[static_cast<size_t>(line) * nx + step] = *reinterpret_cast<const float*>(&f[r][pl]);
+            }
+            wave_sync();
+        }
     }
 };
 
@@ -314,7 +375,10 @@ __global__ __launch_bounds__(BLOCK) void k_stats_fields(Grid T, Grid C, Grid N, 
 // by variant, one batched launch per variant present.
 void launch_ftcs(const Launch& L, std::span<const Grid> gs, std::span<const Ftcs> fs, const char* name) {
     Batch<KFtcs2d> sq;
-    Batch<KFtcs3dV4> v4;
+    Batch<KFtcs3dV4<64>> v64;
+    Batch<KFtcs3dV4<32>> v32;
+    Batch<KFtcs3dV4<16>> v16;
+    Batch<KFtcs3dV4<8>> v8;
     Batch<KFtcs3d> sc;
     for (size_t i = 0; i < gs.size(); ++i) {
         const Grid& g = gs[i];
@@ -323,9 +387,21 @@ void launch_ftcs(const Launch& L, std::span<const Grid> gs, std::span<const Ftcs
             const int tiles = ((g.nx + 63) / 64) * ((g.ny + 3) / 4);
             sq.add({g.value, g.scratch, g.nx, f.coeff, f.bc, f.amb, f.rx}, blocks_for_xcd(tiles));
         } else if (g.nx % 4 == 0) {
-            const int tiles = ((g.nx + 255) / 256) * ((g.ny + 3) / 4) * ((g.nz + FTCS_ZC - 1) / FTCS_ZC);
-            v4.add({g.value, g.scratch, g.nx, g.ny, g.nz, f.cx, f.cy, f.cz, f.bc, f.amb, f.rx, f.ry, f.rz},
-                   blocks_for_xcd(tiles));
+            // the narrowest row group that spans nx in one pass (64 lanes at
+            // most), unless the recorder carries a tuned width
+            int w = g.nx >= 256 ? 64 : (g.nx > 64 ? 32 : (g.nx > 32 ? 16 : 8));
+            if (L.rec) {
+                const auto it = L.rec->ftcs_lanes.find(name);
+                if (it != L.rec->ftcs_lanes.end() && it->second > 0) w = it->second;
+            }
+            const int rb = 4 * (64 / w);
+            const int tiles = ((g.nx + 4 * w - 1) / (4 * w)) * ((g.ny + rb - 1) / rb) * ((g.nz + FTCS_ZC - 1) / FTCS_ZC);
+            const Ftcs3Args a{g.value, g.scratch, g.nx, g.ny, g.nz, f.cx, f.cy, f.cz, f.bc, f.amb, f.rx, f.ry, f.rz};
+            const int nb = blocks_for_xcd(tiles);
+            if (w == 64) v64.add(a, nb);
+            else if (w == 32) v32.add(a, nb);
+            else if (w == 16) v16.add(a, nb);
+            else v8.add(a, nb);
         } else {
             const int txy = ((g.nx + 63) / 64) * ((g.ny + 3) / 4);
             int zc = g.nz;
@@ -336,7 +412,10 @@ void launch_ftcs(const Launch& L, std::span<const Grid> gs, std::span<const Ftcs
         }
     }
     if (!sq.args.empty()) launch_k<KFtcs2d>(L, name, sq.args, sq.blocks);
-    if (!v4.args.empty()) launch_k<KFtcs3dV4>(L, name, v4.args, v4.blocks);
+    if (!v64.args.empty()) launch_k<KFtcs3dV4<64>>(L, name, v64.args, v64.blocks);
+    if (!v32.args.empty()) launch_k<KFtcs3dV4<32>>(L, name, v32.args, v32.blocks);
+    if (!v16.args.empty()) launch_k<KFtcs3dV4<16>>(L, name, v16.args, v16.blocks);
+    if (!v8.args.empty()) launch_k<KFtcs3dV4<8>>(L, name, v8.args, v8.blocks);
     if (!sc.args.empty()) launch_k<KFtcs3d>(L, name, sc.args, sc.blocks);
 }
 
@@ -347,17 +426,22 @@ void launch_stamp(const Launch& L, Views<P> ws) {
     launch_k<KStamp<P>>(L, "occlusion_stamp", b.args, b.blocks);
 }
 
+// Lines along y are coalesced across lanes as they are; lines along x go
+// through the LDS-transposing kernel.
 void launch_sweep(const Launch& L, std::span<const Grid> irr, Ticks ts) {
-    Batch<KSweep> b;
+    Batch<KSweep> cols;
+    Batch<KSweepRows> rows;
     for (size_t i = 0; i < irr.size(); ++i) {
         const Grid& g = irr[i];
         const TickParams& t = ts[i];
+        const SweepArgs a{g.deposit, g.value, g.nx, g.ny, g.nz, t.sweep_axis, t.sweep_sign, g.scale, t.face_area,
+                          t.source, t.ambient_irr};
         const int lines = (t.sweep_axis == 0 ? g.ny : g.nx) * g.nz;
-        b.add({g.deposit, g.value, g.nx, g.ny, g.nz, t.sweep_axis, t.sweep_sign, g.scale, t.face_area, t.source,
-               t.ambient_irr},
-              (lines + BLOCK - 1) / BLOCK);
+        if (t.sweep_axis == 0) rows.add(a, (lines + BLOCK - 1) / BLOCK);
+        else cols.add(a, (lines + BLOCK - 1) / BLOCK);
     }
-    launch_k<KSweep>(L, "light_sweep", b.args, b.blocks);
+    if (!rows.args.empty()) launch_k<KSweepRows>(L, "light_sweep", rows.args, rows.blocks);
+    if (!cols.args.empty()) launch_k<KSweep>(L, "light_sweep", cols.args, cols.blocks);
 }
 
 void launch_fill(const Launch& L, float* v, size_t n, float value) {

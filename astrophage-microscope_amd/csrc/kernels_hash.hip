@@ -29,6 +29,7 @@ namespace {
 
 template <class P>
 struct KHashKey {
+    static constexpr bool stride = true;   // any block count covers the work
     struct Args {
         Cells<P> c;
         Counters* ctr;
@@ -110,6 +111,7 @@ struct KBucketScan {
 
 template <class P>
 struct KHashScatter {
+    static constexpr bool stride = true;   // any block count covers the work
     struct Args {
         Cells<P> c;
         const Counters* ctr;
@@ -144,6 +146,7 @@ struct KHashScatter {
 // Stable order: ascending reference slot inside each bucket.
 template <class P>
 struct KHashRank {
+    static constexpr bool stride = true;   // any block count covers the work
     struct Args {
         const Counters* ctr;
         HashGeo h;
@@ -188,6 +191,7 @@ struct KHashRank {
 // then copy back. Runs only on ticks the key kernel marked.
 template <class P>
 struct KResortGather {
+    static constexpr bool stride = true;   // any block count covers the work
     struct Args {
         Cells<P> live;
         Cells<P> alt;
@@ -208,6 +212,7 @@ struct KResortGather {
 
 template <class P>
 struct KResortCopyback {
+    static constexpr bool stride = true;   // any block count covers the work
     struct Args {
         Cells<P> live;
         Cells<P> alt;
@@ -259,49 +264,97 @@ __device__ __forceinline__ RowRanges row_ranges(const HashGeo& h, Start&& start,
 
 // visit(d, q) for every sorted record index q of the walk, in order; d is the
 // row's z-offset range (0, 1, 2 for dz = -1, 0, +1).
-template <class Visit>
-__device__ __forceinline__ void walk_rows(const RowRanges& rr, Visit&& visit) {
-#pragma unroll
-    for (int r = 0; r < 9; ++r)
-        for (int32_t q = rr.s[r]; q < rr.e[r]; ++q) visit(r / 3, q);
-}
-
 // Position difference, as the float the contact screen uses: exact to one
 // float rounding of the double difference the reference computes.
 __device__ __forceinline__ float screen_diff(float a, float b) { return a - b; }
 __device__ __forceinline__ float screen_diff(double a, double b) { return static_cast<float>(a - b); }
 
+// (2a)^2 (1 + 2^-12): the shadow screen's bound on a neighbour's squared
+// distance from the light line.
+constexpr float SHADOW_SCREEN = static_cast<float>(4.0 * ASTRO_CELL_RADIUS * ASTRO_CELL_RADIUS * (1.0 + 0x1p-12));
+
+// Possible shading neighbours a thread queues before the exact pass.
+constexpr int SHADOW_QUEUE = 8;
+
 // Possible contacts a thread queues before the exact pass (the queue is
 // drained early, in order, if it fills).
 constexpr int CONTACT_QUEUE = 4;
 
-__device__ __forceinline__ int wave_min_i(int v) {
+
+// Records staged per wave and z offset (6 KB of LDS per wave).
+template <class P>
+constexpr int walk_seg() { return (6 * 1024) / static_cast<int>(sizeof(Rec<P>)); }
+
+// The neighbour walk of the whole wave: each active lane visits the records
+// of the 27 buckets around its own (gx, gy, gz) in the reference's walk order
+// (visit(o, q), o the record at sorted position q). Every lane of the wave
+// must call it. The 64 consecutive records of a wave have keys in [kmin,
+// kmax], and keys are linear in (z, y, x), so row r of every lane's walk lies
+// in one contiguous record range (about 64 records); for each z offset the
+// wave copies its three rows' ranges into its LDS slice `seg` with coalesced
+// loads and every lane walks there. A range that does not fit is walked in
+// global memory; the visit order is the same either way.
+template <class P, class Visit>
+__device__ __forceinline__ void staged_walk(const HashGeo& h, const Rec<P>* rec, const int32_t* bstart, Rec<P>* seg,
+                                            bool active, int gx, int gy, int gz, Visit&& visit) {
+    constexpr int SEG = walk_seg<P>();
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    RowRanges rr{};
+    if (active) rr = row_ranges(h, [&](int, int b) { return bstart[b]; }, gx, gy, gz);
+    // Row (dz, dy) of any lane lies in buckets [kmin + o - 1, kmax + o + 1],
+    // o = dz*nx*ny + dy*nx. Lane m < 18 fetches the bucket start of bound m
+    // (the start of row m/2's range, or the end), and the wave reads them
+    // back by lane.
+    const int key = (gz * h.ny + gy) * h.nx + gx;
+    const int kmin = wave_min_i(active ? key : 0x7FFFFFFF);
+    const int kmax = wave_max_i(active ? key : -1);
+    int bound = 0;
+    if (kmax >= 0 && lane < 18) {
+        const int row = lane >> 1;
+        const int64_t o = static_cast<int64_t>(row / 3 - 1) * h.nx * h.ny + static_cast<int64_t>(row % 3 - 1) * h.nx;
+        int64_t b = (lane & 1) ? static_cast<int64_t>(kmax) + o + 2 : static_cast<int64_t>(kmin) + o - 1;
+        b = b < 0 ? 0 : (b > h.nb ? h.nb : b);
+        bound = bstart[b];
+    }
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v = min(v, __shfl_xor(v, d, 64));
-    return v;
-}
-__device__ __forceinline__ int wave_max_i(int v) {
+    for (int d = 0; d < 3; ++d) {
+        int lo[3], off[3], total = 0;
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v = max(v, __shfl_xor(v, d, 64));
-    return v;
-}
-// Orders one wave's LDS writes before its later LDS reads by other lanes.
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int j = 0; j < 3; ++j) {
+            const int row = 3 * d + j;
+            const int a = __builtin_amdgcn_readlane(bound, 2 * row);
+            const int b = __builtin_amdgcn_readlane(bound, 2 * row + 1);
+            lo[j] = a;
+            off[j] = total;
+            total += b > a ? b - a : 0;
+        }
+        if (total <= SEG) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int len = (j < 2 ? off[j + 1] : total) - off[j];
+                for (int t = lane; t < len; t += 64) seg[off[j] + t] = rec[lo[j] + t];
+            }
+            wave_sync();
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int row = 3 * d + j;
+                for (int32_t q = rr.s[row]; q < rr.e[row]; ++q) visit(seg[off[j] + (q - lo[j])], q);
+            }
+            wave_sync();   // the next range reuses the slice
+        } else {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int row = 3 * d + j;
+                for (int32_t q = rr.s[row]; q < rr.e[row]; ++q) visit(rec[q], q);
+            }
+        }
+    }
 }
 
 // The contact half of forces_kernel (src/sim/integrator.cu:60-67), one thread
 // per sorted record, adding to the body force already in fs[k]. Neighbour
 // positions come from the hash's records, fixed since the hash build.
-//   * Staging, per wave and without block barriers: the 64 consecutive
-//     records of a wave have keys in [kmin, kmax], and keys are linear in
-//     (z, y, x), so row r of every lane's walk lies in one contiguous record
-//     range (about 64 records). For each z offset the wave copies its three
-//     rows' ranges into its own LDS slice with coalesced loads, then every
-//     lane walks there. A range that does not fit is walked in global memory;
-//     the visit order is the same either way.
+//   * The neighbours come through the wave's LDS staging (staged_walk).
 //   * Each neighbour is first screened in float: the float r^2 of a pair is
 //     within a few float roundings (relative 2^-21) of the double r^2, so a
 //     2^-10 margin keeps every pair the reference counts as touching. Only the
@@ -312,6 +365,7 @@ __device__ __forceinline__ void wave_sync() {
 //     start and so never -0.0, which makes those adds exact no-ops.
 template <class P>
 struct KContactWalk {
+    static constexpr bool stride = true;   // any block count covers the work
     struct Args {
         const Counters* ctr;
         HashGeo h;
@@ -323,11 +377,9 @@ struct KContactWalk {
     static __device__ void run(const Args& a, VGrid vg) {
         const auto& [ctr, h, rec, bstart, fs, k_contact] = a;
 
-    constexpr int SEG = (6 * 1024) / static_cast<int>(sizeof(Rec<P>));   // records per wave and z offset
-    __shared__ Rec<P> s_seg[BLOCK / 64][SEG];
+    __shared__ Rec<P> s_seg[BLOCK / 64][walk_seg<P>()];
     __shared__ int32_t s_queue[CONTACT_QUEUE * BLOCK];
     Rec<P>* seg = s_seg[threadIdx.x >> 6];
-    const int lane = threadIdx.x & 63;
     const int n = ctr->count;
     const double two_a = ASTRO_CELL_DIAMETER;
     const float screen = static_cast<float>(two_a * two_a) * (1.0f + 1.0f / 1024.0f);
@@ -342,8 +394,6 @@ struct KContactWalk {
                 hash_coords(h, static_cast<double>(r.x), static_cast<double>(r.y), static_cast<double>(r.z), gx, gy,
                             gz);
         }
-        RowRanges rr{};
-        if (occ) rr = row_ranges(h, [&](int, int b) { return bstart[b]; }, gx, gy, gz);
         const uint32_t self = r.meta & REC_SLOT_MASK;
         int32_t* queue = s_queue + threadIdx.x;   // entry j at queue[j * BLOCK]
         int queued = 0;
@@ -380,54 +430,7 @@ struct KContactWalk {
             queue[queued * BLOCK] = q;
             ++queued;
         };
-        // The wave's key span bounds every row it walks: row (dz, dy) of any
-        // lane lies in buckets [kmin + o - 1, kmax + o + 1], o = dz*nx*ny +
-        // dy*nx. Lane m < 18 fetches the bucket start of bound m (the start of
-        // row m/2's range, or the end), and the wave reads them back by lane.
-        const int key = (gz * h.ny + gy) * h.nx + gx;
-        const int kmin = wave_min_i(occ ? key : 0x7FFFFFFF);
-        const int kmax = wave_max_i(occ ? key : -1);
-        int bound = 0;
-        if (kmax >= 0 && lane < 18) {
-            const int row = lane >> 1;
-            const int64_t o = static_cast<int64_t>(row / 3 - 1) * h.nx * h.ny + static_cast<int64_t>(row % 3 - 1) * h.nx;
-            int64_t b = (lane & 1) ? static_cast<int64_t>(kmax) + o + 2 : static_cast<int64_t>(kmin) + o - 1;
-            b = b < 0 ? 0 : (b > h.nb ? h.nb : b);
-            bound = bstart[b];
-        }
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            int lo[3], off[3], total = 0;
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                const int row = 3 * d + j;
-                const int a = __builtin_amdgcn_readlane(bound, 2 * row);
-                const int b = __builtin_amdgcn_readlane(bound, 2 * row + 1);
-                lo[j] = a;
-                off[j] = total;
-                total += b > a ? b - a : 0;
-            }
-            if (total <= SEG) {
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    const int len = (j < 2 ? off[j + 1] : total) - off[j];
-                    for (int t = lane; t < len; t += 64) seg[off[j] + t] = rec[lo[j] + t];
-                }
-                wave_sync();
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    const int row = 3 * d + j;
-                    for (int32_t q = rr.s[row]; q < rr.e[row]; ++q) screen_pair(seg[off[j] + (q - lo[j])], q);
-                }
-                wave_sync();   // the next range reuses the slice
-            } else {
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    const int row = 3 * d + j;
-                    for (int32_t q = rr.s[row]; q < rr.e[row]; ++q) screen_pair(rec[q], q);
-                }
-            }
-        }
+        staged_walk(h, rec, bstart, seg, occ, gx, gy, gz, screen_pair);
         if (queued > 0) drain();
         if (loaded) fs[k] = F;
     });
@@ -438,6 +441,7 @@ struct KContactWalk {
 // times the exact near-field shadow product over upstream live neighbours.
 template <class P>
 struct KFeedWalk {
+    static constexpr bool stride = true;   // any block count covers the work
     struct Args {
         const Counters* ctr;
         HashGeo h;
@@ -453,42 +457,77 @@ struct KFeedWalk {
     };
     static __device__ void run(const Args& a, VGrid vg) {
         const auto& [ctr, h, rec, bstart, irr, ldx, ldy, ldz, ambient, occl, dirs] = a;
+    const float lxf = static_cast<float>(ldx), lyf = static_cast<float>(ldy), lzf = static_cast<float>(ldz);
 
+    __shared__ Rec<P> s_seg[BLOCK / 64][walk_seg<P>()];
+    __shared__ int32_t s_queue[SHADOW_QUEUE * BLOCK];
+    Rec<P>* seg = s_seg[threadIdx.x >> 6];
     const int n = ctr->count;
-    const auto body = [&](const int k) {
-    const Rec<P> r = rec[k];
-    if ((r.meta & REC_UNOCC) || !(r.meta & REC_ALIVE)) return;
-    const uint32_t self = r.meta & REC_SLOT_MASK;
-    const double px = static_cast<double>(r.x), py = static_cast<double>(r.y), pz = static_cast<double>(r.z);
-    double directional = static_cast<double>(grid_sample(irr, static_cast<float>(r.x), static_cast<float>(r.y),
-                                                         static_cast<float>(r.z))) -
-                         static_cast<double>(ambient);
-    if (directional < 0.0) directional = 0.0;
-    if (occl) {
-        double open = 1.0;
-        int gx, gy, gz;
-        hash_coords(h, px, py, pz, gx, gy, gz);
-        walk_rows(row_ranges(h, [&](int, int b) { return bstart[b]; }, gx, gy, gz), [&](int, int32_t q) {
-            const Rec<P> o = rec[q];
-            if ((o.meta & REC_SLOT_MASK) == self || !(o.meta & REC_ALIVE)) return;
-            // shadow_fraction (src/sim/emission.cuh:30-37)
-            const double dx = px - static_cast<double>(o.x);
-            const double dy = py - static_cast<double>(o.y);
-            const double dz = pz - static_cast<double>(o.z);
-            const double along = dx * ldx + dy * ldy + dz * ldz;
-            double frac = 0.0;
-            if (along > 0.0) {
-                const double perp2 = (dx * dx + dy * dy + dz * dz) - along * along;
-                const double perp = perp2 > 0.0 ? sqrt(perp2) : 0.0;
-                frac = disc_overlap_fraction(perp, ASTRO_CELL_RADIUS);
-            }
-            open *= (1.0 - frac);
-        });
-        directional *= open;
-    }
-    dirs[k] = directional;
-    };
-    xcd_tiles(vg, n, [&](const int k, const bool valid) { if (valid) body(k); });
+    xcd_tiles(vg, n, [&](const int k, const bool valid) {
+        Rec<P> r{};
+        bool lit = false;
+        if (valid) {
+            r = rec[k];
+            lit = !(r.meta & REC_UNOCC) && (r.meta & REC_ALIVE);
+        }
+        const uint32_t self = r.meta & REC_SLOT_MASK;
+        const double px = static_cast<double>(r.x), py = static_cast<double>(r.y), pz = static_cast<double>(r.z);
+        double directional = 0.0;
+        if (lit) {
+            directional = static_cast<double>(grid_sample(irr, static_cast<float>(r.x), static_cast<float>(r.y),
+                                                          static_cast<float>(r.z))) -
+                          static_cast<double>(ambient);
+            if (directional < 0.0) directional = 0.0;
+        }
+        if (occl) {
+            double open = 1.0;
+            int gx = 0, gy = 0, gz = 0;
+            if (lit) hash_coords(h, px, py, pz, gx, gy, gz);
+            int32_t* queue = s_queue + threadIdx.x;   // entry j at queue[j * BLOCK]
+            int queued = 0;
+            // shadow_fraction (src/sim/emission.cuh:30-37) of the queued
+            // neighbours, in walk order
+            const auto drain = [&]() {
+                for (int j = 0; j < queued; ++j) {
+                    const Rec<P> o = rec[queue[j * BLOCK]];
+                    const double dx = px - static_cast<double>(o.x);
+                    const double dy = py - static_cast<double>(o.y);
+                    const double dz = pz - static_cast<double>(o.z);
+                    const double along = dx * ldx + dy * ldy + dz * ldz;
+                    double frac = 0.0;
+                    if (along > 0.0) {
+                        const double perp2 = (dx * dx + dy * dy + dz * dz) - along * along;
+                        const double perp = perp2 > 0.0 ? sqrt(perp2) : 0.0;
+                        frac = disc_overlap_fraction(perp, ASTRO_CELL_RADIUS);
+                    }
+                    open *= (1.0 - frac);
+                }
+                queued = 0;
+            };
+            staged_walk(h, rec, bstart, seg, lit, gx, gy, gz, [&](const Rec<P>& o, int32_t q) {
+                if ((o.meta & REC_SLOT_MASK) == self || !(o.meta & REC_ALIVE)) return;
+                // FP32 screen. A neighbour downstream of the cell (along <= 0)
+                // or at least 2a off its light line has an overlap of exactly
+                // 0 and multiplies `open` by exactly 1. The float along and
+                // perp^2 differ from the double ones by under
+                // 2^-21 (|dx|+|dy|+|dz|) and 2^-18 |d|^2, so these margins
+                // never drop a shading neighbour. The survivors are queued and
+                // evaluated in double precision together, mostly after the
+                // walk, when every lane of the wave does so at once.
+                const float sx = screen_diff(r.x, o.x), sy = screen_diff(r.y, o.y), sz = screen_diff(r.z, o.z);
+                const float sa = sx * lxf + sy * lyf + sz * lzf;
+                if (sa < -0x1p-16f * (fabsf(sx) + fabsf(sy) + fabsf(sz))) return;
+                const float s2 = sx * sx + sy * sy + sz * sz;
+                if (s2 - sa * sa > SHADOW_SCREEN + 0x1p-14f * s2) return;
+                if (queued == SHADOW_QUEUE) drain();
+                queue[queued * BLOCK] = q;
+                ++queued;
+            });
+            if (queued > 0) drain();
+            directional *= open;
+        }
+        if (lit) dirs[k] = directional;
+    });
     }
 };
 

@@ -663,6 +663,13 @@ struct astro_ensemble {
     char* eager = nullptr;
     size_t eager_bytes = 0;
 
+    // Blocks a grid-stride stage is spread over, across all chambers, by
+    // default and per stage (set by astro_ensemble_autotune), and the FTCS
+    // row-group widths per field stage.
+    int stride_blocks = 2048;
+    std::map<std::string, int> stage_blocks;
+    std::map<std::string, int> ftcs_lanes;
+
     std::unique_ptr<EventProfiler> prof;
     bool prof_on = false;
     std::string err;
@@ -696,6 +703,9 @@ void drop_ensemble_graph(astro_ensemble* e) {
 // ping-pong, hash validity) advances as the launches are recorded.
 int32_t record_ticks(astro_ensemble* e, Recorder& rec, int ticks) {
     const WorldSpan ws(e->worlds.data(), e->worlds.size());
+    rec.stride_blocks = e->stride_blocks;
+    rec.stage_blocks = e->stage_blocks;
+    rec.ftcs_lanes = e->ftcs_lanes;
     const Launch L{e->stream, nullptr, &rec};
     for (int i = 0; i < ticks; ++i) {
         const int32_t st = tick_once(ws, L);
@@ -1603,6 +1613,83 @@ int32_t astro_ensemble_sync(astro_ensemble* e) {
     if (st != ASTRO_OK) return st;
     for (astro_world* w : e->worlds)
         if ((st = check_device_errors(w)) != ASTRO_OK) return efail(e, st, w->err);
+    return ASTRO_OK;
+}
+
+// Launch shapes tried by the autotuner: total blocks of a grid-stride stage
+// (0: one block per tile of each chamber's capacity) and lanes per x row of
+// the float4 FTCS (0: the widest row group nx fills).
+static const int TUNE_BLOCKS[] = {0, 1024, 2048, 4096, 8192, 16384};
+static const int TUNE_LANES[] = {0, 8, 16, 32, 64};
+
+int32_t astro_ensemble_autotune(astro_ensemble* e, int32_t ticks) {
+    if (!e || ticks <= 0) return ASTRO_INVALID_ARGUMENT;
+    int32_t st = check_shape(e);
+    if (st != ASTRO_OK) return st;
+    const bool was_on = e->prof_on;
+    // One trial: `ticks` eager ticks with per-stage event timing; returns
+    // milliseconds per stage.
+    const auto trial = [&](std::map<std::string, double>& ms) -> int32_t {
+        e->prof->drain();
+        e->prof->totals.clear();
+        e->prof->order.clear();
+        e->prof_on = true;
+        const int32_t rc = astro_ensemble_step(e, ticks);
+        e->prof_on = was_on;
+        if (rc != ASTRO_OK) return rc;
+        e->prof->drain();
+        ms.clear();
+        for (const auto& [name, v] : e->prof->totals) ms[name] = v.second;
+        return ASTRO_OK;
+    };
+    std::map<std::string, std::pair<double, int>> best_blocks, best_lanes;
+    e->stage_blocks.clear();
+    e->ftcs_lanes.clear();
+    std::map<std::string, double> ms;
+    for (const int b : TUNE_BLOCKS) {
+        e->stride_blocks = b;
+        if ((st = trial(ms)) != ASTRO_OK) return st;
+        for (const auto& [name, t] : ms) {
+            auto it = best_blocks.find(name);
+            if (it == best_blocks.end() || t < it->second.first) best_blocks[name] = {t, b};
+        }
+    }
+    e->stride_blocks = 2048;
+    for (const auto& [name, v] : best_blocks) e->stage_blocks[name] = v.second;
+    for (const int l : TUNE_LANES) {
+        for (const char* f : {"ftcs_T", "ftcs_CO2", "ftcs_N2"}) e->ftcs_lanes[f] = l;
+        if ((st = trial(ms)) != ASTRO_OK) return st;
+        for (const auto& [name, t] : ms) {
+            if (name.rfind("ftcs_", 0) != 0) continue;
+            auto it = best_lanes.find(name);
+            if (it == best_lanes.end() || t < it->second.first) best_lanes[name] = {t, l};
+        }
+    }
+    e->ftcs_lanes.clear();
+    for (const auto& [name, v] : best_lanes) e->ftcs_lanes[name] = v.second;
+    drop_ensemble_graph(e);
+    e->prof->totals.clear();
+    e->prof->order.clear();
+    return ASTRO_OK;
+}
+
+int32_t astro_ensemble_tuning(const astro_ensemble* e, char* out, int32_t len) {
+    if (!e || !out || len <= 0) return ASTRO_INVALID_ARGUMENT;
+    std::string s = "{\"default_blocks\": " + std::to_string(e->stride_blocks) + ", \"stage_blocks\": {";
+    bool first = true;
+    for (const auto& [name, b] : e->stage_blocks) {
+        s += (first ? "\"" : ", \"") + name + "\": " + std::to_string(b);
+        first = false;
+    }
+    s += "}, \"ftcs_lanes\": {";
+    first = true;
+    for (const auto& [name, l] : e->ftcs_lanes) {
+        s += (first ? "\"" : ", \"") + name + "\": " + std::to_string(l);
+        first = false;
+    }
+    s += "}}";
+    if (static_cast<int32_t>(s.size()) >= len) return ASTRO_CAPACITY_EXCEEDED;
+    std::memcpy(out, s.c_str(), s.size() + 1);
     return ASTRO_OK;
 }
 

@@ -282,6 +282,15 @@ astro_world*  astro_ensemble_chamber(astro_ensemble* e, int32_t i);
 int32_t       astro_ensemble_step(astro_ensemble* e, int32_t n_ticks);
 /* Waits for the ensemble's work and checks every chamber's device errors. */
 int32_t       astro_ensemble_sync(astro_ensemble* e);
+/* Picks launch shapes by measurement: for every stage, the total block count
+ * a grid-stride stage is spread over across the chambers, and for the 3D FTCS
+ * the lanes per x row, each the fastest of a few candidates. Every candidate
+ * runs `ticks_per_trial` real, timed ticks (eleven trials in all), so the
+ * chambers advance by 11 x ticks_per_trial ticks; launch shapes never change
+ * results. The choice holds for later steps. */
+int32_t       astro_ensemble_autotune(astro_ensemble* e, int32_t ticks_per_trial);
+/* The current choice as a JSON object (NUL-terminated) in out[0, len). */
+int32_t       astro_ensemble_tuning(const astro_ensemble* e, char* out, int32_t len);
 /* Per-stage HIP-event timing of the batched launches; while enabled, steps
  * run eagerly (no graph). Same layout as astro_profile_read. */
 int32_t       astro_ensemble_profile_enable(astro_ensemble* e, int32_t on);

@@ -156,3 +156,27 @@ def test_ensemble_rejects_mismatched_chambers(gpu):
         assert err.value.status == ah.INVALID_ARGUMENT
     for w in (a, b, c):
         w.close()
+
+
+def test_ensemble_autotune_keeps_results(gpu):
+    """The autotuner's trial ticks are real ticks, and the launch shapes it
+    picks change no result: chambers still match standalone worlds."""
+    specs = lit_chambers(gpu, [1200, 80, 4000, 600], ah.FP32_HOT, 90)
+    ens_worlds = [make_world(gpu, *s) for s in specs]
+    solo = [make_world(gpu, *s) for s in specs]
+    with ah.Ensemble(gpu, ens_worlds) as e:
+        tuning = e.autotune(1)   # eleven trials of one tick
+        assert "contact_walk" in tuning["stage_blocks"] and "ftcs_T" in tuning["ftcs_lanes"]
+        for w in solo:
+            w.step(11)
+        e.sync()
+        for i, (a, b) in enumerate(zip(ens_worlds, solo)):
+            assert_same(a, b, f"chamber {i} after tuning")
+        e.step(4)
+        for w in solo:
+            w.step(4)
+        e.sync()
+        for i, (a, b) in enumerate(zip(ens_worlds, solo)):
+            assert_same(a, b, f"chamber {i} with tuned shapes")
+    for w in ens_worlds + solo:
+        w.close()

@@ -126,16 +126,21 @@ def test_tick_parity_fp64_dormant(oracle, gpu):
     assert abs(so["total_energy_j"] - sg["total_energy_j"]) <= 1e-6 * so["total_energy_j"]
 
 
-def test_tick_parity_awake_lit(oracle, gpu):
+@pytest.mark.parametrize("light", [(1.0, 0.2), (-1.0, 0.3), (0.2, 1.0), (0.3, -0.9)])
+def test_tick_parity_awake_lit(oracle, gpu, light):
     """Reference 2D geometry with awake cells, light and taxis on: thermostat
-    deposits, occlusion sweep, exact near-field shadow, feeding, tumbles."""
+    deposits, occlusion sweep, exact near-field shadow, feeding, tumbles.
+    The four light directions sweep along +x, -x, +y and -y; after the first
+    tick (identical positions) the swept irradiance field is bit-exact."""
     kw = dict(REF2D, capacity=3000, chamber_w=1e-3, chamber_h=1e-3, ambient_temp=300.0, light_enabled=1,
-              light_irradiance=1.0e4, light_dir_x=1.0, light_dir_y=0.2, co2_init=1e-3, biology_rate=1e6)
+              light_irradiance=1.0e4, light_dir_x=light[0], light_dir_y=light[1], co2_init=1e-3, biology_rate=1e6)
     cells = oracle_cells(oracle, 3000, 3, spawn=dict(charge_dist=ah.DIST_UNIFORM, charge_b=0.9, awake=True), **kw)
     o, g = pair(oracle, gpu, cells, **kw)
-    for _ in range(3):
+    for t in range(3):
         o.step(1)
         g.step(1)
+        if t == 0:
+            assert np.array_equal(o.field(ah.IRRADIANCE), g.field(ah.IRRADIANCE))
     a, b = o.download(), g.download()
     compare_cells(a, b, exact=("id", "death_cause"))
     flag_diff = np.count_nonzero(a["flags"] != b["flags"])
