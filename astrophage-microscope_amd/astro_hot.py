@@ -185,6 +185,19 @@ PRODUCT_ONLY = {
     "ensemble_profile_read": (_I, [_P, _P, _I, _P, _P, _I, C.POINTER(_I)]),
     "ensemble_profile_reset": (_I, [_P]),
     "ensemble_stream": (_P, [_P]),
+    "surrogate_desc_default": (None, [_P]),
+    "surrogate_create": (_I, [_P, C.POINTER(_P)]),
+    "surrogate_destroy": (None, [_P]),
+    "surrogate_last_error": (C.c_char_p, [_P]),
+    "surrogate_train": (_I, [_P, _I, _I, C.c_float, C.c_uint64, C.POINTER(_D)]),
+    "surrogate_sample": (_I, [_P, C.c_uint64]),
+    "surrogate_set_state": (_I, [_P, _P, _P, _P, _P, C.c_size_t]),
+    "surrogate_get_state": (_I, [_P, _P, _P, _P, _P, C.c_size_t]),
+    "surrogate_advance": (_I, [_P, _I, C.POINTER(_I)]),
+    "surrogate_counters_get": (_I, [_P, _P]),
+    "surrogate_weights": (_I, [_P, _I, _P, C.c_size_t, _I]),
+    "surrogate_eval": (_I, [_P, _P, C.c_size_t, _P, _P, _P, C.c_size_t]),
+    "surrogate_stream": (_P, [_P]),
 }
 
 
@@ -504,3 +517,127 @@ class Ensemble:
 
     def profile_read(self, max_kernels: int = 64) -> dict[str, tuple[int, float]]:
         return _read_profile(self.b.ensemble_profile_read, self.h, max_kernels)
+
+
+class SurrogateDesc(C.Structure):
+    _fields_ = [("abi_version", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+                ("d_temp", C.c_double), ("d_nutrient", C.c_double), ("dt_sub", C.c_double),
+                ("steps_per_call", C.c_int32), ("cells", C.c_int32), ("uptake", C.c_double),
+                ("temp_ref", C.c_double), ("nutrient_ref", C.c_double), ("seed", C.c_uint64),
+                ("leak", C.c_double), ("judge_every", C.c_int32), ("judge_tol", C.c_double), ("budget_tol", C.c_double),
+                ("cooldown", C.c_int32)]
+
+
+class SurrogateCounters(C.Structure):
+    _fields_ = [("calls", C.c_int64), ("fallbacks", C.c_int64), ("judged", C.c_int64),
+                ("last_judge_err", C.c_double), ("last_budget_residual", C.c_double),
+                ("out_scale_temp", C.c_double), ("out_scale_nutrient", C.c_double)]
+
+
+SURROGATE_PLAIN, SURROGATE_NET, SURROGATE_GATED = 0, 1, 2
+SURROGATE_CIN = [8, 32, 32, 32]
+
+
+class Surrogate:
+    """The learned field surrogate (BF16 MFMA 3D-conv operator) behind the C
+    ABI; the reference has none. Fields are float32 arrays [nz][ny][nx]."""
+
+    def __init__(self, binding: Binding, **kw):
+        self.b = binding
+        d = SurrogateDesc()
+        binding.surrogate_desc_default(C.byref(d))
+        for k, v in kw.items():
+            if not hasattr(d, k):
+                raise KeyError(k)
+            setattr(d, k, v)
+        h = C.c_void_p()
+        st = binding.surrogate_create(C.byref(d), C.byref(h))
+        if st != OK:
+            raise AstroError(st, "surrogate_create")
+        self.h = h
+        self.desc = d
+        self.shape = (d.nz, d.ny, d.nx)
+        self.n = d.nx * d.ny * d.nz
+
+    def _check(self, st: int, what: str = ""):
+        if st != OK:
+            detail = (self.b.surrogate_last_error(self.h) or b"").decode() if getattr(self, "h", None) else ""
+            raise AstroError(st, detail or what)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.b.surrogate_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> dict:
+        d = self.desc
+        return {k: getattr(d, k) for k in ("dx", "dy", "dz", "dt_sub", "d_temp", "d_nutrient", "uptake",
+                                           "steps_per_call")}
+
+    def train(self, trajectories: int, epochs: int = 1, lr: float = 2e-3, seed: int = 1) -> float:
+        loss = C.c_double()
+        self._check(self.b.surrogate_train(self.h, trajectories, epochs, lr, seed, C.byref(loss)), "train")
+        return loss.value
+
+    def sample(self, seed: int):
+        self._check(self.b.surrogate_sample(self.h, seed), "sample")
+
+    def set_state(self, T=None, Cn=None, S=None, occ=None):
+        arr = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (T, Cn, S, occ)]
+        self._check(self.b.surrogate_set_state(self.h, *[_ptr(a) for a in arr], self.n), "set_state")
+
+    def state(self):
+        out = [np.empty(self.shape, np.float32) for _ in range(4)]
+        self._check(self.b.surrogate_get_state(self.h, *[_ptr(a) for a in out], self.n), "get_state")
+        return tuple(out)
+
+    def advance(self, mode: int = SURROGATE_GATED) -> bool:
+        used = C.c_int32()
+        self._check(self.b.surrogate_advance(self.h, mode, C.byref(used)), "advance")
+        return bool(used.value)
+
+    def counters(self) -> dict:
+        c = SurrogateCounters()
+        self._check(self.b.surrogate_counters_get(self.h, C.byref(c)))
+        return {name: getattr(c, name) for name, _ in SurrogateCounters._fields_}
+
+    def weights(self, layer: int) -> np.ndarray:
+        w = np.empty(27 * 32 * SURROGATE_CIN[layer] + 32, np.float32)
+        self._check(self.b.surrogate_weights(self.h, layer, _ptr(w), w.size, 0))
+        return w
+
+    def set_weights(self, layer: int, w: np.ndarray):
+        w = np.ascontiguousarray(w, np.float32)
+        self._check(self.b.surrogate_weights(self.h, layer, _ptr(w), w.size, 1))
+
+    def eval(self, x: np.ndarray, target: np.ndarray | None = None, grads: bool = False):
+        """Network on normalized inputs x [..][8]: predictions [..][2], and
+        with a target the per-layer weight gradients of the mean squared
+        error."""
+        x = np.ascontiguousarray(x, np.float32)
+        pred = np.empty(x.shape[:-1] + (2,), np.float32)
+        tgt = None if target is None else np.ascontiguousarray(target, np.float32)
+        sizes = [27 * 32 * c + 32 for c in SURROGATE_CIN]
+        g = np.empty(sum(sizes), np.float32) if grads else None
+        self._check(self.b.surrogate_eval(self.h, _ptr(x), self.n, _ptr(tgt), _ptr(pred), _ptr(g),
+                                          0 if g is None else g.size), "eval")
+        if not grads:
+            return pred
+        out, off = [], 0
+        for sz in sizes:
+            out.append(g[off:off + sz])
+            off += sz
+        return pred, out

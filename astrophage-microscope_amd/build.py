@@ -24,7 +24,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "lib" / "libastro_hot.so"
-SOURCES = ["world.hip", "kernels_hash.hip", "kernels_cells.hip", "kernels_fields.hip"]
+SOURCES = ["world.hip", "kernels_hash.hip", "kernels_cells.hip", "kernels_fields.hip", "surrogate.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

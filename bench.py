@@ -29,6 +29,11 @@ together, each stage one batched launch over every chamber and each step one
 graph launch. The steps are timed before and after the autotuner picks the
 launch shapes; the same chambers stepped one after another with their own
 graphs give the sequential figure beside it.
+
+The `surrogate` object measures the learned field surrogate at 128^3: it is
+trained on 2,000 plain trajectories (about 20 s), then timed per call in its
+three modes (plain FTCS, net, gated) and compared on held-out states with the
+FP64 CPU plain path, on the full fields and on the increments.
 """
 from __future__ import annotations
 
@@ -209,6 +214,93 @@ def bench_ensemble(binding: ah.Binding, chambers: int, steps: int, warmup: int, 
     return out
 
 
+SURROGATE_FLOPS_PER_VOXEL = 2 * 27 * (8 * 32 + 32 * 32 + 32 * 32 + 32 * 2)   # one call, useful MACs x 2
+
+
+def surrogate_cpu_reference(states, params: dict) -> dict:
+    """CPU leg of the surrogate measurement: the FP64 plain path (the oracle)
+    on the held-out states, timed, and used as the reference for the error
+    figures."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import surrogate_oracle as so
+    out, secs, steps = [], 0.0, 0
+    for T0, C0, S, occ in states:
+        t0 = time.perf_counter()
+        Tw, Cw = so.plain_advance(T0, C0, S, occ, params)
+        secs += time.perf_counter() - t0
+        steps += params["steps_per_call"]
+        out.append((Tw, Cw))
+    return {"refs": out, "field_steps_per_s": steps / secs, "rel_l2": so.rel_l2}
+
+
+def bench_surrogate(binding: ah.Binding, trajectories: int, epochs: int, calls: int) -> dict:
+    s = ah.Surrogate(binding)
+    t0 = time.perf_counter()
+    loss = s.train(trajectories, epochs=epochs, lr=1e-3, seed=1)
+    train_s = time.perf_counter() - t0
+
+    # held-out states: the net's and the FP32 plain path's results
+    states, net, plain = [], [], []
+    for seed in (100001, 100002, 100003):
+        s.sample(seed)
+        st = s.state()
+        states.append(st)
+        s.advance(ah.SURROGATE_NET)
+        net.append(s.state()[:2])
+        s.set_state(*st)
+        s.advance(ah.SURROGATE_PLAIN)
+        plain.append(s.state()[:2])
+    cpu = surrogate_cpu_reference(states, s.params())
+    rel = cpu["rel_l2"]
+
+    def errs(res):
+        e = {"temp_field": [], "temp_increment": [], "nutrient_field": [], "nutrient_increment": []}
+        for (T1, C1), (T0, C0, _, _), (Tw, Cw) in zip(res, states, cpu["refs"]):
+            e["temp_field"].append(rel(T1, Tw))
+            e["temp_increment"].append(rel(T1.astype(np.float64) - T0, Tw - T0))
+            e["nutrient_field"].append(rel(C1, Cw))
+            e["nutrient_increment"].append(rel(C1.astype(np.float64) - C0, Cw - C0))
+        return {k: float(np.mean(v)) for k, v in e.items()}
+
+    timing = {}
+    s.sample(7)
+    for name, mode in (("plain", ah.SURROGATE_PLAIN), ("net", ah.SURROGATE_NET), ("gated", ah.SURROGATE_GATED)):
+        s.advance(mode)
+        s.state()
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            s.advance(mode)
+        s.state()
+        timing[name] = (time.perf_counter() - t0) / calls
+    c = s.counters()
+    steps = s.desc.steps_per_call
+    n = s.n
+    net_tflops = SURROGATE_FLOPS_PER_VOXEL * n / timing["net"] / 1e12
+    out = {
+        "workload": "learned field surrogate: 128^3 T + nutrient, 550x550x200 um, 16 FTCS substeps of 6.25 us per "
+                    "call, states with Gaussian hot/cool spots and a clustered 25k-cell culture",
+        "metric": "field steps/s (16 substeps of both fields per call)",
+        "plain_ms_per_call": timing["plain"] * 1e3, "net_ms_per_call": timing["net"] * 1e3,
+        "gated_ms_per_call": timing["gated"] * 1e3,
+        "field_steps_per_s": {"plain": steps / timing["plain"], "net": steps / timing["net"],
+                              "gated": steps / timing["gated"]},
+        "net_vs_plain": timing["plain"] / timing["net"],
+        "error_vs_fp64": {"net": errs(net), "plain_fp32": errs(plain),
+                          "held_out_states": 3},
+        "gate": {"judge_every": s.desc.judge_every, "judge_tol": s.desc.judge_tol,
+                 "calls": c["calls"], "fallbacks": c["fallbacks"], "last_judge_err": c["last_judge_err"]},
+        "training": {"trajectories": trajectories, "epochs": epochs, "seconds": train_s, "final_loss": loss},
+        "roofline": {"kernel": "conv stack (whole net call)", "bound": "mfma",
+                     "achieved": round(net_tflops, 1), "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": round(net_tflops / 2500.0, 4),
+                     "flops_per_call": SURROGATE_FLOPS_PER_VOXEL * n},
+        "cpu_baseline": {"value": cpu["field_steps_per_s"], "unit": "field steps/s", "cores": 1, "kind": "port",
+                         "sample": "FP64 numpy plain path on the 3 held-out states (16 substeps each)"},
+    }
+    s.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -223,6 +315,9 @@ def main() -> int:
     ap.add_argument("--ensemble-only", action="store_true")
     ap.add_argument("--ensemble-chambers", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=100)
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--surrogate-only", action="store_true")
+    ap.add_argument("--surrogate-trajectories", type=int, default=2000)
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -233,6 +328,9 @@ def main() -> int:
     gpu = ah.load()
     if a.ensemble_only:
         print(json.dumps(bench_ensemble(gpu, a.ensemble_chambers, a.ensemble_steps, 10, 10, 10)))
+        return 0
+    if a.surrogate_only:
+        print(json.dumps(bench_surrogate(gpu, a.surrogate_trajectories, 1, 50)))
         return 0
     precision = ah.FP32_HOT if a.precision == "fp32" else ah.FP64_CONTRACT
     w = ah.World(gpu, workload(gpu, a.cells, precision))
@@ -295,6 +393,10 @@ def main() -> int:
     if not a.no_ensemble:
         ens = bench_ensemble(gpu, a.ensemble_chambers, a.ensemble_steps, 10, 10, 10)
 
+    sur = None
+    if not a.no_surrogate:
+        sur = bench_surrogate(gpu, a.surrogate_trajectories, 1, 50)
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(a.cells, a.cpu_steps)
@@ -315,6 +417,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "stages": stages,
         "ensemble": ens,
+        "surrogate": sur,
     }
     print(json.dumps(out))
     return 0

@@ -300,6 +300,76 @@ int32_t       astro_ensemble_profile_read(astro_ensemble* e, char* names, int32_
 int32_t       astro_ensemble_profile_reset(astro_ensemble* e);
 void*         astro_ensemble_stream(astro_ensemble* e);
 
+/* ---- learned field surrogate -------------------------------------------------
+ * No counterpart in the reference. A 4-layer 3D convolutional operator (3x3x3
+ * kernels, 32 channels, BF16 weights and activations, FP32 accumulation on
+ * the matrix cores; hidden activations leaky ReLU with slope `leak`, linear
+ * by default since the dynamics are linear in the fields) advances a temperature and a nutrient field by
+ * `steps_per_call` FTCS substeps in one call. The dynamics it stands in for,
+ * per substep: both fields take one 7-point FTCS substep with Neumann walls
+ * (the reference's diffuse_kernel, src/fields/diffuse.cu:19-44, in 3D), then
+ * T += dt * S and C -= dt * uptake * occupancy * C. It is trained on
+ * trajectories of that plain path from random states (Gaussian warm/cool
+ * spots, nutrient gradients, a clustered culture of `cells` cells giving
+ * occupancy and heating). The plain path is also its judge and fallback. */
+typedef struct astro_surrogate astro_surrogate;
+typedef struct astro_surrogate_desc {
+    int32_t  abi_version;
+    int32_t  nx, ny, nz;            /* nx a multiple of 32 */
+    double   dx, dy, dz;            /* m */
+    double   d_temp, d_nutrient;    /* diffusivities, m^2/s */
+    double   dt_sub;                /* s, stable for both fields */
+    int32_t  steps_per_call;
+    int32_t  cells;                 /* culture size of the training states */
+    double   uptake;                /* nutrient uptake per occupant, 1/s */
+    double   temp_ref, nutrient_ref;
+    uint64_t seed;                  /* weight initialization */
+    double   leak;                  /* hidden activation: leaky ReLU slope, 1 = linear (default) */
+    /* gated calls: every judge_every-th call also runs the plain path from the
+     * same start; a relative L2 error of the T increment above judge_tol, or a
+     * heat-budget residual above budget_tol on any call, keeps the plain
+     * result and runs the next `cooldown` calls plain. */
+    int32_t  judge_every;
+    double   judge_tol, budget_tol;
+    int32_t  cooldown;
+} astro_surrogate_desc;
+
+enum { ASTRO_SURROGATE_PLAIN = 0, ASTRO_SURROGATE_NET = 1, ASTRO_SURROGATE_GATED = 2 };
+
+typedef struct astro_surrogate_counters {
+    int64_t calls, fallbacks, judged;
+    double  last_judge_err;         /* relative L2 of the T increment, -1 if never judged */
+    double  last_budget_residual;
+    double  out_scale_temp, out_scale_nutrient;   /* K and kg/m^3 per unit output */
+} astro_surrogate_counters;
+
+void        astro_surrogate_desc_default(astro_surrogate_desc* d);   /* 128^3, 550x550x200 um, 16 substeps of 6.25 us */
+int32_t     astro_surrogate_create(const astro_surrogate_desc* d, astro_surrogate** out);
+void        astro_surrogate_destroy(astro_surrogate* s);
+const char* astro_surrogate_last_error(const astro_surrogate* s);
+/* Adam over `trajectories` states from `seed`, `epochs` passes, cosine-decayed
+ * rate from lr; the mean loss of the last pass in *loss. */
+int32_t     astro_surrogate_train(astro_surrogate* s, int32_t trajectories, int32_t epochs, float lr,
+                                  uint64_t seed, double* loss);
+/* A random state of the training distribution as the current state. */
+int32_t     astro_surrogate_sample(astro_surrogate* s, uint64_t seed);
+int32_t     astro_surrogate_set_state(astro_surrogate* s, const float* temp, const float* nutrient,
+                                      const float* source, const float* occupancy, size_t n);
+int32_t     astro_surrogate_get_state(astro_surrogate* s, float* temp, float* nutrient,
+                                      float* source, float* occupancy, size_t n);
+/* One call: mode PLAIN (steps_per_call FTCS substeps), NET, or GATED. */
+int32_t     astro_surrogate_advance(astro_surrogate* s, int32_t mode, int32_t* used_net);
+int32_t     astro_surrogate_counters_get(const astro_surrogate* s, astro_surrogate_counters* out);
+/* Master weights of one layer ([27][32 out][cin] then [32] bias; cin = 8 for
+ * layer 0, else 32): read, or write (write = 1, resets the Adam moments). */
+int32_t     astro_surrogate_weights(astro_surrogate* s, int32_t layer, float* w, size_t n, int32_t write);
+/* The network on normalized inputs x[n][8]: predictions pred[n][2]; with a
+ * target[n][2], also the mean-squared-error gradient of every layer's
+ * weights, layers in order (no update). */
+int32_t     astro_surrogate_eval(astro_surrogate* s, const float* x, size_t n, const float* target,
+                                 float* pred, float* grads, size_t ngrads);
+void*       astro_surrogate_stream(astro_surrogate* s);
+
 #ifdef __cplusplus
 }
 #endif
