@@ -130,7 +130,7 @@ def total_cell_steps(worlds: list[ah.World]) -> int:
 
 
 def bench_ensemble(binding: ah.Binding, chambers: int, steps: int, warmup: int, profile_steps: int,
-                   seq_steps: int) -> dict:
+                   seq_steps: int, sequential: bool = True) -> dict:
     worlds = ensemble_chambers(binding, chambers)
     cells = [w.counters().count for w in worlds]
     e = ah.Ensemble(binding, worlds)
@@ -149,16 +149,18 @@ def bench_ensemble(binding: ah.Binding, chambers: int, steps: int, warmup: int, 
     elapsed, done = timed(steps)
 
     # The same chambers one after another, each replaying its own graph.
-    for w in worlds:
-        w.step(2)
-        w.sync()
-    s0 = total_cell_steps(worlds)
-    t1 = time.perf_counter()
-    for w in worlds:
-        w.step(seq_steps)
-        w.sync()
-    seq_elapsed = time.perf_counter() - t1
-    seq_done = total_cell_steps(worlds) - s0
+    seq_elapsed, seq_done = float("nan"), 0
+    if sequential:
+        for w in worlds:
+            w.step(2)
+            w.sync()
+        s0 = total_cell_steps(worlds)
+        t1 = time.perf_counter()
+        for w in worlds:
+            w.step(seq_steps)
+            w.sync()
+        seq_elapsed = time.perf_counter() - t1
+        seq_done = total_cell_steps(worlds) - s0
 
     # Per-stage pass: eager batched launches with HIP events on the ensemble's stream.
     e.profile(True)
@@ -189,7 +191,7 @@ def bench_ensemble(binding: ah.Binding, chambers: int, steps: int, warmup: int, 
     algo_step = sum(v["algo_bytes_per_tick"] for v in stages.values())
     dom = max(kern.items(), key=lambda kv: kv[1][1])[0]
     value = done / elapsed
-    seq_value = seq_done / seq_elapsed
+    seq_value = seq_done / seq_elapsed if sequential else None
     out = {
         "workload": "batched ensemble: 256 chambers 1x1x0.2 mm, cells log-uniform [500, 50000] (seed 7), "
                     "64^3 FP32 T + irradiance each, random in-plane light, all awake, dt 1e-4 s",
@@ -201,7 +203,7 @@ def bench_ensemble(binding: ah.Binding, chambers: int, steps: int, warmup: int, 
         "tuning": tuning,
         "sequential": {"value": seq_value, "ms_per_step_all_chambers": seq_elapsed / seq_steps * 1e3,
                        "steps": seq_steps, "how": "each chamber replays its own graph, one chamber after another"},
-        "speedup_vs_sequential": value / seq_value,
+        "speedup_vs_sequential": value / seq_value if seq_value else None,
         "step_roofline": {"algo_bytes_per_step": algo_step,
                           "achieved_GBs_graph": round(algo_step / (elapsed / steps) / 1e9, 1),
                           "frac_graph": round(algo_step / (elapsed / steps) / 1e9 / HBM_PEAK_GBS, 4),
@@ -315,6 +317,7 @@ def main() -> int:
     ap.add_argument("--ensemble-only", action="store_true")
     ap.add_argument("--ensemble-chambers", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=100)
+    ap.add_argument("--no-sequential", action="store_true", help="skip the per-world sequential ensemble figure")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--surrogate-only", action="store_true")
     ap.add_argument("--surrogate-trajectories", type=int, default=2000)
@@ -327,7 +330,8 @@ def main() -> int:
 
     gpu = ah.load()
     if a.ensemble_only:
-        print(json.dumps(bench_ensemble(gpu, a.ensemble_chambers, a.ensemble_steps, 10, 10, 10)))
+        print(json.dumps(bench_ensemble(gpu, a.ensemble_chambers, a.ensemble_steps, 10, 10, 10,
+                                        sequential=not a.no_sequential)))
         return 0
     if a.surrogate_only:
         print(json.dumps(bench_surrogate(gpu, a.surrogate_trajectories, 1, 50)))
