@@ -130,6 +130,77 @@ __global__ __launch_bounds__(256) void k_conv(const bf16* __restrict__ X, const 
 }
 
 // ---------------------------------------------------------------------------
+// 32 input channels, operands from the caches, R output rows per wave: tile
+// (x0..x0+31, y..y+R-1, z), so each weight operand feeds R MFMAs and the R
+// accumulators are independent MFMA chains.
+// ---------------------------------------------------------------------------
+constexpr int CONV_ROWS = 4;
+
+template <int MODE, int R>
+__global__ __launch_bounds__(256) void k_conv_rows(const bf16* __restrict__ X, const bf16* __restrict__ W,
+                                                   const float* __restrict__ bias, const bf16* __restrict__ aprev,
+                                                   bf16* __restrict__ yb, float* __restrict__ yf, int nx, int ny,
+                                                   int nz, float leak) {
+    const int lane = static_cast<int>(threadIdx.x) & 63, wave = static_cast<int>(threadIdx.x) >> 6;
+    const int tiles_x = nx / 32, ygroups = (ny + R - 1) / R;
+    const int64_t tile = static_cast<int64_t>(blockIdx.x) * 4 + wave;
+    if (tile >= static_cast<int64_t>(tiles_x) * ygroups * nz) return;   // whole wave
+    const int tx = static_cast<int>(tile % tiles_x);
+    const int64_t row = tile / tiles_x;
+    const int y = static_cast<int>(row % ygroups) * R, z = static_cast<int>(row / ygroups);
+    const int x0 = tx * 32;
+    const int i = lane & 31, h = lane >> 5;
+    const bf16x8 zero8 = {};
+    f32x16 acc[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) acc[k] = f32x16{};
+    for (int t = 0; t < TAPS; ++t) {
+        const int zz = z + tap_dz(t);
+        if (zz < 0 || zz >= nz) continue;   // wave-uniform
+        const int xx = x0 + i + tap_dx(t);
+        const bool okx = xx >= 0 && xx < nx;
+        const int xs = okx ? xx : 0;
+        const bf16x8* wp = reinterpret_cast<const bf16x8*>(W + (static_cast<int64_t>(t) * CH + i) * CH + h * 8);
+        const bf16x8 b0 = wp[0], b1 = wp[2];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const int yy = y + k + tap_dy(t);
+            const bool ok = okx && yy >= 0 && yy < ny;
+            const int64_t v = (static_cast<int64_t>(zz) * ny + (ok ? yy : 0)) * nx + xs;
+            const bf16x8* pa = reinterpret_cast<const bf16x8*>(X + v * CH + h * 8);
+            const bf16x8 a0 = ok ? pa[0] : zero8, a1 = ok ? pa[2] : zero8;
+            acc[k] = mfma(a0, b0, acc[k]);
+            acc[k] = mfma(a1, b1, acc[k]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int yo = y + k;
+        if (yo >= ny) break;
+        const int64_t vrow = (static_cast<int64_t>(z) * ny + yo) * nx + x0;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int64_t v = vrow + (q >> 2) * 8 + h * 4 + (q & 3);
+            float val = acc[k][q];
+            if constexpr (MODE == 0) {
+                val += bias[i];
+                val = val > 0.0f ? val : leak * val;
+                yb[v * CH + i] = static_cast<bf16>(val);
+            } else if constexpr (MODE == 1) {
+                if (i < NOUT) yf[v * NOUT + i] = val + bias[i];
+            } else {
+                const float ap = static_cast<float>(aprev[v * CH + i]);
+                yb[v * CH + i] = static_cast<bf16>(ap > 0.0f ? val : leak * val);
+            }
+        }
+    }
+}
+
+inline unsigned conv_rows_grid(int nx, int ny, int nz) {
+    return static_cast<unsigned>((static_cast<int64_t>(nx / 32) * ((ny + CONV_ROWS - 1) / CONV_ROWS) * nz + 3) / 4);
+}
+
+// ---------------------------------------------------------------------------
 // Weight gradient: dW[t][o][c] = sum_v dZ[v][o] * X[v + off_t][c], and
 // db[o] = sum_v dZ[v][o]. Each block takes a contiguous run of (y, z) rows
 // and, for each of the 9 (dz, dy) offsets, stages every row pair transposed
@@ -677,9 +748,13 @@ void forward(astro_surrogate* s) {
     hipStream_t st = s->stream;
     const float lk = static_cast<float>(s->d.leak);
     k_conv<CIN0, 0><<<g, 256, 0, st>>>(s->x0, s->L[0].w, s->L[0].bias, nullptr, s->act[0], nullptr, nx, ny, nz, lk);
-    k_conv<CH, 0><<<g, 256, 0, st>>>(s->act[0], s->L[1].w, s->L[1].bias, nullptr, s->act[1], nullptr, nx, ny, nz, lk);
-    k_conv<CH, 0><<<g, 256, 0, st>>>(s->act[1], s->L[2].w, s->L[2].bias, nullptr, s->act[2], nullptr, nx, ny, nz, lk);
-    k_conv<CH, 1><<<g, 256, 0, st>>>(s->act[2], s->L[3].w, s->L[3].bias, nullptr, nullptr, s->pred, nx, ny, nz, lk);
+    const unsigned g2 = conv_rows_grid(nx, ny, nz);
+    k_conv_rows<0, CONV_ROWS><<<g2, 256, 0, st>>>(s->act[0], s->L[1].w, s->L[1].bias, nullptr, s->act[1], nullptr,
+                                                   nx, ny, nz, lk);
+    k_conv_rows<0, CONV_ROWS><<<g2, 256, 0, st>>>(s->act[1], s->L[2].w, s->L[2].bias, nullptr, s->act[2], nullptr,
+                                                   nx, ny, nz, lk);
+    k_conv_rows<1, CONV_ROWS><<<g2, 256, 0, st>>>(s->act[2], s->L[3].w, s->L[3].bias, nullptr, nullptr, s->pred,
+                                                   nx, ny, nz, lk);
 }
 
 constexpr float GRAD_SCALE = 1024.0f;   // applied to dLoss/dpred before BF16, removed in Adam
@@ -719,8 +794,9 @@ int32_t backward(astro_surrogate* s, bool apply, float lr, std::vector<std::vect
         }
         if (l > 0) {
             // dZ of layer l-1 = conv(dZ_l, flipped W_l) * leaky'(act[l-1])
-            k_conv<CH, 2><<<g, 256, 0, st>>>(s->dz[cur], Ly.wd, nullptr, s->act[l - 1], s->dz[cur ^ 1], nullptr, nx,
-                                             ny, nz, static_cast<float>(s->d.leak));
+            k_conv_rows<2, CONV_ROWS><<<conv_rows_grid(nx, ny, nz), 256, 0, st>>>(
+                s->dz[cur], Ly.wd, nullptr, s->act[l - 1], s->dz[cur ^ 1], nullptr, nx, ny, nz,
+                static_cast<float>(s->d.leak));
         }
         if (apply) {
             AdamArgs a{};

@@ -44,9 +44,13 @@ def random_inputs(rng, shape):
     return x
 
 
-def test_network_forward_matches_oracle(gpu):
+@pytest.mark.parametrize("dims", [(32, 6, 5), (160, 5, 3)])
+def test_network_forward_matches_oracle(gpu, dims):
+    """(160, 5, 3): a partial 32-voxel x chunk after a full 128 and an odd
+    row count, for the LDS-staged layers' work items."""
     rng = np.random.default_rng(11)
-    with ah.Surrogate(gpu, **SMALL) as s:
+    kw = dict(SMALL, nx=dims[0], ny=dims[1], nz=dims[2], dx=550e-6 / dims[0], dy=550e-6 / dims[0])
+    with ah.Surrogate(gpu, **kw) as s:
         layers = random_layers(rng)
         for li, v in enumerate(layers):
             s.set_weights(li, v)
@@ -58,9 +62,11 @@ def test_network_forward_matches_oracle(gpu):
         assert err < 5e-3, err
 
 
-def test_network_gradients_match_oracle(gpu):
+@pytest.mark.parametrize("dims", [(32, 6, 5), (160, 5, 3)])
+def test_network_gradients_match_oracle(gpu, dims):
     rng = np.random.default_rng(12)
-    with ah.Surrogate(gpu, **SMALL) as s:
+    kw = dict(SMALL, nx=dims[0], ny=dims[1], nz=dims[2], dx=550e-6 / dims[0], dy=550e-6 / dims[0])
+    with ah.Surrogate(gpu, **kw) as s:
         layers = random_layers(rng)
         for li, v in enumerate(layers):
             s.set_weights(li, v)
