@@ -249,6 +249,8 @@ struct SweepArgs {
     float ambient;
 };
 
+constexpr int SWEEP_PRELOAD = 16;   // deposits a column walker loads ahead
+
 struct KSweep {
     using Args = SweepArgs;
     static __device__ void run(const Args& a, VGrid vg) {
@@ -259,15 +261,26 @@ struct KSweep {
     const int line = vg.block * BLOCK + threadIdx.x;
     if (line >= other * nz) return;
     const int k = line / other, l = line % other;
-    double t = 1.0;
-    for (int s = 0; s < len; ++s) {
+    const auto index = [&](int s) -> size_t {
         const int step = sign > 0 ? s : (len - 1 - s);
-        const size_t idx = axis == 0 ? (static_cast<size_t>(k) * ny + l) * nx + step
-                                     : (static_cast<size_t>(k) * ny + step) * nx + l;
-        out[idx] = static_cast<float>(static_cast<double>(source) * t) + ambient;
-        const unsigned long long raw = blocked[idx];
-        if (raw != 0ull) blocked[idx] = 0ull;
-        t *= column_transmittance(from_fixed(raw, scale), face);
+        return axis == 0 ? (static_cast<size_t>(k) * ny + l) * nx + step
+                         : (static_cast<size_t>(k) * ny + step) * nx + l;
+    };
+    // Each chunk's deposits are loaded before any is cleared, so the loads
+    // are in flight together instead of one per step.
+    double t = 1.0;
+    for (int s0 = 0; s0 < len; s0 += SWEEP_PRELOAD) {
+        unsigned long long raw[SWEEP_PRELOAD];
+#pragma unroll
+        for (int q = 0; q < SWEEP_PRELOAD; ++q) raw[q] = s0 + q < len ? blocked[index(s0 + q)] : 0ull;
+#pragma unroll
+        for (int q = 0; q < SWEEP_PRELOAD; ++q) {
+            if (s0 + q >= len) break;
+            const size_t idx = index(s0 + q);
+            out[idx] = static_cast<float>(static_cast<double>(source) * t) + ambient;
+            if (raw[q] != 0ull) blocked[idx] = 0ull;
+            t *= column_transmittance(from_fixed(raw[q], scale), face);
+        }
     }
     }
 };
@@ -296,13 +309,19 @@ struct KSweepRows {
         for (int c0 = 0; c0 < nx; c0 += SWEEP_CHUNK) {
             const int pos = c0 + pl;
             const size_t step = static_cast<size_t>(sign > 0 ? pos : nx - 1 - pos);
-            for (int rr = 0; rr < 64; rr += 2) {
-                const int r = rr + half, line = base + r;
+            // all 32 loads of the chunk first, then the clears and factors
+            unsigned long long raw[32];
+#pragma unroll
+            for (int q = 0; q < 32; ++q) {
+                const int line = base + 2 * q + half;
+                raw[q] = (line < lines && pos < nx) ? blocked[static_cast<size_t>(line) * nx + step] : 0ull;
+            }
+#pragma unroll
+            for (int q = 0; q < 32; ++q) {
+                const int r = 2 * q + half, line = base + r;
                 if (line < lines && pos < nx) {
-                    const size_t idx = static_cast<size_t>(line) * nx + step;
-                    const unsigned long long raw = blocked[idx];
-                    if (raw != 0ull) blocked[idx] = 0ull;
-                    f[r][pl] = column_transmittance(from_fixed(raw, scale), face);
+                    if (raw[q] != 0ull) blocked[static_cast<size_t>(line) * nx + step] = 0ull;
+                    f[r][pl] = column_transmittance(from_fixed(raw[q], scale), face);
                 }
             }
             wave_sync();
