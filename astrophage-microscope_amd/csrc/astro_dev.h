@@ -152,6 +152,19 @@ struct Pcg {
     }
 };
 
+// The Gaussian pair in float for the FP32 storage layout: the same two
+// uniform01d draws (the stream stays bit-exact), the transform in float.
+__device__ __forceinline__ void gaussian_pair_f(Pcg& r, float& z0, float& z1) {
+    float u1 = static_cast<float>(r.uniform01d());
+    if (!(u1 >= 1.17549435e-38f)) u1 = 1.17549435e-38f;
+    const float u2 = static_cast<float>(r.uniform01d());
+    const float mag = sqrtf(-2.0f * logf(u1));
+    float s, c;
+    sincosf(6.28318530717958648f * u2, &s, &c);
+    z0 = mag * c;
+    z1 = mag * s;
+}
+
 __device__ __forceinline__ uint64_t stream_inc(uint64_t id) {
     return (splitmix64(id ^ 0xA5A5A5A5DEADBEEFull) << 1u) | 1u;
 }
@@ -224,6 +237,73 @@ __device__ __forceinline__ Ou ou_coefficients(double mass, double gamma, double 
         c.noise_b = sqrt(b2 > 0.0 ? b2 : 0.0);
     }
     return c;
+}
+
+// The FP32 storage layout's integrator: drag, coefficients and the noise in
+// float; displacements are summed onto the position and velocity in double,
+// where the float terms add exactly, so the only rounding of the result is
+// the float store, as in the double path, and the wall tests see the same
+// values to within the float terms' own error (about 1e-7 of a
+// displacement). Small beta * dt, where the float forms would cancel, takes
+// the double coefficients.
+struct OuF { float one_minus_e1, inv_beta, sigma_rr, sigma_vv, noise_a, noise_b; };
+
+__device__ __forceinline__ float drag_coefficient_f(float t) {
+    const float eta = static_cast<float>(ASTRO_VF_A) * expf(static_cast<float>(ASTRO_VF_B) / (t - static_cast<float>(ASTRO_VF_C)));
+    return 6.0f * static_cast<float>(ASTRO_PI) * eta * static_cast<float>(ASTRO_CELL_RADIUS);
+}
+
+__device__ __forceinline__ OuF ou_coefficients_f(float mass, float gamma, float t, float dt) {
+    const float beta = gamma / mass;
+    const float x = beta * dt;
+    OuF c;
+    if (x < 1.0f) {
+        const Ou d = ou_coefficients(static_cast<double>(mass), static_cast<double>(gamma), static_cast<double>(t),
+                                     static_cast<double>(dt));
+        c.one_minus_e1 = static_cast<float>(d.one_minus_e1);
+        c.inv_beta = static_cast<float>(d.inv_beta);
+        c.sigma_rr = static_cast<float>(d.sigma_rr);
+        c.sigma_vv = static_cast<float>(d.sigma_vv);
+        c.noise_a = static_cast<float>(d.noise_a);
+        c.noise_b = static_cast<float>(d.noise_b);
+        return c;
+    }
+    c.one_minus_e1 = -expm1f(-x);
+    c.inv_beta = 1.0f / beta;
+    const float kT_over_m = static_cast<float>(ASTRO_K_BOLTZ) * t / mass;
+    const float var_vv = kT_over_m * (-expm1f(-2.0f * x));
+    const float e = expf(-x);
+    const float shape = 2.0f * x - 3.0f + 4.0f * e - e * e;
+    const float var_rr = kT_over_m * c.inv_beta * c.inv_beta * shape;
+    c.sigma_vv = sqrtf(var_vv > 0.0f ? var_vv : 0.0f);
+    c.sigma_rr = sqrtf(var_rr > 0.0f ? var_rr : 0.0f);
+    const float sigma_rv = kT_over_m * c.inv_beta * c.one_minus_e1 * c.one_minus_e1;
+    c.noise_a = 0.0f;
+    c.noise_b = 0.0f;
+    if (c.sigma_rr > 0.0f) {
+        c.noise_a = sigma_rv / c.sigma_rr;
+        const float b2 = c.sigma_vv * c.sigma_vv - c.noise_a * c.noise_a;
+        c.noise_b = sqrtf(b2 > 0.0f ? b2 : 0.0f);
+    }
+    return c;
+}
+
+__device__ __forceinline__ void ou_advance_axis_f(double& pos, double& vel, float force, float gamma, const OuF& c,
+                                                  float dt, bool noise, float z1, float z2) {
+    const float v = static_cast<float>(vel);
+    const float v_term = force / gamma;
+    const float dpos = v_term * dt + (v - v_term) * c.one_minus_e1 * c.inv_beta;
+    const float dvel = (v_term - v) * c.one_minus_e1;
+    double npos = pos + static_cast<double>(dpos);
+    double nvel = vel + static_cast<double>(dvel);
+    if (noise && c.sigma_rr > 0.0f) {
+        npos += static_cast<double>(c.sigma_rr * z1);
+        nvel += static_cast<double>(c.noise_a * z1 + c.noise_b * z2);
+    } else if (noise) {
+        nvel += static_cast<double>(c.sigma_vv * z2);
+    }
+    pos = npos;
+    vel = nvel;
 }
 
 __device__ __forceinline__ void ou_advance_axis(double& pos, double& vel, double force, double gamma,

@@ -3,8 +3,10 @@
 // the per-cell telemetry reduction. One thread per slot; every per-cell array
 // is read and written coalesced in slot order.
 #include <algorithm>
+#include <type_traits>
 
 #include "cell_ops.h"
+
 #include "kernels.h"
 
 namespace astro_dev {
@@ -329,24 +331,39 @@ struct KIntegrate {
     const double mass = cell_mass(c.biomass[i], c.energy[i]);
     const bool awake = (fl & ASTRO_FLAG_AWAKE) != 0u;
     const double t_drag = awake ? ASTRO_CELL_TEMP_SETPOINT : t_local;
-    double gamma = drag_coefficient(t_drag);
     const Force F = fs[spos[i]];
     double px = static_cast<double>(c.x[i]), py = static_cast<double>(c.y[i]), pz = static_cast<double>(c.z[i]);
     double vx = static_cast<double>(c.vx[i]), vy = static_cast<double>(c.vy[i]), vz = static_cast<double>(c.vz[i]);
-    if (fl & ASTRO_FLAG_STUCK) gamma *= ASTRO_WALL_STUCK_DRAG_MULT;
     const uint64_t id = c.id[i];
     Pcg r = cell_rng(c.rng_state[i], id);
-    const Ou oc = ou_coefficients(mass, gamma, t_drag, dt);
     // The three pairs are drawn in the reference's order (x, y, z); each is
     // used as soon as it is drawn, which keeps fewer values live.
     const bool noise = t.thermal_noise != 0;
-    double z1, z2;
-    r.gaussian_pair(z1, z2);
-    ou_advance_axis(px, vx, F.x, gamma, oc, dt, noise, z1, z2);
-    r.gaussian_pair(z1, z2);
-    ou_advance_axis(py, vy, F.y, gamma, oc, dt, noise, z1, z2);
-    r.gaussian_pair(z1, z2);
-    ou_advance_axis(pz, vz, F.z, gamma, oc, dt, noise, z1, z2);
+    if constexpr (std::is_same_v<P, float>) {
+        // FP32 storage: float drag, coefficients and Gaussians (see OuF)
+        float gf = drag_coefficient_f(static_cast<float>(t_drag));
+        if (fl & ASTRO_FLAG_STUCK) gf *= static_cast<float>(ASTRO_WALL_STUCK_DRAG_MULT);
+        const float dtf = static_cast<float>(dt);
+        const OuF oc = ou_coefficients_f(static_cast<float>(mass), gf, static_cast<float>(t_drag), dtf);
+        float z1, z2;
+        gaussian_pair_f(r, z1, z2);
+        ou_advance_axis_f(px, vx, static_cast<float>(F.x), gf, oc, dtf, noise, z1, z2);
+        gaussian_pair_f(r, z1, z2);
+        ou_advance_axis_f(py, vy, static_cast<float>(F.y), gf, oc, dtf, noise, z1, z2);
+        gaussian_pair_f(r, z1, z2);
+        ou_advance_axis_f(pz, vz, static_cast<float>(F.z), gf, oc, dtf, noise, z1, z2);
+    } else {
+        double gamma = drag_coefficient(t_drag);
+        if (fl & ASTRO_FLAG_STUCK) gamma *= ASTRO_WALL_STUCK_DRAG_MULT;
+        const Ou oc = ou_coefficients(mass, gamma, t_drag, dt);
+        double z1, z2;
+        r.gaussian_pair(z1, z2);
+        ou_advance_axis(px, vx, F.x, gamma, oc, dt, noise, z1, z2);
+        r.gaussian_pair(z1, z2);
+        ou_advance_axis(py, vy, F.y, gamma, oc, dt, noise, z1, z2);
+        r.gaussian_pair(z1, z2);
+        ou_advance_axis(pz, vz, F.z, gamma, oc, dt, noise, z1, z2);
+    }
     const double a = ASTRO_CELL_RADIUS;
     bool kept = apply_boundary_axis(px, vx, 0.5 * t.chamber_w, a, t.boundary_x);
     kept = apply_boundary_axis(py, vy, 0.5 * t.chamber_h, a, t.boundary_y) && kept;

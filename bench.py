@@ -10,8 +10,9 @@ compaction on. To make division and compaction do work every tick:
   * CO2 starts at 1 kg/m^3, the biology clock runs at 1e5, and each cell
     starts with a uniform fraction of the division quota, so a trickle divide;
   * the other 90 % are dormant with log-uniform charge in [1e-3, 0.2].
-The light is off. Cell state is stored FP32 (positions, velocities; FP64
-arithmetic), energies and CO2 stores FP64, fields FP32.
+The light is off. Positions and velocities are stored FP32 (the integrator's
+drag, coefficients and Gaussians in FP32, displacement sums in FP64),
+energies and CO2 stores FP64, fields FP32.
 
 `value` = cells x ticks / second over the timed ticks (the cell count of each
 tick summed on the device), with all state resident in HBM; the ticks are
@@ -413,7 +414,7 @@ def main() -> int:
         "config": {"workload": "plain method, scale: 1M cells, 4x4x0.2 mm, 512x512x32 FP32 T+CO2, 10 um hash, "
                                "division + compaction on, dt 1e-4 s",
                    "cells": a.cells, "cells_end": c1.count, "storage": a.precision,
-                   "arithmetic": "fp64 kinematics and energies, fp32 fields", "graph_replay": True,
+                   "arithmetic": "fp32 integrator transcendentals, fp64 displacement sums and energies, fp32 fields", "graph_replay": True,
                    "divisions": c1.divisions_total - c0.divisions_total,
                    "deaths": c1.deaths_total - c0.deaths_total},
         "roofline": roof,

@@ -15,8 +15,9 @@
  *
  * Precision: ASTRO_FP64_CONTRACT keeps the reference's contract-v1 SoA
  * (f64 kinematics, contracts/cell_store_v1.h:42-78); ASTRO_FP32_HOT stores
- * position and velocity as f32 in device memory. Host-side views are always the
- * f64 contract layout.
+ * position and velocity as f32 in device memory and computes the integrator's
+ * drag, OU coefficients and Gaussian draws in f32 (displacement sums, energies
+ * and budgets stay f64). Host-side views are always the f64 contract layout.
  */
 #ifndef ASTRO_HOT_H
 #define ASTRO_HOT_H

@@ -106,6 +106,35 @@ def test_ftcs_parity(oracle, gpu, dims, bc_field):
     assert np.array_equal(o.field(bc_field), g.field(bc_field))
 
 
+# FP32 storage: the integrator's drag, coefficients and Gaussians run in
+# float (about 1e-7 of a displacement) and each tick rounds positions to
+# float, so a coordinate may land one ulp (2.3e-10 m at 2 mm) from the FP64
+# oracle's rounding of it on some ticks; two ulps bound it over these runs.
+FP32_POS_TOL = 5.0e-10
+# A one-ulp position difference inside a contact changes the contact force by
+# k * 1e-10 m and the drift velocity by that over gamma: ~1.3e-7 m/s per ulp
+# at the C1 stiffness (1.18e-4 N/m, gamma ~ 9e-8 kg/s).
+FP32_VEL_TOL = 5.0e-7
+
+
+def test_tick_parity_fp32_dormant(oracle, gpu):
+    """C1 shape in FP32 storage, 2000 dormant cells, 10 ticks: integer state
+    (flags, RNG streams) bit-exact, positions within FP32_POS_TOL."""
+    kw = dict(C1, capacity=2000, ambient_temp=369.0, precision=ah.FP32_HOT)
+    cells = oracle_cells(oracle, 2000, 2, spawn=dict(charge_dist=ah.DIST_UNIFORM, charge_b=0.2), **kw)
+    for k in ("x", "y", "z", "vx", "vy", "vz"):
+        cells[k] = cells[k].astype(np.float32).astype(np.float64)
+    o, g = pair(oracle, gpu, cells, **kw)
+    o.step(10)
+    g.step(10)
+    a, b = o.download(), g.download()
+    compare_cells(a, b)
+    for name in ("x", "y", "z"):
+        rel_close(a[name], b[name], 0.0, atol=FP32_POS_TOL, name=name)
+    for name in ("vx", "vy", "vz"):
+        rel_close(a[name], b[name], 0.0, atol=FP32_VEL_TOL, name=name)
+
+
 def test_tick_parity_fp64_dormant(oracle, gpu):
     """C1 shape, 2000 dormant cells, T 64^3 at 369.0 K (below ignition)."""
     kw = dict(C1, capacity=2000, ambient_temp=369.0)
@@ -245,7 +274,7 @@ def test_tick_parity_co2_rationing(oracle, gpu, precision):
         assert co.divisions_total > 0 and co.deaths_total > 0
         a, b = o.download(), g.download()
         compare_cells(a, b)
-        tol = 0.0 if precision == ah.FP64_CONTRACT else 1.25e-10
+        tol = 0.0 if precision == ah.FP64_CONTRACT else FP32_POS_TOL
         for name in ("x", "y", "z"):
             rel_close(a[name], b[name], 1e-12, atol=tol, name=name)
         rel_close(a["co2_held"], b["co2_held"], 1e-12, name="co2_held")
