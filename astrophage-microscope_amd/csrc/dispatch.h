@@ -45,14 +45,14 @@ __global__ __launch_bounds__(BLOCK) void k_many(const typename K::Args* a, const
     K::run(args, VGrid{static_cast<int>(blockIdx.x) - m.y, m.z});
 }
 
-// The same wrappers for bodies that need three waves per SIMD of registers.
-template <class K>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_one_w3(typename K::Args a) {
+// The same wrappers with a register budget for at least W waves per SIMD.
+template <class K, int W>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(W, 8))) void k_one_w(typename K::Args a) {
     K::run(a, VGrid{static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x)});
 }
 
-template <class K>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_many_w3(
+template <class K, int W>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(W, 8))) void k_many_w(
     const typename K::Args* a, const int4* map) {
     const int4 m = map[blockIdx.x];
     const typename K::Args args = a[m.x];

@@ -36,12 +36,13 @@ struct Launch {
 template <class K>
 concept GridStride = requires { K::stride; } && static_cast<bool>(K::stride);
 
-template <class K, bool W3 = false>
+// W > 0 caps the body's registers for W waves per SIMD.
+template <class K, int W = 0>
 void launch_k(const Launch& L, const char* name, const std::vector<typename K::Args>& args,
               std::vector<int> blocks) {
     if (!L.rec) {
         if (L.prof) L.prof->begin(name);
-        if constexpr (W3) k_one_w3<K><<<blocks[0], BLOCK, 0, L.stream>>>(args[0]);
+        if constexpr (W > 0) k_one_w<K, W><<<blocks[0], BLOCK, 0, L.stream>>>(args[0]);
         else k_one<K><<<blocks[0], BLOCK, 0, L.stream>>>(args[0]);
         if (L.prof) L.prof->end();
         return;
@@ -55,7 +56,7 @@ void launch_k(const Launch& L, const char* name, const std::vector<typename K::A
     L.rec->launches.push_back({name, [=](hipStream_t s, const char* arena) {
         const auto* a = reinterpret_cast<const typename K::Args*>(arena + a_off);
         const int4* m = reinterpret_cast<const int4*>(arena + m_off);
-        if constexpr (W3) k_many_w3<K><<<total, BLOCK, 0, s>>>(a, m);
+        if constexpr (W > 0) k_many_w<K, W><<<total, BLOCK, 0, s>>>(a, m);
         else k_many<K><<<total, BLOCK, 0, s>>>(a, m);
     }});
 }

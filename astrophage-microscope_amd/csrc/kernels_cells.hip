@@ -236,9 +236,11 @@ struct KFeedTaxis {
 // ---------------------------------------------------------------------------
 // field_sample_kernel (src/sim/integrator.cu:20-30) and the body-force half of
 // forces_kernel (:40-58), per physical entry; the force goes to the cell's
-// sorted position for the contact walk to add to.
+// sorted position for the contact walk to add to. With FEED / TAXIS the same
+// pass first runs feed_kernel / taxis_kernel (src/sim/emission.cu:81-115,
+// src/sim/taxis.cu:13-87) on the cell, which read no other cell.
 // ---------------------------------------------------------------------------
-template <class P>
+template <class P, bool FEED, bool TAXIS>
 struct KSampleBase {
     static constexpr bool stride = true;   // any block count covers the work
     struct Args {
@@ -252,14 +254,28 @@ struct KSampleBase {
         int gaxis;
         const int32_t* spos;
         Force* fs;
+        const double* dirs;
+        float ambient;
+        double dt;
+        double max_power;
     };
     static __device__ void run(const Args& a, VGrid vg) {
-        const auto& [c, ctr, T, C, N, c_on, n_on, gaxis, spos, fs] = a;
+        const auto& [c, ctr, T, C, N, c_on, n_on, gaxis, spos, fs, dirs, ambient, dt, max_power] = a;
 
     const int n = ctr->nphys;
     xcd_tiles(vg, n, [&](const int i, const bool valid) {
         if (!valid || c.lslot[i] == NO_SLOT) return;
-        const Force F = sample_force_cell<P>(c, i, c.flags[i], static_cast<float>(c.x[i]), static_cast<float>(c.y[i]),
+        uint32_t f = c.flags[i];
+        if constexpr (FEED || TAXIS) {
+            // feeding and taxis first, as their own pass would (no other cell
+            // is read), then the force from the updated state
+            if ((f & ASTRO_FLAG_OCCUPIED) && (f & ASTRO_FLAG_ALIVE)) {
+                const double directional = FEED ? dirs[spos[i]] : 0.0;
+                feed_taxis_cell<P, FEED, TAXIS>(c, i, f, directional, ambient, dt, max_power);
+                f = c.flags[i];
+            }
+        }
+        const Force F = sample_force_cell<P>(c, i, f, static_cast<float>(c.x[i]), static_cast<float>(c.y[i]),
                                              static_cast<float>(c.z[i]), T, C, N, c_on, n_on, gaxis);
         fs[spos[i]] = F;
     });
@@ -902,18 +918,27 @@ void launch_feed_taxis(const Launch& L, Views<P> ws, Ticks ts, bool feed, bool t
     else if (taxis) launch_feed_taxis_t<P, false, true>(L, ws, ts, "taxis");
 }
 
-template <class P>
-void launch_cell_forces(const Launch& L, Views<P> ws, Ticks ts, bool feed, bool taxis) {
-    if (feed || taxis) launch_feed_taxis<P>(L, ws, ts, feed, taxis);
-    using K = KSampleBase<P>;
+template <class P, bool FEED, bool TAXIS>
+void launch_sample_t(const Launch& L, Views<P> ws, Ticks ts, const char* name) {
+    using K = KSampleBase<P, FEED, TAXIS>;
     Batch<K> b;
     for (size_t i = 0; i < ws.size(); ++i) {
         const WorldDev<P>& w = ws[i];
+        const TickParams& t = ts[i];
         b.add({w.c, w.ctr, w.g[ASTRO_FIELD_TEMPERATURE], w.g[ASTRO_FIELD_CO2], w.g[ASTRO_FIELD_N2],
-               w.on[ASTRO_FIELD_CO2] ? 1 : 0, w.on[ASTRO_FIELD_N2] ? 1 : 0, ts[i].gravity_axis, w.spos, w.fs},
+               w.on[ASTRO_FIELD_CO2] ? 1 : 0, w.on[ASTRO_FIELD_N2] ? 1 : 0, t.gravity_axis, w.spos, w.fs, w.dirs,
+               t.ambient_irr, t.dt, t.petrova_max_power},
               cell_grid(w.pcap));
     }
-    launch_k<K>(L, "sample_base", b.args, b.blocks);
+    launch_k<K>(L, name, b.args, b.blocks);
+}
+
+template <class P>
+void launch_cell_forces(const Launch& L, Views<P> ws, Ticks ts, bool feed, bool taxis) {
+    if (feed && taxis) launch_sample_t<P, true, true>(L, ws, ts, "feed_taxis_sample");
+    else if (feed) launch_sample_t<P, true, false>(L, ws, ts, "feed_sample");
+    else if (taxis) launch_sample_t<P, false, true>(L, ws, ts, "taxis_sample");
+    else launch_sample_t<P, false, false>(L, ws, ts, "sample_base");
     launch_contact_walk<P>(L, ws, ts);
 }
 
@@ -928,7 +953,9 @@ void launch_integrate_t(const Launch& L, Views<P> ws, Ticks ts, const char* name
     Batch<K> b;
     for (size_t i = 0; i < ws.size(); ++i)
         b.add({ws[i].c, ws[i].ctr, ws[i].spos, ws[i].fs, ts[i], demand_args(ws[i], ts[i])}, cell_grid(ws[i].pcap));
-    launch_k<K, true>(L, name, b.args, b.blocks);
+    // FP64 layout: 168 VGPRs, 3 waves per SIMD; the FP32 layout's float
+    // integrator fits 4
+    launch_k<K, std::is_same_v<P, float> ? 4 : 3>(L, name, b.args, b.blocks);
 }
 
 template <class P>

@@ -56,6 +56,11 @@ def kernel_bytes(name: str, c: dict) -> float:
         # own record, the force read and written; neighbour records and
         # bucket starts are re-reads of data other lanes fetched
         "contact_walk": n * (r + 64),
+        # sample_base with feeding and/or taxis in the same pass: both
+        # kernels' bytes, the flags read once
+        "taxis_sample": 0.0,
+        "feed_sample": 0.0,
+        "feed_taxis_sample": 0.0,
         "integrate": integ,
         # + the voxel and concentration each live cell books
         "integrate_demand": integ + 4 * n + alive * 16,
@@ -73,5 +78,7 @@ def kernel_bytes(name: str, c: dict) -> float:
         "feed_walk": alive * (r + 8 + 4),
         "spawn": n * S,
     }
+    for fused, part in (("taxis_sample", "taxis"), ("feed_sample", "feed"), ("feed_taxis_sample", "feed_taxis")):
+        table[fused] = table["sample_base"] + table[part] - 4 * n
     return float(table.get(name, 0.0))
 
