@@ -172,6 +172,10 @@ PRODUCT_ONLY = {
     "profile_read": (_I, [_P, _P, _I, _P, _P, _I, C.POINTER(_I)]),
     "profile_reset": (_I, [_P]),
     "world_stream": (_P, [_P]),
+    "snapshot_to_bytes": (_I, [_P, C.c_char_p, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "snapshot_save": (_I, [_P, C.c_char_p, C.c_char_p]),
+    "snapshot_from_bytes": (_I, [C.POINTER(WorldDesc), _P, C.c_size_t, C.POINTER(_P)]),
+    "snapshot_load": (_I, [C.POINTER(WorldDesc), C.c_char_p, C.POINTER(_P)]),
     "ensemble_create": (_I, [C.POINTER(_P), _I, C.POINTER(_P)]),
     "ensemble_destroy": (None, [_P]),
     "ensemble_last_error": (C.c_char_p, [_P]),
@@ -257,16 +261,59 @@ def _ptr(a: np.ndarray | None):
 class World:
     """One simulated chamber (src/sim/world.cuh:89-180) behind the C ABI."""
 
-    def __init__(self, binding: Binding, desc: WorldDesc | None = None, **kw):
+    def __init__(self, binding: Binding, desc: WorldDesc | None = None, _handle=None, **kw):
         self.b = binding
-        if desc is None:
-            desc = binding.desc(**kw)
-        h = C.c_void_p()
-        self._check(binding.world_create(C.byref(desc), C.byref(h)), what="world_create")
-        self.h = h
+        if _handle is None:
+            if desc is None:
+                desc = binding.desc(**kw)
+            h = C.c_void_p()
+            self._check(binding.world_create(C.byref(desc), C.byref(h)), what="world_create")
+            _handle = h
+        self.h = _handle
         d = WorldDesc()
         self._check(binding.world_desc_get(self.h, C.byref(d)))
         self.desc = d
+
+    # -- ASPH snapshots (src/sim/snapshot.h:17-40) ---------------------------
+    @classmethod
+    def from_snapshot(cls, binding: Binding, data: bytes, desc: WorldDesc | None = None, **kw) -> "World":
+        """snapshot_from_bytes: a new world holding the snapshot's state; the
+        configuration the file does not carry (grids, motion, light) comes
+        from `desc` or `kw` as for World()."""
+        if desc is None:
+            desc = binding.desc(**kw)
+        buf = C.create_string_buffer(bytes(data), len(data))
+        h = C.c_void_p()
+        st = binding.snapshot_from_bytes(C.byref(desc), C.cast(buf, C.c_void_p), len(data), C.byref(h))
+        if st != OK:
+            raise AstroError(st, "snapshot_from_bytes")
+        return cls(binding, _handle=h)
+
+    @classmethod
+    def load_snapshot(cls, binding: Binding, path, desc: WorldDesc | None = None, **kw) -> "World":
+        """snapshot_load."""
+        if desc is None:
+            desc = binding.desc(**kw)
+        h = C.c_void_p()
+        st = binding.snapshot_load(C.byref(desc), str(path).encode(), C.byref(h))
+        if st != OK:
+            raise AstroError(st, f"snapshot_load {path}")
+        return cls(binding, _handle=h)
+
+    def snapshot(self, scenario_id: str | None = None) -> bytes:
+        """snapshot_to_bytes."""
+        sid = scenario_id.encode() if scenario_id is not None else None
+        n = C.c_size_t()
+        self._check(self.b.snapshot_to_bytes(self.h, sid, None, 0, C.byref(n)), "snapshot size")
+        buf = C.create_string_buffer(n.value)
+        self._check(self.b.snapshot_to_bytes(self.h, sid, C.cast(buf, C.c_void_p), n.value, C.byref(n)),
+                    "snapshot")
+        return buf.raw[:n.value]
+
+    def save_snapshot(self, path, scenario_id: str | None = None):
+        """snapshot_save."""
+        sid = scenario_id.encode() if scenario_id is not None else None
+        self._check(self.b.snapshot_save(self.h, str(path).encode(), sid), f"snapshot_save {path}")
 
     # -- plumbing ----------------------------------------------------------
     def _check(self, st: int, what: str = ""):

@@ -10,6 +10,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstddef>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -212,6 +214,9 @@ struct astro_world {
     double sim_time = 0.0;
     int64_t div_reported = 0, deaths_reported = 0;
     bool hash_valid = false;
+    // The sticky non_canon_run (src/sim/world.cuh:157-158), restored from
+    // a snapshot header; parameter overrides also make a run non-canon.
+    bool non_canon = false;
 
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -608,6 +613,122 @@ int32_t ensure_hash(astro_world* w) {
     else launch_hash_build<double>(L, one(make_view<double>(w)), false);
     w->hash_valid = true;
     return hip_check(w, hipGetLastError(), "hash_build");
+}
+
+// ---------------------------------------------------------------------------
+// Snapshots: the contract's file layout (contracts/snapshot_v1.h:14-57) --
+// a 256-byte header, then the body: parameter overrides, the cell SoA
+// [0, count) array by array in contract order, the (empty) Taumoeba arrays,
+// and the T, CO2, N2 values. The body's FNV-1a 64 is the state hash.
+// ---------------------------------------------------------------------------
+// The contract's FNV starting value (contracts/snapshot_v1.h:59); the
+// published offset basis has one more digit (14695981039346656037).
+constexpr uint64_t FNV_OFFSET = 1469598103934665603ull;
+constexpr uint32_t SNAP_MAGIC = 0x48505341u;   // 'ASPH'
+constexpr uint32_t SNAP_VERSION = 1u;
+
+struct SnapHeader {
+    uint32_t magic, version;
+    uint64_t seed, tick;
+    double sim_time_s;
+    int32_t cell_count, cell_capacity, taumoeba_count, taumoeba_capacity;
+    int32_t n_temp, n_co2, n_n2;
+    double chamber_w, chamber_h, chamber_d, physics_rate, biology_rate;
+    uint64_t next_cell_id;
+    int32_t override_count;
+    uint8_t non_canon_run, pad[3];
+    char scenario_id[64];
+    char build_describe[64];
+    uint64_t state_hash;
+};
+static_assert(sizeof(SnapHeader) == 256, "snapshot header layout");
+static_assert(offsetof(SnapHeader, chamber_w) == 64 && offsetof(SnapHeader, state_hash) == 248, "snapshot header");
+
+struct SnapOverride {
+    char key[48];
+    double value;
+};
+static_assert(sizeof(SnapOverride) == 56, "snapshot override layout");
+
+// The overrides of the canon values a world's description carries.
+std::vector<SnapOverride> overrides_of(const astro_world_desc& d) {
+    std::vector<SnapOverride> out;
+    auto add = [&](const char* key, double v, double canon) {
+        if (v == canon) return;
+        SnapOverride o;
+        std::memset(&o, 0, sizeof(o));   // the key's padding is hashed
+        std::strncpy(o.key, key, sizeof(o.key) - 1);
+        o.value = v;
+        out.push_back(o);
+    };
+    add("PETROVA_MAX_POWER", d.petrova_max_power, ASTRO_PETROVA_MAX_POWER);
+    add("CO2_MASS_PER_DIVISION", d.co2_mass_per_division, ASTRO_CO2_MASS_PER_DIVISION);
+    return out;
+}
+
+// Snapshot size field of one grid: n for the reference's square 2D grid,
+// minus the voxel count for a 3D grid (no snapshot_v1 form), 0 when absent.
+int32_t snap_grid_n(const FieldHost& g) {
+    if (!g.on) return 0;
+    if (g.nz == 1 && g.nx == g.ny) return g.nx;
+    return -static_cast<int32_t>(g.count());
+}
+
+// Host buffers of the contract SoA for n cells.
+struct HostSoa {
+    std::vector<uint64_t> id, rng;
+    std::vector<uint32_t> flags;
+    std::vector<uint8_t> dc;
+    std::vector<double> x, y, z, vx, vy, vz, energy, biomass, co2;
+    std::vector<float> tc, ep, dxv, dyv, dzv, age, tm, rt, tl, ir, cl, nl;
+    explicit HostSoa(size_t n)
+        : id(n), rng(n), flags(n), dc(n), x(n), y(n), z(n), vx(n), vy(n), vz(n), energy(n), biomass(n), co2(n),
+          tc(n), ep(n), dxv(n), dyv(n), dzv(n), age(n), tm(n), rt(n), tl(n), ir(n), cl(n), nl(n) {}
+    astro_cell_soa view() {
+        return astro_cell_soa{id.data(), flags.data(), dc.data(), x.data(), y.data(), z.data(), vx.data(), vy.data(),
+                              vz.data(), energy.data(), tc.data(), ep.data(), dxv.data(), dyv.data(), dzv.data(),
+                              biomass.data(), co2.data(), age.data(), rng.data(), tm.data(), rt.data(), tl.data(),
+                              ir.data(), cl.data(), nl.data()};
+    }
+    // visits the arrays in contract order (contracts/cell_store_v1.h:42-78)
+    template <class F>
+    void each(F&& f) {
+        f(id); f(flags); f(dc); f(x); f(y); f(z); f(vx); f(vy); f(vz); f(energy); f(tc); f(ep); f(dxv); f(dyv);
+        f(dzv); f(biomass); f(co2); f(age); f(rng); f(tm); f(rt); f(tl); f(ir); f(cl); f(nl);
+    }
+};
+
+// Everything after the header, in file order (src/sim/snapshot.cu:96-118).
+int32_t serialize_body(astro_world* w, std::vector<char>& body, int32_t& override_count, int32_t& count) {
+    body.clear();
+    Counters c{};
+    int32_t st = read_counters(w, c);
+    if (st != ASTRO_OK) return st;
+    count = c.count;
+    const std::vector<SnapOverride> ov = overrides_of(w->desc);
+    override_count = static_cast<int32_t>(ov.size());
+    auto put = [&](const void* p, size_t bytes) {
+        const char* q = static_cast<const char*>(p);
+        body.insert(body.end(), q, q + bytes);
+    };
+    if (!ov.empty()) put(ov.data(), sizeof(SnapOverride) * ov.size());
+    if (count > 0) {
+        HostSoa h(count);
+        astro_cell_soa v = h.view();
+        st = w->fp32 ? download_cells(w, w->c32[0], &v, count) : download_cells(w, w->c64[0], &v, count);
+        if (st != ASTRO_OK) return st;
+        h.each([&](const auto& a) { put(a.data(), sizeof(a[0]) * a.size()); });
+    }
+    for (int fi : {ASTRO_FIELD_TEMPERATURE, ASTRO_FIELD_CO2, ASTRO_FIELD_N2}) {
+        const FieldHost& g = w->f[fi];
+        if (!g.on) continue;
+        std::vector<float> v(g.count());
+        if ((st = hip_check(w, hipMemcpy(v.data(), g.value, sizeof(float) * v.size(), hipMemcpyDeviceToHost),
+                            "snapshot field")) != ASTRO_OK)
+            return st;
+        put(v.data(), sizeof(float) * v.size());
+    }
+    return ASTRO_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -1229,56 +1350,18 @@ int32_t astro_world_stats(astro_world* w, astro_stats* s) {
     w->div_reported = c.divisions_total;
     w->deaths_reported = c.deaths_total;
     s->boil_events = s->max_temp_medium_k >= 373.14999999999998 ? 1 : 0;
-    s->non_canon_run = (w->desc.petrova_max_power != ASTRO_PETROVA_MAX_POWER ||
-                        w->desc.co2_mass_per_division != ASTRO_CO2_MASS_PER_DIVISION) ? 1 : 0;
+    s->non_canon_run = (w->non_canon || !overrides_of(w->desc).empty()) ? 1 : 0;
     return ASTRO_OK;
 }
 
 // snapshot_state_hash (src/sim/snapshot.cu:100-129).
 int32_t astro_world_state_hash(astro_world* w, uint64_t* out) {
     if (!w || !out) return ASTRO_INVALID_ARGUMENT;
-    Counters c{};
-    int32_t st = read_counters(w, c);
+    std::vector<char> body;
+    int32_t oc = 0, n = 0;
+    const int32_t st = serialize_body(w, body, oc, n);
     if (st != ASTRO_OK) return st;
-    const int32_t n = c.count;
-    uint64_t h = 1469598103934665603ull;
-    struct Ov { char key[48]; double value; };
-    auto ov = [&](const char* key, double v, double canon) {
-        if (v == canon) return;
-        Ov o;
-        std::memset(&o, 0, sizeof(o));
-        std::strncpy(o.key, key, sizeof(o.key) - 1);
-        o.value = v;
-        h = fnv1a64(&o, sizeof(o), h);
-    };
-    ov("PETROVA_MAX_POWER", w->desc.petrova_max_power, ASTRO_PETROVA_MAX_POWER);
-    ov("CO2_MASS_PER_DIVISION", w->desc.co2_mass_per_division, ASTRO_CO2_MASS_PER_DIVISION);
-    if (n > 0) {
-        std::vector<uint64_t> id(n), rng(n);
-        std::vector<uint32_t> flags(n);
-        std::vector<uint8_t> dc(n);
-        std::vector<double> x(n), y(n), z(n), vx(n), vy(n), vz(n), energy(n), biomass(n), co2(n);
-        std::vector<float> tc(n), ep(n), dxv(n), dyv(n), dzv(n), age(n), tm(n), rt(n), tl(n), ir(n), cl(n), nl(n);
-        astro_cell_soa s{id.data(), flags.data(), dc.data(), x.data(), y.data(), z.data(), vx.data(), vy.data(),
-                         vz.data(), energy.data(), tc.data(), ep.data(), dxv.data(), dyv.data(), dzv.data(),
-                         biomass.data(), co2.data(), age.data(), rng.data(), tm.data(), rt.data(), tl.data(),
-                         ir.data(), cl.data(), nl.data()};
-        st = w->fp32 ? download_cells(w, w->c32[0], &s, n) : download_cells(w, w->c64[0], &s, n);
-        if (st != ASTRO_OK) return st;
-        auto span = [&](const auto& v) { h = fnv1a64(v.data(), sizeof(v[0]) * v.size(), h); };
-        span(id); span(flags); span(dc); span(x); span(y); span(z); span(vx); span(vy); span(vz);
-        span(energy); span(tc); span(ep); span(dxv); span(dyv); span(dzv); span(biomass); span(co2); span(age);
-        span(rng); span(tm); span(rt); span(tl); span(ir); span(cl); span(nl);
-    }
-    for (int fi : {ASTRO_FIELD_TEMPERATURE, ASTRO_FIELD_CO2, ASTRO_FIELD_N2}) {
-        const FieldHost& g = w->f[fi];
-        if (!g.on) continue;
-        std::vector<float> v(g.count());
-        if ((st = hip_check(w, hipMemcpy(v.data(), g.value, sizeof(float) * v.size(), hipMemcpyDeviceToHost), "hash field")) != ASTRO_OK)
-            return st;
-        h = fnv1a64(v.data(), sizeof(float) * v.size(), h);
-    }
-    *out = h;
+    *out = fnv1a64(body.data(), body.size(), FNV_OFFSET);
     return ASTRO_OK;
 }
 
@@ -1509,6 +1592,163 @@ int32_t astro_profile_reset(astro_world* w) {
 }
 
 void* astro_world_stream(astro_world* w) { return w ? static_cast<void*>(w->stream) : nullptr; }
+
+// ---- snapshots (src/sim/snapshot.cu:131-266) ------------------------------------
+int32_t astro_snapshot_to_bytes(astro_world* w, const char* scenario_id, void* buf, size_t cap, size_t* n_out) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    std::vector<char> body;
+    int32_t oc = 0, count = 0;
+    int32_t st = serialize_body(w, body, oc, count);
+    if (st != ASTRO_OK) return st;
+    const size_t total = sizeof(SnapHeader) + body.size();
+    if (n_out) *n_out = total;
+    if (!buf) return ASTRO_OK;
+    if (cap < total) return fail(w, ASTRO_CAPACITY_EXCEEDED, "snapshot buffer too small");
+    SnapHeader h;
+    std::memset(&h, 0, sizeof(h));
+    h.magic = SNAP_MAGIC;
+    h.version = SNAP_VERSION;
+    h.seed = w->desc.seed;
+    h.tick = w->tick;
+    h.sim_time_s = w->sim_time;
+    h.cell_count = count;
+    h.cell_capacity = w->capacity;
+    h.n_temp = snap_grid_n(w->f[ASTRO_FIELD_TEMPERATURE]);
+    h.n_co2 = snap_grid_n(w->f[ASTRO_FIELD_CO2]);
+    h.n_n2 = snap_grid_n(w->f[ASTRO_FIELD_N2]);
+    h.chamber_w = w->desc.chamber_w;
+    h.chamber_h = w->desc.chamber_h;
+    h.chamber_d = w->desc.chamber_d;
+    h.physics_rate = w->physics_rate;
+    h.biology_rate = w->biology_rate;
+    Counters c{};
+    if ((st = read_counters(w, c)) != ASTRO_OK) return st;
+    h.next_cell_id = c.next_id;
+    h.override_count = oc;
+    h.non_canon_run = (oc > 0 || w->non_canon) ? 1 : 0;
+    if (scenario_id) std::strncpy(h.scenario_id, scenario_id, sizeof(h.scenario_id) - 1);
+    std::strncpy(h.build_describe, "astro_hot", sizeof(h.build_describe) - 1);
+    h.state_hash = fnv1a64(body.data(), body.size(), FNV_OFFSET);
+    std::memcpy(buf, &h, sizeof(h));
+    if (!body.empty()) std::memcpy(static_cast<char*>(buf) + sizeof(h), body.data(), body.size());
+    return ASTRO_OK;
+}
+
+int32_t astro_snapshot_save(astro_world* w, const char* path, const char* scenario_id) {
+    if (!w || !path) return ASTRO_INVALID_ARGUMENT;
+    size_t n = 0;
+    int32_t st = astro_snapshot_to_bytes(w, scenario_id, nullptr, 0, &n);
+    if (st != ASTRO_OK) return st;
+    std::vector<char> bytes(n);
+    if ((st = astro_snapshot_to_bytes(w, scenario_id, bytes.data(), n, &n)) != ASTRO_OK) return st;
+    std::FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(w, ASTRO_FILE_IO_ERROR, "snapshot: cannot open file for write");
+    const bool wrote = std::fwrite(bytes.data(), 1, bytes.size(), f) == bytes.size();
+    const bool closed = std::fclose(f) == 0;
+    return wrote && closed ? ASTRO_OK : fail(w, ASTRO_FILE_IO_ERROR, "snapshot: write failed");
+}
+
+int32_t astro_snapshot_from_bytes(const astro_world_desc* base, const void* data, size_t n, astro_world** out) {
+    if (!data || !out) return ASTRO_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (n < sizeof(SnapHeader)) return ASTRO_FILE_IO_ERROR;
+    SnapHeader h;
+    std::memcpy(&h, data, sizeof(h));
+    if (h.magic != SNAP_MAGIC) return ASTRO_INVALID_ARGUMENT;
+    if (h.version != SNAP_VERSION) return ASTRO_CONTRACT_VERSION_MISMATCH;
+    const char* body = static_cast<const char*>(data) + sizeof(h);
+    const size_t body_len = n - sizeof(h);
+    if (fnv1a64(body, body_len, FNV_OFFSET) != h.state_hash) return ASTRO_INVALID_ARGUMENT;   // corrupt
+    if (h.taumoeba_count != 0) return ASTRO_UNSUPPORTED;   // no Taumoeba population here
+    if (h.cell_count < 0 || h.cell_count > h.cell_capacity || h.override_count < 0) return ASTRO_INVALID_ARGUMENT;
+    const char* cur = body;
+    const char* end = body + body_len;
+    // The description: the caller's (motion configuration and grids are not
+    // serialized), with the header's chamber, capacity, seed, clock rates and
+    // overrides.
+    astro_world_desc d;
+    if (base) d = *base;
+    else astro_world_desc_default(&d);
+    d.capacity = h.cell_capacity;
+    d.chamber_w = h.chamber_w;
+    d.chamber_h = h.chamber_h;
+    d.chamber_d = h.chamber_d;
+    d.seed = h.seed;
+    d.physics_rate = h.physics_rate;
+    d.biology_rate = h.biology_rate;
+    for (int32_t i = 0; i < h.override_count; ++i) {
+        if (static_cast<size_t>(end - cur) < sizeof(SnapOverride)) return ASTRO_INVALID_ARGUMENT;
+        SnapOverride o;
+        std::memcpy(&o, cur, sizeof(o));
+        cur += sizeof(o);
+        if (std::strncmp(o.key, "PETROVA_MAX_POWER", sizeof(o.key)) == 0) d.petrova_max_power = o.value;
+        else if (std::strncmp(o.key, "CO2_MASS_PER_DIVISION", sizeof(o.key)) == 0) d.co2_mass_per_division = o.value;
+    }
+    astro_world* w = nullptr;
+    int32_t st = astro_world_create(&d, &w);
+    if (st != ASTRO_OK) return st;
+    // a malformed body leaves no world behind
+    auto bail = [&](int32_t code) {
+        astro_world_destroy(w);
+        return code;
+    };
+    const int32_t want[3] = {h.n_temp, h.n_co2, h.n_n2};
+    const int fields[3] = {ASTRO_FIELD_TEMPERATURE, ASTRO_FIELD_CO2, ASTRO_FIELD_N2};
+    for (int k = 0; k < 3; ++k)
+        if (snap_grid_n(w->f[fields[k]]) != want[k]) return bail(ASTRO_INVALID_ARGUMENT);
+    const int32_t count = h.cell_count;
+    if (count > 0) {
+        HostSoa soa(count);
+        bool ok = true;
+        soa.each([&](auto& a) {
+            const size_t bytes = sizeof(a[0]) * a.size();
+            if (!ok || static_cast<size_t>(end - cur) < bytes) {
+                ok = false;
+                return;
+            }
+            std::memcpy(a.data(), cur, bytes);
+            cur += bytes;
+        });
+        if (!ok) return bail(ASTRO_INVALID_ARGUMENT);
+        astro_cell_soa v = soa.view();
+        if ((st = astro_cells_upload(w, &v, count)) != ASTRO_OK) return bail(st);
+    }
+    for (int k = 0; k < 3; ++k) {
+        const FieldHost& g = w->f[fields[k]];
+        if (!g.on) continue;
+        const size_t bytes = sizeof(float) * g.count();
+        if (static_cast<size_t>(end - cur) < bytes) return bail(ASTRO_INVALID_ARGUMENT);
+        std::vector<float> v(g.count());
+        std::memcpy(v.data(), cur, bytes);
+        cur += bytes;
+        if ((st = astro_field_upload(w, fields[k], v.data(), v.size())) != ASTRO_OK) return bail(st);
+    }
+    // bytes past the fields are left unread, as by the reference's loader
+    astro_counters c{};
+    if ((st = astro_world_get_counters(w, &c)) != ASTRO_OK) return bail(st);
+    c.next_id = h.next_cell_id;
+    c.tick = h.tick;
+    c.sim_time_s = h.sim_time_s;
+    if ((st = astro_world_set_counters(w, &c)) != ASTRO_OK) return bail(st);
+    w->non_canon = h.non_canon_run != 0;
+    *out = w;
+    return ASTRO_OK;
+}
+
+int32_t astro_snapshot_load(const astro_world_desc* base, const char* path, astro_world** out) {
+    if (!path || !out) return ASTRO_INVALID_ARGUMENT;
+    *out = nullptr;
+    std::FILE* f = std::fopen(path, "rb");
+    if (!f) return ASTRO_FILE_IO_ERROR;
+    std::vector<char> all;
+    char buf[65536];
+    size_t r = 0;
+    while ((r = std::fread(buf, 1, sizeof(buf), f)) > 0) all.insert(all.end(), buf, buf + r);
+    const bool err = std::ferror(f) != 0;
+    std::fclose(f);
+    if (err) return ASTRO_FILE_IO_ERROR;
+    return astro_snapshot_from_bytes(base, all.data(), all.size(), out);
+}
 
 // ---- ensembles ----------------------------------------------------------------
 int32_t astro_ensemble_create(astro_world* const* chambers, int32_t n, astro_ensemble** out) {

@@ -210,6 +210,34 @@ int32_t astro_world_state_hash(astro_world* w, uint64_t* out);
 /* world_set_clock with explicit rates (src/sim/step.cu:190-215). */
 int32_t astro_world_set_clock(astro_world* w, double physics_rate, double biology_rate);
 
+/* ---- ASPH snapshots (contracts/snapshot_v1.h, src/sim/snapshot.cu) ---------- *
+ * The file is the contract's: a 256-byte header, then the parameter
+ * overrides that differ from canon, the cell SoA [0, count) in contract
+ * order, no Taumoeba arrays, and the T, CO2, N2 values; the header's
+ * state_hash is FNV-1a 64 of that body (== astro_world_state_hash), started
+ * from the contract's 1469598103934665603 (contracts/snapshot_v1.h:59). Header
+ * field sizes follow the reference for its square 2D grids (n); a 3D grid
+ * is written as minus its voxel count, an absent field as 0. */
+/* snapshot_to_bytes (src/sim/snapshot.cu:131-160). `buf` NULL: only *n_out
+ * (the size) is set; a `cap` below the size is ASTRO_CAPACITY_EXCEEDED. */
+int32_t astro_snapshot_to_bytes(astro_world* w, const char* scenario_id, void* buf, size_t cap,
+                                size_t* n_out);
+/* snapshot_save (src/sim/snapshot.cu:162-170); ASTRO_FILE_IO_ERROR when the
+ * file cannot be written. */
+int32_t astro_snapshot_save(astro_world* w, const char* path, const char* scenario_id);
+/* snapshot_from_bytes (src/sim/snapshot.cu:172-253): creates *out. The
+ * motion configuration, grids and light are not in the file: they come from
+ * `base` (NULL: the defaults), with the header's chamber, capacity, seed,
+ * clock rates and overrides applied. Errors as the reference: shorter than a
+ * header ASTRO_FILE_IO_ERROR; bad magic, hash mismatch (corrupt), field
+ * resolution mismatch or truncated body ASTRO_INVALID_ARGUMENT; another
+ * version ASTRO_CONTRACT_VERSION_MISMATCH; a Taumoeba population
+ * ASTRO_UNSUPPORTED. */
+int32_t astro_snapshot_from_bytes(const astro_world_desc* base, const void* data, size_t n,
+                                  astro_world** out);
+/* snapshot_load (src/sim/snapshot.cu:255-264). */
+int32_t astro_snapshot_load(const astro_world_desc* base, const char* path, astro_world** out);
+
 /* ---- stage entry points for kernel-level parity ---------------------------- */
 /* hash_build (src/sim/hash.cu:106-133) on the current population. */
 int32_t astro_stage_hash_build(astro_world* w);

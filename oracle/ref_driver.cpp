@@ -5,6 +5,7 @@
 // never linked by the product. Only headers whose kernels-free half compiles
 // with g++ are used (ASTRO_HD expands to nothing outside nvcc,
 // src/core/units.h:12-19).
+#include <cstddef>
 #include <cstdint>
 #include <cstring>
 
@@ -109,6 +110,28 @@ double ref_taxis_emit_power(int32_t s, double e, double dt, double mx) {
 int64_t ref_to_fixed(double v, double s) { return to_fixed(v, s); }
 double ref_from_fixed(uint64_t raw, double s) { return from_fixed(raw, s); }
 uint64_t ref_fnv1a64(const void* d, size_t n, uint64_t h) { return contract::fnv1a64(d, n, h); }
+
+// The ASPH layout (contracts/snapshot_v1.h:25-55): header and override sizes,
+// the offset of every header field in declaration order, the magic, the
+// version and the hash's starting value (its hash of no bytes). Returns how
+// many values there are; writes at most `cap`.
+int32_t ref_snapshot_layout(int64_t* out, int32_t cap) {
+    using H = contract::SnapshotHeader;
+    const int64_t v[] = {
+        sizeof(H), sizeof(contract::ParamOverride),
+        offsetof(H, magic), offsetof(H, version), offsetof(H, seed), offsetof(H, tick), offsetof(H, sim_time_s),
+        offsetof(H, cell_count), offsetof(H, cell_capacity), offsetof(H, taumoeba_count),
+        offsetof(H, taumoeba_capacity), offsetof(H, n_temp), offsetof(H, n_co2), offsetof(H, n_n2),
+        offsetof(H, chamber_w), offsetof(H, chamber_h), offsetof(H, chamber_d), offsetof(H, physics_rate),
+        offsetof(H, biology_rate), offsetof(H, next_cell_id), offsetof(H, override_count),
+        offsetof(H, non_canon_run), offsetof(H, scenario_id), offsetof(H, build_describe),
+        offsetof(H, state_hash), contract::SNAPSHOT_MAGIC, contract::SNAPSHOT_CONTRACT_VERSION,
+        static_cast<int64_t>(contract::fnv1a64(nullptr, 0)),
+    };
+    const int32_t n = static_cast<int32_t>(sizeof(v) / sizeof(v[0]));
+    for (int32_t i = 0; i < n && i < cap; ++i) out[i] = v[i];
+    return n;
+}
 int32_t ref_substeps_for_dt(double dx, double D, double dt) { return fields::substeps_for_dt(dx, D, dt); }
 
 // grid_sample / grid_index_nearest on a host array (src/fields/grid.cuh:109-141).

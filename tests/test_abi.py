@@ -36,12 +36,13 @@ def test_hip_library_exports_every_symbol():
 
 def test_oracle_mirrors_the_abi(oracle):
     lib = C.CDLL(str(ORACLE))
-    # Timing hooks, ensembles and the surrogate have no C counterpart: an
-    # ensemble's chambers are checked against standalone GPU worlds, the
-    # surrogate against oracle/surrogate_oracle.py.
+    # Timing hooks, ensembles, snapshots and the surrogate have no C
+    # counterpart: an ensemble's chambers are checked against standalone GPU
+    # worlds, snapshots with the contract reader tests/asph.py, the surrogate
+    # against oracle/surrogate_oracle.py.
     product_only = {"astro_profile_enable", "astro_profile_read", "astro_profile_reset", "astro_world_stream"}
     missing = [s for s in declared_symbols()
-               if s not in product_only and not s.startswith(("astro_ensemble_", "astro_surrogate_"))
+               if s not in product_only and not s.startswith(("astro_ensemble_", "astro_surrogate_", "astro_snapshot_"))
                and not hasattr(lib, "oracle_" + s[len("astro_"):])]
     assert not missing, missing
 
