@@ -33,6 +33,8 @@ OK, OUT_OF_MEMORY, DEVICE_ERROR, INVALID_ARGUMENT, NOT_FOUND, PARSE_ERROR, \
 FP64_CONTRACT, FP32_HOT = 0, 1
 BOUNDARY_REFLECTING, BOUNDARY_PERIODIC, BOUNDARY_ABSORBING = 0, 1, 2
 GRAVITY_Y, GRAVITY_Z = 0, 1
+CLOCK_REALTIME, CLOCK_MOTION, CLOCK_METABOLIC, CLOCK_GENERATIONAL, CLOCK_CUSTOM = range(5)
+BRUSH_HEAT, BRUSH_CHILL, BRUSH_INJECT_CO2, BRUSH_INJECT_N2 = range(4)
 STORE_VOID, STORE_FLASH, STORE_RETAIN = 0, 1, 2
 TEMPERATURE, CO2, N2, IRRADIANCE = 0, 1, 2, 3
 PLACE_UNIFORM, PLACE_GAUSSIAN, PLACE_GRID, PLACE_DISC = 0, 1, 2, 3
@@ -127,6 +129,74 @@ assert C.sizeof(SpawnParams) == 56
 assert C.sizeof(Counters) == 64
 assert C.sizeof(Stats) == 144
 
+
+# ---- scenarios (include/astro_scenario.h) -----------------------------------
+ORGANISM_ASTROPHAGE, ORGANISM_TAUMOEBA = 0, 1
+STIM_HEAT, STIM_CHILL, STIM_INJECT_CO2, STIM_INJECT_N2, STIM_SET_LIGHT, STIM_SET_N2, STIM_SEED_CELLS, \
+    STIM_FLASH = range(8)
+METRICS = ["awake_fraction", "medium_temp_mean", "medium_temp_max", "boil_event_count", "population_live",
+           "population_dead", "mean_charge", "total_energy_j", "mean_tau_tolerance", "max_tau_tolerance",
+           "charge_depth_correlation", "charge_height_correlation", "rise_velocity_empty",
+           "fall_velocity_full", "doubling_time_s", "impulse_per_cycle"]
+OPS = ["==", "!=", "<", "<=", ">", ">=", "~="]
+
+
+class LightSource(C.Structure):
+    _fields_ = [("dir_x", C.c_float), ("dir_y", C.c_float), ("irradiance", C.c_float),
+                ("wavelength", C.c_float), ("enabled", C.c_int32)]
+
+
+class PopulationSpec(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("count", C.c_int32), ("placement", C.c_int32), ("charge_dist", C.c_int32),
+                ("place_x", C.c_double), ("place_y", C.c_double), ("place_radius", C.c_double),
+                ("charge_a", C.c_double), ("charge_b", C.c_double), ("awake", C.c_int32), ("_pad", C.c_int32)]
+
+
+class Stimulus(C.Structure):
+    _fields_ = [("t0_s", C.c_double), ("t1_s", C.c_double), ("x", C.c_double), ("y", C.c_double),
+                ("radius", C.c_double), ("v0", C.c_double), ("v1", C.c_double),
+                ("kind", C.c_int32), ("_pad", C.c_int32)]
+
+
+class AcceptCheck(C.Structure):
+    _fields_ = [("metric", C.c_int32), ("op", C.c_int32), ("value", C.c_double), ("tol", C.c_double),
+                ("after_s", C.c_double), ("tol_absolute", C.c_int32), ("_pad", C.c_int32)]
+
+
+class ParamOverride(C.Structure):
+    _fields_ = [("key", C.c_char * 48), ("value", C.c_double)]
+
+
+class Scenario(C.Structure):
+    _fields_ = [
+        ("schema", C.c_int32), ("id", C.c_char * 64), ("title", C.c_char * 128), ("blurb", C.c_char * 256),
+        ("seed", C.c_uint64), ("chamber_w", C.c_double), ("chamber_h", C.c_double), ("chamber_d", C.c_double),
+        ("boundary_x", C.c_int32), ("boundary_y", C.c_int32),
+        ("temp_init", C.c_double), ("co2_init", C.c_double), ("n2_init", C.c_double), ("ambient_temp", C.c_double),
+        ("thermal_bc", C.c_int32), ("density_model", C.c_int32), ("store_disposition", C.c_int32),
+        ("gravity_axis", C.c_int32), ("thermal_active", C.c_int32), ("compaction", C.c_int32),
+        ("tau_compaction", C.c_int32), ("ambient_irradiance", C.c_float),
+        ("lights", LightSource * 8), ("populations", PopulationSpec * 8), ("population_count", C.c_int32),
+        ("clock", C.c_int32), ("physics_rate", C.c_double), ("biology_rate", C.c_double),
+        ("tools", C.c_uint32), ("override_count", C.c_int32), ("param_overrides", ParamOverride * 32),
+        ("drive", Stimulus * 16), ("drive_count", C.c_int32), ("objective_text", C.c_char * 256),
+        ("accept_count", C.c_int32), ("accept", AcceptCheck * 16), ("run_duration_s", C.c_double),
+    ]
+
+
+class AcceptResult(C.Structure):
+    _fields_ = [("metric", C.c_int32), ("op", C.c_int32), ("measured", C.c_double), ("value", C.c_double),
+                ("tol", C.c_double), ("after_s", C.c_double), ("tol_absolute", C.c_int32), ("pass_", C.c_int32)]
+
+
+class AssertSummary(C.Structure):
+    _fields_ = [("ticks", C.c_int64), ("sim_time_s", C.c_double), ("checks", C.c_int32), ("passed", C.c_int32),
+                ("all_pass", C.c_int32), ("samples", C.c_int32)]
+
+
+assert C.sizeof(Scenario) == 4992 and Scenario.accept.offset == 4344 and Scenario.drive.offset == 3056
+assert C.sizeof(AcceptResult) == 48 and C.sizeof(AssertSummary) == 32
+
 # Every entry point of include/astro_hot.h: name -> (restype, argtypes).
 _P = C.c_void_p
 _I = C.c_int32
@@ -152,6 +222,11 @@ SIGNATURES = {
     "world_stats": (_I, [_P, C.POINTER(Stats)]),
     "world_state_hash": (_I, [_P, C.POINTER(C.c_uint64)]),
     "world_set_clock": (_I, [_P, _D, _D]),
+    "world_set_clock_preset": (_I, [_P, _I, _D, _D]),
+    "world_set_light": (_I, [_P, C.c_float, C.c_float, C.c_float, C.c_float, _I]),
+    "world_apply_brush": (_I, [_P, _I, _D, _D, _D, _D]),
+    "field_fill": (_I, [_P, _I, C.c_float]),
+    "world_time": (_I, [_P, C.POINTER(C.c_uint64), C.POINTER(_D), C.POINTER(_D), C.POINTER(_D)]),
     "stage_hash_build": (_I, [_P]),
     "hash_dims": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_D)]),
     "hash_download": (_I, [_P, _P, _I, _P, _P, _I]),
@@ -202,6 +277,15 @@ PRODUCT_ONLY = {
     "surrogate_weights": (_I, [_P, _I, _P, C.c_size_t, _I]),
     "surrogate_eval": (_I, [_P, _P, C.c_size_t, _P, _P, _P, C.c_size_t]),
     "surrogate_stream": (_P, [_P]),
+    "scenario_parse": (_I, [C.c_char_p, C.c_size_t, C.POINTER(Scenario)]),
+    "scenario_load": (_I, [C.c_char_p, C.POINTER(Scenario)]),
+    "scenario_capacities": (None, [C.POINTER(Scenario), C.POINTER(_I), C.POINTER(_I)]),
+    "scenario_instantiate": (_I, [C.POINTER(Scenario), C.POINTER(WorldDesc), C.POINTER(_P)]),
+    "scenario_apply_drive": (_I, [_P, C.POINTER(Scenario)]),
+    "scenario_run_assert": (_I, [C.POINTER(Scenario), _P, C.c_int64, C.c_char_p, C.POINTER(AcceptResult), _I,
+                                 C.POINTER(AssertSummary)]),
+    "accept_eval": (_I, [C.POINTER(AcceptCheck), _D]),
+    "metric_name": (C.c_char_p, [_I]),
 }
 
 
@@ -270,9 +354,13 @@ class World:
             self._check(binding.world_create(C.byref(desc), C.byref(h)), what="world_create")
             _handle = h
         self.h = _handle
+
+    @property
+    def desc(self) -> WorldDesc:
+        """The world's current description (clock and light included)."""
         d = WorldDesc()
-        self._check(binding.world_desc_get(self.h, C.byref(d)))
-        self.desc = d
+        self._check(self.b.world_desc_get(self.h, C.byref(d)))
+        return d
 
     # -- ASPH snapshots (src/sim/snapshot.h:17-40) ---------------------------
     @classmethod
@@ -414,6 +502,24 @@ class World:
 
     def set_clock(self, physics: float, biology: float):
         self._check(self.b.world_set_clock(self.h, physics, biology))
+
+    def set_clock_preset(self, preset: int, physics: float = 1.0, biology: float = 1.0):
+        self._check(self.b.world_set_clock_preset(self.h, preset, physics, biology))
+
+    def set_light(self, dir_x: float, dir_y: float, irradiance: float, ambient: float, enabled: bool):
+        self._check(self.b.world_set_light(self.h, dir_x, dir_y, irradiance, ambient, 1 if enabled else 0))
+
+    def brush(self, kind: int, x: float, y: float, radius: float, strength: float):
+        self._check(self.b.world_apply_brush(self.h, kind, x, y, radius, strength), "brush")
+
+    def fill(self, f: int, value: float):
+        self._check(self.b.field_fill(self.h, f, value), "fill")
+
+    def time(self) -> tuple[int, float, float, float]:
+        """(tick, sim_time_s, physics_rate, biology_rate) without a device wait."""
+        t, s, p, b = C.c_uint64(), C.c_double(), C.c_double(), C.c_double()
+        self._check(self.b.world_time(self.h, C.byref(t), C.byref(s), C.byref(p), C.byref(b)))
+        return t.value, s.value, p.value, b.value
 
     @property
     def dt(self) -> float:
@@ -688,3 +794,55 @@ class Surrogate:
             out.append(g[off:off + sz])
             off += sz
         return pred, out
+
+
+# ---- scenarios ----------------------------------------------------------------
+def scenario_parse(binding: Binding, text: str | bytes) -> Scenario:
+    """scenario_parse: raises AstroError on a malformed scenario."""
+    data = text.encode() if isinstance(text, str) else bytes(text)
+    sc = Scenario()
+    st = binding.scenario_parse(data, len(data), C.byref(sc))
+    if st != OK:
+        raise AstroError(st, "scenario_parse")
+    return sc
+
+
+def scenario_load(binding: Binding, path) -> Scenario:
+    sc = Scenario()
+    st = binding.scenario_load(str(path).encode(), C.byref(sc))
+    if st != OK:
+        raise AstroError(st, f"scenario_load {path}")
+    return sc
+
+
+def scenario_world(binding: Binding, sc: Scenario, desc: WorldDesc | None = None, **kw) -> World:
+    """scenario_instantiate: a new world; grids, precision and the hash cell
+    come from `desc` or `kw` (as for World())."""
+    if desc is None:
+        desc = binding.desc(**kw)
+    h = C.c_void_p()
+    st = binding.scenario_instantiate(C.byref(sc), C.byref(desc), C.byref(h))
+    if st != OK:
+        raise AstroError(st, "scenario_instantiate")
+    return World(binding, _handle=h)
+
+
+def scenario_drive(binding: Binding, w: World, sc: Scenario):
+    """scenario_apply_drive for the world's current tick."""
+    w._check(binding.scenario_apply_drive(w.h, C.byref(sc)), "scenario_apply_drive")
+
+
+def scenario_assert(binding: Binding, sc: Scenario, w: World, ticks: int = 0, csv_path=None) -> dict:
+    """run_scenario_assert: {'all_pass', 'ticks', 'sim_time_s', 'samples',
+    'checks': [{'metric', 'op', 'measured', 'value', 'tol', 'pass'}]}."""
+    res = (AcceptResult * 16)()
+    summ = AssertSummary()
+    csv = str(csv_path).encode() if csv_path is not None else None
+    w._check(binding.scenario_run_assert(C.byref(sc), w.h, ticks, csv, res, 16, C.byref(summ)), "run_assert")
+    checks = []
+    for i in range(min(summ.checks, 16)):
+        r = res[i]
+        checks.append(dict(metric=METRICS[r.metric], op=OPS[r.op], measured=r.measured, value=r.value, tol=r.tol,
+                           tol_absolute=bool(r.tol_absolute), after_s=r.after_s, passed=bool(r.pass_)))
+    return dict(all_pass=bool(summ.all_pass), ticks=summ.ticks, sim_time_s=summ.sim_time_s,
+                samples=summ.samples, checks=checks)

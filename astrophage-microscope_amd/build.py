@@ -2,8 +2,9 @@
 
     python astrophage-microscope_amd/build.py [--jobs N]
 
-Each .hip translation unit is compiled by hipcc to an object in parallel, then
-linked into astrophage-microscope_amd/lib/libastro_hot.so. Flags:
+Each .hip translation unit is compiled by hipcc to an object in parallel (the
+host-only scenario layer by g++), then linked into
+astrophage-microscope_amd/lib/libastro_hot.so. Flags:
   * --offload-arch=gfx950: MI355X only;
   * -ffp-contract=off: expressions round as written, so results compare with
     the CPU restatement of the reference bit for bit wherever no libm call
@@ -25,6 +26,11 @@ CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "lib" / "libastro_hot.so"
 SOURCES = ["world.hip", "kernels_hash.hip", "kernels_cells.hip", "kernels_fields.hip", "surrogate.hip"]
+# Host-only C++ above the C ABI (no device code), compiled by g++.
+HOST_SOURCES = ["scenario.cpp"]
+HOST_CXX = os.environ.get("CXX_HOST", "g++")
+HOST_FLAGS = ["-O2", "-std=c++20", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra",
+              "-I", str(ROOT / "include"), "-I", str(CSRC)]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -43,7 +49,10 @@ FLAGS = [
 
 def _compile(src: str) -> tuple[str, int, str]:
     obj = BUILD / (Path(src).stem + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", str(CSRC / src), "-o", str(obj)]
+    if src in HOST_SOURCES:
+        cmd = [HOST_CXX, *HOST_FLAGS, "-c", str(CSRC / src), "-o", str(obj)]
+    else:
+        cmd = [HIPCC, *FLAGS, "-c", str(CSRC / src), "-o", str(obj)]
     p = subprocess.run(cmd, capture_output=True, text=True)
     return src, p.returncode, p.stdout + p.stderr
 
@@ -52,7 +61,7 @@ def build(jobs: int = 4, verbose: bool = False) -> Path:
     BUILD.mkdir(exist_ok=True)
     LIB.parent.mkdir(exist_ok=True)
     with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
-        results = list(ex.map(_compile, SOURCES))
+        results = list(ex.map(_compile, SOURCES + HOST_SOURCES))
     failed = False
     for src, rc, out in results:
         noisy = [ln for ln in out.splitlines() if "packed-fp32-ops" not in ln and ln.strip()]
@@ -61,7 +70,7 @@ def build(jobs: int = 4, verbose: bool = False) -> Path:
         failed |= rc != 0
     if failed:
         raise RuntimeError("hipcc failed; see messages above")
-    objs = [str(BUILD / (Path(s).stem + ".o")) for s in SOURCES]
+    objs = [str(BUILD / (Path(s).stem + ".o")) for s in SOURCES + HOST_SOURCES]
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", str(tmp)]
     p = subprocess.run(cmd, capture_output=True, text=True)

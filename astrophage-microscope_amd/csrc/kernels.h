@@ -197,6 +197,7 @@ void launch_ftcs(const Launch& L, std::span<const Grid> gs, std::span<const Ftcs
 template <class P> void launch_stamp(const Launch& L, Views<P> ws);
 void launch_sweep(const Launch& L, std::span<const Grid> irr, Ticks ts);
 void launch_fill(const Launch& L, float* v, size_t n, float value);
+void launch_brush(const Launch& L, const Grid& g, float cx, float cy, float radius, float amount);
 void launch_stats_fields(const Launch& L, const Grid& T, const Grid& C, const Grid& N, bool c_on,
                          bool n_on, unsigned long long* stats);
 

@@ -347,6 +347,31 @@ __global__ __launch_bounds__(BLOCK) void k_fill(float* v, size_t n, float value)
     if (i < n) v[i] = value;
 }
 
+// grid_brush's kernel (src/fields/diffuse.cu:62-77): value += amount * (1 - t^2)^2
+// with t = r / radius inside the disc around (cx, cy), r measured from the
+// voxel centre in the plane; a 3D grid takes the same amount at every depth.
+// Float arithmetic in the reference's order. sqrtf and the divide are
+// correctly rounded (hipcc's default -fhip-fp32-correctly-rounded-divide-sqrt,
+// as CUDA's defaults); __fsqrt_rn would not be: without OCML's rounded
+// operations it is the native approximation.
+__global__ __launch_bounds__(BLOCK) void k_brush(float* v, int nx, int ny, size_t n, float dx, float dy, float cx,
+                                                 float cy, float radius, float amount) {
+    const float halfx = 0.5f * static_cast<float>(nx) * dx;
+    const float halfy = 0.5f * static_cast<float>(ny) * dy;
+    const size_t stride = static_cast<size_t>(gridDim.x) * BLOCK;
+    for (size_t idx = static_cast<size_t>(blockIdx.x) * BLOCK + threadIdx.x; idx < n; idx += stride) {
+        const int i = static_cast<int>(idx % static_cast<size_t>(nx));
+        const int j = static_cast<int>((idx / static_cast<size_t>(nx)) % static_cast<size_t>(ny));
+        const float x = (static_cast<float>(i) + 0.5f) * dx - halfx;
+        const float y = (static_cast<float>(j) + 0.5f) * dy - halfy;
+        const float r = sqrtf((x - cx) * (x - cx) + (y - cy) * (y - cy));
+        if (r > radius) continue;
+        const float t = r / radius;
+        const float w = (1.0f - t * t) * (1.0f - t * t);
+        v[idx] = v[idx] + amount * w;
+    }
+}
+
 __device__ __forceinline__ unsigned long long wsum(unsigned long long v) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
@@ -466,6 +491,14 @@ void launch_sweep(const Launch& L, std::span<const Grid> irr, Ticks ts) {
 void launch_fill(const Launch& L, float* v, size_t n, float value) {
     if (n == 0) return;
     k_fill<<<static_cast<unsigned>((n + BLOCK - 1) / BLOCK), BLOCK, 0, L.stream>>>(v, n, value);
+}
+
+void launch_brush(const Launch& L, const Grid& g, float cx, float cy, float radius, float amount) {
+    const size_t n = static_cast<size_t>(g.nx) * g.ny * g.nz;
+    if (n == 0) return;
+    const size_t want = (n + BLOCK - 1) / BLOCK;
+    const unsigned blocks = static_cast<unsigned>(want < 4096 ? want : 4096);
+    k_brush<<<blocks, BLOCK, 0, L.stream>>>(g.value, g.nx, g.ny, n, g.fdx, g.fdy, cx, cy, radius, amount);
 }
 
 void launch_stats_fields(const Launch& L, const Grid& T, const Grid& C, const Grid& N, bool c_on, bool n_on,

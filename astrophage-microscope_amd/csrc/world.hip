@@ -1297,6 +1297,93 @@ int32_t astro_world_sync(astro_world* w) {
     return check_device_errors(w);
 }
 
+// world_set_clock with a preset (src/sim/step.cu:190-215): the preset's rates,
+// or the custom ones, clamped.
+int32_t astro_world_set_clock_preset(astro_world* w, int32_t preset, double custom_physics, double custom_biology) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    double physics = custom_physics, biology = custom_biology;
+    switch (preset) {
+        case ASTRO_CLOCK_MOTION: physics = ASTRO_CLOCK_MOTION_PHYSICS; biology = ASTRO_CLOCK_MOTION_BIOLOGY; break;
+        case ASTRO_CLOCK_METABOLIC:
+            physics = ASTRO_CLOCK_METABOLIC_PHYSICS;
+            biology = ASTRO_CLOCK_METABOLIC_BIOLOGY;
+            break;
+        case ASTRO_CLOCK_GENERATIONAL:
+            physics = ASTRO_CLOCK_GENERATIONAL_PHYSICS;
+            biology = ASTRO_CLOCK_GENERATIONAL_BIOLOGY;
+            break;
+        case ASTRO_CLOCK_CUSTOM: break;
+        default:   // realtime, and any unknown preset
+            physics = ASTRO_CLOCK_REALTIME_PHYSICS;
+            biology = ASTRO_CLOCK_REALTIME_BIOLOGY;
+            break;
+    }
+    return astro_world_set_clock(w, physics, biology);
+}
+
+// World::light and World::ambient_irradiance (src/sim/world.cuh). The captured
+// tick is dropped only when something changed, so a scripted light level that
+// holds steady costs nothing per tick.
+int32_t astro_world_set_light(astro_world* w, float dir_x, float dir_y, float irradiance, float ambient,
+                              int32_t enabled) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    astro_world_desc& d = w->desc;
+    const uint8_t on = enabled ? 1 : 0;
+    if (d.light_dir_x == dir_x && d.light_dir_y == dir_y && d.light_irradiance == irradiance &&
+        d.ambient_irradiance == ambient && d.light_enabled == on)
+        return ASTRO_OK;
+    d.light_dir_x = dir_x;
+    d.light_dir_y = dir_y;
+    d.light_irradiance = irradiance;
+    d.ambient_irradiance = ambient;
+    d.light_enabled = on;
+    drop_graph(w);
+    return ASTRO_OK;
+}
+
+// world_apply_brush (src/sim/step.cu:80-89) over grid_brush
+// (src/fields/diffuse.cu:81-89); enqueued on the world's stream.
+int32_t astro_world_apply_brush(astro_world* w, int32_t kind, double x, double y, double radius, double strength) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    int field = -1;
+    double amount = strength;
+    switch (kind) {
+        case ASTRO_BRUSH_HEAT: field = ASTRO_FIELD_TEMPERATURE; break;
+        case ASTRO_BRUSH_CHILL: field = ASTRO_FIELD_TEMPERATURE; amount = -strength; break;
+        case ASTRO_BRUSH_INJECT_CO2: field = ASTRO_FIELD_CO2; break;
+        case ASTRO_BRUSH_INJECT_N2: field = ASTRO_FIELD_N2; break;
+        default: return fail(w, ASTRO_INVALID_ARGUMENT, "unknown brush");
+    }
+    const FieldHost& g = w->f[field];
+    if (!g.on) return fail(w, ASTRO_INVALID_ARGUMENT, "brush on a disabled field");
+    if (radius <= 0.0) return ASTRO_OK;
+    launch_brush(Launch{w->stream, nullptr}, g.grid(), static_cast<float>(x), static_cast<float>(y),
+                 static_cast<float>(radius), static_cast<float>(amount));
+    return hip_check(w, hipGetLastError(), "brush");
+}
+
+// grid_fill (src/fields/diffuse.cu:128-134): value and scratch.
+int32_t astro_field_fill(astro_world* w, int32_t field, float value) {
+    if (!w || field < 0 || field >= ASTRO_FIELD_COUNT) return ASTRO_INVALID_ARGUMENT;
+    const FieldHost& g = w->f[field];
+    if (!g.on) return fail(w, ASTRO_INVALID_ARGUMENT, "fill of a disabled field");
+    const Launch L{w->stream, nullptr};
+    launch_fill(L, g.value, g.count(), value);
+    launch_fill(L, g.scratch, g.count(), value);
+    return hip_check(w, hipGetLastError(), "fill");
+}
+
+// The host-side clock, without waiting for the device.
+int32_t astro_world_time(const astro_world* w, uint64_t* tick, double* sim_time_s, double* physics_rate,
+                         double* biology_rate) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    if (tick) *tick = w->tick;
+    if (sim_time_s) *sim_time_s = w->sim_time;
+    if (physics_rate) *physics_rate = w->physics_rate;
+    if (biology_rate) *biology_rate = w->biology_rate;
+    return ASTRO_OK;
+}
+
 int32_t astro_world_set_clock(astro_world* w, double physics, double biology) {
     if (!w) return ASTRO_INVALID_ARGUMENT;
     w->physics_rate = physics < ASTRO_CLOCK_PHYSICS_MIN ? ASTRO_CLOCK_PHYSICS_MIN

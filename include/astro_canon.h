@@ -63,6 +63,14 @@
 #define ASTRO_CLOCK_PHYSICS_MAX    100.0
 #define ASTRO_CLOCK_BIOLOGY_MIN    1.0
 #define ASTRO_CLOCK_BIOLOGY_MAX    1000000.0
+#define ASTRO_CLOCK_REALTIME_PHYSICS     1.0
+#define ASTRO_CLOCK_REALTIME_BIOLOGY     1.0
+#define ASTRO_CLOCK_MOTION_PHYSICS       10.0
+#define ASTRO_CLOCK_MOTION_BIOLOGY       1.0
+#define ASTRO_CLOCK_METABOLIC_PHYSICS    1.0
+#define ASTRO_CLOCK_METABOLIC_BIOLOGY    10000.0
+#define ASTRO_CLOCK_GENERATIONAL_PHYSICS 0.5
+#define ASTRO_CLOCK_GENERATIONAL_BIOLOGY 1000000.0
 
 /* ---- taxis ------------------------------------------------------------------ */
 #define ASTRO_TAXIS_MEMORY_TIME     0.10000000000000001
@@ -82,6 +90,8 @@
 #define ASTRO_FIELD_N_IRRAD        512
 #define ASTRO_MAX_CELLS            2000000
 #define ASTRO_DEFAULT_CELLS        25000
+#define ASTRO_MAX_TAUMOEBA         65536
+#define ASTRO_DEFAULT_TAUMOEBA     256
 
 /* ---- fixed-point deposit scales (contracts/fields_v1.h:135-145) ------------ */
 #define ASTRO_DEPOSIT_SCALE_TEMPERATURE 1.0e9

@@ -210,6 +210,33 @@ int32_t astro_world_state_hash(astro_world* w, uint64_t* out);
 /* world_set_clock with explicit rates (src/sim/step.cu:190-215). */
 int32_t astro_world_set_clock(astro_world* w, double physics_rate, double biology_rate);
 
+/* ---- stimuli a scenario applies between ticks ------------------------------ */
+/* ClockPreset (contracts/scenario_v3.h:37) and BrushKind (src/sim/world.cuh). */
+enum { ASTRO_CLOCK_REALTIME = 0, ASTRO_CLOCK_MOTION = 1, ASTRO_CLOCK_METABOLIC = 2,
+       ASTRO_CLOCK_GENERATIONAL = 3, ASTRO_CLOCK_CUSTOM = 4 };
+enum { ASTRO_BRUSH_HEAT = 0, ASTRO_BRUSH_CHILL = 1, ASTRO_BRUSH_INJECT_CO2 = 2,
+       ASTRO_BRUSH_INJECT_N2 = 3 };
+/* world_set_clock(World&, ClockPreset, custom_physics, custom_biology)
+ * (src/sim/step.cu:190-215): a preset's rates, or the custom ones, clamped. */
+int32_t astro_world_set_clock_preset(astro_world* w, int32_t preset, double custom_physics,
+                                     double custom_biology);
+/* World::light (the primary source) and World::ambient_irradiance
+ * (src/sim/world.cuh, set by src/sim/scenario.cpp:394-395 and :431-434). */
+int32_t astro_world_set_light(astro_world* w, float dir_x, float dir_y, float irradiance,
+                              float ambient, int32_t enabled);
+/* world_apply_brush (src/sim/step.cu:80-89): grid_brush's smooth disc
+ * (src/fields/diffuse.cu:62-89) of `strength` on T (heat; chill subtracts),
+ * CO2 or N2, centred at (x, y); radius <= 0 does nothing. A 3D grid takes the
+ * same amount at every depth. Enqueued on the world's stream. */
+int32_t astro_world_apply_brush(astro_world* w, int32_t kind, double x, double y, double radius,
+                                double strength);
+/* grid_fill (src/fields/diffuse.cu:128-134). */
+int32_t astro_field_fill(astro_world* w, int32_t field, float value);
+/* World::tick, sim_time_s and the clock rates, read on the host without
+ * waiting for the device. */
+int32_t astro_world_time(const astro_world* w, uint64_t* tick, double* sim_time_s,
+                         double* physics_rate, double* biology_rate);
+
 /* ---- ASPH snapshots (contracts/snapshot_v1.h, src/sim/snapshot.cu) ---------- *
  * The file is the contract's: a 256-byte header, then the parameter
  * overrides that differ from canon, the cell SoA [0, count) in contract

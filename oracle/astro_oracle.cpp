@@ -1354,6 +1354,86 @@ int32_t oracle_world_step(oracle_world* w, int32_t n_ticks) {
 
 int32_t oracle_world_sync(oracle_world* w) { return w ? ASTRO_OK : ASTRO_INVALID_ARGUMENT; }
 
+// world_set_clock with a preset (src/sim/step.cu:190-215).
+int32_t oracle_world_set_clock_preset(oracle_world* w, int32_t preset, double physics, double biology) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    if (preset == ASTRO_CLOCK_MOTION) {
+        physics = ASTRO_CLOCK_MOTION_PHYSICS; biology = ASTRO_CLOCK_MOTION_BIOLOGY;
+    } else if (preset == ASTRO_CLOCK_METABOLIC) {
+        physics = ASTRO_CLOCK_METABOLIC_PHYSICS; biology = ASTRO_CLOCK_METABOLIC_BIOLOGY;
+    } else if (preset == ASTRO_CLOCK_GENERATIONAL) {
+        physics = ASTRO_CLOCK_GENERATIONAL_PHYSICS; biology = ASTRO_CLOCK_GENERATIONAL_BIOLOGY;
+    } else if (preset != ASTRO_CLOCK_CUSTOM) {
+        physics = ASTRO_CLOCK_REALTIME_PHYSICS; biology = ASTRO_CLOCK_REALTIME_BIOLOGY;
+    }
+    return oracle_world_set_clock(w, physics, biology);
+}
+
+int32_t oracle_world_set_light(oracle_world* w, float dir_x, float dir_y, float irradiance, float ambient,
+                               int32_t enabled) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    w->desc.light_dir_x = dir_x;
+    w->desc.light_dir_y = dir_y;
+    w->desc.light_irradiance = irradiance;
+    w->desc.ambient_irradiance = ambient;
+    w->desc.light_enabled = enabled ? 1 : 0;
+    return ASTRO_OK;
+}
+
+// world_apply_brush (src/sim/step.cu:80-89) and brush_kernel
+// (src/fields/diffuse.cu:62-89), per voxel column in float.
+int32_t oracle_world_apply_brush(oracle_world* w, int32_t kind, double x, double y, double radius,
+                                 double strength) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    int f = -1;
+    double amount = strength;
+    if (kind == ASTRO_BRUSH_HEAT) f = ASTRO_FIELD_TEMPERATURE;
+    else if (kind == ASTRO_BRUSH_CHILL) { f = ASTRO_FIELD_TEMPERATURE; amount = -strength; }
+    else if (kind == ASTRO_BRUSH_INJECT_CO2) f = ASTRO_FIELD_CO2;
+    else if (kind == ASTRO_BRUSH_INJECT_N2) f = ASTRO_FIELD_N2;
+    else return fail(w, ASTRO_INVALID_ARGUMENT, "unknown brush");
+    Grid& g = w->f[f];
+    if (!g.enabled) return fail(w, ASTRO_INVALID_ARGUMENT, "brush on a disabled field");
+    if (radius <= 0.0) return ASTRO_OK;
+    const float cx = static_cast<float>(x), cy = static_cast<float>(y);
+    const float rad = static_cast<float>(radius), am = static_cast<float>(amount);
+    const float halfx = 0.5f * static_cast<float>(g.nx) * g.fdx;
+    const float halfy = 0.5f * static_cast<float>(g.ny) * g.fdy;
+    for (int k = 0; k < g.nz; ++k)
+        for (int j = 0; j < g.ny; ++j)
+            for (int i = 0; i < g.nx; ++i) {
+                const float px = (static_cast<float>(i) + 0.5f) * g.fdx - halfx;
+                const float py = (static_cast<float>(j) + 0.5f) * g.fdy - halfy;
+                const float r = std::sqrt((px - cx) * (px - cx) + (py - cy) * (py - cy));
+                if (r > rad) continue;
+                const float t = r / rad;
+                const float wgt = (1.0f - t * t) * (1.0f - t * t);
+                float& v = g.value[g.at(i, j, k)];
+                v = v + am * wgt;
+            }
+    return ASTRO_OK;
+}
+
+// grid_fill (src/fields/diffuse.cu:128-134).
+int32_t oracle_field_fill(oracle_world* w, int32_t f, float value) {
+    if (!w || f < 0 || f >= ASTRO_FIELD_COUNT) return ASTRO_INVALID_ARGUMENT;
+    Grid& g = w->f[f];
+    if (!g.enabled) return fail(w, ASTRO_INVALID_ARGUMENT, "fill of a disabled field");
+    std::fill(g.value.begin(), g.value.end(), value);
+    std::fill(g.scratch.begin(), g.scratch.end(), value);
+    return ASTRO_OK;
+}
+
+int32_t oracle_world_time(const oracle_world* w, uint64_t* tick, double* sim_time_s, double* physics_rate,
+                          double* biology_rate) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    if (tick) *tick = w->tick;
+    if (sim_time_s) *sim_time_s = w->sim_time;
+    if (physics_rate) *physics_rate = w->physics_rate;
+    if (biology_rate) *biology_rate = w->biology_rate;
+    return ASTRO_OK;
+}
+
 int32_t oracle_world_set_clock(oracle_world* w, double physics, double biology) {
     if (!w) return ASTRO_INVALID_ARGUMENT;
     w->physics_rate = physics < ASTRO_CLOCK_PHYSICS_MIN ? ASTRO_CLOCK_PHYSICS_MIN

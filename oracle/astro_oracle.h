@@ -44,6 +44,15 @@ int32_t oracle_world_sync(oracle_world* w);
 int32_t oracle_world_stats(oracle_world* w, astro_stats* out);
 int32_t oracle_world_state_hash(oracle_world* w, uint64_t* out);
 int32_t oracle_world_set_clock(oracle_world* w, double physics_rate, double biology_rate);
+int32_t oracle_world_set_clock_preset(oracle_world* w, int32_t preset, double custom_physics,
+                                      double custom_biology);
+int32_t oracle_world_set_light(oracle_world* w, float dir_x, float dir_y, float irradiance, float ambient,
+                               int32_t enabled);
+int32_t oracle_world_apply_brush(oracle_world* w, int32_t kind, double x, double y, double radius,
+                                 double strength);
+int32_t oracle_field_fill(oracle_world* w, int32_t field, float value);
+int32_t oracle_world_time(const oracle_world* w, uint64_t* tick, double* sim_time_s, double* physics_rate,
+                          double* biology_rate);
 
 int32_t oracle_stage_hash_build(oracle_world* w);
 int32_t oracle_hash_dims(const oracle_world* w, int32_t* nx, int32_t* ny, int32_t* nz,

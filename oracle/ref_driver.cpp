@@ -188,6 +188,13 @@ int32_t ref_constant(const char* name, double* out) {
         {"DT_PHYSICS", canon::DT_PHYSICS}, {"CLOCK_PHYSICS_MIN", canon::CLOCK_PHYSICS_MIN},
         {"CLOCK_PHYSICS_MAX", canon::CLOCK_PHYSICS_MAX}, {"CLOCK_BIOLOGY_MIN", canon::CLOCK_BIOLOGY_MIN},
         {"CLOCK_BIOLOGY_MAX", canon::CLOCK_BIOLOGY_MAX}, {"TAXIS_MEMORY_TIME", canon::TAXIS_MEMORY_TIME},
+        {"CLOCK_REALTIME_PHYSICS", canon::CLOCK_REALTIME_PHYSICS},
+        {"CLOCK_REALTIME_BIOLOGY", canon::CLOCK_REALTIME_BIOLOGY},
+        {"CLOCK_MOTION_PHYSICS", canon::CLOCK_MOTION_PHYSICS}, {"CLOCK_MOTION_BIOLOGY", canon::CLOCK_MOTION_BIOLOGY},
+        {"CLOCK_METABOLIC_PHYSICS", canon::CLOCK_METABOLIC_PHYSICS},
+        {"CLOCK_METABOLIC_BIOLOGY", canon::CLOCK_METABOLIC_BIOLOGY},
+        {"CLOCK_GENERATIONAL_PHYSICS", canon::CLOCK_GENERATIONAL_PHYSICS},
+        {"CLOCK_GENERATIONAL_BIOLOGY", canon::CLOCK_GENERATIONAL_BIOLOGY},
         {"TAXIS_DARK_THRESHOLD", canon::TAXIS_DARK_THRESHOLD},
         {"TAXIS_SEEK_FEED_BELOW", canon::TAXIS_SEEK_FEED_BELOW},
         {"TAXIS_SEEK_BREED_ABOVE", canon::TAXIS_SEEK_BREED_ABOVE},
@@ -197,6 +204,7 @@ int32_t ref_constant(const char* name, double* out) {
         {"FIELD_N_TEMP", canon::FIELD_N_TEMP}, {"FIELD_N_CO2", canon::FIELD_N_CO2},
         {"FIELD_N_N2", canon::FIELD_N_N2}, {"FIELD_N_IRRAD", canon::FIELD_N_IRRAD},
         {"MAX_CELLS", canon::MAX_CELLS}, {"DEFAULT_CELLS", canon::DEFAULT_CELLS},
+        {"MAX_TAUMOEBA", canon::MAX_TAUMOEBA}, {"DEFAULT_TAUMOEBA", canon::DEFAULT_TAUMOEBA},
         {"DEPOSIT_SCALE_TEMPERATURE", contract::DEPOSIT_SCALE_TEMPERATURE},
         {"DEPOSIT_SCALE_CO2", contract::DEPOSIT_SCALE_CO2},
         {"DEPOSIT_SCALE_N2", contract::DEPOSIT_SCALE_N2},

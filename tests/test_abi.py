@@ -10,10 +10,11 @@ import pytest
 from conftest import ORACLE, ROOT, ah
 
 HEADER = ROOT / "include" / "astro_hot.h"
+SCENARIO_HEADER = ROOT / "include" / "astro_scenario.h"
 
 
-def declared_symbols():
-    text = HEADER.read_text()
+def declared_symbols(headers=(HEADER, SCENARIO_HEADER)):
+    text = "".join(h.read_text() for h in headers)
     return sorted(set(re.findall(r"\b(astro_[a-z0-9_]+)\s*\(", text)))
 
 
@@ -39,9 +40,11 @@ def test_oracle_mirrors_the_abi(oracle):
     # Timing hooks, ensembles, snapshots and the surrogate have no C
     # counterpart: an ensemble's chambers are checked against standalone GPU
     # worlds, snapshots with the contract reader tests/asph.py, the surrogate
-    # against oracle/surrogate_oracle.py.
+    # against oracle/surrogate_oracle.py. The scenario layer
+    # (astro_scenario.h) is host code over this ABI, checked against the
+    # restatement in tests/scenario_py.py.
     product_only = {"astro_profile_enable", "astro_profile_read", "astro_profile_reset", "astro_world_stream"}
-    missing = [s for s in declared_symbols()
+    missing = [s for s in declared_symbols((HEADER,))
                if s not in product_only and not s.startswith(("astro_ensemble_", "astro_surrogate_", "astro_snapshot_"))
                and not hasattr(lib, "oracle_" + s[len("astro_"):])]
     assert not missing, missing
