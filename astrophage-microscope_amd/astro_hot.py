@@ -58,6 +58,16 @@ CELL_FIELDS = [
 ]
 
 
+# TaumoebaStore arrays in the reference's order (src/sim/predation.cuh:85-104).
+TAU_FIELDS = [
+    ("id", np.uint64), ("flags", np.uint32), ("x", np.float64), ("y", np.float64), ("z", np.float64),
+    ("vx", np.float64), ("vy", np.float64), ("vz", np.float64), ("dir_x", np.float32), ("dir_y", np.float32),
+    ("dir_z", np.float32), ("biomass", np.float64), ("prey_biomass", np.float64), ("digest_timer", np.float32),
+    ("tolerance", np.float32), ("generation", np.uint32), ("rng_state", np.uint64), ("density_ema", np.float32),
+    ("run_timer", np.float32), ("target", np.int32),
+]
+
+
 class AstroError(RuntimeError):
     def __init__(self, status: int, detail: str = ""):
         name = STATUS[status] if 0 <= status < len(STATUS) else f"status {status}"
@@ -73,13 +83,13 @@ class WorldDesc(C.Structure):
     _fields_ = [
         ("abi_version", C.c_int32), ("capacity", C.c_int32), ("seed", C.c_uint64),
         ("chamber_w", C.c_double), ("chamber_h", C.c_double), ("chamber_d", C.c_double),
-        ("precision", C.c_int32), ("_pad0", C.c_int32), ("hash_cell_size", C.c_double),
+        ("precision", C.c_int32), ("tau_capacity", C.c_int32), ("hash_cell_size", C.c_double),
         ("grid", GridDims * 4), ("_pad1", C.c_int32),
         ("boundary_x", C.c_uint8), ("boundary_y", C.c_uint8), ("gravity_axis", C.c_uint8),
         ("thermal_noise", C.c_uint8), ("contact_enabled", C.c_uint8), ("adhesion_enabled", C.c_uint8),
         ("thermal_enabled", C.c_uint8), ("emission_enabled", C.c_uint8), ("taxis_enabled", C.c_uint8),
         ("occlusion_exact", C.c_uint8), ("store_disposition", C.c_uint8), ("compaction_enabled", C.c_uint8),
-        ("lifecycle_enabled", C.c_uint8), ("_pad2", C.c_uint8 * 7),
+        ("lifecycle_enabled", C.c_uint8), ("tau_compaction_enabled", C.c_uint8), ("_pad2", C.c_uint8 * 6),
         ("ambient_temp", C.c_double), ("co2_init", C.c_double), ("division_offset", C.c_double),
         ("physics_rate", C.c_double), ("biology_rate", C.c_double),
         ("petrova_max_power", C.c_double), ("co2_mass_per_division", C.c_double),
@@ -99,6 +109,10 @@ class SpawnParams(C.Structure):
 
 class CellSoA(C.Structure):
     _fields_ = [(name, C.c_void_p) for name, _ in CELL_FIELDS]
+
+
+class TauSoA(C.Structure):
+    _fields_ = [(name, C.c_void_p) for name, _ in TAU_FIELDS]
 
 
 class Counters(C.Structure):
@@ -241,6 +255,11 @@ SIGNATURES = {
     "stage_diffuse": (_I, [_P, _I, _D]),
     "stage_lifecycle": (_I, [_P, _D]),
     "stage_compact": (_I, [_P]),
+    "taumoeba_spawn": (_I, [_P, _I, C.c_uint64]),
+    "taumoeba_upload": (_I, [_P, C.POINTER(TauSoA), _I]),
+    "taumoeba_download": (_I, [_P, C.POINTER(TauSoA), _I, C.POINTER(_I)]),
+    "taumoeba_counters": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(C.c_uint64)]),
+    "stage_predation": (_I, [_P, _D]),
 }
 PRODUCT_ONLY = {
     "profile_enable": (_I, [_P, _I]),
@@ -502,6 +521,30 @@ class World:
 
     def set_clock(self, physics: float, biology: float):
         self._check(self.b.world_set_clock(self.h, physics, biology))
+
+    # -- Taumoeba -------------------------------------------------------------
+    def tau_spawn(self, count: int, seed: int):
+        self._check(self.b.taumoeba_spawn(self.h, count, seed), "taumoeba_spawn")
+
+    def tau_counters(self) -> tuple[int, int, int]:
+        """(count, capacity, next_id)."""
+        n, cap, nid = C.c_int32(), C.c_int32(), C.c_uint64()
+        self._check(self.b.taumoeba_counters(self.h, C.byref(n), C.byref(cap), C.byref(nid)))
+        return n.value, cap.value, nid.value
+
+    def tau_download(self) -> dict[str, np.ndarray]:
+        n = self.tau_counters()[0]
+        arrays = {name: np.zeros(n, dt) for name, dt in TAU_FIELDS}
+        soa = TauSoA(*[_ptr(arrays[name]) for name, _ in TAU_FIELDS])
+        got = C.c_int32()
+        self._check(self.b.taumoeba_download(self.h, C.byref(soa), n, C.byref(got)), "taumoeba_download")
+        return arrays
+
+    def tau_upload(self, arrays: dict[str, np.ndarray]):
+        n = len(arrays["id"])
+        conv = {name: np.ascontiguousarray(arrays[name], dt) for name, dt in TAU_FIELDS}
+        soa = TauSoA(*[_ptr(conv[name]) for name, _ in TAU_FIELDS])
+        self._check(self.b.taumoeba_upload(self.h, C.byref(soa), n), "taumoeba_upload")
 
     def set_clock_preset(self, preset: int, physics: float = 1.0, biology: float = 1.0):
         self._check(self.b.world_set_clock_preset(self.h, preset, physics, biology))

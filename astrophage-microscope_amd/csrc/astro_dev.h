@@ -339,6 +339,18 @@ __device__ __forceinline__ bool apply_boundary_axis(double& p, double& v, double
     return true;
 }
 
+// division_axis (src/sim/lifecycle.cuh:87-99): a unit vector hashed from the
+// daughter id, so placing the pair consumes no draw.
+__device__ __forceinline__ void division_axis(uint64_t did, double& ax, double& ay, double& az) {
+    const uint64_t h = splitmix64(splitmix64(did));
+    const double u = static_cast<double>(h >> 33) * (1.0 / 2147483648.0) - 1.0;
+    const double phi = static_cast<double>(h & 0xFFFFFFFFull) * (ASTRO_TWO_PI / 4294967296.0);
+    const double sq = sqrt(u * u < 1.0 ? 1.0 - u * u : 0.0);
+    ax = sq * cos(phi);
+    ay = sq * sin(phi);
+    az = u;
+}
+
 __device__ __forceinline__ double lumped_exchange_energy(double tc, double tg, double g, double cap, double dt) {
     if (cap <= 0.0 || g <= 0.0) return 0.0;
     const double relax = -expm1(-g * dt / cap);

@@ -201,10 +201,59 @@ void launch_brush(const Launch& L, const Grid& g, float cx, float cy, float radi
 void launch_stats_fields(const Launch& L, const Grid& T, const Grid& C, const Grid& N, bool c_on,
                          bool n_on, unsigned long long* stats);
 
+// ---- Taumoeba (kernels_tau.hip) ----
+// Device-resident bookkeeping of the predator store, so a tick never reads a
+// count back (the reference syncs twice per predation step).
+struct TauCtr {
+    int32_t count, capacity;
+    int32_t births, deads;      // this tick's (clamped) divisions, dead slots before them
+    uint64_t next_id;
+};
+
+// The Taumoeba SoA (src/sim/predation.cuh:85-126), FP64 kinematics in either
+// cell layout, plus the scan and gather scratch of its lifecycle.
+struct TauDev {
+    uint64_t* id;
+    uint32_t* flags;
+    double *x, *y, *z, *vx, *vy, *vz;
+    float *dir_x, *dir_y, *dir_z;
+    double *biomass, *prey_biomass;
+    float *digest_timer, *tolerance;
+    uint32_t* generation;
+    uint64_t* rng_state;
+    float *density_ema, *run_timer;
+    int32_t* target;             // reference slot of the cell claimed this tick, -1 none
+    TauCtr* ctr;
+    int32_t* prey_rec;           // [capacity] sorted hash record of that cell
+    int32_t capacity;
+    int32_t cell_capacity;       // entries of `claim`
+    int32_t* mark;               // [capacity] divides / survives flags
+    int32_t* prefix;             // [capacity] their exclusive prefix
+    int32_t* src;                // [capacity] compaction source slot of each output slot
+    unsigned long long* scratch; // [capacity] gather buffer, 8 bytes a slot
+    unsigned long long* claim;   // [cell capacity] per cell slot; all-ones between ticks
+};
+
+struct TauTick {
+    double dt, dt_bio, ambient_temp;
+    double chamber_w, chamber_h, chamber_d;
+    int boundary_x, boundary_y, thermal_noise, store_disposition, compaction;
+};
+
+// predation_step (src/sim/predation.cu:435-490): hunt and claim, resolve,
+// release the claims, nitrogen, then division and compaction in one block.
+template <class P> void launch_predation(const Launch& L, const WorldDev<P>& w, const TauDev& t, const TauTick& tt);
+// spawn_kernel (src/sim/predation.cu:33-76) for slots [base, base + count).
+void launch_tau_spawn(const Launch& L, const TauDev& t, int32_t count, int32_t base, uint64_t first_id,
+                      uint64_t seed, double chamber_w, double chamber_h, double chamber_d);
+void launch_fill_u64(const Launch& L, unsigned long long* v, size_t n, unsigned long long value);
+// stats_taumoeba_kernel (src/sim/stats.cu:65-74) into the stats words.
+void launch_tau_stats(const Launch& L, const TauDev& t, unsigned long long* stats);
+
 // StatsAccum word layout (src/sim/world.cuh:39-45), all fixed point.
 enum StatsWord {
     SW_ENERGY = 0, SW_TEMP_CELL, SW_TEMP_MEDIUM, SW_CO2, SW_N2, SW_MAX_TEMP,
-    SW_N_ALIVE, SW_N_DEAD, SW_N_AWAKE, SW_N_DORMANT, SW_COUNT
+    SW_N_ALIVE, SW_N_DEAD, SW_N_AWAKE, SW_N_DORMANT, SW_N_TAU, SW_TAU_TOL, SW_COUNT
 };
 
 }  // namespace astro_dev

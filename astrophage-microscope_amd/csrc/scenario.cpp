@@ -415,8 +415,8 @@ double doubling_time(const Aggregates& a, double biology_rate) {
     return (std::log(2.0) / slope) * biology_rate;
 }
 
-// metric_measure (src/sim/accept.cpp:185-206). No Taumoeba and no flash here:
-// their metrics read 0, as the reference's do for a run without them.
+// metric_measure (src/sim/accept.cpp:185-206). No spin-drive flash here: its
+// metric reads 0, as the reference's does for a run without one.
 int32_t measure(int32_t metric, const astro_stats& fin, const Aggregates& a, astro_world* w,
                 const astro_scenario& s, double& out) {
     out = 0.0;
@@ -457,7 +457,30 @@ int32_t measure(int32_t metric, const astro_stats& fin, const Aggregates& a, ast
             out = correlation(charge, other);
             return ASTRO_OK;
         }
-        default: return ASTRO_OK;   // max_tau_tolerance, impulse_per_cycle
+        case ASTRO_METRIC_MAX_TAU_TOLERANCE: {
+            // over live Taumoeba (src/sim/accept.cpp:63-77)
+            int32_t n = 0, cap = 0;
+            int32_t st = astro_taumoeba_counters(w, &n, &cap, nullptr);
+            if (st != ASTRO_OK || n <= 0) return st;
+            std::vector<uint64_t> id(n), rng(n);
+            std::vector<uint32_t> flags(n), gen(n);
+            std::vector<double> d64[8];
+            std::vector<float> f32[7];
+            std::vector<int32_t> target(n);
+            for (auto& v : d64) v.resize(n);
+            for (auto& v : f32) v.resize(n);
+            astro_tau_soa t{id.data(),     flags.data(),  d64[0].data(), d64[1].data(), d64[2].data(),
+                            d64[3].data(), d64[4].data(), d64[5].data(), f32[0].data(), f32[1].data(),
+                            f32[2].data(), d64[6].data(), d64[7].data(), f32[3].data(), f32[4].data(),
+                            gen.data(),    rng.data(),    f32[5].data(), f32[6].data(), target.data()};
+            int32_t got = 0;
+            if ((st = astro_taumoeba_download(w, &t, n, &got)) != ASTRO_OK) return st;
+            for (int32_t i = 0; i < n; ++i)
+                if ((flags[i] & ASTRO_FLAG_OCCUPIED) && (flags[i] & ASTRO_FLAG_ALIVE))
+                    out = std::max(out, static_cast<double>(f32[4][i]));
+            return ASTRO_OK;
+        }
+        default: return ASTRO_OK;   // impulse_per_cycle: no spin-drive flash here
     }
 }
 
@@ -519,9 +542,6 @@ void astro_scenario_capacities(const astro_scenario* s, int32_t* cell_capacity, 
 int32_t astro_scenario_instantiate(const astro_scenario* s, const astro_world_desc* base, astro_world** out) {
     if (!s || !out) return ASTRO_INVALID_ARGUMENT;
     *out = nullptr;
-    for (int i = 0; i < s->population_count; ++i)
-        if (s->populations[i].kind == ASTRO_ORGANISM_TAUMOEBA && s->populations[i].count > 0)
-            return ASTRO_UNSUPPORTED;   // no Taumoeba population in this build
     for (int i = 0; i < s->drive_count; ++i)
         if (s->drive[i].kind == ASTRO_STIM_FLASH) return ASTRO_UNSUPPORTED;   // no spin-drive flash
 
@@ -531,8 +551,7 @@ int32_t astro_scenario_instantiate(const astro_scenario* s, const astro_world_de
     d.chamber_w = s->chamber_w;
     d.chamber_h = s->chamber_h;
     d.chamber_d = s->chamber_d;
-    int32_t tau_cap = 0;
-    astro_scenario_capacities(s, &d.capacity, &tau_cap);
+    astro_scenario_capacities(s, &d.capacity, &d.tau_capacity);
     d.seed = s->seed;
     d.co2_init = s->co2_init;
     d.boundary_x = static_cast<uint8_t>(s->boundary_x);
@@ -542,6 +561,7 @@ int32_t astro_scenario_instantiate(const astro_scenario* s, const astro_world_de
     d.store_disposition = static_cast<uint8_t>(s->store_disposition);
     d.thermal_enabled = s->thermal_active ? 1 : 0;
     d.compaction_enabled = s->compaction ? 1 : 0;
+    d.tau_compaction_enabled = s->tau_compaction ? 1 : 0;
     astro_world* w = nullptr;
     int32_t st = astro_world_create(&d, &w);
     if (st != ASTRO_OK) return st;
@@ -556,6 +576,11 @@ int32_t astro_scenario_instantiate(const astro_scenario* s, const astro_world_de
     for (int i = 0; st == ASTRO_OK && i < s->population_count; ++i) {
         const astro_population_spec& p = s->populations[i];
         if (p.count <= 0) continue;
+        if (p.kind == ASTRO_ORGANISM_TAUMOEBA) {
+            st = astro_taumoeba_spawn(w, p.count, seed);
+            ++seed;
+            continue;
+        }
         astro_spawn_params sp{};
         sp.count = p.count;
         sp.placement = p.placement;

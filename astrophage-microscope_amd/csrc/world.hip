@@ -218,6 +218,11 @@ struct astro_world {
     // a snapshot header; parameter overrides also make a run non-canon.
     bool non_canon = false;
 
+    // The Taumoeba store; the tick runs predation once any were added.
+    TauDev tau{};
+    int32_t tau_cap = 0;
+    bool tau_on = false;
+
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     bool graph_ok = false;
@@ -350,6 +355,23 @@ std::vector<TickParams> ticks_of(WorldSpan ws, double dt) {
     return t;
 }
 
+TauTick tau_tick(const astro_world* w, double dt) {
+    const astro_world_desc& d = w->desc;
+    TauTick t{};
+    t.dt = dt;
+    t.dt_bio = dt * w->biology_rate;
+    t.ambient_temp = d.ambient_temp;
+    t.chamber_w = d.chamber_w;
+    t.chamber_h = d.chamber_h;
+    t.chamber_d = d.chamber_d;
+    t.boundary_x = d.boundary_x;
+    t.boundary_y = d.boundary_y;
+    t.thermal_noise = d.thermal_noise;
+    t.store_disposition = d.store_disposition;
+    t.compaction = d.tau_compaction_enabled;
+    return t;
+}
+
 // grid_diffuse / grid_diffuse_substep: enqueue substeps, swapping on the host.
 int32_t enqueue_diffuse(WorldSpan ws, const Launch& L, int field, double dt) {
     if (!ws[0]->f[field].on) return ASTRO_OK;
@@ -429,9 +451,15 @@ int32_t enqueue_tick(WorldSpan ws, const Launch& L) {
     // reads the CO2 field (the sample above has already taken its values), and
     // the integrate pass can then book the lifecycle's CO2 demand itself.
     if ((st = enqueue_diffuse(ws, L, ASTRO_FIELD_CO2, dt)) != ASTRO_OK) return st;
-    const bool book = ds.lifecycle_enabled && ws[0]->f[ASTRO_FIELD_CO2].on;
+    // With predators the demand is booked inside the lifecycle, after the
+    // engulfs: a cell eaten this tick asks for no CO2.
+    const bool predators = ws[0]->tau_on;
+    const bool book = ds.lifecycle_enabled && ws[0]->f[ASTRO_FIELD_CO2].on && !predators;
     launch_integrate<P>(L, views_of<P>(ws), ts, book);
     if ((st = enqueue_diffuse(ws, L, ASTRO_FIELD_N2, dt)) != ASTRO_OK) return st;
+    // Stage 10 (step.cu:172-175), on the hash of stage 1.
+    if (predators)
+        for (astro_world* w : ws) launch_predation<P>(L, make_view<P>(w), w->tau, tau_tick(w, dt));
     if (ds.lifecycle_enabled) launch_lifecycle<P>(L, views_of<P>(ws), ts, book);
     for (astro_world* w : ws) w->hash_valid = false;
     return ASTRO_OK;
@@ -698,6 +726,69 @@ struct HostSoa {
     }
 };
 
+// Host buffers of the Taumoeba SoA for n predators.
+struct HostTau {
+    std::vector<uint64_t> id, rng;
+    std::vector<uint32_t> flags, gen;
+    std::vector<double> x, y, z, vx, vy, vz, biomass, prey;
+    std::vector<float> dx, dy, dz, digest, tol, ema, run;
+    std::vector<int32_t> target;
+    explicit HostTau(size_t n)
+        : id(n), rng(n), flags(n), gen(n), x(n), y(n), z(n), vx(n), vy(n), vz(n), biomass(n), prey(n), dx(n), dy(n),
+          dz(n), digest(n), tol(n), ema(n), run(n), target(n) {}
+    astro_tau_soa view() {
+        return astro_tau_soa{id.data(),      flags.data(), x.data(),      y.data(),      z.data(),
+                             vx.data(),      vy.data(),    vz.data(),     dx.data(),     dy.data(),
+                             dz.data(),      biomass.data(), prey.data(), digest.data(), tol.data(),
+                             gen.data(),     rng.data(),   ema.data(),    run.data(),    target.data()};
+    }
+    // the reference's serialization order (src/sim/snapshot.cu:59-67)
+    template <class F>
+    void each(F&& f) {
+        f(id); f(flags); f(x); f(y); f(z); f(vx); f(vy); f(vz); f(dx); f(dy); f(dz); f(biomass); f(prey);
+        f(digest); f(tol); f(gen); f(rng); f(ema); f(run); f(target);
+    }
+};
+
+// The store's counters, read and written in the world's stream order.
+int32_t read_tau(astro_world* w, TauCtr& tc) {
+    int32_t st = hip_check(w, hipMemcpyAsync(&tc, w->tau.ctr, sizeof(tc), hipMemcpyDeviceToHost, w->stream),
+                           "read taumoeba counters");
+    if (st != ASTRO_OK) return st;
+    return hip_check(w, hipStreamSynchronize(w->stream), "synchronize");
+}
+
+int32_t write_tau(astro_world* w, const TauCtr& tc) {
+    int32_t st = hip_check(w, hipMemcpyAsync(w->tau.ctr, &tc, sizeof(tc), hipMemcpyHostToDevice, w->stream),
+                           "write taumoeba counters");
+    if (st != ASTRO_OK) return st;
+    return hip_check(w, hipStreamSynchronize(w->stream), "synchronize");
+}
+
+// Device <-> host copies of the Taumoeba arrays [0, n), in one order.
+int32_t copy_tau(astro_world* w, const astro_tau_soa* h, int32_t n, bool to_device) {
+    const TauDev& t = w->tau;
+    const size_t m = static_cast<size_t>(n);
+    const std::pair<void*, void*> pairs[] = {
+        {t.id, h->id}, {t.flags, h->flags}, {t.x, h->x}, {t.y, h->y}, {t.z, h->z}, {t.vx, h->vx},
+        {t.vy, h->vy}, {t.vz, h->vz}, {t.dir_x, h->dir_x}, {t.dir_y, h->dir_y}, {t.dir_z, h->dir_z},
+        {t.biomass, h->biomass}, {t.prey_biomass, h->prey_biomass}, {t.digest_timer, h->digest_timer},
+        {t.tolerance, h->tolerance}, {t.generation, h->generation}, {t.rng_state, h->rng_state},
+        {t.density_ema, h->density_ema}, {t.run_timer, h->run_timer}, {t.target, h->target}};
+    const size_t bytes[] = {8, 4, 8, 8, 8, 8, 8, 8, 4, 4, 4, 8, 8, 4, 4, 4, 8, 4, 4, 4};
+    for (size_t k = 0; k < sizeof(bytes) / sizeof(bytes[0]); ++k)
+        if (!pairs[k].second) return ASTRO_INVALID_ARGUMENT;
+    int32_t st = hip_check(w, hipStreamSynchronize(w->stream), "synchronize");
+    for (size_t k = 0; k < sizeof(bytes) / sizeof(bytes[0]) && st == ASTRO_OK; ++k) {
+        const hipError_t e =
+            to_device ? hipMemcpyAsync(pairs[k].first, pairs[k].second, bytes[k] * m, hipMemcpyHostToDevice, w->stream)
+                      : hipMemcpyAsync(pairs[k].second, pairs[k].first, bytes[k] * m, hipMemcpyDeviceToHost, w->stream);
+        st = hip_check(w, e, "taumoeba copy");
+    }
+    if (st != ASTRO_OK) return st;
+    return hip_check(w, hipStreamSynchronize(w->stream), "synchronize");
+}
+
 // Everything after the header, in file order (src/sim/snapshot.cu:96-118).
 int32_t serialize_body(astro_world* w, std::vector<char>& body, int32_t& override_count, int32_t& count) {
     body.clear();
@@ -717,6 +808,14 @@ int32_t serialize_body(astro_world* w, std::vector<char>& body, int32_t& overrid
         astro_cell_soa v = h.view();
         st = w->fp32 ? download_cells(w, w->c32[0], &v, count) : download_cells(w, w->c64[0], &v, count);
         if (st != ASTRO_OK) return st;
+        h.each([&](const auto& a) { put(a.data(), sizeof(a[0]) * a.size()); });
+    }
+    TauCtr tc{};
+    if ((st = read_tau(w, tc)) != ASTRO_OK) return st;
+    if (tc.count > 0) {
+        HostTau h(static_cast<size_t>(tc.count));
+        astro_tau_soa v = h.view();
+        if ((st = copy_tau(w, &v, tc.count, false)) != ASTRO_OK) return st;
         h.each([&](const auto& a) { put(a.data(), sizeof(a[0]) * a.size()); });
     }
     for (int fi : {ASTRO_FIELD_TEMPERATURE, ASTRO_FIELD_CO2, ASTRO_FIELD_N2}) {
@@ -740,7 +839,7 @@ int32_t serialize_body(astro_world* w, std::vector<char>& body, int32_t& overrid
 struct TickShape {
     bool fp32;
     double dt;
-    bool thermal, light, taxis, lifecycle, contact;
+    bool thermal, light, taxis, lifecycle, contact, predators;
     bool on[ASTRO_FIELD_COUNT];
     int substeps[3];   // T (when thermal), CO2, N2 (when on)
     bool operator==(const TickShape&) const = default;
@@ -756,6 +855,7 @@ TickShape tick_shape(const astro_world* w) {
     s.taxis = d.taxis_enabled != 0;
     s.lifecycle = d.lifecycle_enabled != 0;
     s.contact = d.contact_enabled != 0;
+    s.predators = w->tau_on;
     for (int i = 0; i < ASTRO_FIELD_COUNT; ++i) s.on[i] = w->f[i].on;
     if (s.thermal) s.substeps[0] = substeps_for_dt(w->f[ASTRO_FIELD_TEMPERATURE], s.dt);
     if (s.on[ASTRO_FIELD_CO2]) s.substeps[1] = substeps_for_dt(w->f[ASTRO_FIELD_CO2], s.dt);
@@ -902,6 +1002,10 @@ int32_t eager_tick(astro_ensemble* e) {
 
 int32_t check_shape(astro_ensemble* e) {
     const TickShape s0 = tick_shape(e->worlds[0]);
+    for (size_t i = 0; i < e->worlds.size(); ++i)
+        if (e->worlds[i]->tau_on)
+            return efail(e, ASTRO_UNSUPPORTED, "chamber " + std::to_string(i) + " holds Taumoeba; predation is "
+                         "stepped per world, not batched");
     for (size_t i = 1; i < e->worlds.size(); ++i)
         if (!(tick_shape(e->worlds[i]) == s0))
             return efail(e, ASTRO_INVALID_ARGUMENT,
@@ -975,6 +1079,9 @@ int32_t astro_world_create(const astro_world_desc* din, astro_world** out) {
     if (!(d.chamber_w > 0 && d.chamber_h > 0 && d.chamber_d > 0)) return ASTRO_INVALID_ARGUMENT;
     if (d.precision != ASTRO_FP64_CONTRACT && d.precision != ASTRO_FP32_HOT) return ASTRO_INVALID_ARGUMENT;
     d.capacity = cap;
+    const int32_t tau_cap = d.tau_capacity > 0 ? d.tau_capacity : ASTRO_DEFAULT_TAUMOEBA;
+    if (tau_cap > ASTRO_MAX_TAUMOEBA) return ASTRO_INVALID_ARGUMENT;
+    d.tau_capacity = tau_cap;
     if (d.division_offset <= 0.0) d.division_offset = ASTRO_CELL_RADIUS;
     const int defaults[4] = {ASTRO_FIELD_N_TEMP, ASTRO_FIELD_N_CO2, ASTRO_FIELD_N_N2, ASTRO_FIELD_N_IRRAD};
     for (int f = 0; f < 4; ++f) {
@@ -1069,6 +1176,37 @@ int32_t astro_world_create(const astro_world_desc* din, astro_world** out) {
         w->xsum = cv.take<int32_t>(static_cast<size_t>(scan_tiles_for(cap)));
         w->co2_demand = cv.take<unsigned long long>(std::max<size_t>(1, w->f[ASTRO_FIELD_CO2].count()));
         w->stats = cv.take<unsigned long long>(SW_COUNT);
+        TauDev& t = w->tau;
+        const size_t tn = static_cast<size_t>(tau_cap);
+        t.capacity = tau_cap;
+        t.id = cv.take<uint64_t>(tn);
+        t.flags = cv.take<uint32_t>(tn);
+        t.x = cv.take<double>(tn);
+        t.y = cv.take<double>(tn);
+        t.z = cv.take<double>(tn);
+        t.vx = cv.take<double>(tn);
+        t.vy = cv.take<double>(tn);
+        t.vz = cv.take<double>(tn);
+        t.dir_x = cv.take<float>(tn);
+        t.dir_y = cv.take<float>(tn);
+        t.dir_z = cv.take<float>(tn);
+        t.biomass = cv.take<double>(tn);
+        t.prey_biomass = cv.take<double>(tn);
+        t.digest_timer = cv.take<float>(tn);
+        t.tolerance = cv.take<float>(tn);
+        t.generation = cv.take<uint32_t>(tn);
+        t.rng_state = cv.take<uint64_t>(tn);
+        t.density_ema = cv.take<float>(tn);
+        t.run_timer = cv.take<float>(tn);
+        t.target = cv.take<int32_t>(tn);
+        t.ctr = cv.take<TauCtr>(1);
+        t.prey_rec = cv.take<int32_t>(tn);
+        t.mark = cv.take<int32_t>(tn);
+        t.prefix = cv.take<int32_t>(tn);
+        t.src = cv.take<int32_t>(tn);
+        t.scratch = cv.take<unsigned long long>(tn);
+        t.claim = cv.take<unsigned long long>(n);
+        t.cell_capacity = cap;
     };
     Carver measure;
     layout(measure);
@@ -1091,7 +1229,13 @@ int32_t astro_world_create(const astro_world_desc* din, astro_world** out) {
     c0.capacity = cap;
     c0.next_id = 1;
     hipMemcpyAsync(w->ctr, &c0, sizeof(c0), hipMemcpyHostToDevice, w->stream);
+    w->tau_cap = tau_cap;
+    TauCtr t0{};
+    t0.capacity = tau_cap;
+    t0.next_id = 1;
+    hipMemcpyAsync(w->tau.ctr, &t0, sizeof(t0), hipMemcpyHostToDevice, w->stream);
     const Launch L{w->stream, nullptr};
+    launch_fill_u64(L, w->tau.claim, n, ~0ull);   // no claims
     for (int fi = 0; fi < ASTRO_FIELD_COUNT; ++fi) {
         FieldHost& g = w->f[fi];
         launch_fill(L, g.value, g.count(), static_cast<float>(g.ambient));
@@ -1405,6 +1549,7 @@ int32_t astro_world_stats(astro_world* w, astro_stats* s) {
     if (w->fp32) launch_stats_cells<float>(L, make_view<float>(w));
     else launch_stats_cells<double>(L, make_view<double>(w));
     launch_stats_fields(L, w->f[0].grid(), w->f[1].grid(), w->f[2].grid(), w->f[1].on, w->f[2].on, w->stats);
+    if (w->tau_on) launch_tau_stats(L, w->tau, w->stats);
     if ((st = hip_check(w, hipGetLastError(), "stats kernels")) != ASTRO_OK) return st;
     unsigned long long a[SW_COUNT];
     Counters c{};
@@ -1421,6 +1566,10 @@ int32_t astro_world_stats(astro_world* w, astro_stats* s) {
     s->n_dead = static_cast<int32_t>(nd);
     s->n_awake = static_cast<int32_t>(a[SW_N_AWAKE]);
     s->n_dormant = static_cast<int32_t>(a[SW_N_DORMANT]);
+    s->n_taumoeba = static_cast<int32_t>(a[SW_N_TAU]);
+    s->mean_tau_tolerance = a[SW_N_TAU] > 0 ? from_fixed(a[SW_TAU_TOL], ASTRO_DEPOSIT_SCALE_STATS) /
+                                                  static_cast<double>(a[SW_N_TAU])
+                                            : 0.0;
     s->total_energy_j = from_fixed(a[SW_ENERGY], ASTRO_DEPOSIT_SCALE_STATS);
     s->mean_charge = (na + nd) > 0u ? s->total_energy_j / ((na + nd) * ASTRO_CELL_ENERGY_MAX) : 0.0;
     s->mean_temp_cell_k = na > 0u ? from_fixed(a[SW_TEMP_CELL], ASTRO_DEPOSIT_SCALE_STATS) / na : 0.0;
@@ -1680,6 +1829,78 @@ int32_t astro_profile_reset(astro_world* w) {
 
 void* astro_world_stream(astro_world* w) { return w ? static_cast<void*>(w->stream) : nullptr; }
 
+// ---- Taumoeba (src/sim/predation.cu) ---------------------------------------------
+int32_t astro_taumoeba_spawn(astro_world* w, int32_t count, uint64_t seed) {
+    if (!w || count < 0) return ASTRO_INVALID_ARGUMENT;
+    if (count == 0) return ASTRO_OK;
+    TauCtr tc{};
+    int32_t st = read_tau(w, tc);
+    if (st != ASTRO_OK) return st;
+    if (static_cast<int64_t>(tc.count) + count > tc.capacity)
+        return fail(w, ASTRO_CAPACITY_EXCEEDED, "taumoeba spawn exceeds capacity");
+    const astro_world_desc& d = w->desc;
+    launch_tau_spawn(Launch{w->stream, nullptr}, w->tau, count, tc.count, tc.next_id, seed, d.chamber_w,
+                     d.chamber_h, d.chamber_d);
+    if ((st = hip_check(w, hipGetLastError(), "taumoeba spawn")) != ASTRO_OK) return st;
+    tc.count += count;
+    tc.next_id += static_cast<uint64_t>(count);
+    if ((st = write_tau(w, tc)) != ASTRO_OK) return st;
+    if (!w->tau_on) drop_graph(w);   // the captured tick has no predation stage
+    w->tau_on = true;
+    return ASTRO_OK;
+}
+
+int32_t astro_taumoeba_upload(astro_world* w, const astro_tau_soa* src, int32_t n) {
+    if (!w || !src || n < 0) return ASTRO_INVALID_ARGUMENT;
+    if (n > w->tau_cap) return fail(w, ASTRO_CAPACITY_EXCEEDED, "taumoeba upload exceeds capacity");
+    int32_t st = hip_check(w, hipStreamSynchronize(w->stream), "synchronize");
+    if (st != ASTRO_OK) return st;
+    if (n > 0 && (st = copy_tau(w, src, n, true)) != ASTRO_OK) return st;
+    TauCtr tc{};
+    tc.count = n;
+    tc.capacity = w->tau_cap;
+    uint64_t mx = 0;
+    for (int32_t i = 0; i < n; ++i) mx = src->id[i] > mx ? src->id[i] : mx;
+    tc.next_id = mx + 1;
+    if ((st = write_tau(w, tc)) != ASTRO_OK) return st;
+    if (n > 0 && !w->tau_on) {
+        drop_graph(w);
+        w->tau_on = true;
+    }
+    return ASTRO_OK;
+}
+
+int32_t astro_taumoeba_download(astro_world* w, astro_tau_soa* dst, int32_t n, int32_t* count_out) {
+    if (!w || !dst || n < 0) return ASTRO_INVALID_ARGUMENT;
+    TauCtr tc{};
+    int32_t st = read_tau(w, tc);
+    if (st != ASTRO_OK) return st;
+    if (count_out) *count_out = tc.count;
+    const int32_t m = n < tc.count ? n : tc.count;
+    return m > 0 ? copy_tau(w, dst, m, false) : ASTRO_OK;
+}
+
+int32_t astro_taumoeba_counters(astro_world* w, int32_t* count, int32_t* capacity, uint64_t* next_id) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    TauCtr tc{};
+    const int32_t st = read_tau(w, tc);
+    if (st != ASTRO_OK) return st;
+    if (count) *count = tc.count;
+    if (capacity) *capacity = tc.capacity;
+    if (next_id) *next_id = tc.next_id;
+    return ASTRO_OK;
+}
+
+int32_t astro_stage_predation(astro_world* w, double dt) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    int32_t st = ensure_hash(w);
+    if (st != ASTRO_OK) return st;
+    const Launch L = launch_of(w);
+    if (w->fp32) launch_predation<float>(L, make_view<float>(w), w->tau, tau_tick(w, dt));
+    else launch_predation<double>(L, make_view<double>(w), w->tau, tau_tick(w, dt));
+    return hip_check(w, hipGetLastError(), "predation");
+}
+
 // ---- snapshots (src/sim/snapshot.cu:131-266) ------------------------------------
 int32_t astro_snapshot_to_bytes(astro_world* w, const char* scenario_id, void* buf, size_t cap, size_t* n_out) {
     if (!w) return ASTRO_INVALID_ARGUMENT;
@@ -1700,6 +1921,10 @@ int32_t astro_snapshot_to_bytes(astro_world* w, const char* scenario_id, void* b
     h.sim_time_s = w->sim_time;
     h.cell_count = count;
     h.cell_capacity = w->capacity;
+    TauCtr tc{};
+    if ((st = read_tau(w, tc)) != ASTRO_OK) return st;
+    h.taumoeba_count = tc.count;
+    h.taumoeba_capacity = w->tau_cap;
     h.n_temp = snap_grid_n(w->f[ASTRO_FIELD_TEMPERATURE]);
     h.n_co2 = snap_grid_n(w->f[ASTRO_FIELD_CO2]);
     h.n_n2 = snap_grid_n(w->f[ASTRO_FIELD_N2]);
@@ -1746,8 +1971,8 @@ int32_t astro_snapshot_from_bytes(const astro_world_desc* base, const void* data
     const char* body = static_cast<const char*>(data) + sizeof(h);
     const size_t body_len = n - sizeof(h);
     if (fnv1a64(body, body_len, FNV_OFFSET) != h.state_hash) return ASTRO_INVALID_ARGUMENT;   // corrupt
-    if (h.taumoeba_count != 0) return ASTRO_UNSUPPORTED;   // no Taumoeba population here
     if (h.cell_count < 0 || h.cell_count > h.cell_capacity || h.override_count < 0) return ASTRO_INVALID_ARGUMENT;
+    if (h.taumoeba_count < 0 || h.taumoeba_count > h.taumoeba_capacity) return ASTRO_INVALID_ARGUMENT;
     const char* cur = body;
     const char* end = body + body_len;
     // The description: the caller's (motion configuration and grids are not
@@ -1757,6 +1982,7 @@ int32_t astro_snapshot_from_bytes(const astro_world_desc* base, const void* data
     if (base) d = *base;
     else astro_world_desc_default(&d);
     d.capacity = h.cell_capacity;
+    d.tau_capacity = h.taumoeba_capacity;
     d.chamber_w = h.chamber_w;
     d.chamber_h = h.chamber_h;
     d.chamber_d = h.chamber_d;
@@ -1799,6 +2025,22 @@ int32_t astro_snapshot_from_bytes(const astro_world_desc* base, const void* data
         if (!ok) return bail(ASTRO_INVALID_ARGUMENT);
         astro_cell_soa v = soa.view();
         if ((st = astro_cells_upload(w, &v, count)) != ASTRO_OK) return bail(st);
+    }
+    if (h.taumoeba_count > 0) {
+        HostTau tau(static_cast<size_t>(h.taumoeba_count));
+        bool ok = true;
+        tau.each([&](auto& a) {
+            const size_t bytes = sizeof(a[0]) * a.size();
+            if (!ok || static_cast<size_t>(end - cur) < bytes) {
+                ok = false;
+                return;
+            }
+            std::memcpy(a.data(), cur, bytes);
+            cur += bytes;
+        });
+        if (!ok) return bail(ASTRO_INVALID_ARGUMENT);
+        astro_tau_soa v = tau.view();
+        if ((st = astro_taumoeba_upload(w, &v, h.taumoeba_count)) != ASTRO_OK) return bail(st);
     }
     for (int k = 0; k < 3; ++k) {
         const FieldHost& g = w->f[fields[k]];

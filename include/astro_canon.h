@@ -72,6 +72,25 @@
 #define ASTRO_CLOCK_GENERATIONAL_PHYSICS 0.5
 #define ASTRO_CLOCK_GENERATIONAL_BIOLOGY 1000000.0
 
+/* ---- Taumoeba, the predator ---------------------------------------------------- */
+#define ASTRO_TAU_DIAMETER          4.0000000000000003e-05
+#define ASTRO_TAU_RADIUS            2.0000000000000002e-05
+#define ASTRO_TAU_VOLUME            3.3510321638291131e-14
+#define ASTRO_TAU_MASS              3.3450003059342215e-11
+#define ASTRO_TAU_MASS_DRY          1e-13
+#define ASTRO_TAU_CRAWL_SPEED       5.0000000000000004e-06
+#define ASTRO_TAU_DRAG_20C          3.776489400816919e-07
+#define ASTRO_TAU_CRAWL_THRUST      1.8882447004084599e-12
+#define ASTRO_TAU_DIGEST_TIME       120.0
+#define ASTRO_TAU_BIOMASS_YIELD     0.59999999999999998
+#define ASTRO_TAU_DIVIDE_BIOMASS    2.0000000000000001e-13
+#define ASTRO_TAU_N2_LETHAL_CONC    0.01
+#define ASTRO_TAU_N2_LETHAL_TIME    2000.0
+#define ASTRO_TAU_N2_HAZARD_RATE    0.050000000000000003
+#define ASTRO_TAU_N2_TOLERANCE_INIT 0.0
+#define ASTRO_TAU_N2_TOLERANCE_K    1.0
+#define ASTRO_TAU_MUTATION_SIGMA    0.02
+
 /* ---- taxis ------------------------------------------------------------------ */
 #define ASTRO_TAXIS_MEMORY_TIME     0.10000000000000001
 #define ASTRO_TAXIS_DARK_THRESHOLD  0.001

@@ -81,7 +81,7 @@ typedef struct astro_world_desc {
     uint64_t seed;
     double   chamber_w, chamber_h, chamber_d;
     int32_t  precision;          /* ASTRO_FP64_CONTRACT | ASTRO_FP32_HOT */
-    int32_t  _pad0;
+    int32_t  tau_capacity;       /* Taumoeba store capacity; 0 = ASTRO_DEFAULT_TAUMOEBA */
     double   hash_cell_size;     /* 0 = HASH_CELL_FACTOR * CELL_DIAMETER (22 um) */
     astro_grid_dims grid[ASTRO_FIELD_COUNT];  /* per field; zeros = reference n x n x 1 */
     int32_t  _pad1;
@@ -89,7 +89,7 @@ typedef struct astro_world_desc {
     uint8_t  boundary_x, boundary_y, gravity_axis, thermal_noise;
     uint8_t  contact_enabled, adhesion_enabled, thermal_enabled, emission_enabled;
     uint8_t  taxis_enabled, occlusion_exact, store_disposition, compaction_enabled;
-    uint8_t  lifecycle_enabled, _pad2[7];
+    uint8_t  lifecycle_enabled, tau_compaction_enabled, _pad2[6];
     double   ambient_temp;
     double   co2_init;           /* initial CO2 concentration [kg/m^3] */
     double   division_offset;    /* daughter offset along the axis; 0 = CELL_RADIUS */
@@ -205,7 +205,8 @@ int32_t astro_world_sync(astro_world* w);
 /* world_stats (src/sim/stats.cu:102-153): runs the fixed-point reduction. */
 int32_t astro_world_stats(astro_world* w, astro_stats* out);
 /* snapshot_state_hash (src/sim/snapshot.cu:123-129): FNV-1a 64 over the
- * contract-ordered SoA [0, count) then the T, CO2, N2 values. */
+ * overrides, the contract-ordered SoA [0, count), the Taumoeba SoA, then the
+ * T, CO2, N2 values. */
 int32_t astro_world_state_hash(astro_world* w, uint64_t* out);
 /* world_set_clock with explicit rates (src/sim/step.cu:190-215). */
 int32_t astro_world_set_clock(astro_world* w, double physics_rate, double biology_rate);
@@ -237,11 +238,51 @@ int32_t astro_field_fill(astro_world* w, int32_t field, float value);
 int32_t astro_world_time(const astro_world* w, uint64_t* tick, double* sim_time_s,
                          double* physics_rate, double* biology_rate);
 
+/* ---- Taumoeba, the predator (src/sim/predation.cuh) -------------------------- *
+ * Its own SoA (FP64 kinematics in either cell layout) of desc.tau_capacity
+ * slots. Once a world holds predators, each tick runs predation_step after
+ * the slow fields and before the lifecycle (src/sim/step.cu:172-175): hunt
+ * on the cell hash with lowest-id claims, engulf, N2 death, division with a
+ * mutated tolerance, and, with desc.tau_compaction_enabled, compaction. */
+/* TaumoebaStore's arrays in the reference's order (src/sim/predation.cuh:85-104,
+ * serialized in that order by src/sim/snapshot.cu:59-67). Each pointer
+ * addresses n elements. */
+typedef struct astro_tau_soa {
+    uint64_t* id;
+    uint32_t* flags;
+    double *x, *y, *z;
+    double *vx, *vy, *vz;
+    float *dir_x, *dir_y, *dir_z;
+    double*   biomass;
+    double*   prey_biomass;
+    float*    digest_timer;
+    float*    tolerance;
+    uint32_t* generation;
+    uint64_t* rng_state;
+    float*    density_ema;
+    float*    run_timer;
+    int32_t*  target;
+} astro_tau_soa;
+
+/* taumoeba_spawn (src/sim/predation.cu:421-433): `count` predators placed
+ * uniformly, streams from (splitmix64(seed), id). ASTRO_CAPACITY_EXCEEDED past
+ * the store's capacity, with nothing spawned. */
+int32_t astro_taumoeba_spawn(astro_world* w, int32_t count, uint64_t seed);
+/* Slots [0, n) in; sets the count to n and next_id to max(id) + 1 (the
+ * reference's snapshot rule, src/sim/snapshot.cu:239-252). */
+int32_t astro_taumoeba_upload(astro_world* w, const astro_tau_soa* src, int32_t n);
+/* Slots [0, min(n, count)) out; *count_out = the store's count. */
+int32_t astro_taumoeba_download(astro_world* w, astro_tau_soa* dst, int32_t n, int32_t* count_out);
+int32_t astro_taumoeba_counters(astro_world* w, int32_t* count, int32_t* capacity, uint64_t* next_id);
+/* predation_step (src/sim/predation.cu:435-490) alone, on the hash of the
+ * current population. */
+int32_t astro_stage_predation(astro_world* w, double dt);
+
 /* ---- ASPH snapshots (contracts/snapshot_v1.h, src/sim/snapshot.cu) ---------- *
  * The file is the contract's: a 256-byte header, then the parameter
  * overrides that differ from canon, the cell SoA [0, count) in contract
- * order, no Taumoeba arrays, and the T, CO2, N2 values; the header's
- * state_hash is FNV-1a 64 of that body (== astro_world_state_hash), started
+ * order, the Taumoeba SoA [0, taumoeba_count), and the T, CO2, N2 values;
+ * the header's state_hash is FNV-1a 64 of that body (== astro_world_state_hash), started
  * from the contract's 1469598103934665603 (contracts/snapshot_v1.h:59). Header
  * field sizes follow the reference for its square 2D grids (n); a 3D grid
  * is written as minus its voxel count, an absent field as 0. */
@@ -258,8 +299,8 @@ int32_t astro_snapshot_save(astro_world* w, const char* path, const char* scenar
  * clock rates and overrides applied. Errors as the reference: shorter than a
  * header ASTRO_FILE_IO_ERROR; bad magic, hash mismatch (corrupt), field
  * resolution mismatch or truncated body ASTRO_INVALID_ARGUMENT; another
- * version ASTRO_CONTRACT_VERSION_MISMATCH; a Taumoeba population
- * ASTRO_UNSUPPORTED. */
+ * version ASTRO_CONTRACT_VERSION_MISMATCH. The Taumoeba next_id is
+ * max(id) + 1, as the reference recovers it. */
 int32_t astro_snapshot_from_bytes(const astro_world_desc* base, const void* data, size_t n,
                                   astro_world** out);
 /* snapshot_load (src/sim/snapshot.cu:255-264). */

@@ -9,9 +9,8 @@
  * the C ABI of astro_hot.h and never touches device memory itself.
  *
  * Not carried: the microscope view (`scope`, render-only). Not runnable here:
- * Taumoeba populations and the spin-drive flash stimulus, which this build
- * does not implement; instantiating a scenario that needs either returns
- * ASTRO_UNSUPPORTED.
+ * the spin-drive flash stimulus, which this build does not implement;
+ * instantiating a scenario that needs it returns ASTRO_UNSUPPORTED.
  */
 #ifndef ASTRO_SCENARIO_H
 #define ASTRO_SCENARIO_H

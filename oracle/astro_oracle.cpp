@@ -517,6 +517,19 @@ struct oracle_world {
 
     std::vector<double> fx, fy, fz;
     std::vector<uint64_t> co2_demand;
+
+    // TaumoebaStore (src/sim/predation.cuh:85-126)
+    struct Tau {
+        std::vector<uint64_t> id, rng_state;
+        std::vector<uint32_t> flags, generation;
+        std::vector<double> x, y, z, vx, vy, vz, biomass, prey_biomass;
+        std::vector<float> dir_x, dir_y, dir_z, digest_timer, tolerance, density_ema, run_timer;
+        std::vector<int32_t> target;
+        int32_t count = 0, capacity = 0;
+        uint64_t next_id = 1;
+    } tau;
+    std::vector<uint64_t> claim;   // World::d_predator_claim, per cell slot
+
     std::string err;
 };
 
@@ -1026,6 +1039,271 @@ int slow_fields(oracle_world* w, double dt) {                    // step.cu:168-
     return ASTRO_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Taumoeba (src/sim/predation.cu, src/sim/predation.cuh)
+// ---------------------------------------------------------------------------
+void resize_tau(oracle_world::Tau& t, int32_t cap) {
+    t.capacity = cap;
+    t.id.assign(cap, 0); t.rng_state.assign(cap, 0); t.flags.assign(cap, 0); t.generation.assign(cap, 0);
+    for (auto* v : {&t.x, &t.y, &t.z, &t.vx, &t.vy, &t.vz, &t.biomass, &t.prey_biomass}) v->assign(cap, 0.0);
+    for (auto* v : {&t.dir_x, &t.dir_y, &t.dir_z, &t.digest_timer, &t.tolerance, &t.density_ema, &t.run_timer})
+        v->assign(cap, 0.0f);
+    t.target.assign(cap, -1);
+}
+
+inline bool live_flags(uint32_t f) { return (f & ASTRO_FLAG_OCCUPIED) && (f & ASTRO_FLAG_ALIVE); }
+
+// The predator's pure helpers (src/sim/predation.cuh:24-80).
+inline double tau_drag(double t) { return 6.0 * ASTRO_PI * water_viscosity(t) * ASTRO_TAU_RADIUS; }
+inline bool tau_overlaps_cell(V3 tau, V3 cell) {
+    const double r = ASTRO_TAU_RADIUS + ASTRO_CELL_RADIUS;
+    const V3 d{tau.x - cell.x, tau.y - cell.y, tau.z - cell.z};
+    return dot(d, d) < r * r;
+}
+inline bool tau_should_tumble(float signal, float& ema, float run_timer, double dt) {
+    const double alpha = -std::expm1(-dt / ASTRO_TAXIS_MEMORY_TIME);
+    const double delta = static_cast<double>(signal) - static_cast<double>(ema);
+    ema = static_cast<float>(ema + alpha * delta);
+    return !(delta > 0.0 && run_timer < ASTRO_TAXIS_RUN_MAX);
+}
+inline double tau_n2_hazard(double n2_local, double tolerance) {
+    const double threshold = ASTRO_TAU_N2_LETHAL_CONC * (1.0 + tolerance * ASTRO_TAU_N2_TOLERANCE_K);
+    const double h = n2_local - threshold;
+    return h > 0.0 ? h : 0.0;
+}
+inline double tau_n2_death_prob(double hazard, double dt_bio) {
+    return -std::expm1(-hazard * ASTRO_TAU_N2_HAZARD_RATE * dt_bio);
+}
+inline float tau_daughter_tolerance(float parent, double mutation) {
+    const double v = static_cast<double>(parent) + ASTRO_TAU_MUTATION_SIGMA * mutation;
+    return static_cast<float>(v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v));
+}
+
+// gaussian3 (src/core/rng.cuh:111-117) then normalize (src/core/vec.cuh:32-35).
+inline V3 random_heading(Pcg& r) {
+    double a, b, c, d;
+    gaussian_pair(r, a, b);
+    gaussian_pair(r, c, d);
+    return normalize(V3{a, b, c});
+}
+
+// spawn_kernel (src/sim/predation.cu:33-76).
+void tau_spawn(oracle_world* w, int32_t count, uint64_t seed) {
+    oracle_world::Tau& t = w->tau;
+    const astro_world_desc& d = w->desc;
+    for (int32_t i = 0; i < count; ++i) {
+        const int32_t s = t.count + i;
+        const uint64_t id = t.next_id + static_cast<uint64_t>(i);
+        Pcg r = cell_rng(cell_rng_init(splitmix64(seed), id), id);
+        const double a = ASTRO_TAU_RADIUS;
+        t.x[s] = uniform_range(r, -0.5 * d.chamber_w + a, 0.5 * d.chamber_w - a);
+        t.y[s] = uniform_range(r, -0.5 * d.chamber_h + a, 0.5 * d.chamber_h - a);
+        t.z[s] = uniform_range(r, -0.5 * d.chamber_d + a, 0.5 * d.chamber_d - a);
+        t.vx[s] = 0.0; t.vy[s] = 0.0; t.vz[s] = 0.0;
+        const V3 dir = random_heading(r);
+        t.dir_x[s] = static_cast<float>(dir.x);
+        t.dir_y[s] = static_cast<float>(dir.y);
+        t.dir_z[s] = static_cast<float>(dir.z);
+        t.id[s] = id;
+        t.flags[s] = ASTRO_FLAG_OCCUPIED | ASTRO_FLAG_ALIVE;
+        t.biomass[s] = ASTRO_TAU_MASS_DRY;
+        t.prey_biomass[s] = 0.0;
+        t.digest_timer[s] = 0.0f;
+        t.tolerance[s] = static_cast<float>(ASTRO_TAU_N2_TOLERANCE_INIT);
+        t.generation[s] = 0u;
+        t.rng_state[s] = r.state;
+        t.density_ema[s] = 0.0f;
+        t.run_timer[s] = 0.0f;
+        t.target[s] = -1;
+    }
+    t.count += count;
+    t.next_id += static_cast<uint64_t>(count);
+}
+
+template <class T>
+void tau_compact_array(std::vector<T>& a, const std::vector<int32_t>& src, int32_t live) {
+    std::vector<T> out(live);
+    for (int32_t k = 0; k < live; ++k) out[k] = a[src[k]];
+    std::copy(out.begin(), out.end(), a.begin());
+}
+
+// predation_step (src/sim/predation.cu:435-490).
+void predation_step(oracle_world* w, double dt) {
+    oracle_world::Tau& t = w->tau;
+    const int32_t n = t.count;
+    if (n <= 0) return;
+    const astro_world_desc& d = w->desc;
+    const double dt_bio = dt * w->biology_rate;
+    std::fill(w->claim.begin(), w->claim.end(), ~0ull);
+
+    // hunt_kernel (:78-159)
+    for (int32_t i = 0; i < n; ++i) {
+        if (!live_flags(t.flags[i])) continue;
+        V3 pos{t.x[i], t.y[i], t.z[i]};
+        const bool digesting = t.digest_timer[i] > 0.0f;
+        int prey_count = 0;
+        int32_t prey = -1;
+        for_each_neighbour(w, pos.x, pos.y, pos.z, [&](int32_t j) {
+            if (!live_flags(w->flags[j])) return;
+            ++prey_count;
+            if (prey < 0 && tau_overlaps_cell(pos, pos_of(w, j))) prey = j;
+        });
+        if (!digesting && prey >= 0) {
+            t.target[i] = prey;
+            w->claim[prey] = std::min(w->claim[prey], t.id[i]);
+        } else {
+            t.target[i] = -1;
+        }
+        Pcg rng = cell_rng(t.rng_state[i], t.id[i]);
+        float ema = t.density_ema[i];
+        float run = t.run_timer[i];
+        V3 dir{t.dir_x[i], t.dir_y[i], t.dir_z[i]};
+        if (tau_should_tumble(static_cast<float>(prey_count), ema, run, dt)) {
+            dir = random_heading(rng);
+            run = 0.0f;
+        } else {
+            run += static_cast<float>(dt);
+        }
+        t.density_ema[i] = ema;
+        t.run_timer[i] = run;
+        t.dir_x[i] = static_cast<float>(dir.x);
+        t.dir_y[i] = static_cast<float>(dir.y);
+        t.dir_z[i] = static_cast<float>(dir.z);
+        // integrate_cell (integrator.cuh:154-166) with the crawl thrust
+        V3 vel{t.vx[i], t.vy[i], t.vz[i]};
+        const double t_amb = d.ambient_temp;
+        const double gamma = tau_drag(t_amb);
+        const Ou c = ou_coefficients(ASTRO_TAU_MASS, gamma, t_amb, dt);
+        double z1, z2, z3, z4, z5, z6;
+        gaussian_pair(rng, z1, z2);
+        gaussian_pair(rng, z3, z4);
+        gaussian_pair(rng, z5, z6);
+        const bool noise = d.thermal_noise != 0;
+        ou_advance_axis(pos.x, vel.x, dir.x * ASTRO_TAU_CRAWL_THRUST, gamma, c, dt, noise, z1, z2);
+        ou_advance_axis(pos.y, vel.y, dir.y * ASTRO_TAU_CRAWL_THRUST, gamma, c, dt, noise, z3, z4);
+        ou_advance_axis(pos.z, vel.z, dir.z * ASTRO_TAU_CRAWL_THRUST, gamma, c, dt, noise, z5, z6);
+        t.rng_state[i] = rng.state;
+        const double a = ASTRO_TAU_RADIUS;
+        apply_boundary_axis(pos.x, vel.x, 0.5 * d.chamber_w, a, d.boundary_x);
+        apply_boundary_axis(pos.y, vel.y, 0.5 * d.chamber_h, a, d.boundary_y);
+        apply_boundary_axis(pos.z, vel.z, 0.5 * d.chamber_d, a, ASTRO_BOUNDARY_REFLECTING);
+        t.x[i] = pos.x; t.y[i] = pos.y; t.z[i] = pos.z;
+        t.vx[i] = vel.x; t.vy[i] = vel.y; t.vz[i] = vel.z;
+        if (digesting) {
+            float dg = t.digest_timer[i] - static_cast<float>(dt_bio);
+            if (dg <= 0.0f) {
+                t.biomass[i] += t.prey_biomass[i] * ASTRO_TAU_BIOMASS_YIELD;
+                t.prey_biomass[i] = 0.0;
+                dg = 0.0f;
+            }
+            t.digest_timer[i] = dg;
+        }
+    }
+
+    // resolve_kernel (:163-180)
+    for (int32_t i = 0; i < n; ++i) {
+        const int32_t prey = t.target[i];
+        if (prey < 0 || w->claim[prey] != t.id[i]) continue;
+        const uint32_t cf = w->flags[prey];
+        if (!live_flags(cf)) continue;
+        w->flags[prey] = cf & ~ASTRO_FLAG_ALIVE;
+        w->death_cause[prey] = ASTRO_DEATH_PREDATED;
+        w->emit_power[prey] = 0.0f;
+        w->energy[prey] = d.store_disposition == ASTRO_STORE_RETAIN ? w->energy[prey] : 0.0;   // corpse_energy
+        t.digest_timer[i] = static_cast<float>(ASTRO_TAU_DIGEST_TIME);
+        t.prey_biomass[i] = w->biomass[prey];
+    }
+
+    // nitrogen_kernel (:187-202)
+    const Grid& N = w->f[ASTRO_FIELD_N2];
+    for (int32_t i = 0; i < n; ++i) {
+        const uint32_t f = t.flags[i];
+        if (!live_flags(f)) continue;
+        Pcg rng = cell_rng(t.rng_state[i], t.id[i]);
+        const double u = uniform01d(rng);
+        t.rng_state[i] = rng.state;
+        const double n2 = N.enabled ? static_cast<double>(grid_sample(N, static_cast<float>(t.x[i]),
+                                                                       static_cast<float>(t.y[i]),
+                                                                       static_cast<float>(t.z[i])))
+                                    : 0.0;
+        const double hazard = tau_n2_hazard(n2, static_cast<double>(t.tolerance[i]));
+        if (u < tau_n2_death_prob(hazard, dt_bio)) t.flags[i] = f & ~ASTRO_FLAG_ALIVE;
+    }
+
+    // tau_mark_kernel, scan and tau_divide_kernel (:208-280, :463-481)
+    std::vector<int32_t> prefix(n);
+    int32_t births = 0, deads = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        const uint32_t f = t.flags[i];
+        const bool occ = (f & ASTRO_FLAG_OCCUPIED) != 0u, alive = (f & ASTRO_FLAG_ALIVE) != 0u;
+        prefix[i] = births;
+        if (occ && alive && t.biomass[i] >= ASTRO_TAU_DIVIDE_BIOMASS) ++births;
+        if (occ && !alive) ++deads;
+    }
+    if (births > 0) {
+        if (n + births > t.capacity) births = t.capacity - n;
+        for (int32_t i = 0; births > 0 && i < n; ++i) {
+            if (!live_flags(t.flags[i]) || !(t.biomass[i] >= ASTRO_TAU_DIVIDE_BIOMASS)) continue;
+            const int32_t s = n + prefix[i];
+            if (s >= t.capacity) continue;
+            const uint64_t did = t.next_id + static_cast<uint64_t>(prefix[i]);
+            const double half = 0.5 * t.biomass[i];
+            t.biomass[i] = half;
+            t.biomass[s] = half;
+            Pcg dr = cell_rng(pcg_split(t.rng_state[i], did), did);
+            double mutation, unused;
+            gaussian_pair(dr, mutation, unused);
+            t.tolerance[s] = tau_daughter_tolerance(t.tolerance[i], mutation);
+            t.rng_state[s] = dr.state;
+            t.generation[s] = t.generation[i] + 1u;
+            double ax, ay, az;
+            division_axis(did, ax, ay, az);
+            const double a = ASTRO_TAU_RADIUS;
+            t.x[s] = t.x[i] + a * ax; t.x[i] -= a * ax;
+            t.y[s] = t.y[i] + a * ay; t.y[i] -= a * ay;
+            t.z[s] = t.z[i] + a * az; t.z[i] -= a * az;
+            const double hx = 0.5 * d.chamber_w - a, hy = 0.5 * d.chamber_h - a, hz = 0.5 * d.chamber_d - a;
+            auto cl = [](double v, double hh) { return v < -hh ? -hh : (v > hh ? hh : v); };
+            t.x[s] = cl(t.x[s], hx); t.y[s] = cl(t.y[s], hy); t.z[s] = cl(t.z[s], hz);
+            t.x[i] = cl(t.x[i], hx); t.y[i] = cl(t.y[i], hy); t.z[i] = cl(t.z[i], hz);
+            t.id[s] = did;
+            t.flags[s] = t.flags[i];
+            t.vx[s] = t.vx[i]; t.vy[s] = t.vy[i]; t.vz[s] = t.vz[i];
+            t.dir_x[s] = t.dir_x[i]; t.dir_y[s] = t.dir_y[i]; t.dir_z[s] = t.dir_z[i];
+            t.prey_biomass[s] = 0.0;
+            t.digest_timer[s] = 0.0f;
+            t.density_ema[s] = 0.0f;
+            t.run_timer[s] = 0.0f;
+            t.target[s] = -1;
+        }
+        if (births > 0) {
+            t.next_id += static_cast<uint64_t>(births);
+            t.count += births;
+        }
+    }
+
+    // taumoeba_store_compact (:492-527)
+    if (d.tau_compaction_enabled && deads > 0) {
+        const int32_t cnt = t.count;
+        std::vector<int32_t> src;
+        for (int32_t i = 0; i < cnt; ++i)
+            if (live_flags(t.flags[i])) src.push_back(i);
+        const int32_t live = static_cast<int32_t>(src.size());
+        if (live < cnt) {
+            tau_compact_array(t.id, src, live); tau_compact_array(t.flags, src, live);
+            tau_compact_array(t.x, src, live); tau_compact_array(t.y, src, live); tau_compact_array(t.z, src, live);
+            tau_compact_array(t.vx, src, live); tau_compact_array(t.vy, src, live); tau_compact_array(t.vz, src, live);
+            tau_compact_array(t.dir_x, src, live); tau_compact_array(t.dir_y, src, live);
+            tau_compact_array(t.dir_z, src, live); tau_compact_array(t.biomass, src, live);
+            tau_compact_array(t.prey_biomass, src, live); tau_compact_array(t.digest_timer, src, live);
+            tau_compact_array(t.tolerance, src, live); tau_compact_array(t.generation, src, live);
+            tau_compact_array(t.rng_state, src, live); tau_compact_array(t.density_ema, src, live);
+            tau_compact_array(t.run_timer, src, live); tau_compact_array(t.target, src, live);
+            t.count = live;
+        }
+    }
+}
+
 // world_step (src/sim/step.cu:109-184).
 int world_step_once(oracle_world* w) {
     const double dt = ASTRO_DT_PHYSICS * w->physics_rate;
@@ -1040,6 +1318,7 @@ int world_step_once(oracle_world* w) {
     forces(w);
     integrate(w, dt);
     if ((st = slow_fields(w, dt)) != ASTRO_OK) return st;
+    predation_step(w, dt);
     if (d.lifecycle_enabled) lifecycle_step(w, dt);
     ++w->tick;
     w->sim_time += dt;
@@ -1111,6 +1390,9 @@ int32_t oracle_world_create(const astro_world_desc* din, oracle_world** out) {
     if (cap <= 0 || cap > ASTRO_MAX_CELLS) return ASTRO_INVALID_ARGUMENT;
     if (!(d.chamber_w > 0 && d.chamber_h > 0 && d.chamber_d > 0)) return ASTRO_INVALID_ARGUMENT;
     d.capacity = cap;
+    const int32_t tau_cap = d.tau_capacity > 0 ? d.tau_capacity : ASTRO_DEFAULT_TAUMOEBA;   // step.cu:50
+    if (tau_cap > ASTRO_MAX_TAUMOEBA) return ASTRO_INVALID_ARGUMENT;
+    d.tau_capacity = tau_cap;
     if (d.division_offset <= 0.0) d.division_offset = ASTRO_CELL_RADIUS;
     const int defaults[4] = {ASTRO_FIELD_N_TEMP, ASTRO_FIELD_N_CO2, ASTRO_FIELD_N_N2, ASTRO_FIELD_N_IRRAD};
     for (int f = 0; f < 4; ++f) {
@@ -1120,6 +1402,8 @@ int32_t oracle_world_create(const astro_world_desc* din, oracle_world** out) {
         if (g.nx <= 1 || g.ny <= 1 || g.nz < 1) return ASTRO_INVALID_ARGUMENT;
     }
     auto* w = new oracle_world();
+    resize_tau(w->tau, tau_cap);
+    w->claim.assign(cap, ~0ull);
     w->desc = d;
     w->capacity = cap;
     w->physics_rate = d.physics_rate > 0 ? d.physics_rate : 1.0;
@@ -1434,6 +1718,62 @@ int32_t oracle_world_time(const oracle_world* w, uint64_t* tick, double* sim_tim
     return ASTRO_OK;
 }
 
+// taumoeba_spawn (src/sim/predation.cu:421-433).
+int32_t oracle_taumoeba_spawn(oracle_world* w, int32_t count, uint64_t seed) {
+    if (!w || count < 0) return ASTRO_INVALID_ARGUMENT;
+    if (count == 0) return ASTRO_OK;
+    if (w->tau.count + count > w->tau.capacity)
+        return fail(w, ASTRO_CAPACITY_EXCEEDED, "taumoeba spawn exceeds capacity");
+    tau_spawn(w, count, seed);
+    return ASTRO_OK;
+}
+
+int32_t oracle_taumoeba_upload(oracle_world* w, const astro_tau_soa* s, int32_t n) {
+    if (!w || !s || n < 0) return ASTRO_INVALID_ARGUMENT;
+    if (n > w->tau.capacity) return fail(w, ASTRO_CAPACITY_EXCEEDED, "taumoeba upload exceeds capacity");
+    oracle_world::Tau& t = w->tau;
+    put(t.id, s->id, n); put(t.flags, s->flags, n); put(t.x, s->x, n); put(t.y, s->y, n); put(t.z, s->z, n);
+    put(t.vx, s->vx, n); put(t.vy, s->vy, n); put(t.vz, s->vz, n); put(t.dir_x, s->dir_x, n);
+    put(t.dir_y, s->dir_y, n); put(t.dir_z, s->dir_z, n); put(t.biomass, s->biomass, n);
+    put(t.prey_biomass, s->prey_biomass, n); put(t.digest_timer, s->digest_timer, n);
+    put(t.tolerance, s->tolerance, n); put(t.generation, s->generation, n); put(t.rng_state, s->rng_state, n);
+    put(t.density_ema, s->density_ema, n); put(t.run_timer, s->run_timer, n); put(t.target, s->target, n);
+    t.count = n;
+    uint64_t mx = 0;   // src/sim/snapshot.cu:239-252
+    for (int32_t i = 0; i < n; ++i) mx = std::max(mx, s->id[i]);
+    t.next_id = mx + 1;
+    return ASTRO_OK;
+}
+
+int32_t oracle_taumoeba_download(oracle_world* w, astro_tau_soa* s, int32_t n, int32_t* count_out) {
+    if (!w || !s || n < 0) return ASTRO_INVALID_ARGUMENT;
+    const oracle_world::Tau& t = w->tau;
+    if (count_out) *count_out = t.count;
+    const int32_t m = std::min(n, t.count);
+    get(t.id, s->id, m); get(t.flags, s->flags, m); get(t.x, s->x, m); get(t.y, s->y, m); get(t.z, s->z, m);
+    get(t.vx, s->vx, m); get(t.vy, s->vy, m); get(t.vz, s->vz, m); get(t.dir_x, s->dir_x, m);
+    get(t.dir_y, s->dir_y, m); get(t.dir_z, s->dir_z, m); get(t.biomass, s->biomass, m);
+    get(t.prey_biomass, s->prey_biomass, m); get(t.digest_timer, s->digest_timer, m);
+    get(t.tolerance, s->tolerance, m); get(t.generation, s->generation, m); get(t.rng_state, s->rng_state, m);
+    get(t.density_ema, s->density_ema, m); get(t.run_timer, s->run_timer, m); get(t.target, s->target, m);
+    return ASTRO_OK;
+}
+
+int32_t oracle_taumoeba_counters(oracle_world* w, int32_t* count, int32_t* capacity, uint64_t* next_id) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    if (count) *count = w->tau.count;
+    if (capacity) *capacity = w->tau.capacity;
+    if (next_id) *next_id = w->tau.next_id;
+    return ASTRO_OK;
+}
+
+int32_t oracle_stage_predation(oracle_world* w, double dt) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    if (!w->hash_valid) hash_build(w);
+    predation_step(w, dt);
+    return ASTRO_OK;
+}
+
 int32_t oracle_world_set_clock(oracle_world* w, double physics, double biology) {
     if (!w) return ASTRO_INVALID_ARGUMENT;
     w->physics_rate = physics < ASTRO_CLOCK_PHYSICS_MIN ? ASTRO_CLOCK_PHYSICS_MIN
@@ -1448,6 +1788,14 @@ int32_t oracle_world_set_clock(oracle_world* w, double physics, double biology) 
 // stats_step + world_stats (src/sim/stats.cu:27-153).
 int32_t oracle_world_stats(oracle_world* w, astro_stats* s) {
     if (!w || !s) return ASTRO_INVALID_ARGUMENT;
+    // stats_taumoeba_kernel (src/sim/stats.cu:65-74)
+    uint32_t n_tau = 0;
+    uint64_t tau_tol = 0;
+    for (int32_t i = 0; i < w->tau.count; ++i) {
+        if (!live_flags(w->tau.flags[i])) continue;
+        ++n_tau;
+        tau_tol += static_cast<uint64_t>(to_fixed(static_cast<double>(w->tau.tolerance[i]), ASTRO_DEPOSIT_SCALE_STATS));
+    }
     uint64_t energy = 0, temp_cell = 0, temp_medium = 0, co2 = 0, n2 = 0, max_temp = 0;
     uint32_t n_alive = 0, n_dead = 0, n_awake = 0, n_dormant = 0;
     for (int32_t i = 0; i < w->count; ++i) {
@@ -1482,6 +1830,8 @@ int32_t oracle_world_stats(oracle_world* w, astro_stats* s) {
     s->n_dead = static_cast<int32_t>(n_dead);
     s->n_awake = static_cast<int32_t>(n_awake);
     s->n_dormant = static_cast<int32_t>(n_dormant);
+    s->n_taumoeba = static_cast<int32_t>(n_tau);
+    s->mean_tau_tolerance = n_tau > 0u ? from_fixed(tau_tol, ASTRO_DEPOSIT_SCALE_STATS) / n_tau : 0.0;   // stats.cu:118-120
     s->total_energy_j = from_fixed(energy, ASTRO_DEPOSIT_SCALE_STATS);
     s->mean_charge = (n_alive + n_dead) > 0u ? s->total_energy_j / ((n_alive + n_dead) * ASTRO_CELL_ENERGY_MAX) : 0.0;
     s->mean_temp_cell_k = n_alive > 0u ? from_fixed(temp_cell, ASTRO_DEPOSIT_SCALE_STATS) / n_alive : 0.0;
@@ -1529,6 +1879,16 @@ int32_t oracle_world_state_hash(oracle_world* w, uint64_t* out) {
     span(w->biomass); span(w->co2_held); span(w->age_s);
     span(w->rng_state); span(w->taxis_memory); span(w->run_timer);
     span(w->t_local); span(w->irradiance); span(w->co2_local); span(w->n2_local);
+    // collect_tau_spans (src/sim/snapshot.cu:59-67)
+    const oracle_world::Tau& t = w->tau;
+    const size_t nt = static_cast<size_t>(t.count);
+    auto tspan = [&](const auto& v) {
+        if (nt > 0) h = fnv1a64(v.data(), sizeof(v[0]) * nt, h);
+    };
+    tspan(t.id); tspan(t.flags); tspan(t.x); tspan(t.y); tspan(t.z); tspan(t.vx); tspan(t.vy); tspan(t.vz);
+    tspan(t.dir_x); tspan(t.dir_y); tspan(t.dir_z); tspan(t.biomass); tspan(t.prey_biomass);
+    tspan(t.digest_timer); tspan(t.tolerance); tspan(t.generation); tspan(t.rng_state);
+    tspan(t.density_ema); tspan(t.run_timer); tspan(t.target);
     for (int f : {ASTRO_FIELD_TEMPERATURE, ASTRO_FIELD_CO2, ASTRO_FIELD_N2}) {
         const Grid& g = w->f[f];
         if (g.enabled && g.count() > 0) h = fnv1a64(g.value.data(), sizeof(float) * g.count(), h);
@@ -1646,6 +2006,16 @@ void oracle_gaussian_pair(uint64_t* st, uint64_t inc, double* z0, double* z1) {
     *st = r.state;
 }
 void oracle_division_axis(uint64_t did, double* ax, double* ay, double* az) { division_axis(did, *ax, *ay, *az); }
+double oracle_tau_drag(double t) { return tau_drag(t); }
+int32_t oracle_tau_overlaps_cell(const double a[3], const double b[3]) {
+    return tau_overlaps_cell(V3{a[0], a[1], a[2]}, V3{b[0], b[1], b[2]}) ? 1 : 0;
+}
+int32_t oracle_tau_should_tumble(float signal, float* ema, float run, double dt) {
+    return tau_should_tumble(signal, *ema, run, dt) ? 1 : 0;
+}
+double oracle_tau_n2_hazard(double n2, double tol) { return tau_n2_hazard(n2, tol); }
+double oracle_tau_n2_death_prob(double hazard, double dt_bio) { return tau_n2_death_prob(hazard, dt_bio); }
+float oracle_tau_daughter_tolerance(float parent, double mutation) { return tau_daughter_tolerance(parent, mutation); }
 double oracle_ou_position_shape(double x) { return ou_position_shape(x); }
 void oracle_ou_coefficients(double m, double g, double t, double dt, double o[6]) {
     const Ou c = ou_coefficients(m, g, t, dt);

@@ -53,6 +53,11 @@ int32_t oracle_world_apply_brush(oracle_world* w, int32_t kind, double x, double
 int32_t oracle_field_fill(oracle_world* w, int32_t field, float value);
 int32_t oracle_world_time(const oracle_world* w, uint64_t* tick, double* sim_time_s, double* physics_rate,
                           double* biology_rate);
+int32_t oracle_taumoeba_spawn(oracle_world* w, int32_t count, uint64_t seed);
+int32_t oracle_taumoeba_upload(oracle_world* w, const astro_tau_soa* src, int32_t n);
+int32_t oracle_taumoeba_download(oracle_world* w, astro_tau_soa* dst, int32_t n, int32_t* count_out);
+int32_t oracle_taumoeba_counters(oracle_world* w, int32_t* count, int32_t* capacity, uint64_t* next_id);
+int32_t oracle_stage_predation(oracle_world* w, double dt);
 
 int32_t oracle_stage_hash_build(oracle_world* w);
 int32_t oracle_hash_dims(const oracle_world* w, int32_t* nx, int32_t* ny, int32_t* nz,
@@ -81,6 +86,12 @@ uint64_t oracle_pcg_split(uint64_t parent_state, uint64_t daughter_id);
 double   oracle_uniform01d(uint64_t* state, uint64_t inc);
 void     oracle_gaussian_pair(uint64_t* state, uint64_t inc, double* z0, double* z1);
 void     oracle_division_axis(uint64_t daughter_id, double* ax, double* ay, double* az);
+double   oracle_tau_drag(double t_kelvin);
+int32_t  oracle_tau_overlaps_cell(const double tau[3], const double cell[3]);
+int32_t  oracle_tau_should_tumble(float signal, float* ema, float run_timer, double dt);
+double   oracle_tau_n2_hazard(double n2_local, double tolerance);
+double   oracle_tau_n2_death_prob(double hazard, double dt_bio);
+float    oracle_tau_daughter_tolerance(float parent_tolerance, double mutation);
 double   oracle_ou_position_shape(double x);
 void     oracle_ou_coefficients(double mass, double gamma, double t_kelvin, double dt,
                                 double out6[6]);

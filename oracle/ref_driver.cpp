@@ -19,6 +19,7 @@
 #include "sim/hash.cuh"
 #include "sim/integrator.cuh"
 #include "sim/lifecycle.cuh"
+#include "sim/predation.cuh"
 #include "sim/taxis.cuh"
 #include "sim/thermal.cuh"
 
@@ -110,6 +111,16 @@ double ref_taxis_emit_power(int32_t s, double e, double dt, double mx) {
 int64_t ref_to_fixed(double v, double s) { return to_fixed(v, s); }
 double ref_from_fixed(uint64_t raw, double s) { return from_fixed(raw, s); }
 uint64_t ref_fnv1a64(const void* d, size_t n, uint64_t h) { return contract::fnv1a64(d, n, h); }
+double ref_tau_drag(double t) { return tau_drag(t); }
+int32_t ref_tau_overlaps_cell(const double a[3], const double b[3]) {
+    return tau_overlaps_cell(Vec3{a[0], a[1], a[2]}, Vec3{b[0], b[1], b[2]}) ? 1 : 0;
+}
+int32_t ref_tau_should_tumble(float signal, float* ema, float run, double dt) {
+    return tau_should_tumble(signal, *ema, run, dt) ? 1 : 0;
+}
+double ref_tau_n2_hazard(double n2, double tol) { return tau_n2_hazard(n2, tol); }
+double ref_tau_n2_death_prob(double hazard, double dt_bio) { return tau_n2_death_prob(hazard, dt_bio); }
+float ref_tau_daughter_tolerance(float parent, double mutation) { return tau_daughter_tolerance(parent, mutation); }
 
 // The ASPH layout (contracts/snapshot_v1.h:25-55): header and override sizes,
 // the offset of every header field in declaration order, the magic, the
@@ -205,6 +216,15 @@ int32_t ref_constant(const char* name, double* out) {
         {"FIELD_N_N2", canon::FIELD_N_N2}, {"FIELD_N_IRRAD", canon::FIELD_N_IRRAD},
         {"MAX_CELLS", canon::MAX_CELLS}, {"DEFAULT_CELLS", canon::DEFAULT_CELLS},
         {"MAX_TAUMOEBA", canon::MAX_TAUMOEBA}, {"DEFAULT_TAUMOEBA", canon::DEFAULT_TAUMOEBA},
+        {"TAU_DIAMETER", canon::TAU_DIAMETER}, {"TAU_RADIUS", canon::TAU_RADIUS},
+        {"TAU_VOLUME", canon::TAU_VOLUME}, {"TAU_MASS", canon::TAU_MASS}, {"TAU_MASS_DRY", canon::TAU_MASS_DRY},
+        {"TAU_CRAWL_SPEED", canon::TAU_CRAWL_SPEED}, {"TAU_DRAG_20C", canon::TAU_DRAG_20C},
+        {"TAU_CRAWL_THRUST", canon::TAU_CRAWL_THRUST}, {"TAU_DIGEST_TIME", canon::TAU_DIGEST_TIME},
+        {"TAU_BIOMASS_YIELD", canon::TAU_BIOMASS_YIELD}, {"TAU_DIVIDE_BIOMASS", canon::TAU_DIVIDE_BIOMASS},
+        {"TAU_N2_LETHAL_CONC", canon::TAU_N2_LETHAL_CONC}, {"TAU_N2_LETHAL_TIME", canon::TAU_N2_LETHAL_TIME},
+        {"TAU_N2_HAZARD_RATE", canon::TAU_N2_HAZARD_RATE},
+        {"TAU_N2_TOLERANCE_INIT", canon::TAU_N2_TOLERANCE_INIT},
+        {"TAU_N2_TOLERANCE_K", canon::TAU_N2_TOLERANCE_K}, {"TAU_MUTATION_SIGMA", canon::TAU_MUTATION_SIGMA},
         {"DEPOSIT_SCALE_TEMPERATURE", contract::DEPOSIT_SCALE_TEMPERATURE},
         {"DEPOSIT_SCALE_CO2", contract::DEPOSIT_SCALE_CO2},
         {"DEPOSIT_SCALE_N2", contract::DEPOSIT_SCALE_N2},

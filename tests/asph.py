@@ -3,7 +3,8 @@
 Layout (contracts/snapshot_v1.h:14-57): a 256-byte little-endian header, then
 `override_count` ParamOverride records ({char key[48]; double value}), the
 cell SoA [0, count) array by array in contract order
-(contracts/cell_store_v1.h:42-78), the Taumoeba arrays (none here), and the
+(contracts/cell_store_v1.h:42-78), the Taumoeba arrays
+(src/sim/snapshot.cu:59-67), and the
 T, CO2, N2 values as float32. state_hash is FNV-1a 64 over everything after
 the header. The header format below is checked field by field against the
 reference's own struct in tests/test_snapshot_format.py.
@@ -60,9 +61,10 @@ def field_values(n: int) -> int:
     return n * n if n > 0 else -n
 
 
-def parse(data: bytes, cell_fields) -> dict:
-    """Header, overrides, cell arrays and field values of a snapshot.
-    `cell_fields` is [(name, numpy dtype)] in contract order."""
+def parse(data: bytes, cell_fields, tau_fields=None) -> dict:
+    """Header, overrides, cell arrays, Taumoeba arrays and field values of a
+    snapshot. `cell_fields` and `tau_fields` are [(name, numpy dtype)] in
+    the file's order."""
     if len(data) < HEADER_SIZE:
         raise ValueError("shorter than a header")
     vals = struct.unpack_from(HEADER_FMT, data, 0)
@@ -82,8 +84,14 @@ def parse(data: bytes, cell_fields) -> dict:
         a = np.frombuffer(data, dtype=np.dtype(dt).newbyteorder("<"), count=n, offset=off)
         cells[name] = a.copy()
         off += a.nbytes
+    tau = {}
     if h["taumoeba_count"] != 0:
-        raise ValueError("Taumoeba arrays are not read here")
+        if tau_fields is None:
+            raise ValueError("the snapshot holds Taumoeba: pass their fields")
+        for name, dt in tau_fields:
+            a = np.frombuffer(data, dtype=np.dtype(dt).newbyteorder("<"), count=h["taumoeba_count"], offset=off)
+            tau[name] = a.copy()
+            off += a.nbytes
     fields = []
     for key in ("n_temp", "n_co2", "n_n2"):
         m = field_values(h[key])
@@ -91,7 +99,7 @@ def parse(data: bytes, cell_fields) -> dict:
         off += 4 * m
     if off != len(data):
         raise ValueError(f"{len(data) - off} trailing bytes")
-    return dict(header=h, overrides=overrides, cells=cells, fields=fields)
+    return dict(header=h, overrides=overrides, cells=cells, tau=tau, fields=fields)
 
 
 def body_of(overrides: dict[str, float], cells: dict, cell_fields, fields) -> bytes:

@@ -165,6 +165,42 @@ def ref_vectors(lib) -> dict:
         p3 = [float(rng.uniform(-0.6, 0.6) * n * cell) for n in (nx, ny, nz)]
         keys.append([nx, ny, nz, hx(cell), *(hx(x) for x in p3), int(hk(nx, ny, nz, cell, *p3))])
     v["hash_key"] = keys
+    # the predator's helpers (src/sim/predation.cuh:24-80)
+    td = fn(lib, "ref_tau_drag", D, [D])
+    v["tau_drag"] = [[hx(t), hx(td(float(t)))] for t in list(rng.uniform(273, 400, 12)) + [293.15]]
+    to = fn(lib, "ref_tau_overlaps_cell", I, [D * 3, D * 3])
+    reach = 2.0000000000000002e-05 + 5.0000000000000004e-06
+    ov = []
+    for k in range(24):
+        a3 = rng.uniform(-1e-3, 1e-3, 3)
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        dist = reach * (1.0 + (k - 12) * 1e-15) if k < 24 else reach
+        b3 = a3 + u * (dist if k % 3 else float(rng.uniform(0, 2 * reach)))
+        ov.append([[hx(x) for x in a3], [hx(x) for x in b3], int(to((D * 3)(*a3), (D * 3)(*b3)))])
+    v["tau_overlaps_cell"] = ov
+    tt_ = fn(lib, "ref_tau_should_tumble", I, [F, C.POINTER(F), F, D])
+    tum2 = []
+    for _ in range(24):
+        sig_ = float(np.float32(rng.integers(0, 40)))
+        e0 = float(np.float32(rng.uniform(0, 40)))
+        run = float(np.float32(rng.choice([0.0, 0.1, 0.39, 0.4, 0.41])))
+        dt = float(rng.choice([1e-4, 1e-3, 1e-2]))
+        e = F(e0)
+        r = tt_(sig_, C.byref(e), run, dt)
+        tum2.append([hx(sig_), hx(e0), hx(run), hx(dt), int(r), hx(e.value)])
+    v["tau_should_tumble"] = tum2
+    th = fn(lib, "ref_tau_n2_hazard", D, [D, D])
+    v["tau_n2_hazard"] = [[hx(n), hx(t), hx(th(float(n), float(t)))]
+                          for n, t in zip(rng.uniform(0, 0.03, 16), rng.uniform(0, 1, 16))]
+    tp = fn(lib, "ref_tau_n2_death_prob", D, [D, D])
+    v["tau_n2_death_prob"] = [[hx(h), hx(d), hx(tp(float(h), float(d)))]
+                              for h, d in zip(list(rng.uniform(0, 0.02, 14)) + [0.0, 1e-9],
+                                              list(rng.uniform(0, 500, 14)) + [1.0, 0.5])]
+    tdt = fn(lib, "ref_tau_daughter_tolerance", F, [F, D])
+    v["tau_daughter_tolerance"] = [[hx(float(np.float32(p_))), hx(m), hx(tdt(float(np.float32(p_)), float(m)))]
+                                   for p_, m in zip(list(rng.uniform(0, 1, 14)) + [0.0, 1.0],
+                                                    list(rng.normal(0, 3, 14)) + [-60.0, 60.0])]
     return v
 
 

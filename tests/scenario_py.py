@@ -131,13 +131,19 @@ def capacity(s: dict) -> int:
     return int(min(max(cells * 2 + 4096, CANON["default_cells"]), CANON["max_cells"]))
 
 
+def tau_capacity(s: dict) -> int:
+    tau = sum(p["count"] for p in s["populations"] if p["kind"] == 1)
+    return int(min(tau * 4 + 256, 65536))
+
+
 def instantiate(binding, s: dict, **base) -> ah.World:
     """scenario_instantiate through the World's entry points."""
     w = ah.World(binding, capacity=capacity(s), chamber_w=s["chamber_w"], chamber_h=s["chamber_h"],
                  chamber_d=s["chamber_d"], seed=s["seed"], co2_init=s["co2_init"], boundary_x=s["boundary_x"],
                  boundary_y=s["boundary_y"], gravity_axis=s["gravity_axis"], ambient_temp=s["ambient_temp"],
                  store_disposition=s["store_disposition"], thermal_enabled=int(s["thermal_active"]),
-                 compaction_enabled=int(s["compaction"]), **base)
+                 compaction_enabled=int(s["compaction"]), tau_capacity=tau_capacity(s),
+                 tau_compaction_enabled=int(s["tau_compaction"]), **base)
     if s["temp_init"] != s["ambient_temp"]:
         w.fill(ah.TEMPERATURE, s["temp_init"])
     if s["n2_init"] != 0.0:
@@ -145,6 +151,10 @@ def instantiate(binding, s: dict, **base) -> ah.World:
     seed = s["seed"]
     for p in s["populations"]:
         if p["count"] <= 0:
+            continue
+        if p["kind"] == 1:
+            w.tau_spawn(p["count"], seed)
+            seed += 1
             continue
         w.spawn(p["count"], seed, placement=p["placement"], place_x=p["place_x"], place_y=p["place_y"],
                 place_radius=p["place_radius"], charge_dist=p["charge_dist"], charge_a=p["charge_a"],

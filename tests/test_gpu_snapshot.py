@@ -65,7 +65,7 @@ def test_round_trip_fidelity(gpu, tmp_path, precision):
     assert (h["magic"], h["version"]) == (asph.MAGIC, asph.VERSION)
     assert (h["seed"], h["tick"], h["sim_time_s"]) == (20260802, c0.tick, c0.sim_time_s)
     assert (h["cell_count"], h["cell_capacity"], h["next_cell_id"]) == (c0.count, 1 << 20, c0.next_id)
-    assert (h["taumoeba_count"], h["taumoeba_capacity"]) == (0, 0)
+    assert (h["taumoeba_count"], h["taumoeba_capacity"]) == (0, 256)   # DEFAULT_TAUMOEBA
     dims = [w.field_dims(f)[0] for f in (ah.TEMPERATURE, ah.CO2, ah.N2)]
     assert [h["n_temp"], h["n_co2"], h["n_n2"]] == [d[0] for d in dims]   # square 2D grids: n
     d = w.desc
@@ -177,7 +177,9 @@ def test_rejects_bad_snapshots(gpu, tmp_path):
     assert status(good, grid={**grid, 0: (64, 64, 1)}) == ah.INVALID_ARGUMENT   # field resolution
     bad = bytearray(good)
     struct.pack_into("<i", bad, off["taumoeba_count"], 1)
-    assert status(bad) == ah.UNSUPPORTED   # no Taumoeba population here
+    assert status(bad) == ah.INVALID_ARGUMENT   # a predator the body does not hold: truncated
+    struct.pack_into("<i", bad, off["taumoeba_count"], 300)
+    assert status(bad) == ah.INVALID_ARGUMENT   # more predators than the capacity
     with pytest.raises(ah.AstroError) as e:
         ah.World.load_snapshot(gpu, tmp_path / "missing.asph", **kw)
     assert e.value.status == ah.FILE_IO_ERROR

@@ -152,6 +152,32 @@ def test_light_heat_taxis_match_reference_vectors(lib):
         assert te(fx(e0), fx(s), fx(dt)) == fx(y)
 
 
+def test_predator_helpers_match_reference_vectors(lib):
+    """tau_drag, tau_overlaps_cell, tau_should_tumble, tau_n2_hazard,
+    tau_n2_death_prob, tau_daughter_tolerance (src/sim/predation.cuh:24-80),
+    bit for bit, including overlaps within an ulp of contact and clamped
+    tolerances."""
+    td = sig(lib, "tau_drag", D, [D])
+    for t, y in VEC["tau_drag"]:
+        assert td(fx(t)) == fx(y)
+    to = sig(lib, "tau_overlaps_cell", C.c_int32, [D * 3, D * 3])
+    for a3, b3, y in VEC["tau_overlaps_cell"]:
+        assert to((D * 3)(*map(fx, a3)), (D * 3)(*map(fx, b3))) == y
+    tt = sig(lib, "tau_should_tumble", C.c_int32, [F, C.POINTER(F), F, D])
+    for s_, e0, run, dt, y, e1 in VEC["tau_should_tumble"]:
+        e = F(fx(e0))
+        assert tt(fx(s_), C.byref(e), fx(run), fx(dt)) == y and e.value == fx(e1)
+    th = sig(lib, "tau_n2_hazard", D, [D, D])
+    for n2, tol, y in VEC["tau_n2_hazard"]:
+        assert th(fx(n2), fx(tol)) == fx(y)
+    tp = sig(lib, "tau_n2_death_prob", D, [D, D])
+    for h, dtb, y in VEC["tau_n2_death_prob"]:
+        assert tp(fx(h), fx(dtb)) == fx(y)
+    tdt = sig(lib, "tau_daughter_tolerance", F, [F, D])
+    for par, m, y in VEC["tau_daughter_tolerance"]:
+        assert tdt(fx(par), fx(m)) == fx(y)
+
+
 def test_hash_keys_match_reference_vectors(oracle):
     # One cell per world; its bucket is the only non-empty one.
     for nx, ny, nz, cell, x, y, z, key in VEC["hash_key"][:12]:
