@@ -240,6 +240,8 @@ SIGNATURES = {
     "world_set_light": (_I, [_P, C.c_float, C.c_float, C.c_float, C.c_float, _I]),
     "world_apply_brush": (_I, [_P, _I, _D, _D, _D, _D]),
     "field_fill": (_I, [_P, _I, C.c_float]),
+    "world_set_flash": (_I, [_P, _I]),
+    "world_flash_audit": (_I, [_P, C.POINTER(_D), C.POINTER(_D)]),
     "world_time": (_I, [_P, C.POINTER(C.c_uint64), C.POINTER(_D), C.POINTER(_D), C.POINTER(_D)]),
     "stage_hash_build": (_I, [_P]),
     "hash_dims": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_D)]),
@@ -554,6 +556,15 @@ class World:
 
     def brush(self, kind: int, x: float, y: float, radius: float, strength: float):
         self._check(self.b.world_apply_brush(self.h, kind, x, y, radius, strength), "brush")
+
+    def set_flash(self, armed: bool):
+        self._check(self.b.world_set_flash(self.h, 1 if armed else 0))
+
+    def flash_audit(self) -> tuple[float, float]:
+        """(|impulse| [N s], discharged energy [J])."""
+        i, e = C.c_double(), C.c_double()
+        self._check(self.b.world_flash_audit(self.h, C.byref(i), C.byref(e)))
+        return i.value, e.value
 
     def fill(self, f: int, value: float):
         self._check(self.b.field_fill(self.h, f, value), "fill")

@@ -114,6 +114,7 @@ struct WorldDev {
     uint32_t* dlist;        // [cap] dropped reference slots, ascending
     unsigned long long* co2_demand;   // [CO2 grid] fixed-point demand
     unsigned long long* stats;        // StatsAccum words (see launch_stats)
+    unsigned long long* flash;        // [4] spin-drive audit: impulse x, y, z, energy
 };
 
 // Scalars of one tick, computed on the host from the world description.
@@ -180,6 +181,9 @@ template <class P> void launch_feed_taxis(const Launch& L, Views<P> ws, Ticks ts
 // Feeding/taxis (optional), field sample + body force, then the contact walk:
 // leaves each cell's total force in fs (sorted order).
 template <class P> void launch_cell_forces(const Launch& L, Views<P> ws, Ticks ts, bool feed, bool taxis);
+// flash_step (src/sim/flash.cu:59-67): the armed spin-drive flash in place
+// of taxis.
+template <class P> void launch_flash(const Launch& L, Views<P> ws, Ticks ts);
 // integrate_kernel; with `demand` it also books each moved cell's CO2 demand
 // (the lifecycle's first kernel; the CO2 field must have diffused already).
 template <class P> void launch_integrate(const Launch& L, Views<P> ws, Ticks ts, bool demand);

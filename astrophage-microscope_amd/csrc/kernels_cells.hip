@@ -234,6 +234,39 @@ struct KFeedTaxis {
 };
 
 // ---------------------------------------------------------------------------
+// flash_kernel (src/sim/flash.cu:31-56): every awake charged cell discharges
+// at the flash power toward the objective. Its recoil drives the ship, not
+// the cell (emit_power 0); the photon momentum and the energy go to the
+// fixed-point audit.
+// ---------------------------------------------------------------------------
+template <class P>
+struct KFlash {
+    static constexpr bool stride = true;   // any block count covers the work
+    struct Args {
+        Cells<P> c;
+        const Counters* ctr;
+        double dt;
+        unsigned long long* accum;
+    };
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [c, ctr, dt, accum] = a;
+        xcd_tiles(vg, ctr->nphys, [&](const int i, const bool valid) {
+            if (!valid) return;
+            const uint32_t f = c.flags[i];
+            if (!(f & ASTRO_FLAG_OCCUPIED) || !(f & ASTRO_FLAG_ALIVE) || !(f & ASTRO_FLAG_AWAKE)) return;
+            const double e = c.energy[i];
+            c.emit_power[i] = 0.0f;
+            if (e <= 0.0) return;
+            double de = ASTRO_PETROVA_FLASH_POWER * dt;
+            if (de > e) de = e;
+            c.energy[i] = e - de;
+            deposit_add(&accum[2], -(de / ASTRO_C_LIGHT), ASTRO_FLASH_IMPULSE_SCALE);
+            deposit_add(&accum[3], de, ASTRO_FLASH_ENERGY_SCALE);
+        });
+    }
+};
+
+// ---------------------------------------------------------------------------
 // field_sample_kernel (src/sim/integrator.cu:20-30) and the body-force half of
 // forces_kernel (:40-58), per physical entry; the force goes to the cell's
 // sorted position for the contact walk to add to. With FEED / TAXIS the same
@@ -918,6 +951,14 @@ void launch_feed_taxis(const Launch& L, Views<P> ws, Ticks ts, bool feed, bool t
     else if (taxis) launch_feed_taxis_t<P, false, true>(L, ws, ts, "taxis");
 }
 
+template <class P>
+void launch_flash(const Launch& L, Views<P> ws, Ticks ts) {
+    using K = KFlash<P>;
+    Batch<K> b;
+    for (size_t i = 0; i < ws.size(); ++i) b.add({ws[i].c, ws[i].ctr, ts[i].dt, ws[i].flash}, cell_grid(ws[i].pcap));
+    launch_k<K>(L, "flash", b.args, b.blocks);
+}
+
 template <class P, bool FEED, bool TAXIS>
 void launch_sample_t(const Launch& L, Views<P> ws, Ticks ts, const char* name) {
     using K = KSampleBase<P, FEED, TAXIS>;
@@ -1041,6 +1082,7 @@ void launch_stats_cells(const Launch& L, const WorldDev<P>& w) {
     template void launch_feed_taxis<P>(const Launch&, Views<P>, Ticks, bool, bool);                             \
     template void launch_integrate<P>(const Launch&, Views<P>, Ticks, bool);                                    \
     template void launch_cell_forces<P>(const Launch&, Views<P>, Ticks, bool, bool);                            \
+    template void launch_flash<P>(const Launch&, Views<P>, Ticks);                                                \
     template void launch_lifecycle<P>(const Launch&, Views<P>, Ticks, bool);                                    \
     template void launch_compact<P>(const Launch&, const WorldDev<P>&);                                          \
     template void launch_life_clear<P>(const Launch&, const WorldDev<P>&);                                      \

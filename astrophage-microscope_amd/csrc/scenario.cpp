@@ -415,8 +415,7 @@ double doubling_time(const Aggregates& a, double biology_rate) {
     return (std::log(2.0) / slope) * biology_rate;
 }
 
-// metric_measure (src/sim/accept.cpp:185-206). No spin-drive flash here: its
-// metric reads 0, as the reference's does for a run without one.
+// metric_measure (src/sim/accept.cpp:185-206).
 int32_t measure(int32_t metric, const astro_stats& fin, const Aggregates& a, astro_world* w,
                 const astro_scenario& s, double& out) {
     out = 0.0;
@@ -457,6 +456,12 @@ int32_t measure(int32_t metric, const astro_stats& fin, const Aggregates& a, ast
             out = correlation(charge, other);
             return ASTRO_OK;
         }
+        case ASTRO_METRIC_IMPULSE_PER_CYCLE: {
+            double imp = 0.0, dis = 0.0;
+            const int32_t st = astro_world_flash_audit(w, &imp, &dis);
+            out = dis > 0.0 ? imp * ASTRO_C_LIGHT / dis : 0.0;
+            return st;
+        }
         case ASTRO_METRIC_MAX_TAU_TOLERANCE: {
             // over live Taumoeba (src/sim/accept.cpp:63-77)
             int32_t n = 0, cap = 0;
@@ -480,7 +485,7 @@ int32_t measure(int32_t metric, const astro_stats& fin, const Aggregates& a, ast
                     out = std::max(out, static_cast<double>(f32[4][i]));
             return ASTRO_OK;
         }
-        default: return ASTRO_OK;   // impulse_per_cycle: no spin-drive flash here
+        default: return ASTRO_OK;
     }
 }
 
@@ -542,9 +547,6 @@ void astro_scenario_capacities(const astro_scenario* s, int32_t* cell_capacity, 
 int32_t astro_scenario_instantiate(const astro_scenario* s, const astro_world_desc* base, astro_world** out) {
     if (!s || !out) return ASTRO_INVALID_ARGUMENT;
     *out = nullptr;
-    for (int i = 0; i < s->drive_count; ++i)
-        if (s->drive[i].kind == ASTRO_STIM_FLASH) return ASTRO_UNSUPPORTED;   // no spin-drive flash
-
     astro_world_desc d;
     if (base) d = *base;
     else astro_world_desc_default(&d);
@@ -620,6 +622,8 @@ int32_t astro_scenario_apply_drive(astro_world* w, const astro_scenario* s) {
     const double dt = ASTRO_DT_PHYSICS * physics;
     astro_world_desc d;
     if ((st = astro_world_desc_get(w, &d)) != ASTRO_OK) return st;
+    // the flash is re-armed each tick by an active stimulus
+    bool flash = false;
     for (int i = 0; i < s->drive_count && st == ASTRO_OK; ++i) {
         const astro_stimulus& e = s->drive[i];
         if (!stimulus_on(e, t)) continue;
@@ -669,9 +673,11 @@ int32_t astro_scenario_apply_drive(astro_world* w, const astro_scenario* s) {
                 }
                 break;
             }
-            default: st = ASTRO_UNSUPPORTED; break;   // the spin-drive flash
+            case ASTRO_STIM_FLASH: flash = true; break;
+            default: st = ASTRO_INVALID_ARGUMENT; break;
         }
     }
+    if (st == ASTRO_OK) st = astro_world_set_flash(w, flash ? 1 : 0);
     return st;
 }
 
@@ -771,6 +777,8 @@ int32_t astro_scenario_run_assert(const astro_scenario* s, astro_world* w, int64
             ++t;
             continue;
         }
+        // no stimulus acts in these ticks: the drive only disarms the flash
+        if ((st = astro_scenario_apply_drive(w, s)) != ASTRO_OK) break;
         const int64_t next_sample = t % stride == 0 ? t : (t / stride + 1) * stride;
         const int64_t k = std::min<int64_t>({next_sample - t + 1, ticks - t, quiet});
         st = astro_world_step(w, static_cast<int32_t>(k));

@@ -218,6 +218,10 @@ struct astro_world {
     // a snapshot header; parameter overrides also make a run non-canon.
     bool non_canon = false;
 
+    // World::flash_active (src/sim/world.cuh:178-179) and its audit words.
+    bool flash_on = false;
+    unsigned long long* flash_accum = nullptr;
+
     // The Taumoeba store; the tick runs predation once any were added.
     TauDev tau{};
     int32_t tau_cap = 0;
@@ -288,6 +292,7 @@ WorldDev<P> make_view(astro_world* w) {
     d.dlist = w->dlist;
     d.co2_avail = w->co2_avail;
     d.stats = w->stats;
+    d.flash = w->flash_accum;
     return d;
 }
 
@@ -443,7 +448,15 @@ int32_t enqueue_tick(WorldSpan ws, const Launch& L) {
     if (ds.thermal_enabled && (st = enqueue_thermal<P>(ws, L, dt, ts)) != ASTRO_OK) return st;
     const bool feed = ds.emission_enabled && lit(ws[0]);
     if (feed) enqueue_light<P>(ws, L, ts);
-    launch_cell_forces<P>(L, views_of<P>(ws), ts, feed, ds.taxis_enabled != 0);
+    if (ws[0]->flash_on) {
+        // Stage 9b (step.cu:158-162): the armed flash runs instead of taxis,
+        // after the feed and before the motion reads emit_power.
+        if (feed) launch_feed_taxis<P>(L, views_of<P>(ws), ts, true, false);
+        launch_flash<P>(L, views_of<P>(ws), ts);
+        launch_cell_forces<P>(L, views_of<P>(ws), ts, false, false);
+    } else {
+        launch_cell_forces<P>(L, views_of<P>(ws), ts, feed, ds.taxis_enabled != 0);
+    }
     // Stages 7-8 (step.cu:168-171). Both accumulators are empty here: CO2
     // uptake is flushed inside lifecycle and nothing in this path deposits N2,
     // so the reference's dense flush is a no-op and is not issued. The CO2
@@ -839,7 +852,7 @@ int32_t serialize_body(astro_world* w, std::vector<char>& body, int32_t& overrid
 struct TickShape {
     bool fp32;
     double dt;
-    bool thermal, light, taxis, lifecycle, contact, predators;
+    bool thermal, light, taxis, lifecycle, contact, predators, flash;
     bool on[ASTRO_FIELD_COUNT];
     int substeps[3];   // T (when thermal), CO2, N2 (when on)
     bool operator==(const TickShape&) const = default;
@@ -856,6 +869,7 @@ TickShape tick_shape(const astro_world* w) {
     s.lifecycle = d.lifecycle_enabled != 0;
     s.contact = d.contact_enabled != 0;
     s.predators = w->tau_on;
+    s.flash = w->flash_on;
     for (int i = 0; i < ASTRO_FIELD_COUNT; ++i) s.on[i] = w->f[i].on;
     if (s.thermal) s.substeps[0] = substeps_for_dt(w->f[ASTRO_FIELD_TEMPERATURE], s.dt);
     if (s.on[ASTRO_FIELD_CO2]) s.substeps[1] = substeps_for_dt(w->f[ASTRO_FIELD_CO2], s.dt);
@@ -1176,6 +1190,7 @@ int32_t astro_world_create(const astro_world_desc* din, astro_world** out) {
         w->xsum = cv.take<int32_t>(static_cast<size_t>(scan_tiles_for(cap)));
         w->co2_demand = cv.take<unsigned long long>(std::max<size_t>(1, w->f[ASTRO_FIELD_CO2].count()));
         w->stats = cv.take<unsigned long long>(SW_COUNT);
+        w->flash_accum = cv.take<unsigned long long>(4);   // zeroed with the pool
         TauDev& t = w->tau;
         const size_t tn = static_cast<size_t>(tau_cap);
         t.capacity = tau_cap;
@@ -1515,6 +1530,35 @@ int32_t astro_field_fill(astro_world* w, int32_t field, float value) {
     launch_fill(L, g.value, g.count(), value);
     launch_fill(L, g.scratch, g.count(), value);
     return hip_check(w, hipGetLastError(), "fill");
+}
+
+// World::flash_active (src/sim/world.cuh:178), armed per tick by the driving
+// script; the captured tick is dropped only when it changes.
+int32_t astro_world_set_flash(astro_world* w, int32_t armed) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    const bool on = armed != 0;
+    if (on != w->flash_on) {
+        w->flash_on = on;
+        drop_graph(w);
+    }
+    return ASTRO_OK;
+}
+
+// world_flash_audit (src/sim/flash.cu:69-79).
+int32_t astro_world_flash_audit(astro_world* w, double* impulse_ns, double* discharged_j) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    unsigned long long raw[4] = {0, 0, 0, 0};
+    int32_t st = hip_check(w, hipMemcpyAsync(raw, w->flash_accum, sizeof(raw), hipMemcpyDeviceToHost, w->stream),
+                           "flash audit");
+    if (st == ASTRO_OK) st = hip_check(w, hipStreamSynchronize(w->stream), "synchronize");
+    if (st != ASTRO_OK) return st;
+    auto from_fixed = [](unsigned long long r, double scale) { return static_cast<double>(static_cast<int64_t>(r)) / scale; };
+    const double ix = from_fixed(raw[0], ASTRO_FLASH_IMPULSE_SCALE);
+    const double iy = from_fixed(raw[1], ASTRO_FLASH_IMPULSE_SCALE);
+    const double iz = from_fixed(raw[2], ASTRO_FLASH_IMPULSE_SCALE);
+    if (impulse_ns) *impulse_ns = std::sqrt(ix * ix + iy * iy + iz * iz);
+    if (discharged_j) *discharged_j = from_fixed(raw[3], ASTRO_FLASH_ENERGY_SCALE);
+    return ASTRO_OK;
 }
 
 // The host-side clock, without waiting for the device.

@@ -31,6 +31,10 @@
 #define ASTRO_CELL_ALBEDO_DEAD     0.84999999999999998
 #define ASTRO_CELL_LETHAL_TEMP     573.14999999999998
 #define ASTRO_PETROVA_MAX_POWER    0.050000000000000003
+#define ASTRO_PETROVA_FLASH_POWER  3000000.0
+/* the spin-drive flash audit's fixed-point scales (src/sim/flash.cu:26-27) */
+#define ASTRO_FLASH_IMPULSE_SCALE  1.0e15
+#define ASTRO_FLASH_ENERGY_SCALE   1.0e6
 
 /* ---- biology -------------------------------------------------------------- */
 #define ASTRO_CO2_MASS_PER_DIVISION   2.0999999999999999e-14

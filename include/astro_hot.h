@@ -233,6 +233,13 @@ int32_t astro_world_apply_brush(astro_world* w, int32_t kind, double x, double y
                                 double strength);
 /* grid_fill (src/fields/diffuse.cu:128-134). */
 int32_t astro_field_fill(astro_world* w, int32_t field, float value);
+/* World::flash_active (src/sim/world.cuh:178): while armed, each tick runs
+ * flash_step (src/sim/flash.cu:31-67) instead of taxis -- every awake
+ * charged cell discharges at PETROVA_FLASH_POWER toward the objective. */
+int32_t astro_world_set_flash(astro_world* w, int32_t armed);
+/* world_flash_audit (src/sim/flash.cu:69-79): |impulse| [N s] and the
+ * discharged energy [J] since the world was created. */
+int32_t astro_world_flash_audit(astro_world* w, double* impulse_ns, double* discharged_j);
 /* World::tick, sim_time_s and the clock rates, read on the host without
  * waiting for the device. */
 int32_t astro_world_time(const astro_world* w, uint64_t* tick, double* sim_time_s,

@@ -8,9 +8,7 @@
  * (tools/headless.cpp:179-250). Host code only; it drives the world through
  * the C ABI of astro_hot.h and never touches device memory itself.
  *
- * Not carried: the microscope view (`scope`, render-only). Not runnable here:
- * the spin-drive flash stimulus, which this build does not implement;
- * instantiating a scenario that needs it returns ASTRO_UNSUPPORTED.
+ * Not carried: the microscope view (`scope`, render-only).
  */
 #ifndef ASTRO_SCENARIO_H
 #define ASTRO_SCENARIO_H

@@ -529,6 +529,8 @@ struct oracle_world {
         uint64_t next_id = 1;
     } tau;
     std::vector<uint64_t> claim;   // World::d_predator_claim, per cell slot
+    bool flash_active = false;     // World::flash_active
+    uint64_t flash_accum[4] = {0, 0, 0, 0};
 
     std::string err;
 };
@@ -1304,6 +1306,22 @@ void predation_step(oracle_world* w, double dt) {
     }
 }
 
+// flash_step / flash_kernel (src/sim/flash.cu:31-67).
+void flash_step(oracle_world* w, double dt) {
+    for (int32_t i = 0; i < w->count; ++i) {
+        const uint32_t f = w->flags[i];
+        if (!(f & ASTRO_FLAG_OCCUPIED) || !(f & ASTRO_FLAG_ALIVE) || !(f & ASTRO_FLAG_AWAKE)) continue;
+        const double e = w->energy[i];
+        w->emit_power[i] = 0.0f;
+        if (e <= 0.0) continue;
+        double de = ASTRO_PETROVA_FLASH_POWER * dt;
+        if (de > e) de = e;
+        w->energy[i] = e - de;
+        w->flash_accum[2] += static_cast<uint64_t>(to_fixed(-(de / ASTRO_C_LIGHT), ASTRO_FLASH_IMPULSE_SCALE));
+        w->flash_accum[3] += static_cast<uint64_t>(to_fixed(de, ASTRO_FLASH_ENERGY_SCALE));
+    }
+}
+
 // world_step (src/sim/step.cu:109-184).
 int world_step_once(oracle_world* w) {
     const double dt = ASTRO_DT_PHYSICS * w->physics_rate;
@@ -1313,7 +1331,8 @@ int world_step_once(oracle_world* w) {
     int st;
     if (d.thermal_enabled && (st = thermal_step(w, dt)) != ASTRO_OK) return st;
     if (d.emission_enabled && (st = emission_step(w, dt)) != ASTRO_OK) return st;
-    if (d.taxis_enabled) taxis_step(w, dt);
+    if (w->flash_active) flash_step(w, dt);           // stage 9b, step.cu:158-162
+    else if (d.taxis_enabled) taxis_step(w, dt);
     field_sample(w);
     forces(w);
     integrate(w, dt);
@@ -1705,6 +1724,23 @@ int32_t oracle_field_fill(oracle_world* w, int32_t f, float value) {
     if (!g.enabled) return fail(w, ASTRO_INVALID_ARGUMENT, "fill of a disabled field");
     std::fill(g.value.begin(), g.value.end(), value);
     std::fill(g.scratch.begin(), g.scratch.end(), value);
+    return ASTRO_OK;
+}
+
+int32_t oracle_world_set_flash(oracle_world* w, int32_t armed) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    w->flash_active = armed != 0;
+    return ASTRO_OK;
+}
+
+// world_flash_audit (src/sim/flash.cu:69-79).
+int32_t oracle_world_flash_audit(oracle_world* w, double* impulse_ns, double* discharged_j) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    const double ix = from_fixed(w->flash_accum[0], ASTRO_FLASH_IMPULSE_SCALE);
+    const double iy = from_fixed(w->flash_accum[1], ASTRO_FLASH_IMPULSE_SCALE);
+    const double iz = from_fixed(w->flash_accum[2], ASTRO_FLASH_IMPULSE_SCALE);
+    if (impulse_ns) *impulse_ns = std::sqrt(ix * ix + iy * iy + iz * iz);
+    if (discharged_j) *discharged_j = from_fixed(w->flash_accum[3], ASTRO_FLASH_ENERGY_SCALE);
     return ASTRO_OK;
 }
 

@@ -51,6 +51,8 @@ int32_t oracle_world_set_light(oracle_world* w, float dir_x, float dir_y, float 
 int32_t oracle_world_apply_brush(oracle_world* w, int32_t kind, double x, double y, double radius,
                                  double strength);
 int32_t oracle_field_fill(oracle_world* w, int32_t field, float value);
+int32_t oracle_world_set_flash(oracle_world* w, int32_t armed);
+int32_t oracle_world_flash_audit(oracle_world* w, double* impulse_ns, double* discharged_j);
 int32_t oracle_world_time(const oracle_world* w, uint64_t* tick, double* sim_time_s, double* physics_rate,
                           double* biology_rate);
 int32_t oracle_taumoeba_spawn(oracle_world* w, int32_t count, uint64_t seed);

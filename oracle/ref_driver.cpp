@@ -181,7 +181,7 @@ int32_t ref_constant(const char* name, double* out) {
         {"CELL_SURFACE_AREA", canon::CELL_SURFACE_AREA}, {"CELL_MASS_DRY", canon::CELL_MASS_DRY},
         {"CELL_TEMP_SETPOINT", canon::CELL_TEMP_SETPOINT}, {"CELL_ENERGY_MAX", canon::CELL_ENERGY_MAX},
         {"CELL_ALBEDO_DEAD", canon::CELL_ALBEDO_DEAD}, {"CELL_LETHAL_TEMP", canon::CELL_LETHAL_TEMP},
-        {"PETROVA_MAX_POWER", canon::PETROVA_MAX_POWER},
+        {"PETROVA_MAX_POWER", canon::PETROVA_MAX_POWER}, {"PETROVA_FLASH_POWER", canon::PETROVA_FLASH_POWER},
         {"CO2_MASS_PER_DIVISION", canon::CO2_MASS_PER_DIVISION},
         {"CO2_DIFFUSIVITY_WATER", canon::CO2_DIFFUSIVITY_WATER},
         {"LIFE_DIVISION_ENERGY_COST", canon::LIFE_DIVISION_ENERGY_COST},

@@ -176,6 +176,7 @@ def apply_drive(w: ah.World, s: dict):
     dt = DT_PHYSICS * physics
     d = ah.WorldDesc()
     w._check(w.b.world_desc_get(w.h, ah.C.byref(d)))
+    flash = False
     for e in s["drive"]:
         open_ended = e["t1_s"] <= e["t0_s"]
         if t < e["t0_s"] or (not open_ended and t >= e["t1_s"]):
@@ -201,8 +202,9 @@ def apply_drive(w: ah.World, s: dict):
                 w.spawn(deficit, d.seed + tick, placement=p["placement"], place_x=p["place_x"],
                         place_y=p["place_y"], place_radius=p["place_radius"], charge_dist=p["charge_dist"],
                         charge_a=p["charge_a"], charge_b=p["charge_b"], awake=p["awake"])
-        else:
-            raise NotImplementedError("flash")
+        else:   # the spin-drive flash, re-armed each tick while active
+            flash = True
+    w.set_flash(flash)
 
 
 def drive_run(w: ah.World, s: dict, ticks: int):

@@ -9,8 +9,7 @@
   (state hash), and so do both after driven ticks -- the C++ runner batches
   the ticks outside stimulus windows, the restatement steps one at a time.
 * The reference's own acceptance gates (T24, tools/headless.cpp:179-250) for
-  every scenario this build can run (all but the spin-drive flash): each must
-  ACCEPT.
+  all nine reference scenarios: each must ACCEPT.
 """
 import numpy as np
 import pytest
@@ -21,8 +20,8 @@ from conftest import ROOT, ah, compare_cells
 pytestmark = pytest.mark.gpu
 
 GOLDEN = ROOT / "tests" / "golden" / "scenarios"
-RUNNABLE = ["bloom", "demo-herd", "first-light", "komorov", "sandbox", "shadow-garden", "taumoeba",
-            "three-percent-line"]
+RUNNABLE = ["bloom", "demo-herd", "first-light", "komorov", "sandbox", "shadow-garden", "spin-drive-face",
+            "taumoeba", "three-percent-line"]
 
 
 def text_of(name: str) -> str:
@@ -65,6 +64,33 @@ def test_stimuli_match_oracle(oracle, gpu, dims):
     g.close()
 
 
+@pytest.mark.parametrize("precision", [ah.FP64_CONTRACT, ah.FP32_HOT])
+def test_flash_matches_oracle(oracle, gpu, precision):
+    """The armed flash in place of taxis, with the light feeding first:
+    energies bit for bit and the same audit."""
+    kw = dict(capacity=3000, chamber_w=6e-4, chamber_h=6e-4, thermal_enabled=0, light_enabled=1,
+              light_irradiance=1e4, precision=precision,
+              grid={0: (64, 64, 1), 1: (32, 32, 1), 2: (16, 16, 1), 3: (64, 64, 1)})
+    o, g = ah.World(oracle, **kw), ah.World(gpu, **kw)
+    o.spawn(2500, 6, charge_dist=ah.DIST_UNIFORM, charge_b=1.0, awake=True)
+    g.upload(o.download())
+    for w in (o, g):
+        w.set_flash(True)
+        w.step(3)
+        w.set_flash(False)
+        w.step(2)
+    a, b = o.download(), g.download()
+    compare_cells(a, b)
+    assert rel_max(a["energy"], b["energy"]) < 1e-12
+    assert o.flash_audit() == g.flash_audit()
+    o.close()
+    g.close()
+
+
+def rel_max(a, b):
+    return float((np.abs(a - b) / np.maximum(np.abs(a), 1e-300)).max())
+
+
 @pytest.mark.parametrize("name", RUNNABLE)
 def test_instantiate_matches_restatement(gpu, name):
     lib_sc = ah.scenario_parse(gpu, text_of(name))
@@ -78,13 +104,6 @@ def test_instantiate_matches_restatement(gpu, name):
         assert a.counters().count == sum(p["count"] for p in s["populations"] if p["kind"] == 0)
         assert a.tau_counters()[0] == sum(p["count"] for p in s["populations"] if p["kind"] == 1)
         assert a.state_hash() == b.state_hash()
-
-
-@pytest.mark.parametrize("name", ["spin-drive-face"])
-def test_unsupported_scenarios_are_refused(gpu, name):
-    with pytest.raises(ah.AstroError) as e:
-        ah.scenario_world(gpu, ah.scenario_parse(gpu, text_of(name)))
-    assert e.value.status == ah.UNSUPPORTED
 
 
 def windowed_scenario() -> str:

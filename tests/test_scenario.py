@@ -168,3 +168,36 @@ def test_oracle_clock_presets(oracle):
                               (ah.CLOCK_CUSTOM, (100.0, 1.0))):
             w.set_clock_preset(preset, 500.0 if preset == ah.CLOCK_CUSTOM else 3.0, 0.5)
             assert w.time()[2:] == rates, preset   # custom (500, 0.5) clamps to (100, 1)
+
+
+def test_oracle_flash_discharge_and_audit(oracle):
+    """flash_step (src/sim/flash.cu:31-67): each awake charged cell loses
+    min(PETROVA_FLASH_POWER * dt, E), emits nothing, runs no taxis; dormant
+    cells are untouched; the audit's impulse * c / energy is 1 (the photon
+    momentum identity the spin-drive scenario accepts on)."""
+    kw = dict(capacity=300, chamber_w=5e-4, chamber_h=5e-4, thermal_enabled=0, emission_enabled=0,
+              grid={0: (16, 16, 1), 1: (16, 16, 1), 2: (16, 16, 1), 3: (16, 16, 1)})
+    with ah.World(oracle, **kw) as w:
+        w.spawn(200, 5, charge_dist=ah.DIST_UNIFORM, charge_a=0.0, charge_b=1.0, awake=True)
+        c = w.download()
+        c["energy"][:10] = 1000.0                     # less than one tick of flash
+        c["flags"][10:20] &= ~np.uint32(ah.FLAG_AWAKE)
+        w.upload(c)
+        e0, run0 = w.download()["energy"], w.download()["run_timer"]
+        w.set_flash(True)
+        w.step(1)
+        c = w.download()
+        de = np.minimum(3.0e6 * 1e-3, e0)            # the first ten empty within the tick
+        awake = np.ones(200, bool)
+        awake[10:20] = False
+        assert np.array_equal(c["energy"][awake], e0[awake] - de[awake])
+        assert np.all(c["energy"][:10] == 0.0)
+        assert np.array_equal(c["energy"][~awake], e0[~awake])
+        assert np.all(c["emit_power"][awake] == 0.0)
+        assert np.array_equal(c["run_timer"], run0)   # the flash replaces taxis
+        imp, dis = w.flash_audit()
+        assert abs(dis - de[awake].sum()) <= 1e-6 * awake.sum()
+        assert abs(imp * 299792458.0 / dis - 1.0) < 1e-9
+        w.set_flash(False)
+        w.step(1)
+        assert w.flash_audit() == (imp, dis)          # disarmed: nothing more
