@@ -134,10 +134,13 @@ struct KThermal {
 
     const int n = ctr->nphys;
     const auto body = [&](const int i) {
+    // Flags, position and energy are loaded together, ahead of the branch on
+    // the flags, so one memory latency precedes the voxel read.
     const uint32_t fl = c.flags[i];
+    const float xf = static_cast<float>(c.x[i]), yf = static_cast<float>(c.y[i]), zf = static_cast<float>(c.z[i]);
+    const double e0 = c.energy[i];
     if (!(fl & ASTRO_FLAG_OCCUPIED)) return;
-    const uint32_t idx = grid_nearest(T, static_cast<float>(c.x[i]), static_cast<float>(c.y[i]),
-                                      static_cast<float>(c.z[i]));
+    const uint32_t idx = grid_nearest(T, xf, yf, zf);
     const float t_grid = T.value[idx];
     bool awake = (fl & ASTRO_FLAG_AWAKE) != 0u;
     uint32_t nf = fl;
@@ -150,7 +153,6 @@ struct KThermal {
     c.temp_cell[i] = static_cast<float>(t_cell);
     if (!awake || !(fl & ASTRO_FLAG_ALIVE)) return;
     const double e = lumped_exchange_energy(t_cell, static_cast<double>(t_grid), ASTRO_CONDUCTION_COEFF, cap, dt_sub);
-    const double e0 = c.energy[i];
     double energy = e0 - e;
     double moved = e;
     if (energy < 0.0) {
@@ -297,8 +299,14 @@ struct KSampleBase {
 
     const int n = ctr->nphys;
     xcd_tiles(vg, n, [&](const int i, const bool valid) {
-        if (!valid || c.lslot[i] == NO_SLOT) return;
+        if (!valid) return;
+        // Slot, flags and position are loaded together (in bounds for i
+        // below nphys); nothing before the force sample moves the cell.
+        const uint32_t slot = c.lslot[i];
         uint32_t f = c.flags[i];
+        const float xf = static_cast<float>(c.x[i]), yf = static_cast<float>(c.y[i]),
+                    zf = static_cast<float>(c.z[i]);
+        if (slot == NO_SLOT) return;
         if constexpr (FEED || TAXIS) {
             // feeding and taxis first, as their own pass would (no other cell
             // is read), then the force from the updated state
@@ -308,8 +316,7 @@ struct KSampleBase {
                 f = c.flags[i];
             }
         }
-        const Force F = sample_force_cell<P>(c, i, f, static_cast<float>(c.x[i]), static_cast<float>(c.y[i]),
-                                             static_cast<float>(c.z[i]), T, C, N, c_on, n_on, gaxis);
+        const Force F = sample_force_cell<P>(c, i, f, xf, yf, zf, T, C, N, c_on, n_on, gaxis);
         fs[spos[i]] = F;
     });
     }
@@ -370,21 +377,28 @@ struct KIntegrate {
 
     const int n = ctr->nphys;
     const auto body = [&](const int i) {
+    // The entry's own arrays are all loaded with its flags (in bounds for i
+    // below nphys); only the force, found through the sorted position, waits
+    // for the occupancy test.
     const uint32_t fl = c.flags[i];
+    const int32_t sp = spos[i];
+    const float t_local_f = c.t_local[i];
+    const double biomass = c.biomass[i], energy = c.energy[i];
+    double px = static_cast<double>(c.x[i]), py = static_cast<double>(c.y[i]), pz = static_cast<double>(c.z[i]);
+    double vx = static_cast<double>(c.vx[i]), vy = static_cast<double>(c.vy[i]), vz = static_cast<double>(c.vz[i]);
+    const uint64_t id = c.id[i];
+    const uint64_t rng_state = c.rng_state[i];
     if (!(fl & ASTRO_FLAG_OCCUPIED)) {
         if constexpr (DEMAND) dm.vox[i] = NO_VOXEL;
         return;
     }
     const double dt = t.dt;
-    const double t_local = static_cast<double>(c.t_local[i]);
-    const double mass = cell_mass(c.biomass[i], c.energy[i]);
+    const double t_local = static_cast<double>(t_local_f);
+    const double mass = cell_mass(biomass, energy);
     const bool awake = (fl & ASTRO_FLAG_AWAKE) != 0u;
     const double t_drag = awake ? ASTRO_CELL_TEMP_SETPOINT : t_local;
-    const Force F = fs[spos[i]];
-    double px = static_cast<double>(c.x[i]), py = static_cast<double>(c.y[i]), pz = static_cast<double>(c.z[i]);
-    double vx = static_cast<double>(c.vx[i]), vy = static_cast<double>(c.vy[i]), vz = static_cast<double>(c.vz[i]);
-    const uint64_t id = c.id[i];
-    Pcg r = cell_rng(c.rng_state[i], id);
+    const Force F = fs[sp];
+    Pcg r = cell_rng(rng_state, id);
     // The three pairs are drawn in the reference's order (x, y, z); each is
     // used as soon as it is drawn, which keeps fewer values live.
     const bool noise = t.thermal_noise != 0;
@@ -517,17 +531,26 @@ struct KUptakeMark {
     xcd_tiles(vg, n, [&](const int i, const bool valid) {
         bool divides = false, dead = false;
         if (valid) {
+            // Every per-entry value the pass reads is loaded up front (all in
+            // bounds for i below nphys), so only the voxel's demand waits on
+            // a second memory latency.
+            const uint32_t idx = c_on ? dm.vox[i] : NO_VOXEL;
+            const float avail_f = c_on ? dm.avail[i] : 0.0f;
+            uint32_t f = c.flags[i];
+            const float t_local_f = c.t_local[i];
+            double held = c.co2_held[i];
+            const uint32_t s = c.lslot[i];
             if (c_on) {
-                const uint32_t idx = dm.vox[i];
                 if (idx != NO_VOXEL) {
-                    const double available = static_cast<double>(dm.avail[i]);
+                    const double available = static_cast<double>(avail_f);
                     const double want = co2_uptake_rate(available) * dm.dt_bio;
                     const unsigned long long raw = dm.demand[idx];
                     const double asked = from_fixed(raw, dm.C.scale);
                     double taken = want;
                     if (asked > available && asked > 0.0) taken = want * (available / asked);
                     if (taken > 0.0) {
-                        c.co2_held[i] += taken;
+                        held += taken;
+                        c.co2_held[i] = held;
                         const int64_t fxd = to_fixed(-taken / dm.gv, dm.C.scale);
                         if (fxd != 0) {
                             if (static_cast<int64_t>(raw) == to_fixed(want / dm.gv, dm.C.scale)) {
@@ -542,9 +565,8 @@ struct KUptakeMark {
                     }
                 }
             }
-            uint32_t f = c.flags[i];
             if ((f & ASTRO_FLAG_OCCUPIED) && (f & ASTRO_FLAG_ALIVE) &&
-                static_cast<double>(c.t_local[i]) > ASTRO_CELL_LETHAL_TEMP) {
+                static_cast<double>(t_local_f) > ASTRO_CELL_LETHAL_TEMP) {
                 f &= ~ASTRO_FLAG_ALIVE;
                 c.flags[i] = f;
                 c.death_cause[i] = ASTRO_DEATH_OVERHEATED;
@@ -552,9 +574,8 @@ struct KUptakeMark {
                 c.energy[i] = disposition == ASTRO_STORE_RETAIN ? c.energy[i] : 0.0;
             }
             const bool occ = (f & ASTRO_FLAG_OCCUPIED) != 0u, alive = (f & ASTRO_FLAG_ALIVE) != 0u;
-            divides = occ && alive && c.co2_held[i] >= quota;
+            divides = occ && alive && held >= quota;
             dead = occ && !alive;
-            const uint32_t s = c.lslot[i];
             if (s != NO_SLOT) {
                 // Both arrays and their tile totals start the tick cleared.
                 if (divides) {
@@ -668,10 +689,13 @@ struct KLifeApply {
     if (births > 0 || compacting || c_on) {
         xcd_tiles(vg, np, [&](const int i, const bool valid) {
             if (!valid) return;
+            // The voxel record and the slot are loaded together (in bounds
+            // for i below nphys).
+            const uint32_t v = c_on ? dm.vox[i] : NO_VOXEL;
+            const uint32_t s = c.lslot[i];
             if (c_on) {
                 // The CO2 flush of shared voxels (lifecycle.cu:235) by their
                 // owners, and the demand cleared for the next tick.
-                const uint32_t v = dm.vox[i];
                 if (v != NO_VOXEL) {
                     const uint32_t idx = v & ~VOX_OWNER;
                     if (v & VOX_OWNER) {
@@ -684,7 +708,6 @@ struct KLifeApply {
                     dm.demand[idx] = 0ull;
                 }
             }
-            const uint32_t s = c.lslot[i];
             if (s == NO_SLOT) return;
             uint32_t below = 0u;
             if (compacting) {

@@ -71,12 +71,22 @@ struct KHashKey {
         }
     }
     xcd_tiles(vg, np, [&](const int p, const bool valid) {
-        const bool live = valid && c.lslot[p] != NO_SLOT;
+        // Slot, flags and position are loaded together (all in bounds for p
+        // below nphys), so a single memory latency precedes the key.
+        uint32_t s = NO_SLOT, f = 0u;
+        P x{}, y{}, z{};
+        if (valid) {
+            s = c.lslot[p];
+            f = c.flags[p];
+            x = c.x[p];
+            y = c.y[p];
+            z = c.z[p];
+        }
+        const bool live = s != NO_SLOT;
         uint32_t key = 0u;
         if (live) {
-            const uint32_t f = c.flags[p];
             key = (f & ASTRO_FLAG_OCCUPIED)
-                ? hash_key(h, static_cast<double>(c.x[p]), static_cast<double>(c.y[p]), static_cast<double>(c.z[p]))
+                ? hash_key(h, static_cast<double>(x), static_cast<double>(y), static_cast<double>(z))
                 : static_cast<uint32_t>(h.nb);
             keys[p] = key;
             rin[p] = atomicAdd(&bcount[key], 1u);
@@ -351,18 +361,31 @@ __device__ __forceinline__ void staged_walk(const HashGeo& h, const Rec<P>* rec,
     }
 }
 
-// The contact half of forces_kernel (src/sim/integrator.cu:60-67), one thread
+// Steps of its walk a lane screens per round: one bit of its survivor mask
+// each.
+constexpr int CONTACT_ROUND = 32;
+
+// The contact half of forces_kernel (src/sim/integrator.cu:60-67), one lane
 // per sorted record, adding to the body force already in fs[k]. Neighbour
 // positions come from the hash's records, fixed since the hash build.
-//   * The neighbours come through the wave's LDS staging (staged_walk).
-//   * Each neighbour is first screened in float: the float r^2 of a pair is
-//     within a few float roundings (relative 2^-21) of the double r^2, so a
-//     2^-10 margin keeps every pair the reference counts as touching. Only the
-//     survivors -- about one per cell -- take the exact double-precision
-//     contact_force, in walk order, so F sums the same terms in the same order
-//     as the reference. Pairs the reference finds apart add a zero vector
-//     there; F's components are sums of nonzero terms onto a nonzero or +0.0
-//     start and so never -0.0, which makes those adds exact no-ops.
+//   * Per z offset the wave stages the records of its three rows into LDS,
+//     as staged_walk does: the rows' ranges end to end ("positions" 0 ..
+//     total), in chunks of walk_seg records. Each lane then walks its own
+//     three rows back to back in one loop, in the reference's order (z, y,
+//     x, then sorted position, src/sim/hash.cuh:88-111), skipping its own
+//     record; the wave runs as many iterations as its longest three-row
+//     walk, and the loop body has no branch.
+//   * Each neighbour is screened in float: the float r^2 of a pair is within
+//     a few float roundings (relative 2^-21) of the double r^2, so a 2^-10
+//     margin keeps every pair the reference counts as touching. A lane marks
+//     the steps that pass in a 32-bit mask, and after each round of at most
+//     32 steps queues those records' sorted positions in walk order.
+//   * The queue is drained in order through the exact double-precision
+//     contact_force -- when it fills, and at the end -- so F sums the same
+//     terms in the same order as the reference. Pairs the reference finds
+//     apart add a zero vector there; F's components are sums of nonzero
+//     terms onto a nonzero or +0.0 start and so never -0.0, which makes
+//     those adds exact no-ops.
 template <class P>
 struct KContactWalk {
     static constexpr bool stride = true;   // any block count covers the work
@@ -376,10 +399,12 @@ struct KContactWalk {
     };
     static __device__ void run(const Args& a, VGrid vg) {
         const auto& [ctr, h, rec, bstart, fs, k_contact] = a;
-
-    __shared__ Rec<P> s_seg[BLOCK / 64][walk_seg<P>()];
+    constexpr int SEG = walk_seg<P>();
+    __shared__ Rec<P> s_seg[BLOCK / 64][SEG];
     __shared__ int32_t s_queue[CONTACT_QUEUE * BLOCK];
     Rec<P>* seg = s_seg[threadIdx.x >> 6];
+    int32_t* queue = s_queue + threadIdx.x;   // entry j at queue[j * BLOCK]
+    const int lane = static_cast<int>(threadIdx.x) & 63;
     const int n = ctr->count;
     const double two_a = ASTRO_CELL_DIAMETER;
     const float screen = static_cast<float>(two_a * two_a) * (1.0f + 1.0f / 1024.0f);
@@ -394,8 +419,6 @@ struct KContactWalk {
                 hash_coords(h, static_cast<double>(r.x), static_cast<double>(r.y), static_cast<double>(r.z), gx, gy,
                             gz);
         }
-        const uint32_t self = r.meta & REC_SLOT_MASK;
-        int32_t* queue = s_queue + threadIdx.x;   // entry j at queue[j * BLOCK]
         int queued = 0;
         Force F{};
         bool loaded = false;
@@ -422,15 +445,101 @@ struct KContactWalk {
             }
             queued = 0;
         };
-        const auto screen_pair = [&](const Rec<P>& o, int32_t q) {
-            if ((o.meta & REC_SLOT_MASK) == self) return;
-            const float dx = screen_diff(r.x, o.x), dy = screen_diff(r.y, o.y), dz = screen_diff(r.z, o.z);
-            if (dx * dx + dy * dy + dz * dz >= screen) return;
-            if (queued == CONTACT_QUEUE) drain();
-            queue[queued * BLOCK] = q;
-            ++queued;
-        };
-        staged_walk(h, rec, bstart, seg, occ, gx, gy, gz, screen_pair);
+
+        // Row (dz, dy) of any lane lies in buckets [kmin + o - 1, kmax + o + 1],
+        // o = dz*nx*ny + dy*nx. Lane b < 18 fetches the bucket start of bound
+        // b (the start of row b/2's range, or the end), and the wave reads
+        // them back by lane.
+        const int xlo = gx > 0 ? gx - 1 : 0;
+        const int xhi = gx < h.nx - 1 ? gx + 1 : h.nx - 1;
+        const int key = (gz * h.ny + gy) * h.nx + gx;
+        const int kmin = wave_min_i(occ ? key : 0x7FFFFFFF);
+        const int kmax = wave_max_i(occ ? key : -1);
+        int bound = 0;
+        if (kmax >= 0 && lane < 18) {
+            const int row = lane >> 1;
+            const int64_t o = static_cast<int64_t>(row / 3 - 1) * h.nx * h.ny + static_cast<int64_t>(row % 3 - 1) * h.nx;
+            int64_t b = (lane & 1) ? static_cast<int64_t>(kmax) + o + 2 : static_cast<int64_t>(kmin) + o - 1;
+            b = b < 0 ? 0 : (b > h.nb ? h.nb : b);
+            bound = bstart[b];
+        }
+#pragma unroll 1
+        for (int d = 0; d < 3; ++d) {
+            // The wave's three ranges [lo[j], lo[j] + len) at positions off[j].
+            int lo[3], off[3], total = 0;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int ra = __builtin_amdgcn_readlane(bound, 6 * d + 2 * j);
+                const int rb = __builtin_amdgcn_readlane(bound, 6 * d + 2 * j + 1);
+                lo[j] = ra;
+                off[j] = total;
+                total += rb > ra ? rb - ra : 0;
+            }
+            // This lane's rows (dz = d - 1, dy = j - 1), as row_ranges
+            // computes them: nr[j] records from position p[j]. Each lies
+            // inside the wave's range of its row, so positions rise along
+            // the lane's walk.
+            int p[3], nr[3];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int cz = gz + d - 1, cy = gy + j - 1;
+                const bool ok = occ && cz >= 0 && cz < h.nz && cy >= 0 && cy < h.ny;
+                const int row = ok ? (cz * h.ny + cy) * h.nx : 0;
+                const int rs = ok ? bstart[row + xlo] : 0;
+                const int re = ok ? bstart[row + xhi + 1] : 0;
+                nr[j] = re > rs ? re - rs : 0;
+                p[j] = off[j] + (rs - lo[j]);
+            }
+            const int n01 = nr[0] + nr[1];
+            // The lane's own record, in the middle row of the middle z offset.
+            const int selfpos = (d == 1 && occ) ? off[1] + (k - lo[1]) : -1;
+            // Position of step m of the lane's walk.
+            const auto pos_of = [&](int m) {
+                return m < nr[0] ? p[0] + m : (m < n01 ? p[1] + (m - nr[0]) : p[2] + (m - n01));
+            };
+            int m = 0;
+            for (int c0 = 0; c0 < total; c0 += SEG) {
+                const int cend = total - c0 < SEG ? total : c0 + SEG;
+                for (int v = c0 + lane; v < cend; v += 64) {
+                    const int j = v < off[1] ? 0 : (v < off[2] ? 1 : 2);
+                    seg[v - c0] = rec[lo[j] + (v - off[j])];
+                }
+                wave_sync();
+                // The lane's steps whose positions lie below cend.
+                int mc = 0;
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const int below = cend - p[j];
+                    mc += below < 0 ? 0 : (below < nr[j] ? below : nr[j]);
+                }
+                while (__ballot(m < mc) != 0) {
+                    const int sb = m;
+                    const int mend = mc - sb < CONTACT_ROUND ? mc : sb + CONTACT_ROUND;
+                    uint32_t mask = 0;
+                    int pos = pos_of(m);
+                    for (; m < mend; ++m) {
+                        const Rec<P> o = seg[pos - c0];
+                        const float dx = screen_diff(r.x, o.x), dy = screen_diff(r.y, o.y),
+                                    dz = screen_diff(r.z, o.z);
+                        const bool hit = dx * dx + dy * dy + dz * dz < screen && pos != selfpos;
+                        mask |= static_cast<uint32_t>(hit) << (m - sb);
+                        ++pos;
+                        if (m + 1 == nr[0]) pos = p[1];
+                        if (m + 1 == n01) pos = p[2];
+                    }
+                    // Queue the round's survivors by sorted position, in order.
+                    while (mask != 0u) {
+                        const int at = pos_of(sb + __builtin_ctz(mask));
+                        mask &= mask - 1u;
+                        const int j = at < off[1] ? 0 : (at < off[2] ? 1 : 2);
+                        if (queued == CONTACT_QUEUE) drain();
+                        queue[queued * BLOCK] = at - off[j] + lo[j];
+                        ++queued;
+                    }
+                }
+                wave_sync();   // the next chunk reuses the slice
+            }
+        }
         if (queued > 0) drain();
         if (loaded) fs[k] = F;
     });
@@ -575,7 +684,8 @@ void launch_contact_walk(const Launch& L, Views<P> ws, Ticks ts) {
     Batch<KContactWalk<P>> b;
     for (size_t i = 0; i < ws.size(); ++i)
         b.add({ws[i].ctr, ws[i].hg, ws[i].rec, ws[i].bstart, ws[i].fs, ts[i].k_contact}, cell_grid(ws[i].capacity));
-    launch_k<KContactWalk<P>>(L, "contact_walk", b.args, b.blocks);
+    // Five waves per SIMD, as the LDS allows (28 KB per block).
+    launch_k<KContactWalk<P>, 5>(L, "contact_walk", b.args, b.blocks);
 }
 
 template <class P>

@@ -102,18 +102,19 @@ __device__ void scan_tile(int n, int tile, Load4 load4, Store4 store4, const int
     __shared__ int32_t s_wave[BLOCK / 64];
     const int num_tiles = scan_tiles_for(n);
     if (tile >= num_tiles) return;
+    // The tile's own items are loaded first: they do not depend on the
+    // prefix, so their latency overlaps the sum of the earlier tiles' totals.
+    const int first = tile * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    int4 v[SCAN_ITEMS / 4];
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS / 4; ++k) v[k] = load4(first + 4 * k);
     int32_t before = 0;
     for (int t = threadIdx.x; t < tile; t += BLOCK) before += tile_sum[t];
     const int32_t prefix = block_sum(before, s_wave);
 
-    const int first = tile * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
-    int4 v[SCAN_ITEMS / 4];
     int32_t sum = 0;
 #pragma unroll
-    for (int k = 0; k < SCAN_ITEMS / 4; ++k) {
-        v[k] = load4(first + 4 * k);
-        sum += v[k].x + v[k].y + v[k].z + v[k].w;
-    }
+    for (int k = 0; k < SCAN_ITEMS / 4; ++k) sum += v[k].x + v[k].y + v[k].z + v[k].w;
     const int32_t incl = wave_inclusive_sum(sum);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     __syncthreads();

@@ -89,6 +89,66 @@ def test_hash_parity(oracle, gpu, case):
         g.close()
 
 
+def staging_demand(w) -> int:
+    """Largest number of records one wave of the contact walk stages for one
+    z offset: the 64 consecutive sorted records of a wave have bucket keys in
+    [kmin, kmax], and row (dz, dy) of every lane lies in buckets
+    [kmin + o - 1, kmax + o + 1], o = dz*nx*ny + dy*nx (clamped)."""
+    nx, ny, nz, _ = w.hash_dims()
+    nb = nx * ny * nz
+    _, bs, be = w.hash_download()
+    counts = (be - bs).astype(np.int64)
+    bstart = np.concatenate([[0], np.cumsum(counts)])
+    keys = np.repeat(np.arange(nb), counts)
+    worst = 0
+    for w0 in range(0, keys.size, 64):
+        kmin, kmax = int(keys[w0:w0 + 64].min()), int(keys[w0:w0 + 64].max())
+        for dz in (-1, 0, 1):
+            tot = 0
+            for dy in (-1, 0, 1):
+                o = dz * nx * ny + dy * nx
+                lo = bstart[min(max(kmin + o - 1, 0), nb)]
+                hi = bstart[min(max(kmax + o + 2, 0), nb)]
+                tot += max(int(hi - lo), 0)
+            worst = max(worst, tot)
+    return worst
+
+
+@pytest.mark.parametrize("precision", [ah.FP64_CONTRACT, ah.FP32_HOT])
+@pytest.mark.parametrize("dims", ["2d", "3d"])
+def test_contact_forces_dense_cluster(oracle, gpu, precision, dims):
+    """Contact forces of a cluster so dense that a wave's rows of one z
+    offset hold over twice the records its LDS slice takes (256 float or 192
+    double records), so the walk stages them in three or more chunks: the
+    forces are the oracle's bit for bit (FP64 layout) or to float rounding of
+    the body force (FP32 layout)."""
+    if dims == "2d":
+        kw = dict(REF2D, capacity=6000, hash_cell_size=1e-5, precision=precision)
+        n, radius = 6000, 28e-6
+    else:
+        kw = dict(C1, capacity=8000, hash_cell_size=1e-5, precision=precision)
+        n, radius = 8000, 15e-6
+    cells = oracle_cells(oracle, n, 9, spawn=dict(placement=ah.PLACE_GAUSSIAN, place_radius=radius), **kw)
+    cells["flags"][::501] = 0
+    if precision == ah.FP32_HOT:
+        for k in ("x", "y", "z"):
+            cells[k] = cells[k].astype(np.float32).astype(np.float64)
+    o, g = pair(oracle, gpu, cells, **kw)
+    with o, g:
+        o.hash_build()
+        assert staging_demand(o) > 2 * 256   # three or more chunks in either layout
+        g.hash_build()
+        o.stage("forces")
+        g.stage("forces")
+        fo, fg = o.forces(), g.forces()
+        for a, b, name in zip(fo, fg, "xyz"):
+            assert np.abs(a).max() > 0.0
+            if precision == ah.FP64_CONTRACT:
+                assert np.array_equal(a, b), name
+            else:
+                rel_close(a, b, 1e-6, name=name)
+
+
 @pytest.mark.parametrize("dims,bc_field", [((128, 128, 1), ah.TEMPERATURE), ((96, 80, 24), ah.TEMPERATURE),
                                            ((64, 64, 16), ah.CO2)])
 def test_ftcs_parity(oracle, gpu, dims, bc_field):
