@@ -194,10 +194,11 @@ struct KApplyAwake {
 
     const int n = ctr->nphys;
     const auto body = [&](const int i) {
+    // Flags and position are loaded together (in bounds for i below nphys).
     const uint32_t fl = c.flags[i];
+    const float xf = static_cast<float>(c.x[i]), yf = static_cast<float>(c.y[i]), zf = static_cast<float>(c.z[i]);
     if (!(fl & ASTRO_FLAG_OCCUPIED) || !(fl & ASTRO_FLAG_AWAKE)) return;
-    const uint32_t idx = grid_nearest(g, static_cast<float>(c.x[i]), static_cast<float>(c.y[i]),
-                                      static_cast<float>(c.z[i]));
+    const uint32_t idx = grid_nearest(g, xf, yf, zf);
     const unsigned long long raw = atomicExch(&g.deposit[idx], 0ull);
     if (raw != 0ull) g.value[idx] += static_cast<float>(from_fixed(raw, g.scale));
     };

@@ -145,9 +145,12 @@ struct KFtcs3dV4 {
     if (t >= tx * ty * tz) return;   // whole block: shuffles below stay uniform
     const int lane = threadIdx.x & 63;
     const int xl = lane % W;           // lane within its row
-    const int i0 = (t % tx) * (4 * W) + xl * 4;
-    const int j = ((t / tx) % ty) * RB + static_cast<int>(threadIdx.x) / W;
-    const int k0 = (t / (tx * ty)) * ZC;
+    // z chunks vary fastest, then x, then y: the chunks of one column are
+    // consecutive tiles of one XCD, so each chunk's halo planes are the
+    // planes its neighbours just read through the same L2.
+    const int i0 = ((t / tz) % tx) * (4 * W) + xl * 4;
+    const int j = (t / (tz * tx)) * RB + static_cast<int>(threadIdx.x) / W;
+    const int k0 = (t % tz) * ZC;
     const bool on = i0 < nx && j < ny;
     const size_t pl = static_cast<size_t>(nx) * ny;
     const size_t col = on ? static_cast<size_t>(j) * nx + i0 : 0;
