@@ -375,9 +375,11 @@ constexpr int CONTACT_ROUND = 32;
 //     x, then sorted position, src/sim/hash.cuh:88-111), skipping its own
 //     record; the wave runs as many iterations as its longest three-row
 //     walk, and the loop body has no branch.
-//   * Each neighbour is screened in float: the float r^2 of a pair is within
-//     a few float roundings (relative 2^-21) of the double r^2, so a 2^-10
-//     margin keeps every pair the reference counts as touching. A lane marks
+//   * Each neighbour is screened in float: the float r^2 of a pair (the
+//     differences rounded to float, squared and summed with fused
+//     multiply-adds) is within a few float roundings (relative 2^-21) of the
+//     double r^2, so a 2^-10 margin keeps every pair the reference counts as
+//     touching. A lane marks
 //     the steps that pass in a 32-bit mask, and after each round of at most
 //     32 steps queues those records' sorted positions in walk order.
 //   * The queue is drained in order through the exact double-precision
@@ -517,15 +519,34 @@ struct KContactWalk {
                     const int mend = mc - sb < CONTACT_ROUND ? mc : sb + CONTACT_ROUND;
                     uint32_t mask = 0;
                     int pos = pos_of(m);
-                    for (; m < mend; ++m) {
-                        const Rec<P> o = seg[pos - c0];
+                    // Position after step q at position at: the next in its
+                    // row, or the start of the next row.
+                    const auto next_pos = [&](int at, int q) {
+                        int nx_ = at + 1;
+                        if (q + 1 == nr[0]) nx_ = p[1];
+                        if (q + 1 == n01) nx_ = p[2];
+                        return nx_;
+                    };
+                    const auto hit_at = [&](int at) {
+                        const Rec<P> o = seg[at - c0];
                         const float dx = screen_diff(r.x, o.x), dy = screen_diff(r.y, o.y),
                                     dz = screen_diff(r.z, o.z);
-                        const bool hit = dx * dx + dy * dy + dz * dz < screen && pos != selfpos;
-                        mask |= static_cast<uint32_t>(hit) << (m - sb);
-                        ++pos;
-                        if (m + 1 == nr[0]) pos = p[1];
-                        if (m + 1 == n01) pos = p[2];
+                        // Fused multiply-adds round fewer times than separate
+                        // products and sums, well inside the margin.
+                        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                        return static_cast<uint32_t>(r2 < screen && at != selfpos);
+                    };
+                    // Two steps per iteration: both records are read before
+                    // either is screened, so two LDS reads share one wait.
+                    for (; m + 1 < mend; m += 2) {
+                        const int a0 = pos, a1 = next_pos(a0, m);
+                        pos = next_pos(a1, m + 1);
+                        const uint32_t h0 = hit_at(a0), h1 = hit_at(a1);
+                        mask |= (h0 | (h1 << 1)) << (m - sb);
+                    }
+                    if (m < mend) {
+                        mask |= hit_at(pos) << (m - sb);
+                        ++m;
                     }
                     // Queue the round's survivors by sorted position, in order.
                     while (mask != 0u) {
