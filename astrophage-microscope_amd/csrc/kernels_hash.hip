@@ -455,8 +455,12 @@ struct KContactWalk {
         const int xlo = gx > 0 ? gx - 1 : 0;
         const int xhi = gx < h.nx - 1 ? gx + 1 : h.nx - 1;
         const int key = (gz * h.ny + gy) * h.nx + gx;
-        const int kmin = wave_min_i(occ ? key : 0x7FFFFFFF);
-        const int kmax = wave_max_i(occ ? key : -1);
+        // The wave's records are consecutive in bucket order, unoccupied ones
+        // (the sentinel bucket) last: its occupied lanes are a prefix whose
+        // keys do not decrease, so the first and last of them bound the rest.
+        const unsigned long long act = __ballot(occ);
+        const int kmin = act ? __builtin_amdgcn_readlane(key, __ffsll(static_cast<long long>(act)) - 1) : 0x7FFFFFFF;
+        const int kmax = act ? __builtin_amdgcn_readlane(key, 63 - __clzll(static_cast<long long>(act))) : -1;
         int bound = 0;
         if (kmax >= 0 && lane < 18) {
             const int row = lane >> 1;
@@ -481,15 +485,18 @@ struct KContactWalk {
             // computes them: nr[j] records from position p[j]. Each lies
             // inside the wave's range of its row, so positions rise along
             // the lane's walk.
+            // The six loads are unconditional (a row off the grid reads row
+            // 0, which is in bounds, and counts no records), so they issue
+            // together with no branch.
             int p[3], nr[3];
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
                 const int cz = gz + d - 1, cy = gy + j - 1;
                 const bool ok = occ && cz >= 0 && cz < h.nz && cy >= 0 && cy < h.ny;
                 const int row = ok ? (cz * h.ny + cy) * h.nx : 0;
-                const int rs = ok ? bstart[row + xlo] : 0;
-                const int re = ok ? bstart[row + xhi + 1] : 0;
-                nr[j] = re > rs ? re - rs : 0;
+                const int rs = bstart[row + xlo];
+                const int re = bstart[row + xhi + 1];
+                nr[j] = ok && re > rs ? re - rs : 0;
                 p[j] = off[j] + (rs - lo[j]);
             }
             const int n01 = nr[0] + nr[1];
@@ -502,9 +509,22 @@ struct KContactWalk {
             int m = 0;
             for (int c0 = 0; c0 < total; c0 += SEG) {
                 const int cend = total - c0 < SEG ? total : c0 + SEG;
-                for (int v = c0 + lane; v < cend; v += 64) {
-                    const int j = v < off[1] ? 0 : (v < off[2] ? 1 : 2);
-                    seg[v - c0] = rec[lo[j] + (v - off[j])];
+                // Up to four records per lane are loaded before any is
+                // stored, so the copy costs one memory latency, not four.
+                // Loads and stores are unconditional: a position past the
+                // chunk is clamped to its last one, whose record is read and
+                // written again unchanged.
+                for (int v = c0 + lane; v < cend; v += 4 * 64) {
+                    Rec<P> t[4];
+                    int w[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        w[u] = min(v + 64 * u, cend - 1);
+                        const int j = w[u] < off[1] ? 0 : (w[u] < off[2] ? 1 : 2);
+                        t[u] = rec[lo[j] + (w[u] - off[j])];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) seg[w[u] - c0] = t[u];
                 }
                 wave_sync();
                 // The lane's steps whose positions lie below cend.
