@@ -251,23 +251,27 @@ struct KResortCopyback {
 // rows in the reference's walk order (z, then y, then x, then sorted position,
 // src/sim/hash.cuh:88-111). The three x-adjacent buckets of one (y, z) row are
 // contiguous in the sorted array, so each row is a single range
-// [bstart[x_lo], bstart[x_hi + 1]); all eighteen loads are issued together.
+// [bstart[x_lo], bstart[x_hi + 1]); all eighteen loads are issued together,
+// unconditionally (a row off the grid, or any row of an inactive lane, reads
+// row 0 and is made empty).
 struct RowRanges {
     int32_t s[9], e[9];
 };
 
 template <class Start>
-__device__ __forceinline__ RowRanges row_ranges(const HashGeo& h, Start&& start, int gx, int gy, int gz) {
+__device__ __forceinline__ RowRanges row_ranges(const HashGeo& h, Start&& start, bool active, int gx, int gy,
+                                                int gz) {
     RowRanges rr;
     const int xlo = gx > 0 ? gx - 1 : 0;
     const int xhi = gx < h.nx - 1 ? gx + 1 : h.nx - 1;
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
         const int cz = gz + r / 3 - 1, cy = gy + r % 3 - 1;
-        const bool ok = cz >= 0 && cz < h.nz && cy >= 0 && cy < h.ny;
+        const bool ok = active && cz >= 0 && cz < h.nz && cy >= 0 && cy < h.ny;
         const int row = ok ? (cz * h.ny + cy) * h.nx : 0;
-        rr.s[r] = ok ? start(r / 3, row + xlo) : 0;
-        rr.e[r] = ok ? start(r / 3, row + xhi + 1) : 0;
+        const int32_t s = start(r / 3, row + xlo), e = start(r / 3, row + xhi + 1);
+        rr.s[r] = s;
+        rr.e[r] = ok ? e : s;
     }
     return rr;
 }
@@ -309,15 +313,17 @@ __device__ __forceinline__ void staged_walk(const HashGeo& h, const Rec<P>* rec,
                                             bool active, int gx, int gy, int gz, Visit&& visit) {
     constexpr int SEG = walk_seg<P>();
     const int lane = static_cast<int>(threadIdx.x) & 63;
-    RowRanges rr{};
-    if (active) rr = row_ranges(h, [&](int, int b) { return bstart[b]; }, gx, gy, gz);
+    const RowRanges rr = row_ranges(h, [&](int, int b) { return bstart[b]; }, active, gx, gy, gz);
     // Row (dz, dy) of any lane lies in buckets [kmin + o - 1, kmax + o + 1],
     // o = dz*nx*ny + dy*nx. Lane m < 18 fetches the bucket start of bound m
     // (the start of row m/2's range, or the end), and the wave reads them
-    // back by lane.
+    // back by lane. The wave's records are consecutive in bucket order, so
+    // the keys of its active lanes do not decrease with the lane: the first
+    // and last active lanes hold kmin and kmax.
     const int key = (gz * h.ny + gy) * h.nx + gx;
-    const int kmin = wave_min_i(active ? key : 0x7FFFFFFF);
-    const int kmax = wave_max_i(active ? key : -1);
+    const unsigned long long act = __ballot(active);
+    const int kmin = act ? __builtin_amdgcn_readlane(key, __ffsll(static_cast<long long>(act)) - 1) : 0x7FFFFFFF;
+    const int kmax = act ? __builtin_amdgcn_readlane(key, 63 - __clzll(static_cast<long long>(act))) : -1;
     int bound = 0;
     if (kmax >= 0 && lane < 18) {
         const int row = lane >> 1;
@@ -339,10 +345,20 @@ __device__ __forceinline__ void staged_walk(const HashGeo& h, const Rec<P>* rec,
             total += b > a ? b - a : 0;
         }
         if (total <= SEG) {
+            // The three ranges end to end, up to four records per lane loaded
+            // before any is stored; positions past the end are clamped to
+            // the last one, whose record is read and written again unchanged.
+            for (int v = lane; v < total; v += 4 * 64) {
+                Rec<P> t[4];
+                int w[4];
 #pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                const int len = (j < 2 ? off[j + 1] : total) - off[j];
-                for (int t = lane; t < len; t += 64) seg[off[j] + t] = rec[lo[j] + t];
+                for (int u = 0; u < 4; ++u) {
+                    w[u] = min(v + 64 * u, total - 1);
+                    const int j = w[u] < off[1] ? 0 : (w[u] < off[2] ? 1 : 2);
+                    t[u] = rec[lo[j] + (w[u] - off[j])];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) seg[w[u]] = t[u];
             }
             wave_sync();
 #pragma unroll
