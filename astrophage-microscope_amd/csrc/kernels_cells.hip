@@ -690,10 +690,13 @@ struct KLifeApply {
     if (births > 0 || compacting || c_on) {
         xcd_tiles(vg, np, [&](const int i, const bool valid) {
             if (!valid) return;
-            // The voxel record and the slot are loaded together (in bounds
+            // The voxel record, the slot and (when anything divides) the
+            // division test's flags and CO2 are loaded together (in bounds
             // for i below nphys).
             const uint32_t v = c_on ? dm.vox[i] : NO_VOXEL;
             const uint32_t s = c.lslot[i];
+            const uint32_t f = births > 0 ? c.flags[i] : 0u;
+            const double held = births > 0 ? c.co2_held[i] : 0.0;
             if (c_on) {
                 // The CO2 flush of shared voxels (lifecycle.cu:235) by their
                 // owners, and the demand cleared for the next tick.
@@ -734,9 +737,8 @@ struct KLifeApply {
                 if (below) c.lslot[i] = s - below;
             }
             if (births <= 0) return;
-            const uint32_t f = c.flags[i];
             if (!(f & ASTRO_FLAG_OCCUPIED) || !(f & ASTRO_FLAG_ALIVE)) return;
-            if (!(c.co2_held[i] >= t.quota)) return;
+            if (!(held >= t.quota)) return;
             const int32_t pre = prefix[s];
             const int32_t lslot = n + pre;
             if (lslot >= ctr->capacity) return;
