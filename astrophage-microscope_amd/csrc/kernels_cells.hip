@@ -134,13 +134,16 @@ struct KThermal {
 
     const int n = ctr->nphys;
     const auto body = [&](const int i) {
-    // Flags, position and energy are loaded together, ahead of the branch on
-    // the flags, so one memory latency precedes the voxel read.
+    // Flags and position are loaded together, ahead of the branch on the
+    // flags, so one memory latency precedes the voxel read; a cell already
+    // awake and alive fetches its energy beside that read (a cell that
+    // ignites below fetches it then).
     const uint32_t fl = c.flags[i];
     const float xf = static_cast<float>(c.x[i]), yf = static_cast<float>(c.y[i]), zf = static_cast<float>(c.z[i]);
-    const double e0 = c.energy[i];
     if (!(fl & ASTRO_FLAG_OCCUPIED)) return;
     const uint32_t idx = grid_nearest(T, xf, yf, zf);
+    const bool early = (fl & ASTRO_FLAG_AWAKE) && (fl & ASTRO_FLAG_ALIVE);
+    double e0 = early ? c.energy[i] : 0.0;
     const float t_grid = T.value[idx];
     bool awake = (fl & ASTRO_FLAG_AWAKE) != 0u;
     uint32_t nf = fl;
@@ -152,6 +155,7 @@ struct KThermal {
     const double t_cell = awake ? ASTRO_CELL_TEMP_SETPOINT : static_cast<double>(t_grid);
     c.temp_cell[i] = static_cast<float>(t_cell);
     if (!awake || !(fl & ASTRO_FLAG_ALIVE)) return;
+    if (!early) e0 = c.energy[i];
     const double e = lumped_exchange_energy(t_cell, static_cast<double>(t_grid), ASTRO_CONDUCTION_COEFF, cap, dt_sub);
     double energy = e0 - e;
     double moved = e;
