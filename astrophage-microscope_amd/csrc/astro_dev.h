@@ -112,7 +112,10 @@ constexpr uint32_t REC_SLOT_MASK = 0x3FFFFFFFu;
 constexpr uint32_t REC_UNOCC = 1u << 30;
 constexpr uint32_t REC_ALIVE = 1u << 31;
 
-struct alignas(32) Force { double x, y, z, _pad; };
+// A cell's force, 24 bytes with no padding: the force buffer crosses HBM
+// about four times per cell per tick (sample, contact read and write,
+// integrate).
+struct Force { double x, y, z; };
 
 // ---------------------------------------------------------------------------
 // PCG32 per-cell streams (src/core/rng.cuh:23-117)

@@ -32,7 +32,7 @@ def kernel_bytes(name: str, c: dict) -> float:
     # per-cell bytes of the integrator: flags, t_local, biomass, energy, the
     # force (via spos), position and velocity, id, RNG state and age in;
     # RNG state, flags, position, velocity and age out
-    integ = n * (96 + 12 * p)
+    integ = n * (88 + 12 * p)
     table = {
         "hash_key": n * (16 + 3 * p),
         "hash_bucket_scan": 8 * (c["buckets"] + 1) + 4 * min(n, c["buckets"]),
@@ -52,10 +52,10 @@ def kernel_bytes(name: str, c: dict) -> float:
         "feed": 4 * n + 24 * alive,
         # lslot, flags, position, biomass, energy, emit power, spos in; the
         # local field samples (one value per sampled field) and the force out
-        "sample_base": n * (4 + 4 + 3 * p + 8 + 8 + 4 + 4 + 4 * k_fields + 12 + 32) + awake * 12,
+        "sample_base": n * (4 + 4 + 3 * p + 8 + 8 + 4 + 4 + 4 * k_fields + 12 + 24) + awake * 12,
         # own record, the force read and written; neighbour records and
         # bucket starts are re-reads of data other lanes fetched
-        "contact_walk": n * (r + 64),
+        "contact_walk": n * (r + 48),
         # sample_base with feeding and/or taxis in the same pass: both
         # kernels' bytes, the flags read once
         "taxis_sample": 0.0,
