@@ -395,9 +395,9 @@ constexpr int CONTACT_ROUND = 32;
 //     differences rounded to float, squared and summed with fused
 //     multiply-adds) is within a few float roundings (relative 2^-21) of the
 //     double r^2, so a 2^-10 margin keeps every pair the reference counts as
-//     touching. A lane marks
-//     the steps that pass in a 32-bit mask, and after each round of at most
-//     32 steps queues those records' sorted positions in walk order.
+//     touching. A lane marks the steps that pass in a 32-bit mask, and after
+//     each round of at most 32 steps queues those records' sorted positions
+//     in walk order.
 //   * The queue is drained in order through the exact double-precision
 //     contact_force -- when it fills, and at the end -- so F sums the same
 //     terms in the same order as the reference. Pairs the reference finds
