@@ -489,6 +489,12 @@ __device__ __forceinline__ int xcd_tile(int b, int nlaunched) {
     return (b % NUM_XCD) * per + b / NUM_XCD;
 }
 
+// One block per BLOCK-entry tile of n entries, a multiple of NUM_XCD. With
+// more blocks than fit at once, blocks start as slots free up and a CU's
+// waves drift out of step, so one wave's memory waits overlap another's
+// arithmetic (passes that walk tiles with xcd_tiles work with any grid).
+__host__ __device__ __forceinline__ int tile_grid(int n) { return blocks_for_xcd(n < 1 ? 1 : (n + BLOCK - 1) / BLOCK); }
+
 // Grid size of a per-cell pass over up to n entries: at most MAX_BLOCKS, a
 // multiple of NUM_XCD.
 __host__ __device__ __forceinline__ int cell_grid(int n) { return blocks_for_xcd(stride_grid(n)); }

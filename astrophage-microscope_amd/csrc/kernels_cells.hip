@@ -1074,7 +1074,7 @@ void launch_lifecycle(const Launch& L, Views<P> ws, Ticks ts, bool demand_booked
         const WorldDev<P>& w = ws[i];
         b.add({w.c, w.ctr, demand_args(w, ts[i]), c_on ? 1 : 0, ts[i].store_disposition, ts[i].quota, w.dflag, w.dsum,
                w.drop, w.xsum},
-              cell_grid(w.pcap));
+              tile_grid(w.capacity));
     }
     launch_k<K>(L, "uptake_mark", b.args, b.blocks);
     launch_life_tail<P>(L, ws, ts, false);

@@ -739,12 +739,10 @@ template <class P>
 void launch_contact_walk(const Launch& L, Views<P> ws, Ticks ts) {
     if (!ts[0].contact) return;
     Batch<KContactWalk<P>> b;
-    // One block per 256-record tile rather than the usual capped grid: blocks
-    // then start as slots free up, so a CU's waves drift out of step and one
+    // One block per record tile rather than the usual capped grid: one
     // wave's staging waits overlap another's screening.
     for (size_t i = 0; i < ws.size(); ++i)
-        b.add({ws[i].ctr, ws[i].hg, ws[i].rec, ws[i].bstart, ws[i].fs, ts[i].k_contact},
-              blocks_for_xcd((ws[i].capacity + BLOCK - 1) / BLOCK));
+        b.add({ws[i].ctr, ws[i].hg, ws[i].rec, ws[i].bstart, ws[i].fs, ts[i].k_contact}, tile_grid(ws[i].capacity));
     // Five waves per SIMD, as the LDS allows (28 KB per block).
     launch_k<KContactWalk<P>, 5>(L, "contact_walk", b.args, b.blocks);
 }
