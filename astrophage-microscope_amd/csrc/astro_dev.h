@@ -354,12 +354,6 @@ __device__ __forceinline__ void division_axis(uint64_t did, double& ax, double& 
     az = u;
 }
 
-__device__ __forceinline__ double lumped_exchange_energy(double tc, double tg, double g, double cap, double dt) {
-    if (cap <= 0.0 || g <= 0.0) return 0.0;
-    const double relax = -expm1(-g * dt / cap);
-    return cap * (tc - tg) * relax;
-}
-
 __device__ __forceinline__ double disc_overlap_fraction(double p, double a) {
     if (p <= 0.0) return 1.0;
     if (p >= 2.0 * a) return 0.0;

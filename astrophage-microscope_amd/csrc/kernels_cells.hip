@@ -3,6 +3,7 @@
 // the per-cell telemetry reduction. One thread per slot; every per-cell array
 // is read and written coalesced in slot order.
 #include <algorithm>
+#include <cmath>
 #include <type_traits>
 
 #include "cell_ops.h"
@@ -123,14 +124,14 @@ struct KThermal {
         Cells<P> c;
         const Counters* ctr;
         Grid T;
-        double dt_sub;
+        double relax;   // -expm1(-g dt_sub / cap), the same for every cell of the substep
         double cap;
         Rec<P>* rec;
         const int32_t* spos;
         int update_rec;
     };
     static __device__ void run(const Args& a, VGrid vg) {
-        const auto& [c, ctr, T, dt_sub, cap, rec, spos, update_rec] = a;
+        const auto& [c, ctr, T, relax, cap, rec, spos, update_rec] = a;
 
     const int n = ctr->nphys;
     const auto body = [&](const int i) {
@@ -156,7 +157,9 @@ struct KThermal {
     c.temp_cell[i] = static_cast<float>(t_cell);
     if (!awake || !(fl & ASTRO_FLAG_ALIVE)) return;
     if (!early) e0 = c.energy[i];
-    const double e = lumped_exchange_energy(t_cell, static_cast<double>(t_grid), ASTRO_CONDUCTION_COEFF, cap, dt_sub);
+    // lumped_exchange_energy (src/sim/thermal.cuh:80-86) with its relaxation factor
+    // taken once per substep on the host.
+    const double e = cap > 0.0 ? cap * (t_cell - static_cast<double>(t_grid)) * relax : 0.0;
     double energy = e0 - e;
     double moved = e;
     if (energy < 0.0) {
@@ -948,7 +951,10 @@ void launch_thermal_substep(const Launch& L, Views<P> ws, Ticks ts, double dt_su
     Batch<K> b;
     for (size_t i = 0; i < ws.size(); ++i) {
         const WorldDev<P>& w = ws[i];
-        b.add({w.c, w.ctr, w.g[ASTRO_FIELD_TEMPERATURE], dt_sub, ts[i].T_cap, w.rec, w.spos, update_rec ? 1 : 0},
+        const double cap = ts[i].T_cap;
+        const double g = ASTRO_CONDUCTION_COEFF;
+        const double relax = (cap <= 0.0 || g <= 0.0) ? 0.0 : -std::expm1(-g * dt_sub / cap);
+        b.add({w.c, w.ctr, w.g[ASTRO_FIELD_TEMPERATURE], relax, cap, w.rec, w.spos, update_rec ? 1 : 0},
               cell_grid(w.pcap));
     }
     launch_k<K>(L, "thermal", b.args, b.blocks);
