@@ -154,7 +154,28 @@ __global__ __launch_bounds__(256) void k_conv_rows(const bf16* __restrict__ X, c
     f32x16 acc[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) acc[k] = f32x16{};
-    for (int t = 0; t < TAPS; ++t) {
+    // A tile whose every operand lies inside the volume (rows y-1 .. y+R,
+    // planes z-1 .. z+1, voxels x0-1 .. x0+32) takes each operand at one base
+    // address plus a wave-uniform offset per tap and row, with no bounds
+    // tests or zero selects; the same operands meet the same MFMAs in the
+    // same order as below.
+    const bool inner = x0 > 0 && x0 + 32 < nx && y > 0 && y + R < ny && z > 0 && z + 1 < nz;   // wave-uniform
+    if (inner) {
+        const bf16* base = X + ((static_cast<int64_t>(z) * ny + y) * nx + x0 + i) * CH + h * 8;
+        for (int t = 0; t < TAPS; ++t) {
+            const int64_t toff = ((static_cast<int64_t>(tap_dz(t)) * ny + tap_dy(t)) * nx + tap_dx(t)) * CH;
+            const bf16x8* wp = reinterpret_cast<const bf16x8*>(W + (static_cast<int64_t>(t) * CH + i) * CH + h * 8);
+            const bf16x8 b0 = wp[0], b1 = wp[2];
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                const bf16x8* pa =
+                    reinterpret_cast<const bf16x8*>(base + toff + static_cast<int64_t>(k) * nx * CH);
+                acc[k] = mfma(pa[0], b0, acc[k]);
+                acc[k] = mfma(pa[2], b1, acc[k]);
+            }
+        }
+    }
+    for (int t = 0; t < TAPS && !inner; ++t) {
         const int zz = z + tap_dz(t);
         if (zz < 0 || zz >= nz) continue;   // wave-uniform
         const int xx = x0 + i + tap_dx(t);
