@@ -97,6 +97,21 @@ __global__ __launch_bounds__(256) void k_conv(const bf16* __restrict__ X, const 
             acc = mfma(a0, wp[0], acc);
             acc = mfma(a1, wp[2], acc);
         }
+    } else if (x0 > 0 && x0 + 32 < nx && y > 0 && y + 1 < ny && z > 0 && z + 1 < nz) {
+        // 8 input channels, a tile with every operand inside the volume: the
+        // same taps and MFMAs as below at a base address plus a per-tap
+        // offset, with no bounds tests.
+        const bf16* base = X + ((static_cast<int64_t>(z) * ny + y) * nx + x0 + i) * CIN0;
+        for (int m = 0; m < (TAPS + 1) / 2; ++m) {
+            const int t = 2 * m + h;
+            bf16x8 a = zero8, b = zero8;
+            if (t < TAPS) {
+                const int64_t toff = ((static_cast<int64_t>(tap_dz(t)) * ny + tap_dy(t)) * nx + tap_dx(t)) * CIN0;
+                a = *reinterpret_cast<const bf16x8*>(base + toff);
+                b = *reinterpret_cast<const bf16x8*>(W + (static_cast<int64_t>(t) * CH + i) * CIN0);
+            }
+            acc = mfma(a, b, acc);
+        }
     } else {
         // 8 input channels: the two lane halves take two taps per k step.
         for (int m = 0; m < (TAPS + 1) / 2; ++m) {
