@@ -149,6 +149,53 @@ def test_contact_forces_dense_cluster(oracle, gpu, precision, dims):
                 rel_close(a, b, 1e-6, name=name)
 
 
+# The contact diameter 2a (ASTRO_CELL_DIAMETER, include/astro_canon.h).
+TWO_A = 1.0000000000000001e-05
+
+
+@pytest.mark.parametrize("precision", [ah.FP64_CONTRACT, ah.FP32_HOT])
+def test_contact_screen_threshold_pairs(oracle, gpu, precision):
+    """Isolated pairs of cells whose separations lie within 2e-6 (relative)
+    of the contact diameter, on either side of it and in random directions:
+    the walk's float pre-screen must keep every pair the double test counts
+    as touching, so the forces are the oracle's bit for bit (FP64 layout) or
+    to float rounding of the body force (FP32 layout)."""
+    kw = dict(C1, capacity=1000, hash_cell_size=1e-5, precision=precision)
+    xs = -240e-6 + 40e-6 * np.arange(13)
+    centres = np.array([(x, y, z) for z in (-25e-6, 25e-6) for y in xs for x in xs])
+    npair = centres.shape[0]
+    cells = oracle_cells(oracle, 2 * npair, 3, **kw)
+    rng = np.random.default_rng(11)
+    rel = np.concatenate([np.linspace(-2e-6, 2e-6, npair // 2), rng.uniform(-2e-6, 2e-6, npair - npair // 2)])
+    dirs = rng.normal(size=(npair, 3))
+    dirs[:6] = np.eye(3)[[0, 1, 2, 0, 1, 2]]   # a few along the axes
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    half = 0.5 * TWO_A * (1.0 + rel)[:, None] * dirs
+    pos = np.empty((2 * npair, 3))
+    pos[0::2] = centres + half
+    pos[1::2] = centres - half
+    if precision == ah.FP32_HOT:
+        pos = pos.astype(np.float32).astype(np.float64)
+    for j, k in enumerate(("x", "y", "z")):
+        cells[k] = pos[:, j].copy()
+    r = np.linalg.norm(pos[0::2] - pos[1::2], axis=1)
+    near = np.abs(r / TWO_A - 1.0) < 3e-6
+    assert near.all()
+    assert (r < TWO_A).sum() > 100 and (r >= TWO_A).sum() > 100   # both sides of the threshold
+    o, g = pair(oracle, gpu, cells, **kw)
+    with o, g:
+        o.hash_build()
+        g.hash_build()
+        o.stage("forces")
+        g.stage("forces")
+        fo, fg = o.forces(), g.forces()
+        for a, b, name in zip(fo, fg, "xyz"):
+            if precision == ah.FP64_CONTRACT:
+                assert np.array_equal(a, b), name
+            else:
+                rel_close(a, b, 1e-6, name=name)
+
+
 @pytest.mark.parametrize("dims,bc_field", [((128, 128, 1), ah.TEMPERATURE), ((96, 80, 24), ah.TEMPERATURE),
                                            ((64, 64, 16), ah.CO2)])
 def test_ftcs_parity(oracle, gpu, dims, bc_field):
