@@ -196,6 +196,50 @@ def test_contact_screen_threshold_pairs(oracle, gpu, precision):
                 rel_close(a, b, 1e-6, name=name)
 
 
+@pytest.mark.parametrize("precision", [ah.FP64_CONTRACT, ah.FP32_HOT])
+@pytest.mark.parametrize("light", [(1.0, 0.0), (0.6, 0.8)])
+def test_shadow_screen_threshold_pairs(oracle, gpu, precision, light):
+    """Isolated pairs, an awake cell and an upstream neighbour placed at the
+    edges of the shadow walk's float screen: half at a perpendicular offset
+    within 1e-4..1e-3 (relative) of 2a, inside and outside; half 5 um off the
+    light line with an along-light offset within +-1e-7..1e-5 of zero. On the
+    oracle the near-field product moves the shaded cell's irradiance by at
+    least 12 float ulps in every shading case and by none otherwise, so
+    agreement to two ulps per cell means the screen kept every shading
+    neighbour and no other."""
+    kw = dict(C1, capacity=1000, hash_cell_size=1e-5, thermal_enabled=0, light_enabled=1, light_irradiance=1e4,
+              light_dir_x=light[0], light_dir_y=light[1], precision=precision)
+    xs = -240e-6 + 40e-6 * np.arange(13)
+    centres = np.array([(x, y, z) for z in (-25e-6, 25e-6) for y in xs for x in xs])
+    npair = centres.shape[0]
+    cells = oracle_cells(oracle, 2 * npair, 3, spawn=dict(awake=True, charge_dist=ah.DIST_UNIFORM, charge_b=0.9),
+                         **kw)
+    ld = np.array([light[0], light[1], 0.0])
+    ld /= np.linalg.norm(ld)
+    across = np.array([-ld[1], ld[0], 0.0])
+    h = npair // 2
+    q = (npair - h) // 2
+    perp = np.concatenate([TWO_A * (1 - np.logspace(-4, -3, h // 2)), TWO_A * (1 + np.logspace(-4, -3, h - h // 2)),
+                           np.full(npair - h, 5e-6)])
+    along = np.concatenate([np.full(h, 6e-6), 5e-6 * np.logspace(-7, -5, q), -5e-6 * np.logspace(-7, -5, npair - h - q)])
+    pos = np.empty((2 * npair, 3))
+    pos[0::2] = centres + 0.5 * along[:, None] * ld                          # the shaded cell
+    pos[1::2] = centres - 0.5 * along[:, None] * ld + perp[:, None] * across  # its upstream neighbour
+    if precision == ah.FP32_HOT:
+        pos = pos.astype(np.float32).astype(np.float64)
+    for j, k in enumerate(("x", "y", "z")):
+        cells[k] = pos[:, j].copy()
+    o, g = pair(oracle, gpu, cells, **kw)
+    with o, g:
+        o.step(1)
+        g.step(1)
+        a, b = o.download()["irradiance"], g.download()["irradiance"]
+    ulp = np.spacing(np.abs(a).astype(np.float32)).astype(np.float64)
+    d = np.abs(a.astype(np.float64) - b.astype(np.float64))
+    bad = np.nonzero(d > 2 * ulp)[0]
+    assert bad.size == 0, f"irradiance differs at cells {bad[:10]}: {a[bad[:5]]} vs {b[bad[:5]]}"
+
+
 @pytest.mark.parametrize("dims,bc_field", [((128, 128, 1), ah.TEMPERATURE), ((96, 80, 24), ah.TEMPERATURE),
                                            ((64, 64, 16), ah.CO2)])
 def test_ftcs_parity(oracle, gpu, dims, bc_field):
