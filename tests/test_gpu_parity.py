@@ -240,6 +240,48 @@ def test_shadow_screen_threshold_pairs(oracle, gpu, precision, light):
     assert bad.size == 0, f"irradiance differs at cells {bad[:10]}: {a[bad[:5]]} vs {b[bad[:5]]}"
 
 
+@pytest.mark.parametrize("precision", [ah.FP64_CONTRACT, ah.FP32_HOT])
+def test_hash_bucket_edges(oracle, gpu, precision):
+    """Cells on bucket edges and up to two ulps (of the layout's coordinate
+    type) to either side. The hash keeps its origin and inverse cell size as
+    floats (the reference's geometry), so an edge sits at
+    float(origin) + k / float(1 / cell), not at origin + k * cell; positions
+    within two ulps of that straddle it. The bucket of each cell, and so the
+    sorted slots and bucket ranges, are the oracle's bit for bit."""
+    kw = dict(C1, capacity=2000, hash_cell_size=1e-5, precision=precision)
+    n = 1500
+    cells = oracle_cells(oracle, n, 5, **kw)
+    with ah.World(oracle, **kw) as w:
+        nx, ny, nz, cell = w.hash_dims()
+    dims = np.array([nx, ny, nz])
+    inv = np.float64(np.float32(1.0 / cell))
+    rng = np.random.default_rng(17)
+    ftype = np.float32 if precision == ah.FP32_HOT else np.float64
+    pos = np.empty((n, 3))
+    straddle = 0
+    for a in range(3):
+        origin = np.float64(np.float32(-0.5 * dims[a] * cell))
+        k = rng.integers(1, dims[a], n)
+        v = (origin + k / inv).astype(ftype)
+        step = rng.integers(-2, 3, n)
+        for _ in range(2):
+            v = np.where(step > 0, np.nextafter(v, ftype(np.inf)), np.where(step < 0, np.nextafter(v, ftype(-np.inf)), v))
+            step = step - np.sign(step)
+        pos[:, a] = v.astype(np.float64)
+        straddle += np.count_nonzero(np.trunc((pos[:, a] - origin) * inv) == k - 1)
+    assert straddle > n // 4   # a good share of the cells fall just below an edge
+    for j, k in enumerate(("x", "y", "z")):
+        cells[k] = pos[:, j].copy()
+    o, g = pair(oracle, gpu, cells, **kw)
+    with o, g:
+        o.hash_build()
+        g.hash_build()
+        vo, so, eo = o.hash_download()
+        vg, sg, eg = g.hash_download()
+        assert np.array_equal(vo, vg), f"vals_sorted differs at {np.nonzero(vo != vg)[0][:10]}"
+        assert np.array_equal(so, sg) and np.array_equal(eo, eg)
+
+
 @pytest.mark.parametrize("dims,bc_field", [((128, 128, 1), ah.TEMPERATURE), ((96, 80, 24), ah.TEMPERATURE),
                                            ((64, 64, 16), ah.CO2)])
 def test_ftcs_parity(oracle, gpu, dims, bc_field):
