@@ -375,6 +375,37 @@ def test_tick_parity_awake_lit(oracle, gpu, light):
     rel_close(o.field(ah.CO2), g.field(ah.CO2), 1e-5, name="CO2")
 
 
+@pytest.mark.parametrize("precision", [ah.FP64_CONTRACT, ah.FP32_HOT])
+@pytest.mark.parametrize("n", [0, 1, 2])
+def test_tick_parity_tiny_worlds(oracle, gpu, precision, n):
+    """Worlds of zero, one and two awake cells (capacity equal to the count,
+    one for the empty world), light, thermal and feeding on, 5 ticks: every
+    per-cell pass, the hash and the field passes run over empty or nearly
+    empty ranges. Integer state exact, fields and cell state as the
+    lit-tick bar."""
+    kw = dict(REF2D, capacity=max(n, 1), chamber_w=1e-3, chamber_h=1e-3, ambient_temp=300.0, light_enabled=1,
+              light_irradiance=1.0e4, co2_init=1e-3, biology_rate=1e6, precision=precision)
+    o, g = pair(oracle, gpu, **kw)
+    if n:
+        for w in (o, g):
+            w.spawn(n, 5, charge_dist=ah.DIST_UNIFORM, charge_b=0.9, awake=True)
+    o.step(5)
+    g.step(5)
+    a, b = o.download(), g.download()
+    compare_cells(a, b)
+    pos_tol = 1e-9 * 5e-4 if precision == ah.FP64_CONTRACT else FP32_POS_TOL
+    for name in ("x", "y", "z"):
+        rel_close(a[name], b[name], 0.0, atol=pos_tol, name=name)
+    for name in ("energy", "co2_held"):
+        rel_close(a[name], b[name], 1e-9, name=name)
+    rel_close(o.field(ah.TEMPERATURE), g.field(ah.TEMPERATURE), 1e-6, name="T")
+    rel_close(o.field(ah.CO2), g.field(ah.CO2), 1e-5, name="CO2")
+    rel_close(o.field(ah.IRRADIANCE), g.field(ah.IRRADIANCE), 1e-5, name="irradiance")
+    assert o.stats()["n_live"] == g.stats()["n_live"] == n
+    o.close()
+    g.close()
+
+
 def test_rear_cell_irradiance_exactly_zero(gpu, oracle):
     """T13 (tests/physics/test_emission.cu:73-133): collinear pair along the
     light, 15 um apart: the rear cell's irradiance and energy are exactly 0."""
