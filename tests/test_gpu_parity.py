@@ -406,6 +406,51 @@ def test_tick_parity_tiny_worlds(oracle, gpu, precision, n):
     g.close()
 
 
+@pytest.mark.parametrize("precision", [ah.FP32_HOT, ah.FP64_CONTRACT])
+def test_tick_parity_scale_workload(oracle, gpu, precision):
+    """The 1M-cell scale workload itself (4 x 4 x 0.2 mm chamber, 512x512x32
+    fields, 10 um hash, division and compaction on; the bench's own set-up),
+    2 ticks: about two dozen divisions and two hundred deaths compacted away.
+    Ids, flags, death causes and RNG streams equal the oracle's over all
+    1.25M slots and the population ledger is exact. FP64 positions agree to
+    1e-12 m. The oracle keeps positions in double in both layouts, while the
+    FP32 layout rounds them to float at every store; in the piles of deeply
+    overlapping cells that uniform placement makes at this density, the
+    stiff contact forces carry those roundings further. Measured: after two
+    ticks 32, 32 and 9 of 1M cells differ by more than 5e-10 m in x, y and z,
+    the worst by 2.8e-8 m, in one pile of seven dormant cells. Bar: at most
+    1 cell in 2000 beyond 5e-10 m on any axis, none beyond 1e-7 m (2% of
+    the cell radius)."""
+    import sys
+    from pathlib import Path
+    sys.path.insert(0, str(Path(__file__).resolve().parents[1]))
+    import bench
+    cells = 1_000_000
+    o = ah.World(oracle, bench.workload(oracle, cells, precision))
+    g = ah.World(gpu, bench.workload(gpu, cells, precision))
+    for w in (o, g):
+        bench.populate(w, cells)
+        w.step(2)
+    a, b = o.download(), g.download()
+    compare_cells(a, b)
+    if precision == ah.FP64_CONTRACT:
+        for name in ("x", "y", "z"):
+            rel_close(a[name], b[name], 0.0, atol=1e-12, name=name)
+    else:
+        d = np.max([np.abs(a[k] - b[k]) for k in ("x", "y", "z")], axis=0)
+        far = np.count_nonzero(d > FP32_POS_TOL)
+        assert far <= d.size // 2000 and d.max() <= 1e-7, (far, d.max())
+    for name in ("energy", "co2_held"):
+        rel_close(a[name], b[name], 1e-6, name=name)
+    so, sg = o.stats(), g.stats()
+    for k in ("n_live", "n_dead", "n_awake", "divisions_this_window", "deaths_this_window"):
+        assert so[k] == sg[k], (k, so[k], sg[k])
+    assert so["divisions_this_window"] > 0 and so["deaths_this_window"] > 0
+    assert o.counters().next_id == g.counters().next_id
+    o.close()
+    g.close()
+
+
 def test_rear_cell_irradiance_exactly_zero(gpu, oracle):
     """T13 (tests/physics/test_emission.cu:73-133): collinear pair along the
     light, 15 um apart: the rear cell's irradiance and energy are exactly 0."""
