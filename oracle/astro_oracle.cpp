@@ -537,6 +537,13 @@ struct oracle_world {
 
 namespace {
 
+bool g_float_store = false;
+
+// A position or velocity as the world stores it.
+inline double stored(const astro_world_desc& d, double v) {
+    return (g_float_store && d.precision == ASTRO_FP32_HOT) ? static_cast<double>(static_cast<float>(v)) : v;
+}
+
 int fail(oracle_world* w, int32_t st, const char* what) {
     if (w) w->err = what;
     return st;
@@ -870,8 +877,8 @@ void integrate(oracle_world* w, double dt) {
             w->death_cause[i] = ASTRO_DEATH_CULLED;
             w->emit_power[i] = 0.0f;
         }
-        w->x[i] = pos.x; w->y[i] = pos.y; w->z[i] = pos.z;
-        w->vx[i] = vel.x; w->vy[i] = vel.y; w->vz[i] = vel.z;
+        w->x[i] = stored(d, pos.x); w->y[i] = stored(d, pos.y); w->z[i] = stored(d, pos.z);
+        w->vx[i] = stored(d, vel.x); w->vy[i] = stored(d, vel.y); w->vz[i] = stored(d, vel.z);
         w->age_s[i] += static_cast<float>(dt);
     }
 }
@@ -1004,6 +1011,9 @@ void lifecycle_step(oracle_world* w, double dt) {
                 w->z[slot] = clampv(w->z[slot], hz);
                 w->x[i] = clampv(w->x[i], hx); w->y[i] = clampv(w->y[i], hy);
                 w->z[i] = clampv(w->z[i], hz);
+                for (size_t j : {static_cast<size_t>(slot), static_cast<size_t>(i)}) {
+                    w->x[j] = stored(d, w->x[j]); w->y[j] = stored(d, w->y[j]); w->z[j] = stored(d, w->z[j]);
+                }
                 w->vx[slot] = w->vx[i]; w->vy[slot] = w->vy[i]; w->vz[slot] = w->vz[i];
                 w->id[slot] = did;
                 w->flags[slot] = w->flags[i];
@@ -1360,6 +1370,12 @@ void get(const std::vector<T>& src, T* dst, int32_t n) {
 // C ABI
 // ---------------------------------------------------------------------------
 extern "C" {
+
+// Test-only switch: with it on, an FP32-layout world rounds positions and
+// velocities to float where the device layout stores them (the integrator
+// and division), so the oracle reproduces that layout's storage. Off by
+// default: the oracle then keeps them in double in both layouts.
+void oracle_set_float_store(int on) { g_float_store = on != 0; }
 
 void oracle_world_desc_default(astro_world_desc* d) {
     std::memset(d, 0, sizeof(*d));
