@@ -24,7 +24,7 @@ import numpy as np
 PKG = Path(__file__).resolve().parent
 LIB_PATH = PKG / "lib" / "libastro_hot.so"
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 STATUS = ["Ok", "OutOfMemory", "DeviceError", "InvalidArgument", "NotFound", "ParseError",
           "ContractVersionMismatch", "CapacityExceeded", "FileIoError", "Unsupported"]
 OK, OUT_OF_MEMORY, DEVICE_ERROR, INVALID_ARGUMENT, NOT_FOUND, PARSE_ERROR, \
@@ -89,12 +89,14 @@ class WorldDesc(C.Structure):
         ("thermal_noise", C.c_uint8), ("contact_enabled", C.c_uint8), ("adhesion_enabled", C.c_uint8),
         ("thermal_enabled", C.c_uint8), ("emission_enabled", C.c_uint8), ("taxis_enabled", C.c_uint8),
         ("occlusion_exact", C.c_uint8), ("store_disposition", C.c_uint8), ("compaction_enabled", C.c_uint8),
-        ("lifecycle_enabled", C.c_uint8), ("tau_compaction_enabled", C.c_uint8), ("_pad2", C.c_uint8 * 6),
+        ("lifecycle_enabled", C.c_uint8), ("tau_compaction_enabled", C.c_uint8), ("rng_kind", C.c_uint8),
+        ("_pad2", C.c_uint8 * 5),
         ("ambient_temp", C.c_double), ("co2_init", C.c_double), ("division_offset", C.c_double),
         ("physics_rate", C.c_double), ("biology_rate", C.c_double),
         ("petrova_max_power", C.c_double), ("co2_mass_per_division", C.c_double),
         ("light_dir_x", C.c_float), ("light_dir_y", C.c_float), ("light_irradiance", C.c_float),
         ("ambient_irradiance", C.c_float), ("light_enabled", C.c_uint8), ("_pad3", C.c_uint8 * 7),
+        ("petrova_flash_power", C.c_double),
     ]
 
 
@@ -138,7 +140,7 @@ class Stats(C.Structure):
     ]
 
 
-assert C.sizeof(WorldDesc) == 208, C.sizeof(WorldDesc)
+assert C.sizeof(WorldDesc) == 216, C.sizeof(WorldDesc)
 assert C.sizeof(SpawnParams) == 56
 assert C.sizeof(Counters) == 64
 assert C.sizeof(Stats) == 144

@@ -125,7 +125,7 @@ struct TickParams {
     int    thermal_noise, contact, adhesion, occlusion_exact;
     int    store_disposition, compaction;
     double k_contact;
-    double petrova_max_power, quota, division_offset;
+    double petrova_max_power, petrova_flash_power, quota, division_offset;
     // light
     float  source, ambient_irr;
     double ldx, ldy, ldz;

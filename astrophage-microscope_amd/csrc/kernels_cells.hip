@@ -256,10 +256,11 @@ struct KFlash {
         Cells<P> c;
         const Counters* ctr;
         double dt;
+        double power;   // World::petrova_flash_power [W]
         unsigned long long* accum;
     };
     static __device__ void run(const Args& a, VGrid vg) {
-        const auto& [c, ctr, dt, accum] = a;
+        const auto& [c, ctr, dt, power, accum] = a;
         xcd_tiles(vg, ctr->nphys, [&](const int i, const bool valid) {
             if (!valid) return;
             const uint32_t f = c.flags[i];
@@ -267,7 +268,7 @@ struct KFlash {
             const double e = c.energy[i];
             c.emit_power[i] = 0.0f;
             if (e <= 0.0) return;
-            double de = ASTRO_PETROVA_FLASH_POWER * dt;
+            double de = power * dt;
             if (de > e) de = e;
             c.energy[i] = e - de;
             deposit_add(&accum[2], -(de / ASTRO_C_LIGHT), ASTRO_FLASH_IMPULSE_SCALE);
@@ -991,7 +992,7 @@ template <class P>
 void launch_flash(const Launch& L, Views<P> ws, Ticks ts) {
     using K = KFlash<P>;
     Batch<K> b;
-    for (size_t i = 0; i < ws.size(); ++i) b.add({ws[i].c, ws[i].ctr, ts[i].dt, ws[i].flash}, cell_grid(ws[i].pcap));
+    for (size_t i = 0; i < ws.size(); ++i) b.add({ws[i].c, ws[i].ctr, ts[i].dt, ts[i].petrova_flash_power, ws[i].flash}, cell_grid(ws[i].pcap));
     launch_k<K>(L, "flash", b.args, b.blocks);
 }
 

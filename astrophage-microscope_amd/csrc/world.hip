@@ -231,6 +231,10 @@ struct astro_world {
     hipGraphExec_t gexec = nullptr;
     bool graph_ok = false;
     int graph_ticks = 1;
+    // Bumped by every change that drops the captured tick (clock, light,
+    // flash, stage calls, field swaps, ...). An ensemble that captured this
+    // world's launch arguments compares it to know when to capture again.
+    uint64_t generation = 0;
 
     std::unique_ptr<EventProfiler> prof;
     bool prof_on = false;
@@ -310,6 +314,7 @@ TickParams tick_params(const astro_world* w, double dt) {
     t.store_disposition = d.store_disposition; t.compaction = d.compaction_enabled;
     t.k_contact = ASTRO_CONTACT_STIFFNESS / w->physics_rate;
     t.petrova_max_power = d.petrova_max_power;
+    t.petrova_flash_power = d.petrova_flash_power;
     t.quota = d.co2_mass_per_division;
     t.division_offset = d.division_offset;
     t.source = d.light_enabled ? d.light_irradiance : 0.0f;
@@ -508,6 +513,7 @@ int ticks_per_graph(const astro_world* w) {
 }
 
 void drop_graph(astro_world* w) {
+    ++w->generation;
     if (w->gexec) hipGraphExecDestroy(w->gexec);
     if (w->graph) hipGraphDestroy(w->graph);
     w->gexec = nullptr;
@@ -702,7 +708,9 @@ std::vector<SnapOverride> overrides_of(const astro_world_desc& d) {
         o.value = v;
         out.push_back(o);
     };
+    // the reference's key order (src/sim/snapshot.cu:79-83)
     add("PETROVA_MAX_POWER", d.petrova_max_power, ASTRO_PETROVA_MAX_POWER);
+    add("PETROVA_FLASH_POWER", d.petrova_flash_power, ASTRO_PETROVA_FLASH_POWER);
     add("CO2_MASS_PER_DIVISION", d.co2_mass_per_division, ASTRO_CO2_MASS_PER_DIVISION);
     return out;
 }
@@ -892,6 +900,10 @@ struct astro_ensemble {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int graph_ticks = 0;
+    // Each chamber's generation when the graph was captured: a chamber whose
+    // generation has moved since (its light, clock, flash or field buffers
+    // changed) has launch arguments the graph no longer holds.
+    std::vector<uint64_t> captured_gen;
 
     // Eagerly issued ticks (profiling, or a tick left over from the graph's
     // period) stage their arguments here.
@@ -997,7 +1009,18 @@ int32_t capture_step(astro_ensemble* e) {
         return st;
     }
     e->graph_ticks = k;
+    e->captured_gen.clear();
+    for (astro_world* w : e->worlds) e->captured_gen.push_back(w->generation);
     return ASTRO_OK;
+}
+
+// True when some chamber changed in a way that dropped its own captured tick
+// since the ensemble's graph was captured.
+bool chambers_changed(const astro_ensemble* e) {
+    if (e->captured_gen.size() != e->worlds.size()) return true;
+    for (size_t i = 0; i < e->worlds.size(); ++i)
+        if (e->worlds[i]->generation != e->captured_gen[i]) return true;
+    return false;
 }
 
 int32_t eager_tick(astro_ensemble* e) {
@@ -1055,6 +1078,7 @@ void astro_world_desc_default(astro_world_desc* d) {
     d->biology_rate = 1.0;
     d->petrova_max_power = ASTRO_PETROVA_MAX_POWER;
     d->co2_mass_per_division = ASTRO_CO2_MASS_PER_DIVISION;
+    d->petrova_flash_power = ASTRO_PETROVA_FLASH_POWER;
     d->light_dir_x = 1.0f;
 }
 
@@ -1092,6 +1116,8 @@ int32_t astro_world_create(const astro_world_desc* din, astro_world** out) {
     if (cap <= 0 || cap > ASTRO_MAX_CELLS) return ASTRO_INVALID_ARGUMENT;
     if (!(d.chamber_w > 0 && d.chamber_h > 0 && d.chamber_d > 0)) return ASTRO_INVALID_ARGUMENT;
     if (d.precision != ASTRO_FP64_CONTRACT && d.precision != ASTRO_FP32_HOT) return ASTRO_INVALID_ARGUMENT;
+    if (d.rng_kind != ASTRO_RNG_PCG32 && d.rng_kind != ASTRO_RNG_PHILOX) return ASTRO_INVALID_ARGUMENT;
+    if (!(d.petrova_flash_power >= 0.0)) return ASTRO_INVALID_ARGUMENT;
     d.capacity = cap;
     const int32_t tau_cap = d.tau_capacity > 0 ? d.tau_capacity : ASTRO_DEFAULT_TAUMOEBA;
     if (tau_cap > ASTRO_MAX_TAUMOEBA) return ASTRO_INVALID_ARGUMENT;
@@ -2039,6 +2065,7 @@ int32_t astro_snapshot_from_bytes(const astro_world_desc* base, const void* data
         std::memcpy(&o, cur, sizeof(o));
         cur += sizeof(o);
         if (std::strncmp(o.key, "PETROVA_MAX_POWER", sizeof(o.key)) == 0) d.petrova_max_power = o.value;
+        else if (std::strncmp(o.key, "PETROVA_FLASH_POWER", sizeof(o.key)) == 0) d.petrova_flash_power = o.value;
         else if (std::strncmp(o.key, "CO2_MASS_PER_DIVISION", sizeof(o.key)) == 0) d.co2_mass_per_division = o.value;
     }
     astro_world* w = nullptr;
@@ -2191,7 +2218,7 @@ int32_t astro_ensemble_step(astro_ensemble* e, int32_t n_ticks) {
     if (!e->prof_on) {
         const int k = ticks_per_graph(e->worlds[0]);
         if (n_ticks >= k) {
-            if (!e->gexec || e->graph_ticks != k) {
+            if (!e->gexec || e->graph_ticks != k || chambers_changed(e)) {
                 if ((st = capture_step(e)) != ASTRO_OK) return st;
             }
             while (n_ticks - done >= e->graph_ticks) {

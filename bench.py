@@ -82,6 +82,55 @@ def populate(w: ah.World, cells: int):
     w.set_counters(ctr)
 
 
+def workload_c1(binding: ah.Binding, precision: int, rng_kind: int = 0, cells: int = 2000):
+    """Plain method, small: 2,000 cells in a 550 x 550 x 100 um slab, 64^3 T
+    at 369.0 K (just below the 369.565 K ignition point), no light,
+    dt = 1e-4 s, 22 um hash."""
+    return binding.desc(
+        capacity=cells, seed=1, chamber_w=550e-6, chamber_h=550e-6, chamber_d=100e-6, precision=precision,
+        rng_kind=rng_kind, ambient_temp=369.0, physics_rate=0.1,
+        grid={ah.TEMPERATURE: (64, 64, 64), ah.CO2: (64, 64, 64), ah.N2: (-1, 0, 0), ah.IRRADIANCE: (64, 64, 64)})
+
+
+def populate_c1(w: ah.World, cells: int = 2000):
+    w.spawn(cells, 1, charge_dist=ah.DIST_UNIFORM, charge_a=0.0, charge_b=0.2)
+
+
+C2_GRID = (256, 256, 64)
+C2_HOT_SIGMA = 40e-6
+C2_HOT_DT = 5.0
+
+
+def workload_c2(binding: ah.Binding, precision: int, rng_kind: int = 0, cells: int = 25_000):
+    """Plain method, real shape: 25,000 awake cells in a 550 x 550 x 200 um
+    box, 256 x 256 x 64 FP32 temperature and nutrient (CO2) grids, collimated
+    light along +x at 1e4 W/m^2 (the reference's lit test value,
+    tests/physics/test_emission.cu:84) swept over a 256 x 256 x 64 grid,
+    dt = 1e-4 s, 22 um hash."""
+    return binding.desc(
+        capacity=int(cells * 1.25), seed=1, chamber_w=550e-6, chamber_h=550e-6, chamber_d=200e-6,
+        precision=precision, rng_kind=rng_kind, physics_rate=0.1,
+        grid={ah.TEMPERATURE: C2_GRID, ah.CO2: C2_GRID, ah.N2: (-1, 0, 0), ah.IRRADIANCE: C2_GRID},
+        light_enabled=1, light_irradiance=1e4, light_dir_x=1.0, light_dir_y=0.0)
+
+
+def c2_hot_spot(w: ah.World) -> np.ndarray:
+    """Ambient plus a 5 K Gaussian (sigma 40 um) at the box centre, on the
+    temperature grid's cell centres, z-major (z, y, x)."""
+    (nx, ny, nz), (dx, dy, dz) = w.field_dims(ah.TEMPERATURE)
+    d = w.desc
+    x = (np.arange(nx) + 0.5) * dx - 0.5 * d.chamber_w
+    y = (np.arange(ny) + 0.5) * dy - 0.5 * d.chamber_h
+    z = (np.arange(nz) + 0.5) * dz - 0.5 * d.chamber_d
+    r2 = z[:, None, None] ** 2 + y[None, :, None] ** 2 + x[None, None, :] ** 2
+    return (d.ambient_temp + C2_HOT_DT * np.exp(-r2 / (2.0 * C2_HOT_SIGMA ** 2))).astype(np.float32).ravel()
+
+
+def populate_c2(w: ah.World, cells: int = 25_000):
+    w.spawn(cells, 1, charge_dist=ah.DIST_LOG_UNIFORM, charge_a=1e-3, charge_b=0.2, awake=True)
+    w.set_field(ah.TEMPERATURE, c2_hot_spot(w))
+
+
 def population(w: ah.World) -> dict:
     s = w.stats()
     return {"n": s["n_live"] + s["n_dead"], "alive": s["n_live"], "awake": s["n_awake"],

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define ASTRO_ABI_VERSION 1
+#define ASTRO_ABI_VERSION 2
 
 /* Status codes, in the order of astro::Status (src/core/result.h:12-23). */
 enum {
@@ -46,6 +46,15 @@ enum {
 };
 
 enum { ASTRO_FP64_CONTRACT = 0, ASTRO_FP32_HOT = 1 };
+
+/* Per-cell random streams. ASTRO_RNG_PCG32 is the reference's generator
+ * (src/core/rng.cuh:23-117): a 64-bit PCG32 state per cell in rng_state,
+ * advanced by every draw. ASTRO_RNG_PHILOX is an addition: Philox4x32-10
+ * (Salmon, Moraes, Dror and Shaw, SC'11), counter based, keyed by the seed,
+ * with the counter (stage and draw, tick, cell id); a cell's draws in a tick
+ * are a pure function of (seed, id, tick), so no state is read or written
+ * per draw and rng_state is unused (0). Taumoeba keep PCG32 streams. */
+enum { ASTRO_RNG_PCG32 = 0, ASTRO_RNG_PHILOX = 1 };
 
 /* Boundary (src/sim/integrator.cuh:227), gravity axis (:175),
  * store disposition (contracts/scenario_v3.h:35). */
@@ -89,18 +98,24 @@ typedef struct astro_world_desc {
     uint8_t  boundary_x, boundary_y, gravity_axis, thermal_noise;
     uint8_t  contact_enabled, adhesion_enabled, thermal_enabled, emission_enabled;
     uint8_t  taxis_enabled, occlusion_exact, store_disposition, compaction_enabled;
-    uint8_t  lifecycle_enabled, tau_compaction_enabled, _pad2[6];
+    uint8_t  lifecycle_enabled, tau_compaction_enabled;
+    uint8_t  rng_kind;           /* ASTRO_RNG_PCG32 (default) | ASTRO_RNG_PHILOX */
+    uint8_t  _pad2[5];
     double   ambient_temp;
     double   co2_init;           /* initial CO2 concentration [kg/m^3] */
     double   division_offset;    /* daughter offset along the axis; 0 = CELL_RADIUS */
     /* clock (src/sim/step.cu:133, :190-215) */
     double   physics_rate, biology_rate;
-    /* curated overrides (src/sim/world.cuh:168-170) */
+    /* curated overrides (src/sim/world.cuh:168-170); petrova_flash_power is
+     * at the end of the struct */
     double   petrova_max_power;
     double   co2_mass_per_division;
     /* directional light (contracts/fields_v1.h:189-194) */
     float    light_dir_x, light_dir_y, light_irradiance, ambient_irradiance;
     uint8_t  light_enabled, _pad3[7];
+    /* World::petrova_flash_power (src/sim/world.cuh:169): the spin-drive
+     * flash's discharge rate [W], a snapshot override like the other two */
+    double   petrova_flash_power;
 } astro_world_desc;
 
 /* SpawnParams (src/sim/cell_store.cuh:18-27). */

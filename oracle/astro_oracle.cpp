@@ -531,17 +531,18 @@ struct oracle_world {
     std::vector<uint64_t> claim;   // World::d_predator_claim, per cell slot
     bool flash_active = false;     // World::flash_active
     uint64_t flash_accum[4] = {0, 0, 0, 0};
+    // Test-only: round FP32-layout positions and velocities to float where
+    // the device layout stores them (oracle_world_set_float_store).
+    bool float_store = false;
 
     std::string err;
 };
 
 namespace {
 
-bool g_float_store = false;
-
 // A position or velocity as the world stores it.
-inline double stored(const astro_world_desc& d, double v) {
-    return (g_float_store && d.precision == ASTRO_FP32_HOT) ? static_cast<double>(static_cast<float>(v)) : v;
+inline double stored(const oracle_world* w, double v) {
+    return (w->float_store && w->desc.precision == ASTRO_FP32_HOT) ? static_cast<double>(static_cast<float>(v)) : v;
 }
 
 int fail(oracle_world* w, int32_t st, const char* what) {
@@ -877,8 +878,8 @@ void integrate(oracle_world* w, double dt) {
             w->death_cause[i] = ASTRO_DEATH_CULLED;
             w->emit_power[i] = 0.0f;
         }
-        w->x[i] = stored(d, pos.x); w->y[i] = stored(d, pos.y); w->z[i] = stored(d, pos.z);
-        w->vx[i] = stored(d, vel.x); w->vy[i] = stored(d, vel.y); w->vz[i] = stored(d, vel.z);
+        w->x[i] = stored(w, pos.x); w->y[i] = stored(w, pos.y); w->z[i] = stored(w, pos.z);
+        w->vx[i] = stored(w, vel.x); w->vy[i] = stored(w, vel.y); w->vz[i] = stored(w, vel.z);
         w->age_s[i] += static_cast<float>(dt);
     }
 }
@@ -1012,7 +1013,7 @@ void lifecycle_step(oracle_world* w, double dt) {
                 w->x[i] = clampv(w->x[i], hx); w->y[i] = clampv(w->y[i], hy);
                 w->z[i] = clampv(w->z[i], hz);
                 for (size_t j : {static_cast<size_t>(slot), static_cast<size_t>(i)}) {
-                    w->x[j] = stored(d, w->x[j]); w->y[j] = stored(d, w->y[j]); w->z[j] = stored(d, w->z[j]);
+                    w->x[j] = stored(w, w->x[j]); w->y[j] = stored(w, w->y[j]); w->z[j] = stored(w, w->z[j]);
                 }
                 w->vx[slot] = w->vx[i]; w->vy[slot] = w->vy[i]; w->vz[slot] = w->vz[i];
                 w->id[slot] = did;
@@ -1324,7 +1325,7 @@ void flash_step(oracle_world* w, double dt) {
         const double e = w->energy[i];
         w->emit_power[i] = 0.0f;
         if (e <= 0.0) continue;
-        double de = ASTRO_PETROVA_FLASH_POWER * dt;
+        double de = w->desc.petrova_flash_power * dt;
         if (de > e) de = e;
         w->energy[i] = e - de;
         w->flash_accum[2] += static_cast<uint64_t>(to_fixed(-(de / ASTRO_C_LIGHT), ASTRO_FLASH_IMPULSE_SCALE));
@@ -1371,11 +1372,15 @@ void get(const std::vector<T>& src, T* dst, int32_t n) {
 // ---------------------------------------------------------------------------
 extern "C" {
 
-// Test-only switch: with it on, an FP32-layout world rounds positions and
-// velocities to float where the device layout stores them (the integrator
-// and division), so the oracle reproduces that layout's storage. Off by
-// default: the oracle then keeps them in double in both layouts.
-void oracle_set_float_store(int on) { g_float_store = on != 0; }
+// Test-only switch, per world: with it on, an FP32-layout world rounds
+// positions and velocities to float where the device layout stores them (the
+// integrator and division), so the oracle reproduces that layout's storage.
+// Off by default: the oracle then keeps them in double in both layouts.
+int32_t oracle_world_set_float_store(oracle_world* w, int32_t on) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    w->float_store = on != 0;
+    return ASTRO_OK;
+}
 
 void oracle_world_desc_default(astro_world_desc* d) {
     std::memset(d, 0, sizeof(*d));
@@ -1401,6 +1406,7 @@ void oracle_world_desc_default(astro_world_desc* d) {
     d->biology_rate = 1.0;
     d->petrova_max_power = ASTRO_PETROVA_MAX_POWER;
     d->co2_mass_per_division = ASTRO_CO2_MASS_PER_DIVISION;
+    d->petrova_flash_power = ASTRO_PETROVA_FLASH_POWER;
     d->light_dir_x = 1.0f;
     d->light_dir_y = 0.0f;
     d->light_irradiance = 0.0f;
@@ -1424,6 +1430,8 @@ int32_t oracle_world_create(const astro_world_desc* din, oracle_world** out) {
     const int32_t cap = d.capacity > 0 ? d.capacity : ASTRO_DEFAULT_CELLS;
     if (cap <= 0 || cap > ASTRO_MAX_CELLS) return ASTRO_INVALID_ARGUMENT;
     if (!(d.chamber_w > 0 && d.chamber_h > 0 && d.chamber_d > 0)) return ASTRO_INVALID_ARGUMENT;
+    if (d.rng_kind != ASTRO_RNG_PCG32 && d.rng_kind != ASTRO_RNG_PHILOX) return ASTRO_INVALID_ARGUMENT;
+    if (!(d.petrova_flash_power >= 0.0)) return ASTRO_INVALID_ARGUMENT;
     d.capacity = cap;
     const int32_t tau_cap = d.tau_capacity > 0 ? d.tau_capacity : ASTRO_DEFAULT_TAUMOEBA;   // step.cu:50
     if (tau_cap > ASTRO_MAX_TAUMOEBA) return ASTRO_INVALID_ARGUMENT;
@@ -1900,6 +1908,7 @@ int32_t oracle_world_stats(oracle_world* w, astro_stats* s) {
     w->deaths_reported = w->deaths_total;
     s->boil_events = s->max_temp_medium_k >= 373.14999999999998 ? 1 : 0;
     s->non_canon_run = (w->desc.petrova_max_power != ASTRO_PETROVA_MAX_POWER ||
+                        w->desc.petrova_flash_power != ASTRO_PETROVA_FLASH_POWER ||
                         w->desc.co2_mass_per_division != ASTRO_CO2_MASS_PER_DIVISION) ? 1 : 0;
     return ASTRO_OK;
 }
@@ -1918,7 +1927,9 @@ int32_t oracle_world_state_hash(oracle_world* w, uint64_t* out) {
         o.value = v;
         h = fnv1a64(&o, sizeof(o), h);
     };
+    // src/sim/snapshot.cu:79-83, in that order
     ov("PETROVA_MAX_POWER", w->desc.petrova_max_power, ASTRO_PETROVA_MAX_POWER);
+    ov("PETROVA_FLASH_POWER", w->desc.petrova_flash_power, ASTRO_PETROVA_FLASH_POWER);
     ov("CO2_MASS_PER_DIVISION", w->desc.co2_mass_per_division, ASTRO_CO2_MASS_PER_DIVISION);
     const size_t n = static_cast<size_t>(w->count);
     auto span = [&](const auto& v) {

@@ -23,9 +23,10 @@ typedef struct oracle_world oracle_world;
 void        oracle_world_desc_default(astro_world_desc* d);
 int32_t     oracle_world_create(const astro_world_desc* d, oracle_world** out);
 void        oracle_world_destroy(oracle_world* w);
-/* Off by default. On: FP32-layout worlds round positions and velocities to
- * float where the device layout stores them (integrator, division). */
-void        oracle_set_float_store(int on);
+/* Test-only, per world, off by default. On: an FP32-layout world rounds
+ * positions and velocities to float where the device layout stores them
+ * (integrator, division). */
+int32_t     oracle_world_set_float_store(oracle_world* w, int32_t on);
 const char* oracle_status_str(int32_t status);
 const char* oracle_last_error(const oracle_world* w);
 int32_t     oracle_world_desc_get(const oracle_world* w, astro_world_desc* out);

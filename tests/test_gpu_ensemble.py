@@ -180,3 +180,61 @@ def test_ensemble_autotune_keeps_results(gpu):
             assert_same(a, b, f"chamber {i} with tuned shapes")
     for w in ens_worlds + solo:
         w.close()
+
+
+def test_ensemble_follows_chamber_changes(gpu):
+    """A chamber changed between ensemble steps (its light, every chamber's
+    clock) is stepped with the new values: the ensemble captures its step
+    again instead of replaying the arguments it captured before."""
+    specs = lit_chambers(gpu, [900, 2400, 300], ah.FP32_HOT, 130)
+    ens_worlds = [make_world(gpu, *s) for s in specs]
+    solo = [make_world(gpu, *s) for s in specs]
+    with ah.Ensemble(gpu, ens_worlds) as e:
+        e.step(4)
+        for w in solo:
+            w.step(4)
+        e.sync()
+        for i, (a, b) in enumerate(zip(ens_worlds, solo)):
+            assert_same(a, b, f"chamber {i} before the change")
+        h_before = [w.state_hash() for w in ens_worlds]
+        for w in (ens_worlds[1], solo[1]):
+            w.set_light(0.0, -1.0, 2.5e4, 5.0, True)
+        for w in ens_worlds + solo:
+            w.set_clock(0.2, 3.0)
+        e.step(4)
+        for w in solo:
+            w.step(4)
+        e.sync()
+        for i, (a, b) in enumerate(zip(ens_worlds, solo)):
+            assert_same(a, b, f"chamber {i} after the change")
+            assert a.time()[1] == b.time()[1], f"chamber {i} clock"
+        assert [w.state_hash() for w in ens_worlds] != h_before
+    for w in ens_worlds + solo:
+        w.close()
+
+
+def test_ensemble_full_shape_seed7(gpu):
+    """The batched-ensemble workload at its full shape: 256 chambers of
+    1 x 1 x 0.2 mm, cell counts log-uniform in [500, 50,000] from seed 7
+    (about 2.76M cells in all), 64^3 temperature and irradiance grids each, a
+    random in-plane light direction per chamber, every cell awake. Three
+    ensemble steps (graph-replayed) against the same 256 chambers stepped one
+    after another: every chamber's state hash and counters are equal."""
+    import sys
+    from pathlib import Path
+    sys.path.insert(0, str(Path(__file__).resolve().parents[1]))
+    import bench
+    ens_worlds = bench.ensemble_chambers(gpu, 256)
+    solo = bench.ensemble_chambers(gpu, 256)
+    counts = [w.counters().count for w in solo]
+    assert len(counts) == 256 and min(counts) >= 500 and max(counts) <= 50_000
+    assert 2.0e6 < sum(counts) < 3.5e6
+    with ah.Ensemble(gpu, ens_worlds) as e:
+        e.step(3)
+        for w in solo:
+            w.step(3)
+        e.sync()
+        for i, (a, b) in enumerate(zip(ens_worlds, solo)):
+            assert_same(a, b, f"chamber {i} (of {counts[i]} cells)")
+    for w in ens_worlds + solo:
+        w.close()

@@ -198,3 +198,35 @@ def test_rejects_bad_snapshots(gpu, tmp_path):
         with ah.World.from_snapshot(gpu, bytes(data), **kw) as w2:
             assert w2.state_hash() == w.state_hash()
     w.close()
+
+
+@pytest.mark.parametrize("precision", [ah.FP64_CONTRACT, ah.FP32_HOT])
+def test_flash_power_override_round_trip(gpu, oracle, precision):
+    """PETROVA_FLASH_POWER is a snapshot override like the other two
+    (src/sim/snapshot.cu:79-83): saved in the reference's key order, applied
+    on load, and the restored world's flash discharges at that power,
+    matching the oracle bit for bit."""
+    power = 7.5e5
+    kw = dict(capacity=400, precision=precision, petrova_flash_power=power, petrova_max_power=2.0e4,
+              thermal_enabled=0, grid={0: (64, 64, 1), 1: (32, 32, 1), 2: (16, 16, 1), 3: (64, 64, 1)})
+    w = ah.World(gpu, **kw)
+    w.spawn(300, 12, charge_dist=ah.DIST_UNIFORM, charge_a=0.2, charge_b=0.9, awake=True)
+    data = w.snapshot("flash")
+    snap = asph.parse(data, ah.CELL_FIELDS)
+    assert list(snap["overrides"].items()) == [("PETROVA_MAX_POWER", 2.0e4), ("PETROVA_FLASH_POWER", power)]
+    base = dict(kw)
+    base.pop("petrova_flash_power")
+    base.pop("petrova_max_power")
+    with ah.World.from_snapshot(gpu, data, **base) as r, ah.World(oracle, **kw) as o:
+        assert r.desc.petrova_flash_power == power and r.desc.petrova_max_power == 2.0e4
+        assert r.state_hash() == w.state_hash()
+        o.upload(r.download())
+        for x in (r, o):
+            x.set_flash(True)
+            x.step(2)
+        a, b = o.download(), r.download()
+        assert np.array_equal(a["energy"], b["energy"])
+        ia, ea = o.flash_audit()
+        ib, eb = r.flash_audit()
+        assert ea == eb and abs(ia - ib) <= 1e-12 * abs(ia)
+    w.close()

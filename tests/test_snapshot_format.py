@@ -81,3 +81,39 @@ def test_oracle_state_hash_is_the_body_hash(oracle, grids):
                 w2.set_field(f, w.field(f))
         assert w2.state_hash() == asph.fnv1a64(world_body(w2, {}))
         w2.close()
+
+
+def test_oracle_state_hash_carries_every_override_in_order(oracle):
+    """All three curated overrides non-canon: the body carries their records
+    in the reference's key order, max power, flash power, CO2 per division
+    (src/sim/snapshot.cu:79-83)."""
+    kw = dict(REF2D, capacity=300, petrova_max_power=2.5e4, petrova_flash_power=1.0e6,
+              co2_mass_per_division=0.75 * 2.1e-14)
+    with ah.World(oracle, **kw) as w:
+        w.spawn(200, 3, charge_dist=ah.DIST_UNIFORM, charge_b=0.5, awake=True)
+        w.step(2)
+        body = world_body(w, {"PETROVA_MAX_POWER": 2.5e4, "PETROVA_FLASH_POWER": 1.0e6,
+                              "CO2_MASS_PER_DIVISION": 0.75 * 2.1e-14})
+        assert w.state_hash() == asph.fnv1a64(body)
+        assert w.stats()["non_canon_run"] == 1
+    with ah.World(oracle, **dict(REF2D, capacity=10, petrova_flash_power=1.0e6)) as w:
+        assert w.state_hash() == asph.fnv1a64(world_body(w, {"PETROVA_FLASH_POWER": 1.0e6}))
+        assert w.stats()["non_canon_run"] == 1
+
+
+def test_oracle_flash_runs_at_the_world_flash_power(oracle):
+    """flash_step discharges at World::petrova_flash_power (src/sim/flash.cu:
+    42, :66): a tenth of canon takes a tenth of the energy per tick."""
+    canon = 3.0e6
+    taken = {}
+    for power in (canon, 0.1 * canon):
+        with ah.World(oracle, **dict(REF2D, capacity=50, petrova_flash_power=power, emission_enabled=0,
+                                          thermal_enabled=0)) as w:
+            w.spawn(50, 5, charge_dist=ah.DIST_CONSTANT, charge_a=1.0, awake=True)
+            e0 = w.download()["energy"].copy()
+            w.set_flash(True)
+            w.step(1)
+            taken[power] = e0 - w.download()["energy"]
+    dt = 1e-3
+    assert np.allclose(taken[canon], canon * dt, rtol=1e-12)
+    assert np.allclose(taken[0.1 * canon], 0.1 * canon * dt, rtol=1e-12)
