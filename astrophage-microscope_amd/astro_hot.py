@@ -31,6 +31,8 @@ OK, OUT_OF_MEMORY, DEVICE_ERROR, INVALID_ARGUMENT, NOT_FOUND, PARSE_ERROR, \
     CONTRACT_VERSION_MISMATCH, CAPACITY_EXCEEDED, FILE_IO_ERROR, UNSUPPORTED = range(10)
 
 FP64_CONTRACT, FP32_HOT = 0, 1
+RNG_PCG32, RNG_PHILOX = 0, 1
+RNG_STAGE_SPAWN, RNG_STAGE_TAXIS, RNG_STAGE_INTEGRATE = 1, 2, 3
 BOUNDARY_REFLECTING, BOUNDARY_PERIODIC, BOUNDARY_ABSORBING = 0, 1, 2
 GRAVITY_Y, GRAVITY_Z = 0, 1
 CLOCK_REALTIME, CLOCK_MOTION, CLOCK_METABOLIC, CLOCK_GENERATIONAL, CLOCK_CUSTOM = range(5)
@@ -218,6 +220,8 @@ _P = C.c_void_p
 _I = C.c_int32
 _D = C.c_double
 SIGNATURES = {
+    "philox_blocks": (_I, [C.c_void_p, _I, C.c_void_p]),
+    "philox_stream": (_I, [C.c_uint64, C.c_uint64, C.c_uint64, _I, _I, C.c_void_p]),
     "world_desc_default": (None, [C.POINTER(WorldDesc)]),
     "world_create": (_I, [C.POINTER(WorldDesc), C.POINTER(_P)]),
     "world_destroy": (None, [_P]),
@@ -329,7 +333,23 @@ class Binding:
             fn = getattr(self.lib, prefix + name)
             fn.restype = res
             fn.argtypes = args
-            setattr(self, name, fn)
+            setattr(self, "_" + name if name.startswith("philox_") else name, fn)
+
+    def philox_blocks(self, words: np.ndarray) -> np.ndarray:
+        """Philox4x32-10 of each row (c0, c1, c2, c3, k0, k1) of `words`."""
+        w = np.ascontiguousarray(words, np.uint32).reshape(-1, 6)
+        out = np.zeros((len(w), 4), np.uint32)
+        st = self._philox_blocks(_ptr(w), len(w), _ptr(out))
+        if st != OK:
+            raise AstroError(st, "philox_blocks")
+        return out
+
+    def philox_stream(self, seed: int, cid: int, tick: int, stage: int, n: int) -> np.ndarray:
+        out = np.zeros(n, np.uint32)
+        st = self._philox_stream(seed, cid, tick, stage, n, _ptr(out))
+        if st != OK:
+            raise AstroError(st, "philox_stream")
+        return out
 
     def desc(self, **kw) -> WorldDesc:
         """A WorldDesc with the reference defaults, overridden by `kw`.

@@ -82,6 +82,8 @@ struct Counters {
     uint32_t error_flags;      // sticky device-side errors (see ERR_*)
     int32_t  _pad;
     int64_t  cell_steps;       // sum of the count each tick began with
+    uint64_t tick_count;       // ticks begun (the host's World::tick between steps)
+    uint64_t rng_tick;         // the tick in progress: counter word of Philox streams
 };
 
 enum : uint32_t { ERR_SCAN = 1u, ERR_COEFF = 2u };
@@ -127,18 +129,15 @@ __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
     return x ^ (x >> 31);
 }
 
-struct Pcg {
-    uint64_t state, inc;
-    __device__ __forceinline__ uint32_t next() {
-        const uint64_t old = state;
-        state = old * 6364136223846793005ull + inc;
-        const uint32_t xs = static_cast<uint32_t>(((old >> 18) ^ old) >> 27);
-        const uint32_t rot = static_cast<uint32_t>(old >> 59);
-        return (xs >> rot) | (xs << ((0u - rot) & 31u));
-    }
+// The uniforms and Gaussians every generator hands out, built on its 32-bit
+// draws (src/core/rng.cuh:76-101): 53-bit doubles from two draws, Box-Muller
+// pairs from two doubles.
+template <class G>
+struct Draws {
     __device__ __forceinline__ double uniform01d() {
-        const uint64_t hi = static_cast<uint64_t>(next()) << 21;
-        const uint64_t lo = static_cast<uint64_t>(next()) >> 11;
+        G& g = static_cast<G&>(*this);
+        const uint64_t hi = static_cast<uint64_t>(g.next()) << 21;
+        const uint64_t lo = static_cast<uint64_t>(g.next()) >> 11;
         return static_cast<double>((hi ^ lo) & ((1ull << 53) - 1)) * (1.0 / 9007199254740992.0);
     }
     __device__ __forceinline__ double uniform(double lo, double hi) { return lo + (hi - lo) * uniform01d(); }
@@ -155,9 +154,81 @@ struct Pcg {
     }
 };
 
+struct Pcg : Draws<Pcg> {
+    uint64_t state, inc;
+    __device__ __forceinline__ Pcg(uint64_t s, uint64_t i) : state(s), inc(i) {}
+    __device__ __forceinline__ uint32_t next() {
+        const uint64_t old = state;
+        state = old * 6364136223846793005ull + inc;
+        const uint32_t xs = static_cast<uint32_t>(((old >> 18) ^ old) >> 27);
+        const uint32_t rot = static_cast<uint32_t>(old >> 59);
+        return (xs >> rot) | (xs << ((0u - rot) & 31u));
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror and Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC'11): ten rounds over a 128-bit counter, each two
+// 32 x 32 -> 64-bit products whose halves are crossed with the other two
+// words and the key; the 64-bit key is advanced by the Weyl constants between
+// rounds. The counter-based alternative to the per-cell PCG32 state.
+// ---------------------------------------------------------------------------
+constexpr uint32_t PHILOX_M0 = 0xD2511F53u, PHILOX_M1 = 0xCD9E8D57u;
+constexpr uint32_t PHILOX_W0 = 0x9E3779B9u, PHILOX_W1 = 0xBB67AE85u;
+
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) {
+            k0 += PHILOX_W0;
+            k1 += PHILOX_W1;
+        }
+        const uint32_t lo0 = PHILOX_M0 * c.x, hi0 = __umulhi(PHILOX_M0, c.x);
+        const uint32_t lo1 = PHILOX_M1 * c.z, hi1 = __umulhi(PHILOX_M1, c.z);
+        c = uint4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+    }
+    return c;
+}
+
+// Stages that draw from a cell's counter stream; each starts its own run of
+// blocks, so no two stages of a tick share a counter.
+enum : uint32_t { RNG_STAGE_SPAWN = ASTRO_RNG_STAGE_SPAWN, RNG_STAGE_TAXIS = ASTRO_RNG_STAGE_TAXIS,
+                  RNG_STAGE_INTEGRATE = ASTRO_RNG_STAGE_INTEGRATE };
+
+// A cell's draws within one stage of one tick: key = the seed, counter =
+// (stage << 16 | block, tick, id low, id high); each block yields four draws,
+// handed out in order.
+struct Philox : Draws<Philox> {
+    uint32_t k0, k1, tick, id_lo, id_hi, word;
+    uint4 buf;
+    int left;
+    __device__ __forceinline__ Philox(uint64_t seed, uint32_t stage, uint64_t tick_, uint64_t id)
+        : k0(static_cast<uint32_t>(seed)), k1(static_cast<uint32_t>(seed >> 32)),
+          tick(static_cast<uint32_t>(tick_)), id_lo(static_cast<uint32_t>(id)),
+          id_hi(static_cast<uint32_t>(id >> 32)), word(stage << 16), buf{0u, 0u, 0u, 0u}, left(0) {}
+    __device__ __forceinline__ uint32_t next() {
+        if (left == 0) {
+            buf = philox4x32_10(uint4{word, tick, id_lo, id_hi}, k0, k1);
+            ++word;
+            left = 4;
+        }
+        const uint32_t v = left == 4 ? buf.x : (left == 3 ? buf.y : (left == 2 ? buf.z : buf.w));
+        --left;
+        return v;
+    }
+};
+
+// Which per-cell generator a world draws from, and what a Philox stream needs.
+struct RngCtx {
+    uint64_t seed;          // Philox key
+    const uint64_t* tick;   // device word holding the tick in progress (Counters::rng_tick)
+    int philox;             // 0: the cell's PCG32 state in rng_state
+};
+
 // The Gaussian pair in float for the FP32 storage layout: the same two
 // uniform01d draws (the stream stays bit-exact), the transform in float.
-__device__ __forceinline__ void gaussian_pair_f(Pcg& r, float& z0, float& z1) {
+template <class G>
+__device__ __forceinline__ void gaussian_pair_f(G& r, float& z0, float& z1) {
     float u1 = static_cast<float>(r.uniform01d());
     if (!(u1 >= 1.17549435e-38f)) u1 = 1.17549435e-38f;
     const float u2 = static_cast<float>(r.uniform01d());

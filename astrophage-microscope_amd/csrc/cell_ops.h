@@ -28,7 +28,7 @@ __device__ __forceinline__ void any_perpendicular(double hx, double hy, double h
 // `directional` is the cell's shadowed far-field irradiance (FEED only).
 template <class P, bool FEED, bool TAXIS>
 __device__ __forceinline__ void feed_taxis_cell(const Cells<P>& c, int i, uint32_t f, double directional,
-                                                float ambient, double dt, double max_power) {
+                                                float ambient, double dt, double max_power, const RngCtx& rng) {
     if (!(f & ASTRO_FLAG_OCCUPIED) || !(f & ASTRO_FLAG_ALIVE)) return;
     double energy = c.energy[i];
     float irr_f = 0.0f;
@@ -75,11 +75,20 @@ __device__ __forceinline__ void feed_taxis_cell(const Cells<P>& c, int i, uint32
         double run = static_cast<double>(c.run_timer[i]) + dt;
         if (delta <= 0.0 || run >= ASTRO_TAXIS_RUN_MAX) {
             // taxis_tumble (src/sim/taxis.cuh:108-118)
-            Pcg r = cell_rng(c.rng_state[i], c.id[i]);
-            double theta = -ASTRO_TAXIS_TUMBLE_ANGLE_MEAN * log(1.0 - r.uniform01d());
+            double u_theta, u_azimuth;
+            if (rng.philox) {
+                Philox r(rng.seed, RNG_STAGE_TAXIS, *rng.tick, c.id[i]);
+                u_theta = r.uniform01d();
+                u_azimuth = r.uniform01d();
+            } else {
+                Pcg r = cell_rng(c.rng_state[i], c.id[i]);
+                u_theta = r.uniform01d();
+                u_azimuth = r.uniform01d();
+                c.rng_state[i] = r.state;
+            }
+            double theta = -ASTRO_TAXIS_TUMBLE_ANGLE_MEAN * log(1.0 - u_theta);
             if (theta > ASTRO_PI) theta = ASTRO_PI;
-            const double azimuth = ASTRO_TWO_PI * r.uniform01d();
-            c.rng_state[i] = r.state;
+            const double azimuth = ASTRO_TWO_PI * u_azimuth;
             double l = sqrt(hx * hx + hy * hy + hz * hz);
             if (l > 0.0) {
                 const double inv = 1.0 / l;

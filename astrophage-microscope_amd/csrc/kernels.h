@@ -135,6 +135,9 @@ struct TickParams {
     double T_cap;
     // lifecycle
     double co2_gv;
+    // per-cell streams: Philox key (the world seed) when philox != 0
+    uint64_t rng_seed;
+    int    philox;
 };
 
 // The device views and tick scalars of the chambers one launch covers.
@@ -169,6 +172,7 @@ template <class P> void launch_resort(const Launch& L, Views<P> ws);
 // ---- cells (kernels_cells.hip) ----
 struct SpawnArgs {
     int32_t count, placement, charge_dist, awake;
+    int32_t philox;              // draw from the Philox stream keyed by the spawn seed
     double place_x, place_y, place_radius, charge_a, charge_b;
     double chamber_w, chamber_h, chamber_d;
 };

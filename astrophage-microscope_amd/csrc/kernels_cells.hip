@@ -26,15 +26,10 @@ __device__ __forceinline__ void wave_count_add(int32_t* dst, bool pred) {
 // ---------------------------------------------------------------------------
 // spawn_kernel (src/sim/cell_store.cu:42-130); LOG_UNIFORM charge added.
 // ---------------------------------------------------------------------------
-template <class P>
-__global__ __launch_bounds__(BLOCK) void k_spawn(Cells<P> c, SpawnArgs p, int32_t base, int32_t lbase,
-                                                 uint64_t first_id, uint64_t seed) {
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= p.count) return;
-    const int32_t slot = base + i;
-    c.lslot[slot] = static_cast<uint32_t>(lbase + i);
-    const uint64_t id = first_id + static_cast<uint64_t>(i);
-    Pcg r = cell_rng(cell_rng_init(seed, id), id);
+// One new cell's draws and state, in the reference's draw order.
+template <class P, class G>
+__device__ __forceinline__ void spawn_cell(const Cells<P>& c, const SpawnArgs& p, int i, int32_t slot, uint64_t id,
+                                           G& r) {
     double x = 0.0, y = 0.0;
     if (p.placement == ASTRO_PLACE_GAUSSIAN) {
         double gx, gy;
@@ -103,13 +98,31 @@ __global__ __launch_bounds__(BLOCK) void k_spawn(Cells<P> c, SpawnArgs p, int32_
     c.biomass[slot] = ASTRO_CELL_MASS_DRY;
     c.co2_held[slot] = 0.0;
     c.age_s[slot] = 0.0f;
-    c.rng_state[slot] = r.state;
     c.taxis_memory[slot] = 0.0f;
     c.run_timer[slot] = 0.0f;
     c.t_local[slot] = tc;
     c.irradiance[slot] = 0.0f;
     c.co2_local[slot] = 0.0f;
     c.n2_local[slot] = 0.0f;
+}
+
+template <class P>
+__global__ __launch_bounds__(BLOCK) void k_spawn(Cells<P> c, SpawnArgs p, int32_t base, int32_t lbase,
+                                                 uint64_t first_id, uint64_t seed) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= p.count) return;
+    const int32_t slot = base + i;
+    c.lslot[slot] = static_cast<uint32_t>(lbase + i);
+    const uint64_t id = first_id + static_cast<uint64_t>(i);
+    if (p.philox) {
+        Philox r(seed, RNG_STAGE_SPAWN, 0u, id);
+        spawn_cell(c, p, i, slot, id, r);
+        c.rng_state[slot] = 0ull;
+    } else {
+        Pcg r = cell_rng(cell_rng_init(seed, id), id);
+        spawn_cell(c, p, i, slot, id, r);
+        c.rng_state[slot] = r.state;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -228,17 +241,20 @@ struct KFeedTaxis {
         float ambient;
         double dt;
         double max_power;
+        uint64_t rng_seed;
+        int philox;
     };
     static __device__ void run(const Args& a, VGrid vg) {
-        const auto& [c, ctr, spos, dirs, ambient, dt, max_power] = a;
+        const auto& [c, ctr, spos, dirs, ambient, dt, max_power, rng_seed, philox] = a;
 
     const int n = ctr->nphys;
+    const RngCtx rng{rng_seed, &ctr->rng_tick, philox};
     xcd_tiles(vg, n, [&](const int i, const bool valid) {
         if (!valid) return;
         const uint32_t f = c.flags[i];
         if (!(f & ASTRO_FLAG_OCCUPIED) || !(f & ASTRO_FLAG_ALIVE)) return;
         const double directional = FEED ? dirs[spos[i]] : 0.0;
-        feed_taxis_cell<P, FEED, TAXIS>(c, i, f, directional, ambient, dt, max_power);
+        feed_taxis_cell<P, FEED, TAXIS>(c, i, f, directional, ambient, dt, max_power, rng);
     });
     }
 };
@@ -302,11 +318,15 @@ struct KSampleBase {
         float ambient;
         double dt;
         double max_power;
+        uint64_t rng_seed;
+        int philox;
     };
     static __device__ void run(const Args& a, VGrid vg) {
-        const auto& [c, ctr, T, C, N, c_on, n_on, gaxis, spos, fs, dirs, ambient, dt, max_power] = a;
+        const auto& [c, ctr, T, C, N, c_on, n_on, gaxis, spos, fs, dirs, ambient, dt, max_power, rng_seed,
+                     philox] = a;
 
     const int n = ctr->nphys;
+    const RngCtx rng{rng_seed, &ctr->rng_tick, philox};
     xcd_tiles(vg, n, [&](const int i, const bool valid) {
         if (!valid) return;
         // Slot, flags and position are loaded together (in bounds for i
@@ -321,7 +341,7 @@ struct KSampleBase {
             // is read), then the force from the updated state
             if ((f & ASTRO_FLAG_OCCUPIED) && (f & ASTRO_FLAG_ALIVE)) {
                 const double directional = FEED ? dirs[spos[i]] : 0.0;
-                feed_taxis_cell<P, FEED, TAXIS>(c, i, f, directional, ambient, dt, max_power);
+                feed_taxis_cell<P, FEED, TAXIS>(c, i, f, directional, ambient, dt, max_power, rng);
                 f = c.flags[i];
             }
         }
@@ -370,7 +390,7 @@ __device__ __forceinline__ void book_demand(const DemandArgs& a, int i, bool liv
 // integrate_kernel (src/sim/integrator.cu:79-150). With DEMAND the same pass
 // then books the moved cell's CO2 demand (lifecycle's first kernel, which
 // reads nothing the steps in between change once the CO2 field has diffused).
-template <class P, bool DEMAND>
+template <class P, bool DEMAND, bool PHILOX>
 struct KIntegrate {
     static constexpr bool stride = true;   // any block count covers the work
     struct Args {
@@ -396,7 +416,8 @@ struct KIntegrate {
     double px = static_cast<double>(c.x[i]), py = static_cast<double>(c.y[i]), pz = static_cast<double>(c.z[i]);
     double vx = static_cast<double>(c.vx[i]), vy = static_cast<double>(c.vy[i]), vz = static_cast<double>(c.vz[i]);
     const uint64_t id = c.id[i];
-    const uint64_t rng_state = c.rng_state[i];
+    // A counter stream needs no per-cell state (Philox: seed, tick, id).
+    const uint64_t rng_state = PHILOX ? 0ull : c.rng_state[i];
     if (!(fl & ASTRO_FLAG_OCCUPIED)) {
         if constexpr (DEMAND) dm.vox[i] = NO_VOXEL;
         return;
@@ -407,7 +428,10 @@ struct KIntegrate {
     const bool awake = (fl & ASTRO_FLAG_AWAKE) != 0u;
     const double t_drag = awake ? ASTRO_CELL_TEMP_SETPOINT : t_local;
     const Force F = fs[sp];
-    Pcg r = cell_rng(rng_state, id);
+    auto r = [&] {
+        if constexpr (PHILOX) return Philox(t.rng_seed, RNG_STAGE_INTEGRATE, ctr->rng_tick, id);
+        else return cell_rng(rng_state, id);
+    }();
     // The three pairs are drawn in the reference's order (x, y, z); each is
     // used as soon as it is drawn, which keeps fewer values live.
     const bool noise = t.thermal_noise != 0;
@@ -446,7 +470,8 @@ struct KIntegrate {
     if (t.adhesion) {
         if (touched && !(fl & ASTRO_FLAG_STUCK)) {
             // The reference re-derives the stream from the state it just
-            // stored (integrator.cu:124-127); r.state is that state.
+            // stored (integrator.cu:124-127); r.state is that state. A
+            // counter stream takes its next draws.
             const bool sticks = r.uniform01d() < ASTRO_WALL_STICKINESS;
             if (sticks) nf |= ASTRO_FLAG_STUCK;
         } else if (fl & ASTRO_FLAG_STUCK) {
@@ -454,7 +479,7 @@ struct KIntegrate {
             if (away > ASTRO_WALL_RELEASE_FORCE) nf &= ~ASTRO_FLAG_STUCK;
         }
     }
-    c.rng_state[i] = r.state;
+    if constexpr (!PHILOX) c.rng_state[i] = r.state;
     if (!kept) {
         nf &= ~ASTRO_FLAG_ALIVE;
         c.death_cause[i] = ASTRO_DEATH_CULLED;
@@ -793,7 +818,8 @@ struct KLifeApply {
             c.emit_power[slot] = 0.0f;
             c.dir_x[slot] = c.dir_x[i]; c.dir_y[slot] = c.dir_y[i]; c.dir_z[slot] = c.dir_z[i];
             c.age_s[slot] = 0.0f;
-            c.rng_state[slot] = pcg_split(c.rng_state[i], did);
+            // a counter stream is keyed by the daughter's own id: no state to split
+            c.rng_state[slot] = t.philox ? 0ull : pcg_split(c.rng_state[i], did);
             c.taxis_memory[slot] = 0.0f;
             c.run_timer[slot] = 0.0f;
             c.t_local[slot] = c.t_local[i];
@@ -975,7 +1001,8 @@ void launch_feed_taxis_t(const Launch& L, Views<P> ws, Ticks ts, const char* nam
     Batch<K> b;
     for (size_t i = 0; i < ws.size(); ++i) {
         const WorldDev<P>& w = ws[i];
-        b.add({w.c, w.ctr, w.spos, w.dirs, ts[i].ambient_irr, ts[i].dt, ts[i].petrova_max_power},
+        b.add({w.c, w.ctr, w.spos, w.dirs, ts[i].ambient_irr, ts[i].dt, ts[i].petrova_max_power, ts[i].rng_seed,
+               ts[i].philox},
               cell_grid(w.pcap));
     }
     launch_k<K>(L, name, b.args, b.blocks);
@@ -1005,7 +1032,7 @@ void launch_sample_t(const Launch& L, Views<P> ws, Ticks ts, const char* name) {
         const TickParams& t = ts[i];
         b.add({w.c, w.ctr, w.g[ASTRO_FIELD_TEMPERATURE], w.g[ASTRO_FIELD_CO2], w.g[ASTRO_FIELD_N2],
                w.on[ASTRO_FIELD_CO2] ? 1 : 0, w.on[ASTRO_FIELD_N2] ? 1 : 0, t.gravity_axis, w.spos, w.fs, w.dirs,
-               t.ambient_irr, t.dt, t.petrova_max_power},
+               t.ambient_irr, t.dt, t.petrova_max_power, t.rng_seed, t.philox},
               cell_grid(w.pcap));
     }
     launch_k<K>(L, name, b.args, b.blocks);
@@ -1025,9 +1052,9 @@ DemandArgs demand_args(const WorldDev<P>& w, const TickParams& t) {
     return DemandArgs{w.g[ASTRO_FIELD_CO2], t.dt_bio, t.co2_gv, w.co2_demand, w.rin, w.co2_avail};
 }
 
-template <class P, bool DEMAND>
+template <class P, bool DEMAND, bool PHILOX>
 void launch_integrate_t(const Launch& L, Views<P> ws, Ticks ts, const char* name) {
-    using K = KIntegrate<P, DEMAND>;
+    using K = KIntegrate<P, DEMAND, PHILOX>;
     Batch<K> b;
     for (size_t i = 0; i < ws.size(); ++i)
         b.add({ws[i].c, ws[i].ctr, ws[i].spos, ws[i].fs, ts[i], demand_args(ws[i], ts[i])}, cell_grid(ws[i].pcap));
@@ -1038,8 +1065,14 @@ void launch_integrate_t(const Launch& L, Views<P> ws, Ticks ts, const char* name
 
 template <class P>
 void launch_integrate(const Launch& L, Views<P> ws, Ticks ts, bool demand) {
-    if (demand) launch_integrate_t<P, true>(L, ws, ts, "integrate_demand");
-    else launch_integrate_t<P, false>(L, ws, ts, "integrate");
+    // chambers of one launch agree on the generator (the ensemble checks it)
+    if (ts[0].philox) {
+        if (demand) launch_integrate_t<P, true, true>(L, ws, ts, "integrate_demand");
+        else launch_integrate_t<P, false, true>(L, ws, ts, "integrate");
+    } else {
+        if (demand) launch_integrate_t<P, true, false>(L, ws, ts, "integrate_demand");
+        else launch_integrate_t<P, false, false>(L, ws, ts, "integrate");
+    }
 }
 
 // The division / compaction tail shared by the lifecycle and a direct

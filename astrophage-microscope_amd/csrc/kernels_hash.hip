@@ -58,6 +58,10 @@ struct KHashKey {
             const uint32_t t = ctr->tick_ctr;
             ctr->resort = (t % static_cast<uint32_t>(resort_period) == 0u || np > cap) ? 1 : 0;
             ctr->tick_ctr = t + 1u;
+            // the tick in progress, for counter-based streams
+            const uint64_t k = ctr->tick_count;
+            ctr->rng_tick = k;
+            ctr->tick_count = k + 1u;
         }
         // The lifecycle's per-slot flags and their scan-tile totals start
         // each tick cleared.

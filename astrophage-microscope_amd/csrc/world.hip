@@ -336,6 +336,8 @@ TickParams tick_params(const astro_world* w, double dt) {
     t.T_cap = ASTRO_WATER_DENSITY * ASTRO_WATER_SPECIFIC_HEAT * T.dx * T.dy * T.dz;
     const FieldHost& C = w->f[ASTRO_FIELD_CO2];
     t.co2_gv = C.dx * C.dy * C.dz;
+    t.rng_seed = d.seed;
+    t.philox = d.rng_kind == ASTRO_RNG_PHILOX ? 1 : 0;
     return t;
 }
 
@@ -859,6 +861,7 @@ int32_t serialize_body(astro_world* w, std::vector<char>& body, int32_t& overrid
 // What the host decides for a tick; every chamber of an ensemble must agree.
 struct TickShape {
     bool fp32;
+    bool philox;
     double dt;
     bool thermal, light, taxis, lifecycle, contact, predators, flash;
     bool on[ASTRO_FIELD_COUNT];
@@ -870,6 +873,7 @@ TickShape tick_shape(const astro_world* w) {
     TickShape s{};
     const astro_world_desc& d = w->desc;
     s.fp32 = w->fp32;
+    s.philox = d.rng_kind == ASTRO_RNG_PHILOX;
     s.dt = ASTRO_DT_PHYSICS * w->physics_rate;
     s.thermal = d.thermal_enabled != 0;
     s.light = d.emission_enabled && lit(w);
@@ -1309,6 +1313,7 @@ int32_t astro_cells_spawn(astro_world* w, const astro_spawn_params* p, uint64_t 
     a.place_x = p->place_x; a.place_y = p->place_y; a.place_radius = p->place_radius;
     a.charge_a = p->charge_a; a.charge_b = p->charge_b;
     a.chamber_w = w->desc.chamber_w; a.chamber_h = w->desc.chamber_h; a.chamber_d = w->desc.chamber_d;
+    a.philox = w->desc.rng_kind == ASTRO_RNG_PHILOX ? 1 : 0;
     const Launch L = launch_of(w);
     if (w->fp32) launch_spawn<float>(L, make_view<float>(w), a, c.nphys, c.count, c.next_id, seed);
     else launch_spawn<double>(L, make_view<double>(w), a, c.nphys, c.count, c.next_id, seed);
@@ -1378,6 +1383,7 @@ int32_t astro_world_set_counters(astro_world* w, const astro_counters* in) {
     c.deaths_total = in->deaths_total;
     c.dead_slots_prev = in->dead_slots_prev;
     c.cell_steps = in->cell_steps;
+    c.tick_count = in->tick;
     w->tick = in->tick;
     w->sim_time = in->sim_time_s;
     w->hash_valid = false;
@@ -1742,10 +1748,17 @@ int32_t astro_stage_emission(astro_world* w, double dt) {
     return hip_check(w, hipGetLastError(), "emission");
 }
 
+// A stage called on its own draws from the current tick's counter streams.
+static void set_rng_tick(astro_world* w, const Launch& L) {
+    launch_fill_u64(L, reinterpret_cast<unsigned long long*>(&w->ctr->rng_tick), 1,
+                    static_cast<unsigned long long>(w->tick));
+}
+
 int32_t astro_stage_taxis(astro_world* w, double dt) {
     if (!w) return ASTRO_INVALID_ARGUMENT;
     const Launch L = launch_of(w);
     const TickParams t = tick_params(w, dt);
+    set_rng_tick(w, L);
     if (w->fp32) launch_feed_taxis<float>(L, one(make_view<float>(w)), one(t), false, true);
     else launch_feed_taxis<double>(L, one(make_view<double>(w)), one(t), false, true);
     return hip_check(w, hipGetLastError(), "taxis");
@@ -1792,6 +1805,7 @@ int32_t astro_stage_integrate(astro_world* w, double dt) {
     if (!w) return ASTRO_INVALID_ARGUMENT;
     const Launch L = launch_of(w);
     const TickParams t = tick_params(w, dt);
+    set_rng_tick(w, L);
     if (w->fp32) launch_integrate<float>(L, one(make_view<float>(w)), one(t), false);
     else launch_integrate<double>(L, one(make_view<double>(w)), one(t), false);
     w->hash_valid = false;
@@ -1804,6 +1818,7 @@ int32_t astro_stage_motion(astro_world* w, double dt) {
     if (st != ASTRO_OK) return st;
     const Launch L = launch_of(w);
     const TickParams t = tick_params(w, dt);
+    set_rng_tick(w, L);
     if (w->fp32) {
         const WorldDev<float> d = make_view<float>(w);
         launch_cell_forces<float>(L, one(d), one(t), false, false);
@@ -2368,6 +2383,71 @@ int32_t astro_ensemble_profile_reset(astro_ensemble* e) {
 }
 
 void* astro_ensemble_stream(astro_ensemble* e) { return e ? static_cast<void*>(e->stream) : nullptr; }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Counter-based streams, evaluated on the device for known-answer checks.
+// ---------------------------------------------------------------------------
+namespace {
+
+__global__ void k_philox_blocks(const uint32_t* words, int32_t n, uint32_t* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t* w = words + 6 * static_cast<size_t>(i);
+    const uint4 r = philox4x32_10(uint4{w[0], w[1], w[2], w[3]}, w[4], w[5]);
+    uint32_t* o = out + 4 * static_cast<size_t>(i);
+    o[0] = r.x; o[1] = r.y; o[2] = r.z; o[3] = r.w;
+}
+
+__global__ void k_philox_stream(uint64_t seed, uint64_t id, uint64_t tick, uint32_t stage, int32_t n,
+                                uint32_t* out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    Philox r(seed, stage, tick, id);
+    for (int32_t k = 0; k < n; ++k) out[k] = r.next();
+}
+
+// Runs `launch` on a device copy of `in` (in_words uint32) writing out_words
+// uint32 back to `out`, on the current device's null stream.
+template <class F>
+int32_t run_words(const uint32_t* in, size_t in_words, uint32_t* out, size_t out_words, F launch) {
+    uint32_t *din = nullptr, *dout = nullptr;
+    int32_t st = ASTRO_OK;
+    if (in_words && hipMalloc(&din, in_words * 4) != hipSuccess) return ASTRO_OUT_OF_MEMORY;
+    if (hipMalloc(&dout, std::max<size_t>(out_words, 1) * 4) != hipSuccess) st = ASTRO_OUT_OF_MEMORY;
+    if (st == ASTRO_OK && in_words && hipMemcpy(din, in, in_words * 4, hipMemcpyHostToDevice) != hipSuccess)
+        st = ASTRO_DEVICE_ERROR;
+    if (st == ASTRO_OK) {
+        launch(din, dout);
+        if (hipGetLastError() != hipSuccess) st = ASTRO_DEVICE_ERROR;
+    }
+    if (st == ASTRO_OK && hipMemcpy(out, dout, out_words * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        st = ASTRO_DEVICE_ERROR;
+    if (din) hipFree(din);
+    if (dout) hipFree(dout);
+    return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t astro_philox_blocks(const uint32_t* words, int32_t n, uint32_t* out) {
+    if (n < 0 || (n > 0 && (!words || !out))) return ASTRO_INVALID_ARGUMENT;
+    if (n == 0) return ASTRO_OK;
+    const size_t m = static_cast<size_t>(n);
+    return run_words(words, 6 * m, out, 4 * m, [&](const uint32_t* din, uint32_t* dout) {
+        k_philox_blocks<<<(n + BLOCK - 1) / BLOCK, BLOCK>>>(din, n, dout);
+    });
+}
+
+int32_t astro_philox_stream(uint64_t seed, uint64_t id, uint64_t tick, int32_t stage, int32_t n, uint32_t* out) {
+    if (n < 0 || stage < 0 || stage > 0xFFFF || (n > 0 && !out)) return ASTRO_INVALID_ARGUMENT;
+    if (n == 0) return ASTRO_OK;
+    return run_words(nullptr, 0, out, static_cast<size_t>(n), [&](const uint32_t*, uint32_t* dout) {
+        k_philox_stream<<<1, 1>>>(seed, id, tick, static_cast<uint32_t>(stage), n, dout);
+    });
+}
 
 }  // extern "C"
 

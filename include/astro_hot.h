@@ -419,6 +419,23 @@ int32_t       astro_ensemble_profile_read(astro_ensemble* e, char* names, int32_
 int32_t       astro_ensemble_profile_reset(astro_ensemble* e);
 void*         astro_ensemble_stream(astro_ensemble* e);
 
+/* ---- counter-based streams (ASTRO_RNG_PHILOX) --------------------------------
+ * Replace the per-cell PCG32 state (src/core/rng.cuh:23-117) in worlds
+ * created with rng_kind = ASTRO_RNG_PHILOX. A cell's stream in one stage of
+ * one tick is Philox4x32-10 keyed by the 64-bit seed (low word first) over the
+ * counter (stage << 16 | block, tick, id low, id high), block = 0, 1, ...,
+ * each block's four words drawn in order. Stages: 1 spawn (keyed by the
+ * spawn seed, tick 0), 2 taxis tumble, 3 integrator (keyed by the world seed,
+ * the tick in progress = ticks completed before it). The two calls below
+ * evaluate the generator on the device, for known-answer checks. */
+enum { ASTRO_RNG_STAGE_SPAWN = 1, ASTRO_RNG_STAGE_TAXIS = 2, ASTRO_RNG_STAGE_INTEGRATE = 3 };
+/* out[4i .. 4i+3] = Philox4x32-10 of counter words[6i .. 6i+3] under key
+ * words[6i+4], words[6i+5], for i < n. */
+int32_t astro_philox_blocks(const uint32_t* words, int32_t n, uint32_t* out);
+/* The first n 32-bit draws of the stream of cell `id` in `stage` of `tick`. */
+int32_t astro_philox_stream(uint64_t seed, uint64_t id, uint64_t tick, int32_t stage, int32_t n,
+                            uint32_t* out);
+
 /* ---- learned field surrogate -------------------------------------------------
  * No counterpart in the reference. A 4-layer 3D convolutional operator (3x3x3
  * kernels, 32 channels, BF16 weights and activations, FP32 accumulation on

@@ -35,9 +35,19 @@ inline uint64_t splitmix64(uint64_t x) {                     // rng.cuh:23-28
     return x ^ (x >> 31);
 }
 
-struct Pcg { uint64_t state, inc; };
+// A cell's 32-bit draws: the reference's PCG32 state (state, inc), or, in a
+// world created with rng_kind = ASTRO_RNG_PHILOX, a Philox4x32-10 counter
+// stream (key, counter, the unread words of the last block).
+struct Rng {
+    uint64_t state = 0, inc = 0;
+    bool philox = false;
+    uint32_t key[2] = {0u, 0u};
+    uint32_t ctr[4] = {0u, 0u, 0u, 0u};
+    uint32_t buf[4] = {0u, 0u, 0u, 0u};
+    int left = 0;
+};
 
-inline uint32_t pcg_next(Pcg& r) {                           // rng.cuh:35-41
+inline uint32_t pcg_step(Rng& r) {                           // rng.cuh:35-41
     const uint64_t old = r.state;
     r.state = old * 6364136223846793005ull + r.inc;
     const uint32_t xs = static_cast<uint32_t>(((old >> 18) ^ old) >> 27);
@@ -45,18 +55,64 @@ inline uint32_t pcg_next(Pcg& r) {                           // rng.cuh:35-41
     return (xs >> rot) | (xs << ((0u - rot) & 31u));
 }
 
-inline Pcg pcg_seed(uint64_t initstate, uint64_t initseq) {  // rng.cuh:44-50
-    Pcg r{0ull, (initseq << 1u) | 1u};
-    pcg_next(r);
+// Philox4x32-10 (Salmon, Moraes, Dror and Shaw, SC'11): ten rounds; each
+// multiplies counter words 0 and 2 by M0 and M1 (32 x 32 -> 64 bits), and the
+// new counter is (hi1 ^ c1 ^ k0, lo1, hi0 ^ c3 ^ k1, lo0); the key advances by
+// the Weyl constants W0, W1 before every round after the first.
+inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    const uint64_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+    for (int round = 0; round < 10; ++round) {
+        if (round > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint64_t p0 = M0 * c[0], p1 = M1 * c[2];
+        const uint32_t hi0 = static_cast<uint32_t>(p0 >> 32), lo0 = static_cast<uint32_t>(p0);
+        const uint32_t hi1 = static_cast<uint32_t>(p1 >> 32), lo1 = static_cast<uint32_t>(p1);
+        const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+    }
+}
+
+// The stream of cell `id` in `stage` of `tick`: key = seed (low word first),
+// counter = (stage << 16 | block, tick, id low, id high), blocks 0, 1, ...,
+// each block's four words in order (include/astro_hot.h, ASTRO_RNG_PHILOX).
+inline Rng philox_stream(uint64_t seed, uint32_t stage, uint64_t tick, uint64_t id) {
+    Rng r;
+    r.philox = true;
+    r.key[0] = static_cast<uint32_t>(seed);
+    r.key[1] = static_cast<uint32_t>(seed >> 32);
+    r.ctr[0] = stage << 16;
+    r.ctr[1] = static_cast<uint32_t>(tick);
+    r.ctr[2] = static_cast<uint32_t>(id);
+    r.ctr[3] = static_cast<uint32_t>(id >> 32);
+    return r;
+}
+
+inline uint32_t philox_next(Rng& r) {
+    if (r.left == 0) {
+        for (int k = 0; k < 4; ++k) r.buf[k] = r.ctr[k];
+        philox4x32_10(r.buf, r.key[0], r.key[1]);
+        ++r.ctr[0];
+        r.left = 4;
+    }
+    return r.buf[4 - r.left--];
+}
+
+inline uint32_t pcg_next(Rng& r) { return r.philox ? philox_next(r) : pcg_step(r); }
+
+inline Rng pcg_seed(uint64_t initstate, uint64_t initseq) {  // rng.cuh:44-50
+    Rng r{0ull, (initseq << 1u) | 1u};
+    pcg_step(r);
     r.state += initstate;
-    pcg_next(r);
+    pcg_step(r);
     return r;
 }
 
 inline uint64_t stream_inc(uint64_t id) {                    // rng.cuh:54-56
     return (splitmix64(id ^ 0xA5A5A5A5DEADBEEFull) << 1u) | 1u;
 }
-inline Pcg cell_rng(uint64_t st, uint64_t id) { return Pcg{st, stream_inc(id)}; }
+inline Rng cell_rng(uint64_t st, uint64_t id) { return Rng{st, stream_inc(id)}; }
 inline uint64_t cell_rng_init(uint64_t seed, uint64_t id) {  // rng.cuh:62-65
     return pcg_seed(seed ^ splitmix64(id), splitmix64(id + 1)).state;
 }
@@ -64,14 +120,14 @@ inline uint64_t pcg_split(uint64_t parent, uint64_t did) {   // rng.cuh:69-71
     return splitmix64(parent ^ splitmix64(did * 0x2545F4914F6CDD1Dull));
 }
 
-inline double uniform01d(Pcg& r) {                           // rng.cuh:80-84
+inline double uniform01d(Rng& r) {                           // rng.cuh:80-84
     const uint64_t hi = static_cast<uint64_t>(pcg_next(r)) << 21;
     const uint64_t lo = static_cast<uint64_t>(pcg_next(r)) >> 11;
     return static_cast<double>((hi ^ lo) & ((1ull << 53) - 1)) * (1.0 / 9007199254740992.0);
 }
-inline double uniform_range(Pcg& r, double lo, double hi) { return lo + (hi - lo) * uniform01d(r); }
+inline double uniform_range(Rng& r, double lo, double hi) { return lo + (hi - lo) * uniform01d(r); }
 
-inline void gaussian_pair(Pcg& r, double& z0, double& z1) {  // rng.cuh:92-101
+inline void gaussian_pair(Rng& r, double& z0, double& z1) {  // rng.cuh:92-101
     double u1 = uniform01d(r);
     if (u1 < 1.0e-300) u1 = 1.0e-300;
     const double u2 = uniform01d(r);
@@ -260,11 +316,11 @@ inline V3 rotate_by(V3 h, double theta, double az) {         // taxis.cuh:94-99
     const V3 perp = u * std::cos(az) + w * std::sin(az);
     return normalize(h * std::cos(theta) + perp * std::sin(theta));
 }
-inline double taxis_tumble_angle(Pcg& r) {                    // taxis.cuh:108-111
+inline double taxis_tumble_angle(Rng& r) {                    // taxis.cuh:108-111
     const double theta = -ASTRO_TAXIS_TUMBLE_ANGLE_MEAN * std::log(1.0 - uniform01d(r));
     return theta > ASTRO_PI ? ASTRO_PI : theta;
 }
-inline V3 taxis_tumble(V3 heading, Pcg& r) {                 // taxis.cuh:114-118
+inline V3 taxis_tumble(V3 heading, Rng& r) {                 // taxis.cuh:114-118
     const double theta = taxis_tumble_angle(r);
     const double azimuth = ASTRO_TWO_PI * uniform01d(r);
     return rotate_by(normalize(heading), theta, azimuth);
@@ -753,6 +809,7 @@ int emission_step(oracle_world* w, double dt) {
 
 // taxis_kernel (src/sim/taxis.cu:13-87).
 void taxis_step(oracle_world* w, double dt) {
+    const astro_world_desc& d = w->desc;
     for (int32_t i = 0; i < w->count; ++i) {
         const uint32_t fl = w->flags[i];
         if (!(fl & ASTRO_FLAG_OCCUPIED) || !(fl & ASTRO_FLAG_ALIVE)) continue;
@@ -774,9 +831,14 @@ void taxis_step(oracle_world* w, double dt) {
                          static_cast<double>(w->dir_z[i])};
         double run = static_cast<double>(w->run_timer[i]) + dt;
         if (delta <= 0.0 || run >= ASTRO_TAXIS_RUN_MAX) {
-            Pcg r = cell_rng(w->rng_state[i], w->id[i]);
-            heading = taxis_tumble(heading, r);
-            w->rng_state[i] = r.state;
+            if (d.rng_kind == ASTRO_RNG_PHILOX) {
+                Rng r = philox_stream(d.seed, ASTRO_RNG_STAGE_TAXIS, w->tick, w->id[i]);
+                heading = taxis_tumble(heading, r);
+            } else {
+                Rng r = cell_rng(w->rng_state[i], w->id[i]);
+                heading = taxis_tumble(heading, r);
+                w->rng_state[i] = r.state;
+            }
             run = 0.0;
         }
         w->run_timer[i] = static_cast<float>(run);
@@ -843,7 +905,9 @@ void integrate(oracle_world* w, double dt) {
         V3 pos = pos_of(w, i);
         V3 vel{w->vx[i], w->vy[i], w->vz[i]};
         if (fl & ASTRO_FLAG_STUCK) gamma *= ASTRO_WALL_STUCK_DRAG_MULT;
-        Pcg r = cell_rng(w->rng_state[i], w->id[i]);
+        const bool philox = d.rng_kind == ASTRO_RNG_PHILOX;
+        Rng r = philox ? philox_stream(d.seed, ASTRO_RNG_STAGE_INTEGRATE, w->tick, w->id[i])
+                       : cell_rng(w->rng_state[i], w->id[i]);
         const Ou c = ou_coefficients(mass, gamma, t_drag, dt);   // integrate_cell, integrator.cuh:158-169
         double z1, z2, z3, z4, z5, z6;
         gaussian_pair(r, z1, z2);
@@ -853,7 +917,7 @@ void integrate(oracle_world* w, double dt) {
         ou_advance_axis(pos.x, vel.x, force.x, gamma, c, dt, noise, z1, z2);
         ou_advance_axis(pos.y, vel.y, force.y, gamma, c, dt, noise, z3, z4);
         ou_advance_axis(pos.z, vel.z, force.z, gamma, c, dt, noise, z5, z6);
-        w->rng_state[i] = r.state;
+        if (!philox) w->rng_state[i] = r.state;
         const double a = ASTRO_CELL_RADIUS;
         bool kept = apply_boundary_axis(pos.x, vel.x, 0.5 * d.chamber_w, a, d.boundary_x);
         kept = apply_boundary_axis(pos.y, vel.y, 0.5 * d.chamber_h, a, d.boundary_y) && kept;
@@ -863,9 +927,16 @@ void integrate(oracle_world* w, double dt) {
         uint32_t nf = fl;
         if (d.adhesion_enabled) {
             if (touched && !(fl & ASTRO_FLAG_STUCK)) {
-                Pcg ar = cell_rng(w->rng_state[i], w->id[i]);
-                const bool sticks = uniform01d(ar) < ASTRO_WALL_STICKINESS;
-                w->rng_state[i] = ar.state;
+                // PCG32: re-derived from the state just stored (integrator.cu:
+                // 124-127); Philox: the stage stream's next draws
+                bool sticks;
+                if (philox) {
+                    sticks = uniform01d(r) < ASTRO_WALL_STICKINESS;
+                } else {
+                    Rng ar = cell_rng(w->rng_state[i], w->id[i]);
+                    sticks = uniform01d(ar) < ASTRO_WALL_STICKINESS;
+                    w->rng_state[i] = ar.state;
+                }
                 if (sticks) nf |= ASTRO_FLAG_STUCK;
             } else if (fl & ASTRO_FLAG_STUCK) {
                 const double away = (pos.z < 0.0) ? force.z : -force.z;
@@ -1023,7 +1094,8 @@ void lifecycle_step(oracle_world* w, double dt) {
                 w->emit_power[slot] = 0.0f;
                 w->dir_x[slot] = w->dir_x[i]; w->dir_y[slot] = w->dir_y[i]; w->dir_z[slot] = w->dir_z[i];
                 w->age_s[slot] = 0.0f;
-                w->rng_state[slot] = pcg_split(w->rng_state[i], did);
+                // a counter stream is keyed by the daughter's own id
+                w->rng_state[slot] = d.rng_kind == ASTRO_RNG_PHILOX ? 0ull : pcg_split(w->rng_state[i], did);
                 w->taxis_memory[slot] = 0.0f;
                 w->run_timer[slot] = 0.0f;
                 w->t_local[slot] = w->t_local[i];
@@ -1093,7 +1165,7 @@ inline float tau_daughter_tolerance(float parent, double mutation) {
 }
 
 // gaussian3 (src/core/rng.cuh:111-117) then normalize (src/core/vec.cuh:32-35).
-inline V3 random_heading(Pcg& r) {
+inline V3 random_heading(Rng& r) {
     double a, b, c, d;
     gaussian_pair(r, a, b);
     gaussian_pair(r, c, d);
@@ -1107,7 +1179,7 @@ void tau_spawn(oracle_world* w, int32_t count, uint64_t seed) {
     for (int32_t i = 0; i < count; ++i) {
         const int32_t s = t.count + i;
         const uint64_t id = t.next_id + static_cast<uint64_t>(i);
-        Pcg r = cell_rng(cell_rng_init(splitmix64(seed), id), id);
+        Rng r = cell_rng(cell_rng_init(splitmix64(seed), id), id);
         const double a = ASTRO_TAU_RADIUS;
         t.x[s] = uniform_range(r, -0.5 * d.chamber_w + a, 0.5 * d.chamber_w - a);
         t.y[s] = uniform_range(r, -0.5 * d.chamber_h + a, 0.5 * d.chamber_h - a);
@@ -1167,7 +1239,7 @@ void predation_step(oracle_world* w, double dt) {
         } else {
             t.target[i] = -1;
         }
-        Pcg rng = cell_rng(t.rng_state[i], t.id[i]);
+        Rng rng = cell_rng(t.rng_state[i], t.id[i]);
         float ema = t.density_ema[i];
         float run = t.run_timer[i];
         V3 dir{t.dir_x[i], t.dir_y[i], t.dir_z[i]};
@@ -1232,7 +1304,7 @@ void predation_step(oracle_world* w, double dt) {
     for (int32_t i = 0; i < n; ++i) {
         const uint32_t f = t.flags[i];
         if (!live_flags(f)) continue;
-        Pcg rng = cell_rng(t.rng_state[i], t.id[i]);
+        Rng rng = cell_rng(t.rng_state[i], t.id[i]);
         const double u = uniform01d(rng);
         t.rng_state[i] = rng.state;
         const double n2 = N.enabled ? static_cast<double>(grid_sample(N, static_cast<float>(t.x[i]),
@@ -1263,7 +1335,7 @@ void predation_step(oracle_world* w, double dt) {
             const double half = 0.5 * t.biomass[i];
             t.biomass[i] = half;
             t.biomass[s] = half;
-            Pcg dr = cell_rng(pcg_split(t.rng_state[i], did), did);
+            Rng dr = cell_rng(pcg_split(t.rng_state[i], did), did);
             double mutation, unused;
             gaussian_pair(dr, mutation, unused);
             t.tolerance[s] = tau_daughter_tolerance(t.tolerance[i], mutation);
@@ -1495,7 +1567,8 @@ int32_t oracle_cells_spawn(oracle_world* w, const astro_spawn_params* p, uint64_
     for (int32_t i = 0; i < p->count; ++i) {
         const int32_t slot = w->count + i;
         const uint64_t id = w->next_id + static_cast<uint64_t>(i);
-        Pcg r = cell_rng(cell_rng_init(seed, id), id);
+        const bool philox = d.rng_kind == ASTRO_RNG_PHILOX;
+        Rng r = philox ? philox_stream(seed, ASTRO_RNG_STAGE_SPAWN, 0u, id) : cell_rng(cell_rng_init(seed, id), id);
         double x = 0.0, y = 0.0;
         switch (p->placement) {
             case ASTRO_PLACE_GAUSSIAN: {
@@ -1562,7 +1635,7 @@ int32_t oracle_cells_spawn(oracle_world* w, const astro_spawn_params* p, uint64_
         w->biomass[slot] = ASTRO_CELL_MASS_DRY;
         w->co2_held[slot] = 0.0;
         w->age_s[slot] = 0.0f;
-        w->rng_state[slot] = r.state;
+        w->rng_state[slot] = philox ? 0ull : r.state;
         w->taxis_memory[slot] = 0.0f;
         w->run_timer[slot] = 0.0f;
         w->t_local[slot] = w->temp_cell[slot];
@@ -2044,12 +2117,30 @@ int32_t oracle_stage_compact(oracle_world* w) {
 // ---- building blocks for known-answer tests ----
 uint64_t oracle_splitmix64(uint64_t x) { return splitmix64(x); }
 void oracle_pcg32_seed(uint64_t is, uint64_t iq, uint64_t* st, uint64_t* inc) {
-    const Pcg r = pcg_seed(is, iq);
+    const Rng r = pcg_seed(is, iq);
     *st = r.state;
     *inc = r.inc;
 }
+int32_t oracle_philox_blocks(const uint32_t* words, int32_t n, uint32_t* out) {
+    if (n < 0 || (n > 0 && (!words || !out))) return ASTRO_INVALID_ARGUMENT;
+    for (int32_t i = 0; i < n; ++i) {
+        const uint32_t* w = words + 6 * static_cast<size_t>(i);
+        uint32_t c[4] = {w[0], w[1], w[2], w[3]};
+        philox4x32_10(c, w[4], w[5]);
+        for (int k = 0; k < 4; ++k) out[4 * static_cast<size_t>(i) + k] = c[k];
+    }
+    return ASTRO_OK;
+}
+
+int32_t oracle_philox_stream(uint64_t seed, uint64_t id, uint64_t tick, int32_t stage, int32_t n, uint32_t* out) {
+    if (n < 0 || stage < 0 || stage > 0xFFFF || (n > 0 && !out)) return ASTRO_INVALID_ARGUMENT;
+    Rng r = philox_stream(seed, static_cast<uint32_t>(stage), tick, id);
+    for (int32_t k = 0; k < n; ++k) out[k] = pcg_next(r);
+    return ASTRO_OK;
+}
+
 uint32_t oracle_pcg32_next(uint64_t* st, uint64_t inc) {
-    Pcg r{*st, inc};
+    Rng r{*st, inc};
     const uint32_t v = pcg_next(r);
     *st = r.state;
     return v;
@@ -2058,13 +2149,13 @@ uint64_t oracle_cell_stream_inc(uint64_t id) { return stream_inc(id); }
 uint64_t oracle_cell_rng_init(uint64_t seed, uint64_t id) { return cell_rng_init(seed, id); }
 uint64_t oracle_pcg_split(uint64_t p, uint64_t d) { return pcg_split(p, d); }
 double oracle_uniform01d(uint64_t* st, uint64_t inc) {
-    Pcg r{*st, inc};
+    Rng r{*st, inc};
     const double v = uniform01d(r);
     *st = r.state;
     return v;
 }
 void oracle_gaussian_pair(uint64_t* st, uint64_t inc, double* z0, double* z1) {
-    Pcg r{*st, inc};
+    Rng r{*st, inc};
     gaussian_pair(r, *z0, *z1);
     *st = r.state;
 }
@@ -2100,7 +2191,7 @@ double oracle_lumped_exchange_energy(double tc, double tg, double g, double c, d
 }
 float oracle_taxis_ema_update(float ema, double s, double dt) { return taxis_ema_update(ema, s, dt); }
 double oracle_taxis_tumble_angle(uint64_t* st, uint64_t inc) {
-    Pcg r{*st, inc};
+    Rng r{*st, inc};
     const double v = taxis_tumble_angle(r);
     *st = r.state;
     return v;

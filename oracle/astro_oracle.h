@@ -82,6 +82,11 @@ int32_t oracle_stage_diffuse(oracle_world* w, int32_t field, double dt);
 int32_t oracle_stage_lifecycle(oracle_world* w, double dt);
 int32_t oracle_stage_compact(oracle_world* w);
 
+/* Counter-based streams (include/astro_hot.h, ASTRO_RNG_PHILOX). */
+int32_t oracle_philox_blocks(const uint32_t* words, int32_t n, uint32_t* out);
+int32_t oracle_philox_stream(uint64_t seed, uint64_t id, uint64_t tick, int32_t stage, int32_t n,
+                             uint32_t* out);
+
 /* Building blocks exposed for known-answer tests. */
 uint64_t oracle_splitmix64(uint64_t x);
 void     oracle_pcg32_seed(uint64_t initstate, uint64_t initseq, uint64_t* state, uint64_t* inc);

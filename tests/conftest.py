@@ -23,6 +23,7 @@ PKG = ROOT / "astrophage-microscope_amd"
 ORACLE = ROOT / "oracle" / "liboracle.so"
 REF = ROOT / "oracle" / "_ref" / "libastro_ref.so"
 sys.path.insert(0, str(PKG))
+sys.path.insert(0, str(ROOT))
 
 import astro_hot as ah  # noqa: E402
 
