@@ -270,6 +270,7 @@ SIGNATURES = {
     "stage_predation": (_I, [_P, _D]),
 }
 PRODUCT_ONLY = {
+    "hash_build_noalloc": (_I, [_P, _P, _P, _P, _I, _I, _I, _I, _D, _P, _P, _P, _P]),
     "profile_enable": (_I, [_P, _I]),
     "profile_read": (_I, [_P, _P, _I, _P, _P, _I, C.POINTER(_I)]),
     "profile_reset": (_I, [_P]),

@@ -25,7 +25,8 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "lib" / "libastro_hot.so"
-SOURCES = ["world.hip", "kernels_hash.hip", "kernels_cells.hip", "kernels_fields.hip", "kernels_tau.hip", "surrogate.hip"]
+SOURCES = ["world.hip", "kernels_hash.hip", "kernels_cells.hip", "kernels_fields.hip", "kernels_tau.hip", "surrogate.hip",
+           "hash_noalloc.hip"]
 # Host-only C++ above the C ABI (no device code), compiled by g++.
 HOST_SOURCES = ["scenario.cpp"]
 HOST_CXX = os.environ.get("CXX_HOST", "g++")

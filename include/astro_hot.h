@@ -328,6 +328,32 @@ int32_t astro_snapshot_from_bytes(const astro_world_desc* base, const void* data
 /* snapshot_load (src/sim/snapshot.cu:255-264). */
 int32_t astro_snapshot_load(const astro_world_desc* base, const char* path, astro_world** out);
 
+/* ---- the hash build without allocation ---------------------------------------
+ * No counterpart in the reference. For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc. It is astro_stage_hash_build + astro_hash_download
+ * (hash_build, src/sim/hash.cu:106-133) on caller-owned DEVICE arrays: the n
+ * cells' positions (contract f64) and flags in, the stable order of slots by
+ * bucket (free slots last, in slot order) and every bucket's [start, end)
+ * out (start == end == 0 when empty), for a hash of nx x ny x nz buckets of
+ * cell_size metres centred on the origin (as astro_hash_dims reports).
+ * Nothing is allocated: every intermediate lives in one static device buffer
+ * of ASTRO_HASH_NOALLOC_SCRATCH_BYTES (32 MiB) that each GPU the library runs
+ * on holds until the process ends. n above ASTRO_HASH_NOALLOC_MAX_CELLS or
+ * nx*ny*nz above ASTRO_HASH_NOALLOC_MAX_BUCKETS is ASTRO_CAPACITY_EXCEEDED.
+ * Enqueued on `stream` (a hipStream_t; NULL = the null stream) without
+ * waiting: the outputs are valid once the stream reaches this point. All
+ * calls on one GPU share the buffer, so calls that overlap on a GPU (other
+ * streams, other threads) race; serialize them. Should the build's scan
+ * self-check fail, every bucket_start is -1. */
+#define ASTRO_HASH_NOALLOC_MAX_CELLS   1048576
+#define ASTRO_HASH_NOALLOC_MAX_BUCKETS 4194304
+#define ASTRO_HASH_NOALLOC_SCRATCH_BYTES 33560320u
+int32_t astro_hash_build_noalloc(const double* x, const double* y, const double* z, const uint32_t* flags,
+                                 int32_t n, int32_t nx, int32_t ny, int32_t nz, double cell_size,
+                                 uint32_t* vals_sorted, int32_t* bucket_start, int32_t* bucket_end,
+                                 void* stream);
+
 /* ---- stage entry points for kernel-level parity ---------------------------- */
 /* hash_build (src/sim/hash.cu:106-133) on the current population. */
 int32_t astro_stage_hash_build(astro_world* w);
