@@ -8,6 +8,7 @@
 // it follows.
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -89,142 +90,213 @@ const Named METRICS[] = {{"awake_fraction", ASTRO_METRIC_AWAKE_FRACTION},
                          {"doubling_time_s", ASTRO_METRIC_DOUBLING_TIME_S},
                          {"impulse_per_cycle", ASTRO_METRIC_IMPULSE_PER_CYCLE}};
 
-// scenario_parse's field reads (src/sim/scenario.cpp:139-318), onto a
-// zeroed scenario.
+// ---- the scenario schema -----------------------------------------------------
+// The JSON vocabulary, defaults and caps are the reference's
+// (src/sim/scenario.cpp:139-318, contracts/scenario_v3.h); the reader here is
+// declarative: each object of the document is a table of Fields, read the same
+// way whether the object is present or not (absent members, and an absent
+// object, take the defaults).
+enum class Ty { F64, F32, I32, U64, FLAG, TEXT, NAME };
+
+struct Field {
+    const char* keys;     // "key" or "key|alternative|...": the first present wins
+    Ty ty;
+    size_t off;           // byte offset in the target struct
+    double def = 0.0;     // F64 / F32 / I32 / U64 / FLAG default
+    const char* def_name = "";             // TEXT / NAME default
+    const Named* names = nullptr;          // NAME vocabulary
+    size_t n_names = 0;
+    size_t cap = 0;                        // TEXT capacity
+};
+
+template <size_t N>
+Field name_field(const char* key, size_t off, const Named (&v)[N], const char* def) {
+    return Field{key, Ty::NAME, off, 0.0, def, v, N, 0};
+}
+Field text_field(const char* key, size_t off, size_t cap) { return Field{key, Ty::TEXT, off, 0.0, "", nullptr, 0, cap}; }
+
+// The member an alternative-key list names, or null.
+const jsonc::Node* member(const jsonc::Node* obj, const char* keys) {
+    if (!obj) return nullptr;
+    const char* k = keys;
+    while (*k) {
+        const char* bar = std::strchr(k, '|');
+        const size_t len = bar ? static_cast<size_t>(bar - k) : std::strlen(k);
+        if (const jsonc::Node* m = obj->at(std::string_view(k, len))) return m;
+        k += len + (bar ? 1 : 0);
+    }
+    return nullptr;
+}
+
+// Reads every field of `table` from `obj` (null: all defaults) into `base`.
+// A NAME outside its vocabulary takes the default, unless `strict`, which
+// makes it an error (a dropped stimulus or check would change the run).
+template <size_t N>
+bool read_fields(const jsonc::Node* obj, void* base, const Field (&table)[N], bool strict = false) {
+    char* b = static_cast<char*>(base);
+    for (const Field& f : table) {
+        const jsonc::Node* m = member(obj, f.keys);
+        const double num = m ? m->as_num(f.def) : f.def;
+        switch (f.ty) {
+            case Ty::F64: *reinterpret_cast<double*>(b + f.off) = num; break;
+            case Ty::F32: *reinterpret_cast<float*>(b + f.off) = static_cast<float>(num); break;
+            case Ty::I32: *reinterpret_cast<int32_t*>(b + f.off) = static_cast<int32_t>(num); break;
+            case Ty::U64: *reinterpret_cast<uint64_t*>(b + f.off) = static_cast<uint64_t>(num); break;
+            case Ty::FLAG:
+                *reinterpret_cast<int32_t*>(b + f.off) = (m ? m->as_bool(f.def != 0.0) : f.def != 0.0) ? 1 : 0;
+                break;
+            case Ty::TEXT: copy_text(b + f.off, f.cap, m ? m->as_str("") : std::string()); break;
+            case Ty::NAME: {
+                const std::string text = m ? m->as_str(f.def_name) : std::string(f.def_name);
+                int32_t v = -1;
+                for (size_t i = 0; i < f.n_names; ++i)
+                    if (text == f.names[i].name) v = f.names[i].value;
+                if (v < 0) {
+                    if (strict) return false;
+                    for (size_t i = 0; i < f.n_names; ++i)
+                        if (std::string_view(f.def_name) == f.names[i].name) v = f.names[i].value;
+                }
+                *reinterpret_cast<int32_t*>(b + f.off) = v;
+                break;
+            }
+        }
+    }
+    return true;
+}
+
+// Reads up to `cap` elements of the list `key` of `obj`, each with `table`,
+// into consecutive structs of type T; returns the count, or -1 on a strict
+// name outside its vocabulary.
+template <class T, size_t N>
+int read_list(const jsonc::Node& obj, const char* key, T* out, int cap, const Field (&table)[N], bool strict = false) {
+    int n = 0;
+    for (const jsonc::Node& e : obj.list(key)) {
+        if (n >= cap) break;
+        if (!read_fields(&e, &out[n], table, strict)) return -1;
+        ++n;
+    }
+    return n;
+}
+
+const Named GRAVITY_AXES[] = {{"y", ASTRO_GRAVITY_Y}, {"z", ASTRO_GRAVITY_Z}};
+
+#define AT(field) offsetof(astro_scenario, field)
+const Field TOP[] = {
+    {"schema", Ty::I32, AT(schema), 1.0},
+    text_field("id", AT(id), sizeof(astro_scenario::id)),
+    text_field("title", AT(title), sizeof(astro_scenario::title)),
+    text_field("blurb", AT(blurb), sizeof(astro_scenario::blurb)),
+    {"seed", Ty::U64, AT(seed), 20260802.0},
+    {"compaction", Ty::FLAG, AT(compaction), 0.0},
+    {"tau_compaction", Ty::FLAG, AT(tau_compaction), 0.0},
+};
+const Field CHAMBER[] = {
+    {"w", Ty::F64, AT(chamber_w), ASTRO_CHAMBER_W},
+    {"h", Ty::F64, AT(chamber_h), ASTRO_CHAMBER_H},
+    {"d", Ty::F64, AT(chamber_d), ASTRO_CHAMBER_D},
+    name_field("boundary_x", AT(boundary_x), BOUNDARIES, "reflecting"),
+    name_field("boundary_y", AT(boundary_y), BOUNDARIES, "reflecting"),
+};
+const Field MEDIUM[] = {
+    {"temp_init", Ty::F64, AT(temp_init), ASTRO_AMBIENT_TEMP_DEFAULT},
+    {"co2_init", Ty::F64, AT(co2_init), 0.0},
+    {"n2_init", Ty::F64, AT(n2_init), 0.0},
+    {"ambient_temp", Ty::F64, AT(ambient_temp), ASTRO_AMBIENT_TEMP_DEFAULT},
+    name_field("thermal_bc", AT(thermal_bc), THERMAL_BCS, "robin"),
+    name_field("density_model", AT(density_model), DENSITIES, "canon-mass"),
+    name_field("store_disposition", AT(store_disposition), DISPOSITIONS, "void"),
+    name_field("gravity_axis", AT(gravity_axis), GRAVITY_AXES, "y"),
+    {"thermal_active", Ty::FLAG, AT(thermal_active), 1.0},
+};
+const Field LIGHT[] = {{"ambient", Ty::F32, AT(ambient_irradiance), 0.0}};
+const Field CLOCK[] = {
+    name_field("preset", AT(clock), CLOCKS, "realtime"),
+    {"physics_rate", Ty::F64, AT(physics_rate), 1.0},
+    {"biology_rate", Ty::F64, AT(biology_rate), 1.0},
+};
+const Field OBJECTIVE[] = {
+    text_field("text", AT(objective_text), sizeof(astro_scenario::objective_text)),
+    {"duration_s", Ty::F64, AT(run_duration_s), 0.0},
+};
+#undef AT
+
+#define AT(field) offsetof(astro_light_source, field)
+const Field SOURCE[] = {
+    {"dir_x", Ty::F32, AT(dir_x), 1.0}, {"dir_y", Ty::F32, AT(dir_y), 0.0},
+    {"irradiance", Ty::F32, AT(irradiance), 0.0}, {"wavelength", Ty::F32, AT(wavelength), 0.0},
+};
+#undef AT
+#define AT(field) offsetof(astro_population_spec, field)
+const Field POPULATION[] = {
+    name_field("kind", AT(kind), ORGANISMS, "astrophage"),
+    {"count", Ty::I32, AT(count), 0.0},
+    name_field("placement", AT(placement), PLACEMENTS, "uniform"),
+    {"place_x", Ty::F64, AT(place_x), 0.0},
+    {"place_y", Ty::F64, AT(place_y), 0.0},
+    {"place_radius", Ty::F64, AT(place_radius), 0.0},
+    {"awake", Ty::FLAG, AT(awake), 0.0},
+};
+const Field CHARGE[] = {
+    name_field("dist", AT(charge_dist), DISTRIBUTIONS, "constant"),
+    {"value|a", Ty::F64, AT(charge_a), 0.0},
+    {"b", Ty::F64, AT(charge_b), 0.0},
+};
+#undef AT
+#define AT(field) offsetof(astro_stimulus, field)
+const Field STIMULUS[] = {
+    {"kind", Ty::NAME, AT(kind), 0.0, "", STIMULI, std::size(STIMULI), 0},
+    {"t0_s", Ty::F64, AT(t0_s), 0.0}, {"t1_s", Ty::F64, AT(t1_s), 0.0},
+    {"x", Ty::F64, AT(x), 0.0}, {"y", Ty::F64, AT(y), 0.0}, {"radius", Ty::F64, AT(radius), 0.0},
+    {"v0|value", Ty::F64, AT(v0), 0.0},
+};
+#undef AT
+#define AT(field) offsetof(astro_accept_check, field)
+const Field CHECK[] = {
+    {"metric", Ty::NAME, AT(metric), 0.0, "", METRICS, std::size(METRICS), 0},
+    name_field("op", AT(op), OPS, "=="),
+    {"value", Ty::F64, AT(value), 0.0},
+    {"tol", Ty::F64, AT(tol), 0.0},
+    {"after_s", Ty::F64, AT(after_s), 0.0},
+    {"tol_absolute", Ty::FLAG, AT(tol_absolute), 0.0},
+};
+#undef AT
+
+// scenario_parse (src/sim/scenario.cpp:139-318) onto a zeroed scenario.
 int32_t read_scenario(const jsonc::Node& root, astro_scenario& o) {
-    o.schema = static_cast<int32_t>(root.num("schema", 1));
-    copy_text(o.id, sizeof(o.id), root.str("id", ""));
-    copy_text(o.title, sizeof(o.title), root.str("title", ""));
-    copy_text(o.blurb, sizeof(o.blurb), root.str("blurb", ""));
-    o.seed = static_cast<uint64_t>(root.num("seed", 20260802.0));
-
-    o.chamber_w = ASTRO_CHAMBER_W;
-    o.chamber_h = ASTRO_CHAMBER_H;
-    o.chamber_d = ASTRO_CHAMBER_D;
-    if (const jsonc::Node* ch = root.at("chamber")) {
-        o.chamber_w = ch->num("w", ASTRO_CHAMBER_W);
-        o.chamber_h = ch->num("h", ASTRO_CHAMBER_H);
-        o.chamber_d = ch->num("d", ASTRO_CHAMBER_D);
-        o.boundary_x = pick(BOUNDARIES, ch->str("boundary_x", "reflecting"), ASTRO_BOUNDARY_REFLECTING);
-        o.boundary_y = pick(BOUNDARIES, ch->str("boundary_y", "reflecting"), ASTRO_BOUNDARY_REFLECTING);
+    read_fields(&root, &o, TOP);
+    read_fields(root.at("chamber"), &o, CHAMBER);
+    read_fields(root.at("medium"), &o, MEDIUM);
+    read_fields(root.at("clock"), &o, CLOCK);
+    const jsonc::Node* light = root.at("light");
+    read_fields(light, &o, LIGHT);
+    if (light) {
+        const int n = read_list(*light, "sources", o.lights, ASTRO_SCENARIO_MAX_LIGHTS, SOURCE);
+        for (int i = 0; i < n; ++i) o.lights[i].enabled = 1;
     }
-
-    o.temp_init = ASTRO_AMBIENT_TEMP_DEFAULT;
-    o.ambient_temp = ASTRO_AMBIENT_TEMP_DEFAULT;
-    o.thermal_bc = ASTRO_BC_ROBIN;
-    o.density_model = ASTRO_DENSITY_CANON_MASS;
-    o.store_disposition = ASTRO_STORE_VOID;
-    o.thermal_active = 1;
-    if (const jsonc::Node* md = root.at("medium")) {
-        o.temp_init = md->num("temp_init", ASTRO_AMBIENT_TEMP_DEFAULT);
-        o.co2_init = md->num("co2_init", 0.0);
-        o.n2_init = md->num("n2_init", 0.0);
-        o.ambient_temp = md->num("ambient_temp", ASTRO_AMBIENT_TEMP_DEFAULT);
-        o.thermal_bc = pick(THERMAL_BCS, md->str("thermal_bc", "robin"), ASTRO_BC_ROBIN);
-        o.density_model = pick(DENSITIES, md->str("density_model", "canon-mass"), ASTRO_DENSITY_CANON_MASS);
-        o.store_disposition = pick(DISPOSITIONS, md->str("store_disposition", "void"), ASTRO_STORE_VOID);
-        o.gravity_axis = md->str("gravity_axis", "y") == "z" ? ASTRO_GRAVITY_Z : ASTRO_GRAVITY_Y;
-        o.thermal_active = md->flag("thermal_active", true) ? 1 : 0;
-    }
-
-    if (const jsonc::Node* lt = root.at("light")) {
-        o.ambient_irradiance = static_cast<float>(lt->num("ambient", 0.0));
-        int n = 0;
-        for (const jsonc::Node& src : lt->list("sources")) {
-            if (n >= ASTRO_SCENARIO_MAX_LIGHTS) break;
-            astro_light_source& l = o.lights[n++];
-            l.dir_x = static_cast<float>(src.num("dir_x", 1.0));
-            l.dir_y = static_cast<float>(src.num("dir_y", 0.0));
-            l.irradiance = static_cast<float>(src.num("irradiance", 0.0));
-            l.wavelength = static_cast<float>(src.num("wavelength", 0.0));
-            l.enabled = 1;
-        }
-    }
-
-    int np = 0;
-    for (const jsonc::Node& p : root.list("populations")) {
-        if (np >= ASTRO_SCENARIO_MAX_POPULATIONS) break;
-        astro_population_spec& ps = o.populations[np++];
-        ps.kind = pick(ORGANISMS, p.str("kind", "astrophage"), ASTRO_ORGANISM_ASTROPHAGE);
-        ps.count = static_cast<int32_t>(p.num("count", 0));
-        ps.placement = pick(PLACEMENTS, p.str("placement", "uniform"), ASTRO_PLACE_UNIFORM);
-        ps.place_x = p.num("place_x", 0.0);
-        ps.place_y = p.num("place_y", 0.0);
-        ps.place_radius = p.num("place_radius", 0.0);
-        ps.charge_dist = ASTRO_DIST_CONSTANT;
-        if (const jsonc::Node* cg = p.at("charge")) {
-            ps.charge_dist = pick(DISTRIBUTIONS, cg->str("dist", "constant"), ASTRO_DIST_CONSTANT);
-            ps.charge_a = cg->num("value", cg->num("a", 0.0));
-            ps.charge_b = cg->num("b", 0.0);
-        }
-        ps.awake = p.flag("awake", false) ? 1 : 0;
-    }
-    o.population_count = np;
-
-    o.compaction = root.flag("compaction", false) ? 1 : 0;
-    o.tau_compaction = root.flag("tau_compaction", false) ? 1 : 0;
-
-    o.clock = ASTRO_CLOCK_REALTIME;
-    o.physics_rate = 1.0;
-    o.biology_rate = 1.0;
-    if (const jsonc::Node* ck = root.at("clock")) {
-        o.clock = pick(CLOCKS, ck->str("preset", "realtime"), ASTRO_CLOCK_REALTIME);
-        o.physics_rate = ck->num("physics_rate", 1.0);
-        o.biology_rate = ck->num("biology_rate", 1.0);
-    }
-
+    o.population_count = read_list(root, "populations", o.populations, ASTRO_SCENARIO_MAX_POPULATIONS, POPULATION);
+    for (int i = 0; i < o.population_count; ++i) read_fields(root.list("populations")[i].at("charge"), &o.populations[i], CHARGE);
     for (const jsonc::Node& t : root.list("tools"))
         if (t.kind == jsonc::Node::STR) o.tools |= static_cast<uint32_t>(pick(TOOLS, t.s, 0));
-
-    int nd = 0;
-    for (const jsonc::Node& e : root.list("drive")) {
-        if (nd >= ASTRO_SCENARIO_MAX_STIMULI) break;
-        bool known = false;
-        const int32_t kind = pick(STIMULI, e.str("kind", ""), ASTRO_STIM_HEAT, &known);
-        if (!known) return ASTRO_INVALID_ARGUMENT;   // a dropped stimulus would change the run
-        astro_stimulus& st = o.drive[nd++];
-        st.kind = kind;
-        st.t0_s = e.num("t0_s", 0.0);
-        st.t1_s = e.num("t1_s", 0.0);
-        st.x = e.num("x", 0.0);
-        st.y = e.num("y", 0.0);
-        st.radius = e.num("radius", 0.0);
-        st.v0 = e.num("v0", e.num("value", 0.0));
-        st.v1 = e.num("v1", e.num("value", st.v0));
+    o.drive_count = read_list(root, "drive", o.drive, ASTRO_SCENARIO_MAX_STIMULI, STIMULUS, true);
+    if (o.drive_count < 0) return ASTRO_INVALID_ARGUMENT;
+    // a window's end value defaults to its start value
+    for (int i = 0; i < o.drive_count; ++i) {
+        const jsonc::Node* v1 = member(&root.list("drive")[i], "v1|value");
+        o.drive[i].v1 = v1 ? v1->as_num(o.drive[i].v0) : o.drive[i].v0;
     }
-    o.drive_count = nd;
-
     // Overrides are carried, not applied, as in the reference.
-    if (const jsonc::Node* po = root.at("param_overrides")) {
-        int n = 0;
-        if (po->kind == jsonc::Node::OBJ)
-            for (const auto& m : po->members) {
-                if (n >= ASTRO_SCENARIO_MAX_OVERRIDES) break;
-                copy_text(o.param_overrides[n].key, sizeof(o.param_overrides[n].key), m.first);
-                o.param_overrides[n].value = m.second.as_num(0.0);
-                ++n;
-            }
-        o.override_count = n;
-    }
-
-    if (const jsonc::Node* ob = root.at("objective")) {
-        copy_text(o.objective_text, sizeof(o.objective_text), ob->str("text", ""));
-        o.run_duration_s = ob->num("duration_s", 0.0);
-        int na = 0;
-        for (const jsonc::Node& c : ob->list("accept")) {
-            if (na >= ASTRO_SCENARIO_MAX_ACCEPT) break;
-            bool known = false;
-            const int32_t metric = pick(METRICS, c.str("metric", ""), ASTRO_METRIC_COUNT, &known);
-            if (!known) return ASTRO_INVALID_ARGUMENT;
-            astro_accept_check& k = o.accept[na++];
-            k.metric = metric;
-            k.op = pick(OPS, c.str("op", "=="), ASTRO_OP_EQ);
-            k.value = c.num("value", 0.0);
-            k.tol = c.num("tol", 0.0);
-            k.after_s = c.num("after_s", 0.0);
-            k.tol_absolute = c.flag("tol_absolute", false) ? 1 : 0;
+    if (const jsonc::Node* po = root.at("param_overrides"); po && po->kind == jsonc::Node::OBJ)
+        for (const auto& [key, value] : po->members) {
+            if (o.override_count >= ASTRO_SCENARIO_MAX_OVERRIDES) break;
+            astro_param_override& d = o.param_overrides[o.override_count++];
+            copy_text(d.key, sizeof(d.key), key);
+            d.value = value.as_num(0.0);
         }
-        o.accept_count = na;
+    const jsonc::Node* objective = root.at("objective");
+    read_fields(objective, &o, OBJECTIVE);
+    if (objective) {
+        o.accept_count = read_list(*objective, "accept", o.accept, ASTRO_SCENARIO_MAX_ACCEPT, CHECK, true);
+        if (o.accept_count < 0) return ASTRO_INVALID_ARGUMENT;
     }
     return ASTRO_OK;
 }
@@ -238,27 +310,32 @@ bool stimulus_on(const astro_stimulus& st, double t) {
 
 // ---- acceptance (src/sim/accept.cpp) -----------------------------------------
 
-// Pearson correlation; 0 when undefined (src/sim/accept.cpp:19-34).
-double correlation(const std::vector<double>& a, const std::vector<double>& b) {
-    const size_t n = a.size();
-    if (n < 2 || b.size() != n) return 0.0;
-    double ma = 0.0, mb = 0.0;
-    for (size_t i = 0; i < n; ++i) {
-        ma += a[i];
-        mb += b[i];
+// Least squares of y on x over pairs added one at a time: running means and
+// co-moments updated as in Welford's method, so no second pass and no
+// cancellation between large sums. Gives the three fits the acceptance
+// metrics use (src/sim/accept.cpp): Pearson's r (0 when either variance is
+// 0 or fewer than two pairs), and a line y = a + b x (b = 0 when x does not
+// vary).
+class LineFit {
+public:
+    void add(double x, double y) {
+        n_ += 1.0;
+        const double dx = x - mx_;
+        mx_ += dx / n_;
+        const double dy = y - my_;
+        my_ += dy / n_;
+        sxx_ += dx * (x - mx_);
+        syy_ += dy * (y - my_);
+        sxy_ += dx * (y - my_);
     }
-    ma /= static_cast<double>(n);
-    mb /= static_cast<double>(n);
-    double sab = 0.0, saa = 0.0, sbb = 0.0;
-    for (size_t i = 0; i < n; ++i) {
-        const double da = a[i] - ma, db = b[i] - mb;
-        sab += da * db;
-        saa += da * da;
-        sbb += db * db;
-    }
-    if (saa <= 0.0 || sbb <= 0.0) return 0.0;
-    return sab / std::sqrt(saa * sbb);
-}
+    size_t count() const { return static_cast<size_t>(n_); }
+    double r() const { return (n_ >= 2.0 && sxx_ > 0.0 && syy_ > 0.0) ? sxy_ / std::sqrt(sxx_ * syy_) : 0.0; }
+    double slope() const { return sxx_ > 0.0 ? sxy_ / sxx_ : 0.0; }
+    double intercept() const { return my_ - slope() * mx_; }
+
+private:
+    double n_ = 0.0, mx_ = 0.0, my_ = 0.0, sxx_ = 0.0, syy_ = 0.0, sxy_ = 0.0;
+};
 
 // Positions and energies of slots [0, count) (the reference's
 // cell_store_download_positions / _energy).
@@ -288,205 +365,168 @@ int32_t sample_cells(astro_world* w, CellSample& out) {
     return n > 0 ? astro_cells_download(w, &s, n, &got) : ASTRO_OK;
 }
 
-// RunAggregates (src/sim/accept.h:31-53).
-struct Aggregates {
-    int64_t boil_events = 0;
-    double max_medium_k = 0.0;
-    std::vector<double> t_series, n_series;
-    int vel_stage = 0;
-    double vel_ta = 0.0;
-    std::vector<double> vel_y_a, vel_charge_a;
-    double rise_velocity_empty = 0.0, fall_velocity_full = 0.0;
-};
-
-struct Needs {
-    bool velocity = false, population = false;
-};
-
-// metric_needs (src/sim/accept.cpp:113-125).
-Needs needs_of(const astro_scenario& s) {
-    Needs n;
-    for (int i = 0; i < s.accept_count; ++i) {
-        const int32_t m = s.accept[i].metric;
-        if (m == ASTRO_METRIC_RISE_VELOCITY_EMPTY || m == ASTRO_METRIC_FALL_VELOCITY_FULL) n.velocity = true;
-        if (m == ASTRO_METRIC_DOUBLING_TIME_S) n.population = true;
+// What a run accumulates for its checks between stats samples (the
+// reference's RunAggregates and aggregates_sample, src/sim/accept.h:31-53,
+// src/sim/accept.cpp:113-183): the worst medium temperature and the boil
+// events; the live population over time, when a doubling time is checked;
+// and, when a settling speed is checked, two position snapshots `interval_s`
+// apart taken after `settle_s`, fitted to speed = a + b * charge.
+class RunRecord {
+public:
+    RunRecord(const astro_scenario& s, double settle_s, double interval_s)
+        : settle_s_(settle_s), interval_s_(interval_s), half_h_(0.5 * s.chamber_h) {
+        for (int i = 0; i < s.accept_count; ++i) {
+            const int32_t m = s.accept[i].metric;
+            track_speed_ |= m == ASTRO_METRIC_RISE_VELOCITY_EMPTY || m == ASTRO_METRIC_FALL_VELOCITY_FULL;
+            track_growth_ |= m == ASTRO_METRIC_DOUBLING_TIME_S;
+        }
     }
-    return n;
-}
 
-// aggregates_sample (src/sim/accept.cpp:127-183).
-int32_t sample(Aggregates& a, const astro_stats& st, astro_world* w, const Needs& needs, double settle_s,
-               double interval_s, double chamber_h) {
-    a.boil_events += st.boil_events;
-    a.max_medium_k = std::max(a.max_medium_k, st.max_temp_medium_k);
-    if (needs.population) {
-        a.t_series.push_back(st.sim_time_s);
-        a.n_series.push_back(static_cast<double>(st.n_live));
-    }
-    if (!needs.velocity || a.vel_stage >= 2) return ASTRO_OK;
-    const bool take_a = a.vel_stage == 0 && st.sim_time_s >= settle_s;
-    const bool take_b = a.vel_stage == 1 && st.sim_time_s >= a.vel_ta + interval_s;
-    if (!take_a && !take_b) return ASTRO_OK;
-    CellSample c;
-    const int32_t rc = sample_cells(w, c);
-    if (rc != ASTRO_OK) return rc;
-    const size_t n = c.x.size();
-    if (n == 0) return ASTRO_OK;
-    if (take_a) {
-        // first position sample: y and charge of every cell
-        a.vel_y_a = c.y;
-        a.vel_charge_a.resize(n);
-        for (size_t i = 0; i < n; ++i) a.vel_charge_a[i] = c.energy[i] / ASTRO_CELL_ENERGY_MAX;
-        a.vel_ta = st.sim_time_s;
-        a.vel_stage = 1;
+    int32_t observe(const astro_stats& st, astro_world* w) {
+        boil_events_ += st.boil_events;
+        hottest_k_ = std::max(hottest_k_, st.max_temp_medium_k);
+        if (track_growth_) growth_.emplace_back(st.sim_time_s, static_cast<double>(st.n_live));
+        if (!track_speed_ || speed_done_) return ASTRO_OK;
+        const bool first = !have_first_ && st.sim_time_s >= settle_s_;
+        const bool second = have_first_ && st.sim_time_s >= t_first_ + interval_s_;
+        if (!first && !second) return ASTRO_OK;
+        CellSample c;
+        const int32_t rc = sample_cells(w, c);
+        if (rc != ASTRO_OK || c.x.empty()) return rc;
+        if (first) {
+            y_first_ = c.y;
+            t_first_ = st.sim_time_s;
+            charge_first_.clear();
+            for (double e : c.energy) charge_first_.push_back(e / ASTRO_CELL_ENERGY_MAX);
+            have_first_ = true;
+            return ASTRO_OK;
+        }
+        fit_speed(c, st.sim_time_s - t_first_);
+        speed_done_ = true;
         return ASTRO_OK;
     }
-    // second sample: settling speed = a + b * charge fitted to the drift of
-    // cells away from the y walls; a is the empty, a + b the full cell's speed
-    const double half_h = 0.5 * chamber_h;
-    const double margin = 3.0 * ASTRO_CELL_DIAMETER;
-    const double dt = st.sim_time_s - a.vel_ta;
-    const size_t m = std::min(n, a.vel_y_a.size());
-    std::vector<double> v, q;
-    v.reserve(m);
-    q.reserve(m);
-    for (size_t i = 0; i < m; ++i) {
-        if (std::fabs(c.y[i]) > half_h - margin) continue;
-        v.push_back(-(c.y[i] - a.vel_y_a[i]) / dt);
-        q.push_back(a.vel_charge_a[i]);
+
+    int64_t boil_events() const { return boil_events_; }
+    double hottest_k() const { return hottest_k_; }
+    double empty_speed() const { return empty_speed_; }
+    double full_speed() const { return full_speed_; }
+
+    // ln N = c + t ln2 / T_d over the samples below 60 % of the peak (past
+    // the peak the medium is exhausted), T_d in culture time: the simulated
+    // time scaled by the biology rate (src/sim/accept.cpp:85-108).
+    double doubling_time(double biology_rate) const {
+        double peak = 0.0;
+        for (const auto& [t, n] : growth_) peak = std::max(peak, n);
+        LineFit fit;
+        for (const auto& [t, n] : growth_)
+            if (n > 0.0 && n <= 0.6 * peak) fit.add(t, std::log(n));
+        const double rate = fit.count() >= 2 ? fit.slope() : 0.0;
+        return rate > 0.0 ? biology_rate * std::log(2.0) / rate : 0.0;
     }
-    const size_t k = v.size();
-    if (k >= 2 && dt > 0.0) {
-        double mq = 0.0, mv = 0.0;
-        for (size_t i = 0; i < k; ++i) {
-            mq += q[i];
-            mv += v[i];
-        }
-        mq /= static_cast<double>(k);
-        mv /= static_cast<double>(k);
-        double sqv = 0.0, sqq = 0.0;
-        for (size_t i = 0; i < k; ++i) {
-            const double dq = q[i] - mq;
-            sqv += dq * (v[i] - mv);
-            sqq += dq * dq;
-        }
-        const double b = sqq > 0.0 ? sqv / sqq : 0.0;
-        const double a0 = mv - b * mq;
-        a.rise_velocity_empty = a0;
-        a.fall_velocity_full = a0 + b;
+
+private:
+    // Downward speed of each cell away from the y walls (3 diameters) against
+    // its charge at the first snapshot: a = the empty cell's rise, a + b the
+    // full cell's fall.
+    void fit_speed(const CellSample& c, double dt) {
+        if (!(dt > 0.0)) return;
+        LineFit fit;
+        const size_t m = std::min(c.y.size(), y_first_.size());
+        for (size_t i = 0; i < m; ++i)
+            if (std::fabs(c.y[i]) <= half_h_ - 3.0 * ASTRO_CELL_DIAMETER)
+                fit.add(charge_first_[i], (y_first_[i] - c.y[i]) / dt);
+        if (fit.count() < 2) return;
+        empty_speed_ = fit.intercept();
+        full_speed_ = fit.intercept() + fit.slope();
     }
-    a.vel_stage = 2;
+
+    double settle_s_, interval_s_, half_h_;
+    bool track_speed_ = false, track_growth_ = false;
+    int64_t boil_events_ = 0;
+    double hottest_k_ = 0.0;
+    std::vector<std::pair<double, double>> growth_;
+    bool have_first_ = false, speed_done_ = false;
+    double t_first_ = 0.0;
+    std::vector<double> y_first_, charge_first_;
+    double empty_speed_ = 0.0, full_speed_ = 0.0;
+};
+
+// Pearson r of charge against height (y), or against depth along the first
+// light source (P1 and P5 of the reference's acceptance metrics).
+int32_t charge_correlation(astro_world* w, bool along_light, const astro_scenario& s, double& out) {
+    CellSample c;
+    const int32_t st = sample_cells(w, c);
+    if (st != ASTRO_OK) return st;
+    LineFit fit;
+    for (size_t i = 0; i < c.x.size(); ++i) {
+        const double pos = along_light ? c.x[i] * s.lights[0].dir_x + c.y[i] * s.lights[0].dir_y : c.y[i];
+        fit.add(c.energy[i] / ASTRO_CELL_ENERGY_MAX, pos);
+    }
+    out = fit.r();
     return ASTRO_OK;
 }
 
-// measure_doubling_time (src/sim/accept.cpp:85-108): ln N against t below 60 %
-// of the peak, reported in culture time.
-double doubling_time(const Aggregates& a, double biology_rate) {
-    const size_t n = a.n_series.size();
-    if (n < 2) return 0.0;
-    const double peak = *std::max_element(a.n_series.begin(), a.n_series.end());
-    if (peak <= 0.0) return 0.0;
-    const double cap = 0.6 * peak;
-    std::vector<double> t, l;
-    for (size_t i = 0; i < n; ++i) {
-        const double N = a.n_series[i];
-        if (N > 0.0 && N <= cap) {
-            t.push_back(a.t_series[i]);
-            l.push_back(std::log(N));
-        }
-    }
-    if (t.size() < 2) return 0.0;
-    double mt = 0.0, ml = 0.0;
-    for (size_t i = 0; i < t.size(); ++i) {
-        mt += t[i];
-        ml += l[i];
-    }
-    mt /= static_cast<double>(t.size());
-    ml /= static_cast<double>(t.size());
-    double stl = 0.0, stt = 0.0;
-    for (size_t i = 0; i < t.size(); ++i) {
-        const double d = t[i] - mt;
-        stl += d * (l[i] - ml);
-        stt += d * d;
-    }
-    if (stt <= 0.0) return 0.0;
-    const double slope = stl / stt;
-    if (slope <= 0.0) return 0.0;
-    return (std::log(2.0) / slope) * biology_rate;
+// The largest N2 tolerance among live Taumoeba.
+int32_t max_tolerance(astro_world* w, double& out) {
+    int32_t n = 0, cap = 0;
+    int32_t st = astro_taumoeba_counters(w, &n, &cap, nullptr);
+    if (st != ASTRO_OK || n <= 0) return st;
+    std::vector<uint64_t> id(n), rng(n);
+    std::vector<uint32_t> flags(n), gen(n);
+    std::vector<double> d64[8];
+    std::vector<float> f32[7];
+    std::vector<int32_t> target(n);
+    for (auto& v : d64) v.resize(n);
+    for (auto& v : f32) v.resize(n);
+    astro_tau_soa t{id.data(),     flags.data(),  d64[0].data(), d64[1].data(), d64[2].data(),
+                    d64[3].data(), d64[4].data(), d64[5].data(), f32[0].data(), f32[1].data(),
+                    f32[2].data(), d64[6].data(), d64[7].data(), f32[3].data(), f32[4].data(),
+                    gen.data(),    rng.data(),    f32[5].data(), f32[6].data(), target.data()};
+    int32_t got = 0;
+    if ((st = astro_taumoeba_download(w, &t, n, &got)) != ASTRO_OK) return st;
+    const uint32_t live = ASTRO_FLAG_OCCUPIED | ASTRO_FLAG_ALIVE;
+    for (int32_t i = 0; i < n; ++i)
+        if ((flags[i] & live) == live) out = std::max(out, static_cast<double>(f32[4][i]));
+    return ASTRO_OK;
 }
 
-// metric_measure (src/sim/accept.cpp:185-206).
-int32_t measure(int32_t metric, const astro_stats& fin, const Aggregates& a, astro_world* w,
+// A check's measured value: from the final stats, from what the run
+// recorded, or from the world's final state.
+int32_t measure(int32_t metric, const astro_stats& fin, const RunRecord& rec, astro_world* w,
                 const astro_scenario& s, double& out) {
     out = 0.0;
+    const double live = static_cast<double>(fin.n_live);
     switch (metric) {
-        case ASTRO_METRIC_AWAKE_FRACTION:
-            out = fin.n_live > 0 ? static_cast<double>(fin.n_awake) / static_cast<double>(fin.n_live) : 0.0;
-            return ASTRO_OK;
-        case ASTRO_METRIC_MEDIUM_TEMP_MEAN: out = fin.mean_temp_medium_k; return ASTRO_OK;
-        case ASTRO_METRIC_MEDIUM_TEMP_MAX: out = a.max_medium_k; return ASTRO_OK;
-        case ASTRO_METRIC_BOIL_EVENT_COUNT: out = static_cast<double>(a.boil_events); return ASTRO_OK;
-        case ASTRO_METRIC_POPULATION_LIVE: out = static_cast<double>(fin.n_live); return ASTRO_OK;
-        case ASTRO_METRIC_POPULATION_DEAD: out = static_cast<double>(fin.n_dead); return ASTRO_OK;
-        case ASTRO_METRIC_MEAN_CHARGE: out = fin.mean_charge; return ASTRO_OK;
-        case ASTRO_METRIC_TOTAL_ENERGY_J: out = fin.total_energy_j; return ASTRO_OK;
-        case ASTRO_METRIC_MEAN_TAU_TOLERANCE: out = fin.mean_tau_tolerance; return ASTRO_OK;
-        case ASTRO_METRIC_RISE_VELOCITY_EMPTY: out = a.rise_velocity_empty; return ASTRO_OK;
-        case ASTRO_METRIC_FALL_VELOCITY_FULL: out = a.fall_velocity_full; return ASTRO_OK;
+        case ASTRO_METRIC_AWAKE_FRACTION: out = live > 0.0 ? fin.n_awake / live : 0.0; break;
+        case ASTRO_METRIC_MEDIUM_TEMP_MEAN: out = fin.mean_temp_medium_k; break;
+        case ASTRO_METRIC_MEDIUM_TEMP_MAX: out = rec.hottest_k(); break;
+        case ASTRO_METRIC_BOIL_EVENT_COUNT: out = static_cast<double>(rec.boil_events()); break;
+        case ASTRO_METRIC_POPULATION_LIVE: out = live; break;
+        case ASTRO_METRIC_POPULATION_DEAD: out = static_cast<double>(fin.n_dead); break;
+        case ASTRO_METRIC_MEAN_CHARGE: out = fin.mean_charge; break;
+        case ASTRO_METRIC_TOTAL_ENERGY_J: out = fin.total_energy_j; break;
+        case ASTRO_METRIC_MEAN_TAU_TOLERANCE: out = fin.mean_tau_tolerance; break;
+        case ASTRO_METRIC_RISE_VELOCITY_EMPTY: out = rec.empty_speed(); break;
+        case ASTRO_METRIC_FALL_VELOCITY_FULL: out = rec.full_speed(); break;
+        case ASTRO_METRIC_MAX_TAU_TOLERANCE: return max_tolerance(w, out);
+        case ASTRO_METRIC_CHARGE_HEIGHT_CORRELATION: return charge_correlation(w, false, s, out);
+        case ASTRO_METRIC_CHARGE_DEPTH_CORRELATION: return charge_correlation(w, true, s, out);
         case ASTRO_METRIC_DOUBLING_TIME_S: {
             double bio = 1.0;
             const int32_t st = astro_world_time(w, nullptr, nullptr, nullptr, &bio);
-            out = doubling_time(a, bio);
+            out = rec.doubling_time(bio);
             return st;
-        }
-        case ASTRO_METRIC_CHARGE_HEIGHT_CORRELATION:
-        case ASTRO_METRIC_CHARGE_DEPTH_CORRELATION: {
-            // charge against y (P1), or against depth along the light (P5)
-            CellSample c;
-            const int32_t st = sample_cells(w, c);
-            if (st != ASTRO_OK) return st;
-            const size_t n = c.x.size();
-            if (n < 2) return ASTRO_OK;
-            std::vector<double> charge(n), other(n);
-            const double lx = s.lights[0].dir_x, ly = s.lights[0].dir_y;
-            for (size_t i = 0; i < n; ++i) {
-                charge[i] = c.energy[i] / ASTRO_CELL_ENERGY_MAX;
-                other[i] = metric == ASTRO_METRIC_CHARGE_HEIGHT_CORRELATION ? c.y[i] : c.x[i] * lx + c.y[i] * ly;
-            }
-            out = correlation(charge, other);
-            return ASTRO_OK;
         }
         case ASTRO_METRIC_IMPULSE_PER_CYCLE: {
-            double imp = 0.0, dis = 0.0;
-            const int32_t st = astro_world_flash_audit(w, &imp, &dis);
-            out = dis > 0.0 ? imp * ASTRO_C_LIGHT / dis : 0.0;
+            // |impulse| c / discharged energy: 1 for a perfect photon rocket
+            double impulse = 0.0, energy = 0.0;
+            const int32_t st = astro_world_flash_audit(w, &impulse, &energy);
+            out = energy > 0.0 ? impulse * ASTRO_C_LIGHT / energy : 0.0;
             return st;
         }
-        case ASTRO_METRIC_MAX_TAU_TOLERANCE: {
-            // over live Taumoeba (src/sim/accept.cpp:63-77)
-            int32_t n = 0, cap = 0;
-            int32_t st = astro_taumoeba_counters(w, &n, &cap, nullptr);
-            if (st != ASTRO_OK || n <= 0) return st;
-            std::vector<uint64_t> id(n), rng(n);
-            std::vector<uint32_t> flags(n), gen(n);
-            std::vector<double> d64[8];
-            std::vector<float> f32[7];
-            std::vector<int32_t> target(n);
-            for (auto& v : d64) v.resize(n);
-            for (auto& v : f32) v.resize(n);
-            astro_tau_soa t{id.data(),     flags.data(),  d64[0].data(), d64[1].data(), d64[2].data(),
-                            d64[3].data(), d64[4].data(), d64[5].data(), f32[0].data(), f32[1].data(),
-                            f32[2].data(), d64[6].data(), d64[7].data(), f32[3].data(), f32[4].data(),
-                            gen.data(),    rng.data(),    f32[5].data(), f32[6].data(), target.data()};
-            int32_t got = 0;
-            if ((st = astro_taumoeba_download(w, &t, n, &got)) != ASTRO_OK) return st;
-            for (int32_t i = 0; i < n; ++i)
-                if ((flags[i] & ASTRO_FLAG_OCCUPIED) && (flags[i] & ASTRO_FLAG_ALIVE))
-                    out = std::max(out, static_cast<double>(f32[4][i]));
-            return ASTRO_OK;
-        }
-        default: return ASTRO_OK;
+        default: break;
     }
+    return ASTRO_OK;
 }
 
 // csv_header / csv_row (tools/headless.cpp:157-173).
@@ -734,8 +774,7 @@ int32_t astro_scenario_run_assert(const astro_scenario* s, astro_world* w, int64
     const int64_t stride = ticks > 2000 ? ticks / 2000 : 1;
     const double settle_s = 2.0, interval_s = 4.0;   // velocity capture windows
 
-    const Needs needs = needs_of(*s);
-    Aggregates agg;
+    RunRecord rec(*s, settle_s, interval_s);
     astro_stats stats{};
     std::FILE* csv = csv_path ? std::fopen(csv_path, "w") : nullptr;
     if (csv_path && !csv) return ASTRO_FILE_IO_ERROR;
@@ -747,7 +786,7 @@ int32_t astro_scenario_run_assert(const astro_scenario* s, astro_world* w, int64
             csv_header(csv, *s, stats.non_canon_run);
             header = true;
         }
-        rc = sample(agg, stats, w, needs, settle_s, interval_s, s->chamber_h);
+        rc = rec.observe(stats, w);
         if (rc == ASTRO_OK && csv) csv_row(csv, stats);
         ++sum.samples;
         return rc;
@@ -797,7 +836,7 @@ int32_t astro_scenario_run_assert(const astro_scenario* s, astro_world* w, int64
     for (int i = 0; i < s->accept_count; ++i) {
         const astro_accept_check& c = s->accept[i];
         double m = 0.0;
-        if ((st = measure(c.metric, stats, agg, w, *s, m)) != ASTRO_OK) return st;
+        if ((st = measure(c.metric, stats, rec, w, *s, m)) != ASTRO_OK) return st;
         const int32_t pass = astro_accept_eval(&c, m);
         sum.passed += pass;
         sum.all_pass &= pass;
