@@ -1,6 +1,20 @@
 """Algorithmic bytes per kernel launch, for the HBM-roofline figures.
 
-"Algorithmic" means compulsory traffic: every per-cell or per-voxel array a
+Two counts live here.
+
+`survey_kernel_bytes` is the basis of every reported roofline fraction: the
+per-stage compulsory traffic of the plain step, frozen in STAGE_BYTES (bytes
+per cell, per awake or live cell, per bucket, per grid point), shared out over
+the kernels that implement each stage by KERNEL_STAGES. The table is fixed:
+it does not follow this build's data layout, so a layout that moves more
+bytes than the stage needs shows up as a lower fraction, and one that skips
+traffic the reference's design pays for gets it credited as the reference
+would need it.
+
+`kernel_bytes` is the builder's own, finer count of what each kernel of this
+layout must touch; it is printed beside the basis, never in its place.
+
+For the builder's count, "algorithmic" means compulsory traffic: every per-cell or per-voxel array a
 kernel must touch is counted once per read and once per write, at its stored
 width. Gathers from a field grid count one element; neighbour records and
 bucket starts reached by the contact and shadow walks count zero (they are
@@ -82,3 +96,93 @@ def kernel_bytes(name: str, c: dict) -> float:
         table[fused] = table["sample_base"] + table[part] - 4 * n
     return float(table.get(name, 0.0))
 
+
+
+# ---------------------------------------------------------------------------
+# The frozen basis: compulsory bytes per unit of each stage of the plain step
+# (hot FP32 cell layout, FP32 fields). Units: "cell" = every slot of the
+# population, "awake" / "alive" = those cells only, "bucket" = hash bucket,
+# "point" = grid point of the field named, per launch of the stage.
+# ---------------------------------------------------------------------------
+STAGE_BYTES = {
+    "hash_keys": (24, "cell"),
+    "sort_pass": (16, "cell"),          # per radix pass; passes = ceil(key bits / 11)
+    "sort_tail": (4, "cell"),
+    "ranges": (4, "cell"),
+    "buckets": (8, "bucket"),           # clear + range write
+    "field_sample": (24, "cell"),
+    "forces": (52, "cell"),
+    "integrate_pcg": (108, "cell"),
+    "integrate_philox": (92, "cell"),
+    "thermal": (28, "cell"),            # per temperature substep
+    "light": (44, "cell"),              # lit worlds
+    "taxis": (84, "awake"),
+    "lifecycle": (72, "cell"),
+    "compaction": (12 + 2 * 93, "cell"),   # ticks that compact
+    "ftcs": (8, "point"),               # per substep, per field
+    "fold": (8, "point"),               # deposit fold: per T substep, per tick for the nutrient
+    "irradiance": (20, "point"),        # per tick when lit
+}
+
+# Kernel -> [(stage, share of that stage's bytes, grid the points count on)].
+# Shares of one stage over its kernels add to 1.
+KERNEL_STAGES = {
+    "hash_key": [("hash_keys", 1.0, None)],
+    "hash_bucket_scan": [("buckets", 1.0, None)],
+    "hash_scatter": [("sort_pass", 1.0, None), ("sort_tail", 0.5, None)],
+    "hash_rank": [("sort_pass", 1.0, None), ("sort_tail", 0.5, None), ("ranges", 1.0, None)],
+    "thermal": [("thermal", 1.0, None)],
+    "deposit_apply": [("fold", 1.0, 0)],
+    "ftcs_T": [("ftcs", 1.0, 0)],
+    "ftcs_CO2": [("ftcs", 1.0, 1)],
+    "ftcs_N2": [("ftcs", 1.0, 2)],
+    "sample_base": [("field_sample", 1.0, None)],
+    "taxis_sample": [("field_sample", 1.0, None), ("taxis", 1.0, None)],
+    "feed_sample": [("field_sample", 1.0, None), ("light", 0.25, None)],
+    "feed_taxis_sample": [("field_sample", 1.0, None), ("light", 0.25, None), ("taxis", 1.0, None)],
+    "feed": [("light", 0.25, None)],
+    "taxis": [("taxis", 1.0, None)],
+    "feed_taxis": [("light", 0.25, None), ("taxis", 1.0, None)],
+    "contact_walk": [("forces", 1.0, None)],
+    "integrate": [("integrate", 1.0, None)],
+    "integrate_demand": [("integrate", 1.0, None), ("lifecycle", 0.25, None)],
+    "uptake_mark": [("lifecycle", 0.5, None), ("fold", 1.0, 1)],
+    "life_scan": [("lifecycle", 0.1, None)],
+    "life_apply": [("lifecycle", 0.15, None), ("compaction", 1.0, None)],
+    "occlusion_stamp": [("light", 0.25, None)],
+    "light_sweep": [("irradiance", 1.0, 3)],
+    "feed_walk": [("light", 0.25, None)],
+}
+
+
+def radix_passes(buckets: int) -> int:
+    # keys run 0 .. buckets (the free-slot sentinel): that many bits, 11 a pass
+    bits = max(1, int(buckets).bit_length())
+    return -(-bits // 11)
+
+
+def survey_kernel_bytes(name: str, c: dict) -> float:
+    """Bytes per launch of kernel `name` on the frozen basis. `c` holds n,
+    awake, alive, buckets, pts{field: points}, compacting (fraction of
+    ticks), philox (bool) and, for lifecycle kernels, nothing else."""
+    total = 0.0
+    for stage, share, field in KERNEL_STAGES.get(name, []):
+        if stage == "integrate":
+            stage = "integrate_philox" if c.get("philox") else "integrate_pcg"
+        per, unit = STAGE_BYTES[stage]
+        if unit == "cell":
+            units = c["n"]
+        elif unit == "awake":
+            units = c.get("awake", 0)
+        elif unit == "alive":
+            units = c.get("alive", c["n"])
+        elif unit == "bucket":
+            units = c["buckets"]
+        else:
+            units = c["pts"].get(field, 0)
+        if stage == "sort_pass":
+            units *= radix_passes(c["buckets"]) / 2.0   # the two sort kernels share the passes
+        if stage == "compaction":
+            units *= c.get("compacting", 0.0)
+        total += share * per * units
+    return float(total)

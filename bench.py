@@ -19,8 +19,16 @@ tick summed on the device), with all state resident in HBM; the ticks are
 replayed from a captured hipGraph. Afterwards a shorter pass re-runs the same
 ticks eagerly with HIP events around every kernel on the world's stream to
 give per-kernel times and achieved bandwidth against algorithmic bytes.
-The CPU baseline is the CPU restatement of the same step (FP64, one thread)
-on the same workload for a few ticks.
+Each kernel's achieved bandwidth is taken on the frozen per-stage byte basis
+(roofline.STAGE_BYTES); `roofline` is the kernel that takes the most time per
+tick, `step_roofline` the whole tick. The CPU baseline is the CPU restatement
+of the same step (FP64, the reference's stage order) on the same workload for
+a bounded number of ticks, single-threaded and on every core with OpenMP.
+Per-cell streams are Philox4x32-10 by default (--rng pcg32 for the
+reference's generator).
+
+`configs` holds the same measurements for the small (2,000 cells, 64^3) and
+real-shape (25,000 awake lit cells, 256 x 256 x 64) plain workloads.
 
 The `ensemble` object measures the batched-ensemble workload: 256 independent
 chambers (1 x 1 x 0.2 mm) with cell counts log-uniform in [500, 50,000] drawn
@@ -55,15 +63,17 @@ import numpy as np  # noqa: E402
 import astro_hot as ah  # noqa: E402
 import roofline  # noqa: E402
 
+C_INT = ah.C.c_int32
+
 METRIC = "cell-steps/s at 1M cells; field steps/s surrogate vs FTCS at 1e-3 rel err"
 HBM_PEAK_GBS = 8000.0
 QUOTA = 2.1e-14
 
 
-def workload(binding: ah.Binding, cells: int, precision: int):
+def workload(binding: ah.Binding, cells: int, precision: int, rng_kind: int = 0):
     return binding.desc(
         capacity=int(cells * 1.25), seed=1, chamber_w=4e-3, chamber_h=4e-3, chamber_d=2e-4,
-        precision=precision, hash_cell_size=1e-5,
+        precision=precision, hash_cell_size=1e-5, rng_kind=rng_kind,
         grid={ah.TEMPERATURE: (512, 512, 32), ah.CO2: (512, 512, 32), ah.N2: (-1, 0, 0),
               ah.IRRADIANCE: (64, 64, 1)},
         physics_rate=0.1, biology_rate=1e5, co2_init=1.0, division_offset=0.5e-6, compaction_enabled=1,
@@ -137,23 +147,117 @@ def population(w: ah.World) -> dict:
             "dead": s["n_dead"], "divisions": s["divisions_this_window"], "deaths": s["deaths_this_window"]}
 
 
-def cpu_baseline(cells: int, steps: int) -> dict:
-    oracle_lib = ROOT / "oracle" / "liboracle.so"
-    if not oracle_lib.exists():
-        subprocess.run(["make", "-C", str(ROOT / "oracle"), "oracle"], check=True, stdout=subprocess.DEVNULL)
-    ob = ah.Binding(oracle_lib, "oracle_")
-    with ah.World(ob, workload(ob, cells, ah.FP64_CONTRACT)) as w:
-        populate(w, cells)
-        w.step(1)
-        c0 = w.counters().cell_steps
-        t0 = time.perf_counter()
-        w.step(steps)
-        dt = time.perf_counter() - t0
-        done = w.counters().cell_steps - c0
-    return {"value": done / dt, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} ticks of the same {cells}-cell workload after 1 warm-up tick, "
-                      f"FP64 CPU restatement, single thread, {dt:.1f} s",
-            "seconds": dt}
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def oracle_binding(all_cores: bool) -> ah.Binding:
+    name = "liboracle_omp.so" if all_cores else "liboracle.so"
+    lib = ROOT / "oracle" / name
+    if not lib.exists():
+        subprocess.run(["make", "-C", str(ROOT / "oracle"), "omp" if all_cores else "oracle"], check=True,
+                       stdout=subprocess.DEVNULL)
+    return ah.Binding(lib, "oracle_", {"threads": (C_INT, [])})
+
+
+def cpu_baseline(make, fill, serial_ticks: int, omp_ticks: int, what: str) -> dict:
+    """The oracle (FP64, the reference's stage order) on the same workload,
+    timed on this host: single-threaded, and built with OpenMP on every core
+    it is given (bit-identical results). One warm-up tick each, then a bounded
+    sample of ticks; value = the all-cores rate."""
+    out = {}
+    for all_cores, ticks in ((False, serial_ticks), (True, omp_ticks)):
+        b = oracle_binding(all_cores)
+        with make(b) as w:
+            fill(w)
+            w.step(1)
+            c0 = w.counters().cell_steps
+            t0 = time.perf_counter()
+            w.step(ticks)
+            dt = time.perf_counter() - t0
+            done = w.counters().cell_steps - c0
+        out[all_cores] = (done / dt, int(b.threads()), ticks, dt)
+    rate, threads, ticks, dt = out[True]
+    rate1, _, ticks1, dt1 = out[False]
+    return {"value": rate, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{what}: {ticks} ticks after 1 warm-up tick on {threads} threads ({dt:.1f} s), "
+                      f"FP64 CPU restatement built with OpenMP",
+            "single_thread": {"value": rate1, "cores": 1, "sample": f"{ticks1} ticks ({dt1:.1f} s)"},
+            "cpu_model": cpu_model(), "hardware_threads": os.cpu_count()}
+
+
+def world_counts(w: ah.World, before: dict, after: dict, ticks: int, precision: int, philox: bool) -> dict:
+    """Population and grid figures of a measured interval, for the byte tables."""
+    nx, ny, nz, _ = w.hash_dims()
+    pts = {}
+    for f in (ah.TEMPERATURE, ah.CO2, ah.N2, ah.IRRADIANCE):
+        (gx, gy, gz), _ = w.field_dims(f)
+        if gx > 0:
+            pts[f] = gx * gy * gz
+    return {"n": 0.5 * (before["n"] + after["n"]), "alive": 0.5 * (before["alive"] + after["alive"]),
+            "awake": 0.5 * (before["awake"] + after["awake"]), "dead": after["deaths"] / max(ticks, 1),
+            "births": after["divisions"] / max(ticks, 1), "compacting": 1.0 if after["deaths"] > 0 else 0.0,
+            "p": 4 if precision == ah.FP32_HOT else 8, "buckets": nx * ny * nz, "pts": pts,
+            "sampled_fields": 2, "resort_frac": 1.0 / 16.0, "philox": philox}
+
+
+def measure_world(w: ah.World, steps: int, warmup: int, profile_steps: int, precision: int, philox: bool,
+                  traffic: dict | None = None) -> dict:
+    """Graph-replayed ticks timed end to end, then the same ticks eager with
+    HIP events around every kernel on the world's stream. Each kernel's
+    achieved bandwidth is its bytes on the frozen survey basis
+    (roofline.survey_kernel_bytes) over its average launch time; the
+    builder's own count is printed beside it."""
+    w.step(warmup)
+    w.sync()
+    c0 = w.counters()
+    t0 = time.perf_counter()
+    w.step(steps)
+    w.sync()
+    elapsed = time.perf_counter() - t0
+    c1 = w.counters()
+    value = (c1.cell_steps - c0.cell_steps) / elapsed
+
+    before = population(w)
+    w.profile(True)
+    w.profile_reset()
+    w.step(profile_steps)
+    kern = w.profile_read()
+    w.profile(False)
+    after = population(w)
+    counts = world_counts(w, before, after, profile_steps, precision, philox)
+    stages, total_ms = {}, 0.0
+    for name, (launches, ms) in kern.items():
+        total_ms += ms
+        per = ms / max(launches, 1)
+        basis = roofline.survey_kernel_bytes(name, counts)
+        own = roofline.kernel_bytes(name, counts)
+        stages[name] = {"launches_per_tick": launches / profile_steps, "avg_us": round(per * 1e3, 2),
+                        "bytes_per_launch": basis,
+                        "achieved_GBs": round(basis / (per * 1e-3) / 1e9, 1) if per > 0 else None,
+                        "builder_bytes_per_launch": own,
+                        "builder_GBs": round(own / (per * 1e-3) / 1e9, 1) if per > 0 else None}
+    dom = max(kern.items(), key=lambda kv: kv[1][1])[0]
+    d = stages[dom]
+    roof = {"kernel": dom, "bound": "hbm", "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(d["achieved_GBs"] / HBM_PEAK_GBS, 4) if d["achieved_GBs"] else None,
+            "traffic": (traffic or {}).get(dom), "avg_launch_us": d["avg_us"],
+            "algo_bytes_per_launch": d["bytes_per_launch"], "basis": "frozen per-stage bytes (roofline.STAGE_BYTES)",
+            "builder_achieved": d["builder_GBs"], "builder_bytes_per_launch": d["builder_bytes_per_launch"]}
+    per_tick = sum(roofline.survey_kernel_bytes(k, counts) * v[0] for k, v in kern.items()) / profile_steps
+    own_tick = sum(roofline.kernel_bytes(k, counts) * v[0] for k, v in kern.items()) / profile_steps
+    ms_step = elapsed / steps * 1e3
+    step_roof = {"bytes_per_tick": per_tick, "achieved_GBs": round(per_tick / (ms_step * 1e-3) / 1e9, 1),
+                 "frac": round(per_tick / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                 "builder_bytes_per_tick": own_tick, "eager_ms_per_tick": round(total_ms / profile_steps, 4)}
+    return {"value": value, "ms_per_step": ms_step, "steps": steps, "warmup": warmup, "c0": c0, "c1": c1,
+            "roofline": roof, "step_roofline": step_roof, "stages": stages}
 
 
 ENSEMBLE_GRID = {ah.TEMPERATURE: (64, 64, 64), ah.CO2: (16, 16, 4), ah.N2: (-1, 0, 0), ah.IRRADIANCE: (64, 64, 64)}
@@ -179,8 +283,29 @@ def total_cell_steps(worlds: list[ah.World]) -> int:
     return sum(w.counters().cell_steps for w in worlds)
 
 
+def ensemble_cpu_baseline(chambers: int, sample: int, ticks: int) -> dict:
+    """The oracle on every core, stepping the first `sample` chambers of the
+    ensemble one after another for `ticks` ticks each."""
+    b = oracle_binding(True)
+    worlds = ensemble_chambers(b, chambers)
+    done, secs = 0, 0.0
+    for w in worlds[:sample]:
+        w.step(1)
+        c0 = w.counters().cell_steps
+        t0 = time.perf_counter()
+        w.step(ticks)
+        secs += time.perf_counter() - t0
+        done += w.counters().cell_steps - c0
+    for w in worlds:
+        w.close()
+    return {"value": done / secs, "unit": "cell-steps/s", "cores": int(b.threads()), "kind": "port",
+            "sample": f"the first {sample} of the {chambers} chambers, {ticks} ticks each after 1 warm-up tick "
+                      f"({secs:.1f} s), FP64 CPU restatement built with OpenMP",
+            "cpu_model": cpu_model(), "hardware_threads": os.cpu_count()}
+
+
 def bench_ensemble(binding: ah.Binding, chambers: int, steps: int, warmup: int, profile_steps: int,
-                   seq_steps: int, sequential: bool = True) -> dict:
+                   seq_steps: int, sequential: bool = True, cpu: bool = True) -> dict:
     worlds = ensemble_chambers(binding, chambers)
     cells = [w.counters().count for w in worlds]
     e = ah.Ensemble(binding, worlds)
@@ -224,7 +349,7 @@ def bench_ensemble(binding: ah.Binding, chambers: int, steps: int, warmup: int, 
         per_chamber.append({"n": st["n_live"] + st["n_dead"], "alive": st["n_live"], "awake": st["n_awake"],
                             "dead": 0, "births": 0, "compacting": 0.0, "p": 4, "buckets": int(np.prod(w.hash_dims()[:3])),
                             "pts": {0: 64 ** 3, 1: 16 * 16 * 4, 3: 64 ** 3}, "sampled_fields": 2,
-                            "resort_frac": 1.0 / 16.0})
+                            "resort_frac": 1.0 / 16.0, "philox": False})
     # Per tick: a stage may be several batched launches (substeps, or one
     # launch per kernel variant when chambers differ, as the two sweep
     # directions do); its algorithmic bytes are every chamber's, times the
@@ -233,11 +358,13 @@ def bench_ensemble(binding: ah.Binding, chambers: int, steps: int, warmup: int, 
     stages = {}
     for name, (launches, ms) in kern.items():
         mult = launches / profile_steps if name in substepped else 1.0
-        algo = mult * sum(roofline.kernel_bytes(name, c) for c in per_chamber)
+        algo = mult * sum(roofline.survey_kernel_bytes(name, c) for c in per_chamber)
+        own = mult * sum(roofline.kernel_bytes(name, c) for c in per_chamber)
         per_tick = ms / profile_steps
         stages[name] = {"launches_per_tick": launches / profile_steps, "us_per_tick": round(per_tick * 1e3, 2),
                         "algo_bytes_per_tick": algo,
-                        "achieved_GBs": round(algo / (per_tick * 1e-3) / 1e9, 1) if per_tick > 0 else None}
+                        "achieved_GBs": round(algo / (per_tick * 1e-3) / 1e9, 1) if per_tick > 0 else None,
+                        "builder_bytes_per_tick": own}
     algo_step = sum(v["algo_bytes_per_tick"] for v in stages.values())
     dom = max(kern.items(), key=lambda kv: kv[1][1])[0]
     value = done / elapsed
@@ -259,6 +386,7 @@ def bench_ensemble(binding: ah.Binding, chambers: int, steps: int, warmup: int, 
                           "frac_graph": round(algo_step / (elapsed / steps) / 1e9 / HBM_PEAK_GBS, 4),
                           "dominant_stage": dom},
         "stages": stages,
+        "cpu_baseline": ensemble_cpu_baseline(chambers, 8, 2) if cpu else None,
     }
     e.close()
     for w in worlds:
@@ -353,6 +481,27 @@ def bench_surrogate(binding: ah.Binding, trajectories: int, epochs: int, calls: 
     return out
 
 
+RNG = {"philox": ah.RNG_PHILOX, "pcg32": ah.RNG_PCG32}
+
+
+def bench_plain(gpu: ah.Binding, name: str, make, fill, precision: int, rng: str, steps: int, warmup: int,
+                profile_steps: int, cpu: tuple | None, traffic: dict | None = None) -> dict:
+    """One plain-method configuration: GPU throughput and rooflines, and the
+    CPU baseline on a bounded sample of the same workload."""
+    w = make(gpu)
+    fill(w)
+    m = measure_world(w, steps, warmup, profile_steps, precision, rng == "philox", traffic)
+    w.close()
+    c0, c1 = m.pop("c0"), m.pop("c1")
+    m["divisions"] = c1.divisions_total - c0.divisions_total
+    m["deaths"] = c1.deaths_total - c0.deaths_total
+    m["cells_end"] = c1.count
+    if cpu is not None:
+        serial_ticks, omp_ticks = cpu
+        m["cpu_baseline"] = cpu_baseline(make, fill, serial_ticks, omp_ticks, name)
+    return m
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -360,9 +509,13 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cells", type=int, default=1_000_000)
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--rng", choices=sorted(RNG), default="philox",
+                    help="per-cell streams: counter-based Philox4x32-10 (default) or the reference's PCG32")
     ap.add_argument("--profile-steps", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-ticks", type=int, nargs=2, default=[4, 12], metavar=("SERIAL", "ALL_CORES"),
+                    help="ticks of the 1M-cell CPU baseline, single-threaded and on all cores")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the small and real-shape lines")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-only", action="store_true")
     ap.add_argument("--ensemble-chambers", type=int, default=256)
@@ -381,95 +534,66 @@ def main() -> int:
     gpu = ah.load()
     if a.ensemble_only:
         print(json.dumps(bench_ensemble(gpu, a.ensemble_chambers, a.ensemble_steps, 10, 10, 10,
-                                        sequential=not a.no_sequential)))
+                                        sequential=not a.no_sequential, cpu=not a.no_cpu)))
         return 0
     if a.surrogate_only:
         print(json.dumps(bench_surrogate(gpu, a.surrogate_trajectories, 1, 50)))
         return 0
     precision = ah.FP32_HOT if a.precision == "fp32" else ah.FP64_CONTRACT
-    w = ah.World(gpu, workload(gpu, a.cells, precision))
-    populate(w, a.cells)
-    w.step(a.warmup)
-    w.sync()
-
-    c0 = w.counters()
-    t0 = time.perf_counter()
-    w.step(a.steps)
-    w.sync()
-    elapsed = time.perf_counter() - t0
-    c1 = w.counters()
-    cell_steps = c1.cell_steps - c0.cell_steps
-    value = cell_steps / elapsed
-
-    # Per-kernel pass: the same ticks, eager, HIP events on the world's stream.
-    before = population(w)
-    w.profile(True)
-    w.profile_reset()
-    w.step(a.profile_steps)
-    kern = w.profile_read()
-    w.profile(False)
-    after = population(w)
-    ticks = a.profile_steps
-    pbytes = 4 if precision == ah.FP32_HOT else 8
-    counts = {
-        "n": 0.5 * (before["n"] + after["n"]), "alive": 0.5 * (before["alive"] + after["alive"]),
-        "awake": 0.5 * (before["awake"] + after["awake"]), "dead": after["deaths"] / max(ticks, 1),
-        "births": after["divisions"] / max(ticks, 1), "compacting": 1.0 if after["deaths"] > 0 else 0.0,
-        "p": pbytes, "buckets": 400 * 400 * 20, "pts": {0: 512 * 512 * 32, 1: 512 * 512 * 32},
-        "sampled_fields": 2, "resort_frac": 1.0 / 16.0,
-    }
-    stages = {}
-    total_ms = 0.0
-    for name, (launches, ms) in kern.items():
-        total_ms += ms
-        per = ms / max(launches, 1)
-        algo = roofline.kernel_bytes(name, counts)
-        stages[name] = {"launches": launches, "avg_us": round(per * 1e3, 2),
-                        "algo_bytes_per_launch": algo,
-                        "achieved_GBs": round(algo / (per * 1e-3) / 1e9, 1) if per > 0 else None}
-    dom = max(kern.items(), key=lambda kv: kv[1][1])[0]
-    d = stages[dom]
+    rng = RNG[a.rng]
     traffic = None
     tfile = ROOT / "profiles" / "traffic_c3.json"
     if tfile.exists():
-        traffic = json.loads(tfile.read_text()).get(dom)
-    roof = {"kernel": dom, "bound": "hbm", "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(d["achieved_GBs"] / HBM_PEAK_GBS, 4) if d["achieved_GBs"] else None,
-            "traffic": traffic, "avg_launch_us": d["avg_us"], "algo_bytes_per_launch": d["algo_bytes_per_launch"]}
-    algo_step = sum(roofline.kernel_bytes(k, counts) * v[0] for k, v in kern.items()) / max(ticks, 1)
-    step_roof = {"algo_bytes_per_tick": algo_step,
-                 "achieved_GBs_eager": round(algo_step / (total_ms / ticks * 1e-3) / 1e9, 1),
-                 "achieved_GBs_graph": round(algo_step / (elapsed / a.steps) / 1e9, 1)}
-    step_roof["frac_graph"] = round(step_roof["achieved_GBs_graph"] / HBM_PEAK_GBS, 4)
+        traffic = json.loads(tfile.read_text())
 
-    w.close()
+    # the 1M-cell scale workload: the headline line
+    big = bench_plain(gpu, f"{a.cells}-cell scale workload", lambda b: ah.World(b, workload(b, a.cells, precision, rng)),
+                      lambda w: populate(w, a.cells), precision, a.rng, a.steps, a.warmup, a.profile_steps,
+                      None if a.no_cpu else tuple(a.cpu_ticks), traffic)
+    print(json.dumps({"progress": "scale workload", "value": big["value"], "ms_per_step": big["ms_per_step"]}),
+          file=sys.stderr, flush=True)
+
+    configs = {}
+    if not a.no_configs:
+        configs["small"] = bench_plain(
+            gpu, "2,000-cell small workload", lambda b: ah.World(b, workload_c1(b, precision, rng)), populate_c1,
+            precision, a.rng, 1000, 20, 20, None if a.no_cpu else (100, 300))
+        configs["small"]["workload"] = ("plain method, small: 2,000 cells in 550x550x100 um, 64^3 FP32 T at 369.0 K, "
+                                        "no light, dt 1e-4 s, 1,000 steps, seed 1")
+        configs["real_shape"] = bench_plain(
+            gpu, "25,000-cell real-shape workload", lambda b: ah.World(b, workload_c2(b, precision, rng)), populate_c2,
+            precision, a.rng, 2000, 20, 10, None if a.no_cpu else (2, 6))
+        configs["real_shape"]["workload"] = (
+            "plain method, real shape: 25,000 awake cells in 550x550x200 um, log-uniform charge [1e-3, 0.2], "
+            "256x256x64 FP32 T + nutrient with a 5 K sigma-40-um hot spot, light along +x at 1e4 W/m^2, "
+            "dt 1e-4 s, 2,000 steps")
+        for k, v in configs.items():
+            print(json.dumps({"progress": k, "value": v["value"], "ms_per_step": v["ms_per_step"]}), file=sys.stderr,
+                  flush=True)
+
     ens = None
     if not a.no_ensemble:
-        ens = bench_ensemble(gpu, a.ensemble_chambers, a.ensemble_steps, 10, 10, 10)
+        ens = bench_ensemble(gpu, a.ensemble_chambers, a.ensemble_steps, 10, 10, 10, cpu=not a.no_cpu)
+        print(json.dumps({"progress": "ensemble", "value": ens["value"]}), file=sys.stderr, flush=True)
 
     sur = None
     if not a.no_surrogate:
         sur = bench_surrogate(gpu, a.surrogate_trajectories, 1, 50)
 
-    cpu = None
-    if not a.no_cpu:
-        cpu = cpu_baseline(a.cells, a.cpu_steps)
-        cpu.pop("seconds", None)
-
     out = {
-        "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": elapsed / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "metric": METRIC, "value": big["value"], "unit": "cell-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": big["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32" if precision == ah.FP32_HOT else "f64", "data": "synthetic",
         "config": {"workload": "plain method, scale: 1M cells, 4x4x0.2 mm, 512x512x32 FP32 T+CO2, 10 um hash, "
                                "division + compaction on, dt 1e-4 s",
-                   "cells": a.cells, "cells_end": c1.count, "storage": a.precision,
-                   "arithmetic": "fp32 integrator transcendentals, fp64 displacement sums and energies, fp32 fields", "graph_replay": True,
-                   "divisions": c1.divisions_total - c0.divisions_total,
-                   "deaths": c1.deaths_total - c0.deaths_total},
-        "roofline": roof,
-        "step_roofline": step_roof,
-        "cpu_baseline": cpu,
-        "stages": stages,
+                   "cells": a.cells, "cells_end": big["cells_end"], "storage": a.precision, "rng": a.rng,
+                   "arithmetic": "fp32 integrator transcendentals, fp64 displacement sums and energies, fp32 fields",
+                   "graph_replay": True, "divisions": big["divisions"], "deaths": big["deaths"]},
+        "roofline": big["roofline"],
+        "step_roofline": big["step_roofline"],
+        "cpu_baseline": big.get("cpu_baseline"),
+        "stages": big["stages"],
+        "configs": configs,
         "ensemble": ens,
         "surrogate": sur,
     }

@@ -23,6 +23,31 @@
 #include <string>
 #include <vector>
 
+// The all-cores CPU baseline builds this file with -fopenmp
+// (liboracle_omp.so). Loops over cells or voxels that write only their own
+// element, or add into int64 fixed-point accumulators (integer sums, so the
+// order does not matter), then run on every core; all else stays sequential,
+// and the results equal the single-threaded build's bit for bit. Without
+// OpenMP the macros are plain loops and adds.
+#ifdef _OPENMP
+#include <omp.h>
+#include <parallel/algorithm>
+#define PAR_FOR _Pragma("omp parallel for schedule(static)")
+#define PAR_FOR2 _Pragma("omp parallel for collapse(2) schedule(static)")
+#define FIXED_ADD(dst, v)                 \
+    do {                                  \
+        const uint64_t fixed_add_v = (v); \
+        _Pragma("omp atomic")             \
+        (dst) += fixed_add_v;             \
+    } while (0)
+#define STABLE_SORT __gnu_parallel::stable_sort
+#else
+#define PAR_FOR
+#define PAR_FOR2
+#define FIXED_ADD(dst, v) ((dst) += (v))
+#define STABLE_SORT std::stable_sort
+#endif
+
 namespace {
 
 // ---------------------------------------------------------------------------
@@ -455,7 +480,9 @@ size_t grid_nearest(const Grid& g, float x, float y, float z) {
 // flush_kernel (src/fields/diffuse.cu:51-60).
 void grid_flush(Grid& g) {
     if (!g.enabled) return;
-    for (size_t i = 0; i < g.count(); ++i) {
+    const size_t count = g.count();
+    PAR_FOR
+    for (size_t i = 0; i < count; ++i) {
         const uint64_t raw = g.deposit[i];
         if (raw != 0ull) {
             g.value[i] += static_cast<float>(from_fixed(raw, g.scale));
@@ -475,6 +502,7 @@ bool grid_substep(Grid& g, double dt_sub) {
         if (!(coeff <= 0.25f)) return false;
         const float robin = static_cast<float>(g.robin_x);
         const int n = nx;
+        PAR_FOR
         for (int j = 0; j < n; ++j) {
             for (int i = 0; i < n; ++i) {
                 const float c = g.value[j * n + i];
@@ -498,6 +526,7 @@ bool grid_substep(Grid& g, double dt_sub) {
         const float rx = static_cast<float>(g.robin_x);
         const float ry = static_cast<float>(g.robin_y);
         const float rz = static_cast<float>(g.robin_z);
+        PAR_FOR2
         for (int k = 0; k < nz; ++k) {
             for (int j = 0; j < ny; ++j) {
                 for (int i = 0; i < nx; ++i) {
@@ -665,6 +694,7 @@ void hash_build(oracle_world* w) {
     w->hash_valid = true;
     if (n <= 0) return;
     std::vector<uint32_t> keys(n);
+    PAR_FOR
     for (int32_t i = 0; i < n; ++i) {
         if (w->flags[i] & ASTRO_FLAG_OCCUPIED) {
             int32_t gx, gy, gz;
@@ -676,7 +706,7 @@ void hash_build(oracle_world* w) {
     }
     std::vector<uint32_t> order(n);
     std::iota(order.begin(), order.end(), 0u);
-    std::stable_sort(order.begin(), order.end(),
+    STABLE_SORT(order.begin(), order.end(),
                      [&](uint32_t a, uint32_t b) { return keys[a] < keys[b]; });
     for (int32_t k = 0; k < n; ++k) w->vals_sorted[k] = order[k];
     for (int32_t k = 0; k < n; ++k) {                          // range_kernel, hash.cu:29-38
@@ -701,6 +731,7 @@ int thermal_step(oracle_world* w, double dt) {
     const double g = ASTRO_CONDUCTION_COEFF;
     for (int s = 0; s < substeps; ++s) {
         if (n > 0 && w->desc.thermal_enabled) {
+            PAR_FOR
             for (int32_t i = 0; i < n; ++i) {
                 const uint32_t fl = w->flags[i];
                 if (!(fl & ASTRO_FLAG_OCCUPIED)) continue;
@@ -726,7 +757,7 @@ int thermal_step(oracle_world* w, double dt) {
                     energy = ASTRO_CELL_ENERGY_MAX;
                 }
                 w->energy[i] = energy;
-                T.deposit[idx] += static_cast<uint64_t>(to_fixed(moved / cap, T.scale));
+                FIXED_ADD(T.deposit[idx], static_cast<uint64_t>(to_fixed(moved / cap, T.scale)));
                 if (energy <= 0.0) {
                     w->flags[i] = w->flags[i] & ~ASTRO_FLAG_ALIVE;
                     w->death_cause[i] = ASTRO_DEATH_STARVED;
@@ -751,6 +782,7 @@ int emission_step(oracle_world* w, double dt) {
     std::fill(I.deposit.begin(), I.deposit.end(), 0ull);       // clear_kernel
     const int32_t n = w->count;
     if (n > 0 && source > 0.0f) {                                // occlusion_stamp_kernel
+        PAR_FOR
         for (int32_t i = 0; i < n; ++i) {
             const uint32_t fl = w->flags[i];
             if (!(fl & ASTRO_FLAG_OCCUPIED)) continue;
@@ -758,7 +790,7 @@ int emission_step(oracle_world* w, double dt) {
                                 : ASTRO_CELL_CROSS_SECTION * (1.0 - ASTRO_CELL_ALBEDO_DEAD);
             const size_t idx = grid_nearest(I, static_cast<float>(w->x[i]), static_cast<float>(w->y[i]),
                                             static_cast<float>(w->z[i]));
-            I.deposit[idx] += static_cast<uint64_t>(to_fixed(area, I.scale));
+            FIXED_ADD(I.deposit[idx], static_cast<uint64_t>(to_fixed(area, I.scale)));
         }
     }
     const int axis = (std::fabs(dir.x) >= std::fabs(dir.y)) ? 0 : 1;
@@ -767,6 +799,7 @@ int emission_step(oracle_world* w, double dt) {
     // sweep_kernel (emission.cu:37-53): one sequential march per line.
     const int len = axis == 0 ? I.nx : I.ny;
     const int other = axis == 0 ? I.ny : I.nx;
+    PAR_FOR2
     for (int k = 0; k < I.nz; ++k) {
         for (int line = 0; line < other; ++line) {
             double t = 1.0;
@@ -782,6 +815,7 @@ int emission_step(oracle_world* w, double dt) {
     // feed_kernel (emission.cu:81-115)
     if (n > 0) {
         const V3 ld = dir;
+        PAR_FOR
         for (int32_t i = 0; i < n; ++i) {
             const uint32_t fl = w->flags[i];
             if (!(fl & ASTRO_FLAG_OCCUPIED) || !(fl & ASTRO_FLAG_ALIVE)) continue;
@@ -810,6 +844,7 @@ int emission_step(oracle_world* w, double dt) {
 // taxis_kernel (src/sim/taxis.cu:13-87).
 void taxis_step(oracle_world* w, double dt) {
     const astro_world_desc& d = w->desc;
+    PAR_FOR
     for (int32_t i = 0; i < w->count; ++i) {
         const uint32_t fl = w->flags[i];
         if (!(fl & ASTRO_FLAG_OCCUPIED) || !(fl & ASTRO_FLAG_ALIVE)) continue;
@@ -854,6 +889,7 @@ void taxis_step(oracle_world* w, double dt) {
 
 // field_sample_kernel (src/sim/integrator.cu:20-30).
 void field_sample(oracle_world* w) {
+    PAR_FOR
     for (int32_t i = 0; i < w->count; ++i) {
         if (!(w->flags[i] & ASTRO_FLAG_OCCUPIED)) continue;
         const float x = static_cast<float>(w->x[i]);
@@ -868,6 +904,7 @@ void field_sample(oracle_world* w) {
 // forces_kernel (src/sim/integrator.cu:40-76).
 void forces(oracle_world* w) {
     const double k = ASTRO_CONTACT_STIFFNESS / w->physics_rate;
+    PAR_FOR
     for (int32_t i = 0; i < w->count; ++i) {
         const uint32_t fl = w->flags[i];
         if (!(fl & ASTRO_FLAG_OCCUPIED)) { w->fx[i] = w->fy[i] = w->fz[i] = 0.0; continue; }
@@ -893,6 +930,7 @@ void forces(oracle_world* w) {
 // integrate_kernel (src/sim/integrator.cu:79-150).
 void integrate(oracle_world* w, double dt) {
     const astro_world_desc& d = w->desc;
+    PAR_FOR
     for (int32_t i = 0; i < w->count; ++i) {
         const uint32_t fl = w->flags[i];
         if (!(fl & ASTRO_FLAG_OCCUPIED)) continue;
@@ -1006,14 +1044,16 @@ void lifecycle_step(oracle_world* w, double dt) {
                                static_cast<float>(w->z[i]));
             return co2_uptake_rate(static_cast<double>(C.value[idx])) * dt_bio;
         };
+        PAR_FOR
         for (int32_t i = 0; i < n; ++i) {                                  // demand_kernel
             const uint32_t f = w->flags[i];
             if (!(f & ASTRO_FLAG_OCCUPIED) || !(f & ASTRO_FLAG_ALIVE)) continue;
             size_t idx;
             const double want = want_of(i, idx);
             if (want <= 0.0) continue;
-            w->co2_demand[idx] += static_cast<uint64_t>(to_fixed(want / gv, C.scale));
+            FIXED_ADD(w->co2_demand[idx], static_cast<uint64_t>(to_fixed(want / gv, C.scale)));
         }
+        PAR_FOR
         for (int32_t i = 0; i < n; ++i) {                                  // uptake_kernel
             const uint32_t f = w->flags[i];
             if (!(f & ASTRO_FLAG_OCCUPIED) || !(f & ASTRO_FLAG_ALIVE)) continue;
@@ -1026,10 +1066,11 @@ void lifecycle_step(oracle_world* w, double dt) {
             if (asked > available && asked > 0.0) taken = want * (available / asked);
             if (taken <= 0.0) continue;
             w->co2_held[i] += taken;
-            C.deposit[idx] += static_cast<uint64_t>(to_fixed(-taken / gv, C.scale));
+            FIXED_ADD(C.deposit[idx], static_cast<uint64_t>(to_fixed(-taken / gv, C.scale)));
         }
         grid_flush(C);
     }
+    PAR_FOR
     for (int32_t i = 0; i < n; ++i) {                                      // death_kernel
         const uint32_t f = w->flags[i];
         if (!(f & ASTRO_FLAG_OCCUPIED) || !(f & ASTRO_FLAG_ALIVE)) continue;
@@ -1061,6 +1102,7 @@ void lifecycle_step(oracle_world* w, double dt) {
             const double hy = 0.5 * d.chamber_h - ASTRO_CELL_RADIUS;
             const double hz = 0.5 * d.chamber_d - ASTRO_CELL_RADIUS;
             auto clampv = [](double v, double h) { return v < -h ? -h : (v > h ? h : v); };
+            PAR_FOR
             for (int32_t i = 0; i < n; ++i) {                              // divide_kernel, :140-204
                 const uint32_t f = w->flags[i];
                 if (!(f & ASTRO_FLAG_OCCUPIED) || !(f & ASTRO_FLAG_ALIVE)) continue;
@@ -2121,6 +2163,16 @@ void oracle_pcg32_seed(uint64_t is, uint64_t iq, uint64_t* st, uint64_t* inc) {
     *st = r.state;
     *inc = r.inc;
 }
+// Threads the parallel loops use: OpenMP's team size in the all-cores build,
+// 1 in the single-threaded one.
+int32_t oracle_threads(void) {
+#ifdef _OPENMP
+    return static_cast<int32_t>(omp_get_max_threads());
+#else
+    return 1;
+#endif
+}
+
 int32_t oracle_philox_blocks(const uint32_t* words, int32_t n, uint32_t* out) {
     if (n < 0 || (n > 0 && (!words || !out))) return ASTRO_INVALID_ARGUMENT;
     for (int32_t i = 0; i < n; ++i) {

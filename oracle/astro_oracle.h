@@ -82,6 +82,9 @@ int32_t oracle_stage_diffuse(oracle_world* w, int32_t field, double dt);
 int32_t oracle_stage_lifecycle(oracle_world* w, double dt);
 int32_t oracle_stage_compact(oracle_world* w);
 
+/* Threads of the parallel loops: the OpenMP team in liboracle_omp.so, else 1. */
+int32_t oracle_threads(void);
+
 /* Counter-based streams (include/astro_hot.h, ASTRO_RNG_PHILOX). */
 int32_t oracle_philox_blocks(const uint32_t* words, int32_t n, uint32_t* out);
 int32_t oracle_philox_stream(uint64_t seed, uint64_t id, uint64_t tick, int32_t stage, int32_t n,
