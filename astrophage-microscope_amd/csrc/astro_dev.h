@@ -114,10 +114,11 @@ constexpr uint32_t REC_SLOT_MASK = 0x3FFFFFFFu;
 constexpr uint32_t REC_UNOCC = 1u << 30;
 constexpr uint32_t REC_ALIVE = 1u << 31;
 
-// A cell's force, 24 bytes with no padding: the force buffer crosses HBM
-// about four times per cell per tick (sample, contact read and write,
-// integrate).
-struct Force { double x, y, z; };
+// A cell's force at the layout's width: three doubles (24 bytes) in the
+// contract layout, three floats (12 bytes) in the FP32 layout, whose
+// integrator takes the force in float. The buffer crosses HBM about four
+// times per cell per tick (sample, contact read and write, integrate).
+template <class P> struct Force { P x, y, z; };
 
 // ---------------------------------------------------------------------------
 // PCG32 per-cell streams (src/core/rng.cuh:23-117)

@@ -141,10 +141,10 @@ __device__ __forceinline__ void feed_taxis_cell(const Cells<P>& c, int i, uint32
 // t/co2/n2_local and returns buoyancy + photon recoil (zero if unoccupied).
 // (x, y, z) is the cell's position, as the float the samplers take.
 template <class P>
-__device__ __forceinline__ Force sample_force_cell(const Cells<P>& c, int i, uint32_t fl, float x, float y, float z,
+__device__ __forceinline__ Force<double> sample_force_cell(const Cells<P>& c, int i, uint32_t fl, float x, float y, float z,
                                                    const Grid& T, const Grid& C, const Grid& N, int c_on, int n_on,
                                                    int gaxis) {
-    Force F{0.0, 0.0, 0.0};
+    Force<double> F{0.0, 0.0, 0.0};
     if (!(fl & ASTRO_FLAG_OCCUPIED)) return F;
     c.t_local[i] = grid_sample(T, x, y, z);
     c.co2_local[i] = c_on ? grid_sample(C, x, y, z) : 0.0f;

@@ -101,7 +101,7 @@ struct WorldDev {
     uint32_t* vals_sorted;  // [cap] reference slot of each sorted record
     uint32_t* sphys;        // [cap] physical index of each sorted record
     int32_t*  spos;         // [pcap] sorted position of each physical entry
-    Force*    fs;           // [cap] force in sorted order
+    Force<P>* fs;           // [cap] force in sorted order, at the layout's width
     double*   dirs;         // [cap] directional irradiance in sorted order
 
     int32_t*  bsum;         // [tiles(nb + 1)] per-scan-tile totals of bcount

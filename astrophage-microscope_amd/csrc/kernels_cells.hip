@@ -313,7 +313,7 @@ struct KSampleBase {
         int n_on;
         int gaxis;
         const int32_t* spos;
-        Force* fs;
+        Force<P>* fs;
         const double* dirs;
         float ambient;
         double dt;
@@ -345,8 +345,8 @@ struct KSampleBase {
                 f = c.flags[i];
             }
         }
-        const Force F = sample_force_cell<P>(c, i, f, xf, yf, zf, T, C, N, c_on, n_on, gaxis);
-        fs[spos[i]] = F;
+        const Force<double> F = sample_force_cell<P>(c, i, f, xf, yf, zf, T, C, N, c_on, n_on, gaxis);
+        fs[spos[i]] = Force<P>{static_cast<P>(F.x), static_cast<P>(F.y), static_cast<P>(F.z)};
     });
     }
 };
@@ -397,7 +397,7 @@ struct KIntegrate {
         Cells<P> c;
         const Counters* ctr;
         const int32_t* spos;
-        const Force* fs;
+        const Force<P>* fs;
         TickParams t;
         DemandArgs dm;
     };
@@ -427,7 +427,7 @@ struct KIntegrate {
     const double mass = cell_mass(biomass, energy);
     const bool awake = (fl & ASTRO_FLAG_AWAKE) != 0u;
     const double t_drag = awake ? ASTRO_CELL_TEMP_SETPOINT : t_local;
-    const Force F = fs[sp];
+    const Force<P> F = fs[sp];
     auto r = [&] {
         if constexpr (PHILOX) return Philox(t.rng_seed, RNG_STAGE_INTEGRATE, ctr->rng_tick, id);
         else return cell_rng(rng_state, id);

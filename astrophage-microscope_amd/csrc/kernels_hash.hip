@@ -402,12 +402,15 @@ constexpr int CONTACT_ROUND = 32;
 //     touching. A lane marks the steps that pass in a 32-bit mask, and after
 //     each round of at most 32 steps queues those records' sorted positions
 //     in walk order.
-//   * The queue is drained in order through the exact double-precision
-//     contact_force -- when it fills, and at the end -- so F sums the same
-//     terms in the same order as the reference. Pairs the reference finds
-//     apart add a zero vector there; F's components are sums of nonzero
-//     terms onto a nonzero or +0.0 start and so never -0.0, which makes
-//     those adds exact no-ops.
+//   * The queue is drained in order through contact_force -- when it fills,
+//     and at the end -- so F sums the same terms in the same order as the
+//     reference: in double in the contract layout, bit for bit, and in float
+//     in the FP32 layout, whose force buffer and integrator are float (each
+//     term then differs from the double one by a few float roundings, at
+//     most about k * 2^-21 * 2a for pairs at the contact threshold). Pairs the
+//     exact test finds apart add a zero vector; F's components are sums of
+//     nonzero terms onto a nonzero or +0.0 start and so never -0.0, which
+//     makes those adds exact no-ops.
 template <class P>
 struct KContactWalk {
     static constexpr bool stride = true;   // any block count covers the work
@@ -416,7 +419,7 @@ struct KContactWalk {
         HashGeo h;
         const Rec<P>* rec;
         const int32_t* bstart;
-        Force* fs;
+        Force<P>* fs;
         double k_contact;
     };
     static __device__ void run(const Args& a, VGrid vg) {
@@ -442,25 +445,25 @@ struct KContactWalk {
                             gz);
         }
         int queued = 0;
-        Force F{};
+        Force<P> F{};
         bool loaded = false;
+        // contact_force (src/sim/integrator.cuh:210-219) in the layout's width
+        const P two_a_p = static_cast<P>(two_a), k_p = static_cast<P>(k_contact);
         const auto drain = [&]() {
             if (!loaded) {
                 F = fs[k];
                 loaded = true;
             }
-            const double px = static_cast<double>(r.x), py = static_cast<double>(r.y), pz = static_cast<double>(r.z);
             for (int j = 0; j < queued; ++j) {
                 const Rec<P> o = rec[queue[j * BLOCK]];
-                // contact_force (src/sim/integrator.cuh:210-219)
-                const double dx = px - static_cast<double>(o.x);
-                const double dy = py - static_cast<double>(o.y);
-                const double dz = pz - static_cast<double>(o.z);
-                const double r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 >= two_a * two_a || r2 <= 0.0) continue;
-                const double rr2 = sqrt(r2);
-                const double overlap = two_a - rr2;
-                const double s = k_contact * overlap / rr2;
+                const P dx = r.x - o.x;
+                const P dy = r.y - o.y;
+                const P dz = r.z - o.z;
+                const P r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 >= two_a_p * two_a_p || r2 <= P(0)) continue;
+                const P rr2 = sqrt(r2);
+                const P overlap = two_a_p - rr2;
+                const P s = k_p * overlap / rr2;
                 F.x += dx * s;
                 F.y += dy * s;
                 F.z += dz * s;

@@ -204,7 +204,7 @@ struct astro_world {
     uint32_t* dlist = nullptr;
     float* co2_avail = nullptr;
     void *trec = nullptr, *rec = nullptr;
-    Force* fs = nullptr;
+    void* fs = nullptr;              // Force<double> or Force<float> per layout
     double* dirs = nullptr;
     unsigned long long* co2_demand = nullptr;
     unsigned long long* stats = nullptr;
@@ -289,7 +289,7 @@ WorldDev<P> make_view(astro_world* w) {
     }
     d.hg = w->hg;
     d.keys = w->keys; d.rin = w->rin; d.bcount = w->bcount; d.bstart = w->bstart;
-    d.vals_sorted = w->vals_sorted; d.spos = w->spos; d.fs = w->fs; d.dirs = w->dirs;
+    d.vals_sorted = w->vals_sorted; d.spos = w->spos; d.fs = static_cast<Force<P>*>(w->fs); d.dirs = w->dirs;
     d.tphys = w->tphys; d.sphys = w->sphys;
     d.dflag = w->dflag; d.prefix = w->prefix; d.co2_demand = w->co2_demand;
     d.drop = w->drop; d.dprefix = w->dprefix; d.bsum = w->bsum; d.dsum = w->dsum; d.xsum = w->xsum;
@@ -1208,7 +1208,7 @@ int32_t astro_world_create(const astro_world_desc* din, astro_world** out) {
         w->vals_sorted = cv.take<uint32_t>(n);
         w->sphys = cv.take<uint32_t>(n);
         w->spos = cv.take<int32_t>(pn);
-        w->fs = cv.take<Force>(n);
+        w->fs = cv.take<char>(n * (w->fp32 ? sizeof(Force<float>) : sizeof(Force<double>)));
         w->dirs = cv.take<double>(n);
         w->dflag = cv.take<int32_t>(n);
         w->prefix = cv.take<int32_t>(n);
@@ -1783,20 +1783,28 @@ int32_t astro_forces_download(astro_world* w, double* fx, double* fy, double* fz
     const int32_t m = std::min(n, c.count);
     if (m <= 0) return ASTRO_OK;
     const int32_t np = c.nphys;
-    std::vector<Force> f(c.count);
+    std::vector<double> f(3 * static_cast<size_t>(c.count));
     std::vector<int32_t> sp(np);
     std::vector<uint32_t> ls(np);
     const uint32_t* lslot_dev = w->fp32 ? w->c32[0].lslot : w->c64[0].lslot;
-    if ((st = hip_check(w, hipMemcpy(f.data(), w->fs, sizeof(Force) * c.count, hipMemcpyDeviceToHost), "forces")) != ASTRO_OK) return st;
+    if (w->fp32) {
+        std::vector<Force<float>> ff(c.count);
+        if ((st = hip_check(w, hipMemcpy(ff.data(), w->fs, sizeof(Force<float>) * c.count, hipMemcpyDeviceToHost), "forces")) != ASTRO_OK) return st;
+        for (int32_t q = 0; q < c.count; ++q) {
+            f[3 * q] = ff[q].x; f[3 * q + 1] = ff[q].y; f[3 * q + 2] = ff[q].z;
+        }
+    } else if ((st = hip_check(w, hipMemcpy(f.data(), w->fs, sizeof(Force<double>) * c.count, hipMemcpyDeviceToHost), "forces")) != ASTRO_OK) {
+        return st;
+    }
     if ((st = hip_check(w, hipMemcpy(sp.data(), w->spos, sizeof(int32_t) * np, hipMemcpyDeviceToHost), "spos")) != ASTRO_OK) return st;
     if ((st = hip_check(w, hipMemcpy(ls.data(), lslot_dev, sizeof(uint32_t) * np, hipMemcpyDeviceToHost), "lslot")) != ASTRO_OK) return st;
     for (int32_t p = 0; p < np; ++p) {
         const uint32_t i = ls[p];
         if (i == NO_SLOT || i >= static_cast<uint32_t>(m)) continue;
-        const Force& F = f[sp[p]];
-        if (fx) fx[i] = F.x;
-        if (fy) fy[i] = F.y;
-        if (fz) fz[i] = F.z;
+        const double* F = &f[3 * static_cast<size_t>(sp[p])];
+        if (fx) fx[i] = F[0];
+        if (fy) fy[i] = F[1];
+        if (fz) fz[i] = F[2];
     }
     return ASTRO_OK;
 }
