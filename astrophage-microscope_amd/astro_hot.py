@@ -33,6 +33,7 @@ OK, OUT_OF_MEMORY, DEVICE_ERROR, INVALID_ARGUMENT, NOT_FOUND, PARSE_ERROR, \
 FP64_CONTRACT, FP32_HOT = 0, 1
 RNG_PCG32, RNG_PHILOX = 0, 1
 RNG_STAGE_SPAWN, RNG_STAGE_TAXIS, RNG_STAGE_INTEGRATE = 1, 2, 3
+DEPOSIT_AUTO, DEPOSIT_ATOMIC, DEPOSIT_DIRECT = 0, 1, 2
 BOUNDARY_REFLECTING, BOUNDARY_PERIODIC, BOUNDARY_ABSORBING = 0, 1, 2
 GRAVITY_Y, GRAVITY_Z = 0, 1
 CLOCK_REALTIME, CLOCK_MOTION, CLOCK_METABOLIC, CLOCK_GENERATIONAL, CLOCK_CUSTOM = range(5)
@@ -271,6 +272,7 @@ SIGNATURES = {
 }
 PRODUCT_ONLY = {
     "hash_build_noalloc": (_I, [_P, _P, _P, _P, _I, _I, _I, _I, _D, _P, _P, _P, _P]),
+    "world_set_deposit_path": (_I, [_P, _I]),
     "profile_enable": (_I, [_P, _I]),
     "profile_read": (_I, [_P, _P, _I, _P, _P, _I, C.POINTER(_I)]),
     "profile_reset": (_I, [_P]),
@@ -570,6 +572,11 @@ class World:
         conv = {name: np.ascontiguousarray(arrays[name], dt) for name, dt in TAU_FIELDS}
         soa = TauSoA(*[_ptr(conv[name]) for name, _ in TAU_FIELDS])
         self._check(self.b.taumoeba_upload(self.h, C.byref(soa), n), "taumoeba_upload")
+
+    def set_deposit_path(self, path: int):
+        """Thermostat deposit path (DEPOSIT_AUTO / ATOMIC / DIRECT); the fields
+        are the same bit for bit on each."""
+        self._check(self.b.world_set_deposit_path(self.h, path))
 
     def set_clock_preset(self, preset: int, physics: float = 1.0, biology: float = 1.0):
         self._check(self.b.world_set_clock_preset(self.h, preset, physics, biology))

@@ -113,6 +113,8 @@ struct WorldDev {
     int32_t*  xsum;         // [tiles(cap)] per-scan-tile totals of drop
     uint32_t* dlist;        // [cap] dropped reference slots, ascending
     unsigned long long* co2_demand;   // [CO2 grid] fixed-point demand
+    uint32_t* tcount;       // [T grid] live cells per voxel during the thermal step, else 0
+    uint8_t*  talone;       // [pcap] each entry's voxel this tick: 1 alone, 2 shared, 0 not live
     unsigned long long* stats;        // StatsAccum words (see launch_stats)
     unsigned long long* flash;        // [4] spin-drive audit: impulse x, y, z, energy
 };
@@ -178,9 +180,12 @@ struct SpawnArgs {
 };
 template <class P> void launch_spawn(const Launch& L, const WorldDev<P>& w, const SpawnArgs& a,
                                      int32_t base, int32_t lbase, uint64_t first_id, uint64_t seed);
+// The thermostat's deposit paths (THERMAL_* in kernels_cells.hip): 0 atomic;
+// 1 direct, first substep of the tick (after launch_thermal_count); 2 direct.
+template <class P> void launch_thermal_count(const Launch& L, Views<P> ws);
 template <class P> void launch_thermal_substep(const Launch& L, Views<P> ws, Ticks ts, double dt_sub,
-                                               bool update_rec);
-template <class P> void launch_apply_awake(const Launch& L, Views<P> ws, int field);
+                                               bool update_rec, int mode);
+template <class P> void launch_apply_awake(const Launch& L, Views<P> ws, int field, int mode);
 template <class P> void launch_feed_taxis(const Launch& L, Views<P> ws, Ticks ts, bool feed, bool taxis);
 // Feeding/taxis (optional), field sample + body force, then the contact walk:
 // leaves each cell's total force in fs (sorted order).

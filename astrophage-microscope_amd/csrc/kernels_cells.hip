@@ -129,7 +129,46 @@ __global__ __launch_bounds__(BLOCK) void k_spawn(Cells<P> c, SpawnArgs p, int32_
 // thermal_kernel (src/sim/thermal.cu:19-71). A starving cell also clears the
 // alive bit of its sorted neighbour record, so the feed walk later this tick
 // sees the same flags the reference's feed_kernel reads.
+//
+// Two deposit paths, equal bit for bit (mode, below):
+//   * ATOMIC: every depositor adds into its voxel's fixed-point accumulator
+//     and KApplyAwake flushes it, value += float(from_fixed(total));
+//   * DIRECT: the occupied entries of each voxel are counted once per tick
+//     (KThermalCount). A cell that is the only entry in its voxel applies its
+//     own deposit, value += float(from_fixed(raw)) -- exactly the flush of a
+//     single depositor, and no other thread of the pass reads or writes that
+//     voxel, since every occupied entry reads its voxel's value -- so only
+//     shared voxels use the accumulator. The first substep reads the counts
+//     into talone (1 alone, 2 shared), the later ones read talone, and the
+//     first apply pass clears the counts again.
 // ---------------------------------------------------------------------------
+enum : int { THERMAL_ATOMIC = 0, THERMAL_DIRECT_FIRST = 1, THERMAL_DIRECT = 2 };
+
+// The occupied entries counted per temperature voxel: each of them, live or
+// dead, reads its voxel's value in the thermal pass. Positions do not change
+// during the thermal step, so the counts hold for all its substeps.
+template <class P>
+struct KThermalCount {
+    static constexpr bool stride = true;   // any block count covers the work
+    struct Args {
+        Cells<P> c;
+        const Counters* ctr;
+        Grid T;
+        uint32_t* tcount;
+    };
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [c, ctr, T, tcount] = a;
+        xcd_tiles(vg, ctr->nphys, [&](const int i, const bool valid) {
+            if (!valid) return;
+            const uint32_t fl = c.flags[i];
+            const float xf = static_cast<float>(c.x[i]), yf = static_cast<float>(c.y[i]),
+                        zf = static_cast<float>(c.z[i]);
+            if (fl & ASTRO_FLAG_OCCUPIED)
+                atomicAdd(&tcount[grid_nearest(T, xf, yf, zf)], 1u);
+        });
+    }
+};
+
 template <class P>
 struct KThermal {
     static constexpr bool stride = true;   // any block count covers the work
@@ -142,9 +181,12 @@ struct KThermal {
         Rec<P>* rec;
         const int32_t* spos;
         int update_rec;
+        const uint32_t* tcount;   // DIRECT_FIRST: occupied entries per voxel
+        uint8_t* talone;          // DIRECT: per occupied physical entry, 1 alone, 2 shared
+        int mode;
     };
     static __device__ void run(const Args& a, VGrid vg) {
-        const auto& [c, ctr, T, relax, cap, rec, spos, update_rec] = a;
+        const auto& [c, ctr, T, relax, cap, rec, spos, update_rec, tcount, talone, mode] = a;
 
     const int n = ctr->nphys;
     const auto body = [&](const int i) {
@@ -159,6 +201,13 @@ struct KThermal {
     const bool early = (fl & ASTRO_FLAG_AWAKE) && (fl & ASTRO_FLAG_ALIVE);
     double e0 = early ? c.energy[i] : 0.0;
     const float t_grid = T.value[idx];
+    uint8_t share = 2;
+    if (mode == THERMAL_DIRECT_FIRST) {
+        share = tcount[idx] == 1u ? 1 : 2;
+        talone[i] = share;
+    } else if (mode == THERMAL_DIRECT) {
+        share = talone[i];
+    }
     bool awake = (fl & ASTRO_FLAG_AWAKE) != 0u;
     uint32_t nf = fl;
     if (!awake && static_cast<double>(t_grid) >= ASTRO_CELL_TEMP_SETPOINT) {
@@ -183,7 +232,12 @@ struct KThermal {
         energy = ASTRO_CELL_ENERGY_MAX;
     }
     c.energy[i] = energy;
-    deposit_add(&T.deposit[idx], moved / cap, T.scale);
+    if (share == 1) {
+        const unsigned long long raw = static_cast<unsigned long long>(to_fixed(moved / cap, T.scale));
+        if (raw != 0ull) T.value[idx] = t_grid + static_cast<float>(from_fixed(raw, T.scale));
+    } else {
+        deposit_add(&T.deposit[idx], moved / cap, T.scale);
+    }
     if (energy <= 0.0) {
         c.flags[i] = nf & ~ASTRO_FLAG_ALIVE;
         c.death_cause[i] = ASTRO_DEATH_STARVED;
@@ -200,7 +254,9 @@ struct KThermal {
 // Every depositor exchanges its voxel's accumulator with zero; exactly one of
 // them receives the total and applies it, so the voxel sees the same single
 // `value += float(from_fixed(total))` as a dense flush. AWAKE marks every
-// thermostat depositor (thermal.cu:130), dead or alive afterwards.
+// thermostat depositor (thermal.cu:130), dead or alive afterwards. On the
+// DIRECT path only cells of shared voxels take part, and the first pass of the
+// tick also clears the voxel counts.
 template <class P>
 struct KApplyAwake {
     static constexpr bool stride = true;   // any block count covers the work
@@ -208,16 +264,22 @@ struct KApplyAwake {
         Cells<P> c;
         const Counters* ctr;
         Grid g;
+        uint32_t* tcount;
+        const uint8_t* talone;
+        int mode;
     };
     static __device__ void run(const Args& a, VGrid vg) {
-        const auto& [c, ctr, g] = a;
+        const auto& [c, ctr, g, tcount, talone, mode] = a;
 
     const int n = ctr->nphys;
     const auto body = [&](const int i) {
     // Flags and position are loaded together (in bounds for i below nphys).
     const uint32_t fl = c.flags[i];
     const float xf = static_cast<float>(c.x[i]), yf = static_cast<float>(c.y[i]), zf = static_cast<float>(c.z[i]);
-    if (!(fl & ASTRO_FLAG_OCCUPIED) || !(fl & ASTRO_FLAG_AWAKE)) return;
+    const uint8_t share = mode == THERMAL_ATOMIC ? 2 : talone[i];
+    if (!(fl & ASTRO_FLAG_OCCUPIED)) return;
+    if (mode == THERMAL_DIRECT_FIRST) tcount[grid_nearest(g, xf, yf, zf)] = 0u;
+    if (share != 2 || !(fl & ASTRO_FLAG_AWAKE)) return;
     const uint32_t idx = grid_nearest(g, xf, yf, zf);
     const unsigned long long raw = atomicExch(&g.deposit[idx], 0ull);
     if (raw != 0ull) g.value[idx] += static_cast<float>(from_fixed(raw, g.scale));
@@ -973,7 +1035,15 @@ static_assert(GridStride<KThermal<double>> && GridStride<KLifeApply<float>>);
 static_assert(!GridStride<KLifeScan> && !GridStride<KLifeFinalize>);
 
 template <class P>
-void launch_thermal_substep(const Launch& L, Views<P> ws, Ticks ts, double dt_sub, bool update_rec) {
+void launch_thermal_count(const Launch& L, Views<P> ws) {
+    using K = KThermalCount<P>;
+    Batch<K> b;
+    for (const WorldDev<P>& w : ws) b.add({w.c, w.ctr, w.g[ASTRO_FIELD_TEMPERATURE], w.tcount}, cell_grid(w.pcap));
+    launch_k<K>(L, "thermal_count", b.args, b.blocks);
+}
+
+template <class P>
+void launch_thermal_substep(const Launch& L, Views<P> ws, Ticks ts, double dt_sub, bool update_rec, int mode) {
     using K = KThermal<P>;
     Batch<K> b;
     for (size_t i = 0; i < ws.size(); ++i) {
@@ -981,17 +1051,18 @@ void launch_thermal_substep(const Launch& L, Views<P> ws, Ticks ts, double dt_su
         const double cap = ts[i].T_cap;
         const double g = ASTRO_CONDUCTION_COEFF;
         const double relax = (cap <= 0.0 || g <= 0.0) ? 0.0 : -std::expm1(-g * dt_sub / cap);
-        b.add({w.c, w.ctr, w.g[ASTRO_FIELD_TEMPERATURE], relax, cap, w.rec, w.spos, update_rec ? 1 : 0},
+        b.add({w.c, w.ctr, w.g[ASTRO_FIELD_TEMPERATURE], relax, cap, w.rec, w.spos, update_rec ? 1 : 0, w.tcount,
+               w.talone, mode},
               cell_grid(w.pcap));
     }
     launch_k<K>(L, "thermal", b.args, b.blocks);
 }
 
 template <class P>
-void launch_apply_awake(const Launch& L, Views<P> ws, int field) {
+void launch_apply_awake(const Launch& L, Views<P> ws, int field, int mode) {
     using K = KApplyAwake<P>;
     Batch<K> b;
-    for (const WorldDev<P>& w : ws) b.add({w.c, w.ctr, w.g[field]}, cell_grid(w.pcap));
+    for (const WorldDev<P>& w : ws) b.add({w.c, w.ctr, w.g[field], w.tcount, w.talone, mode}, cell_grid(w.pcap));
     launch_k<K>(L, "deposit_apply", b.args, b.blocks);
 }
 
@@ -1147,8 +1218,9 @@ void launch_stats_cells(const Launch& L, const WorldDev<P>& w) {
 #define INSTANTIATE(P)                                                                                          \
     template void launch_spawn<P>(const Launch&, const WorldDev<P>&, const SpawnArgs&, int32_t, int32_t,       \
                                   uint64_t, uint64_t);                                                          \
-    template void launch_thermal_substep<P>(const Launch&, Views<P>, Ticks, double, bool);                      \
-    template void launch_apply_awake<P>(const Launch&, Views<P>, int);                                          \
+    template void launch_thermal_count<P>(const Launch&, Views<P>);                                             \
+    template void launch_thermal_substep<P>(const Launch&, Views<P>, Ticks, double, bool, int);                 \
+    template void launch_apply_awake<P>(const Launch&, Views<P>, int, int);                                     \
     template void launch_feed_taxis<P>(const Launch&, Views<P>, Ticks, bool, bool);                             \
     template void launch_integrate<P>(const Launch&, Views<P>, Ticks, bool);                                    \
     template void launch_cell_forces<P>(const Launch&, Views<P>, Ticks, bool, bool);                            \

@@ -207,6 +207,13 @@ struct astro_world {
     void* fs = nullptr;              // Force<double> or Force<float> per layout
     double* dirs = nullptr;
     unsigned long long* co2_demand = nullptr;
+    uint32_t* tcount = nullptr;   // live cells per temperature voxel, during the thermal step
+    uint8_t* talone = nullptr;    // per physical entry: alone in its voxel this tick
+    // The thermostat's deposit path (astro_world_set_deposit_path) and the
+    // host's estimate of the live and awake populations that the automatic
+    // choice weighs, refreshed by spawn, upload and stats.
+    int32_t deposit_path = ASTRO_DEPOSIT_AUTO;
+    int64_t live_hint = 0, awake_hint = 0;
     unsigned long long* stats = nullptr;
 
     double physics_rate = 1.0, biology_rate = 1.0;
@@ -291,7 +298,7 @@ WorldDev<P> make_view(astro_world* w) {
     d.keys = w->keys; d.rin = w->rin; d.bcount = w->bcount; d.bstart = w->bstart;
     d.vals_sorted = w->vals_sorted; d.spos = w->spos; d.fs = static_cast<Force<P>*>(w->fs); d.dirs = w->dirs;
     d.tphys = w->tphys; d.sphys = w->sphys;
-    d.dflag = w->dflag; d.prefix = w->prefix; d.co2_demand = w->co2_demand;
+    d.dflag = w->dflag; d.prefix = w->prefix; d.co2_demand = w->co2_demand; d.tcount = w->tcount; d.talone = w->talone;
     d.drop = w->drop; d.dprefix = w->dprefix; d.bsum = w->bsum; d.dsum = w->dsum; d.xsum = w->xsum;
     d.dlist = w->dlist;
     d.co2_avail = w->co2_avail;
@@ -404,6 +411,16 @@ int32_t enqueue_diffuse(WorldSpan ws, const Launch& L, int field, double dt) {
     return ASTRO_OK;
 }
 
+// Which thermostat deposit path a tick takes (the field is the same bit for
+// bit either way). Counting the occupied entries per voxel costs about three
+// memory operations per entry per tick; each awake cell alone in its voxel
+// then saves an atomic add and an exchange per substep.
+bool thermal_direct(const astro_world* w, int substeps) {
+    if (w->deposit_path == ASTRO_DEPOSIT_ATOMIC) return false;
+    if (w->deposit_path == ASTRO_DEPOSIT_DIRECT) return true;
+    return 2 * w->awake_hint * substeps > 3 * w->live_hint;
+}
+
 // thermal_step (src/sim/thermal.cu:75-103).
 template <class P>
 int32_t enqueue_thermal(WorldSpan ws, const Launch& L, double dt, Ticks ts) {
@@ -414,10 +431,13 @@ int32_t enqueue_thermal(WorldSpan ws, const Launch& L, double dt, Ticks ts) {
     for (size_t i = 0; i < ws.size(); ++i)
         if (!ftcs_coeffs(ws[i]->f[ASTRO_FIELD_TEMPERATURE], dt_sub, fc[i]))
             return fail(ws[i], ASTRO_INVALID_ARGUMENT, "diffusion coefficient unstable");
+    const bool direct = thermal_direct(ws[0], substeps);
+    if (direct) launch_thermal_count<P>(L, views_of<P>(ws));
     for (int s = 0; s < substeps; ++s) {
         const std::vector<WorldDev<P>> d = views_of<P>(ws);
-        launch_thermal_substep<P>(L, d, ts, dt_sub, ws[0]->hash_valid);
-        launch_apply_awake<P>(L, d, ASTRO_FIELD_TEMPERATURE);
+        const int mode = !direct ? 0 : (s == 0 ? 1 : 2);
+        launch_thermal_substep<P>(L, d, ts, dt_sub, ws[0]->hash_valid, mode);
+        launch_apply_awake<P>(L, d, ASTRO_FIELD_TEMPERATURE, mode);
         for (size_t i = 0; i < ws.size(); ++i) gs[i] = ws[i]->f[ASTRO_FIELD_TEMPERATURE].grid();
         launch_ftcs(L, gs, fc, "ftcs_T");
         for (astro_world* w : ws) std::swap(w->f[ASTRO_FIELD_TEMPERATURE].value, w->f[ASTRO_FIELD_TEMPERATURE].scratch);
@@ -1219,6 +1239,8 @@ int32_t astro_world_create(const astro_world_desc* din, astro_world** out) {
         w->dsum = cv.take<int32_t>(static_cast<size_t>(scan_tiles_for(cap)));
         w->xsum = cv.take<int32_t>(static_cast<size_t>(scan_tiles_for(cap)));
         w->co2_demand = cv.take<unsigned long long>(std::max<size_t>(1, w->f[ASTRO_FIELD_CO2].count()));
+        w->tcount = cv.take<uint32_t>(std::max<size_t>(1, w->f[ASTRO_FIELD_TEMPERATURE].count()));   // zeroed with the pool
+        w->talone = cv.take<uint8_t>(pn);
         w->stats = cv.take<unsigned long long>(SW_COUNT);
         w->flash_accum = cv.take<unsigned long long>(4);   // zeroed with the pool
         TauDev& t = w->tau;
@@ -1322,6 +1344,8 @@ int32_t astro_cells_spawn(astro_world* w, const astro_spawn_params* p, uint64_t 
     c.nphys += p->count;
     c.next_id += static_cast<uint64_t>(p->count);
     w->hash_valid = false;
+    w->live_hint += p->count;
+    if (p->awake) w->awake_hint += p->count;
     return write_counters(w, c);
 }
 
@@ -1333,6 +1357,16 @@ int32_t astro_cells_upload(astro_world* w, const astro_cell_soa* s, int32_t n) {
     if (st != ASTRO_OK) return st;
     st = w->fp32 ? upload_cells(w, w->c32[0], s, n) : upload_cells(w, w->c64[0], s, n);
     if (st != ASTRO_OK) return st;
+    if (s->flags) {
+        w->live_hint = w->awake_hint = 0;
+        for (int32_t i = 0; i < n; ++i) {
+            const uint32_t f = s->flags[i];
+            if ((f & ASTRO_FLAG_OCCUPIED) && (f & ASTRO_FLAG_ALIVE)) {
+                ++w->live_hint;
+                if (f & ASTRO_FLAG_AWAKE) ++w->awake_hint;
+            }
+        }
+    }
     c.count = n;
     c.nphys = n;
     w->hash_valid = false;
@@ -1616,6 +1650,15 @@ int32_t astro_world_set_clock(astro_world* w, double physics, double biology) {
     return ASTRO_OK;
 }
 
+int32_t astro_world_set_deposit_path(astro_world* w, int32_t path) {
+    if (!w) return ASTRO_INVALID_ARGUMENT;
+    if (path < ASTRO_DEPOSIT_AUTO || path > ASTRO_DEPOSIT_DIRECT)
+        return fail(w, ASTRO_INVALID_ARGUMENT, "unknown deposit path");
+    w->deposit_path = path;
+    drop_graph(w);   // a captured tick keeps the path it was captured with
+    return ASTRO_OK;
+}
+
 // stats_step + world_stats (src/sim/stats.cu:79-153).
 int32_t astro_world_stats(astro_world* w, astro_stats* s) {
     if (!w || !s) return ASTRO_INVALID_ARGUMENT;
@@ -1638,6 +1681,8 @@ int32_t astro_world_stats(astro_world* w, astro_stats* s) {
     s->physics_rate = w->physics_rate;
     s->biology_rate = w->biology_rate;
     const uint32_t na = static_cast<uint32_t>(a[SW_N_ALIVE]), nd = static_cast<uint32_t>(a[SW_N_DEAD]);
+    w->live_hint = na;
+    w->awake_hint = static_cast<int64_t>(a[SW_N_AWAKE]);
     s->n_live = static_cast<int32_t>(na);
     s->n_dead = static_cast<int32_t>(nd);
     s->n_awake = static_cast<int32_t>(a[SW_N_AWAKE]);

@@ -225,6 +225,15 @@ int32_t astro_world_stats(astro_world* w, astro_stats* out);
 int32_t astro_world_state_hash(astro_world* w, uint64_t* out);
 /* world_set_clock with explicit rates (src/sim/step.cu:190-215). */
 int32_t astro_world_set_clock(astro_world* w, double physics_rate, double biology_rate);
+/* How the thermostat folds a cell's exchange into the temperature field. No
+ * counterpart in the reference, whose thermal_kernel always adds into the
+ * fixed-point accumulator (src/sim/thermal.cu:19-71): ATOMIC does that for
+ * every depositor; DIRECT counts the entries per voxel once per tick and lets
+ * a cell alone in its voxel add its own exchange to the value; AUTO (the
+ * default) takes DIRECT when most live cells are awake. The field is the same
+ * bit for bit on every path. An ensemble follows its first chamber. */
+enum { ASTRO_DEPOSIT_AUTO = 0, ASTRO_DEPOSIT_ATOMIC = 1, ASTRO_DEPOSIT_DIRECT = 2 };
+int32_t astro_world_set_deposit_path(astro_world* w, int32_t path);
 
 /* ---- stimuli a scenario applies between ticks ------------------------------ */
 /* ClockPreset (contracts/scenario_v3.h:37) and BrushKind (src/sim/world.cuh). */

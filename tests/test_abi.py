@@ -43,9 +43,11 @@ def test_oracle_mirrors_the_abi(oracle):
     # against oracle/surrogate_oracle.py. The scenario layer
     # (astro_scenario.h) is host code over this ABI, checked against the
     # restatement in tests/scenario_py.py. The allocation-free hash build is
-    # checked against the world's own hash build on the device.
+    # checked against the world's own hash build on the device. The oracle has
+    # one thermostat deposit path; the device's paths are checked against each
+    # other and against it.
     product_only = {"astro_profile_enable", "astro_profile_read", "astro_profile_reset", "astro_world_stream",
-                    "astro_hash_build_noalloc"}
+                    "astro_hash_build_noalloc", "astro_world_set_deposit_path"}
     missing = [s for s in declared_symbols((HEADER,))
                if s not in product_only and not s.startswith(("astro_ensemble_", "astro_surrogate_", "astro_snapshot_"))
                and not hasattr(lib, "oracle_" + s[len("astro_"):])]
