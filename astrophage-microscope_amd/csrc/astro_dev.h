@@ -84,6 +84,7 @@ struct Counters {
     int64_t  cell_steps;       // sum of the count each tick began with
     uint64_t tick_count;       // ticks begun (the host's World::tick between steps)
     uint64_t rng_tick;         // the tick in progress: counter word of Philox streams
+    uint32_t tqueued[2];       // thermostat substeps by parity: a deposit waits in the accumulator
 };
 
 enum : uint32_t { ERR_SCAN = 1u, ERR_COEFF = 2u };

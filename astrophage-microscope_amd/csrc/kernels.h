@@ -182,10 +182,11 @@ template <class P> void launch_spawn(const Launch& L, const WorldDev<P>& w, cons
                                      int32_t base, int32_t lbase, uint64_t first_id, uint64_t seed);
 // The thermostat's deposit paths (THERMAL_* in kernels_cells.hip): 0 atomic;
 // 1 direct, first substep of the tick (after launch_thermal_count); 2 direct.
+// `substep` counts the tick's substeps from 0; `last` marks the final one.
 template <class P> void launch_thermal_count(const Launch& L, Views<P> ws);
 template <class P> void launch_thermal_substep(const Launch& L, Views<P> ws, Ticks ts, double dt_sub,
-                                               bool update_rec, int mode);
-template <class P> void launch_apply_awake(const Launch& L, Views<P> ws, int field, int mode);
+                                               bool update_rec, int mode, int substep, bool last);
+template <class P> void launch_apply_awake(const Launch& L, Views<P> ws, int field, int mode, int substep);
 template <class P> void launch_feed_taxis(const Launch& L, Views<P> ws, Ticks ts, bool feed, bool taxis);
 // Feeding/taxis (optional), field sample + body force, then the contact walk:
 // leaves each cell's total force in fs (sorted order).

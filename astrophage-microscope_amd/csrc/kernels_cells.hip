@@ -174,7 +174,7 @@ struct KThermal {
     static constexpr bool stride = true;   // any block count covers the work
     struct Args {
         Cells<P> c;
-        const Counters* ctr;
+        Counters* ctr;
         Grid T;
         double relax;   // -expm1(-g dt_sub / cap), the same for every cell of the substep
         double cap;
@@ -184,19 +184,26 @@ struct KThermal {
         const uint32_t* tcount;   // DIRECT_FIRST: occupied entries per voxel
         uint8_t* talone;          // DIRECT: per occupied physical entry, 1 alone, 2 shared
         int mode;
+        int parity;               // the substep's parity, for Counters::tqueued
+        int last;                 // the tick's last substep
     };
     static __device__ void run(const Args& a, VGrid vg) {
-        const auto& [c, ctr, T, relax, cap, rec, spos, update_rec, tcount, talone, mode] = a;
+        const auto& [c, ctr, T, relax, cap, rec, spos, update_rec, tcount, talone, mode, parity, last] = a;
 
+    // This substep reports in tqueued[parity] whether any deposit waits in
+    // the accumulator, and clears the other parity's report, which the
+    // previous substep's apply pass has read.
+    if (vg.block == 0 && threadIdx.x == 0) ctr->tqueued[parity ^ 1] = 0u;
     const int n = ctr->nphys;
-    const auto body = [&](const int i) {
+    // Returns whether the cell left a deposit in the accumulator.
+    const auto body = [&](const int i) -> bool {
     // Flags and position are loaded together, ahead of the branch on the
     // flags, so one memory latency precedes the voxel read; a cell already
     // awake and alive fetches its energy beside that read (a cell that
     // ignites below fetches it then).
     const uint32_t fl = c.flags[i];
     const float xf = static_cast<float>(c.x[i]), yf = static_cast<float>(c.y[i]), zf = static_cast<float>(c.z[i]);
-    if (!(fl & ASTRO_FLAG_OCCUPIED)) return;
+    if (!(fl & ASTRO_FLAG_OCCUPIED)) return false;
     const uint32_t idx = grid_nearest(T, xf, yf, zf);
     const bool early = (fl & ASTRO_FLAG_AWAKE) && (fl & ASTRO_FLAG_ALIVE);
     double e0 = early ? c.energy[i] : 0.0;
@@ -216,8 +223,9 @@ struct KThermal {
         c.flags[i] = nf;
     }
     const double t_cell = awake ? ASTRO_CELL_TEMP_SETPOINT : static_cast<double>(t_grid);
-    c.temp_cell[i] = static_cast<float>(t_cell);
-    if (!awake || !(fl & ASTRO_FLAG_ALIVE)) return;
+    // nothing reads temp_cell between substeps: the last one's value stands
+    if (last) c.temp_cell[i] = static_cast<float>(t_cell);
+    if (!awake || !(fl & ASTRO_FLAG_ALIVE)) return false;
     if (!early) e0 = c.energy[i];
     // lumped_exchange_energy (src/sim/thermal.cuh:80-86) with its relaxation factor
     // taken once per substep on the host.
@@ -232,11 +240,13 @@ struct KThermal {
         energy = ASTRO_CELL_ENERGY_MAX;
     }
     c.energy[i] = energy;
+    bool queued = false;
     if (share == 1) {
         const unsigned long long raw = static_cast<unsigned long long>(to_fixed(moved / cap, T.scale));
         if (raw != 0ull) T.value[idx] = t_grid + static_cast<float>(from_fixed(raw, T.scale));
     } else {
         deposit_add(&T.deposit[idx], moved / cap, T.scale);
+        queued = true;
     }
     if (energy <= 0.0) {
         c.flags[i] = nf & ~ASTRO_FLAG_ALIVE;
@@ -244,8 +254,12 @@ struct KThermal {
         c.emit_power[i] = 0.0f;
         if (update_rec) rec[spos[i]].meta &= ~REC_ALIVE;
     }
+    return queued;
     };
-    xcd_tiles(vg, n, [&](const int i, const bool valid) { if (valid) body(i); });
+    xcd_tiles(vg, n, [&](const int i, const bool valid) {
+        const bool q = valid && body(i);
+        if (__ballot(q) != 0ull && (threadIdx.x & 63) == 0 && ctr->tqueued[parity] == 0u) ctr->tqueued[parity] = 1u;
+    });
     }
 };
 
@@ -256,7 +270,8 @@ struct KThermal {
 // `value += float(from_fixed(total))` as a dense flush. AWAKE marks every
 // thermostat depositor (thermal.cu:130), dead or alive afterwards. On the
 // DIRECT path only cells of shared voxels take part, and the first pass of the
-// tick also clears the voxel counts.
+// tick also clears the voxel counts. A substep whose thermal pass queued no
+// deposit has nothing to fold, and the pass returns at once.
 template <class P>
 struct KApplyAwake {
     static constexpr bool stride = true;   // any block count covers the work
@@ -267,9 +282,12 @@ struct KApplyAwake {
         uint32_t* tcount;
         const uint8_t* talone;
         int mode;
+        int parity;
     };
     static __device__ void run(const Args& a, VGrid vg) {
-        const auto& [c, ctr, g, tcount, talone, mode] = a;
+        const auto& [c, ctr, g, tcount, talone, mode, parity] = a;
+
+    if (mode != THERMAL_DIRECT_FIRST && ctr->tqueued[parity] == 0u) return;   // uniform over the grid
 
     const int n = ctr->nphys;
     const auto body = [&](const int i) {
@@ -1043,7 +1061,8 @@ void launch_thermal_count(const Launch& L, Views<P> ws) {
 }
 
 template <class P>
-void launch_thermal_substep(const Launch& L, Views<P> ws, Ticks ts, double dt_sub, bool update_rec, int mode) {
+void launch_thermal_substep(const Launch& L, Views<P> ws, Ticks ts, double dt_sub, bool update_rec, int mode,
+                            int substep, bool last) {
     using K = KThermal<P>;
     Batch<K> b;
     for (size_t i = 0; i < ws.size(); ++i) {
@@ -1052,17 +1071,18 @@ void launch_thermal_substep(const Launch& L, Views<P> ws, Ticks ts, double dt_su
         const double g = ASTRO_CONDUCTION_COEFF;
         const double relax = (cap <= 0.0 || g <= 0.0) ? 0.0 : -std::expm1(-g * dt_sub / cap);
         b.add({w.c, w.ctr, w.g[ASTRO_FIELD_TEMPERATURE], relax, cap, w.rec, w.spos, update_rec ? 1 : 0, w.tcount,
-               w.talone, mode},
+               w.talone, mode, substep & 1, last ? 1 : 0},
               cell_grid(w.pcap));
     }
     launch_k<K>(L, "thermal", b.args, b.blocks);
 }
 
 template <class P>
-void launch_apply_awake(const Launch& L, Views<P> ws, int field, int mode) {
+void launch_apply_awake(const Launch& L, Views<P> ws, int field, int mode, int substep) {
     using K = KApplyAwake<P>;
     Batch<K> b;
-    for (const WorldDev<P>& w : ws) b.add({w.c, w.ctr, w.g[field], w.tcount, w.talone, mode}, cell_grid(w.pcap));
+    for (const WorldDev<P>& w : ws)
+        b.add({w.c, w.ctr, w.g[field], w.tcount, w.talone, mode, substep & 1}, cell_grid(w.pcap));
     launch_k<K>(L, "deposit_apply", b.args, b.blocks);
 }
 
@@ -1219,8 +1239,8 @@ void launch_stats_cells(const Launch& L, const WorldDev<P>& w) {
     template void launch_spawn<P>(const Launch&, const WorldDev<P>&, const SpawnArgs&, int32_t, int32_t,       \
                                   uint64_t, uint64_t);                                                          \
     template void launch_thermal_count<P>(const Launch&, Views<P>);                                             \
-    template void launch_thermal_substep<P>(const Launch&, Views<P>, Ticks, double, bool, int);                 \
-    template void launch_apply_awake<P>(const Launch&, Views<P>, int, int);                                     \
+    template void launch_thermal_substep<P>(const Launch&, Views<P>, Ticks, double, bool, int, int, bool);      \
+    template void launch_apply_awake<P>(const Launch&, Views<P>, int, int, int);                                \
     template void launch_feed_taxis<P>(const Launch&, Views<P>, Ticks, bool, bool);                             \
     template void launch_integrate<P>(const Launch&, Views<P>, Ticks, bool);                                    \
     template void launch_cell_forces<P>(const Launch&, Views<P>, Ticks, bool, bool);                            \

@@ -23,6 +23,10 @@
 // as A and B agree). A wave computes a tile of 32 voxels along x (rows) by 32
 // output channels (columns); lane (i, h) = (lane%32, lane/32) feeds voxel or
 // output channel i and the 8 k values h*8..h*8+7 of each 16-wide k step.
+// The inference kernels (a net call) run the same network on a halo-padded
+// activation layout, the output layer on v_mfma_f32_16x16x32_bf16 (16 output
+// columns for the 2 real ones instead of 32); the training kernels keep the
+// unpadded layout their gradients read.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -42,6 +46,7 @@ namespace astro_sur {
 typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int CH = 32;        // hidden channels (and the MFMA tile width)
 constexpr int CIN0 = 8;       // input channels, padded
@@ -429,6 +434,7 @@ struct PackArgs {
     float ctx, cty, ctz;   // call-integrated diffusion numbers of T (16 x c)
     float ccx, ccy, ccz;   // of C
     float uptake_call;     // dt_call * uptake
+    int px, py;            // padded destination: row length and rows per plane (0: unpadded [V][8])
 };
 
 __device__ __forceinline__ float lap_neumann(const float* u, int64_t v, int x, int y, int z, int nx, int ny, int nz,
@@ -457,7 +463,8 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs a, Norm nm) {
     o[5] = static_cast<bf16>(lap_neumann(a.T, v, x, y, z, a.nx, a.ny, a.nz, a.ctx, a.cty, a.ctz) / nm.dt_out);
     o[6] = static_cast<bf16>(lap_neumann(a.C, v, x, y, z, a.nx, a.ny, a.nz, a.ccx, a.ccy, a.ccz) / nm.dc_out);
     o[7] = static_cast<bf16>(-a.uptake_call * oc * cc / nm.dc_out);
-    *reinterpret_cast<bf16x8*>(a.x + v * CIN0) = o;
+    const int64_t dst = a.px ? (static_cast<int64_t>(z + 1) * a.py + (y + 1)) * a.px + (x + 1) : v;
+    *reinterpret_cast<bf16x8*>(a.x + dst * CIN0) = o;
 }
 
 // target = normalized increments; also squared-norm partials of the raw
@@ -499,20 +506,316 @@ __global__ __launch_bounds__(256) void k_loss(const float* pred, const float* tg
     if (threadIdx.x == 0) lpart[blockIdx.x] = s[0];
 }
 
-__global__ __launch_bounds__(256) void k_apply(const float* pred, float* T, float* C, int64_t n, Norm nm) {
-    const int64_t v = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-    if (v >= n) return;
-    T[v] += nm.dt_out * pred[v * NOUT];
-    C[v] += nm.dc_out * pred[v * NOUT + 1];
+// ---------------------------------------------------------------------------
+// The plain path's substep in one pass over both fields: the FTCS of T and of
+// C (Neumann walls: a missing neighbour is the cell's own value), then the
+// sources, T' = T* + dt S and C' = C* - (dt k) occ C*. A thread takes 4
+// consecutive x points over PZC planes, so each plane it loads serves as the
+// centre of one output and the z neighbour of two others. 24 compulsory bytes
+// per point: T, C, S, occupancy in, T', C' out.
+// ---------------------------------------------------------------------------
+constexpr int PZC = 4;
+
+struct PlainArgs {
+    const float *T, *C, *S, *occ;
+    float *To, *Co;
+    int nx, ny, nz;
+    float tcx, tcy, tcz;   // diffusion numbers of T per substep
+    float ccx, ccy, ccz;   // of C
+    float dt, du;          // dt_sub and dt_sub * uptake
+};
+
+__device__ __forceinline__ float4 ld4f(const float* p) { return *reinterpret_cast<const float4*>(p); }
+
+// One field's 4 x points at planes k0 .. k0+PZC-1 (those below nz).
+__device__ __forceinline__ void ftcs4(const float* __restrict__ u, int x, int y, int k0, int nx, int ny, int nz,
+                                      float cx, float cy, float cz, float4 (&out)[PZC]) {
+    const int64_t pl = static_cast<int64_t>(nx) * ny;
+    const int64_t row = static_cast<int64_t>(y) * nx + x;
+    float4 pz[PZC + 2];
+#pragma unroll
+    for (int m = 0; m < PZC + 2; ++m) {
+        const int k = k0 - 1 + m;
+        pz[m] = (k >= 0 && k < nz) ? ld4f(u + k * pl + row) : float4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+#pragma unroll
+    for (int m = 0; m < PZC; ++m) {
+        const int k = k0 + m;
+        if (k >= nz) break;
+        const int64_t id = k * pl + row;
+        const float4 c4 = pz[m + 1];
+        const float4 b4 = k > 0 ? pz[m] : c4, f4 = k < nz - 1 ? pz[m + 2] : c4;
+        const float4 d4 = y > 0 ? ld4f(u + id - nx) : c4, p4 = y < ny - 1 ? ld4f(u + id + nx) : c4;
+        const float l0 = x > 0 ? u[id - 1] : c4.x, r3 = x + 4 < nx ? u[id + 4] : c4.w;
+        const float c[4] = {c4.x, c4.y, c4.z, c4.w}, b[4] = {b4.x, b4.y, b4.z, b4.w};
+        const float f[4] = {f4.x, f4.y, f4.z, f4.w}, dn[4] = {d4.x, d4.y, d4.z, d4.w};
+        const float up[4] = {p4.x, p4.y, p4.z, p4.w};
+        float o[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const float l = e > 0 ? c[e - 1] : l0, r = e < 3 ? c[e + 1] : r3, cc = c[e];
+            o[e] = cc + (cx * ((l + r) - 2.0f * cc) + cy * ((dn[e] + up[e]) - 2.0f * cc) +
+                         cz * ((b[e] + f[e]) - 2.0f * cc));
+        }
+        out[m] = float4{o[0], o[1], o[2], o[3]};
+    }
 }
 
-// The per-substep sources after each FTCS substep.
-__global__ __launch_bounds__(256) void k_sources(float* T, float* C, const float* S, const float* occ, int64_t n,
-                                                 float dt, float uptake) {
+__global__ __launch_bounds__(256) void k_plain_substep(PlainArgs a) {
+    const int nx4 = a.nx / 4;
+    const int zchunks = (a.nz + PZC - 1) / PZC;
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (t >= static_cast<int64_t>(nx4) * a.ny * zchunks) return;
+    const int x = static_cast<int>(t % nx4) * 4;
+    const int64_t r = t / nx4;
+    const int y = static_cast<int>(r % a.ny), k0 = static_cast<int>(r / a.ny) * PZC;
+    float4 tv[PZC], cv[PZC];
+    ftcs4(a.T, x, y, k0, a.nx, a.ny, a.nz, a.tcx, a.tcy, a.tcz, tv);
+    ftcs4(a.C, x, y, k0, a.nx, a.ny, a.nz, a.ccx, a.ccy, a.ccz, cv);
+    const int64_t pl = static_cast<int64_t>(a.nx) * a.ny;
+#pragma unroll
+    for (int m = 0; m < PZC; ++m) {
+        const int k = k0 + m;
+        if (k >= a.nz) break;
+        const int64_t id = k * pl + static_cast<int64_t>(y) * a.nx + x;
+        const float4 s4 = ld4f(a.S + id), q4 = ld4f(a.occ + id);
+        const float4 t4 = tv[m], c4 = cv[m];
+        *reinterpret_cast<float4*>(a.To + id) =
+            float4{t4.x + a.dt * s4.x, t4.y + a.dt * s4.y, t4.z + a.dt * s4.z, t4.w + a.dt * s4.w};
+        *reinterpret_cast<float4*>(a.Co + id) =
+            float4{c4.x - a.du * q4.x * c4.x, c4.y - a.du * q4.y * c4.y, c4.z - a.du * q4.z * c4.z,
+                   c4.w - a.du * q4.w * c4.w};
+    }
+}
+
+inline unsigned plain_grid(int nx, int ny, int nz) {
+    return static_cast<unsigned>((static_cast<int64_t>(nx / 4) * ny * ((nz + PZC - 1) / PZC) + 255) / 256);
+}
+
+// ---------------------------------------------------------------------------
+// Inference on a halo-padded layout. Activations live in [PZ][PY][PX][ch]
+// with PX = nx + 2, PY = ny rounded up to a multiple of RROWS, + 2, and
+// PZ = nz + 2; voxel (x, y, z) sits at (x + 1, y + 1, z + 1) and everything
+// else stays zero (written once at allocation, never again), which is exactly
+// the zero padding of the convolution. Every tile then reads its operands at
+// a base address plus wave-uniform offsets, with no bounds tests or selects.
+//
+// A wave owns 32 voxels along x by RROWS rows along y of one plane. For each
+// (dz, dx) it loads the RROWS + 2 input rows once and feeds each to the (up
+// to) three output rows whose dy tap reads it, so an input row costs one
+// load for three taps. Per output row the taps are summed in (dz, dx, dy)
+// order.
+// ---------------------------------------------------------------------------
+constexpr int RROWS = 8;
+
+struct Pad {
+    int px, py, pz;
+    __host__ __device__ int64_t at(int x, int y, int z) const {
+        return (static_cast<int64_t>(z + 1) * py + (y + 1)) * px + (x + 1);
+    }
+};
+
+inline Pad pad_of(int nx, int ny, int nz) {
+    return Pad{nx + 2, (ny + RROWS - 1) / RROWS * RROWS + 2, nz + 2};
+}
+
+__device__ __forceinline__ f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// Input layer (8 channels), padded in, padded out: the two lane halves take
+// two taps per 16-wide k step, as in k_conv; the weight operands are loaded
+// once per wave and reused over its RROWS rows.
+__global__ __launch_bounds__(256) void k_conv_in_p(const bf16* __restrict__ X, const bf16* __restrict__ W,
+                                                   const float* __restrict__ bias, bf16* __restrict__ Y, int nx,
+                                                   int ny, int nz, Pad p, float leak) {
+    const int lane = static_cast<int>(threadIdx.x) & 63, wave = static_cast<int>(threadIdx.x) >> 6;
+    const int tiles_x = nx / 32, yg = (ny + RROWS - 1) / RROWS;
+    const int64_t tile = static_cast<int64_t>(blockIdx.x) * 4 + wave;
+    if (tile >= static_cast<int64_t>(tiles_x) * yg * nz) return;   // whole wave
+    const int x0 = static_cast<int>(tile % tiles_x) * 32;
+    const int64_t row = tile / tiles_x;
+    const int y0 = static_cast<int>(row % yg) * RROWS, z = static_cast<int>(row / yg);
+    const int i = lane & 31, h = lane >> 5;
+    constexpr int STEPS = (TAPS + 1) / 2;
+    bf16x8 b[STEPS];
+    int64_t toff[STEPS];
+#pragma unroll
+    for (int m = 0; m < STEPS; ++m) {
+        const int t = 2 * m + h;
+        b[m] = t < TAPS ? *reinterpret_cast<const bf16x8*>(W + (static_cast<int64_t>(t) * CH + i) * CIN0) : bf16x8{};
+        toff[m] = t < TAPS ? ((static_cast<int64_t>(tap_dz(t)) * p.py + tap_dy(t)) * p.px + tap_dx(t)) * CIN0 : 0;
+    }
+    const float bi = bias[i];
+    for (int k = 0; k < RROWS; ++k) {
+        const int y = y0 + k;
+        if (y >= ny) break;   // wave-uniform
+        const bf16* base = X + p.at(x0 + i, y, z) * CIN0;
+        f32x16 acc = {};
+#pragma unroll
+        for (int m = 0; m < STEPS; ++m) {
+            const bf16x8 a = 2 * m + h < TAPS ? *reinterpret_cast<const bf16x8*>(base + toff[m]) : bf16x8{};
+            acc = mfma(a, b[m], acc);
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int x = x0 + (q >> 2) * 8 + h * 4 + (q & 3);
+            float val = acc[q] + bi;
+            val = val > 0.0f ? val : leak * val;
+            Y[p.at(x, y, z) * CH + i] = static_cast<bf16>(val);
+        }
+    }
+}
+
+// A hidden layer (32 -> 32, + bias, leaky ReLU), padded in, padded out; R
+// output rows per wave (R divides RROWS).
+template <int R>
+__global__ __launch_bounds__(256) void k_conv_hid_p(const bf16* __restrict__ X, const bf16* __restrict__ W,
+                                                    const float* __restrict__ bias, bf16* __restrict__ Y, int nx,
+                                                    int ny, int nz, Pad p, float leak) {
+    const int lane = static_cast<int>(threadIdx.x) & 63, wave = static_cast<int>(threadIdx.x) >> 6;
+    const int tiles_x = nx / 32, yg = (ny + R - 1) / R;
+    const int64_t tile = static_cast<int64_t>(blockIdx.x) * 4 + wave;
+    if (tile >= static_cast<int64_t>(tiles_x) * yg * nz) return;   // whole wave
+    const int x0 = static_cast<int>(tile % tiles_x) * 32;
+    const int64_t row = tile / tiles_x;
+    const int y0 = static_cast<int>(row % yg) * R, z = static_cast<int>(row / yg);
+    const int i = lane & 31, h = lane >> 5;
+    const int64_t rstride = static_cast<int64_t>(p.px) * CH;   // one padded row
+    f32x16 acc[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) acc[k] = f32x16{};
+#pragma unroll 1
+    for (int dz = -1; dz <= 1; ++dz) {
+#pragma unroll 1
+        for (int dx = -1; dx <= 1; ++dx) {
+            bf16x8 b[3][2];
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int t = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+                const bf16x8* wp =
+                    reinterpret_cast<const bf16x8*>(W + (static_cast<int64_t>(t) * CH + i) * CH + h * 8);
+                b[dy + 1][0] = wp[0];
+                b[dy + 1][1] = wp[2];
+            }
+            // input row y0 - 1 + m, m = 0 .. R + 1
+            const bf16* rb = X + p.at(x0 + i + dx, y0 - 1, z + dz) * CH + h * 8;
+#pragma unroll
+            for (int m = 0; m < R + 2; ++m) {
+                const bf16x8* pa = reinterpret_cast<const bf16x8*>(rb + m * rstride);
+                const bf16x8 a0 = pa[0], a1 = pa[2];
+#pragma unroll
+                for (int dy = -1; dy <= 1; ++dy) {
+                    const int k = m - 1 - dy;
+                    if (k < 0 || k >= R) continue;   // resolved at compile time
+                    acc[k] = mfma(a0, b[dy + 1][0], acc[k]);
+                    acc[k] = mfma(a1, b[dy + 1][1], acc[k]);
+                }
+            }
+        }
+    }
+    const float bi = bias[i];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int y = y0 + k;
+        if (y >= ny) break;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int x = x0 + (q >> 2) * 8 + h * 4 + (q & 3);
+            float val = acc[k][q] + bi;
+            val = val > 0.0f ? val : leak * val;
+            Y[p.at(x, y, z) * CH + i] = static_cast<bf16>(val);
+        }
+    }
+}
+
+// The output layer (32 -> 2) on 16 x 16 x 32 MFMAs: 16 voxels by 16 output
+// columns (2 of them real) per instruction, all 32 input channels of one tap
+// in its k. Lane l feeds voxel l % 16 with channels (l / 16) * 8 .. + 7, and
+// weight column l % 16 with the same channels; it receives column l % 16 of
+// rows (l / 16) * 4 .. + 3. With `apply`, the outputs go straight into the
+// fields (T += dt_out y0, C += dc_out y1); else into pred [V][2].
+template <int R>
+__global__ __launch_bounds__(256) void k_conv_out_p(const bf16* __restrict__ X, const bf16* __restrict__ W,
+                                                    const float* __restrict__ bias, float* __restrict__ pred,
+                                                    float* __restrict__ T, float* __restrict__ C, int nx, int ny,
+                                                    int nz, Pad p, float dt_out, float dc_out, int apply) {
+    const int lane = static_cast<int>(threadIdx.x) & 63, wave = static_cast<int>(threadIdx.x) >> 6;
+    const int tiles_x = nx / 32, yg = (ny + R - 1) / R;
+    const int64_t tile = static_cast<int64_t>(blockIdx.x) * 4 + wave;
+    if (tile >= static_cast<int64_t>(tiles_x) * yg * nz) return;   // whole wave
+    const int x0 = static_cast<int>(tile % tiles_x) * 32;
+    const int64_t row = tile / tiles_x;
+    const int y0 = static_cast<int>(row % yg) * R, z = static_cast<int>(row / yg);
+    const int n = lane & 15, g = lane >> 4;
+    const int64_t rstride = static_cast<int64_t>(p.px) * CH;
+    f32x4 acc[R][2];
+#pragma unroll
+    for (int k = 0; k < R; ++k) acc[k][0] = acc[k][1] = f32x4{};
+#pragma unroll 1
+    for (int dz = -1; dz <= 1; ++dz) {
+#pragma unroll 1
+        for (int dx = -1; dx <= 1; ++dx) {
+            bf16x8 b[3];
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int t = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+                b[dy + 1] = *reinterpret_cast<const bf16x8*>(W + (static_cast<int64_t>(t) * CH + n) * CH + g * 8);
+            }
+            const bf16* rb = X + p.at(x0 + n + dx, y0 - 1, z + dz) * CH + g * 8;
+#pragma unroll
+            for (int m = 0; m < R + 2; ++m) {
+                const bf16x8 lo = *reinterpret_cast<const bf16x8*>(rb + m * rstride);
+                const bf16x8 hi = *reinterpret_cast<const bf16x8*>(rb + m * rstride + 16 * CH);
+#pragma unroll
+                for (int dy = -1; dy <= 1; ++dy) {
+                    const int k = m - 1 - dy;
+                    if (k < 0 || k >= R) continue;
+                    acc[k][0] = mfma16(lo, b[dy + 1], acc[k][0]);
+                    acc[k][1] = mfma16(hi, b[dy + 1], acc[k][1]);
+                }
+            }
+        }
+    }
+    if (n >= NOUT) return;
+    const float bi = bias[n];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int y = y0 + k;
+        if (y >= ny) break;
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int x = x0 + half * 16 + g * 4 + r;
+                const int64_t v = (static_cast<int64_t>(z) * ny + y) * nx + x;
+                const float val = acc[k][half][r] + bi;
+                if (!apply) pred[v * NOUT + n] = val;
+                else if (n == 0) T[v] += dt_out * val;
+                else C[v] += dc_out * val;
+            }
+        }
+    }
+}
+
+inline unsigned padded_grid(int nx, int ny, int nz, int rows = RROWS) {
+    return static_cast<unsigned>((static_cast<int64_t>(nx / 32) * ((ny + rows - 1) / rows) * nz + 3) / 4);
+}
+
+// Output rows per wave, from a sweep of 2, 4 and 8 at 128^3 (the hidden layer
+// took 278, 216 and 220 us; the output layer 269, 290 and 274 us).
+constexpr int HID_ROWS = 4;
+constexpr int OUT_ROWS = 2;
+static_assert(RROWS % HID_ROWS == 0 && RROWS % OUT_ROWS == 0, "the padding must cover every row group");
+
+// Unpadded inputs [V][8] into the padded layout's interior.
+__global__ __launch_bounds__(256) void k_pad_input(const bf16* __restrict__ x, bf16* __restrict__ xp, int nx, int ny,
+                                                   int nz, Pad p) {
     const int64_t v = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-    if (v >= n) return;
-    T[v] += dt * S[v];
-    C[v] -= dt * uptake * occ[v] * C[v];
+    if (v >= static_cast<int64_t>(nx) * ny * nz) return;
+    const int xx = static_cast<int>(v % nx), yy = static_cast<int>((v / nx) % ny);
+    const int zz = static_cast<int>(v / (static_cast<int64_t>(nx) * ny));
+    *reinterpret_cast<bf16x8*>(xp + p.at(xx, yy, zz) * CIN0) = *reinterpret_cast<const bf16x8*>(x + v * CIN0);
 }
 
 // Per block: the squared difference of two increments from the same start
@@ -667,6 +970,10 @@ struct astro_surrogate {
     // network
     bf16* x0 = nullptr;           // [V][8]
     bf16* act[3] = {};            // hidden activations [V][32]
+    // inference: the same network on the halo-padded layout (see Pad)
+    Pad pad{};
+    bf16* x0p = nullptr;          // [PZ][PY][PX][8]
+    bf16* actp[2] = {};           // [PZ][PY][PX][32], ping-pong
     bf16* dz[2] = {};             // gradient ping-pong [V][32]
     float* pred = nullptr;        // [V][2]
     float* tgt = nullptr;         // [V][2]
@@ -716,20 +1023,6 @@ int32_t salloc(astro_surrogate* s, T*& p, size_t count) {
                : sfail(s, ASTRO_DEVICE_ERROR, "memset");
 }
 
-Grid field_grid(const astro_surrogate* s, float* v, float* scratch) {
-    Grid g{};
-    g.value = v;
-    g.scratch = scratch;
-    g.nx = s->d.nx;
-    g.ny = s->d.ny;
-    g.nz = s->d.nz;
-    g.fdx = static_cast<float>(s->d.dx);
-    g.fdy = static_cast<float>(s->d.dy);
-    g.fdz = static_cast<float>(s->d.dz);
-    g.scale = 1.0;
-    return g;
-}
-
 Ftcs ftcs_of(const astro_surrogate* s, double D) {
     Ftcs f{};
     f.square2d = false;
@@ -743,16 +1036,30 @@ Ftcs ftcs_of(const astro_surrogate* s, double D) {
 // The plain path: steps_per_call x (FTCS of T and C, then the sources), on
 // (T, C) in place (ping-pong through the scratch buffers).
 void plain_advance(astro_surrogate* s, float*& T, float*& C, float*& Ts, float*& Cs) {
-    const Launch Lc{s->stream, nullptr, nullptr};
     const Ftcs ft = ftcs_of(s, s->d.d_temp), fc = ftcs_of(s, s->d.d_nutrient);
+    PlainArgs a{};
+    a.S = s->S;
+    a.occ = s->occ;
+    a.nx = s->d.nx;
+    a.ny = s->d.ny;
+    a.nz = s->d.nz;
+    a.tcx = ft.cx;
+    a.tcy = ft.cy;
+    a.tcz = ft.cz;
+    a.ccx = fc.cx;
+    a.ccy = fc.cy;
+    a.ccz = fc.cz;
+    a.dt = static_cast<float>(s->d.dt_sub);
+    a.du = a.dt * static_cast<float>(s->d.uptake);
+    const unsigned g = plain_grid(a.nx, a.ny, a.nz);
     for (int k = 0; k < s->d.steps_per_call; ++k) {
-        const Grid gt = field_grid(s, T, Ts), gc = field_grid(s, C, Cs);
-        launch_ftcs(Lc, one(gt), one(ft), "sur_ftcs");
-        launch_ftcs(Lc, one(gc), one(fc), "sur_ftcs");
+        a.T = T;
+        a.C = C;
+        a.To = Ts;
+        a.Co = Cs;
+        k_plain_substep<<<g, 256, 0, s->stream>>>(a);
         std::swap(T, Ts);
         std::swap(C, Cs);
-        k_sources<<<grid1(s->n), 256, 0, s->stream>>>(T, C, s->S, s->occ, s->n, static_cast<float>(s->d.dt_sub),
-                                                     static_cast<float>(s->d.uptake));
     }
 }
 
@@ -776,6 +1083,32 @@ PackArgs pack_args(const astro_surrogate* s, const float* T, const float* C) {
     a.ccz = static_cast<float>(k * fc.cz);
     a.uptake_call = static_cast<float>(k * s->d.dt_sub * s->d.uptake);
     return a;
+}
+
+// The inputs of a call straight into the padded layout.
+PackArgs pack_args_padded(const astro_surrogate* s, const float* T, const float* C) {
+    PackArgs a = pack_args(s, T, C);
+    a.x = s->x0p;
+    a.px = s->pad.px;
+    a.py = s->pad.py;
+    return a;
+}
+
+// Inference from x0p: with `apply`, the increments go into (T, C); else the
+// normalized outputs into pred.
+void forward_padded(astro_surrogate* s, bool apply) {
+    const int nx = s->d.nx, ny = s->d.ny, nz = s->d.nz;
+    const unsigned g = padded_grid(nx, ny, nz);
+    hipStream_t st = s->stream;
+    const float lk = static_cast<float>(s->d.leak);
+    const Pad p = s->pad;
+    k_conv_in_p<<<g, 256, 0, st>>>(s->x0p, s->L[0].w, s->L[0].bias, s->actp[0], nx, ny, nz, p, lk);
+    const unsigned gh = padded_grid(nx, ny, nz, HID_ROWS);
+    k_conv_hid_p<HID_ROWS><<<gh, 256, 0, st>>>(s->actp[0], s->L[1].w, s->L[1].bias, s->actp[1], nx, ny, nz, p, lk);
+    k_conv_hid_p<HID_ROWS><<<gh, 256, 0, st>>>(s->actp[1], s->L[2].w, s->L[2].bias, s->actp[0], nx, ny, nz, p, lk);
+    k_conv_out_p<OUT_ROWS><<<padded_grid(nx, ny, nz, OUT_ROWS), 256, 0, st>>>(
+        s->actp[0], s->L[3].w, s->L[3].bias, s->pred, s->T, s->C, nx, ny, nz, p, s->nm.dt_out, s->nm.dc_out,
+        apply ? 1 : 0);
 }
 
 void forward(astro_surrogate* s) {
@@ -1042,6 +1375,11 @@ int32_t astro_surrogate_create(const astro_surrogate_desc* din, astro_surrogate*
     if (st == ASTRO_OK) st = salloc(s.get(), s->count, n);
     if (st == ASTRO_OK) st = salloc(s.get(), s->x0, n * CIN0);
     for (int k = 0; k < 3 && st == ASTRO_OK; ++k) st = salloc(s.get(), s->act[k], n * CH);
+    // padded buffers: zeroed here, and only their interiors are ever written
+    s->pad = pad_of(d.nx, d.ny, d.nz);
+    const int64_t np = static_cast<int64_t>(s->pad.px) * s->pad.py * s->pad.pz;
+    if (st == ASTRO_OK) st = salloc(s.get(), s->x0p, np * CIN0);
+    for (int k = 0; k < 2 && st == ASTRO_OK; ++k) st = salloc(s.get(), s->actp[k], np * CH);
     for (int k = 0; k < 2 && st == ASTRO_OK; ++k) st = salloc(s.get(), s->dz[k], n * CH);
     if (st == ASTRO_OK) st = salloc(s.get(), s->pred, n * NOUT);
     if (st == ASTRO_OK) st = salloc(s.get(), s->tgt, n * NOUT);
@@ -1202,9 +1540,8 @@ int32_t astro_surrogate_advance(astro_surrogate* s, int32_t mode, int32_t* used_
         hipMemcpyAsync(s->T0, s->T, sizeof(float) * s->n, hipMemcpyDeviceToDevice, s->stream);
         hipMemcpyAsync(s->C0, s->C, sizeof(float) * s->n, hipMemcpyDeviceToDevice, s->stream);
     }
-    k_pack<<<grid1(s->n), 256, 0, s->stream>>>(pack_args(s, s->T, s->C), s->nm);
-    forward(s);
-    k_apply<<<grid1(s->n), 256, 0, s->stream>>>(s->pred, s->T, s->C, s->n, s->nm);
+    k_pack<<<grid1(s->n), 256, 0, s->stream>>>(pack_args_padded(s, s->T, s->C), s->nm);
+    forward_padded(s, true);
     if (used_surrogate) *used_surrogate = 1;
     if (mode != ASTRO_SURROGATE_GATED) return scheck(s, hipGetLastError(), "advance");
     // Heat budget: with Neumann walls the plain path changes the total heat
@@ -1279,7 +1616,14 @@ int32_t astro_surrogate_eval(astro_surrogate* s, const float* x, size_t n, const
     for (size_t k = 0; k < n * CIN0; ++k) xb[k] = static_cast<bf16>(x[k]);
     int32_t st = scheck(s, hipMemcpy(s->x0, xb.data(), sizeof(bf16) * xb.size(), hipMemcpyHostToDevice), "upload");
     if (st != ASTRO_OK) return st;
-    forward(s);
+    // predictions alone come from the inference kernels; with a target, from
+    // the training kernels whose activations the gradients need
+    if (target) {
+        forward(s);
+    } else {
+        k_pad_input<<<grid1(s->n), 256, 0, s->stream>>>(s->x0, s->x0p, s->d.nx, s->d.ny, s->d.nz, s->pad);
+        forward_padded(s, false);
+    }
     if (target) {
         if ((st = scheck(s, hipMemcpyAsync(s->tgt, target, sizeof(float) * n * NOUT, hipMemcpyHostToDevice, s->stream),
                          "upload")) != ASTRO_OK)

@@ -436,8 +436,8 @@ int32_t enqueue_thermal(WorldSpan ws, const Launch& L, double dt, Ticks ts) {
     for (int s = 0; s < substeps; ++s) {
         const std::vector<WorldDev<P>> d = views_of<P>(ws);
         const int mode = !direct ? 0 : (s == 0 ? 1 : 2);
-        launch_thermal_substep<P>(L, d, ts, dt_sub, ws[0]->hash_valid, mode);
-        launch_apply_awake<P>(L, d, ASTRO_FIELD_TEMPERATURE, mode);
+        launch_thermal_substep<P>(L, d, ts, dt_sub, ws[0]->hash_valid, mode, s, s == substeps - 1);
+        launch_apply_awake<P>(L, d, ASTRO_FIELD_TEMPERATURE, mode, s);
         for (size_t i = 0; i < ws.size(); ++i) gs[i] = ws[i]->f[ASTRO_FIELD_TEMPERATURE].grid();
         launch_ftcs(L, gs, fc, "ftcs_T");
         for (astro_world* w : ws) std::swap(w->f[ASTRO_FIELD_TEMPERATURE].value, w->f[ASTRO_FIELD_TEMPERATURE].scratch);

@@ -456,6 +456,11 @@ def bench_surrogate(binding: ah.Binding, trajectories: int, epochs: int, calls: 
     steps = s.desc.steps_per_call
     n = s.n
     net_tflops = SURROGATE_FLOPS_PER_VOXEL * n / timing["net"] / 1e12
+    # the plain path's compulsory bytes: per substep and voxel T, C, S and
+    # occupancy in, T' and C' out, 4 bytes each
+    plain_bytes = 24.0 * steps * n
+    net_at_peak = SURROGATE_FLOPS_PER_VOXEL * n / 2.5e15
+    plain_at_peak = plain_bytes / (HBM_PEAK_GBS * 1e9)
     out = {
         "workload": "learned field surrogate: 128^3 T + nutrient, 550x550x200 um, 16 FTCS substeps of 6.25 us per "
                     "call, states with Gaussian hot/cool spots and a clustered 25k-cell culture",
@@ -474,6 +479,13 @@ def bench_surrogate(binding: ah.Binding, trajectories: int, epochs: int, calls: 
                      "achieved": round(net_tflops, 1), "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": round(net_tflops / 2500.0, 4),
                      "flops_per_call": SURROGATE_FLOPS_PER_VOXEL * n},
+        "plain_roofline": {"kernel": "k_plain_substep (whole plain call)", "bound": "hbm",
+                           "achieved": round(plain_bytes / timing["plain"] / 1e9, 1), "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": round(plain_bytes / timing["plain"] / 1e9 / HBM_PEAK_GBS, 4),
+                           "bytes_per_call": plain_bytes},
+        "ceiling": {"net_us_at_mfma_peak": round(net_at_peak * 1e6, 1),
+                    "plain_us_at_hbm_peak": round(plain_at_peak * 1e6, 1),
+                    "net_vs_plain_at_both_peaks": round(plain_at_peak / net_at_peak, 3)},
         "cpu_baseline": {"value": cpu["field_steps_per_s"], "unit": "field steps/s", "cores": 1, "kind": "port",
                          "sample": "FP64 numpy plain path on the 3 held-out states (16 substeps each)"},
     }

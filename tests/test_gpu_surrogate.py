@@ -6,9 +6,13 @@ Bars:
     oracle that rounds at the same points (inputs, weights and hidden
     activations to BF16); weight gradients within 3e-2 relative L2 per layer
     (the device also rounds the backpropagated gradients to BF16);
-  * the plain path (the product FTCS kernels plus the sources), FP32 against
-    the FP64 oracle: fields within 1e-6 relative L2, increments within 2e-3
-    (FP32 rounding of fields near 369 K);
+  * predictions come from two sets of kernels: the inference kernels on the
+    halo-padded layout (eval without a target, and every net call) and the
+    training kernels (eval with a target); both are held to the oracle;
+  * the plain path (one fused pass per substep: both fields' FTCS plus the
+    sources), FP32 against the FP64 oracle: fields within 1e-6 relative L2,
+    increments within 2e-3 (FP32 rounding of fields near 369 K), at a small
+    shape and at the benchmarked 128^3 shape;
   * a gated call that the judge rejects returns exactly the plain result;
   * training brings the increment errors well below predicting no change.
 """
@@ -46,8 +50,8 @@ def random_inputs(rng, shape):
 
 @pytest.mark.parametrize("dims", [(32, 6, 5), (160, 5, 3)])
 def test_network_forward_matches_oracle(gpu, dims):
-    """(160, 5, 3): a partial 32-voxel x chunk after a full 128 and an odd
-    row count, for the LDS-staged layers' work items."""
+    """(160, 5, 3): five 32-voxel x tiles, and row counts (6 and 5) that end
+    in partial row groups of the inference kernels."""
     rng = np.random.default_rng(11)
     kw = dict(SMALL, nx=dims[0], ny=dims[1], nz=dims[2], dx=550e-6 / dims[0], dy=550e-6 / dims[0])
     with ah.Surrogate(gpu, **kw) as s:
@@ -73,7 +77,8 @@ def test_network_gradients_match_oracle(gpu, dims):
         x = random_inputs(rng, s.shape)
         tgt = rng.normal(0, 1, s.shape + (2,)).astype(np.float32)
         pred, grads = s.eval(x, tgt, grads=True)
-        _, _, want = so.loss_and_grads(x, tgt, so.unpack(layers, ah.SURROGATE_CIN), leak=0.1)
+        _, want_pred, want = so.loss_and_grads(x, tgt, so.unpack(layers, ah.SURROGATE_CIN), leak=0.1)
+        assert so.rel_l2(pred, want_pred) < 5e-3   # the training kernels' forward pass
         for li, (gw, gb) in enumerate(want):
             c = ah.SURROGATE_CIN[li]
             got_w = grads[li][:27 * 32 * c].reshape(27, 32, c)
@@ -95,6 +100,23 @@ def test_plain_path_matches_fp64_oracle(gpu):
         Tw, Cw = so.plain_advance(T0, C0, S, occ, s.params())
         # FP32 fields near 369 K carry 3e-5 K per rounding; the increments
         # (tenths of a kelvin) inherit that: ~1e-3 relative at most
+        assert so.rel_l2(T1, Tw) < 1e-6 and so.rel_l2(C1, Cw) < 1e-6
+        assert so.rel_l2(T1.astype(np.float64) - T0, Tw - T0) < 2e-3
+        assert so.rel_l2(C1.astype(np.float64) - C0, Cw - C0) < 2e-3
+
+
+def test_plain_path_full_shape_matches_fp64_oracle(gpu):
+    """The benchmarked shape: 128^3 over 550 x 550 x 200 um, 25,000 cells,
+    16 substeps per call; two calls."""
+    with ah.Surrogate(gpu) as s:
+        assert s.shape == (128, 128, 128)
+        s.sample(21)
+        T0, C0, S, occ = s.state()
+        assert occ.sum() == 25000
+        s.advance(ah.SURROGATE_PLAIN)
+        s.advance(ah.SURROGATE_PLAIN)
+        T1, C1, _, _ = s.state()
+        Tw, Cw = so.plain_advance(T0, C0, S, occ, s.params(), steps=2 * s.params()["steps_per_call"])
         assert so.rel_l2(T1, Tw) < 1e-6 and so.rel_l2(C1, Cw) < 1e-6
         assert so.rel_l2(T1.astype(np.float64) - T0, Tw - T0) < 2e-3
         assert so.rel_l2(C1.astype(np.float64) - C0, Cw - C0) < 2e-3
