@@ -1,20 +1,24 @@
 // astro_dev.h -- device-side data layout and physics of the MI355X hot path.
 //
 // Layout in HBM (one world):
-//   * cell SoA in slot order, one 256 B-aligned carve per array, in the
-//     reference's contract order (contracts/cell_store_v1.h:42-74). Position
-//     and velocity are `P` (double for the contract layout, float for the hot
-//     layout); every other array keeps its contract type.
-//   * a second SoA of the same shape, the out-of-place compaction target.
+//   * cell SoA, one 256 B-aligned carve per array, in the reference's
+//     contract order (contracts/cell_store_v1.h:42-74), stored physically in
+//     spatial-hash bucket order; each entry carries its reference slot in
+//     `lslot` (NO_SLOT for a tombstone). Position and velocity are `P`
+//     (double for the contract layout, float for the hot layout); every other
+//     array keeps its contract type.
+//   * a second SoA of the same shape, the target of the periodic re-sort.
 //   * the spatial hash as bucket-sorted neighbour records (position + slot +
-//     alive bit), a dense bucket prefix `bstart[nb + 2]`, and the slot ->
-//     sorted-position map `spos`, rebuilt every tick.
+//     alive bit), a dense bucket prefix `bstart[nb + 2]`, and the
+//     physical <-> sorted-position maps, rebuilt every tick.
 //   * per field: value, ping-pong scratch, int64 fixed-point accumulator.
 //   * a small Counters block: the population count, next id and the per-tick
 //     tallies live on the device, so a tick needs no host synchronisation.
 //
-// All arithmetic on positions, velocities and energies is double precision,
-// whatever `P` is; `P` only sets the storage width. Compiled with
+// Energies, heat and nutrient budgets and the stats are double precision in
+// both layouts. In the hot layout the integrator's drag, OU coefficients and
+// Gaussian draws and the contact arithmetic are float, and displacements are
+// summed onto the stored coordinates in double. Compiled with
 // -ffp-contract=off so expressions round as written.
 #pragma once
 
