@@ -34,6 +34,7 @@
 #include <memory>
 #include <random>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "astro_hot.h"
@@ -595,16 +596,11 @@ inline unsigned plain_grid(int nx, int ny, int nz) {
 // ---------------------------------------------------------------------------
 // Inference on a halo-padded layout. Activations live in [PZ][PY][PX][ch]
 // with PX = nx + 2, PY = ny rounded up to a multiple of RROWS, + 2, and
-// PZ = nz + 2; voxel (x, y, z) sits at (x + 1, y + 1, z + 1) and everything
-// else stays zero (written once at allocation, never again), which is exactly
-// the zero padding of the convolution. Every tile then reads its operands at
-// a base address plus wave-uniform offsets, with no bounds tests or selects.
-//
-// A wave owns 32 voxels along x by RROWS rows along y of one plane. For each
-// (dz, dx) it loads the RROWS + 2 input rows once and feeds each to the (up
-// to) three output rows whose dy tap reads it, so an input row costs one
-// load for three taps. Per output row the taps are summed in (dz, dx, dy)
-// order.
+// PZ = nz rounded up to a multiple of ZPLANES, + 2; voxel (x, y, z) sits at
+// (x + 1, y + 1, z + 1) and everything else stays zero (written once at
+// allocation, never again), which is exactly the zero padding of the
+// convolution. Every tile then reads its operands at a base address plus
+// wave-uniform offsets, with no bounds tests or selects.
 // ---------------------------------------------------------------------------
 constexpr int RROWS = 8;
 
@@ -615,8 +611,12 @@ struct Pad {
     }
 };
 
+// Output planes a z-marching wave owns (k_conv_hid_z); the padding covers
+// whole groups of them.
+constexpr int ZPLANES = 16;
+
 inline Pad pad_of(int nx, int ny, int nz) {
-    return Pad{nx + 2, (ny + RROWS - 1) / RROWS * RROWS + 2, nz + 2};
+    return Pad{nx + 2, (ny + RROWS - 1) / RROWS * RROWS + 2, (nz + ZPLANES - 1) / ZPLANES * ZPLANES + 2};
 }
 
 __device__ __forceinline__ f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c) {
@@ -667,134 +667,191 @@ __global__ __launch_bounds__(256) void k_conv_in_p(const bf16* __restrict__ X, c
     }
 }
 
-// A hidden layer (32 -> 32, + bias, leaky ReLU), padded in, padded out; R
-// output rows per wave (R divides RROWS).
-template <int R>
-__global__ __launch_bounds__(256) void k_conv_hid_p(const bf16* __restrict__ X, const bf16* __restrict__ W,
+// A hidden layer (32 -> 32, + bias, leaky ReLU), padded in, padded out,
+// marching along z. A wave owns 32 voxels along x by R rows
+// by Z planes. It walks the input planes z0-1 .. z0+Z once; each input plane
+// feeds three output planes (dz = -1, 0, +1), whose accumulators it keeps in
+// a ring of three, so every input row is loaded once per dx instead of once
+// per (dz, dx). The block's four waves share the layer's weights, staged once
+// in LDS (27 x 32 x 32 BF16, 54 KB). Per output row the taps are summed in
+// (dz, dx, dy) order. Each input row loaded for a dx feeds the (up to) three
+// output rows whose dy tap reads it. The ring slot of output plane zo is
+// (zo - z0 + 1) mod 3, fixed at compile time by unrolling the march by three.
+template <int R, int Z>
+__global__ __launch_bounds__(256) void k_conv_hid_z(const bf16* __restrict__ X, const bf16* __restrict__ W,
                                                     const float* __restrict__ bias, bf16* __restrict__ Y, int nx,
                                                     int ny, int nz, Pad p, float leak) {
+    __shared__ __align__(16) bf16 sw[TAPS * CH * CH];
+    for (int q = static_cast<int>(threadIdx.x); q < TAPS * CH * CH / 8; q += 256)
+        reinterpret_cast<bf16x8*>(sw)[q] = reinterpret_cast<const bf16x8*>(W)[q];
+    __syncthreads();   // the last block barrier: waves without a tile leave below
     const int lane = static_cast<int>(threadIdx.x) & 63, wave = static_cast<int>(threadIdx.x) >> 6;
-    const int tiles_x = nx / 32, yg = (ny + R - 1) / R;
+    const int tiles_x = nx / 32, yg = (ny + R - 1) / R, zg = (nz + Z - 1) / Z;
     const int64_t tile = static_cast<int64_t>(blockIdx.x) * 4 + wave;
-    if (tile >= static_cast<int64_t>(tiles_x) * yg * nz) return;   // whole wave
+    if (tile >= static_cast<int64_t>(tiles_x) * yg * zg) return;   // whole wave
     const int x0 = static_cast<int>(tile % tiles_x) * 32;
-    const int64_t row = tile / tiles_x;
-    const int y0 = static_cast<int>(row % yg) * R, z = static_cast<int>(row / yg);
+    const int64_t rest = tile / tiles_x;
+    const int y0 = static_cast<int>(rest % yg) * R, z0 = static_cast<int>(rest / yg) * Z;
+    const int zend = z0 + Z < nz ? z0 + Z : nz;   // output planes [z0, zend)
     const int i = lane & 31, h = lane >> 5;
-    const int64_t rstride = static_cast<int64_t>(p.px) * CH;   // one padded row
-    f32x16 acc[R];
+    const int64_t rstride = static_cast<int64_t>(p.px) * CH;
+    const float bi = bias[i];
+    f32x16 acc[3][R];
 #pragma unroll
-    for (int k = 0; k < R; ++k) acc[k] = f32x16{};
-#pragma unroll 1
-    for (int dz = -1; dz <= 1; ++dz) {
+    for (int s = 0; s < 3; ++s)
+#pragma unroll
+        for (int k = 0; k < R; ++k) acc[s][k] = f32x16{};
+    // One input plane zi whose ring position is J = (zi - z0 + 1) mod 3:
+    // output plane zi - dz sits in slot (J - dz) mod 3.
+    const auto plane = [&](const int zi, auto J) {
+        constexpr int j = decltype(J)::value;
 #pragma unroll 1
         for (int dx = -1; dx <= 1; ++dx) {
-            bf16x8 b[3][2];
-#pragma unroll
-            for (int dy = -1; dy <= 1; ++dy) {
-                const int t = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
-                const bf16x8* wp =
-                    reinterpret_cast<const bf16x8*>(W + (static_cast<int64_t>(t) * CH + i) * CH + h * 8);
-                b[dy + 1][0] = wp[0];
-                b[dy + 1][1] = wp[2];
-            }
-            // input row y0 - 1 + m, m = 0 .. R + 1
-            const bf16* rb = X + p.at(x0 + i + dx, y0 - 1, z + dz) * CH + h * 8;
+            const bf16* rb = X + p.at(x0 + i + dx, y0 - 1, zi) * CH + h * 8;
+            bf16x8 a[R + 2][2];
 #pragma unroll
             for (int m = 0; m < R + 2; ++m) {
                 const bf16x8* pa = reinterpret_cast<const bf16x8*>(rb + m * rstride);
-                const bf16x8 a0 = pa[0], a1 = pa[2];
+                a[m][0] = pa[0];
+                a[m][1] = pa[2];
+            }
+#pragma unroll
+            for (int dz = -1; dz <= 1; ++dz) {
+                const int zo = zi - dz;
+                if (zo < z0 || zo >= zend) continue;   // wave-uniform
 #pragma unroll
                 for (int dy = -1; dy <= 1; ++dy) {
-                    const int k = m - 1 - dy;
-                    if (k < 0 || k >= R) continue;   // resolved at compile time
-                    acc[k] = mfma(a0, b[dy + 1][0], acc[k]);
-                    acc[k] = mfma(a1, b[dy + 1][1], acc[k]);
+                    const int t = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+                    const bf16x8* wp = reinterpret_cast<const bf16x8*>(sw + (t * CH + i) * CH + h * 8);
+                    const bf16x8 b0 = wp[0], b1 = wp[2];
+#pragma unroll
+                    for (int k = 0; k < R; ++k) {
+                        // slot of output plane zi - dz
+                        f32x16& c = acc[(j - dz + 3) % 3][k];
+                        c = mfma(a[k + 1 + dy][0], b0, c);
+                        c = mfma(a[k + 1 + dy][1], b1, c);
+                    }
                 }
             }
         }
-    }
-    const float bi = bias[i];
+        // Output plane zi - 1 has had its last input plane: store and clear.
+        constexpr int done = (j + 2) % 3;
+        const int zo = zi - 1;
+        if (zo >= z0 && zo < zend) {
 #pragma unroll
-    for (int k = 0; k < R; ++k) {
-        const int y = y0 + k;
-        if (y >= ny) break;
+            for (int k = 0; k < R; ++k) {
+                const int y = y0 + k;
+                if (y >= ny) break;
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int x = x0 + (q >> 2) * 8 + h * 4 + (q & 3);
-            float val = acc[k][q] + bi;
-            val = val > 0.0f ? val : leak * val;
-            Y[p.at(x, y, z) * CH + i] = static_cast<bf16>(val);
+                for (int q = 0; q < 16; ++q) {
+                    const int x = x0 + (q >> 2) * 8 + h * 4 + (q & 3);
+                    float val = acc[done][k][q] + bi;
+                    val = val > 0.0f ? val : leak * val;
+                    Y[p.at(x, y, zo) * CH + i] = static_cast<bf16>(val);
+                }
+            }
         }
+#pragma unroll
+        for (int k = 0; k < R; ++k) acc[done][k] = f32x16{};
+    };
+    for (int zi = z0 - 1; zi <= zend; zi += 3) {
+        plane(zi, std::integral_constant<int, 0>{});
+        if (zi + 1 <= zend) plane(zi + 1, std::integral_constant<int, 1>{});
+        if (zi + 2 <= zend) plane(zi + 2, std::integral_constant<int, 2>{});
     }
 }
 
-// The output layer (32 -> 2) on 16 x 16 x 32 MFMAs: 16 voxels by 16 output
-// columns (2 of them real) per instruction, all 32 input channels of one tap
-// in its k. Lane l feeds voxel l % 16 with channels (l / 16) * 8 .. + 7, and
-// weight column l % 16 with the same channels; it receives column l % 16 of
-// rows (l / 16) * 4 .. + 3. With `apply`, the outputs go straight into the
-// fields (T += dt_out y0, C += dc_out y1); else into pred [V][2].
-template <int R>
-__global__ __launch_bounds__(256) void k_conv_out_p(const bf16* __restrict__ X, const bf16* __restrict__ W,
+// The output layer (32 -> 2), marching along z as k_conv_hid_z, on
+// 16 x 16 x 32 MFMAs: 16 voxels by 16 output columns (2 of them real) per
+// instruction, all 32 input channels of one tap in its k. Lane l feeds voxel
+// l % 16 with channels (l / 16) * 8 .. + 7, and weight column l % 16 with the
+// same channels; it receives column l % 16 of rows (l / 16) * 4 .. + 3. The
+// LDS holds the 16 weight columns the MFMAs read. With `apply`, the outputs go
+// straight into the fields (T += dt_out y0, C += dc_out y1); else into pred
+// [V][2].
+template <int R, int Z>
+__global__ __launch_bounds__(256) void k_conv_out_z(const bf16* __restrict__ X, const bf16* __restrict__ W,
                                                     const float* __restrict__ bias, float* __restrict__ pred,
                                                     float* __restrict__ T, float* __restrict__ C, int nx, int ny,
                                                     int nz, Pad p, float dt_out, float dc_out, int apply) {
+    __shared__ __align__(16) bf16 sw[TAPS * 16 * CH];   // [tap][column < 16][input channel]
+    for (int q = static_cast<int>(threadIdx.x); q < TAPS * 16 * CH / 8; q += 256) {
+        const int t = q / (16 * CH / 8), rem = q % (16 * CH / 8);
+        reinterpret_cast<bf16x8*>(sw)[q] = reinterpret_cast<const bf16x8*>(W + static_cast<int64_t>(t) * CH * CH)[rem];
+    }
+    __syncthreads();   // the last block barrier: waves without a tile leave below
     const int lane = static_cast<int>(threadIdx.x) & 63, wave = static_cast<int>(threadIdx.x) >> 6;
-    const int tiles_x = nx / 32, yg = (ny + R - 1) / R;
+    const int tiles_x = nx / 32, yg = (ny + R - 1) / R, zg = (nz + Z - 1) / Z;
     const int64_t tile = static_cast<int64_t>(blockIdx.x) * 4 + wave;
-    if (tile >= static_cast<int64_t>(tiles_x) * yg * nz) return;   // whole wave
+    if (tile >= static_cast<int64_t>(tiles_x) * yg * zg) return;   // whole wave
     const int x0 = static_cast<int>(tile % tiles_x) * 32;
-    const int64_t row = tile / tiles_x;
-    const int y0 = static_cast<int>(row % yg) * R, z = static_cast<int>(row / yg);
+    const int64_t rest = tile / tiles_x;
+    const int y0 = static_cast<int>(rest % yg) * R, z0 = static_cast<int>(rest / yg) * Z;
+    const int zend = z0 + Z < nz ? z0 + Z : nz;
     const int n = lane & 15, g = lane >> 4;
     const int64_t rstride = static_cast<int64_t>(p.px) * CH;
-    f32x4 acc[R][2];
+    const float bi = n < NOUT ? bias[n] : 0.0f;
+    f32x4 acc[3][R][2];
 #pragma unroll
-    for (int k = 0; k < R; ++k) acc[k][0] = acc[k][1] = f32x4{};
-#pragma unroll 1
-    for (int dz = -1; dz <= 1; ++dz) {
+    for (int s = 0; s < 3; ++s)
+#pragma unroll
+        for (int k = 0; k < R; ++k) acc[s][k][0] = acc[s][k][1] = f32x4{};
+    const auto plane = [&](const int zi, auto J) {
+        constexpr int j = decltype(J)::value;
 #pragma unroll 1
         for (int dx = -1; dx <= 1; ++dx) {
-            bf16x8 b[3];
-#pragma unroll
-            for (int dy = -1; dy <= 1; ++dy) {
-                const int t = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
-                b[dy + 1] = *reinterpret_cast<const bf16x8*>(W + (static_cast<int64_t>(t) * CH + n) * CH + g * 8);
-            }
-            const bf16* rb = X + p.at(x0 + n + dx, y0 - 1, z + dz) * CH + g * 8;
+            const bf16* rb = X + p.at(x0 + n + dx, y0 - 1, zi) * CH + g * 8;
+            bf16x8 a[R + 2][2];
 #pragma unroll
             for (int m = 0; m < R + 2; ++m) {
-                const bf16x8 lo = *reinterpret_cast<const bf16x8*>(rb + m * rstride);
-                const bf16x8 hi = *reinterpret_cast<const bf16x8*>(rb + m * rstride + 16 * CH);
+                a[m][0] = *reinterpret_cast<const bf16x8*>(rb + m * rstride);
+                a[m][1] = *reinterpret_cast<const bf16x8*>(rb + m * rstride + 16 * CH);
+            }
+#pragma unroll
+            for (int dz = -1; dz <= 1; ++dz) {
+                const int zo = zi - dz;
+                if (zo < z0 || zo >= zend) continue;   // wave-uniform
 #pragma unroll
                 for (int dy = -1; dy <= 1; ++dy) {
-                    const int k = m - 1 - dy;
-                    if (k < 0 || k >= R) continue;
-                    acc[k][0] = mfma16(lo, b[dy + 1], acc[k][0]);
-                    acc[k][1] = mfma16(hi, b[dy + 1], acc[k][1]);
+                    const int t = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+                    const bf16x8 b = *reinterpret_cast<const bf16x8*>(sw + (t * 16 + n) * CH + g * 8);
+#pragma unroll
+                    for (int k = 0; k < R; ++k) {
+                        f32x4* c = acc[(j - dz + 3) % 3][k];
+                        c[0] = mfma16(a[k + 1 + dy][0], b, c[0]);
+                        c[1] = mfma16(a[k + 1 + dy][1], b, c[1]);
+                    }
                 }
             }
         }
-    }
-    if (n >= NOUT) return;
-    const float bi = bias[n];
+        constexpr int done = (j + 2) % 3;
+        const int zo = zi - 1;
+        if (n < NOUT && zo >= z0 && zo < zend) {
 #pragma unroll
-    for (int k = 0; k < R; ++k) {
-        const int y = y0 + k;
-        if (y >= ny) break;
+            for (int k = 0; k < R; ++k) {
+                const int y = y0 + k;
+                if (y >= ny) break;
 #pragma unroll
-        for (int half = 0; half < 2; ++half) {
+                for (int half = 0; half < 2; ++half) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int x = x0 + half * 16 + g * 4 + r;
-                const int64_t v = (static_cast<int64_t>(z) * ny + y) * nx + x;
-                const float val = acc[k][half][r] + bi;
-                if (!apply) pred[v * NOUT + n] = val;
-                else if (n == 0) T[v] += dt_out * val;
-                else C[v] += dc_out * val;
+                    for (int r = 0; r < 4; ++r) {
+                        const int x = x0 + half * 16 + g * 4 + r;
+                        const int64_t v = (static_cast<int64_t>(zo) * ny + y) * nx + x;
+                        const float val = acc[done][k][half][r] + bi;
+                        if (!apply) pred[v * NOUT + n] = val;
+                        else if (n == 0) T[v] += dt_out * val;
+                        else C[v] += dc_out * val;
+                    }
+                }
             }
         }
+#pragma unroll
+        for (int k = 0; k < R; ++k) acc[done][k][0] = acc[done][k][1] = f32x4{};
+    };
+    for (int zi = z0 - 1; zi <= zend; zi += 3) {
+        plane(zi, std::integral_constant<int, 0>{});
+        if (zi + 1 <= zend) plane(zi + 1, std::integral_constant<int, 1>{});
+        if (zi + 2 <= zend) plane(zi + 2, std::integral_constant<int, 2>{});
     }
 }
 
@@ -802,9 +859,11 @@ inline unsigned padded_grid(int nx, int ny, int nz, int rows = RROWS) {
     return static_cast<unsigned>((static_cast<int64_t>(nx / 32) * ((ny + rows - 1) / rows) * nz + 3) / 4);
 }
 
-// Output rows per wave, from a sweep of 2, 4 and 8 at 128^3 (the hidden layer
-// took 278, 216 and 220 us; the output layer 269, 290 and 274 us).
-constexpr int HID_ROWS = 4;
+// Rows and planes per wave, measured at 128^3: the z-marching hidden layer at
+// 2 rows x 16 planes takes 131 us (133 at 8 planes; one plane per wave with 4
+// rows took 216); the output layer at 2 rows x 16 planes 182 us (207 at 8
+// planes; 4 rows leave one wave per SIMD and take 255).
+constexpr int HID_ROWS = 2;
 constexpr int OUT_ROWS = 2;
 static_assert(RROWS % HID_ROWS == 0 && RROWS % OUT_ROWS == 0, "the padding must cover every row group");
 
@@ -1103,12 +1162,16 @@ void forward_padded(astro_surrogate* s, bool apply) {
     const float lk = static_cast<float>(s->d.leak);
     const Pad p = s->pad;
     k_conv_in_p<<<g, 256, 0, st>>>(s->x0p, s->L[0].w, s->L[0].bias, s->actp[0], nx, ny, nz, p, lk);
-    const unsigned gh = padded_grid(nx, ny, nz, HID_ROWS);
-    k_conv_hid_p<HID_ROWS><<<gh, 256, 0, st>>>(s->actp[0], s->L[1].w, s->L[1].bias, s->actp[1], nx, ny, nz, p, lk);
-    k_conv_hid_p<HID_ROWS><<<gh, 256, 0, st>>>(s->actp[1], s->L[2].w, s->L[2].bias, s->actp[0], nx, ny, nz, p, lk);
-    k_conv_out_p<OUT_ROWS><<<padded_grid(nx, ny, nz, OUT_ROWS), 256, 0, st>>>(
-        s->actp[0], s->L[3].w, s->L[3].bias, s->pred, s->T, s->C, nx, ny, nz, p, s->nm.dt_out, s->nm.dc_out,
-        apply ? 1 : 0);
+    const unsigned gh = static_cast<unsigned>(
+        (static_cast<int64_t>(nx / 32) * ((ny + HID_ROWS - 1) / HID_ROWS) * ((nz + ZPLANES - 1) / ZPLANES) + 3) / 4);
+    k_conv_hid_z<HID_ROWS, ZPLANES><<<gh, 256, 0, st>>>(s->actp[0], s->L[1].w, s->L[1].bias, s->actp[1], nx, ny, nz,
+                                                         p, lk);
+    k_conv_hid_z<HID_ROWS, ZPLANES><<<gh, 256, 0, st>>>(s->actp[1], s->L[2].w, s->L[2].bias, s->actp[0], nx, ny, nz,
+                                                         p, lk);
+    const unsigned go = static_cast<unsigned>(
+        (static_cast<int64_t>(nx / 32) * ((ny + OUT_ROWS - 1) / OUT_ROWS) * ((nz + ZPLANES - 1) / ZPLANES) + 3) / 4);
+    k_conv_out_z<OUT_ROWS, ZPLANES><<<go, 256, 0, st>>>(s->actp[0], s->L[3].w, s->L[3].bias, s->pred, s->T, s->C, nx,
+                                                         ny, nz, p, s->nm.dt_out, s->nm.dc_out, apply ? 1 : 0);
 }
 
 void forward(astro_surrogate* s) {
