@@ -214,12 +214,13 @@ def test_ensemble_follows_chamber_changes(gpu):
 
 
 def test_ensemble_full_shape_seed7(gpu):
-    """The batched-ensemble workload at its full shape: 256 chambers of
-    1 x 1 x 0.2 mm, cell counts log-uniform in [500, 50,000] from seed 7
-    (about 2.76M cells in all), 64^3 temperature and irradiance grids each, a
-    random in-plane light direction per chamber, every cell awake. Three
-    ensemble steps (graph-replayed) against the same 256 chambers stepped one
-    after another: every chamber's state hash and counters are equal."""
+    """The batched-ensemble workload at its full shape and horizon: 256
+    chambers of 1 x 1 x 0.2 mm, cell counts log-uniform in [500, 50,000] from
+    seed 7 (about 2.76M cells in all), 64^3 temperature and irradiance grids
+    each, a random in-plane light direction per chamber, every cell awake.
+    1,000 ensemble steps (graph-replayed) against the same 256 chambers
+    stepped one after another, each on its own graph: every chamber's state
+    hash and counters are equal after 3 steps and after 1,000."""
     import sys
     from pathlib import Path
     sys.path.insert(0, str(Path(__file__).resolve().parents[1]))
@@ -230,11 +231,13 @@ def test_ensemble_full_shape_seed7(gpu):
     assert len(counts) == 256 and min(counts) >= 500 and max(counts) <= 50_000
     assert 2.0e6 < sum(counts) < 3.5e6
     with ah.Ensemble(gpu, ens_worlds) as e:
-        e.step(3)
-        for w in solo:
-            w.step(3)
-        e.sync()
-        for i, (a, b) in enumerate(zip(ens_worlds, solo)):
-            assert_same(a, b, f"chamber {i} (of {counts[i]} cells)")
+        for steps in (3, 997):
+            e.step(steps)
+            for w in solo:
+                w.step(steps)
+            e.sync()
+            for i, (a, b) in enumerate(zip(ens_worlds, solo)):
+                assert_same(a, b, f"chamber {i} (of {counts[i]} cells) after {steps}")
+            print(f"256 chambers equal after {steps} more steps", flush=True)
     for w in ens_worlds + solo:
         w.close()
