@@ -36,13 +36,16 @@ __global__ __launch_bounds__(BLOCK) void k_one(typename K::Args a) {
     K::run(a, VGrid{static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x)});
 }
 
-// The chamber's arguments are copied out of device memory first: read through
-// the pointer, every field would be re-read after each store the body makes.
+// The body reads the chamber's arguments in place, through a restrict
+// pointer: the block's chamber is uniform, so the fields load into scalar
+// registers, and no store the body makes can alias them. (A local copy of the
+// arguments would have its address taken by the body's lambdas and live in
+// per-lane scratch memory.)
 template <class K>
-__global__ __launch_bounds__(BLOCK) void k_many(const typename K::Args* a, const int4* map) {
+__global__ __launch_bounds__(BLOCK) void k_many(const typename K::Args* __restrict__ a,
+                                                const int4* __restrict__ map) {
     const int4 m = map[blockIdx.x];
-    const typename K::Args args = a[m.x];
-    K::run(args, VGrid{static_cast<int>(blockIdx.x) - m.y, m.z});
+    K::run(a[m.x], VGrid{static_cast<int>(blockIdx.x) - m.y, m.z});
 }
 
 // The same wrappers with a register budget for at least W waves per SIMD.
@@ -53,10 +56,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(W, 8))) v
 
 template <class K, int W>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(W, 8))) void k_many_w(
-    const typename K::Args* a, const int4* map) {
+    const typename K::Args* __restrict__ a, const int4* __restrict__ map) {
     const int4 m = map[blockIdx.x];
-    const typename K::Args args = a[m.x];
-    K::run(args, VGrid{static_cast<int>(blockIdx.x) - m.y, m.z});
+    K::run(a[m.x], VGrid{static_cast<int>(blockIdx.x) - m.y, m.z});
 }
 
 // Host: batched launches of one ensemble step, recorded for later issue.

@@ -292,6 +292,25 @@ __device__ __forceinline__ float screen_diff(double a, double b) { return static
 constexpr float SHADOW_SCREEN = static_cast<float>(4.0 * ASTRO_CELL_RADIUS * ASTRO_CELL_RADIUS * (1.0 + 0x1p-12));
 
 // Possible shading neighbours a thread queues before the exact pass.
+// Records copied as raw 16-byte words (a local array of Rec<P> ended up in
+// per-lane scratch memory in the batched walks; vectors stay in registers).
+template <class P>
+struct RecWords {
+    uint4 w[sizeof(Rec<P>) / 16];
+};
+template <class P>
+__device__ __forceinline__ RecWords<P> load_rec(const Rec<P>* p) {
+    RecWords<P> r;
+#pragma unroll
+    for (int q = 0; q < static_cast<int>(sizeof(Rec<P>) / 16); ++q) r.w[q] = reinterpret_cast<const uint4*>(p)[q];
+    return r;
+}
+template <class P>
+__device__ __forceinline__ void store_rec(Rec<P>* p, const RecWords<P>& r) {
+#pragma unroll
+    for (int q = 0; q < static_cast<int>(sizeof(Rec<P>) / 16); ++q) reinterpret_cast<uint4*>(p)[q] = r.w[q];
+}
+
 constexpr int SHADOW_QUEUE = 8;
 
 // Possible contacts a thread queues before the exact pass (the queue is
@@ -353,16 +372,17 @@ __device__ __forceinline__ void staged_walk(const HashGeo& h, const Rec<P>* rec,
             // before any is stored; positions past the end are clamped to
             // the last one, whose record is read and written again unchanged.
             for (int v = lane; v < total; v += 4 * 64) {
-                Rec<P> t[4];
+                RecWords<P> t[4];
                 int w[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     w[u] = min(v + 64 * u, total - 1);
-                    const int j = w[u] < off[1] ? 0 : (w[u] < off[2] ? 1 : 2);
-                    t[u] = rec[lo[j] + (w[u] - off[j])];
+                    // the range holding position w[u], by selects
+                    const int shift = w[u] < off[1] ? lo[0] - off[0] : (w[u] < off[2] ? lo[1] - off[1] : lo[2] - off[2]);
+                    t[u] = load_rec(rec + (w[u] + shift));
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u) seg[w[u]] = t[u];
+                for (int u = 0; u < 4; ++u) store_rec(seg + w[u], t[u]);
             }
             wave_sync();
 #pragma unroll
@@ -538,16 +558,17 @@ struct KContactWalk {
                 // chunk is clamped to its last one, whose record is read and
                 // written again unchanged.
                 for (int v = c0 + lane; v < cend; v += 4 * 64) {
-                    Rec<P> t[4];
+                    RecWords<P> t[4];
                     int w[4];
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         w[u] = min(v + 64 * u, cend - 1);
-                        const int j = w[u] < off[1] ? 0 : (w[u] < off[2] ? 1 : 2);
-                        t[u] = rec[lo[j] + (w[u] - off[j])];
+                        const int shift =
+                            w[u] < off[1] ? lo[0] - off[0] : (w[u] < off[2] ? lo[1] - off[1] : lo[2] - off[2]);
+                        t[u] = load_rec(rec + (w[u] + shift));
                     }
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) seg[w[u] - c0] = t[u];
+                    for (int u = 0; u < 4; ++u) store_rec(seg + (w[u] - c0), t[u]);
                 }
                 wave_sync();
                 // The lane's steps whose positions lie below cend.
@@ -595,9 +616,9 @@ struct KContactWalk {
                     while (mask != 0u) {
                         const int at = pos_of(sb + __builtin_ctz(mask));
                         mask &= mask - 1u;
-                        const int j = at < off[1] ? 0 : (at < off[2] ? 1 : 2);
+                        const int shift = at < off[1] ? lo[0] - off[0] : (at < off[2] ? lo[1] - off[1] : lo[2] - off[2]);
                         if (queued == CONTACT_QUEUE) drain();
-                        queue[queued * BLOCK] = at - off[j] + lo[j];
+                        queue[queued * BLOCK] = at + shift;
                         ++queued;
                     }
                 }
