@@ -420,10 +420,11 @@ constexpr int CONTACT_ROUND = 32;
 //     multiply-adds) is within a few float roundings (relative 2^-21) of the
 //     double r^2, so a 2^-10 margin keeps every pair the reference counts as
 //     touching. A lane marks the steps that pass in a 32-bit mask, and after
-//     each round of at most 32 steps queues those records' sorted positions
-//     in walk order.
-//   * The queue is drained in order through contact_force -- when it fills,
-//     and at the end -- so F sums the same terms in the same order as the
+//     each round of at most 32 steps queues those records' slice indices in
+//     walk order.
+//   * The queue is drained in order through contact_force, reading the
+//     records from the slice -- when it fills, and at the end of each chunk
+//     -- so F sums the same terms in the same order as the
 //     reference: in double in the contract layout, bit for bit, and in float
 //     in the FP32 layout, whose force buffer and integrator are float (each
 //     term then differs from the double one by a few float roundings, at
@@ -465,17 +466,17 @@ struct KContactWalk {
                             gz);
         }
         int queued = 0;
+        // The body force the sample stored, loaded now so that its latency
+        // overlaps the walk; it is written back only if a contact was queued.
         Force<P> F{};
-        bool loaded = false;
+        if (occ) F = fs[k];
+        bool touched = false;
         // contact_force (src/sim/integrator.cuh:210-219) in the layout's width
         const P two_a_p = static_cast<P>(two_a), k_p = static_cast<P>(k_contact);
         const auto drain = [&]() {
-            if (!loaded) {
-                F = fs[k];
-                loaded = true;
-            }
+            touched = true;
             for (int j = 0; j < queued; ++j) {
-                const Rec<P> o = rec[queue[j * BLOCK]];
+                const Rec<P> o = seg[queue[j * BLOCK]];
                 const P dx = r.x - o.x;
                 const P dy = r.y - o.y;
                 const P dz = r.z - o.z;
@@ -612,21 +613,22 @@ struct KContactWalk {
                         mask |= hit_at(pos) << (m - sb);
                         ++m;
                     }
-                    // Queue the round's survivors by sorted position, in order.
+                    // Queue the round's survivors by slice index, in order.
                     while (mask != 0u) {
                         const int at = pos_of(sb + __builtin_ctz(mask));
                         mask &= mask - 1u;
-                        const int shift = at < off[1] ? lo[0] - off[0] : (at < off[2] ? lo[1] - off[1] : lo[2] - off[2]);
                         if (queued == CONTACT_QUEUE) drain();
-                        queue[queued * BLOCK] = at + shift;
+                        queue[queued * BLOCK] = at - c0;
                         ++queued;
                     }
                 }
-                wave_sync();   // the next chunk reuses the slice
+                // The queue refers to this chunk's slice: drain it before
+                // the next chunk reuses the slice.
+                if (queued > 0) drain();
+                wave_sync();
             }
         }
-        if (queued > 0) drain();
-        if (loaded) fs[k] = F;
+        if (touched) fs[k] = F;
     });
     }
 };
