@@ -112,7 +112,7 @@ struct WorldDev {
     int32_t*  dprefix;      // [cap] exclusive prefix of drop
     int32_t*  xsum;         // [tiles(cap)] per-scan-tile totals of drop
     uint32_t* dlist;        // [cap] dropped reference slots, ascending
-    unsigned long long* co2_demand;   // [CO2 grid] fixed-point demand
+    uint32_t* co2_demand;   // [CO2 grid] cells booking CO2 demand per voxel, zero between ticks
     uint32_t* tcount;       // [T grid] live cells per voxel during the thermal step, else 0
     uint8_t*  talone;       // [pcap] each entry's voxel this tick: 1 alone, 2 shared, 0 not live
     unsigned long long* stats;        // StatsAccum words (see launch_stats)

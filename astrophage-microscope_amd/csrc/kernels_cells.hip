@@ -437,6 +437,14 @@ struct KSampleBase {
 // Books one live cell's CO2 demand against its nearest voxel: demand_kernel
 // (src/sim/lifecycle.cu:41-52) for one cell. Records the voxel (NO_VOXEL if
 // none) and the CO2 it saw there, for the uptake.
+//   The reference adds each cell's fixed-point demand into a 64-bit total per
+// voxel. Every cell of one voxel reads the same concentration from the same
+// field, so every one of them asks the same fixed-point amount d, and the
+// voxel's total is its count of booking cells times d (modulo 2^64, as the
+// reference's wrapping adds). Here a cell only counts itself into a 32-bit
+// array -- half the width, so twice the voxels on each cache line, in the
+// booking, in the uptake's read and in the clear -- and the uptake forms
+// count * d from the amount it asked itself.
 constexpr uint32_t NO_VOXEL = 0xFFFFFFFFu;
 // Set on the one cell per shared voxel whose uptake deposit found the
 // accumulator empty: that cell alone flushes the voxel. Voxel indices stay
@@ -446,7 +454,7 @@ constexpr uint32_t VOX_OWNER = 0x80000000u;
 struct DemandArgs {
     Grid C;
     double dt_bio, gv;
-    unsigned long long* demand;
+    uint32_t* demand;   // booking cells per voxel
     uint32_t* vox;
     float* avail;
 };
@@ -459,7 +467,7 @@ __device__ __forceinline__ void book_demand(const DemandArgs& a, int i, bool liv
         const double want = co2_uptake_rate(static_cast<double>(cv)) * a.dt_bio;
         if (want > 0.0) {
             const int64_t fxd = to_fixed(want / a.gv, a.C.scale);
-            if (fxd != 0) atomicAdd(&a.demand[idx], static_cast<unsigned long long>(fxd));
+            if (fxd != 0) atomicAdd(&a.demand[idx], 1u);
             v = idx;
             a.avail[i] = cv;
         }
@@ -658,7 +666,11 @@ struct KUptakeMark {
                 if (idx != NO_VOXEL) {
                     const double available = static_cast<double>(avail_f);
                     const double want = co2_uptake_rate(available) * dm.dt_bio;
-                    const unsigned long long raw = dm.demand[idx];
+                    // the voxel's total: its booking cells, each asking what
+                    // this cell asked (see book_demand)
+                    const int64_t own = to_fixed(want / dm.gv, dm.C.scale);
+                    const unsigned long long raw =
+                        static_cast<unsigned long long>(dm.demand[idx]) * static_cast<unsigned long long>(own);
                     const double asked = from_fixed(raw, dm.C.scale);
                     double taken = want;
                     if (asked > available && asked > 0.0) taken = want * (available / asked);
@@ -667,7 +679,7 @@ struct KUptakeMark {
                         c.co2_held[i] = held;
                         const int64_t fxd = to_fixed(-taken / dm.gv, dm.C.scale);
                         if (fxd != 0) {
-                            if (static_cast<int64_t>(raw) == to_fixed(want / dm.gv, dm.C.scale)) {
+                            if (static_cast<int64_t>(raw) == own) {
                                 dm.C.value[idx] += static_cast<float>(
                                     from_fixed(static_cast<unsigned long long>(fxd), dm.C.scale));
                             } else {
@@ -822,7 +834,7 @@ struct KLifeApply {
                             dm.C.deposit[idx] = 0ull;
                         }
                     }
-                    dm.demand[idx] = 0ull;
+                    dm.demand[idx] = 0u;
                 }
             }
             if (s == NO_SLOT) return;

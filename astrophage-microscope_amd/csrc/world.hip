@@ -206,7 +206,7 @@ struct astro_world {
     void *trec = nullptr, *rec = nullptr;
     void* fs = nullptr;              // Force<double> or Force<float> per layout
     double* dirs = nullptr;
-    unsigned long long* co2_demand = nullptr;
+    uint32_t* co2_demand = nullptr;
     uint32_t* tcount = nullptr;   // live cells per temperature voxel, during the thermal step
     uint8_t* talone = nullptr;    // per physical entry: alone in its voxel this tick
     // The thermostat's deposit path (astro_world_set_deposit_path) and the
@@ -1238,7 +1238,7 @@ int32_t astro_world_create(const astro_world_desc* din, astro_world** out) {
         w->bsum = cv.take<int32_t>(static_cast<size_t>(scan_tiles_for(static_cast<int>(nb + 1))));
         w->dsum = cv.take<int32_t>(static_cast<size_t>(scan_tiles_for(cap)));
         w->xsum = cv.take<int32_t>(static_cast<size_t>(scan_tiles_for(cap)));
-        w->co2_demand = cv.take<unsigned long long>(std::max<size_t>(1, w->f[ASTRO_FIELD_CO2].count()));
+        w->co2_demand = cv.take<uint32_t>(std::max<size_t>(1, w->f[ASTRO_FIELD_CO2].count()));   // zeroed with the pool
         w->tcount = cv.take<uint32_t>(std::max<size_t>(1, w->f[ASTRO_FIELD_TEMPERATURE].count()));   // zeroed with the pool
         w->talone = cv.take<uint8_t>(pn);
         w->stats = cv.take<unsigned long long>(SW_COUNT);

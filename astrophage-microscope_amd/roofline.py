@@ -76,12 +76,13 @@ def kernel_bytes(name: str, c: dict) -> float:
         "feed_sample": 0.0,
         "feed_taxis_sample": 0.0,
         "integrate": integ,
-        # + the voxel and concentration each live cell books
-        "integrate_demand": integ + 4 * n + alive * 16,
-        "co2_demand": alive * (20 + 3 * p) + 4 * n,
-        # voxel, flags, t_local, lslot per cell; concentration, demand and
-        # the store per live cell
-        "uptake_mark": 16 * n + alive * 36,
+        # + the voxel and concentration each live cell books (its voxel's
+        # 4-byte count of booking cells included)
+        "integrate_demand": integ + 4 * n + alive * 12,
+        "co2_demand": alive * (16 + 3 * p) + 4 * n,
+        # voxel, flags, t_local, lslot per cell; concentration, the voxel's
+        # booking count and the store per live cell
+        "uptake_mark": 16 * n + alive * 32,
         "life_scan": 8 * n * ((1 if births > 0 else 0) + compacting),
         # the slot of every entry (flush and renumbering) and each daughter
         "life_apply": 8 * n + births * 2 * S,
