@@ -288,59 +288,84 @@ struct KSweep {
     }
 };
 
-// The same march for lines along x (axis 0), whose points are contiguous in
-// memory: one wave per 64 lines, in chunks of SWEEP_CHUNK steps. The wave
-// loads a chunk of all 64 lines coalesced (two lines per load) and turns each
-// deposit into its column transmittance, each lane then marches its own line
-// through the chunk in LDS, in the reference's order, leaving the irradiance
-// in place of the factor, and the wave stores the chunk back coalesced.
-constexpr int SWEEP_CHUNK = 32;
+// The march for lines along x (axis 0), spread over lanes. A voxel without
+// a deposit has the factor column_transmittance(0) = 1 - 0 / face = 1.0
+// exactly, and multiplying by 1.0 changes nothing, so the running product at
+// each point is the reference's sequential product of the nonzero-deposit
+// factors before it, in march order. Each line gets `seg` adjacent lanes
+// (64 / seg lines per wave), each lane four consecutive steps of every chunk
+// of 4 * seg steps:
+//   * the lanes load their deposits coalesced, clear the nonzero ones and
+//     form their four factors;
+//   * within each line, the lanes holding a nonzero factor are visited in
+//     march order: each applies its four factors to the running product in
+//     order (t *= f, as the reference), and the lanes before it up to the
+//     previous such lane take the product as it stood, untouched by factors;
+//   * every lane then writes its four points from the product entering its
+//     first step, multiplying through its own factors in order.
+// Every product is formed by the same multiplications, in the same order, as
+// the one-lane march (its extra multiplications are by 1.0), so transmittances
+// and irradiances are the same bit for bit; the march costs one visit per
+// lane that holds a deposit instead of one step per point.
+struct SweepLanesArgs {
+    SweepArgs s;
+    int seg;   // lanes per line: a power of two, 4 * seg >= nx or seg == 64
+};
 
-struct KSweepRows {
-    using Args = SweepArgs;
+struct KSweepLanes {
+    using Args = SweepLanesArgs;
     static __device__ void run(const Args& a, VGrid vg) {
-        const auto& [blocked, out, nx, ny, nz, axis, sign, scale, face, source, ambient] = a;
-        __shared__ double tile[BLOCK / 64][64][SWEEP_CHUNK + 1];
-        const int wave = static_cast<int>(threadIdx.x) >> 6, lane = static_cast<int>(threadIdx.x) & 63;
+        const auto& [blocked, out, nx, ny, nz, axis, sign, scale, face, source, ambient] = a.s;
+        const int seg = a.seg;
+        const int lane = static_cast<int>(threadIdx.x) & 63, wave = static_cast<int>(threadIdx.x) >> 6;
         const int lines = ny * nz;
-        const int base = (vg.block * (BLOCK / 64) + wave) * 64;
-        if (base >= lines) return;   // whole wave
-        double(*f)[SWEEP_CHUNK + 1] = tile[wave];
-        const int half = lane >> 5, pl = lane & 31;
+        const int first = (vg.block * (BLOCK / 64) + wave) * (64 / seg);
+        if (first >= lines) return;   // whole wave
+        const int j = lane & (seg - 1), sbase = lane - j;
+        const int line = first + lane / seg;
+        const bool on = line < lines;
+        const unsigned long long segmask = seg == 64 ? ~0ull : ((1ull << seg) - 1ull) << sbase;
         const double src = static_cast<double>(source);
-        double t = 1.0;
-        for (int c0 = 0; c0 < nx; c0 += SWEEP_CHUNK) {
-            const int pos = c0 + pl;
-            const size_t step = static_cast<size_t>(sign > 0 ? pos : nx - 1 - pos);
-            // all 32 loads of the chunk first, then the clears and factors
-            unsigned long long raw[32];
+        double tline = 1.0;   // the line's running product, the same in all its lanes
+        for (int c0 = 0; c0 < nx; c0 += 4 * seg) {
+            unsigned long long raw[4];
+            size_t idx[4];
+            bool ok[4];
 #pragma unroll
-            for (int q = 0; q < 32; ++q) {
-                const int line = base + 2 * q + half;
-                raw[q] = (line < lines && pos < nx) ? blocked[static_cast<size_t>(line) * nx + step] : 0ull;
+            for (int q = 0; q < 4; ++q) {
+                const int pos = c0 + 4 * j + q;
+                ok[q] = on && pos < nx;
+                idx[q] = static_cast<size_t>(line) * nx + static_cast<size_t>(sign > 0 ? pos : nx - 1 - pos);
+                raw[q] = ok[q] ? blocked[idx[q]] : 0ull;
             }
+            double f[4];
 #pragma unroll
-            for (int q = 0; q < 32; ++q) {
-                const int r = 2 * q + half, line = base + r;
-                if (line < lines && pos < nx) {
-                    if (raw[q] != 0ull) blocked[static_cast<size_t>(line) * nx + step] = 0ull;
-                    f[r][pl] = column_transmittance(from_fixed(raw[q], scale), face);
+            for (int q = 0; q < 4; ++q) {
+                if (ok[q] && raw[q] != 0ull) blocked[idx[q]] = 0ull;
+                f[q] = column_transmittance(from_fixed(raw[q], scale), face);
+            }
+            const bool holds = (raw[0] | raw[1] | raw[2] | raw[3]) != 0ull;
+            unsigned long long pending = __ballot(holds) & segmask;
+            int prev = sbase - 1;   // last lane of the line already visited
+            double tin = tline;     // the product entering this lane's first step
+            while (__ballot(pending != 0ull) != 0ull) {
+                const int b = pending ? __ffsll(static_cast<long long>(pending)) - 1 : lane;
+                const double chain = ((tline * f[0]) * f[1] * f[2]) * f[3];
+                const double next = __shfl(chain, b, 64);
+                if (pending) {
+                    if (lane > prev && lane <= b) tin = tline;
+                    tline = next;
+                    prev = b;
+                    pending &= pending - 1ull;
                 }
             }
-            wave_sync();
-            const int steps = min(SWEEP_CHUNK, nx - c0);
-            for (int p = 0; p < steps; ++p) {
-                const double fac = f[lane][p];
-                *reinterpret_cast<float*>(&f[lane][p]) = static_cast<float>(src * t) + ambient;
-                t *= fac;
+            if (lane > prev) tin = tline;
+            double t = tin;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (ok[q]) out[idx[q]] = static_cast<float>(src * t) + ambient;
+                t *= f[q];
             }
-            wave_sync();
-            for (int rr = 0; rr < 64; rr += 2) {
-                const int r = rr + half, line = base + r;
-                if (line < lines && pos < nx)
-                    out[static_cast<size_t>(line) * nx + step] = *reinterpret_cast<const float*>(&f[r][pl]);
-            }
-            wave_sync();
         }
     }
 };
@@ -477,17 +502,24 @@ void launch_stamp(const Launch& L, Views<P> ws) {
 // through the LDS-transposing kernel.
 void launch_sweep(const Launch& L, std::span<const Grid> irr, Ticks ts) {
     Batch<KSweep> cols;
-    Batch<KSweepRows> rows;
+    Batch<KSweepLanes> rows;
     for (size_t i = 0; i < irr.size(); ++i) {
         const Grid& g = irr[i];
         const TickParams& t = ts[i];
         const SweepArgs a{g.deposit, g.value, g.nx, g.ny, g.nz, t.sweep_axis, t.sweep_sign, g.scale, t.face_area,
                           t.source, t.ambient_irr};
         const int lines = (t.sweep_axis == 0 ? g.ny : g.nx) * g.nz;
-        if (t.sweep_axis == 0) rows.add(a, (lines + BLOCK - 1) / BLOCK);
-        else cols.add(a, (lines + BLOCK - 1) / BLOCK);
+        if (t.sweep_axis == 0) {
+            // lanes per line: the fewest whose four steps each span the line, at most a wave
+            int seg = 1;
+            while (seg < 64 && 4 * seg < g.nx) seg *= 2;
+            const int per_block = (BLOCK / 64) * (64 / seg);
+            rows.add(SweepLanesArgs{a, seg}, (lines + per_block - 1) / per_block);
+        } else {
+            cols.add(a, (lines + BLOCK - 1) / BLOCK);
+        }
     }
-    if (!rows.args.empty()) launch_k<KSweepRows>(L, "light_sweep", rows.args, rows.blocks);
+    if (!rows.args.empty()) launch_k<KSweepLanes>(L, "light_sweep", rows.args, rows.blocks);
     if (!cols.args.empty()) launch_k<KSweep>(L, "light_sweep", cols.args, cols.blocks);
 }
 
