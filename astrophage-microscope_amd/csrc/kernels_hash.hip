@@ -727,6 +727,123 @@ struct KFeedWalk {
     }
 };
 
+// shadow_fraction's factor (src/sim/emission.cuh:30-37) for neighbour o of
+// the cell at (px, py, pz): 1 - the overlap of the two discs seen along the
+// light, 1.0 for a neighbour downstream.
+template <class P>
+__device__ __forceinline__ double shadow_factor(double px, double py, double pz, const Rec<P>& o, double ldx,
+                                                double ldy, double ldz) {
+    const double dx = px - static_cast<double>(o.x);
+    const double dy = py - static_cast<double>(o.y);
+    const double dz = pz - static_cast<double>(o.z);
+    const double along = dx * ldx + dy * ldy + dz * ldz;
+    double frac = 0.0;
+    if (along > 0.0) {
+        const double perp2 = (dx * dx + dy * dy + dz * dz) - along * along;
+        const double perp = perp2 > 0.0 ? sqrt(perp2) : 0.0;
+        frac = disc_overlap_fraction(perp, ASTRO_CELL_RADIUS);
+    }
+    return 1.0 - frac;
+}
+
+// The feed walk for crowded worlds (several cells per hash bucket, so each
+// cell screens on the order of a hundred neighbours): one 16-lane group per
+// cell, lane r < 9 walking row r of the 27-bucket neighbourhood (the rows in
+// the reference's walk order, each in sorted position order). A lane screens
+// its row as KFeedWalk does and evaluates its survivors' factors in double,
+// keeping them in order in LDS; the group's first lane then multiplies all of
+// them into `open` row after row, in the same order and with the same factors
+// as the one-lane walk, so the product is the same bit for bit. A row with
+// more survivors than the LDS keeps is walked again, in order, by the first
+// lane itself. The neighbour records are read from the hash array directly:
+// a crowded world's records fit in the L2.
+constexpr int FEED_GROUP = 16;
+constexpr int FEED_ROW_CAP = 24;
+
+template <class P>
+struct KFeedWalkRows {
+    static constexpr bool stride = true;   // any block count covers the work
+    using Args = typename KFeedWalk<P>::Args;
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [ctr, h, rec, bstart, irr, ldx, ldy, ldz, ambient, occl, dirs] = a;
+        constexpr int G = FEED_GROUP, CELLS = BLOCK / FEED_GROUP;
+        __shared__ double s_f[CELLS][9][FEED_ROW_CAP];
+        __shared__ int s_n[CELLS][9];
+        const float lxf = static_cast<float>(ldx), lyf = static_cast<float>(ldy), lzf = static_cast<float>(ldz);
+        const int j = static_cast<int>(threadIdx.x) % G, grp = static_cast<int>(threadIdx.x) / G;
+        const int n = ctr->count;
+        for (int k0 = vg.block * CELLS; k0 < n; k0 += vg.grid * CELLS) {   // uniform over the block
+            const int k = k0 + grp;
+            Rec<P> r{};
+            bool lit = false;
+            if (k < n) {
+                r = rec[k];
+                lit = !(r.meta & REC_UNOCC) && (r.meta & REC_ALIVE);
+            }
+            const uint32_t self = r.meta & REC_SLOT_MASK;
+            const double px = static_cast<double>(r.x), py = static_cast<double>(r.y), pz = static_cast<double>(r.z);
+            // the row's range, as row_ranges computes it
+            const auto row_range = [&](int row, int& s, int& e) {
+                int gx, gy, gz;
+                hash_coords(h, px, py, pz, gx, gy, gz);
+                const int xlo = gx > 0 ? gx - 1 : 0;
+                const int xhi = gx < h.nx - 1 ? gx + 1 : h.nx - 1;
+                const int cz = gz + row / 3 - 1, cy = gy + row % 3 - 1;
+                const bool ok = cz >= 0 && cz < h.nz && cy >= 0 && cy < h.ny;
+                const int base = ok ? (cz * h.ny + cy) * h.nx : 0;
+                s = bstart[base + xlo];
+                e = ok ? bstart[base + xhi + 1] : s;
+            };
+            // the screen of KFeedWalk: true for a possible shading neighbour
+            const auto screen = [&](const Rec<P>& o) {
+                if ((o.meta & REC_SLOT_MASK) == self || !(o.meta & REC_ALIVE)) return false;
+                const float sx = screen_diff(r.x, o.x), sy = screen_diff(r.y, o.y), sz = screen_diff(r.z, o.z);
+                const float sa = sx * lxf + sy * lyf + sz * lzf;
+                if (sa < -0x1p-16f * (fabsf(sx) + fabsf(sy) + fabsf(sz))) return false;
+                const float s2 = sx * sx + sy * sy + sz * sz;
+                return !(s2 - sa * sa > SHADOW_SCREEN + 0x1p-14f * s2);
+            };
+            if (occl && lit && j < 9) {
+                int s, e, cnt = 0;
+                row_range(j, s, e);
+                for (int q = s; q < e; ++q) {
+                    const Rec<P> o = rec[q];
+                    if (!screen(o)) continue;
+                    if (cnt < FEED_ROW_CAP) s_f[grp][j][cnt] = shadow_factor(px, py, pz, o, ldx, ldy, ldz);
+                    ++cnt;
+                }
+                s_n[grp][j] = cnt;
+            }
+            __syncthreads();
+            if (j == 0 && lit) {
+                double directional = static_cast<double>(grid_sample(irr, static_cast<float>(r.x),
+                                                                     static_cast<float>(r.y), static_cast<float>(r.z))) -
+                                     static_cast<double>(ambient);
+                if (directional < 0.0) directional = 0.0;
+                if (occl) {
+                    double open = 1.0;
+                    for (int row = 0; row < 9; ++row) {
+                        const int cnt = s_n[grp][row];
+                        if (cnt <= FEED_ROW_CAP) {
+                            for (int i = 0; i < cnt; ++i) open *= s_f[grp][row][i];
+                        } else {
+                            int s, e;
+                            row_range(row, s, e);
+                            for (int q = s; q < e; ++q) {
+                                const Rec<P> o = rec[q];
+                                if (screen(o)) open *= shadow_factor(px, py, pz, o, ldx, ldy, ldz);
+                            }
+                        }
+                    }
+                    directional *= open;
+                }
+                dirs[k] = directional;
+            }
+            __syncthreads();   // the slots are reused by the next cells
+        }
+    }
+};
+
 }  // namespace
 
 template <class P>
@@ -778,22 +895,25 @@ void launch_contact_walk(const Launch& L, Views<P> ws, Ticks ts) {
 }
 
 template <class P>
-void launch_feed_walk(const Launch& L, Views<P> ws, Ticks ts) {
+void launch_feed_walk(const Launch& L, Views<P> ws, Ticks ts, bool crowded) {
     Batch<KFeedWalk<P>> b;
     for (size_t i = 0; i < ws.size(); ++i) {
         const WorldDev<P>& w = ws[i];
         const TickParams& t = ts[i];
         b.add({w.ctr, w.hg, w.rec, w.bstart, w.g[ASTRO_FIELD_IRRADIANCE], t.ldx, t.ldy, t.ldz, t.ambient_irr,
                t.occlusion_exact, w.dirs},
-              cell_grid(w.capacity));
+              // crowded: a block per BLOCK / FEED_GROUP cells, grid-stride past 16384 blocks
+              crowded ? blocks_for_xcd(std::min((w.capacity + BLOCK / FEED_GROUP - 1) / (BLOCK / FEED_GROUP), 16384))
+                      : cell_grid(w.capacity));
     }
-    launch_k<KFeedWalk<P>>(L, "feed_walk", b.args, b.blocks);
+    if (crowded) launch_k<KFeedWalkRows<P>>(L, "feed_walk", b.args, b.blocks);
+    else launch_k<KFeedWalk<P>>(L, "feed_walk", b.args, b.blocks);
 }
 
 #define INSTANTIATE(P)                                                         \
     template void launch_hash_build<P>(const Launch&, Views<P>, bool);         \
     template void launch_contact_walk<P>(const Launch&, Views<P>, Ticks);      \
-    template void launch_feed_walk<P>(const Launch&, Views<P>, Ticks);         \
+    template void launch_feed_walk<P>(const Launch&, Views<P>, Ticks, bool);   \
     template void launch_resort<P>(const Launch&, Views<P>);
 INSTANTIATE(double)
 INSTANTIATE(float)

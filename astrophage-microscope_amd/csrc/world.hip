@@ -459,7 +459,13 @@ void enqueue_light(WorldSpan ws, const Launch& L, Ticks ts) {
     }
     if (!stamped.empty()) launch_stamp<P>(L, stamped);
     launch_sweep(L, irr, ts);
-    launch_feed_walk<P>(L, d, ts);
+    // crowded: at least one live cell per hash bucket on average
+    int64_t live = 0, buckets = 0;
+    for (astro_world* w : ws) {
+        live += w->live_hint;
+        buckets += w->hg.nb;
+    }
+    launch_feed_walk<P>(L, d, ts, live >= buckets);
 }
 
 // One tick of world_step (src/sim/step.cu:109-184). Host clock untouched.
