@@ -167,10 +167,10 @@ struct Ftcs {
 template <class P> void launch_hash_build(const Launch& L, Views<P> ws, bool count_tick);
 // The contact half of forces_kernel over the sorted records, adding to fs.
 template <class P> void launch_contact_walk(const Launch& L, Views<P> ws, Ticks ts);
-// `crowded` walks each cell's neighbourhood with a 16-lane group (one lane
-// per row) instead of one lane: the same result, faster where cells screen
-// many neighbours each.
-template <class P> void launch_feed_walk(const Launch& L, Views<P> ws, Ticks ts, bool crowded);
+// A world with crowded[i] set has each cell's neighbourhood walked by a
+// 16-lane group (one lane per row) instead of one lane: the same result,
+// faster where cells screen many neighbours each.
+template <class P> void launch_feed_walk(const Launch& L, Views<P> ws, Ticks ts, const std::vector<char>& crowded);
 // Re-sorts the SoA into bucket order when due (device-side decision).
 template <class P> void launch_resort(const Launch& L, Views<P> ws);
 

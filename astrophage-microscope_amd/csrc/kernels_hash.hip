@@ -895,25 +895,27 @@ void launch_contact_walk(const Launch& L, Views<P> ws, Ticks ts) {
 }
 
 template <class P>
-void launch_feed_walk(const Launch& L, Views<P> ws, Ticks ts, bool crowded) {
-    Batch<KFeedWalk<P>> b;
+void launch_feed_walk(const Launch& L, Views<P> ws, Ticks ts, const std::vector<char>& crowded) {
+    Batch<KFeedWalk<P>> sparse, dense;
     for (size_t i = 0; i < ws.size(); ++i) {
         const WorldDev<P>& w = ws[i];
         const TickParams& t = ts[i];
-        b.add({w.ctr, w.hg, w.rec, w.bstart, w.g[ASTRO_FIELD_IRRADIANCE], t.ldx, t.ldy, t.ldz, t.ambient_irr,
-               t.occlusion_exact, w.dirs},
-              // crowded: a block per BLOCK / FEED_GROUP cells, grid-stride past 16384 blocks
-              crowded ? blocks_for_xcd(std::min((w.capacity + BLOCK / FEED_GROUP - 1) / (BLOCK / FEED_GROUP), 16384))
-                      : cell_grid(w.capacity));
+        const typename KFeedWalk<P>::Args a{w.ctr, w.hg, w.rec, w.bstart, w.g[ASTRO_FIELD_IRRADIANCE],
+                                            t.ldx, t.ldy, t.ldz, t.ambient_irr, t.occlusion_exact, w.dirs};
+        // crowded: a block per BLOCK / FEED_GROUP cells, grid-stride past 16384 blocks
+        if (crowded[i])
+            dense.add(a, blocks_for_xcd(std::min((w.capacity + BLOCK / FEED_GROUP - 1) / (BLOCK / FEED_GROUP), 16384)));
+        else
+            sparse.add(a, cell_grid(w.capacity));
     }
-    if (crowded) launch_k<KFeedWalkRows<P>>(L, "feed_walk", b.args, b.blocks);
-    else launch_k<KFeedWalk<P>>(L, "feed_walk", b.args, b.blocks);
+    if (!dense.args.empty()) launch_k<KFeedWalkRows<P>>(L, "feed_walk", dense.args, dense.blocks);
+    if (!sparse.args.empty()) launch_k<KFeedWalk<P>>(L, "feed_walk", sparse.args, sparse.blocks);
 }
 
 #define INSTANTIATE(P)                                                         \
     template void launch_hash_build<P>(const Launch&, Views<P>, bool);         \
     template void launch_contact_walk<P>(const Launch&, Views<P>, Ticks);      \
-    template void launch_feed_walk<P>(const Launch&, Views<P>, Ticks, bool);   \
+    template void launch_feed_walk<P>(const Launch&, Views<P>, Ticks, const std::vector<char>&); \
     template void launch_resort<P>(const Launch&, Views<P>);
 INSTANTIATE(double)
 INSTANTIATE(float)

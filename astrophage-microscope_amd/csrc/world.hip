@@ -459,13 +459,14 @@ void enqueue_light(WorldSpan ws, const Launch& L, Ticks ts) {
     }
     if (!stamped.empty()) launch_stamp<P>(L, stamped);
     launch_sweep(L, irr, ts);
-    // crowded: at least one live cell per hash bucket on average
-    int64_t live = 0, buckets = 0;
-    for (astro_world* w : ws) {
-        live += w->live_hint;
-        buckets += w->hg.nb;
-    }
-    launch_feed_walk<P>(L, d, ts, live >= buckets);
+    // crowded: at least one live cell per hash bucket on average, in a launch
+    // whose sorted records all fit in one XCD's 4 MiB L2 (the row walk reads
+    // its neighbours from global memory, one dependent load after another)
+    int64_t rec_bytes = 0;
+    for (astro_world* w : ws) rec_bytes += static_cast<int64_t>(w->capacity) * static_cast<int64_t>(sizeof(Rec<P>));
+    std::vector<char> crowded;
+    for (astro_world* w : ws) crowded.push_back(w->live_hint >= w->hg.nb && rec_bytes <= (int64_t{4} << 20));
+    launch_feed_walk<P>(L, d, ts, crowded);
 }
 
 // One tick of world_step (src/sim/step.cu:109-184). Host clock untouched.
