@@ -251,37 +251,23 @@ struct KResortCopyback {
     }
 };
 
-// The record ranges of the 27 buckets around bucket (gx, gy, gz), as nine
-// rows in the reference's walk order (z, then y, then x, then sorted position,
-// src/sim/hash.cuh:88-111). The three x-adjacent buckets of one (y, z) row are
-// contiguous in the sorted array, so each row is a single range
-// [bstart[x_lo], bstart[x_hi + 1]); all eighteen loads are issued together,
-// unconditionally (a row off the grid, or any row of an inactive lane, reads
-// row 0 and is made empty).
-struct RowRanges {
-    int32_t s[9], e[9];
-};
-
-template <class Start>
-__device__ __forceinline__ RowRanges row_ranges(const HashGeo& h, Start&& start, bool active, int gx, int gy,
-                                                int gz) {
-    RowRanges rr;
+// Record range [s, e) of row `row` (0..8) of the 27 buckets around bucket
+// (gx, gy, gz), the rows in the reference's walk order (z, then y, then x,
+// then sorted position, src/sim/hash.cuh:88-111). The three x-adjacent
+// buckets of one (y, z) row are contiguous in the sorted array, so a row is
+// the single range [bstart[x_lo], bstart[x_hi + 1]); a row off the grid is
+// empty.
+__device__ __forceinline__ void row_range(const HashGeo& h, const int32_t* bstart, int gx, int gy, int gz, int row,
+                                          int& s, int& e) {
     const int xlo = gx > 0 ? gx - 1 : 0;
     const int xhi = gx < h.nx - 1 ? gx + 1 : h.nx - 1;
-#pragma unroll
-    for (int r = 0; r < 9; ++r) {
-        const int cz = gz + r / 3 - 1, cy = gy + r % 3 - 1;
-        const bool ok = active && cz >= 0 && cz < h.nz && cy >= 0 && cy < h.ny;
-        const int row = ok ? (cz * h.ny + cy) * h.nx : 0;
-        const int32_t s = start(r / 3, row + xlo), e = start(r / 3, row + xhi + 1);
-        rr.s[r] = s;
-        rr.e[r] = ok ? e : s;
-    }
-    return rr;
+    const int cz = gz + row / 3 - 1, cy = gy + row % 3 - 1;
+    const bool ok = cz >= 0 && cz < h.nz && cy >= 0 && cy < h.ny;
+    const int base = ok ? (cz * h.ny + cy) * h.nx : 0;
+    s = bstart[base + xlo];
+    e = ok ? bstart[base + xhi + 1] : s;
 }
 
-// visit(d, q) for every sorted record index q of the walk, in order; d is the
-// row's z-offset range (0, 1, 2 for dz = -1, 0, +1).
 // Position difference, as the float the contact screen uses: exact to one
 // float rounding of the double difference the reference computes.
 __device__ __forceinline__ float screen_diff(float a, float b) { return a - b; }
@@ -291,7 +277,22 @@ __device__ __forceinline__ float screen_diff(double a, double b) { return static
 // distance from the light line.
 constexpr float SHADOW_SCREEN = static_cast<float>(4.0 * ASTRO_CELL_RADIUS * ASTRO_CELL_RADIUS * (1.0 + 0x1p-12));
 
-// Possible shading neighbours a thread queues before the exact pass.
+// The shadow walk's float screen: true for a live neighbour o that may shade
+// the cell r. A neighbour downstream of the cell (along <= 0) or at least 2a
+// off its light line has an overlap of exactly 0 and multiplies the open
+// fraction by exactly 1. The float along and perp^2 differ from the double
+// ones by under 2^-21 (|dx|+|dy|+|dz|) and 2^-18 |d|^2, so these margins never
+// drop a shading neighbour.
+template <class P>
+__device__ __forceinline__ bool shadow_hit(const Rec<P>& r, const Rec<P>& o, float lxf, float lyf, float lzf) {
+    const float sx = screen_diff(r.x, o.x), sy = screen_diff(r.y, o.y), sz = screen_diff(r.z, o.z);
+    const float sa = sx * lxf + sy * lyf + sz * lzf;
+    const float s2 = sx * sx + sy * sy + sz * sz;
+    const bool upstream = !(sa < -0x1p-16f * (fabsf(sx) + fabsf(sy) + fabsf(sz)));
+    const bool near_line = !(s2 - sa * sa > SHADOW_SCREEN + 0x1p-14f * s2);
+    return ((o.meta & REC_ALIVE) != 0) & upstream & near_line;
+}
+
 // Records copied as raw 16-byte words (a local array of Rec<P> ended up in
 // per-lane scratch memory in the batched walks; vectors stay in registers).
 template <class P>
@@ -311,6 +312,7 @@ __device__ __forceinline__ void store_rec(Rec<P>* p, const RecWords<P>& r) {
     for (int q = 0; q < static_cast<int>(sizeof(Rec<P>) / 16); ++q) reinterpret_cast<uint4*>(p)[q] = r.w[q];
 }
 
+// Possible shading neighbours a thread queues before the exact pass.
 constexpr int SHADOW_QUEUE = 8;
 
 // Possible contacts a thread queues before the exact pass (the queue is
@@ -322,28 +324,55 @@ constexpr int CONTACT_QUEUE = 4;
 template <class P>
 constexpr int walk_seg() { return (6 * 1024) / static_cast<int>(sizeof(Rec<P>)); }
 
-// The neighbour walk of the whole wave: each active lane visits the records
-// of the 27 buckets around its own (gx, gy, gz) in the reference's walk order
-// (visit(o, q), o the record at sorted position q). Every lane of the wave
-// must call it. The 64 consecutive records of a wave have keys in [kmin,
-// kmax], and keys are linear in (z, y, x), so row r of every lane's walk lies
-// in one contiguous record range (about 64 records); for each z offset the
-// wave copies its three rows' ranges into its LDS slice `seg` with coalesced
-// loads and every lane walks there. A range that does not fit is walked in
-// global memory; the visit order is the same either way.
-template <class P, class Visit>
-__device__ __forceinline__ void staged_walk(const HashGeo& h, const Rec<P>* rec, const int32_t* bstart, Rec<P>* seg,
-                                            bool active, int gx, int gy, int gz, Visit&& visit) {
+// Steps of its walk a lane screens per round: one bit of its survivor mask
+// each.
+constexpr int WALK_ROUND = 32;
+
+// The neighbour walk of a whole wave, in masked rounds. Each active lane
+// visits the records of the 27 buckets around its own (gx, gy, gz) in the
+// reference's walk order (z, y, x, then sorted position,
+// src/sim/hash.cuh:88-111), skipping its own record (sorted position k);
+// every lane of the wave must call it.
+//   * Per z offset the wave stages the records of its three rows into its
+//     LDS slice `seg`: the 64 consecutive records of a wave have keys in
+//     [kmin, kmax], and keys are linear in (z, y, x), so row r of every
+//     lane's walk lies in one contiguous record range. The three ranges sit
+//     end to end ("positions" 0 .. total) and are copied in chunks of
+//     walk_seg records with coalesced loads. Each lane then walks its own
+//     three rows back to back in one loop; the wave runs as many iterations
+//     as its longest three-row walk, and the loop body has no branch.
+//   * hit(o) screens neighbour o. A lane marks the steps that pass in a
+//     32-bit mask, and after each round of at most WALK_ROUND steps queues
+//     those records' slice indices in walk order (QUEUE entries per lane,
+//     entry j at queue[j * BLOCK]).
+//   * term(o) takes the queued records in walk order, reading them from the
+//     slice: when the queue fills, and at the end of each chunk, before the
+//     next chunk reuses the slice. So a lane sees its survivors in the order
+//     of the reference's walk.
+// Returns true if the lane queued any record.
+template <class P, int QUEUE, class Hit, class Term>
+__device__ __forceinline__ bool masked_walk(const HashGeo& h, const Rec<P>* rec, const int32_t* bstart, Rec<P>* seg,
+                                            int32_t* queue, bool active, int k, int gx, int gy, int gz, Hit&& hit,
+                                            Term&& term) {
     constexpr int SEG = walk_seg<P>();
     const int lane = static_cast<int>(threadIdx.x) & 63;
-    const RowRanges rr = row_ranges(h, [&](int, int b) { return bstart[b]; }, active, gx, gy, gz);
+    int queued = 0;
+    bool any = false;
+    const auto drain = [&]() {
+        any = true;
+        for (int j = 0; j < queued; ++j) term(seg[queue[j * BLOCK]]);
+        queued = 0;
+    };
     // Row (dz, dy) of any lane lies in buckets [kmin + o - 1, kmax + o + 1],
-    // o = dz*nx*ny + dy*nx. Lane m < 18 fetches the bucket start of bound m
-    // (the start of row m/2's range, or the end), and the wave reads them
-    // back by lane. The wave's records are consecutive in bucket order, so
-    // the keys of its active lanes do not decrease with the lane: the first
-    // and last active lanes hold kmin and kmax.
+    // o = dz*nx*ny + dy*nx. Lane b < 18 fetches the bucket start of bound
+    // b (the start of row b/2's range, or the end), and the wave reads
+    // them back by lane.
+    const int xlo = gx > 0 ? gx - 1 : 0;
+    const int xhi = gx < h.nx - 1 ? gx + 1 : h.nx - 1;
     const int key = (gz * h.ny + gy) * h.nx + gx;
+    // The wave's records are consecutive in bucket order, unoccupied ones
+    // (the sentinel bucket) last: its active lanes hold keys that do not
+    // decrease with the lane, so the first and last of them bound the rest.
     const unsigned long long act = __ballot(active);
     const int kmin = act ? __builtin_amdgcn_readlane(key, __ffsll(static_cast<long long>(act)) - 1) : 0x7FFFFFFF;
     const int kmax = act ? __builtin_amdgcn_readlane(key, 63 - __clzll(static_cast<long long>(act))) : -1;
@@ -355,377 +384,116 @@ __device__ __forceinline__ void staged_walk(const HashGeo& h, const Rec<P>* rec,
         b = b < 0 ? 0 : (b > h.nb ? h.nb : b);
         bound = bstart[b];
     }
-#pragma unroll
+#pragma unroll 1
     for (int d = 0; d < 3; ++d) {
+        // The wave's three ranges [lo[j], lo[j] + len) at positions off[j].
         int lo[3], off[3], total = 0;
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            const int row = 3 * d + j;
-            const int a = __builtin_amdgcn_readlane(bound, 2 * row);
-            const int b = __builtin_amdgcn_readlane(bound, 2 * row + 1);
-            lo[j] = a;
+            const int ra = __builtin_amdgcn_readlane(bound, 6 * d + 2 * j);
+            const int rb = __builtin_amdgcn_readlane(bound, 6 * d + 2 * j + 1);
+            lo[j] = ra;
             off[j] = total;
-            total += b > a ? b - a : 0;
+            total += rb > ra ? rb - ra : 0;
         }
-        if (total <= SEG) {
-            // The three ranges end to end, up to four records per lane loaded
-            // before any is stored; positions past the end are clamped to
-            // the last one, whose record is read and written again unchanged.
-            for (int v = lane; v < total; v += 4 * 64) {
+        // This lane's rows (dz = d - 1, dy = j - 1), as row_ranges computes
+        // them: nr[j] records from position p[j]. Each lies inside the
+        // wave's range of its row, so positions rise along the lane's walk.
+        // The six loads are unconditional (a row off the grid reads row 0,
+        // which is in bounds, and counts no records), so they issue together
+        // with no branch.
+        int p[3], nr[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int cz = gz + d - 1, cy = gy + j - 1;
+            const bool ok = active && cz >= 0 && cz < h.nz && cy >= 0 && cy < h.ny;
+            const int row = ok ? (cz * h.ny + cy) * h.nx : 0;
+            const int rs = bstart[row + xlo];
+            const int re = bstart[row + xhi + 1];
+            nr[j] = ok && re > rs ? re - rs : 0;
+            p[j] = off[j] + (rs - lo[j]);
+        }
+        const int n01 = nr[0] + nr[1];
+        // The lane's own record, in the middle row of the middle z offset.
+        const int selfpos = (d == 1 && active) ? off[1] + (k - lo[1]) : -1;
+        // Position of step m of the lane's walk.
+        const auto pos_of = [&](int m) {
+            return m < nr[0] ? p[0] + m : (m < n01 ? p[1] + (m - nr[0]) : p[2] + (m - n01));
+        };
+        int m = 0;
+        for (int c0 = 0; c0 < total; c0 += SEG) {
+            const int cend = total - c0 < SEG ? total : c0 + SEG;
+            // Up to four records per lane are loaded before any is stored,
+            // so the copy costs one memory latency, not four. Loads and
+            // stores are unconditional: a position past the chunk is clamped
+            // to its last one, whose record is read and written again
+            // unchanged.
+            for (int v = c0 + lane; v < cend; v += 4 * 64) {
                 RecWords<P> t[4];
                 int w[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    w[u] = min(v + 64 * u, total - 1);
-                    // the range holding position w[u], by selects
-                    const int shift = w[u] < off[1] ? lo[0] - off[0] : (w[u] < off[2] ? lo[1] - off[1] : lo[2] - off[2]);
+                    w[u] = min(v + 64 * u, cend - 1);
+                    const int shift =
+                        w[u] < off[1] ? lo[0] - off[0] : (w[u] < off[2] ? lo[1] - off[1] : lo[2] - off[2]);
                     t[u] = load_rec(rec + (w[u] + shift));
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u) store_rec(seg + w[u], t[u]);
+                for (int u = 0; u < 4; ++u) store_rec(seg + (w[u] - c0), t[u]);
             }
             wave_sync();
+            // The lane's steps whose positions lie below cend.
+            int mc = 0;
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
-                const int row = 3 * d + j;
-                for (int32_t q = rr.s[row]; q < rr.e[row]; ++q) visit(seg[off[j] + (q - lo[j])], q);
+                const int below = cend - p[j];
+                mc += below < 0 ? 0 : (below < nr[j] ? below : nr[j]);
             }
-            wave_sync();   // the next range reuses the slice
-        } else {
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                const int row = 3 * d + j;
-                for (int32_t q = rr.s[row]; q < rr.e[row]; ++q) visit(rec[q], q);
-            }
-        }
-    }
-}
-
-// Steps of its walk a lane screens per round: one bit of its survivor mask
-// each.
-constexpr int CONTACT_ROUND = 32;
-
-// The contact half of forces_kernel (src/sim/integrator.cu:60-67), one lane
-// per sorted record, adding to the body force already in fs[k]. Neighbour
-// positions come from the hash's records, fixed since the hash build.
-//   * Per z offset the wave stages the records of its three rows into LDS,
-//     as staged_walk does: the rows' ranges end to end ("positions" 0 ..
-//     total), in chunks of walk_seg records. Each lane then walks its own
-//     three rows back to back in one loop, in the reference's order (z, y,
-//     x, then sorted position, src/sim/hash.cuh:88-111), skipping its own
-//     record; the wave runs as many iterations as its longest three-row
-//     walk, and the loop body has no branch.
-//   * Each neighbour is screened in float: the float r^2 of a pair (the
-//     differences rounded to float, squared and summed with fused
-//     multiply-adds) is within a few float roundings (relative 2^-21) of the
-//     double r^2, so a 2^-10 margin keeps every pair the reference counts as
-//     touching. A lane marks the steps that pass in a 32-bit mask, and after
-//     each round of at most 32 steps queues those records' slice indices in
-//     walk order.
-//   * The queue is drained in order through contact_force, reading the
-//     records from the slice -- when it fills, and at the end of each chunk
-//     -- so F sums the same terms in the same order as the
-//     reference: in double in the contract layout, bit for bit, and in float
-//     in the FP32 layout, whose force buffer and integrator are float (each
-//     term then differs from the double one by a few float roundings, at
-//     most about k * 2^-21 * 2a for pairs at the contact threshold). Pairs the
-//     exact test finds apart add a zero vector; F's components are sums of
-//     nonzero terms onto a nonzero or +0.0 start and so never -0.0, which
-//     makes those adds exact no-ops.
-template <class P>
-struct KContactWalk {
-    static constexpr bool stride = true;   // any block count covers the work
-    struct Args {
-        const Counters* ctr;
-        HashGeo h;
-        const Rec<P>* rec;
-        const int32_t* bstart;
-        Force<P>* fs;
-        double k_contact;
-    };
-    static __device__ void run(const Args& a, VGrid vg) {
-        const auto& [ctr, h, rec, bstart, fs, k_contact] = a;
-    constexpr int SEG = walk_seg<P>();
-    __shared__ Rec<P> s_seg[BLOCK / 64][SEG];
-    __shared__ int32_t s_queue[CONTACT_QUEUE * BLOCK];
-    Rec<P>* seg = s_seg[threadIdx.x >> 6];
-    int32_t* queue = s_queue + threadIdx.x;   // entry j at queue[j * BLOCK]
-    const int lane = static_cast<int>(threadIdx.x) & 63;
-    const int n = ctr->count;
-    const double two_a = ASTRO_CELL_DIAMETER;
-    const float screen = static_cast<float>(two_a * two_a) * (1.0f + 1.0f / 1024.0f);
-    xcd_tiles(vg, n, [&](const int k, const bool valid) {
-        Rec<P> r{};
-        bool occ = false;
-        int gx = 0, gy = 0, gz = 0;
-        if (valid) {
-            r = rec[k];
-            occ = !(r.meta & REC_UNOCC);
-            if (occ)
-                hash_coords(h, static_cast<double>(r.x), static_cast<double>(r.y), static_cast<double>(r.z), gx, gy,
-                            gz);
-        }
-        int queued = 0;
-        // The body force the sample stored, loaded now so that its latency
-        // overlaps the walk; it is written back only if a contact was queued.
-        Force<P> F{};
-        if (occ) F = fs[k];
-        bool touched = false;
-        // contact_force (src/sim/integrator.cuh:210-219) in the layout's width
-        const P two_a_p = static_cast<P>(two_a), k_p = static_cast<P>(k_contact);
-        const auto drain = [&]() {
-            touched = true;
-            for (int j = 0; j < queued; ++j) {
-                const Rec<P> o = seg[queue[j * BLOCK]];
-                const P dx = r.x - o.x;
-                const P dy = r.y - o.y;
-                const P dz = r.z - o.z;
-                const P r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 >= two_a_p * two_a_p || r2 <= P(0)) continue;
-                const P rr2 = sqrt(r2);
-                const P overlap = two_a_p - rr2;
-                const P s = k_p * overlap / rr2;
-                F.x += dx * s;
-                F.y += dy * s;
-                F.z += dz * s;
-            }
-            queued = 0;
-        };
-
-        // Row (dz, dy) of any lane lies in buckets [kmin + o - 1, kmax + o + 1],
-        // o = dz*nx*ny + dy*nx. Lane b < 18 fetches the bucket start of bound
-        // b (the start of row b/2's range, or the end), and the wave reads
-        // them back by lane.
-        const int xlo = gx > 0 ? gx - 1 : 0;
-        const int xhi = gx < h.nx - 1 ? gx + 1 : h.nx - 1;
-        const int key = (gz * h.ny + gy) * h.nx + gx;
-        // The wave's records are consecutive in bucket order, unoccupied ones
-        // (the sentinel bucket) last: its occupied lanes are a prefix whose
-        // keys do not decrease, so the first and last of them bound the rest.
-        const unsigned long long act = __ballot(occ);
-        const int kmin = act ? __builtin_amdgcn_readlane(key, __ffsll(static_cast<long long>(act)) - 1) : 0x7FFFFFFF;
-        const int kmax = act ? __builtin_amdgcn_readlane(key, 63 - __clzll(static_cast<long long>(act))) : -1;
-        int bound = 0;
-        if (kmax >= 0 && lane < 18) {
-            const int row = lane >> 1;
-            const int64_t o = static_cast<int64_t>(row / 3 - 1) * h.nx * h.ny + static_cast<int64_t>(row % 3 - 1) * h.nx;
-            int64_t b = (lane & 1) ? static_cast<int64_t>(kmax) + o + 2 : static_cast<int64_t>(kmin) + o - 1;
-            b = b < 0 ? 0 : (b > h.nb ? h.nb : b);
-            bound = bstart[b];
-        }
-#pragma unroll 1
-        for (int d = 0; d < 3; ++d) {
-            // The wave's three ranges [lo[j], lo[j] + len) at positions off[j].
-            int lo[3], off[3], total = 0;
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                const int ra = __builtin_amdgcn_readlane(bound, 6 * d + 2 * j);
-                const int rb = __builtin_amdgcn_readlane(bound, 6 * d + 2 * j + 1);
-                lo[j] = ra;
-                off[j] = total;
-                total += rb > ra ? rb - ra : 0;
-            }
-            // This lane's rows (dz = d - 1, dy = j - 1), as row_ranges
-            // computes them: nr[j] records from position p[j]. Each lies
-            // inside the wave's range of its row, so positions rise along
-            // the lane's walk.
-            // The six loads are unconditional (a row off the grid reads row
-            // 0, which is in bounds, and counts no records), so they issue
-            // together with no branch.
-            int p[3], nr[3];
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                const int cz = gz + d - 1, cy = gy + j - 1;
-                const bool ok = occ && cz >= 0 && cz < h.nz && cy >= 0 && cy < h.ny;
-                const int row = ok ? (cz * h.ny + cy) * h.nx : 0;
-                const int rs = bstart[row + xlo];
-                const int re = bstart[row + xhi + 1];
-                nr[j] = ok && re > rs ? re - rs : 0;
-                p[j] = off[j] + (rs - lo[j]);
-            }
-            const int n01 = nr[0] + nr[1];
-            // The lane's own record, in the middle row of the middle z offset.
-            const int selfpos = (d == 1 && occ) ? off[1] + (k - lo[1]) : -1;
-            // Position of step m of the lane's walk.
-            const auto pos_of = [&](int m) {
-                return m < nr[0] ? p[0] + m : (m < n01 ? p[1] + (m - nr[0]) : p[2] + (m - n01));
-            };
-            int m = 0;
-            for (int c0 = 0; c0 < total; c0 += SEG) {
-                const int cend = total - c0 < SEG ? total : c0 + SEG;
-                // Up to four records per lane are loaded before any is
-                // stored, so the copy costs one memory latency, not four.
-                // Loads and stores are unconditional: a position past the
-                // chunk is clamped to its last one, whose record is read and
-                // written again unchanged.
-                for (int v = c0 + lane; v < cend; v += 4 * 64) {
-                    RecWords<P> t[4];
-                    int w[4];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        w[u] = min(v + 64 * u, cend - 1);
-                        const int shift =
-                            w[u] < off[1] ? lo[0] - off[0] : (w[u] < off[2] ? lo[1] - off[1] : lo[2] - off[2]);
-                        t[u] = load_rec(rec + (w[u] + shift));
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) store_rec(seg + (w[u] - c0), t[u]);
+            while (__ballot(m < mc) != 0) {
+                const int sb = m;
+                const int mend = mc - sb < WALK_ROUND ? mc : sb + WALK_ROUND;
+                uint32_t mask = 0;
+                int pos = pos_of(m);
+                // Position after step q at position at: the next in its row,
+                // or the start of the next row.
+                const auto next_pos = [&](int at, int q) {
+                    int nx_ = at + 1;
+                    if (q + 1 == nr[0]) nx_ = p[1];
+                    if (q + 1 == n01) nx_ = p[2];
+                    return nx_;
+                };
+                const auto hit_at = [&](int at) {
+                    return static_cast<uint32_t>(hit(seg[at - c0]) && at != selfpos);
+                };
+                // Two steps per iteration: both records are read before
+                // either is screened, so two LDS reads share one wait.
+                for (; m + 1 < mend; m += 2) {
+                    const int a0 = pos, a1 = next_pos(a0, m);
+                    pos = next_pos(a1, m + 1);
+                    const uint32_t h0 = hit_at(a0), h1 = hit_at(a1);
+                    mask |= (h0 | (h1 << 1)) << (m - sb);
                 }
-                wave_sync();
-                // The lane's steps whose positions lie below cend.
-                int mc = 0;
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    const int below = cend - p[j];
-                    mc += below < 0 ? 0 : (below < nr[j] ? below : nr[j]);
+                if (m < mend) {
+                    mask |= hit_at(pos) << (m - sb);
+                    ++m;
                 }
-                while (__ballot(m < mc) != 0) {
-                    const int sb = m;
-                    const int mend = mc - sb < CONTACT_ROUND ? mc : sb + CONTACT_ROUND;
-                    uint32_t mask = 0;
-                    int pos = pos_of(m);
-                    // Position after step q at position at: the next in its
-                    // row, or the start of the next row.
-                    const auto next_pos = [&](int at, int q) {
-                        int nx_ = at + 1;
-                        if (q + 1 == nr[0]) nx_ = p[1];
-                        if (q + 1 == n01) nx_ = p[2];
-                        return nx_;
-                    };
-                    const auto hit_at = [&](int at) {
-                        const Rec<P> o = seg[at - c0];
-                        const float dx = screen_diff(r.x, o.x), dy = screen_diff(r.y, o.y),
-                                    dz = screen_diff(r.z, o.z);
-                        // Fused multiply-adds round fewer times than separate
-                        // products and sums, well inside the margin.
-                        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                        return static_cast<uint32_t>(r2 < screen && at != selfpos);
-                    };
-                    // Two steps per iteration: both records are read before
-                    // either is screened, so two LDS reads share one wait.
-                    for (; m + 1 < mend; m += 2) {
-                        const int a0 = pos, a1 = next_pos(a0, m);
-                        pos = next_pos(a1, m + 1);
-                        const uint32_t h0 = hit_at(a0), h1 = hit_at(a1);
-                        mask |= (h0 | (h1 << 1)) << (m - sb);
-                    }
-                    if (m < mend) {
-                        mask |= hit_at(pos) << (m - sb);
-                        ++m;
-                    }
-                    // Queue the round's survivors by slice index, in order.
-                    while (mask != 0u) {
-                        const int at = pos_of(sb + __builtin_ctz(mask));
-                        mask &= mask - 1u;
-                        if (queued == CONTACT_QUEUE) drain();
-                        queue[queued * BLOCK] = at - c0;
-                        ++queued;
-                    }
+                // Queue the round's survivors by slice index, in order.
+                while (mask != 0u) {
+                    const int at = pos_of(sb + __builtin_ctz(mask));
+                    mask &= mask - 1u;
+                    if (queued == QUEUE) drain();
+                    queue[queued * BLOCK] = at - c0;
+                    ++queued;
                 }
-                // The queue refers to this chunk's slice: drain it before
-                // the next chunk reuses the slice.
-                if (queued > 0) drain();
-                wave_sync();
             }
-        }
-        if (touched) fs[k] = F;
-    });
-    }
-};
-
-// feed_kernel's irradiance half (src/sim/emission.cu:81-105): far-field sample
-// times the exact near-field shadow product over upstream live neighbours.
-template <class P>
-struct KFeedWalk {
-    static constexpr bool stride = true;   // any block count covers the work
-    struct Args {
-        const Counters* ctr;
-        HashGeo h;
-        const Rec<P>* rec;
-        const int32_t* bstart;
-        Grid irr;
-        double ldx;
-        double ldy;
-        double ldz;
-        float ambient;
-        int occl;
-        double* dirs;
-    };
-    static __device__ void run(const Args& a, VGrid vg) {
-        const auto& [ctr, h, rec, bstart, irr, ldx, ldy, ldz, ambient, occl, dirs] = a;
-    const float lxf = static_cast<float>(ldx), lyf = static_cast<float>(ldy), lzf = static_cast<float>(ldz);
-
-    __shared__ Rec<P> s_seg[BLOCK / 64][walk_seg<P>()];
-    __shared__ int32_t s_queue[SHADOW_QUEUE * BLOCK];
-    Rec<P>* seg = s_seg[threadIdx.x >> 6];
-    const int n = ctr->count;
-    xcd_tiles(vg, n, [&](const int k, const bool valid) {
-        Rec<P> r{};
-        bool lit = false;
-        if (valid) {
-            r = rec[k];
-            lit = !(r.meta & REC_UNOCC) && (r.meta & REC_ALIVE);
-        }
-        const uint32_t self = r.meta & REC_SLOT_MASK;
-        const double px = static_cast<double>(r.x), py = static_cast<double>(r.y), pz = static_cast<double>(r.z);
-        double directional = 0.0;
-        if (lit) {
-            directional = static_cast<double>(grid_sample(irr, static_cast<float>(r.x), static_cast<float>(r.y),
-                                                          static_cast<float>(r.z))) -
-                          static_cast<double>(ambient);
-            if (directional < 0.0) directional = 0.0;
-        }
-        if (occl) {
-            double open = 1.0;
-            int gx = 0, gy = 0, gz = 0;
-            if (lit) hash_coords(h, px, py, pz, gx, gy, gz);
-            int32_t* queue = s_queue + threadIdx.x;   // entry j at queue[j * BLOCK]
-            int queued = 0;
-            // shadow_fraction (src/sim/emission.cuh:30-37) of the queued
-            // neighbours, in walk order
-            const auto drain = [&]() {
-                for (int j = 0; j < queued; ++j) {
-                    const Rec<P> o = rec[queue[j * BLOCK]];
-                    const double dx = px - static_cast<double>(o.x);
-                    const double dy = py - static_cast<double>(o.y);
-                    const double dz = pz - static_cast<double>(o.z);
-                    const double along = dx * ldx + dy * ldy + dz * ldz;
-                    double frac = 0.0;
-                    if (along > 0.0) {
-                        const double perp2 = (dx * dx + dy * dy + dz * dz) - along * along;
-                        const double perp = perp2 > 0.0 ? sqrt(perp2) : 0.0;
-                        frac = disc_overlap_fraction(perp, ASTRO_CELL_RADIUS);
-                    }
-                    open *= (1.0 - frac);
-                }
-                queued = 0;
-            };
-            staged_walk(h, rec, bstart, seg, lit, gx, gy, gz, [&](const Rec<P>& o, int32_t q) {
-                if ((o.meta & REC_SLOT_MASK) == self || !(o.meta & REC_ALIVE)) return;
-                // FP32 screen. A neighbour downstream of the cell (along <= 0)
-                // or at least 2a off its light line has an overlap of exactly
-                // 0 and multiplies `open` by exactly 1. The float along and
-                // perp^2 differ from the double ones by under
-                // 2^-21 (|dx|+|dy|+|dz|) and 2^-18 |d|^2, so these margins
-                // never drop a shading neighbour. The survivors are queued and
-                // evaluated in double precision together, mostly after the
-                // walk, when every lane of the wave does so at once.
-                const float sx = screen_diff(r.x, o.x), sy = screen_diff(r.y, o.y), sz = screen_diff(r.z, o.z);
-                const float sa = sx * lxf + sy * lyf + sz * lzf;
-                if (sa < -0x1p-16f * (fabsf(sx) + fabsf(sy) + fabsf(sz))) return;
-                const float s2 = sx * sx + sy * sy + sz * sz;
-                if (s2 - sa * sa > SHADOW_SCREEN + 0x1p-14f * s2) return;
-                if (queued == SHADOW_QUEUE) drain();
-                queue[queued * BLOCK] = q;
-                ++queued;
-            });
+            // The queue refers to this chunk's slice: drain it before the
+            // next chunk reuses the slice.
             if (queued > 0) drain();
-            directional *= open;
+            wave_sync();
         }
-        if (lit) dirs[k] = directional;
-    });
     }
-};
+    return any;
+}
 
 // shadow_fraction's factor (src/sim/emission.cuh:30-37) for neighbour o of
 // the cell at (px, py, pz): 1 - the overlap of the two discs seen along the
@@ -745,6 +513,148 @@ __device__ __forceinline__ double shadow_factor(double px, double py, double pz,
     }
     return 1.0 - frac;
 }
+
+// The contact half of forces_kernel (src/sim/integrator.cu:60-67), one lane
+// per sorted record, adding to the body force already in fs[k]. Neighbour
+// positions come from the hash's records, fixed since the hash build; the
+// walk is masked_walk's.
+//   * Each neighbour is screened in float: the float r^2 of a pair (the
+//     differences rounded to float, squared and summed with fused
+//     multiply-adds) is within a few float roundings (relative 2^-21) of the
+//     double r^2, so a 2^-10 margin keeps every pair the reference counts as
+//     touching.
+//   * The survivors go through contact_force in walk order, so F sums the
+//     same terms in the same order as the reference: in double in the
+//     contract layout, bit for bit, and in float in the FP32 layout, whose
+//     force buffer and integrator are float (each term then differs from the
+//     double one by a few float roundings, at most about k * 2^-21 * 2a for
+//     pairs at the contact threshold). Pairs the exact test finds apart add
+//     a zero vector; F's components are sums of nonzero terms onto a nonzero
+//     or +0.0 start and so never -0.0, which makes those adds exact no-ops.
+template <class P>
+struct KContactWalk {
+    static constexpr bool stride = true;   // any block count covers the work
+    struct Args {
+        const Counters* ctr;
+        HashGeo h;
+        const Rec<P>* rec;
+        const int32_t* bstart;
+        Force<P>* fs;
+        double k_contact;
+    };
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [ctr, h, rec, bstart, fs, k_contact] = a;
+        __shared__ Rec<P> s_seg[BLOCK / 64][walk_seg<P>()];
+        __shared__ int32_t s_queue[CONTACT_QUEUE * BLOCK];
+        Rec<P>* seg = s_seg[threadIdx.x >> 6];
+        int32_t* queue = s_queue + threadIdx.x;
+        const int n = ctr->count;
+        const double two_a = ASTRO_CELL_DIAMETER;
+        const float screen = static_cast<float>(two_a * two_a) * (1.0f + 1.0f / 1024.0f);
+        xcd_tiles(vg, n, [&](const int k, const bool valid) {
+            Rec<P> r{};
+            bool occ = false;
+            int gx = 0, gy = 0, gz = 0;
+            if (valid) {
+                r = rec[k];
+                occ = !(r.meta & REC_UNOCC);
+                if (occ)
+                    hash_coords(h, static_cast<double>(r.x), static_cast<double>(r.y), static_cast<double>(r.z), gx,
+                                gy, gz);
+            }
+            // The body force the sample stored, loaded now so that its latency
+            // overlaps the walk; it is written back only if a contact was
+            // queued.
+            Force<P> F{};
+            if (occ) F = fs[k];
+            // contact_force (src/sim/integrator.cuh:210-219) in the layout's width
+            const P two_a_p = static_cast<P>(two_a), k_p = static_cast<P>(k_contact);
+            const bool touched = masked_walk<P, CONTACT_QUEUE>(
+                h, rec, bstart, seg, queue, occ, k, gx, gy, gz,
+                [&](const Rec<P>& o) {
+                    const float dx = screen_diff(r.x, o.x), dy = screen_diff(r.y, o.y), dz = screen_diff(r.z, o.z);
+                    // Fused multiply-adds round fewer times than separate
+                    // products and sums, well inside the margin.
+                    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                    return r2 < screen;
+                },
+                [&](const Rec<P>& o) {
+                    const P dx = r.x - o.x;
+                    const P dy = r.y - o.y;
+                    const P dz = r.z - o.z;
+                    const P r2 = dx * dx + dy * dy + dz * dz;
+                    if (r2 >= two_a_p * two_a_p || r2 <= P(0)) return;
+                    const P rr2 = sqrt(r2);
+                    const P overlap = two_a_p - rr2;
+                    const P s = k_p * overlap / rr2;
+                    F.x += dx * s;
+                    F.y += dy * s;
+                    F.z += dz * s;
+                });
+            if (touched) fs[k] = F;
+        });
+    }
+};
+
+// feed_kernel's irradiance half (src/sim/emission.cu:81-105): far-field sample
+// times the exact near-field shadow product over upstream live neighbours,
+// walked as the contact walk walks (masked_walk).
+//   * The screen is shadow_hit's, in float.
+//   * The survivors' factors are evaluated in double and multiplied into
+//     `open` in walk order, as the reference multiplies them.
+template <class P>
+struct KFeedWalk {
+    static constexpr bool stride = true;   // any block count covers the work
+    struct Args {
+        const Counters* ctr;
+        HashGeo h;
+        const Rec<P>* rec;
+        const int32_t* bstart;
+        Grid irr;
+        double ldx;
+        double ldy;
+        double ldz;
+        float ambient;
+        int occl;
+        double* dirs;
+    };
+    static __device__ void run(const Args& a, VGrid vg) {
+        const auto& [ctr, h, rec, bstart, irr, ldx, ldy, ldz, ambient, occl, dirs] = a;
+        const float lxf = static_cast<float>(ldx), lyf = static_cast<float>(ldy), lzf = static_cast<float>(ldz);
+        __shared__ Rec<P> s_seg[BLOCK / 64][walk_seg<P>()];
+        __shared__ int32_t s_queue[SHADOW_QUEUE * BLOCK];
+        Rec<P>* seg = s_seg[threadIdx.x >> 6];
+        int32_t* queue = s_queue + threadIdx.x;
+        const int n = ctr->count;
+        xcd_tiles(vg, n, [&](const int k, const bool valid) {
+            Rec<P> r{};
+            bool lit = false;
+            if (valid) {
+                r = rec[k];
+                lit = !(r.meta & REC_UNOCC) && (r.meta & REC_ALIVE);
+            }
+            const double px = static_cast<double>(r.x), py = static_cast<double>(r.y), pz = static_cast<double>(r.z);
+            double directional = 0.0;
+            if (lit) {
+                directional = static_cast<double>(grid_sample(irr, static_cast<float>(r.x), static_cast<float>(r.y),
+                                                              static_cast<float>(r.z))) -
+                              static_cast<double>(ambient);
+                if (directional < 0.0) directional = 0.0;
+            }
+            if (occl) {
+                double open = 1.0;
+                int gx = 0, gy = 0, gz = 0;
+                if (lit) hash_coords(h, px, py, pz, gx, gy, gz);
+                masked_walk<P, SHADOW_QUEUE>(
+                    h, rec, bstart, seg, queue, lit, k, gx, gy, gz,
+                    [&](const Rec<P>& o) { return shadow_hit(r, o, lxf, lyf, lzf); },
+                    [&](const Rec<P>& o) { open *= shadow_factor(px, py, pz, o, ldx, ldy, ldz); });
+                directional *= open;
+            }
+            if (lit) dirs[k] = directional;
+        });
+    }
+};
 
 // The feed walk for crowded worlds (several cells per hash bucket, so each
 // cell screens on the order of a hundred neighbours): one 16-lane group per
@@ -782,30 +692,14 @@ struct KFeedWalkRows {
             }
             const uint32_t self = r.meta & REC_SLOT_MASK;
             const double px = static_cast<double>(r.x), py = static_cast<double>(r.y), pz = static_cast<double>(r.z);
-            // the row's range, as row_ranges computes it
-            const auto row_range = [&](int row, int& s, int& e) {
-                int gx, gy, gz;
-                hash_coords(h, px, py, pz, gx, gy, gz);
-                const int xlo = gx > 0 ? gx - 1 : 0;
-                const int xhi = gx < h.nx - 1 ? gx + 1 : h.nx - 1;
-                const int cz = gz + row / 3 - 1, cy = gy + row % 3 - 1;
-                const bool ok = cz >= 0 && cz < h.nz && cy >= 0 && cy < h.ny;
-                const int base = ok ? (cz * h.ny + cy) * h.nx : 0;
-                s = bstart[base + xlo];
-                e = ok ? bstart[base + xhi + 1] : s;
-            };
-            // the screen of KFeedWalk: true for a possible shading neighbour
+            int gx = 0, gy = 0, gz = 0;
+            if (lit) hash_coords(h, px, py, pz, gx, gy, gz);
             const auto screen = [&](const Rec<P>& o) {
-                if ((o.meta & REC_SLOT_MASK) == self || !(o.meta & REC_ALIVE)) return false;
-                const float sx = screen_diff(r.x, o.x), sy = screen_diff(r.y, o.y), sz = screen_diff(r.z, o.z);
-                const float sa = sx * lxf + sy * lyf + sz * lzf;
-                if (sa < -0x1p-16f * (fabsf(sx) + fabsf(sy) + fabsf(sz))) return false;
-                const float s2 = sx * sx + sy * sy + sz * sz;
-                return !(s2 - sa * sa > SHADOW_SCREEN + 0x1p-14f * s2);
+                return (o.meta & REC_SLOT_MASK) != self && shadow_hit(r, o, lxf, lyf, lzf);
             };
             if (occl && lit && j < 9) {
                 int s, e, cnt = 0;
-                row_range(j, s, e);
+                row_range(h, bstart, gx, gy, gz, j, s, e);
                 for (int q = s; q < e; ++q) {
                     const Rec<P> o = rec[q];
                     if (!screen(o)) continue;
@@ -828,7 +722,7 @@ struct KFeedWalkRows {
                             for (int i = 0; i < cnt; ++i) open *= s_f[grp][row][i];
                         } else {
                             int s, e;
-                            row_range(row, s, e);
+                            row_range(h, bstart, gx, gy, gz, row, s, e);
                             for (int q = s; q < e; ++q) {
                                 const Rec<P> o = rec[q];
                                 if (screen(o)) open *= shadow_factor(px, py, pz, o, ldx, ldy, ldz);
