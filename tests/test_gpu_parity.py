@@ -497,6 +497,67 @@ def test_tick_parity_scale_workload(oracle, gpu, precision):
     g.close()
 
 
+@pytest.mark.parametrize("precision", [ah.FP64_CONTRACT, ah.FP32_HOT])
+def test_crowded_feed_walk_full_rows(oracle, gpu, precision):
+    """A crowded world (441 cells in 300 hash buckets of 20 um) takes the
+    feed walk that gives each cell a 16-lane group, one lane per neighbour
+    row. Forty cells sit upstream of cell 0 along the light, 6-9.5 um off its
+    light line and inside its own (dy, dz) = (0, 0) row, so that row holds
+    more survivors than the group keeps in LDS (24) and is walked again in
+    order by the group's first lane. None of them, and no other cell, lies in
+    cell 0's column of the 6.25 x 7.5 um irradiance voxels, so its far-field
+    sample stays positive and its irradiance is the product of the forty
+    factors. Irradiance must agree with the oracle to two float ulps per
+    cell, as in the screen test."""
+    kw = dict(capacity=500, chamber_w=200e-6, chamber_h=200e-6, chamber_d=60e-6, hash_cell_size=2e-5,
+              physics_rate=0.1, thermal_enabled=0, light_enabled=1, light_irradiance=1e4, light_dir_x=1.0,
+              light_dir_y=0.0, precision=precision,
+              grid={0: (32, 32, 8), 1: (32, 32, 8), 2: (-1, 0, 0), 3: (32, 32, 8)})
+    n, k = 441, 40
+    cells = oracle_cells(oracle, n, 5, spawn=dict(awake=True, charge_dist=ah.DIST_UNIFORM, charge_b=0.9), **kw)
+    rng = np.random.default_rng(11)
+    # cell 0 in hash bucket (5, 5, 1) and irradiance column y [6.25, 12.5) um,
+    # z [0, 7.5) um
+    a0 = np.array([10e-6, 10e-6, 3.75e-6])
+    off = []
+    while len(off) < k:
+        dy = rng.choice([-1.0, 1.0]) * rng.uniform(4.5e-6, 9e-6)
+        dz = rng.uniform(-6e-6, 6e-6)
+        if 6e-6 <= np.hypot(dy, dz) < 9.5e-6:
+            off.append((dy, dz))
+    off = np.array(off)
+    cluster = np.stack([rng.uniform(-18e-6, 8e-6, k), a0[1] + off[:, 0], a0[2] + off[:, 1]], axis=1)
+    pos = np.stack([cells["x"], cells["y"], cells["z"]], axis=1)
+    pos[0] = a0
+    pos[1:k + 1] = cluster
+    rest = pos[k + 1:]
+    in_col = (rest[:, 1] >= 6e-6) & (rest[:, 1] < 13e-6) & (rest[:, 2] >= -0.5e-6) & (rest[:, 2] < 8e-6)
+    rest[in_col, 1] -= 30e-6
+    if precision == ah.FP32_HOT:
+        pos = pos.astype(np.float32).astype(np.float64)
+    # every cluster cell is upstream of cell 0, within 2a of its light line,
+    # in its bucket row (x buckets 4..6, y bucket 5, z bucket 1) and outside
+    # its irradiance column
+    d = pos[0] - pos[1:k + 1]
+    assert np.all(d[:, 0] > 0.0) and np.all(np.hypot(d[:, 1], d[:, 2]) < TWO_A)
+    gb = np.floor((pos[1:k + 1] - np.array([-100e-6, -100e-6, -30e-6])) / 2e-5).astype(int)
+    assert np.all((gb[:, 0] >= 4) & (gb[:, 0] <= 6) & (gb[:, 1] == 5) & (gb[:, 2] == 1))
+    assert np.all((pos[1:, 1] < 6.25e-6) | (pos[1:, 1] >= 12.5e-6) | (pos[1:, 2] < 0.0) | (pos[1:, 2] >= 7.5e-6))
+    for j, name in enumerate(("x", "y", "z")):
+        cells[name] = pos[:, j].copy()
+    o, g = pair(oracle, gpu, cells, **kw)
+    with o, g:
+        o.step(1)
+        g.step(1)
+        a, b = o.download()["irradiance"], g.download()["irradiance"]
+    # forty factors of 0.5-0.97 each: far below what any one neighbour leaves
+    assert 0.0 < a[0] < 1e-2 * a.max(), "cell 0 should sit deep in the cluster's shadow"
+    ulp = np.spacing(np.abs(a).astype(np.float32)).astype(np.float64)
+    diff = np.abs(a.astype(np.float64) - b.astype(np.float64))
+    bad = np.nonzero(diff > 2 * ulp)[0]
+    assert bad.size == 0, f"irradiance differs at cells {bad[:10]}: {a[bad[:5]]} vs {b[bad[:5]]}"
+
+
 def test_rear_cell_irradiance_exactly_zero(gpu, oracle):
     """T13 (tests/physics/test_emission.cu:73-133): collinear pair along the
     light, 15 um apart: the rear cell's irradiance and energy are exactly 0."""
