@@ -54,6 +54,12 @@ struct FieldHost {
     }
 };
 
+// The ping-pong buffers of every field, as a tick leaves them on the host.
+struct BufPtrs {
+    float* value[ASTRO_FIELD_COUNT];
+    float* scratch[ASTRO_FIELD_COUNT];
+};
+
 void field_init(FieldHost& g, astro_grid_dims d, double w, double h, double depth, double D, double ambient,
                 double scale, int bc) {
     g.on = d.nx > 0;
@@ -234,10 +240,18 @@ struct astro_world {
     int32_t tau_cap = 0;
     bool tau_on = false;
 
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
+    // Captured ticks. A field with an odd number of substeps per tick ends
+    // each tick with its ping-pong buffers swapped, and a captured launch
+    // bakes the buffers it read: so the graphs are kept per phase of that
+    // swap. graph[p][t - 1] runs t ticks (t = 1, 2) starting in phase p;
+    // phase 0 is the arrangement at capture, and a tick flips the phase when
+    // some field's substep count is odd. Without such a field only
+    // graph[0][0] exists and the phase stays 0.
+    hipGraph_t graph[2][2] = {};
+    hipGraphExec_t gexec[2][2] = {};
+    BufPtrs gstart[2] = {};   // the buffers each phase starts from
     bool graph_ok = false;
-    int graph_ticks = 1;
+    int phase = 0;
     // Bumped by every change that drops the captured tick (clock, light,
     // flash, stage calls, field swaps, ...). An ensemble that captured this
     // world's launch arguments compares it to know when to capture again.
@@ -543,11 +557,31 @@ int ticks_per_graph(const astro_world* w) {
 
 void drop_graph(astro_world* w) {
     ++w->generation;
-    if (w->gexec) hipGraphExecDestroy(w->gexec);
-    if (w->graph) hipGraphDestroy(w->graph);
-    w->gexec = nullptr;
-    w->graph = nullptr;
+    for (int p = 0; p < 2; ++p)
+        for (int t = 0; t < 2; ++t) {
+            if (w->gexec[p][t]) hipGraphExecDestroy(w->gexec[p][t]);
+            if (w->graph[p][t]) hipGraphDestroy(w->graph[p][t]);
+            w->gexec[p][t] = nullptr;
+            w->graph[p][t] = nullptr;
+        }
     w->graph_ok = false;
+    w->phase = 0;
+}
+
+BufPtrs bufs_of(const astro_world* w) {
+    BufPtrs b{};
+    for (int f = 0; f < ASTRO_FIELD_COUNT; ++f) {
+        b.value[f] = w->f[f].value;
+        b.scratch[f] = w->f[f].scratch;
+    }
+    return b;
+}
+
+void set_bufs(astro_world* w, const BufPtrs& b) {
+    for (int f = 0; f < ASTRO_FIELD_COUNT; ++f) {
+        w->f[f].value = b.value[f];
+        w->f[f].scratch = b.scratch[f];
+    }
 }
 
 int32_t check_device_errors(astro_world* w) {
@@ -1082,6 +1116,54 @@ int32_t check_shape(astro_ensemble* e) {
     return ASTRO_OK;
 }
 
+// Captures `ticks` ticks of w starting from the buffers `from` into
+// graph[p][ticks - 1], and returns the buffers they leave in *to. The host's
+// own buffers and hash flag are left as they were.
+int32_t capture_ticks(astro_world* w, int p, int ticks, const BufPtrs& from, BufPtrs* to) {
+    const BufPtrs keep = bufs_of(w);
+    const bool hv = w->hash_valid;
+    set_bufs(w, from);
+    int32_t st = hip_check(w, hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (st != ASTRO_OK) {
+        set_bufs(w, keep);
+        return st;
+    }
+    int32_t cst = ASTRO_OK;
+    for (int i = 0; i < ticks && cst == ASTRO_OK; ++i) cst = tick_once(w, Launch{w->stream, nullptr, nullptr});
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(w->stream, &g);
+    *to = bufs_of(w);
+    set_bufs(w, keep);
+    w->hash_valid = hv;
+    if (cst != ASTRO_OK) {
+        if (g) hipGraphDestroy(g);
+        return cst;
+    }
+    if ((st = hip_check(w, ee, "end capture")) != ASTRO_OK) return st;
+    w->graph[p][ticks - 1] = g;
+    return hip_check(w, hipGraphInstantiate(&w->gexec[p][ticks - 1], g, nullptr, nullptr, 0), "instantiate");
+}
+
+// Captures the world's tick graphs from its current buffers (phase 0): one
+// tick, and, when some field's substep count is odd, two ticks from phase 0
+// and one and two ticks from phase 1.
+int32_t capture_graphs(astro_world* w) {
+    drop_graph(w);
+    const bool odd = ticks_per_graph(w) == 2;
+    BufPtrs end{};
+    w->gstart[0] = bufs_of(w);
+    int32_t st = capture_ticks(w, 0, 1, w->gstart[0], &w->gstart[1]);
+    if (st == ASTRO_OK && odd) st = capture_ticks(w, 0, 2, w->gstart[0], &end);
+    if (st == ASTRO_OK && odd) st = capture_ticks(w, 1, 1, w->gstart[1], &end);
+    if (st == ASTRO_OK && odd) st = capture_ticks(w, 1, 2, w->gstart[1], &end);
+    if (st != ASTRO_OK) {
+        drop_graph(w);
+        return st;
+    }
+    w->graph_ok = true;
+    return ASTRO_OK;
+}
+
 }  // namespace
 
 
@@ -1475,39 +1557,23 @@ int32_t astro_world_step(astro_world* w, int32_t n_ticks) {
     if (n_ticks == 0) return ASTRO_OK;
     int32_t st;
     int32_t done = 0;
-    if (!w->prof_on) {
-        const int k = ticks_per_graph(w);
-        if (n_ticks >= k) {
-            if (!w->graph_ok) {
-                drop_graph(w);
-                st = hip_check(w, hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal), "begin capture");
-                if (st != ASTRO_OK) return st;
-                const bool hv = w->hash_valid;
-                int32_t cst = ASTRO_OK;
-                for (int i = 0; i < k && cst == ASTRO_OK; ++i) cst = tick_once(w, Launch{w->stream, nullptr, nullptr});
-                hipGraph_t g = nullptr;
-                const hipError_t ee = hipStreamEndCapture(w->stream, &g);
-                w->hash_valid = hv;
-                if (cst != ASTRO_OK) {
-                    if (g) hipGraphDestroy(g);
-                    return cst;
-                }
-                if ((st = hip_check(w, ee, "end capture")) != ASTRO_OK) return st;
-                w->graph = g;
-                if ((st = hip_check(w, hipGraphInstantiate(&w->gexec, g, nullptr, nullptr, 0), "instantiate")) != ASTRO_OK) {
-                    drop_graph(w);
-                    return st;
-                }
-                w->graph_ok = true;
-                w->graph_ticks = k;
-            }
-            while (n_ticks - done >= w->graph_ticks) {
-                if ((st = hip_check(w, hipGraphLaunch(w->gexec, w->stream), "graph launch")) != ASTRO_OK) return st;
-                advance_clock(w, w->graph_ticks);
-                done += w->graph_ticks;
-            }
-            w->hash_valid = false;
+    const bool odd = ticks_per_graph(w) == 2;
+    if (!w->prof_on && (w->graph_ok || n_ticks >= (odd ? 2 : 1))) {
+        if (!w->graph_ok && (st = capture_graphs(w)) != ASTRO_OK) return st;
+        // Ticks in pairs where a pair's buffers end where they started, a
+        // single tick for an odd remainder; no tick runs eagerly and nothing
+        // is captured again.
+        while (done < n_ticks) {
+            const int t = odd && n_ticks - done >= 2 ? 2 : 1;
+            if ((st = hip_check(w, hipGraphLaunch(w->gexec[w->phase][t - 1], w->stream), "graph launch")) != ASTRO_OK)
+                return st;
+            advance_clock(w, t);
+            done += t;
+            if (odd && t == 1) w->phase ^= 1;
+            set_bufs(w, w->gstart[w->phase]);
         }
+        w->hash_valid = false;
+        return ASTRO_OK;
     }
     const Launch L = launch_of(w);
     while (done < n_ticks) {
@@ -1515,9 +1581,10 @@ int32_t astro_world_step(astro_world* w, int32_t n_ticks) {
         if ((st = hip_check(w, hipGetLastError(), "tick launch")) != ASTRO_OK) return st;
         advance_clock(w, 1);
         ++done;
-        // An eager tick leaves a field's ping-pong swapped when its substep
-        // count is odd; the captured graph baked the other parity.
-        if (w->graph_ok && ticks_per_graph(w) == 2) drop_graph(w);
+        // An eager tick swaps the buffers of a field with an odd substep
+        // count, as graph[phase][0] would: the captured graphs still apply,
+        // from the other phase.
+        if (w->graph_ok && odd) w->phase ^= 1;
     }
     return ASTRO_OK;
 }
