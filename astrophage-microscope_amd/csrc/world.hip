@@ -1141,7 +1141,11 @@ int32_t capture_ticks(astro_world* w, int p, int ticks, const BufPtrs& from, Buf
     }
     if ((st = hip_check(w, ee, "end capture")) != ASTRO_OK) return st;
     w->graph[p][ticks - 1] = g;
-    return hip_check(w, hipGraphInstantiate(&w->gexec[p][ticks - 1], g, nullptr, nullptr, 0), "instantiate");
+    if ((st = hip_check(w, hipGraphInstantiate(&w->gexec[p][ticks - 1], g, nullptr, nullptr, 0), "instantiate")) !=
+        ASTRO_OK)
+        return st;
+    // uploaded now, so that its first launch carries no set-up
+    return hip_check(w, hipGraphUpload(w->gexec[p][ticks - 1], w->stream), "graph upload");
 }
 
 // Captures the world's tick graphs from its current buffers (phase 0): one
