@@ -674,6 +674,39 @@ def test_replay_and_graph_determinism(gpu, oracle):
     assert hashes[0] == hashes[1] == hashes[2]
 
 
+def test_graph_phases_odd_tick_counts(gpu, oracle):
+    """A world whose CO2 field runs an odd number of substeps per tick (the
+    1M-cell workload's voxel spacing: 7.8 x 7.8 x 6.25 um, dt 1e-5 s) ends
+    each tick with that field's buffers swapped, so its graphs are kept per
+    buffer arrangement. Stepping in odd counts, with eager ticks between
+    replays, must give the state hash of the same ticks run eagerly."""
+    kw = dict(capacity=3000, chamber_w=500e-6, chamber_h=500e-6, chamber_d=2e-4, hash_cell_size=1e-5,
+              physics_rate=0.1, biology_rate=1e5, co2_init=1.0, division_offset=0.5e-6, compaction_enabled=1,
+              grid={0: (64, 64, 32), 1: (64, 64, 32), 2: (-1, 0, 0), 3: (16, 16, 1)})
+    cells = oracle_cells(oracle, 2500, 3, spawn=dict(charge_dist=ah.DIST_LOG_UNIFORM, charge_a=1e-3, charge_b=0.2),
+                         **kw)
+    cells["flags"][::10] |= ah.FLAG_AWAKE
+    plans = {"eager": [("e", 23)], "graph": [("g", 23)],
+             "mixed": [("g", 5), ("g", 1), ("e", 1), ("g", 3), ("e", 2), ("g", 1), ("g", 10)]}
+    hashes = {}
+    for name, plan in plans.items():
+        with ah.World(gpu, **kw) as w:
+            w.upload(cells)
+            for mode, n in plan:
+                w.profile(mode == "e")
+                if mode == "e":
+                    w.profile_reset()
+                w.step(n)
+                if mode == "e" and name == "eager":
+                    kern = w.profile_read()
+                    launches = kern["ftcs_CO2"][0]
+                    assert launches % n == 0 and (launches // n) % 2 == 1, kern["ftcs_CO2"]
+            w.profile(False)
+            assert w.counters().tick == 23
+            hashes[name] = w.state_hash()
+    assert hashes["graph"] == hashes["eager"] and hashes["mixed"] == hashes["eager"], hashes
+
+
 def test_stats_ledger(gpu, oracle):
     """world_stats' fixed-point ledger equals the host sum (T23,
     tests/physics/test_stats.cu:60-76)."""
