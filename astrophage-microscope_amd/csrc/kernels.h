@@ -164,7 +164,9 @@ struct Ftcs {
 // must be the same across the chambers of one launch.
 
 // ---- hash (kernels_hash.hip) ----
-template <class P> void launch_hash_build(const Launch& L, Views<P> ws, bool count_tick);
+// thermal_count: the key pass also counts the occupied entries per
+// temperature voxel into tcount, as launch_thermal_count would.
+template <class P> void launch_hash_build(const Launch& L, Views<P> ws, bool count_tick, bool thermal_count = false);
 // The contact half of forces_kernel over the sorted records, adding to fs.
 template <class P> void launch_contact_walk(const Launch& L, Views<P> ws, Ticks ts);
 // A world with crowded[i] set has each cell's neighbourhood walked by a

@@ -45,9 +45,12 @@ struct KHashKey {
         int cap;
         int count_tick;
         int resort_period;
+        Grid T;            // with tcount: the temperature grid whose voxels are counted
+        uint32_t* tcount;  // non-null: also count the occupied entries per T voxel
     };
     static __device__ void run(const Args& a, VGrid vg) {
-        const auto& [c, ctr, h, keys, rin, bcount, bsum, dflag, drop, dsum, xsum, cap, count_tick, resort_period] = a;
+        const auto& [c, ctr, h, keys, rin, bcount, bsum, dflag, drop, dsum, xsum, cap, count_tick, resort_period, T,
+                     tcount] = a;
 
     const int np = ctr->nphys;
     const int gid = vg.block * BLOCK + threadIdx.x, stride = vg.grid * BLOCK;
@@ -94,6 +97,11 @@ struct KHashKey {
                 : static_cast<uint32_t>(h.nb);
             keys[p] = key;
             rin[p] = atomicAdd(&bcount[key], 1u);
+            // the thermostat's voxel count (KThermalCount), from the same
+            // positions: nothing moves a cell before the thermal step
+            if (tcount && (f & ASTRO_FLAG_OCCUPIED))
+                atomicAdd(&tcount[grid_nearest(T, static_cast<float>(x), static_cast<float>(y), static_cast<float>(z))],
+                          1u);
         }
         wave_count_by_key(bsum, key >> SCAN_TILE_SHIFT, live);
     });
@@ -741,7 +749,7 @@ struct KFeedWalkRows {
 }  // namespace
 
 template <class P>
-void launch_hash_build(const Launch& L, Views<P> ws, bool count_tick) {
+void launch_hash_build(const Launch& L, Views<P> ws, bool count_tick, bool thermal_count) {
     Batch<KHashKey<P>> key;
     Batch<KBucketScan> scan;
     Batch<KHashScatter<P>> scatter;
@@ -750,7 +758,7 @@ void launch_hash_build(const Launch& L, Views<P> ws, bool count_tick) {
         const int gp = cell_grid(w.pcap);
         const int nbp1 = w.hg.nb + 1;
         key.add({w.c, w.ctr, w.hg, w.keys, w.rin, w.bcount, w.bsum, w.dflag, w.drop, w.dsum, w.xsum, w.capacity,
-                 count_tick ? 1 : 0, w.resort_period},
+                 count_tick ? 1 : 0, w.resort_period, w.g[ASTRO_FIELD_TEMPERATURE], thermal_count ? w.tcount : nullptr},
                 gp);
         scan.add({w.ctr, w.bcount, w.bsum, w.bstart, nbp1}, scan_tiles_for(nbp1));
         scatter.add({w.c, w.ctr, w.keys, w.rin, w.bstart, w.trec, w.tphys}, gp);
@@ -807,7 +815,7 @@ void launch_feed_walk(const Launch& L, Views<P> ws, Ticks ts, const std::vector<
 }
 
 #define INSTANTIATE(P)                                                         \
-    template void launch_hash_build<P>(const Launch&, Views<P>, bool);         \
+    template void launch_hash_build<P>(const Launch&, Views<P>, bool, bool);   \
     template void launch_contact_walk<P>(const Launch&, Views<P>, Ticks);      \
     template void launch_feed_walk<P>(const Launch&, Views<P>, Ticks, const std::vector<char>&); \
     template void launch_resort<P>(const Launch&, Views<P>);

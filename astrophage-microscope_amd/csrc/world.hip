@@ -437,7 +437,8 @@ bool thermal_direct(const astro_world* w, int substeps) {
 
 // thermal_step (src/sim/thermal.cu:75-103).
 template <class P>
-int32_t enqueue_thermal(WorldSpan ws, const Launch& L, double dt, Ticks ts) {
+// counted: the tick's key pass has already counted the entries per voxel.
+int32_t enqueue_thermal(WorldSpan ws, const Launch& L, double dt, Ticks ts, bool counted = false) {
     const int substeps = substeps_for_dt(ws[0]->f[ASTRO_FIELD_TEMPERATURE], dt);
     const double dt_sub = dt / substeps;
     std::vector<Ftcs> fc(ws.size());
@@ -446,7 +447,7 @@ int32_t enqueue_thermal(WorldSpan ws, const Launch& L, double dt, Ticks ts) {
         if (!ftcs_coeffs(ws[i]->f[ASTRO_FIELD_TEMPERATURE], dt_sub, fc[i]))
             return fail(ws[i], ASTRO_INVALID_ARGUMENT, "diffusion coefficient unstable");
     const bool direct = thermal_direct(ws[0], substeps);
-    if (direct) launch_thermal_count<P>(L, views_of<P>(ws));
+    if (direct && !counted) launch_thermal_count<P>(L, views_of<P>(ws));
     for (int s = 0; s < substeps; ++s) {
         const std::vector<WorldDev<P>> d = views_of<P>(ws);
         const int mode = !direct ? 0 : (s == 0 ? 1 : 2);
@@ -489,11 +490,15 @@ int32_t enqueue_tick(WorldSpan ws, const Launch& L) {
     const double dt = ASTRO_DT_PHYSICS * ws[0]->physics_rate;
     const std::vector<TickParams> ts = ticks_of(ws, dt);
     const astro_world_desc& ds = ws[0]->desc;
-    launch_hash_build<P>(L, views_of<P>(ws), true);
+    // When the thermal step takes the direct deposit path, its per-voxel
+    // counts ride in the key pass, which reads the same positions.
+    const bool count_t =
+        ds.thermal_enabled && thermal_direct(ws[0], substeps_for_dt(ws[0]->f[ASTRO_FIELD_TEMPERATURE], dt));
+    launch_hash_build<P>(L, views_of<P>(ws), true, count_t);
     launch_resort<P>(L, views_of<P>(ws));
     for (astro_world* w : ws) w->hash_valid = true;
     int32_t st;
-    if (ds.thermal_enabled && (st = enqueue_thermal<P>(ws, L, dt, ts)) != ASTRO_OK) return st;
+    if (ds.thermal_enabled && (st = enqueue_thermal<P>(ws, L, dt, ts, count_t)) != ASTRO_OK) return st;
     const bool feed = ds.emission_enabled && lit(ws[0]);
     if (feed) enqueue_light<P>(ws, L, ts);
     if (ws[0]->flash_on) {
