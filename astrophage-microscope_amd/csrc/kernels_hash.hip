@@ -673,8 +673,9 @@ struct KFeedWalk {
 // them into `open` row after row, in the same order and with the same factors
 // as the one-lane walk, so the product is the same bit for bit. A row with
 // more survivors than the LDS keeps is walked again, in order, by the first
-// lane itself. The neighbour records are read from the hash array directly:
-// a crowded world's records fit in the L2.
+// lane itself. The neighbour records are read from the hash array directly,
+// one dependent load after another, so the world's caller takes this walk
+// only where all the launch's records fit one XCD's 4 MiB L2.
 constexpr int FEED_GROUP = 16;
 constexpr int FEED_ROW_CAP = 24;
 
