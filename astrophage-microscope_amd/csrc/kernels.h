@@ -188,9 +188,12 @@ template <class P> void launch_spawn(const Launch& L, const WorldDev<P>& w, cons
 // The thermostat's deposit paths (THERMAL_* in kernels_cells.hip): 0 atomic;
 // 1 direct, first substep of the tick (after launch_thermal_count); 2 direct.
 // `substep` counts the tick's substeps from 0; `last` marks the final one.
+// `stamp`: the last substep also stamps the occluders of the chambers with a
+// positive light source, as launch_stamp would after it.
 template <class P> void launch_thermal_count(const Launch& L, Views<P> ws);
 template <class P> void launch_thermal_substep(const Launch& L, Views<P> ws, Ticks ts, double dt_sub,
-                                               bool update_rec, int mode, int substep, bool last);
+                                               bool update_rec, int mode, int substep, bool last,
+                                               bool stamp = false);
 template <class P> void launch_apply_awake(const Launch& L, Views<P> ws, int field, int mode, int substep);
 template <class P> void launch_feed_taxis(const Launch& L, Views<P> ws, Ticks ts, bool feed, bool taxis);
 // Feeding/taxis (optional), field sample + body force, then the contact walk:

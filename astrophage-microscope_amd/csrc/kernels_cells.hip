@@ -186,15 +186,20 @@ struct KThermal {
         int mode;
         int parity;               // the substep's parity, for Counters::tqueued
         int last;                 // the tick's last substep
+        Grid irr;                 // deposit non-null: also stamp occluders (KStamp) with the final flags
     };
     static __device__ void run(const Args& a, VGrid vg) {
-        const auto& [c, ctr, T, relax, cap, rec, spos, update_rec, tcount, talone, mode, parity, last] = a;
+        const auto& [c, ctr, T, relax, cap, rec, spos, update_rec, tcount, talone, mode, parity, last, irr] = a;
 
     // This substep reports in tqueued[parity] whether any deposit waits in
     // the accumulator, and clears the other parity's report, which the
     // previous substep's apply pass has read.
     if (vg.block == 0 && threadIdx.x == 0) ctr->tqueued[parity ^ 1] = 0u;
     const int n = ctr->nphys;
+    // The entry's flags after this substep (0 while unoccupied) and its
+    // position, for the occluder stamp.
+    uint32_t fin = 0u;
+    float xf = 0.0f, yf = 0.0f, zf = 0.0f;
     // Returns whether the cell left a deposit in the accumulator.
     const auto body = [&](const int i) -> bool {
     // Flags and position are loaded together, ahead of the branch on the
@@ -202,8 +207,11 @@ struct KThermal {
     // awake and alive fetches its energy beside that read (a cell that
     // ignites below fetches it then).
     const uint32_t fl = c.flags[i];
-    const float xf = static_cast<float>(c.x[i]), yf = static_cast<float>(c.y[i]), zf = static_cast<float>(c.z[i]);
+    xf = static_cast<float>(c.x[i]);
+    yf = static_cast<float>(c.y[i]);
+    zf = static_cast<float>(c.z[i]);
     if (!(fl & ASTRO_FLAG_OCCUPIED)) return false;
+    fin = fl;
     const uint32_t idx = grid_nearest(T, xf, yf, zf);
     const bool early = (fl & ASTRO_FLAG_AWAKE) && (fl & ASTRO_FLAG_ALIVE);
     double e0 = early ? c.energy[i] : 0.0;
@@ -221,6 +229,7 @@ struct KThermal {
         awake = true;
         nf = fl | ASTRO_FLAG_AWAKE;
         c.flags[i] = nf;
+        fin = nf;
     }
     const double t_cell = awake ? ASTRO_CELL_TEMP_SETPOINT : static_cast<double>(t_grid);
     // nothing reads temp_cell between substeps: the last one's value stands
@@ -250,6 +259,7 @@ struct KThermal {
     }
     if (energy <= 0.0) {
         c.flags[i] = nf & ~ASTRO_FLAG_ALIVE;
+        fin = nf & ~ASTRO_FLAG_ALIVE;
         c.death_cause[i] = ASTRO_DEATH_STARVED;
         c.emit_power[i] = 0.0f;
         if (update_rec) rec[spos[i]].meta &= ~REC_ALIVE;
@@ -257,8 +267,16 @@ struct KThermal {
     return queued;
     };
     xcd_tiles(vg, n, [&](const int i, const bool valid) {
+        fin = 0u;
         const bool q = valid && body(i);
         if (__ballot(q) != 0ull && (threadIdx.x & 63) == 0 && ctr->tqueued[parity] == 0u) ctr->tqueued[parity] = 1u;
+        // occlusion_stamp_kernel (src/sim/emission.cu:17-27) on the flags the
+        // thermal step leaves, as the emission stage that follows reads them
+        if (irr.deposit && (fin & ASTRO_FLAG_OCCUPIED)) {
+            const double area = (fin & ASTRO_FLAG_ALIVE) ? ASTRO_CELL_CROSS_SECTION
+                                                         : ASTRO_CELL_CROSS_SECTION * (1.0 - ASTRO_CELL_ALBEDO_DEAD);
+            deposit_add(&irr.deposit[grid_nearest(irr, xf, yf, zf)], area, irr.scale);
+        }
     });
     }
 };
@@ -1074,7 +1092,7 @@ void launch_thermal_count(const Launch& L, Views<P> ws) {
 
 template <class P>
 void launch_thermal_substep(const Launch& L, Views<P> ws, Ticks ts, double dt_sub, bool update_rec, int mode,
-                            int substep, bool last) {
+                            int substep, bool last, bool stamp) {
     using K = KThermal<P>;
     Batch<K> b;
     for (size_t i = 0; i < ws.size(); ++i) {
@@ -1082,8 +1100,10 @@ void launch_thermal_substep(const Launch& L, Views<P> ws, Ticks ts, double dt_su
         const double cap = ts[i].T_cap;
         const double g = ASTRO_CONDUCTION_COEFF;
         const double relax = (cap <= 0.0 || g <= 0.0) ? 0.0 : -std::expm1(-g * dt_sub / cap);
+        // chambers with a positive source stamp occluders (enqueue_light)
+        const Grid irr = stamp && last && ts[i].source > 0.0f ? w.g[ASTRO_FIELD_IRRADIANCE] : Grid{};
         b.add({w.c, w.ctr, w.g[ASTRO_FIELD_TEMPERATURE], relax, cap, w.rec, w.spos, update_rec ? 1 : 0, w.tcount,
-               w.talone, mode, substep & 1, last ? 1 : 0},
+               w.talone, mode, substep & 1, last ? 1 : 0, irr},
               cell_grid(w.pcap));
     }
     launch_k<K>(L, "thermal", b.args, b.blocks);
@@ -1251,7 +1271,7 @@ void launch_stats_cells(const Launch& L, const WorldDev<P>& w) {
     template void launch_spawn<P>(const Launch&, const WorldDev<P>&, const SpawnArgs&, int32_t, int32_t,       \
                                   uint64_t, uint64_t);                                                          \
     template void launch_thermal_count<P>(const Launch&, Views<P>);                                             \
-    template void launch_thermal_substep<P>(const Launch&, Views<P>, Ticks, double, bool, int, int, bool);      \
+    template void launch_thermal_substep<P>(const Launch&, Views<P>, Ticks, double, bool, int, int, bool, bool); \
     template void launch_apply_awake<P>(const Launch&, Views<P>, int, int, int);                                \
     template void launch_feed_taxis<P>(const Launch&, Views<P>, Ticks, bool, bool);                             \
     template void launch_integrate<P>(const Launch&, Views<P>, Ticks, bool);                                    \

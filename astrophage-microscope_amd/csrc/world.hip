@@ -438,7 +438,9 @@ bool thermal_direct(const astro_world* w, int substeps) {
 // thermal_step (src/sim/thermal.cu:75-103).
 template <class P>
 // counted: the tick's key pass has already counted the entries per voxel.
-int32_t enqueue_thermal(WorldSpan ws, const Launch& L, double dt, Ticks ts, bool counted = false) {
+// stamp: the last substep also stamps the occluders for the emission stage.
+int32_t enqueue_thermal(WorldSpan ws, const Launch& L, double dt, Ticks ts, bool counted = false,
+                        bool stamp = false) {
     const int substeps = substeps_for_dt(ws[0]->f[ASTRO_FIELD_TEMPERATURE], dt);
     const double dt_sub = dt / substeps;
     std::vector<Ftcs> fc(ws.size());
@@ -451,7 +453,7 @@ int32_t enqueue_thermal(WorldSpan ws, const Launch& L, double dt, Ticks ts, bool
     for (int s = 0; s < substeps; ++s) {
         const std::vector<WorldDev<P>> d = views_of<P>(ws);
         const int mode = !direct ? 0 : (s == 0 ? 1 : 2);
-        launch_thermal_substep<P>(L, d, ts, dt_sub, ws[0]->hash_valid, mode, s, s == substeps - 1);
+        launch_thermal_substep<P>(L, d, ts, dt_sub, ws[0]->hash_valid, mode, s, s == substeps - 1, stamp);
         launch_apply_awake<P>(L, d, ASTRO_FIELD_TEMPERATURE, mode, s);
         for (size_t i = 0; i < ws.size(); ++i) gs[i] = ws[i]->f[ASTRO_FIELD_TEMPERATURE].grid();
         launch_ftcs(L, gs, fc, "ftcs_T");
@@ -463,8 +465,9 @@ int32_t enqueue_thermal(WorldSpan ws, const Launch& L, double dt, Ticks ts, bool
 // emission_step, directional path (src/sim/emission.cu:145-176), minus the
 // absorption which rides in the feed/taxis kernel. Only chambers with a
 // positive source stamp occluders.
+// stamped: the thermal step's last substep has stamped the occluders already.
 template <class P>
-void enqueue_light(WorldSpan ws, const Launch& L, Ticks ts) {
+void enqueue_light(WorldSpan ws, const Launch& L, Ticks ts, bool stamped_already = false) {
     const std::vector<WorldDev<P>> d = views_of<P>(ws);
     std::vector<WorldDev<P>> stamped;
     std::vector<Grid> irr;
@@ -472,7 +475,7 @@ void enqueue_light(WorldSpan ws, const Launch& L, Ticks ts) {
         if (ts[i].source > 0.0f) stamped.push_back(d[i]);
         irr.push_back(ws[i]->f[ASTRO_FIELD_IRRADIANCE].grid());
     }
-    if (!stamped.empty()) launch_stamp<P>(L, stamped);
+    if (!stamped.empty() && !stamped_already) launch_stamp<P>(L, stamped);
     launch_sweep(L, irr, ts);
     // crowded: at least one live cell per hash bucket on average, in a launch
     // whose sorted records all fit in one XCD's 4 MiB L2 (the row walk reads
@@ -498,9 +501,12 @@ int32_t enqueue_tick(WorldSpan ws, const Launch& L) {
     launch_resort<P>(L, views_of<P>(ws));
     for (astro_world* w : ws) w->hash_valid = true;
     int32_t st;
-    if (ds.thermal_enabled && (st = enqueue_thermal<P>(ws, L, dt, ts, count_t)) != ASTRO_OK) return st;
+    // A lit tick's occluder stamp rides in the thermal step's last substep,
+    // which reads every entry's flags as the emission stage needs them.
     const bool feed = ds.emission_enabled && lit(ws[0]);
-    if (feed) enqueue_light<P>(ws, L, ts);
+    const bool stamp_t = feed && ds.thermal_enabled;
+    if (ds.thermal_enabled && (st = enqueue_thermal<P>(ws, L, dt, ts, count_t, stamp_t)) != ASTRO_OK) return st;
+    if (feed) enqueue_light<P>(ws, L, ts, stamp_t);
     if (ws[0]->flash_on) {
         // Stage 9b (step.cu:158-162): the armed flash runs instead of taxis,
         // after the feed and before the motion reads emit_power.
